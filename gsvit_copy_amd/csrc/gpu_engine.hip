@@ -1,0 +1,1272 @@
+/*
+ * gpu_engine.hip -- host side of the MI355X engine: the drop-in sv_gpu_new /
+ * sv_pool_gpu_* entry points (src3d/pool.h:98-124,167; reference driver
+ * src3d/gpu.c), device memory, coefficient classes and kernel launches.
+ *
+ * Design (replacing the reference's worker thread, command queue and
+ * per-launch device synchronisation, src3d/gpu.c:786-802,1792-1955):
+ *   - one HIP stream per engine, every launch asynchronous; the host waits
+ *     only in copyfrom and in the session API's explicit sync;
+ *   - material handling is resolved on the host into per-cell coefficient
+ *     classes (uint8 when at most 256 distinct classes exist, else uint16),
+ *     computed with the same averaging, dispersive flags and PEC rules as
+ *     the CPU path, so the device executes the CPU semantics exactly;
+ *   - E-side boundary work is fused into ystep_e so that the CPU order
+ *     (boundary before source, src3d/svit.c:228-229) holds whichever order
+ *     the caller uses for apply_source_estep / boundary_estep.
+ */
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include <unordered_map>
+#include <omp.h>
+#include "gsvit_internal.h"
+#include "engine.h"
+#include "kernels.h"
+
+#define EPS0 SV_EPSILON_0
+#define MU0 SV_MU_0
+
+#define CK(x)                                                                                 \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess) {                                                               \
+            fprintf(stderr, "HIP error: %s (%s) at %s:%d\n", hipGetErrorString(e_), #x, __FILE__, __LINE__); \
+            return 1;                                                                         \
+        }                                                                                     \
+    } while (0)
+
+struct EKey {
+    uint32_t w[7];
+    bool operator==(const EKey &o) const { return memcmp(w, o.w, sizeof(w)) == 0; }
+};
+struct HKey {
+    uint32_t w[5];
+    bool operator==(const HKey &o) const { return memcmp(w, o.w, sizeof(w)) == 0; }
+};
+template <int N>
+static size_t
+hash_words(const uint32_t *w)
+{
+    uint64_t h = 1469598103934665603ull;
+    for (int q = 0; q < N; q++) {
+        h ^= w[q];
+        h *= 1099511628211ull;
+    }
+    return (size_t)h;
+}
+struct EKeyHash { size_t operator()(const EKey &k) const { return hash_words<7>(k.w); } };
+struct HKeyHash { size_t operator()(const HKey &k) const { return hash_words<5>(k.w); } };
+
+static uint32_t
+fbits(double v)
+{
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+
+struct Engine {
+    SvPool *mp;
+    SvSet *set;
+    int dev;
+    hipStream_t st;
+    int ready;
+    GsvGrid g;
+    int idbytes_e, idbytes_h;
+    int bx, by, kchunk_h, kchunk_e;
+    std::vector<GsvECoef> etab;
+    std::vector<GsvHCoef> htab;
+    std::vector<uint8_t> ecls_host, hcls_host;   /* device layout, idbytes per cell */
+    int ade_slot_of_mat[256];
+    std::vector<void *> allocs;
+    float *fbase[6];
+    size_t field_floats;
+    /* sources */
+    int has_tsf;
+    GsvTsf tsf;
+    int has_src;
+    GsvPointSrc src;
+    /* traces */
+    int npts;
+    long long *d_ppos;
+    float *d_rec;
+    /* boundaries */
+    int has_liao;
+    GsvLiao liao;
+    int rezero[3];
+    int has_per;
+    GsvPeriodic per;
+    /* far field */
+    int has_ff;
+    GsvFarfieldTD ff;
+    int has_dft;
+    GsvDft dft;
+    /* timing of the two volume kernels */
+    int timing;
+    hipEvent_t ev[4];
+    double t_h_ms, t_e_ms;
+    long n_h, n_e;
+};
+
+template <typename T>
+static int
+dmalloc(Engine *E, T **p, size_t count)
+{
+    void *q = NULL;
+    if (count == 0)
+        count = 1;
+    CK(hipMalloc(&q, count * sizeof(T)));
+    E->allocs.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+template <typename T>
+static int
+dupload(Engine *E, T **p, const T *h, size_t count)
+{
+    if (dmalloc(E, p, count))
+        return 1;
+    /* synchronous: the host staging vectors of the callers are temporaries */
+    CK(hipMemcpyAsync(*p, h, count * sizeof(T), hipMemcpyHostToDevice, E->st));
+    CK(hipStreamSynchronize(E->st));
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* coefficient classes                                                  */
+/* ------------------------------------------------------------------ */
+
+static void
+h_coeffs(const SvPool *mp, int i, int j, int k, double c[5])
+{
+    /* src3d/yee.c:210-233 and sv_yee_data_get_db (src3d/yee.c:1621-1645) */
+    const SvSet *set = mp->set;
+    const SvYeeData *d = mp->d;
+    double dt = d->dt, mu, sg;
+    if (mp->nmat <= 0 && !(set->plan.matmode == SV_MATMODE_FULL || set->plan.matmode == SV_MATMODE_MAGNETIC)) {
+        c[0] = 1;
+        c[1] = dt / MU0 / d->dx;
+        c[2] = dt / MU0 / d->dy;
+        c[3] = dt / MU0 / d->dz;
+        c[4] = dt / MU0 / d->dx;
+        return;
+    }
+    sv_pool_mu_sigast(mp, i, j, k, &mu, &sg);
+    c[0] = (1 - sg * dt / 2 / mu) / (1 + sg * dt / 2 / mu);
+    c[1] = (dt / mu / d->dx) / (1 + sg * dt / 2 / mu);
+    c[2] = (dt / mu / d->dy) / (1 + sg * dt / 2 / mu);
+    c[3] = (dt / mu / d->dz) / (1 + sg * dt / 2 / mu);
+    c[4] = c[1];
+}
+
+static int
+pec_type(const SvPool *mp, int i, int j, int k)
+{
+    return mp->mats[mp->d->mat->data[i][j][k]].type == SV_MAT_PEC;
+}
+
+static void
+e_key(const Engine *E, int i, int j, int k, EKey *key)
+{
+    const SvPool *mp = E->mp;
+    int xt, yt, zt, mi;
+    double ca[3], cb[3];
+    uint32_t fl = 0;
+    mi = sv_pool_get_tbc(mp, i, j, k, &xt, &yt, &zt);
+    sv_pool_get_cabs(mp, i, j, k, ca, ca + 1, ca + 2, cb, cb + 1, cb + 2);
+    if (xt) fl |= GSV_E_TBCX;
+    if (yt) fl |= GSV_E_TBCY;
+    if (zt) fl |= GSV_E_TBCZ;
+    if (mp->nmat > 0 && xt * yt * zt == 0 && mp->mats[mi].type == SV_MAT_ADE)
+        fl |= GSV_E_ADE | ((uint32_t)E->ade_slot_of_mat[mi] << GSV_E_SLOT_SHIFT);
+    if (mi >= 0 && i > 0 && j > 0 && k > 0) {
+        /* src3d/yee.c:1139-1237 */
+        int p0 = pec_type(mp, i, j, k);
+        int pxm = pec_type(mp, i - 1, j, k), pym = pec_type(mp, i, j - 1, k), pzm = pec_type(mp, i, j, k - 1);
+        int pxy = pec_type(mp, i - 1, j - 1, k), pyz = pec_type(mp, i, j - 1, k - 1), pxz = pec_type(mp, i - 1, j, k - 1);
+        int pxyz = pec_type(mp, i - 1, j - 1, k - 1);
+        int pm = mp->mats[mi].type == SV_MAT_PEC;
+        if (pm || pxm || pym || pzm || pxy || pyz || pxz || pxyz) {
+            if (pxm != p0) fl |= GSV_E_PECY | GSV_E_PECZ;
+            if (pym != p0) fl |= GSV_E_PECX | GSV_E_PECZ;
+            if (pzm != p0) fl |= GSV_E_PECX | GSV_E_PECY;
+            if (pxy && !p0) fl |= GSV_E_PECZ;
+            if (pxz && !p0) fl |= GSV_E_PECY;
+            if (pyz && !p0) fl |= GSV_E_PECX;
+        }
+    }
+    key->w[0] = fbits(ca[0]); key->w[1] = fbits(ca[1]); key->w[2] = fbits(ca[2]);
+    key->w[3] = fbits(cb[0]); key->w[4] = fbits(cb[1]); key->w[5] = fbits(cb[2]);
+    key->w[6] = fl;
+}
+
+/* classes for every cell; ids in device layout (x fastest, row pitch px) */
+template <typename KEY, typename HASH, typename KEYFN>
+static int
+build_classes(Engine *E, KEYFN keyfn, std::vector<KEY> &table, std::vector<uint8_t> &ids_out, int *idbytes)
+{
+    const int nx = E->g.nx, ny = E->g.ny, nz = E->g.nz, px = E->g.px;
+    const size_t ncell = (size_t)px * ny * nz;
+    std::vector<uint32_t> tmp(ncell, 0);
+    int nthr = omp_get_max_threads();
+    std::vector<std::vector<KEY>> lkeys(nthr);
+
+    #pragma omp parallel
+    {
+        int t = omp_get_thread_num();
+        std::unordered_map<KEY, uint32_t, HASH> lm;
+        KEY last;
+        uint32_t lastid = 0;
+        int have_last = 0;
+        #pragma omp for schedule(static)
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    KEY key;
+                    uint32_t id;
+                    keyfn(i, j, k, &key);
+                    if (have_last && key == last)
+                        id = lastid;
+                    else {
+                        auto it = lm.find(key);
+                        if (it == lm.end()) {
+                            id = (uint32_t)lkeys[t].size();
+                            lkeys[t].push_back(key);
+                            lm.emplace(key, id);
+                        } else
+                            id = it->second;
+                        last = key;
+                        lastid = id;
+                        have_last = 1;
+                    }
+                    tmp[(size_t)i + (size_t)px * j + (size_t)px * ny * k] = ((uint32_t)t << 24) | id;
+                }
+    }
+    std::unordered_map<KEY, uint32_t, HASH> gm;
+    std::vector<std::vector<uint32_t>> remap(nthr);
+    table.clear();
+    for (int t = 0; t < nthr; t++) {
+        for (const KEY &key : lkeys[t]) {
+            auto it = gm.find(key);
+            uint32_t id;
+            if (it == gm.end()) {
+                id = (uint32_t)table.size();
+                table.push_back(key);
+                gm.emplace(key, id);
+            } else
+                id = it->second;
+            remap[t].push_back(id);
+        }
+    }
+    if (table.size() > 65536) {
+        fprintf(stderr, "Error: %zu distinct coefficient classes exceed the 65536 the engine supports\n", table.size());
+        return 1;
+    }
+    *idbytes = table.size() <= 256 ? 1 : 2;
+    ids_out.assign(ncell * (*idbytes), 0);
+    #pragma omp parallel for schedule(static)
+    for (long long q = 0; q < (long long)ncell; q++) {
+        uint32_t v = tmp[q];
+        uint32_t id = ((v >> 24) < (uint32_t)nthr && (v & 0xFFFFFF) < remap[v >> 24].size()) ? remap[v >> 24][v & 0xFFFFFF] : 0;
+        if (*idbytes == 1)
+            ids_out[q] = (uint8_t)id;
+        else
+            ((uint16_t *)ids_out.data())[q] = (uint16_t)id;
+    }
+    return 0;
+}
+
+static int
+build_tables(Engine *E)
+{
+    SvPool *mp = E->mp;
+    int n, nslot = 0;
+    for (n = 0; n < 256; n++)
+        E->ade_slot_of_mat[n] = 0;
+    if (mp->nmat > 256) {
+        fprintf(stderr, "Error: at most 256 tabulated materials are supported\n");
+        return 1;
+    }
+    E->g.ade = 0;
+    E->g.ade_poles = 0;
+    for (n = 0; n < mp->nmat; n++) {
+        const SvMatProp *m = mp->mats + n;
+        if (m->type != SV_MAT_ADE)
+            continue;
+        if (nslot >= GSV_MAX_ADE) {
+            fprintf(stderr, "Error: at most %d ADE materials are supported\n", GSV_MAX_ADE);
+            return 1;
+        }
+        GsvAde &a = E->g.adem[nslot];
+        double dt = mp->d->dt;
+        a.c0 = (float)m->ade_c0; a.c1 = (float)m->ade_c1; a.c2 = (float)m->ade_c2; a.c3 = (float)m->ade_c3;
+        a.a0 = (float)m->ade_a0; a.a1 = (float)m->ade_a1; a.a2 = (float)m->ade_a2;
+        for (int q = 0; q < 2; q++) {
+            a.bp0[q] = (float)m->ade_bp0[q]; a.bp1[q] = (float)m->ade_bp1[q]; a.bp2[q] = (float)m->ade_bp2[q];
+            a.bp3[q] = (float)m->ade_bp3[q]; a.bp4[q] = (float)m->ade_bp4[q];
+        }
+        a.cbm[0] = (float)((dt / m->epsilon / mp->d->dx) / (1 + m->sigma * dt / 2 / m->epsilon));
+        a.cbm[1] = (float)((dt / m->epsilon / mp->d->dy) / (1 + m->sigma * dt / 2 / m->epsilon));
+        a.cbm[2] = (float)((dt / m->epsilon / mp->d->dz) / (1 + m->sigma * dt / 2 / m->epsilon));
+        a.poles = (m->cp3_a[0] != 0 || m->cp3_a[1] != 0 || m->ade_bp0[0] != -1 || m->ade_bp0[1] != -1
+                   || m->ade_bp1[0] != 2 || m->ade_bp1[1] != 2);
+        if (a.poles)
+            E->g.ade_poles = 1;
+        E->ade_slot_of_mat[n] = nslot++;
+        E->g.ade = 1;
+    }
+
+    std::vector<EKey> ekeys;
+    std::vector<HKey> hkeys;
+    auto ek = [E](int i, int j, int k, EKey *key) { e_key(E, i, j, k, key); };
+    auto hk = [mp](int i, int j, int k, HKey *key) {
+        double c[5];
+        h_coeffs(mp, i, j, k, c);
+        for (int q = 0; q < 5; q++)
+            key->w[q] = fbits(c[q]);
+    };
+    if (build_classes<EKey, EKeyHash>(E, ek, ekeys, E->ecls_host, &E->idbytes_e))
+        return 1;
+    if (build_classes<HKey, HKeyHash>(E, hk, hkeys, E->hcls_host, &E->idbytes_h))
+        return 1;
+    E->etab.resize(ekeys.size());
+    for (size_t q = 0; q < ekeys.size(); q++) {
+        GsvECoef &c = E->etab[q];
+        memcpy(c.ca, ekeys[q].w, 12);
+        memcpy(c.cb, ekeys[q].w + 3, 12);
+        c.flags = ekeys[q].w[6];
+        c.pad = 0;
+    }
+    E->htab.resize(hkeys.size());
+    for (size_t q = 0; q < hkeys.size(); q++) {
+        GsvHCoef &c = E->htab[q];
+        memcpy(&c.da, hkeys[q].w, 4);
+        memcpy(c.db, hkeys[q].w + 1, 12);
+        memcpy(&c.dbc, hkeys[q].w + 4, 4);
+        c.pad[0] = c.pad[1] = c.pad[2] = 0;
+    }
+    if (mp->set->sc.verbose > 0)
+        printf("GPU plan: %zu E classes (%d-byte ids), %zu H classes (%d-byte ids), %d ADE materials\n",
+               E->etab.size(), E->idbytes_e, E->htab.size(), E->idbytes_h, nslot);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* sv_gpu_new                                                            */
+/* ------------------------------------------------------------------ */
+
+static int
+engine_env_int(const char *name, int def)
+{
+    const char *v = getenv(name);
+    return v ? atoi(v) : def;
+}
+
+extern "C" void *
+sv_gpu_new(SvPool *mp, SvSet *set)
+{
+    int ngpu = 0, q;
+    Engine *E;
+    if (set->sc.usegpu == 0)
+        return NULL;
+    if (hipGetDeviceCount(&ngpu) != hipSuccess || ngpu == 0) {
+        fprintf(stderr, "Error: no HIP device found for GPU computation\n");
+        return NULL;
+    }
+    E = new Engine();
+    E->mp = mp;
+    E->set = set;
+    E->dev = 0;
+    for (q = 0; q < SV_MAX_GPUS && q < ngpu; q++)
+        if (set->sc.ugpu[q]) {
+            E->dev = q;
+            break;
+        }
+    if (hipSetDevice(E->dev) != hipSuccess) {
+        delete E;
+        return NULL;
+    }
+    memset(&E->g, 0, sizeof(E->g));
+    E->g.nx = set->sp.xres;
+    E->g.ny = set->sp.yres;
+    E->g.nz = set->sp.zres;
+    E->g.px = (set->sp.xres + 15) / 16 * 16;
+    E->g.plane = (long long)E->g.px * E->g.ny;
+    E->g.dx = (float)set->sp.dx;
+    E->g.dy = (float)set->sp.dy;
+    E->g.dz = (float)set->sp.dz;
+    E->bx = engine_env_int("GSV_BX", 64);
+    E->by = engine_env_int("GSV_BY", 4);
+    E->kchunk_h = engine_env_int("GSV_KCHUNK_H", 16);
+    E->kchunk_e = engine_env_int("GSV_KCHUNK_E", 16);
+    if (E->bx * E->by > 256 || E->bx <= 0 || E->by <= 0 || E->kchunk_h <= 0 || E->kchunk_e <= 0) {
+        fprintf(stderr, "Error: invalid launch shape (%d x %d, k chunk %d/%d)\n", E->bx, E->by, E->kchunk_h, E->kchunk_e);
+        delete E;
+        return NULL;
+    }
+    if (mp->src->tsf) {
+        SvSourceTSF *t = mp->src->tsf;
+        if (t->box_in > set->sp.xres - 2 || t->box_jn > set->sp.yres - 2 || t->box_kn > set->sp.zres - 2) {
+            fprintf(stderr, "Error: the TSF box must end at least two cells before the far faces\n");
+            delete E;
+            return NULL;
+        }
+        if (!(t->ia_theta >= 0 && t->ia_theta < M_PI)) {
+            fprintf(stderr, "Error: TSF theta outside [0, pi) is not supported\n");
+            delete E;
+            return NULL;
+        }
+    }
+    if (build_tables(E)) {
+        delete E;
+        return NULL;
+    }
+    set->plan.gpumode = SV_GPUMODE_FULL;
+    mp->gpu = E;
+    return E;
+}
+
+/* ------------------------------------------------------------------ */
+/* sv_pool_gpu_init: device allocation and static data                  */
+/* ------------------------------------------------------------------ */
+
+static int
+kappa_axis(const SvCpmlFace *f0, int on0, const SvCpmlFace *fn, int onn, int n, int e_side, std::vector<float> &rk)
+{
+    rk.assign(n, 1.f);
+    for (int i = 0; i < n; i++) {
+        double k = 1;
+        if (on0 && i < f0->depth) {
+            double v = e_side ? f0->kappae[i] : f0->kappah[i];
+            if (v > k) k = v;
+        }
+        if (onn && i >= n - fn->depth) {
+            int r = i - (n - fn->depth);
+            double v = e_side ? fn->kappae[r] : fn->kappah[r];
+            if (v > k) k = v;
+        }
+        rk[i] = (float)(1.0 / k);
+    }
+    return 0;
+}
+
+static int
+setup_cpml(Engine *E)
+{
+    SvSet *set = E->set;
+    SvBoundary *b = E->mp->bnd;
+    const SvCpmlFace *faces[6] = { &b->x0, &b->xn, &b->y0, &b->yn, &b->z0, &b->zn };
+    int on[6] = { set->sb.bx0 == SV_BOUNDARY_CPML, set->sb.bxn == SV_BOUNDARY_CPML, set->sb.by0 == SV_BOUNDARY_CPML,
+                  set->sb.byn == SV_BOUNDARY_CPML, set->sb.bz0 == SV_BOUNDARY_CPML, set->sb.bzn == SV_BOUNDARY_CPML };
+    double dn[3] = { set->sp.dx, set->sp.dy, set->sp.dz };
+    GsvGrid &g = E->g;
+    std::vector<float> rk;
+
+    kappa_axis(faces[0], on[0], faces[1], on[1], g.nx, 1, rk);
+    if (dupload(E, (float **)&g.rkex, rk.data(), rk.size())) return 1;
+    kappa_axis(faces[2], on[2], faces[3], on[3], g.ny, 1, rk);
+    if (dupload(E, (float **)&g.rkey, rk.data(), rk.size())) return 1;
+    kappa_axis(faces[4], on[4], faces[5], on[5], g.nz, 1, rk);
+    if (dupload(E, (float **)&g.rkez, rk.data(), rk.size())) return 1;
+    kappa_axis(faces[0], on[0], faces[1], on[1], g.nx, 0, rk);
+    if (dupload(E, (float **)&g.rkhx, rk.data(), rk.size())) return 1;
+    kappa_axis(faces[2], on[2], faces[3], on[3], g.ny, 0, rk);
+    if (dupload(E, (float **)&g.rkhy, rk.data(), rk.size())) return 1;
+    kappa_axis(faces[4], on[4], faces[5], on[5], g.nz, 0, rk);
+    if (dupload(E, (float **)&g.rkhz, rk.data(), rk.size())) return 1;
+
+    for (int f = 0; f < 6; f++) {
+        GsvCpmlFace &F = g.f[f];
+        memset(&F, 0, sizeof(F));
+        if (!on[f])
+            continue;
+        const SvCpmlFace *h = faces[f];
+        int dep = h->depth, axis = f / 2;
+        size_t cnt;
+        if (dep < 1 || (axis == 0 && dep > g.nx) || (axis == 1 && dep > g.ny) || (axis == 2 && dep > g.nz)) {
+            fprintf(stderr, "Error: CPML depth %d does not fit the pool\n", dep);
+            return 1;
+        }
+        F.on = 1;
+        F.depth = dep;
+        std::vector<float> be(dep), ced(dep), bh(dep), chd(dep), cer(dep);
+        for (int q = 0; q < dep; q++) {
+            be[q] = (float)h->be[q];
+            ced[q] = (float)(h->ce[q] / dn[axis]);
+            bh[q] = (float)h->bh[q];
+            chd[q] = (float)(h->ch[q] / dn[axis]);
+            cer[q] = (float)h->ce[q];
+        }
+        if (dupload(E, &F.be, be.data(), dep) || dupload(E, &F.ce_d, ced.data(), dep) || dupload(E, &F.bh, bh.data(), dep)
+            || dupload(E, &F.ch_d, chd.data(), dep) || dupload(E, &F.ce_raw, cer.data(), dep))
+            return 1;
+        if (axis == 0) cnt = (size_t)dep * g.ny * g.nz;
+        else if (axis == 1) cnt = (size_t)g.px * dep * g.nz;
+        else cnt = (size_t)g.px * g.ny * dep;
+        if (dmalloc(E, &F.pe1, cnt) || dmalloc(E, &F.pe2, cnt) || dmalloc(E, &F.ph1, cnt) || dmalloc(E, &F.ph2, cnt))
+            return 1;
+        CK(hipMemsetAsync(F.pe1, 0, cnt * 4, E->st));
+        CK(hipMemsetAsync(F.pe2, 0, cnt * 4, E->st));
+        CK(hipMemsetAsync(F.ph1, 0, cnt * 4, E->st));
+        CK(hipMemsetAsync(F.ph2, 0, cnt * 4, E->st));
+    }
+    return 0;
+}
+
+static int
+setup_tsf(Engine *E)
+{
+    SvPool *mp = E->mp;
+    SvSet *set = E->set;
+    SvSourceTSF *t = mp->src->tsf;
+    GsvTsf &T = E->tsf;
+    memset(&T, 0, sizeof(T));
+    if (!t)
+        return 0;
+    E->has_tsf = 1;
+    T.i0 = t->box_i0; T.j0 = t->box_j0; T.k0 = t->box_k0;
+    T.in = t->box_in; T.jn = t->box_jn; T.kn = t->box_kn;
+    double th = t->ia_theta, ph = t->ia_phi, ps = t->ia_psi;
+    T.corr = t->corr;
+    T.ax = sin(th) * cos(ph);
+    T.ay = sin(th) * sin(ph);
+    T.az = cos(th);
+    /* src3d/source.c:53-107 with i0 = j0 = k0 = 0 and i1, j1, k1 = pool size */
+    int X = set->sp.xres, Y = set->sp.yres, Z = set->sp.zres;
+    int xo, yo;
+    if (ph >= 0 && ph <= M_PI / 2) { xo = 0; yo = 0; }
+    else if (ph > M_PI / 2 && ph <= M_PI) { xo = X; yo = 0; }
+    else if (ph > M_PI && ph <= 3 * M_PI / 2) { xo = X; yo = Y; }
+    else { xo = 0; yo = Y; }
+    T.oi = xo;
+    T.oj = yo;
+    T.ok = (th <= M_PI / 2) ? 0 : Z;
+    if (th > M_PI / 2 && ph > M_PI / 2 && ph <= M_PI)
+        T.ax = 0; /* rx stays 0 in this quadrant, src3d/source.c:89-91 */
+    T.gex = cos(ps) * sin(ph) - sin(ps) * cos(th) * cos(ph);
+    T.gey = -cos(ps) * cos(ph) - sin(ps) * cos(th) * sin(ph);
+    T.gez = sin(ps) * sin(th);
+    T.ghx = sin(ps) * sin(ph) + cos(ps) * cos(th) * cos(ph);
+    T.ghy = -sin(ps) * cos(ph) + cos(ps) * cos(th) * sin(ph);
+    T.ghz = -cos(ps) * sin(th);
+    T.res = t->jp_res;
+    double dt = set->plan.dt;
+    T.multe = 1.0 / t->jp_epsilon * dt / (EPS0 * t->jp_dx);
+    T.multh = 1.0 / t->jp_mu * dt / (MU0 * t->jp_dx);
+    T.coef = (dt * SV_LIGHT_SPEED - t->jp_dx) / (dt * SV_LIGHT_SPEED + t->jp_dx);
+    T.ndata = t->ndata;
+    if (dmalloc(E, &T.je, T.res) || dmalloc(E, &T.jh, T.res) || dmalloc(E, &T.jb, 8))
+        return 1;
+    CK(hipMemsetAsync(T.je, 0, T.res * 8, E->st));
+    CK(hipMemsetAsync(T.jh, 0, T.res * 8, E->st));
+    CK(hipMemsetAsync(T.jb, 0, 64, E->st));
+    {
+        double *w;
+        if (dupload(E, &w, t->e, t->ndata))
+            return 1;
+        T.wave = w;
+    }
+    /* per-face coefficients: activity (skip flags, skip-near-material map), PEC at z0, local epsilon */
+    double ecor = dt / set->sp.dx, hcor = dt / set->sp.dx / MU0;
+    int skips[6] = { t->skip_i0, t->skip_in, t->skip_j0, t->skip_jn, t->skip_k0, t->skip_kn };
+    for (int f = 0; f < 6; f++) {
+        int axis = f / 2, far = f % 2, na, nb;
+        if (axis == 0) { na = T.jn - T.j0 + 1; nb = T.kn - T.k0 + 1; }
+        else if (axis == 1) { na = T.in - T.i0 + 1; nb = T.kn - T.k0 + 1; }
+        else { na = T.in - T.i0 + 1; nb = T.jn - T.j0 + 1; }
+        std::vector<float> ce((size_t)na * nb, 0.f), chh((size_t)na * nb, 0.f);
+        for (int b = 0; b < nb; b++)
+            for (int a = 0; a < na; a++) {
+                int i, j, k;
+                if (axis == 0) { i = far ? T.in : T.i0; j = T.j0 + a; k = T.k0 + b; }
+                else if (axis == 1) { i = T.i0 + a; j = far ? T.jn : T.j0; k = T.k0 + b; }
+                else { i = T.i0 + a; j = T.j0 + b; k = far ? T.kn : T.k0; }
+                if (skips[f] || !sv_tsf_face_active(mp, set, i, j, k))
+                    continue;
+                if (f == 4 && sv_pool_is_pec(mp, i, j, k))
+                    continue;
+                double eps, sig;
+                sv_pool_epsilon_sigma(mp, i, j, k, &eps, &sig);
+                ce[(size_t)a + (size_t)na * b] = (float)(ecor / eps);
+                chh[(size_t)a + (size_t)na * b] = (float)hcor;
+            }
+        float *dce, *dch;
+        if (dupload(E, &dce, ce.data(), ce.size()) || dupload(E, &dch, chh.data(), chh.size()))
+            return 1;
+        T.coefE[f] = dce;
+        T.coefH[f] = dch;
+    }
+    return 0;
+}
+
+static long long
+cell_index(const GsvGrid &g, int i, int j, int k)
+{
+    return (long long)i + (long long)g.px * j + g.plane * k;
+}
+
+static int
+setup_sources_and_outputs(Engine *E)
+{
+    SvPool *mp = E->mp;
+    SvSet *set = E->set;
+    GsvGrid &g = E->g;
+    int n, s;
+    /* point sources: per-step table, 0 = no write (src3d/source.c:405-437) */
+    E->has_src = mp->src->npnts > 0;
+    memset(&E->src, 0, sizeof(E->src));
+    if (E->has_src) {
+        int ns = mp->src->npnts, nst = set->sc.nsteps;
+        std::vector<long long> pos(ns);
+        std::vector<float> vals((size_t)nst * ns * 6, 0.f);
+        for (n = 0; n < ns; n++) {
+            const SvSourcePoint *sp = mp->src->sp + n;
+            pos[n] = cell_index(g, sp->i, sp->j, sp->k);
+            for (s = 0; s < nst; s++) {
+                int p = -1, q;
+                if (s < sp->ndata && sp->layered_zpos[s] == s)
+                    p = s;
+                else
+                    for (q = 0; q < sp->ndata; q++)
+                        if (sp->layered_zpos[q] == s) { p = q; break; }
+                if (p < 0)
+                    continue;
+                float *v = vals.data() + ((size_t)s * ns + n) * 6;
+                v[0] = (float)sp->ex[p]; v[1] = (float)sp->ey[p]; v[2] = (float)sp->ez[p];
+                v[3] = (float)sp->hx[p]; v[4] = (float)sp->hy[p]; v[5] = (float)sp->hz[p];
+            }
+        }
+        long long *dp;
+        float *dv;
+        if (dupload(E, &dp, pos.data(), pos.size()) || dupload(E, &dv, vals.data(), vals.size()))
+            return 1;
+        E->src.n = ns;
+        E->src.nsteps = nst;
+        E->src.pos = dp;
+        E->src.vals = dv;
+    }
+    /* point traces */
+    E->npts = set->so.npnts;
+    if (E->npts) {
+        std::vector<long long> pos(E->npts);
+        for (n = 0; n < E->npts; n++)
+            pos[n] = cell_index(g, set->so.pnts[n].i, set->so.pnts[n].j, set->so.pnts[n].k);
+        if (dupload(E, &E->d_ppos, pos.data(), pos.size()))
+            return 1;
+        size_t cnt = (size_t)(set->sc.nsteps + 1) * E->npts * 6;
+        if (dmalloc(E, &E->d_rec, cnt))
+            return 1;
+        CK(hipMemsetAsync(E->d_rec, 0, cnt * 4, E->st));
+    }
+    return 0;
+}
+
+static int
+setup_liao_periodic(Engine *E)
+{
+    SvSet *set = E->set;
+    SvPool *mp = E->mp;
+    GsvGrid &g = E->g;
+    const SvSetBoundary *b = &set->sb;
+    int types[6] = { b->bx0, b->bxn, b->by0, b->byn, b->bz0, b->bzn };
+    memset(&E->liao, 0, sizeof(E->liao));
+    E->has_liao = 0;
+    for (int f = 0; f < 6; f++)
+        if (types[f] == SV_BOUNDARY_LIAO) {
+            E->liao.on[f] = 1;
+            E->has_liao = 1;
+        }
+    if (E->has_liao) {
+        double dt = set->plan.dt, d3[3] = { set->sp.dx, set->sp.dy, set->sp.dz };
+        int res[3] = { g.nx, g.ny, g.nz };
+        for (int axis = 0; axis < 3; axis++) {
+            if (!E->liao.on[2 * axis] && !E->liao.on[2 * axis + 1])
+                continue;
+            int na, nb;
+            if (axis == 0) { na = g.ny; nb = g.nz; }
+            else if (axis == 1) { na = g.nx; nb = g.nz; }
+            else { na = g.nx; nb = g.ny; }
+            size_t cnt = (size_t)24 * na * nb;
+            if (dmalloc(E, &E->liao.store[axis], cnt))
+                return 1;
+            CK(hipMemsetAsync(E->liao.store[axis], 0, cnt * 4, E->st));
+            for (int far = 0; far < 2; far++) {
+                if (!E->liao.on[2 * axis + far])
+                    continue;
+                std::vector<float> cf((size_t)na * nb * 3);
+                for (int a = 0; a < na; a++)
+                    for (int bb = 0; bb < nb; bb++) {
+                        int pos[3];
+                        if (axis == 0) { pos[1] = a; pos[2] = bb; }
+                        else if (axis == 1) { pos[0] = a; pos[2] = bb; }
+                        else { pos[0] = a; pos[1] = bb; }
+                        /* epsilon probe on the face plane; zn uses zres-2 (src3d/boundary.c:657) */
+                        pos[axis] = far ? (axis == 2 ? res[axis] - 2 : res[axis] - 1) : 0;
+                        double ind = 1;
+                        if (set->plan.matmode == SV_MATMODE_FULL || set->plan.matmode == SV_MATMODE_ELECTRIC)
+                            ind = sqrt(mp->d->epsilon->data[pos[0]][pos[1]][pos[2]] / EPS0);
+                        for (int q = 0; q < 3; q++)
+                            cf[((size_t)a * nb + bb) * 3 + q] = (float)((dt * SV_LIGHT_SPEED / ind - d3[q]) / (dt * SV_LIGHT_SPEED / ind + d3[q]));
+                    }
+                float *dcf;
+                if (dupload(E, &dcf, cf.data(), cf.size()))
+                    return 1;
+                E->liao.coef[2 * axis + far] = dcf;
+            }
+        }
+        E->rezero[0] = b->bx0 == SV_BOUNDARY_CPML;
+        E->rezero[1] = b->by0 == SV_BOUNDARY_CPML;
+        E->rezero[2] = b->bz0 == SV_BOUNDARY_CPML;
+    }
+    const SvSetMediumBoundary *m = &set->smb;
+    memset(&E->per, 0, sizeof(E->per));
+    E->per.x = m->bx0 == SV_BOUNDARY_PERIODIC || m->bxn == SV_BOUNDARY_PERIODIC;
+    E->per.y = m->by0 == SV_BOUNDARY_PERIODIC || m->byn == SV_BOUNDARY_PERIODIC;
+    E->per.z = m->bz0 == SV_BOUNDARY_PERIODIC || m->bzn == SV_BOUNDARY_PERIODIC;
+    E->has_per = E->per.x || E->per.y || E->per.z;
+    E->per.x0 = m->bx0pos; E->per.xn = m->bxnpos;
+    E->per.y0 = m->by0pos; E->per.yn = m->bynpos;
+    E->per.z0 = m->bz0pos; E->per.zn = m->bznpos;
+    if (E->has_per) {
+        if ((E->per.x && (m->bx0pos < 1 || m->bxnpos > g.nx - 1)) || (E->per.y && (m->by0pos < 1 || m->bynpos > g.ny - 1))
+            || (E->per.z && (m->bz0pos < 1 || m->bznpos > g.nz - 1)) || m->bx0pos < 0 || m->by0pos < 0 || m->bz0pos < 0
+            || m->bxnpos > g.nx || m->bynpos > g.ny || m->bznpos > g.nz) {
+            fprintf(stderr, "Error: periodic medium boundary planes must lie inside the pool\n");
+            return 1;
+        }
+    }
+    return 0;
+}
+
+static int
+setup_farfield(Engine *E)
+{
+    SvSet *set = E->set;
+    SvPool *mp = E->mp;
+    SvSetFarfield *sf = &set->sf;
+    GsvGrid &g = E->g;
+    memset(&E->ff, 0, sizeof(E->ff));
+    memset(&E->dft, 0, sizeof(E->dft));
+    E->has_ff = sf->nrs > 0;
+    E->has_dft = sf->ndft > 0;
+    if (!E->has_ff && !E->has_dft)
+        return 0;
+    GsvFarfieldTD &F = E->ff;
+    F.i0 = sf->box_i0; F.j0 = sf->box_j0; F.k0 = sf->box_k0;
+    F.i1 = sf->box_in; F.j1 = sf->box_jn; F.k1 = sf->box_kn;
+    F.ix = F.i1 - F.i0; F.iy = F.j1 - F.j0; F.iz = F.k1 - F.k0;
+    F.ncells = 2LL * ((long long)F.iy * F.iz + (long long)F.ix * F.iz + (long long)F.ix * F.iy);
+    if (E->has_ff) {
+        int sk[6][5] = {
+            { sf->box_boundary_skipi0, sf->skipi0_jmin, sf->skipi0_kmin, sf->skipi0_jmax, sf->skipi0_kmax },
+            { sf->box_boundary_skipin, sf->skipin_jmin, sf->skipin_kmin, sf->skipin_jmax, sf->skipin_kmax },
+            { sf->box_boundary_skipj0, sf->skipj0_imin, sf->skipj0_kmin, sf->skipj0_imax, sf->skipj0_kmax },
+            { sf->box_boundary_skipjn, sf->skipjn_imin, sf->skipjn_kmin, sf->skipjn_imax, sf->skipjn_kmax },
+            { sf->box_boundary_skipk0, sf->skipk0_imin, sf->skipk0_jmin, sf->skipk0_imax, sf->skipk0_jmax },
+            { sf->box_boundary_skipkn, sf->skipkn_imin, sf->skipkn_jmin, sf->skipkn_imax, sf->skipkn_jmax } };
+        memcpy(F.skip, sk, sizeof(sk));
+        F.nrs = sf->nrs;
+        F.ndata = mp->farfield->ndata;
+        std::vector<int> ist(F.nrs);
+        for (int n = 0; n < F.nrs; n++)
+            ist[n] = mp->farfield->rpoints[n].istart;
+        int *p;
+        if (dupload(E, &p, sf->ri, F.nrs)) return 1;
+        F.ri = p;
+        if (dupload(E, &p, sf->rj, F.nrs)) return 1;
+        F.rj = p;
+        if (dupload(E, &p, sf->rk, F.nrs)) return 1;
+        F.rk = p;
+        if (dupload(E, &p, ist.data(), F.nrs)) return 1;
+        F.istart = p;
+        size_t cnt = (size_t)F.nrs * 3 * F.ndata;
+        if (dmalloc(E, &F.bins, cnt))
+            return 1;
+        CK(hipMemsetAsync(F.bins, 0, cnt * 8, E->st));
+        F.dx = set->sp.dx; F.dy = set->sp.dy; F.dz = set->sp.dz;
+        F.dt = set->plan.dt;
+        F.c = SV_LIGHT_SPEED;
+    }
+    if (E->has_dft) {
+        GsvDft &D = E->dft;
+        std::vector<long long> cells;
+        std::vector<int> faces;
+        for (int j = F.j0; j < F.j1; j++)
+            for (int k = F.k0; k < F.k1; k++) {
+                cells.push_back(cell_index(g, F.i0, j, k)); faces.push_back(0);
+                cells.push_back(cell_index(g, F.i1, j, k)); faces.push_back(1);
+            }
+        for (int i = F.i0; i < F.i1; i++)
+            for (int k = F.k0; k < F.k1; k++) {
+                cells.push_back(cell_index(g, i, F.j0, k)); faces.push_back(2);
+                cells.push_back(cell_index(g, i, F.j1, k)); faces.push_back(3);
+            }
+        for (int i = F.i0; i < F.i1; i++)
+            for (int j = F.j0; j < F.j1; j++) {
+                cells.push_back(cell_index(g, i, j, F.k0)); faces.push_back(4);
+                cells.push_back(cell_index(g, i, j, F.k1)); faces.push_back(5);
+            }
+        D.ncells = (long long)cells.size();
+        D.nw = sf->ndft;
+        D.dt = set->plan.dt;
+        long long *pc;
+        int *pf;
+        double *po;
+        if (dupload(E, &pc, cells.data(), cells.size()) || dupload(E, &pf, faces.data(), faces.size())
+            || dupload(E, &po, mp->farfield->dft_omega, D.nw))
+            return 1;
+        D.cell = pc;
+        D.face = pf;
+        D.omega = po;
+        size_t cnt = (size_t)D.ncells * D.nw * 8;
+        if (dmalloc(E, &D.acc, cnt))
+            return 1;
+        CK(hipMemsetAsync(D.acc, 0, cnt * 8, E->st));
+    }
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_init(SvPool *mp, SvSet *set)
+{
+    Engine *E = (Engine *)mp->gpu;
+    (void)set;
+    if (!E) {
+        fprintf(stderr, "Error: sv_pool_gpu_init called without a GPU plan\n");
+        return 1;
+    }
+    GsvGrid &g = E->g;
+    CK(hipSetDevice(E->dev));
+    CK(hipStreamCreateWithFlags(&E->st, hipStreamNonBlocking));
+    E->field_floats = (size_t)g.plane * (g.nz + 2);
+    float **fp[6] = { &g.ex, &g.ey, &g.ez, &g.hx, &g.hy, &g.hz };
+    for (int q = 0; q < 6; q++) {
+        if (dmalloc(E, &E->fbase[q], E->field_floats))
+            return 1;
+        CK(hipMemsetAsync(E->fbase[q], 0, E->field_floats * 4, E->st));
+        *fp[q] = E->fbase[q] + g.plane;
+    }
+    {
+        uint8_t *d;
+        if (dupload(E, &d, E->ecls_host.data(), E->ecls_host.size())) return 1;
+        g.ecls = d;
+        if (dupload(E, &d, E->hcls_host.data(), E->hcls_host.size())) return 1;
+        g.hcls = d;
+        GsvECoef *et;
+        GsvHCoef *ht;
+        if (dupload(E, &et, E->etab.data(), E->etab.size())) return 1;
+        if (dupload(E, &ht, E->htab.data(), E->htab.size())) return 1;
+        g.etab = et;
+        g.htab = ht;
+        g.netab = (int)E->etab.size();
+        g.nhtab = (int)E->htab.size();
+    }
+    if (g.ade) {
+        size_t n = (size_t)g.plane * g.nz;
+        float **arr[] = { &g.exp_, &g.eyp, &g.ezp, &g.dpx, &g.dpy, &g.dpz, &g.dpxp, &g.dpyp, &g.dpzp };
+        for (auto a : arr) {
+            if (dmalloc(E, a, n)) return 1;
+            CK(hipMemsetAsync(*a, 0, n * 4, E->st));
+        }
+        if (g.ade_poles) {
+            float **par[] = { &g.px0, &g.px1, &g.py0, &g.py1, &g.pz0, &g.pz1, &g.pxp0, &g.pxp1, &g.pyp0, &g.pyp1, &g.pzp0, &g.pzp1 };
+            for (auto a : par) {
+                if (dmalloc(E, a, n)) return 1;
+                CK(hipMemsetAsync(*a, 0, n * 4, E->st));
+            }
+        }
+    }
+    if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E))
+        return 1;
+    for (int q = 0; q < 4; q++)
+        CK(hipEventCreate(&E->ev[q]));
+    CK(hipStreamSynchronize(E->st));
+    /* the host copy of the class ids is not needed any more */
+    std::vector<uint8_t>().swap(E->ecls_host);
+    std::vector<uint8_t>().swap(E->hcls_host);
+    E->ready = 1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* host <-> device                                                      */
+/* ------------------------------------------------------------------ */
+
+static int
+upload_field(Engine *E, const SvDCube *c, float *dst)
+{
+    const GsvGrid &g = E->g;
+    size_t n = (size_t)g.plane * g.nz, q;
+    int nonzero = 0;
+    for (q = 0; q < (size_t)c->xres * c->yres * c->zres; q++)
+        if (c->block[q] != 0) {
+            nonzero = 1;
+            break;
+        }
+    if (!nonzero) {
+        CK(hipMemsetAsync(dst, 0, n * 4, E->st));
+        return 0;
+    }
+    std::vector<float> h(n, 0.f);
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < g.nz; k++)
+        for (int j = 0; j < g.ny; j++)
+            for (int i = 0; i < g.nx; i++)
+                h[(size_t)i + (size_t)g.px * j + (size_t)g.plane * k] = (float)c->data[i][j][k];
+    CK(hipMemcpyAsync(dst, h.data(), n * 4, hipMemcpyHostToDevice, E->st));
+    CK(hipStreamSynchronize(E->st));
+    return 0;
+}
+
+static int
+download_field(Engine *E, const float *src, SvDCube *c)
+{
+    const GsvGrid &g = E->g;
+    size_t n = (size_t)g.plane * g.nz;
+    std::vector<float> h(n);
+    CK(hipMemcpyAsync(h.data(), src, n * 4, hipMemcpyDeviceToHost, E->st));
+    CK(hipStreamSynchronize(E->st));
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < g.nx; i++)
+        for (int j = 0; j < g.ny; j++)
+            for (int k = 0; k < g.nz; k++)
+                c->data[i][j][k] = h[(size_t)i + (size_t)g.px * j + (size_t)g.plane * k];
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_copyto(SvPool *mp, SvSet *set, SvGpuSync what)
+{
+    Engine *E = (Engine *)mp->gpu;
+    (void)set;
+    if (!E || !E->ready)
+        return 1;
+    CK(hipSetDevice(E->dev));
+    SvYeeData *d = mp->d;
+    if (what == SV_SYNC_ALL || what == SV_SYNC_EH || what == SV_SYNC_E) {
+        if (upload_field(E, d->ex, E->g.ex) || upload_field(E, d->ey, E->g.ey) || upload_field(E, d->ez, E->g.ez))
+            return 1;
+    }
+    if (what == SV_SYNC_ALL || what == SV_SYNC_EH || what == SV_SYNC_H) {
+        if (upload_field(E, d->hx, E->g.hx) || upload_field(E, d->hy, E->g.hy) || upload_field(E, d->hz, E->g.hz))
+            return 1;
+    }
+    CK(hipStreamSynchronize(E->st));
+    return 0;
+}
+
+static int
+fetch_aux(Engine *E, SvPool *mp, SvSet *set)
+{
+    /* traces and far-field bins, src3d/gpu.c:727-780 */
+    if (E->npts) {
+        int nst = set->sc.nsteps + 1;
+        std::vector<float> rec((size_t)nst * E->npts * 6);
+        CK(hipMemcpyAsync(rec.data(), E->d_rec, rec.size() * 4, hipMemcpyDeviceToHost, E->st));
+        CK(hipStreamSynchronize(E->st));
+        for (int n = 0; n < E->npts; n++)
+            for (int s = 0; s < nst; s++)
+                for (int q = 0; q < 6; q++)
+                    mp->out->outpointdata[n][(size_t)s * 6 + q] = rec[((size_t)s * E->npts + n) * 6 + q];
+    }
+    if (E->has_ff) {
+        int nd = E->ff.ndata;
+        std::vector<double> b((size_t)E->ff.nrs * 3 * nd);
+        CK(hipMemcpyAsync(b.data(), E->ff.bins, b.size() * 8, hipMemcpyDeviceToHost, E->st));
+        CK(hipStreamSynchronize(E->st));
+        for (int n = 0; n < E->ff.nrs; n++)
+            memcpy(mp->farfield->rpoints[n].ex, b.data() + (size_t)n * 3 * nd, 3 * (size_t)nd * 8);
+    }
+    if (E->has_dft) {
+        size_t cnt = (size_t)E->dft.ncells * E->dft.nw * 8;
+        CK(hipMemcpyAsync(mp->farfield->dft_data, E->dft.acc, cnt * 8, hipMemcpyDeviceToHost, E->st));
+        CK(hipStreamSynchronize(E->st));
+    }
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_copyfrom(SvPool *mp, SvSet *set, SvGpuSync what)
+{
+    Engine *E = (Engine *)mp->gpu;
+    if (!E || !E->ready)
+        return 1;
+    CK(hipSetDevice(E->dev));
+    SvYeeData *d = mp->d;
+    if (what == SV_SYNC_ALL || what == SV_SYNC_EH || what == SV_SYNC_E) {
+        if (download_field(E, E->g.ex, d->ex) || download_field(E, E->g.ey, d->ey) || download_field(E, E->g.ez, d->ez))
+            return 1;
+    }
+    if (what == SV_SYNC_ALL || what == SV_SYNC_EH || what == SV_SYNC_H) {
+        if (download_field(E, E->g.hx, d->hx) || download_field(E, E->g.hy, d->hy) || download_field(E, E->g.hz, d->hz))
+            return 1;
+    }
+    return fetch_aux(E, mp, set);
+}
+
+/* ------------------------------------------------------------------ */
+/* step entry points                                                    */
+/* ------------------------------------------------------------------ */
+
+#define ENGINE_OR_FAIL(mp)                                                          \
+    Engine *E = (Engine *)(mp)->gpu;                                                \
+    if (!E || !E->ready) {                                                          \
+        fprintf(stderr, "Error: GPU engine not initialised (%s)\n", __func__);     \
+        return 1;                                                                   \
+    }
+
+extern "C" int
+sv_pool_gpu_boundary_copy(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    (void)set;
+    if (E->has_liao)
+        CK(gsv_launch_liao_store(&E->g, &E->liao, E->st));
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_ystep_h(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    (void)set;
+    if (E->timing)
+        CK(hipEventRecord(E->ev[0], E->st));
+    CK(gsv_launch_hstep(&E->g, E->idbytes_h, E->bx, E->by, E->kchunk_h, E->st));
+    if (E->timing) {
+        float ms;
+        CK(hipEventRecord(E->ev[1], E->st));
+        CK(hipEventSynchronize(E->ev[1]));
+        CK(hipEventElapsedTime(&ms, E->ev[0], E->ev[1]));
+        E->t_h_ms += ms;
+        E->n_h++;
+    }
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_apply_source_hstep(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    if (E->has_src)
+        CK(gsv_launch_point_src(&E->g, &E->src, set->sc.step_act, 1, E->st));
+    if (E->has_tsf) {
+        CK(gsv_launch_tsf_line(&E->tsf, set->sc.step_act, E->st));
+        CK(gsv_launch_tsf_faces(&E->g, &E->tsf, 1, E->st));
+    }
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_boundary_hstep(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    (void)set;
+    if (E->has_per)
+        CK(gsv_launch_periodic(&E->g, &E->per, 1, E->st));
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_ystep_e(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    (void)set;
+    if (E->timing)
+        CK(hipEventRecord(E->ev[2], E->st));
+    CK(gsv_launch_estep(&E->g, E->idbytes_e, E->bx, E->by, E->kchunk_e, E->st));
+    if (E->timing) {
+        float ms;
+        CK(hipEventRecord(E->ev[3], E->st));
+        CK(hipEventSynchronize(E->ev[3]));
+        CK(hipEventElapsedTime(&ms, E->ev[2], E->ev[3]));
+        E->t_e_ms += ms;
+        E->n_e++;
+    }
+    /* the rest of the CPU boundary E step, in its order (src3d/boundary.c:521-936) */
+    if (E->has_liao) {
+        CK(gsv_launch_liao_faces(&E->g, &E->liao, E->st));
+        for (int axis = 0; axis < 3; axis++)
+            if (E->rezero[axis])
+                CK(gsv_launch_zero_planes(&E->g, axis, E->st));
+    }
+    if (E->has_per)
+        CK(gsv_launch_periodic(&E->g, &E->per, 0, E->st));
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_apply_source_estep(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    if (E->has_src)
+        CK(gsv_launch_point_src(&E->g, &E->src, set->sc.step_act, 0, E->st));
+    if (E->has_tsf)
+        CK(gsv_launch_tsf_faces(&E->g, &E->tsf, 0, E->st));
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_boundary_estep(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    (void)E;
+    (void)set;
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_getsum(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    (void)E;
+    if (set->so.nsums) {
+        fprintf(stderr, "Error: OUT_SUM is not supported by the MI355X engine\n");
+        return 1;
+    }
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_apply_farfield(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    if (E->has_ff)
+        CK(gsv_launch_farfield_td(&E->g, &E->ff, set->sc.step_act, E->st));
+    if (E->has_dft)
+        CK(gsv_launch_dft(&E->g, &E->dft, set->sc.step_act, E->st));
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_apply_pfarfield(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    (void)E;
+    (void)set;
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_getpoints(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    if (E->npts) {
+        if (set->sc.step_act > set->sc.nsteps)
+            return 1;
+        CK(gsv_launch_record(&E->g, E->d_ppos, E->npts,
+                             E->d_rec + (size_t)set->sc.step_act * E->npts * 6, E->st));
+    }
+    return 0;
+}
+
+extern "C" int
+sv_pool_gpu_getforce(SvPool *mp, SvSet *set)
+{
+    ENGINE_OR_FAIL(mp);
+    (void)E;
+    (void)set;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* engine helpers used by the session API                               */
+/* ------------------------------------------------------------------ */
+
+extern "C" int
+gsv_engine_sync(void *eng)
+{
+    Engine *E = (Engine *)eng;
+    CK(hipSetDevice(E->dev));
+    CK(hipStreamSynchronize(E->st));
+    return 0;
+}
+
+extern "C" void
+gsv_engine_set_timing(void *eng, int on)
+{
+    Engine *E = (Engine *)eng;
+    E->timing = on;
+    E->t_h_ms = E->t_e_ms = 0;
+    E->n_h = E->n_e = 0;
+}
+
+/* out[0..3] = total ms of H kernels, count, total ms of E kernels, count */
+extern "C" void
+gsv_engine_kernel_times(void *eng, double *out)
+{
+    Engine *E = (Engine *)eng;
+    out[0] = E->t_h_ms;
+    out[1] = (double)E->n_h;
+    out[2] = E->t_e_ms;
+    out[3] = (double)E->n_e;
+}
+
+/* info[0..7] = E classes, E id bytes, H classes, H id bytes, ADE, poles, bx*by, kchunk_e */
+extern "C" void
+gsv_engine_info(void *eng, int *info)
+{
+    Engine *E = (Engine *)eng;
+    info[0] = (int)E->etab.size();
+    info[1] = E->idbytes_e;
+    info[2] = (int)E->htab.size();
+    info[3] = E->idbytes_h;
+    info[4] = E->g.ade;
+    info[5] = E->g.ade_poles;
+    info[6] = E->bx * E->by;
+    info[7] = E->kchunk_e;
+}
+
+extern "C" int
+gsv_engine_get_field(void *eng, int comp, float *out)
+{
+    Engine *E = (Engine *)eng;
+    const GsvGrid &g = E->g;
+    float *src[6] = { g.ex, g.ey, g.ez, g.hx, g.hy, g.hz };
+    if (comp < 0 || comp > 5)
+        return 1;
+    size_t n = (size_t)g.plane * g.nz;
+    std::vector<float> h(n);
+    CK(hipSetDevice(E->dev));
+    CK(hipMemcpyAsync(h.data(), src[comp], n * 4, hipMemcpyDeviceToHost, E->st));
+    CK(hipStreamSynchronize(E->st));
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < g.nx; i++)
+        for (int j = 0; j < g.ny; j++)
+            for (int k = 0; k < g.nz; k++)
+                out[((size_t)i * g.ny + j) * g.nz + k] = h[(size_t)i + (size_t)g.px * j + (size_t)g.plane * k];
+    return 0;
+}
+
+extern "C" int
+gsv_engine_fetch_aux(void *eng)
+{
+    Engine *E = (Engine *)eng;
+    CK(hipSetDevice(E->dev));
+    return fetch_aux(E, E->mp, E->set);
+}
+
+extern "C" void
+gsv_engine_free(void *eng)
+{
+    Engine *E = (Engine *)eng;
+    if (!E)
+        return;
+    hipSetDevice(E->dev);
+    if (E->st)
+        hipStreamSynchronize(E->st);
+    for (void *p : E->allocs)
+        hipFree(p);
+    if (E->ready)
+        for (int q = 0; q < 4; q++)
+            hipEventDestroy(E->ev[q]);
+    if (E->st)
+        hipStreamDestroy(E->st);
+    delete E;
+}

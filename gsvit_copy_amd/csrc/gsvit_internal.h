@@ -1,0 +1,53 @@
+/*
+ * gsvit_internal.h -- declarations shared by the host-side modules of the
+ * MI355X engine (parameter reader, planner, log writers, device driver).
+ * Not part of the public C-ABI (include/ holds that).
+ */
+#ifndef GSVIT_INTERNAL_H
+#define GSVIT_INTERNAL_H
+
+#include "gsvit_settings.h"
+#include "gsvit_pool.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* settings.c */
+int sv_generate_waveform(int type, int nsteps, double wavelength, int pulse_width, double dt,
+                         double amplitude, double *vals);
+double sv_round_g6(double v);
+
+/* plan.c */
+int sv_material_is_tabulated(int type);   /* LINTAB, DRUDE, PLRC, ADE, CP, CP3 */
+int sv_material_is_dispersive(int type);  /* DRUDE, CP3, CP, ADE, PLRC */
+/* per-voxel epsilon/sigma as seen by the E update, src3d/yee.c:1317-1350 */
+int sv_pool_epsilon_sigma(const SvPool *mp, int i, int j, int k, double *eps, double *sig);
+/* per-voxel mu/sigast as seen by the H update, src3d/yee.c:1615-1645 */
+int sv_pool_mu_sigast(const SvPool *mp, int i, int j, int k, double *mu, double *sigast);
+/* component-wise averaged E coefficients, src3d/yee.c:1396-1492 */
+void sv_pool_get_cabs(const SvPool *mp, int i, int j, int k,
+                      double *cax, double *cay, double *caz, double *cbx, double *cby, double *cbz);
+/* dispersive-neighbour flags, src3d/yee.c:453-526; returns the material index used */
+int sv_pool_get_tbc(const SvPool *mp, int i, int j, int k, int *xtbc, int *ytbc, int *ztbc);
+/* PEC test of a tabulated voxel, src3d/yee.c:1250-1263 */
+int sv_pool_is_pec(const SvPool *mp, int i, int j, int k);
+/* TSF plane-wave geometry, src3d/source.c:53-165 */
+double sv_dcomp(int i, int j, int k, double theta, double phi, int i0, int i1, int j0, int j1, int k0, int k1);
+double sv_gex(double f, double theta, double phi, double psi);
+double sv_gey(double f, double theta, double phi, double psi);
+double sv_gez(double f, double theta, double phi, double psi);
+double sv_ghx(double f, double theta, double phi, double psi);
+double sv_ghy(double f, double theta, double phi, double psi);
+double sv_ghz(double f, double theta, double phi, double psi);
+/* TSF face test with the skip-near-material map, src3d/source.c:253-264 */
+int sv_tsf_face_active(const SvPool *mp, const SvSet *set, int i, int j, int k);
+/* far-field geometry helpers, src3d/farfield.c (dist, angle) */
+double sv_ff_dist(int x, int y, int z, int i, int j, int k);
+double sv_ff_angle(int i, int j, int k, int x, int y, int z, int face);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,89 @@
+/*
+ * kernels.h -- parameter blocks of the surface kernels and the launcher
+ * entry points of kernels.hip (all launches go to the engine's stream).
+ */
+#ifndef GSV_KERNELS_H
+#define GSV_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include "engine.h"
+
+/* TSF plane wave: the 1-D line in double, per-face coefficient arrays */
+typedef struct {
+    int i0, j0, k0, in, jn, kn;
+    /* d(i,j,k) = corr * (10 + ax (i - oi) + ay (j - oj) + az (k - ok)), src3d/source.c:53-107 */
+    double corr, ax, ay, az;
+    int oi, oj, ok;
+    /* polarisation factors of gex..ghz, src3d/source.c:130-165 */
+    double gex, gey, gez, ghx, ghy, ghz;
+    int res;                  /* 1-D line length */
+    double *je, *jh, *jb;     /* line fields and the 8 boundary scalars */
+    const double *wave;       /* incident waveform */
+    int ndata;
+    double multe, multh, coef;
+    /* per face (x0, xn, y0, yn, z0, zn): dt/dx/eps (E) and dt/dx/mu0 (H) per face
+     * cell, 0 where the face is skipped or (z0) the cell is PEC */
+    const float *coefE[6];
+    const float *coefH[6];
+} GsvTsf;
+
+typedef struct {
+    int n;                    /* number of point sources */
+    int nsteps;
+    const long long *pos;     /* cell index per source */
+    const float *vals;        /* [nsteps][n][6]: Ex Ey Ez Hx Hy Hz, 0 = no write */
+} GsvPointSrc;
+
+typedef struct {
+    int on[6];                /* x0, xn, y0, yn, z0, zn use Liao */
+    float *store[3];          /* [24][na][nb] per axis */
+    const float *coef[6];     /* per face cell, 3 coefficients (dx, dy, dz based) */
+} GsvLiao;
+
+typedef struct {
+    int x, y, z;              /* periodic along this axis */
+    int x0, xn, y0, yn, z0, zn;
+} GsvPeriodic;
+
+typedef struct {
+    int i0, j0, k0, i1, j1, k1;
+    int ix, iy, iz;
+    long long ncells;
+    int skip[6][5];           /* per face (x0, xn, y0, yn, z0, zn): on, amin, bmin, amax, bmax */
+    int nrs;
+    const int *ri, *rj, *rk, *istart;
+    int ndata;
+    double *bins;             /* [nrs][3][ndata] */
+    double dx, dy, dz, dt, c;
+} GsvFarfieldTD;
+
+typedef struct {
+    long long ncells;
+    const long long *cell;    /* cell index of each face cell */
+    const int *face;          /* 0/1 x faces, 2/3 y faces, 4/5 z faces */
+    int nw;
+    const double *omega;
+    double *acc;              /* [ncells][nw][8] */
+    double dt;
+} GsvDft;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+hipError_t gsv_launch_hstep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st);
+hipError_t gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st);
+hipError_t gsv_launch_tsf_line(const GsvTsf *t, int step, hipStream_t st);
+hipError_t gsv_launch_tsf_faces(const GsvGrid *g, const GsvTsf *t, int hside, hipStream_t st);
+hipError_t gsv_launch_point_src(const GsvGrid *g, const GsvPointSrc *s, int step, int hside, hipStream_t st);
+hipError_t gsv_launch_record(const GsvGrid *g, const long long *pos, int n, float *rec, hipStream_t st);
+hipError_t gsv_launch_liao_store(const GsvGrid *g, const GsvLiao *L, hipStream_t st);
+hipError_t gsv_launch_liao_faces(const GsvGrid *g, const GsvLiao *L, hipStream_t st);
+hipError_t gsv_launch_zero_planes(const GsvGrid *g, int axis, hipStream_t st);
+hipError_t gsv_launch_periodic(const GsvGrid *g, const GsvPeriodic *p, int hside, hipStream_t st);
+hipError_t gsv_launch_farfield_td(const GsvGrid *g, const GsvFarfieldTD *f, int step, hipStream_t st);
+hipError_t gsv_launch_dft(const GsvGrid *g, const GsvDft *d, int step, hipStream_t st);
+#ifdef __cplusplus
+}
+#endif
+
+#endif

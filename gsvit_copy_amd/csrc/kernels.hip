@@ -1,0 +1,970 @@
+/*
+ * kernels.hip -- CDNA4 (gfx950) kernels of the FDTD time step.
+ *
+ * Volume kernels (one wave64 covers 64 consecutive x cells of one row, a block
+ * covers BX x BY columns and marches up a chunk of z planes, keeping the
+ * previous plane's values in registers):
+ *   k_hstep   Yee H update (src3d/yee.c:145-265) fused with the CPML H psi
+ *             update, H correction and far-plane zeroing (src3d/boundary.c:347-518)
+ *   k_estep   Yee E update (src3d/yee.c:528-1247: per-component averaged
+ *             coefficients via classes, ADE branch with its inline CPML, PEC)
+ *             fused with the CPML E psi update, E correction and near-plane
+ *             zeroing (src3d/boundary.c:691-887)
+ * Surface kernels:
+ *   k_tsf_line            1-D auxiliary line step, src3d/1dpool.c:83-162
+ *   k_tsf_h / k_tsf_e     TSF face corrections, src3d/source.c:547-783, 1464-1657
+ *   k_point_src           hard point sources, src3d/source.c:405-437, 1441-1462
+ *   k_liao_store / k_liao_face   Liao ABC, src3d/boundary.c:549-689, 938-1066
+ *   k_zero_planes         CPML near-plane zeroing after Liao on mixed boundaries
+ *   k_periodic_h / _e     interior periodic plane copies, src3d/boundary.c:304-346, 889-932
+ *   k_record_points       point traces, src3d/output.c:340-352
+ *   k_farfield_td         time-domain Kirchhoff bins, src3d/farfield.c:174-407
+ *   k_dft_faces           running DFT of tangential fields on the NFFF box
+ */
+#include <hip/hip_runtime.h>
+#include "engine.h"
+#include "kernels.h"
+
+#define LD(p) __builtin_nontemporal_load(p)
+
+/* ------------------------------------------------------------------ */
+/* H update                                                             */
+/* ------------------------------------------------------------------ */
+
+template <typename ID>
+__global__ __launch_bounds__(256) void
+k_hstep(GsvGrid g, int kchunk)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int k0 = blockIdx.z * kchunk;
+    const int k1 = min(k0 + kchunk, nz);
+    const long long P = g.plane;
+    const int px = g.px;
+    const ID *cls = (const ID *)g.hcls;
+    long long c = (long long)i + (long long)px * j + P * k0;
+
+    const float rkx = g.rkhx[i], rky = g.rkhy[j];
+    const bool iin = (i >= 1 && i < nx - 1), jin = (j >= 1 && j < ny - 1);
+    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
+    const bool sx0 = F0.on && i < F0.depth - 1 && jin;
+    const int irn = i - (nx - F1.depth);
+    const bool sxn = F1.on && i >= nx - F1.depth + 1 && i < nx - 1 && jin;
+    const bool sy0 = F2.on && j < F2.depth - 1 && iin;
+    const int jrn = j - (ny - F3.depth);
+    const bool syn = F3.on && j >= ny - F3.depth + 1 && j < ny - 1 && iin;
+    const bool zx = F1.on && i == nx - 1 && jin;
+    const bool zy = F3.on && j == ny - 1 && iin;
+    const bool ijin = iin && jin;
+
+    float exk = g.ex[c], eyk = g.ey[c];
+    for (int k = k0; k < k1; ++k, c += P) {
+        const float ezk = g.ez[c];
+        const float exk1 = g.ex[c + P], eyk1 = g.ey[c + P];
+        const float ezj1 = g.ez[c + px], exj1 = g.ex[c + px];
+        const float ezi1 = g.ez[c + 1], eyi1 = g.ey[c + 1];
+        const GsvHCoef hc = g.htab[cls[c]];
+        const float rkz = g.rkhz[k];
+        float hx = g.hx[c], hy = g.hy[c], hz = g.hz[c];
+        const bool kin = (k >= 1 && k < nz - 1);
+
+        if (!(j == ny - 1 || k == nz - 1))
+            hx = hc.da * hx + hc.db[0] * ((eyk1 - eyk) * rkz - (ezj1 - ezk) * rky);
+        if (!(i == nx - 1 || k == nz - 1))
+            hy = hc.da * hy + hc.db[1] * ((ezi1 - ezk) * rkx - (exk1 - exk) * rkz);
+        if (!(i == nx - 1 || j == ny - 1))
+            hz = hc.da * hz + hc.db[2] * ((exj1 - exk) * rky - (eyi1 - eyk) * rkx);
+
+        if (kin) {
+            if (sx0) {
+                const long long p = ((long long)k * ny + j) * F0.depth + i;
+                const float b = F0.bh[i], cc = F0.ch_d[i];
+                const float a1 = b * F0.ph1[p] + cc * (eyi1 - eyk);
+                const float a2 = b * F0.ph2[p] + cc * (ezi1 - ezk);
+                F0.ph1[p] = a1;
+                F0.ph2[p] = a2;
+                hy += g.dx * hc.dbc * a2;
+                hz -= g.dx * hc.dbc * a1;
+            }
+            if (sxn) {
+                const long long p = ((long long)k * ny + j) * F1.depth + irn;
+                const float b = F1.bh[irn], cc = F1.ch_d[irn];
+                const float a1 = b * F1.ph1[p] + cc * (eyi1 - eyk);
+                const float a2 = b * F1.ph2[p] + cc * (ezi1 - ezk);
+                F1.ph1[p] = a1;
+                F1.ph2[p] = a2;
+                hy += g.dx * hc.dbc * a2;
+                hz -= g.dx * hc.dbc * a1;
+            }
+            if (sy0) {
+                const long long p = ((long long)k * F2.depth + j) * px + i;
+                const float b = F2.bh[j], cc = F2.ch_d[j];
+                const float a1 = b * F2.ph1[p] + cc * (ezj1 - ezk);
+                const float a2 = b * F2.ph2[p] + cc * (exj1 - exk);
+                F2.ph1[p] = a1;
+                F2.ph2[p] = a2;
+                hx -= g.dy * hc.dbc * a1;
+                hz += g.dy * hc.dbc * a2;
+            }
+            if (syn) {
+                const long long p = ((long long)k * F3.depth + jrn) * px + i;
+                const float b = F3.bh[jrn], cc = F3.ch_d[jrn];
+                const float a1 = b * F3.ph1[p] + cc * (ezj1 - ezk);
+                const float a2 = b * F3.ph2[p] + cc * (exj1 - exk);
+                F3.ph1[p] = a1;
+                F3.ph2[p] = a2;
+                hx -= g.dy * hc.dbc * a1;
+                hz += g.dy * hc.dbc * a2;
+            }
+        }
+        if (ijin) {
+            if (F4.on && k < F4.depth - 1) {
+                const long long p = ((long long)k * ny + j) * px + i;
+                const float b = F4.bh[k], cc = F4.ch_d[k];
+                const float a1 = b * F4.ph1[p] + cc * (exk1 - exk);
+                const float a2 = b * F4.ph2[p] + cc * (eyk1 - eyk);
+                F4.ph1[p] = a1;
+                F4.ph2[p] = a2;
+                hy -= g.dz * hc.dbc * a1;
+                hx += g.dz * hc.dbc * a2;
+            }
+            if (F5.on && k >= nz - F5.depth + 1 && k < nz - 1) {
+                const int kr = k - (nz - F5.depth);
+                const long long p = ((long long)kr * ny + j) * px + i;
+                const float b = F5.bh[kr], cc = F5.ch_d[kr];
+                const float a1 = b * F5.ph1[p] + cc * (exk1 - exk);
+                const float a2 = b * F5.ph2[p] + cc * (eyk1 - eyk);
+                F5.ph1[p] = a1;
+                F5.ph2[p] = a2;
+                hy -= g.dz * hc.dbc * a1;
+                hx += g.dz * hc.dbc * a2;
+            }
+        }
+        if ((kin && (zx || zy)) || (F5.on && k == nz - 1 && ijin)) {
+            hx = 0.f;
+            hy = 0.f;
+            hz = 0.f;
+        }
+        g.hx[c] = hx;
+        g.hy[c] = hy;
+        g.hz[c] = hz;
+        exk = exk1;
+        eyk = eyk1;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* E update                                                             */
+/* ------------------------------------------------------------------ */
+
+/* ADE branch of one cell (src3d/yee.c:914-1108); updates the components whose
+ * flag is clear, runs the branch's own CPML psi update and advances the
+ * polarisation state.  Rarely taken: dispersive cells and their +1 neighbours. */
+__device__ __noinline__ void
+ade_cell(const GsvGrid &g, long long c, int i, int j, int k, unsigned fl, float &ex, float &ey, float &ez,
+         float hx, float hy, float hz, float hxkm1, float hykm1, float hzi, float hyi, float hzj, float hxj,
+         float rkx, float rky, float rkz)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
+    const GsvAde &m = g.adem[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
+    const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
+    const float vexp = ex, veyp = ey, vezp = ez;
+    const float expv = g.exp_[c], eypv = g.eyp[c], ezpv = g.ezp[c];
+    const float dpx = g.dpx[c], dpy = g.dpy[c], dpz = g.dpz[c];
+    const float dpxp = g.dpxp[c], dpyp = g.dpyp[c], dpzp = g.dpzp[c];
+    float px0 = 0, px1 = 0, py0 = 0, py1 = 0, pz0 = 0, pz1 = 0;
+    float pxp0 = 0, pxp1 = 0, pyp0 = 0, pyp1 = 0, pzp0 = 0, pzp1 = 0;
+    if (m.poles) {
+        px0 = g.px0[c]; px1 = g.px1[c]; py0 = g.py0[c]; py1 = g.py1[c]; pz0 = g.pz0[c]; pz1 = g.pz1[c];
+        pxp0 = g.pxp0[c]; pxp1 = g.pxp1[c]; pyp0 = g.pyp0[c]; pyp1 = g.pyp1[c]; pzp0 = g.pzp0[c]; pzp1 = g.pzp1[c];
+    }
+
+    if (j > 0 && k > 0 && !tx)
+        ex = m.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx
+             + m.c1 * (-m.a0 * dpxp + (1.f - m.a1) * dpx)
+             + m.c1 * (-m.bp0[0] * pxp0 + (1.f - m.bp1[0]) * px0)
+             + m.c1 * (-m.bp0[1] * pxp1 + (1.f - m.bp1[1]) * px1)
+             + m.c2 * expv + m.c3 * ex;
+    if (i > 0 && k > 0 && !ty)
+        ey = m.c0 * ((hx - hxkm1) * rkz - (hz - hzi) * rkx) / g.dy
+             + m.c1 * (-m.a0 * dpyp + (1.f - m.a1) * dpy)
+             + m.c1 * (-m.bp0[0] * pyp0 + (1.f - m.bp1[0]) * py0)
+             + m.c1 * (-m.bp0[1] * pyp1 + (1.f - m.bp1[1]) * py1)
+             + m.c2 * eypv + m.c3 * ey;
+    if (i > 0 && j > 0 && !tz)
+        ez = m.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz
+             + m.c1 * (-m.a0 * dpzp + (1.f - m.a1) * dpz)
+             + m.c1 * (-m.bp0[0] * pzp0 + (1.f - m.bp1[0]) * pz0)
+             + m.c1 * (-m.bp0[1] * pzp1 + (1.f - m.bp1[1]) * pz1)
+             + m.c2 * ezpv + m.c3 * ez;
+
+    /* the branch's own CPML update; its x0 correction multiplies by a zero cb */
+    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
+    if (F0.on && i > 1 && i < F0.depth) {
+        const long long p = ((long long)k * ny + j) * F0.depth + i;
+        F0.pe1[p] = F0.be[i] * F0.pe1[p] + F0.ce_raw[i] * (hz - hzi) / g.dy;
+        F0.pe2[p] = F0.be[i] * F0.pe2[p] + F0.ce_raw[i] * (hy - hyi) / g.dz;
+    }
+    if (F2.on && j > 1 && j < F2.depth) {
+        const long long p = ((long long)k * F2.depth + j) * px + i;
+        const float a1 = F2.be[j] * F2.pe1[p] + F2.ce_raw[j] * (hz - hzj) / g.dz;
+        const float a2 = F2.be[j] * F2.pe2[p] + F2.ce_raw[j] * (hx - hxj) / g.dx;
+        F2.pe1[p] = a1;
+        F2.pe2[p] = a2;
+        if (!(i == 0 || k == 0) && !tx) ex += g.dx * m.cbm[0] * a1;
+        if (!(i == 0 || k == 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
+    }
+    if (F4.on && k > 1 && k < F4.depth) {
+        const long long p = ((long long)k * ny + j) * px + i;
+        const float a1 = F4.be[k] * F4.pe1[p] + F4.ce_raw[k] * (hx - hxkm1) / g.dy;
+        const float a2 = F4.be[k] * F4.pe2[p] + F4.ce_raw[k] * (hy - hykm1) / g.dx;
+        F4.pe1[p] = a1;
+        F4.pe2[p] = a2;
+        if (!(i == 0 || j == 0) && !ty) ey += g.dy * m.cbm[1] * a1;
+        if (!(i == 0 || j == 0) && !tx) ex -= g.dx * m.cbm[0] * a2;
+    }
+    if (F1.on && i >= nx - F1.depth + 1) {
+        const int ir = i - (nx - F1.depth);
+        const long long p = ((long long)k * ny + j) * F1.depth + ir;
+        const float a1 = F1.be[ir] * F1.pe1[p] + F1.ce_raw[ir] * (hz - hzi) / g.dy;
+        const float a2 = F1.be[ir] * F1.pe2[p] + F1.ce_raw[ir] * (hy - hyi) / g.dz;
+        F1.pe1[p] = a1;
+        F1.pe2[p] = a2;
+        if (!(j == 0 || k == 0) && !ty) ey -= g.dy * m.cbm[1] * a1;
+        if (!(j == 0 || k == 0) && !tz) ez += g.dz * m.cbm[2] * a2;
+    }
+    if (F3.on && j >= ny - F3.depth + 1) {
+        const int jr = j - (ny - F3.depth);
+        const long long p = ((long long)k * F3.depth + jr) * px + i;
+        const float a1 = F3.be[jr] * F3.pe1[p] + F3.ce_raw[jr] * (hz - hzj) / g.dx;
+        const float a2 = F3.be[jr] * F3.pe2[p] + F3.ce_raw[jr] * (hx - hxj) / g.dz;
+        F3.pe1[p] = a1;
+        F3.pe2[p] = a2;
+        if (!(i == 0 || k == 0) && !tx) ex += g.dx * m.cbm[0] * a1;
+        if (!(i == 0 || k == 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
+    }
+    if (F5.on && k >= nz - F5.depth + 1) {
+        const int kr = k - (nz - F5.depth);
+        const long long p = ((long long)kr * ny + j) * px + i;
+        const float a1 = F5.be[kr] * F5.pe1[p] + F5.ce_raw[kr] * (hx - hxkm1) / g.dy;
+        const float a2 = F5.be[kr] * F5.pe2[p] + F5.ce_raw[kr] * (hy - hykm1) / g.dx;
+        F5.pe1[p] = a1;
+        F5.pe2[p] = a2;
+        if (!(i == 0 || j == 0) && !ty) ey += g.dy * m.cbm[1] * a1;
+        if (!(i == 0 || j == 0) && !tx) ex -= g.dx * m.cbm[0] * a2;
+    }
+
+    /* polarisation state (all three components, with the values at this point) */
+    g.dpx[c] = m.a0 * dpxp + m.a1 * dpx + m.a2 * (expv + 2.f * vexp + ex);
+    g.dpy[c] = m.a0 * dpyp + m.a1 * dpy + m.a2 * (eypv + 2.f * veyp + ey);
+    g.dpz[c] = m.a0 * dpzp + m.a1 * dpz + m.a2 * (ezpv + 2.f * vezp + ez);
+    if (m.poles) {
+        g.px0[c] = m.bp0[0] * pxp0 + m.bp1[0] * px0 + m.bp2[0] * expv + m.bp3[0] * vexp + m.bp4[0] * ex;
+        g.px1[c] = m.bp0[1] * pxp1 + m.bp1[1] * px1 + m.bp2[1] * expv + m.bp3[1] * vexp + m.bp4[1] * ex;
+        g.py0[c] = m.bp0[0] * pyp0 + m.bp1[0] * py0 + m.bp2[0] * eypv + m.bp3[0] * veyp + m.bp4[0] * ey;
+        g.py1[c] = m.bp0[1] * pyp1 + m.bp1[1] * py1 + m.bp2[1] * eypv + m.bp3[1] * veyp + m.bp4[1] * ey;
+        g.pz0[c] = m.bp0[0] * pzp0 + m.bp1[0] * pz0 + m.bp2[0] * ezpv + m.bp3[0] * vezp + m.bp4[0] * ez;
+        g.pz1[c] = m.bp0[1] * pzp1 + m.bp1[1] * pz1 + m.bp2[1] * ezpv + m.bp3[1] * vezp + m.bp4[1] * ez;
+        g.pxp0[c] = px0; g.pxp1[c] = px1; g.pyp0[c] = py0; g.pyp1[c] = py1; g.pzp0[c] = pz0; g.pzp1[c] = pz1;
+    }
+    g.exp_[c] = vexp;
+    g.eyp[c] = veyp;
+    g.ezp[c] = vezp;
+    g.dpxp[c] = dpx;
+    g.dpyp[c] = dpy;
+    g.dpzp[c] = dpz;
+}
+
+template <typename ID>
+__global__ __launch_bounds__(256) void
+k_estep(GsvGrid g, int kchunk)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int k0 = blockIdx.z * kchunk;
+    const int k1 = min(k0 + kchunk, nz);
+    const long long P = g.plane;
+    const int px = g.px;
+    const ID *cls = (const ID *)g.ecls;
+    long long c = (long long)i + (long long)px * j + P * k0;
+
+    const float rkx = g.rkex[i], rky = g.rkey[j];
+    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
+    const bool sx0 = F0.on && i >= 1 && i < F0.depth;
+    const int irn = i - (nx - F1.depth);
+    const bool sxn = F1.on && i >= nx - F1.depth + 1;
+    const bool sy0 = F2.on && j >= 1 && j < F2.depth;
+    const int jrn = j - (ny - F3.depth);
+    const bool syn = F3.on && j >= ny - F3.depth + 1;
+    const bool zplane_xy = (F0.on && i == 0) || (F2.on && j == 0);
+
+    float hxkm1 = g.hx[c - P], hykm1 = g.hy[c - P];
+    for (int k = k0; k < k1; ++k, c += P) {
+        const float hx = g.hx[c], hy = g.hy[c], hz = g.hz[c];
+        const float hzj = g.hz[c - px], hxj = g.hx[c - px];
+        const float hzi = g.hz[c - 1], hyi = g.hy[c - 1];
+        const float rkz = g.rkez[k];
+        const GsvECoef ec = g.etab[cls[c]];
+        float ex = g.ex[c], ey = g.ey[c], ez = g.ez[c];
+        const unsigned fl = ec.flags;
+        const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
+
+        if (fl & GSV_E_ADE)
+            ade_cell(g, c, i, j, k, fl, ex, ey, ez, hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
+
+        if (tx && j > 0 && k > 0)
+            ex = ec.ca[0] * ex + ec.cb[0] * ((hz - hzj) * rky - (hy - hykm1) * rkz);
+        if (ty && i > 0 && k > 0)
+            ey = ec.ca[1] * ey + ec.cb[1] * ((hx - hxkm1) * rkz - (hz - hzi) * rkx);
+        if (tz && i > 0 && j > 0)
+            ez = ec.ca[2] * ez + ec.cb[2] * ((hy - hyi) * rkx - (hx - hxj) * rky);
+
+        if (fl & (GSV_E_PECX | GSV_E_PECY | GSV_E_PECZ)) {
+            if (fl & GSV_E_PECX) ex = 0.f;
+            if (fl & GSV_E_PECY) ey = 0.f;
+            if (fl & GSV_E_PECZ) ez = 0.f;
+        }
+
+        /* CPML E psi and correction, src3d/boundary.c:691-887 */
+        if (sx0) {
+            const long long p = ((long long)k * ny + j) * F0.depth + i;
+            const float a1 = F0.be[i] * F0.pe1[p] + F0.ce_d[i] * (hz - hzi);
+            const float a2 = F0.be[i] * F0.pe2[p] + F0.ce_d[i] * (hy - hyi);
+            F0.pe1[p] = a1;
+            F0.pe2[p] = a2;
+            if (j != 0 && k != 0) {
+                if (ty) ey -= g.dx * ec.cb[1] * a1;
+                if (tz) ez += g.dx * ec.cb[2] * a2;
+            }
+        }
+        if (sy0) {
+            const long long p = ((long long)k * F2.depth + j) * px + i;
+            const float a1 = F2.be[j] * F2.pe1[p] + F2.ce_d[j] * (hz - hzj);
+            const float a2 = F2.be[j] * F2.pe2[p] + F2.ce_d[j] * (hx - hxj);
+            F2.pe1[p] = a1;
+            F2.pe2[p] = a2;
+            if (i != 0 && k != 0) {
+                if (tx) ex += g.dy * ec.cb[0] * a1;
+                if (tz) ez -= g.dy * ec.cb[2] * a2;
+            }
+        }
+        if (F4.on && k >= 1 && k < F4.depth) {
+            const long long p = ((long long)k * ny + j) * px + i;
+            const float a1 = F4.be[k] * F4.pe1[p] + F4.ce_d[k] * (hx - hxkm1);
+            const float a2 = F4.be[k] * F4.pe2[p] + F4.ce_d[k] * (hy - hykm1);
+            F4.pe1[p] = a1;
+            F4.pe2[p] = a2;
+            if (i != 0 && j != 0) {
+                if (ty) ey += g.dx * ec.cb[1] * a1;
+                /* the reference gates this Ex correction on the z flag (src3d/boundary.c:800-801) */
+                if (tz) ex -= g.dx * ec.cb[0] * a2;
+            }
+        }
+        if (sxn) {
+            const long long p = ((long long)k * ny + j) * F1.depth + irn;
+            const float a1 = F1.be[irn] * F1.pe1[p] + F1.ce_d[irn] * (hz - hzi);
+            const float a2 = F1.be[irn] * F1.pe2[p] + F1.ce_d[irn] * (hy - hyi);
+            F1.pe1[p] = a1;
+            F1.pe2[p] = a2;
+            if (j != 0 && k != 0) {
+                if (ty) ey -= g.dx * ec.cb[1] * a1;
+                if (tz) ez += g.dx * ec.cb[2] * a2;
+            }
+        }
+        if (syn) {
+            const long long p = ((long long)k * F3.depth + jrn) * px + i;
+            const float a1 = F3.be[jrn] * F3.pe1[p] + F3.ce_d[jrn] * (hz - hzj);
+            const float a2 = F3.be[jrn] * F3.pe2[p] + F3.ce_d[jrn] * (hx - hxj);
+            F3.pe1[p] = a1;
+            F3.pe2[p] = a2;
+            if (i != 0 && k != 0) {
+                if (tx) ex += g.dy * ec.cb[0] * a1;
+                if (tz) ez -= g.dy * ec.cb[2] * a2;
+            }
+        }
+        if (F5.on && k >= nz - F5.depth + 1) {
+            const int kr = k - (nz - F5.depth);
+            const long long p = ((long long)kr * ny + j) * px + i;
+            const float a1 = F5.be[kr] * F5.pe1[p] + F5.ce_d[kr] * (hx - hxkm1);
+            const float a2 = F5.be[kr] * F5.pe2[p] + F5.ce_d[kr] * (hy - hykm1);
+            F5.pe1[p] = a1;
+            F5.pe2[p] = a2;
+            if (i != 0 && j != 0) {
+                if (ty) ey += g.dx * ec.cb[1] * a1;
+                if (tx) ex -= g.dx * ec.cb[0] * a2;
+            }
+        }
+        if (zplane_xy || (F4.on && k == 0)) {
+            ex = 0.f;
+            ey = 0.f;
+            ez = 0.f;
+        }
+        g.ex[c] = ex;
+        g.ey[c] = ey;
+        g.ez[c] = ez;
+        hxkm1 = hx;
+        hykm1 = hy;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* TSF plane wave                                                       */
+/* ------------------------------------------------------------------ */
+
+__device__ __forceinline__ double
+line_val(const double *data, int res, double x)
+{
+    /* cubic interpolation of the 1-D line, src3d/dline.c:89-112; samples outside the line are 0 */
+    const int l = (int)floor(x);
+    const double a = x - (double)l;
+#define DV(q) (((q) >= 0 && (q) < res) ? data[q] : 0.0)
+    if (x >= 1 && x < (res - 1)) {
+        double w1 = a + 1, w2 = a, w3 = 1 - a, w4 = 2 - a;
+        w1 = 4 - 8 * w1 + 5 * w1 * w1 - w1 * w1 * w1;
+        w2 = 1 - 2 * w2 * w2 + w2 * w2 * w2;
+        w3 = 1 - 2 * w3 * w3 + w3 * w3 * w3;
+        w4 = 4 - 8 * w4 + 5 * w4 * w4 - w4 * w4 * w4;
+        return w1 * DV(l - 1) + w2 * DV(l) + w3 * DV(l + 1) + w4 * DV(l + 2);
+    } else if ((x < 1 && x >= 0) || (x >= (res - 1) && x < res))
+        return (1 - a) * DV(l) + a * DV(l + 1);
+#undef DV
+    return 0;
+}
+
+__device__ __forceinline__ double
+tsf_d(const GsvTsf &t, int i, int j, int k)
+{
+    return t.corr * (10.0 + t.ax * (double)(i - t.oi) + t.ay * (double)(j - t.oj) + t.az * (double)(k - t.ok));
+}
+
+__global__ __launch_bounds__(1024) void
+k_tsf_line(GsvTsf t, int step)
+{
+    /* one step of the 1-D line (double precision), src3d/1dpool.c:83-162 */
+    const int res = t.res;
+    double *e = t.je, *h = t.jh, *b = t.jb;
+    for (int q = threadIdx.x + 1; q < res; q += blockDim.x)
+        e[q] += t.multe * (h[q - 1] - h[q]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        e[0] = b[1] + t.coef * (e[1] - b[0]);
+        h[0] = b[3] + t.coef * (h[1] - b[2]);
+        e[res - 1] = b[5] + t.coef * (e[res - 2] - b[4]);
+        h[res - 1] = b[7] + t.coef * (h[res - 2] - b[6]);
+        b[0] = e[0]; b[1] = e[1]; b[2] = h[0]; b[3] = h[1];
+        b[4] = e[res - 1]; b[5] = e[res - 2]; b[6] = h[res - 1]; b[7] = h[res - 2];
+        e[0] = (step < t.ndata) ? t.wave[step] : 0.0;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < res - 1; q += blockDim.x)
+        h[q] += t.multh * (e[q] - e[q + 1]);
+}
+
+/* H corrections on the two faces normal to `axis` (blockIdx.z: 0 near, 1 far) */
+__global__ __launch_bounds__(256) void
+k_tsf_h(GsvGrid g, GsvTsf t, int axis)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    const int far = blockIdx.z;
+    int na, nb;
+    if (axis == 0) { na = t.jn - t.j0 + 1; nb = t.kn - t.k0 + 1; }
+    else if (axis == 1) { na = t.in - t.i0 + 1; nb = t.kn - t.k0 + 1; }
+    else { na = t.in - t.i0 + 1; nb = t.jn - t.j0 + 1; }
+    if (a >= na || b >= nb)
+        return;
+    const float cf = t.coefH[axis * 2 + far][a + (long long)na * b];
+    if (cf == 0.f)
+        return;
+    const long long P = g.plane;
+    const int px = g.px;
+    if (axis == 0) {
+        const int j = t.j0 + a, k = t.k0 + b;
+        const int ip = far ? t.in : t.i0;
+        const double ve = line_val(t.je, t.res, tsf_d(t, ip, j, k));
+        const int iw = far ? t.in : t.i0 - 1;
+        const long long c = iw + (long long)px * j + P * k;
+        const float s = far ? 1.f : -1.f;
+        if (k < t.kn) g.hy[c] += s * (float)(cf * t.gez * ve);
+        if (j < t.jn) g.hz[c] -= s * (float)(cf * t.gey * ve);
+    } else if (axis == 1) {
+        const int i = t.i0 + a, k = t.k0 + b;
+        const int jp = far ? t.jn : t.j0;
+        const double ve = line_val(t.je, t.res, tsf_d(t, i, jp, k));
+        const int jw = far ? t.jn : t.j0 - 1;
+        const long long c = i + (long long)px * jw + P * k;
+        const float s = far ? 1.f : -1.f;
+        if (k < t.kn) g.hx[c] -= s * (float)(cf * t.gez * ve);
+        if (i < t.in) g.hz[c] += s * (float)(cf * t.gex * ve);
+    } else {
+        const int i = t.i0 + a, j = t.j0 + b;
+        const int kp = far ? t.kn : t.k0;
+        const double ve = line_val(t.je, t.res, tsf_d(t, i, j, kp));
+        const int kw = far ? t.kn : t.k0 - 1;
+        const long long c = i + (long long)px * j + P * kw;
+        const float s = far ? 1.f : -1.f;
+        if (j < t.jn) g.hx[c] += s * (float)(cf * t.gey * ve);
+        if (i < t.in) g.hy[c] -= s * (float)(cf * t.gex * ve);
+    }
+}
+
+/* E corrections on the two faces normal to `axis` */
+__global__ __launch_bounds__(256) void
+k_tsf_e(GsvGrid g, GsvTsf t, int axis)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    const int far = blockIdx.z;
+    int na, nb;
+    if (axis == 0) { na = t.jn - t.j0 + 1; nb = t.kn - t.k0 + 1; }
+    else if (axis == 1) { na = t.in - t.i0 + 1; nb = t.kn - t.k0 + 1; }
+    else { na = t.in - t.i0 + 1; nb = t.jn - t.j0 + 1; }
+    if (a >= na || b >= nb)
+        return;
+    const float cf = t.coefE[axis * 2 + far][a + (long long)na * b];
+    if (cf == 0.f)
+        return;
+    const long long P = g.plane;
+    const int px = g.px;
+    const float s = far ? -1.f : 1.f;
+    if (axis == 0) {
+        const int j = t.j0 + a, k = t.k0 + b;
+        const int ip = far ? t.in : t.i0 - 1;
+        const double vh = line_val(t.jh, t.res, tsf_d(t, ip, j, k));
+        const long long c = (far ? t.in : t.i0) + (long long)px * j + P * k;
+        if (j < t.jn) g.ey[c] += s * (float)(cf * t.ghz * vh);
+        if (k < t.kn) g.ez[c] -= s * (float)(cf * t.ghy * vh);
+    } else if (axis == 1) {
+        const int i = t.i0 + a, k = t.k0 + b;
+        const int jp = far ? t.jn : t.j0 - 1;
+        const double vh = line_val(t.jh, t.res, tsf_d(t, i, jp, k));
+        const long long c = i + (long long)px * (far ? t.jn : t.j0) + P * k;
+        if (i < t.in) g.ex[c] -= s * (float)(cf * t.ghz * vh);
+        if (k < t.kn) g.ez[c] += s * (float)(cf * t.ghx * vh);
+    } else {
+        const int i = t.i0 + a, j = t.j0 + b;
+        const int kp = far ? t.kn : t.k0 - 1;
+        const double vh = line_val(t.jh, t.res, tsf_d(t, i, j, kp));
+        const long long c = i + (long long)px * j + P * (far ? t.kn : t.k0);
+        if (i < t.in) g.ex[c] += s * (float)(cf * t.ghy * vh);
+        if (j < t.jn) g.ey[c] -= s * (float)(cf * t.ghx * vh);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* point sources, traces, boundaries                                    */
+/* ------------------------------------------------------------------ */
+
+__global__ void
+k_point_src(GsvGrid g, GsvPointSrc s, int step, int hside)
+{
+    /* sources in order, one thread: later sources win on a shared cell */
+    if (threadIdx.x != 0 || blockIdx.x != 0 || step >= s.nsteps)
+        return;
+    for (int n = 0; n < s.n; n++) {
+        const long long c = s.pos[n];
+        const float *v = s.vals + ((long long)step * s.n + n) * 6 + (hside ? 3 : 0);
+        float *f0 = hside ? g.hx : g.ex, *f1 = hside ? g.hy : g.ey, *f2 = hside ? g.hz : g.ez;
+        if (v[0] != 0.f) f0[c] = v[0];
+        if (v[1] != 0.f) f1[c] = v[1];
+        if (v[2] != 0.f) f2[c] = v[2];
+    }
+}
+
+__global__ void
+k_record_points(GsvGrid g, const long long *pos, int n, float *rec)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n)
+        return;
+    const long long c = pos[q];
+    float *r = rec + (long long)q * 6;
+    r[0] = g.ex[c];
+    r[1] = g.ey[c];
+    r[2] = g.ez[c];
+    r[3] = g.hx[c];
+    r[4] = g.hy[c];
+    r[5] = g.hz[c];
+}
+
+__device__ __forceinline__ long long
+face_cell(const GsvGrid &g, int axis, int plane_index, int a, int b)
+{
+    if (axis == 0) return plane_index + (long long)g.px * a + g.plane * b;
+    if (axis == 1) return a + (long long)g.px * plane_index + g.plane * b;
+    return a + (long long)g.px * b + g.plane * plane_index;
+}
+
+/* store the two outer planes of each Liao face, src3d/boundary.c:938-1066 */
+__global__ void
+k_liao_store(GsvGrid g, GsvLiao L, int axis)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    int na, nb, n;
+    if (axis == 0) { na = g.ny; nb = g.nz; n = g.nx; }
+    else if (axis == 1) { na = g.nx; nb = g.nz; n = g.ny; }
+    else { na = g.nx; nb = g.ny; n = g.nz; }
+    if (a >= na || b >= nb)
+        return;
+    float *F[6] = { g.ex, g.ey, g.ez, g.hx, g.hy, g.hz };
+    float *st = L.store[axis];
+    const long long pl = (long long)na * nb, s = (long long)a * nb + b;
+    const long long c0 = face_cell(g, axis, 0, a, b), c1 = face_cell(g, axis, 1, a, b);
+    const long long cn = face_cell(g, axis, n - 1, a, b), cn1 = face_cell(g, axis, n - 2, a, b);
+    for (int q = 0; q < 6; q++) {
+        if (L.on[axis * 2]) {
+            st[(0 + q) * pl + s] = F[q][c0];
+            st[(12 + q) * pl + s] = F[q][c1];
+        }
+        if (L.on[axis * 2 + 1]) {
+            st[(6 + q) * pl + s] = F[q][cn];
+            st[(18 + q) * pl + s] = F[q][cn1];
+        }
+    }
+}
+
+__global__ void
+k_liao_face(GsvGrid g, GsvLiao L, int axis, int far)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    int na, nb, n;
+    if (axis == 0) { na = g.ny; nb = g.nz; n = g.nx; }
+    else if (axis == 1) { na = g.nx; nb = g.nz; n = g.ny; }
+    else { na = g.nx; nb = g.ny; n = g.nz; }
+    if (a >= na || b >= nb)
+        return;
+    float *F[6] = { g.ex, g.ey, g.ez, g.hx, g.hy, g.hz };
+    const float *st = L.store[axis];
+    const long long pl = (long long)na * nb, s = (long long)a * nb + b;
+    const long long c = face_cell(g, axis, far ? n - 1 : 0, a, b);
+    const long long ci = face_cell(g, axis, far ? n - 2 : 1, a, b);
+    const float *cf = L.coef[axis * 2 + far] + s * 3;
+    for (int q = 0; q < 6; q++) {
+        const int p0 = far ? 18 + q : 12 + q, p00 = far ? 6 + q : q;
+        F[q][c] = st[p0 * pl + s] + cf[q % 3] * (F[q][ci] - st[p00 * pl + s]);
+    }
+}
+
+/* near-plane zeroing of CPML faces, re-applied after Liao faces on mixed boundaries */
+__global__ void
+k_zero_planes(GsvGrid g, int axis)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    int na, nb;
+    if (axis == 0) { na = g.ny; nb = g.nz; }
+    else if (axis == 1) { na = g.nx; nb = g.nz; }
+    else { na = g.nx; nb = g.ny; }
+    if (a >= na || b >= nb)
+        return;
+    const long long c = face_cell(g, axis, 0, a, b);
+    g.ex[c] = 0.f;
+    g.ey[c] = 0.f;
+    g.ez[c] = 0.f;
+}
+
+__global__ void
+k_periodic_h(GsvGrid g, GsvPeriodic p)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    const long long P = g.plane;
+    const int px = g.px;
+    if (p.x) {
+        const int j = p.y0 + a, k = p.z0 + b;
+        if (j < p.yn && k < p.zn) {
+            const long long d = (p.x0 - 1) + (long long)px * j + P * k, s = (p.xn - 1) + (long long)px * j + P * k;
+            g.hy[d] = g.hy[s];
+            g.hz[d] = g.hz[s];
+        }
+    }
+    if (p.y) {
+        const int i = p.x0 + a, k = p.z0 + b;
+        if (i < p.xn && k < p.zn) {
+            const long long d = i + (long long)px * (p.y0 - 1) + P * k, s = i + (long long)px * (p.yn - 1) + P * k;
+            g.hx[d] = g.hx[s];
+            g.hz[d] = g.hz[s];
+        }
+    }
+    if (p.z) {
+        const int i = p.x0 + a, j = p.y0 + b;
+        if (i < p.xn && j < p.yn) {
+            const long long d = i + (long long)px * j + P * (p.z0 - 1), s = i + (long long)px * j + P * (p.zn - 1);
+            g.hx[d] = g.hx[s];
+            g.hy[d] = g.hy[s];
+        }
+    }
+}
+
+__global__ void
+k_periodic_e(GsvGrid g, GsvPeriodic p)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    const long long P = g.plane;
+    const int px = g.px;
+    if (p.x) {
+        const int j = p.y0 + a, k = p.z0 + b;
+        if (j < p.yn && k < p.zn) {
+            const long long d = p.xn + (long long)px * j + P * k, s = p.x0 + (long long)px * j + P * k;
+            g.ey[d] = g.ey[s];
+            g.ez[d] = g.ez[s];
+        }
+    }
+    if (p.y) {
+        const int i = p.x0 + a, k = p.z0 + b;
+        if (i < p.xn && k < p.zn) {
+            const long long d = i + (long long)px * p.yn + P * k, s = i + (long long)px * p.y0 + P * k;
+            g.ex[d] = g.ex[s];
+            g.ez[d] = g.ez[s];
+        }
+    }
+    if (p.z) {
+        const int i = p.x0 + a, j = p.y0 + b;
+        if (i < p.xn && j < p.yn) {
+            const long long d = i + (long long)px * j + P * p.zn, s = i + (long long)px * j + P * p.z0;
+            g.ex[d] = g.ex[s];
+            g.ey[d] = g.ey[s];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* far field                                                            */
+/* ------------------------------------------------------------------ */
+
+/* time-domain Kirchhoff accumulation: one thread per (point, surface cell) */
+__global__ __launch_bounds__(256) void
+k_farfield_td(GsvGrid g, GsvFarfieldTD f, int step)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = blockIdx.y;
+    if (q >= f.ncells)
+        return;
+    int face, a, b;
+    long long r = q;
+    const long long sxy = (long long)f.ix * f.iy, sxz = (long long)f.ix * f.iz, syz = (long long)f.iy * f.iz;
+    if (r < 2 * syz) { face = (r < syz) ? 2 : 3; r %= syz; a = (int)(r / f.iz); b = (int)(r % f.iz); }
+    else if ((r -= 2 * syz) < 2 * sxz) { face = (r < sxz) ? 0 : 1; r %= sxz; a = (int)(r / f.iz); b = (int)(r % f.iz); }
+    else { r -= 2 * sxz; face = (r < sxy) ? 4 : 5; r %= sxy; a = (int)(r / f.iy); b = (int)(r % f.iy); }
+    int i, j, k, ni = 0, nj = 0, nk = 0;
+    double size;
+    /* faces: 2/3 x0/xn (a=j, b=k), 0/1 y0/yn (a=i, b=k), 4/5 z0/zn (a=i, b=j) */
+    if (face == 2 || face == 3) {
+        j = f.j0 + a; k = f.k0 + b; i = (face == 2) ? f.i0 : f.i1; ni = 1; size = f.dy * f.dz;
+        const int *sk = f.skip[face == 2 ? 0 : 1];
+        if (sk[0] && j >= sk[1] && k >= sk[2] && j <= sk[3] && k <= sk[4]) return;
+    } else if (face == 0 || face == 1) {
+        i = f.i0 + a; k = f.k0 + b; j = (face == 0) ? f.j0 : f.j1; nj = 1; size = f.dx * f.dz;
+        const int *sk = f.skip[face == 0 ? 2 : 3];
+        if (sk[0] && i >= sk[1] && k >= sk[2] && i <= sk[3] && k <= sk[4]) return;
+    } else {
+        i = f.i0 + a; j = f.j0 + b; k = (face == 4) ? f.k0 : f.k1; nk = 1; size = f.dx * f.dy;
+        const int *sk = f.skip[face == 4 ? 4 : 5];
+        if (sk[0] && i >= sk[1] && j >= sk[2] && i <= sk[3] && j <= sk[4]) return;
+    }
+    const int x = f.ri[n], y = f.rj[n], z = f.rk[n];
+    const double ddx = x - i, ddy = y - j, ddz = z - k;
+    const double rc = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+    double nvx = 0, nvy = 0, nvz = 0;
+    if (face == 0) nvy = -1; else if (face == 1) nvy = 1; else if (face == 2) nvx = -1;
+    else if (face == 3) nvx = 1; else if (face == 4) nvz = -1; else nvz = 1;
+    const double ct = (ddx * nvx + ddy * nvy + ddz * nvz) / rc;
+    const double rr = rc * f.dx;
+    const int nn = (int)(step + 1 + rr / (f.c * f.dt)) - f.istart[n];
+    if (nn < 3 || nn >= f.ndata)
+        return;
+    const double A = 1.0 / (4.0 * M_PI * rr);
+    const double B = -1.0 / (4.0 * M_PI) * ct / (rr * rr);
+    const double C = -1.0 / (4.0 * M_PI) * ct / (f.c * rr);
+    const double sgn = (face == 0 || face == 2 || face == 4) ? -1.0 : 1.0;
+    const long long c = i + (long long)g.px * j + g.plane * k;
+    const long long off = ni + (long long)g.px * nj + g.plane * nk;
+    const float *FF[3] = { g.ex, g.ey, g.ez };
+    double *bins = f.bins + (long long)n * 3 * f.ndata;
+    for (int qq = 0; qq < 3; qq++) {
+        const double v = FF[qq][c], vp = FF[qq][c + off], vm = FF[qq][c - off];
+        double *bq = bins + (long long)qq * f.ndata;
+        atomicAdd(bq + nn, -C / 2.0 / f.dt * v * size);
+        atomicAdd(bq + nn - 1, (sgn * A * (vp - vm) / 2.0 / f.dx + B * v) * size);
+        atomicAdd(bq + nn - 2, C / 2.0 / f.dt * v * size);
+    }
+}
+
+/* running DFT of the four tangential field components on every NFFF box face
+ * cell: acc += F(t) exp(-i w t), E at (step+1) dt, H at (step+1/2) dt */
+__global__ __launch_bounds__(256) void
+k_dft_faces(GsvGrid g, GsvDft d, int step)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.ncells)
+        return;
+    const long long c = d.cell[q];
+    const int face = d.face[q];
+    float v[4];
+    if (face < 2) { v[0] = g.ey[c]; v[1] = g.ez[c]; v[2] = g.hy[c]; v[3] = g.hz[c]; }
+    else if (face < 4) { v[0] = g.ex[c]; v[1] = g.ez[c]; v[2] = g.hx[c]; v[3] = g.hz[c]; }
+    else { v[0] = g.ex[c]; v[1] = g.ey[c]; v[2] = g.hx[c]; v[3] = g.hy[c]; }
+    double *acc = d.acc + q * d.nw * 8;
+    const double te = (step + 1) * d.dt, th = (step + 0.5) * d.dt;
+    for (int w = 0; w < d.nw; w++) {
+        double se, ce, sh, ch;
+        sincos(-d.omega[w] * te, &se, &ce);
+        sincos(-d.omega[w] * th, &sh, &ch);
+        double *a = acc + w * 8;
+        a[0] += v[0] * ce; a[1] += v[0] * se;
+        a[2] += v[1] * ce; a[3] += v[1] * se;
+        a[4] += v[2] * ch; a[5] += v[2] * sh;
+        a[6] += v[3] * ch; a[7] += v[3] * sh;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* launchers                                                            */
+/* ------------------------------------------------------------------ */
+
+extern "C" hipError_t
+gsv_launch_hstep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st)
+{
+    dim3 blk(bx, by, 1), grd((g->nx + bx - 1) / bx, (g->ny + by - 1) / by, (g->nz + kchunk - 1) / kchunk);
+    if (idbytes == 1)
+        hipLaunchKernelGGL(k_hstep<uint8_t>, grd, blk, 0, st, *g, kchunk);
+    else
+        hipLaunchKernelGGL(k_hstep<uint16_t>, grd, blk, 0, st, *g, kchunk);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st)
+{
+    dim3 blk(bx, by, 1), grd((g->nx + bx - 1) / bx, (g->ny + by - 1) / by, (g->nz + kchunk - 1) / kchunk);
+    if (idbytes == 1)
+        hipLaunchKernelGGL(k_estep<uint8_t>, grd, blk, 0, st, *g, kchunk);
+    else
+        hipLaunchKernelGGL(k_estep<uint16_t>, grd, blk, 0, st, *g, kchunk);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_tsf_line(const GsvTsf *t, int step, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tsf_line, dim3(1), dim3(1024), 0, st, *t, step);
+    return hipGetLastError();
+}
+
+static void
+tsf_face_dims(const GsvTsf *t, int axis, int *na, int *nb)
+{
+    if (axis == 0) { *na = t->jn - t->j0 + 1; *nb = t->kn - t->k0 + 1; }
+    else if (axis == 1) { *na = t->in - t->i0 + 1; *nb = t->kn - t->k0 + 1; }
+    else { *na = t->in - t->i0 + 1; *nb = t->jn - t->j0 + 1; }
+}
+
+extern "C" hipError_t
+gsv_launch_tsf_faces(const GsvGrid *g, const GsvTsf *t, int hside, hipStream_t st)
+{
+    for (int axis = 0; axis < 3; axis++) {
+        int na, nb;
+        tsf_face_dims(t, axis, &na, &nb);
+        dim3 blk(64, 4, 1), grd((na + 63) / 64, (nb + 3) / 4, 2);
+        if (hside)
+            hipLaunchKernelGGL(k_tsf_h, grd, blk, 0, st, *g, *t, axis);
+        else
+            hipLaunchKernelGGL(k_tsf_e, grd, blk, 0, st, *g, *t, axis);
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_point_src(const GsvGrid *g, const GsvPointSrc *s, int step, int hside, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_point_src, dim3(1), dim3(64), 0, st, *g, *s, step, hside);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_record(const GsvGrid *g, const long long *pos, int n, float *rec, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_record_points, dim3((n + 63) / 64), dim3(64), 0, st, *g, pos, n, rec);
+    return hipGetLastError();
+}
+
+static void
+axis_dims(const GsvGrid *g, int axis, int *na, int *nb)
+{
+    if (axis == 0) { *na = g->ny; *nb = g->nz; }
+    else if (axis == 1) { *na = g->nx; *nb = g->nz; }
+    else { *na = g->nx; *nb = g->ny; }
+}
+
+extern "C" hipError_t
+gsv_launch_liao_store(const GsvGrid *g, const GsvLiao *L, hipStream_t st)
+{
+    for (int axis = 0; axis < 3; axis++) {
+        if (!L->on[2 * axis] && !L->on[2 * axis + 1])
+            continue;
+        int na, nb;
+        axis_dims(g, axis, &na, &nb);
+        hipLaunchKernelGGL(k_liao_store, dim3((na + 63) / 64, (nb + 3) / 4), dim3(64, 4), 0, st, *g, *L, axis);
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_liao_faces(const GsvGrid *g, const GsvLiao *L, hipStream_t st)
+{
+    for (int axis = 0; axis < 3; axis++)
+        for (int far = 0; far < 2; far++) {
+            if (!L->on[2 * axis + far])
+                continue;
+            int na, nb;
+            axis_dims(g, axis, &na, &nb);
+            hipLaunchKernelGGL(k_liao_face, dim3((na + 63) / 64, (nb + 3) / 4), dim3(64, 4), 0, st, *g, *L, axis, far);
+        }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_zero_planes(const GsvGrid *g, int axis, hipStream_t st)
+{
+    int na, nb;
+    axis_dims(g, axis, &na, &nb);
+    hipLaunchKernelGGL(k_zero_planes, dim3((na + 63) / 64, (nb + 3) / 4), dim3(64, 4), 0, st, *g, axis);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_periodic(const GsvGrid *g, const GsvPeriodic *p, int hside, hipStream_t st)
+{
+    int m = g->nx > g->ny ? g->nx : g->ny;
+    if (g->nz > m) m = g->nz;
+    dim3 blk(64, 4), grd((m + 63) / 64, (m + 3) / 4);
+    if (hside)
+        hipLaunchKernelGGL(k_periodic_h, grd, blk, 0, st, *g, *p);
+    else
+        hipLaunchKernelGGL(k_periodic_e, grd, blk, 0, st, *g, *p);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_farfield_td(const GsvGrid *g, const GsvFarfieldTD *f, int step, hipStream_t st)
+{
+    dim3 grd((unsigned)((f->ncells + 255) / 256), f->nrs);
+    hipLaunchKernelGGL(k_farfield_td, grd, dim3(256), 0, st, *g, *f, step);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_dft(const GsvGrid *g, const GsvDft *d, int step, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dft_faces, dim3((unsigned)((d->ncells + 255) / 256)), dim3(256), 0, st, *g, *d, step);
+    return hipGetLastError();
+}

@@ -1,0 +1,86 @@
+/*
+ * output.c -- text logs of the step loop: point traces and time-domain far
+ * field, in the reference formats (src3d/output.c:51-99 is_output,
+ * :645-681 point logs "%d %g" / "%d %g %g %g %g %g %g", :1266-1271 far-field
+ * log "%d %g %g %g" of the negated bins).  The running-DFT face data, which
+ * has no reference counterpart, is written as one line per face cell.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include "gsvit_internal.h"
+
+int
+is_output(SvPool *mp, SvSet *set)
+{
+    int i;
+    (void)mp;
+    for (i = 0; i < set->so.npnts; i++) {
+        if (set->so.pnts[i].step == 0)
+            return 0;
+        if (set->sc.step_act % set->so.pnts[i].step == 0)
+            return 1;
+    }
+    return 0;
+}
+
+void
+sv_pool_output(SvPool *mp, SvSet *set)
+{
+    int i, j, n;
+    FILE *fh;
+
+    for (i = 0; i < set->so.npnts; i++) {
+        SvOutputPar *o = set->so.pnts + i;
+        if (o->step == 0)
+            continue;
+        if (set->sc.step_act % o->step != 0 && set->sc.step_act < (set->sc.nsteps - 1))
+            continue;
+        fh = fopen(o->filebase, "w");
+        if (!fh) {
+            fprintf(stderr, "Error: point output: cannot open file %s for writing\n", o->filebase);
+            continue;
+        }
+        for (j = 0; j < set->sc.step_act; j++) {
+            const double *v = mp->out->outpointdata[i] + (size_t)j * 6;
+            if (o->component == SV_COMP_ALL)
+                fprintf(fh, "%d %g %g %g %g %g %g\n", j, v[0], v[1], v[2], v[3], v[4], v[5]);
+            else
+                fprintf(fh, "%d %g\n", j, v[o->component]);
+        }
+        fclose(fh);
+    }
+
+    /* far-field logs are written at the last step only */
+    if (set->sc.step_act != set->sc.nsteps - 1)
+        return;
+    for (n = 0; n < set->sf.nrs; n++) {
+        SvRPoint *r = mp->farfield->rpoints + n;
+        fh = fopen(set->sf.source_filename[n], "w");
+        if (!fh) {
+            fprintf(stderr, "Error: farfield: cannot open file %s for writing\n", set->sf.source_filename[n]);
+            continue;
+        }
+        for (i = 0; i < mp->farfield->ndata; i++)
+            fprintf(fh, "%d %g %g %g\n", i, -r->ex[i], -r->ey[i], -r->ez[i]);
+        fclose(fh);
+    }
+    if (set->sf.ndft && set->sf.dft_filename[0]) {
+        SvFarfield *ff = mp->farfield;
+        long c;
+        int q;
+        fh = fopen(set->sf.dft_filename, "w");
+        if (!fh) {
+            fprintf(stderr, "Error: farfield DFT: cannot open file %s for writing\n", set->sf.dft_filename);
+            return;
+        }
+        fprintf(fh, "# running DFT of tangential fields on the NFFF box: cell, then per wavelength "
+                    "(re, im) of the 4 tangential components\n");
+        for (c = 0; c < ff->dft_ncells; c++) {
+            fprintf(fh, "%ld", c);
+            for (q = 0; q < ff->ndft * 8; q++)
+                fprintf(fh, " %g", ff->dft_data[(size_t)c * ff->ndft * 8 + q]);
+            fprintf(fh, "\n");
+        }
+        fclose(fh);
+    }
+}

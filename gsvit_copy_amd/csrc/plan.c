@@ -1,0 +1,1219 @@
+/*
+ * plan.c -- planner for the 3-D time-stepping path: time step, sources,
+ * material rasterisation, internal-boundary map, CPML grading and far-field
+ * geometry.  Restates the host-side preparation that GSvit performs before its
+ * step loop (src3d/plan.c:5442-5604, init_and_make_plan) for the entities and
+ * materials that the hot path consumes.
+ *
+ * Deliberately reproduced reference behaviour (each cited where it happens):
+ *   - materials are scanned as float and widened (src3d/plan.c:3112-3300);
+ *   - material de-duplication with almost_eq, vacuum at index 0, including the
+ *     cp3_phi[2] / cp3_a[2] comparison (src3d/plan.c:2318-2367, 2409-2428);
+ *   - sphere voxel test at cell centres in float (src3d/plan.c:2136-2143, 4378-4391);
+ *   - matmode selection including the electric+magnetic -> NONE case of
+ *     test_script_file (src3d/plan.c:573-711, 5489-5518);
+ *   - internal-boundary map on faces and 26-neighbour differences
+ *     (src3d/plan.c:5381-5417).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include "gsvit_internal.h"
+
+#define EPS0 SV_EPSILON_0
+#define MU0 SV_MU_0
+
+int
+sv_material_is_tabulated(int type)
+{
+    return type == SV_MAT_LINTAB || type == SV_MAT_DRUDE || type == SV_MAT_PLRC || type == SV_MAT_ADE
+           || type == SV_MAT_CP || type == SV_MAT_CP3;
+}
+
+int
+sv_material_is_dispersive(int type)
+{
+    return type == SV_MAT_DRUDE || type == SV_MAT_CP3 || type == SV_MAT_CP || type == SV_MAT_ADE
+           || type == SV_MAT_PLRC;
+}
+
+/* ------------------------------------------------------------------ */
+/* material accessors (src3d/yee.c:1265-1682)                          */
+/* ------------------------------------------------------------------ */
+
+int
+sv_pool_epsilon_sigma(const SvPool *mp, int i, int j, int k, double *eps, double *sig)
+{
+    const SvYeeData *d = mp->d;
+    int matindex = (mp->nmat > 0) ? d->mat->data[i][j][k] : -1;
+    if (matindex > 0 && sv_material_is_tabulated(mp->mats[matindex].type)) {
+        *eps = mp->mats[matindex].epsilon;
+        *sig = mp->mats[matindex].sigma;
+    } else if (d->epsilon && d->sigma) {
+        *eps = d->epsilon->data[i][j][k];
+        *sig = d->sigma->data[i][j][k];
+    } else {
+        *eps = EPS0;
+        *sig = 0;
+    }
+    return matindex;
+}
+
+int
+sv_pool_mu_sigast(const SvPool *mp, int i, int j, int k, double *mu, double *sigast)
+{
+    const SvYeeData *d = mp->d;
+    const SvSet *set = mp->set;
+    int matindex = (mp->nmat > 0) ? d->mat->data[i][j][k] : -1;
+    if (matindex > 0 && sv_material_is_tabulated(mp->mats[matindex].type)) {
+        *mu = mp->mats[matindex].mu;
+        *sigast = mp->mats[matindex].sigast;
+    } else if (set->plan.matmode == SV_MATMODE_FULL || set->plan.matmode == SV_MATMODE_MAGNETIC) {
+        *mu = d->mu->data[i][j][k];
+        *sigast = d->sigast->data[i][j][k];
+    } else {
+        *mu = MU0;
+        *sigast = 0;
+    }
+    return matindex;
+}
+
+void
+sv_pool_get_cabs(const SvPool *mp, int i, int j, int k,
+                 double *cax, double *cay, double *caz, double *cbx, double *cby, double *cbz)
+{
+    const SvYeeData *d = mp->d;
+    double dt = d->dt;
+    double ex, ey, ez, e, sx, sy, sz, s;
+    int nx, ny, nz;
+
+    sv_pool_epsilon_sigma(mp, i, j, k, &e, &s);
+    ex = ey = ez = e;
+    sx = sy = sz = s;
+
+    if (d->bnds->data[i][j][k] > 0) {
+        nx = ny = nz = 1;
+        if (i > 0) {
+            sv_pool_epsilon_sigma(mp, i - 1, j, k, &e, &s);
+            ey += e; sy += s; ez += e; sz += s; ny++; nz++;
+        }
+        if (j > 0) {
+            sv_pool_epsilon_sigma(mp, i, j - 1, k, &e, &s);
+            ex += e; sx += s; ez += e; sz += s; nx++; nz++;
+        }
+        if (k > 0) {
+            sv_pool_epsilon_sigma(mp, i, j, k - 1, &e, &s);
+            ex += e; sx += s; ey += e; sy += s; nx++; ny++;
+        }
+        if (j > 0 && k > 0) {
+            sv_pool_epsilon_sigma(mp, i, j - 1, k - 1, &e, &s);
+            ex += e; sx += s; nx++;
+        }
+        if (i > 0 && k > 0) {
+            sv_pool_epsilon_sigma(mp, i - 1, j, k - 1, &e, &s);
+            ey += e; sy += s; ny++;
+        }
+        if (i > 0 && j > 0) {
+            sv_pool_epsilon_sigma(mp, i - 1, j - 1, k, &e, &s);
+            ez += e; sz += s; nz++;
+        }
+        ex /= nx; sx /= nx;
+        ey /= ny; sy /= ny;
+        ez /= nz; sz /= nz;
+    }
+
+    *cax = (1 - sx * dt / 2 / ex) / (1 + sx * dt / 2 / ex);
+    *cbx = (dt / ex / d->dx) / (1 + sx * dt / 2 / ex);
+    if (d->bnds->data[i][j][k] > 0 || (d->dx != d->dy || d->dx != d->dz)) {
+        *cay = (1 - sy * dt / 2 / ey) / (1 + sy * dt / 2 / ey);
+        *cby = (dt / ey / d->dy) / (1 + sy * dt / 2 / ey);
+        *caz = (1 - sz * dt / 2 / ez) / (1 + sz * dt / 2 / ez);
+        *cbz = (dt / ez / d->dz) / (1 + sz * dt / 2 / ez);
+    } else {
+        *cay = *caz = *cax;
+        *cby = *cbz = *cbx;
+    }
+}
+
+static int
+disp_at(const SvPool *mp, int i, int j, int k)
+{
+    int m = mp->d->mat->data[i][j][k];
+    return m > 0 && sv_material_is_dispersive(mp->mats[m].type);
+}
+
+int
+sv_pool_get_tbc(const SvPool *mp, int i, int j, int k, int *xtbc, int *ytbc, int *ztbc)
+{
+    const SvYeeData *d = mp->d;
+    int matindex;
+
+    *xtbc = *ytbc = *ztbc = 1;
+    if (mp->nmat <= 0)
+        return -1;
+    matindex = d->mat->data[i][j][k];
+    if (matindex > 0 && sv_material_is_dispersive(mp->mats[matindex].type)) {
+        *xtbc = *ytbc = *ztbc = 0;
+        return matindex;
+    }
+    /* dispersive neighbour on the minus side: the last match wins, src3d/yee.c:474-522 */
+    if (i > 0 && disp_at(mp, i - 1, j, k)) { matindex = d->mat->data[i - 1][j][k]; *ytbc = *ztbc = 0; }
+    if (j > 0 && disp_at(mp, i, j - 1, k)) { matindex = d->mat->data[i][j - 1][k]; *xtbc = *ztbc = 0; }
+    if (k > 0 && disp_at(mp, i, j, k - 1)) { matindex = d->mat->data[i][j][k - 1]; *xtbc = *ytbc = 0; }
+    if (i > 0 && j > 0 && disp_at(mp, i - 1, j - 1, k)) { matindex = d->mat->data[i - 1][j - 1][k]; *ztbc = 0; }
+    if (j > 0 && k > 0 && disp_at(mp, i, j - 1, k - 1)) { matindex = d->mat->data[i][j - 1][k - 1]; *xtbc = 0; }
+    if (i > 0 && k > 0 && disp_at(mp, i - 1, j, k - 1)) { matindex = d->mat->data[i - 1][j][k - 1]; *ytbc = 0; }
+    return matindex;
+}
+
+int
+sv_pool_is_pec(const SvPool *mp, int i, int j, int k)
+{
+    int m;
+    if (mp->nmat <= 0)
+        return 0;
+    m = mp->d->mat->data[i][j][k];
+    return m > 0 && mp->mats[m].type == SV_MAT_PEC;
+}
+
+/* ------------------------------------------------------------------ */
+/* TSF geometry (src3d/source.c:53-165)                                */
+/* ------------------------------------------------------------------ */
+
+double
+sv_dcomp(int i, int j, int k, double theta, double phi, int i0, int i1, int j0, int j1, int k0, int k1)
+{
+    double rx = 0, ry = 0, rz = 0;
+    double ax = sin(theta) * cos(phi);
+    double ay = sin(theta) * sin(phi);
+    double az = cos(theta);
+
+    if (theta >= 0 && theta <= (M_PI / 2.0)) {
+        if (phi >= 0 && phi <= (M_PI / 2.0)) { rx = i - i0; ry = j - j0; rz = k - k0; }
+        else if (phi > (M_PI / 2.0) && phi <= M_PI) { rx = i - i1; ry = j - j0; rz = k - k0; }
+        else if (phi > M_PI && phi <= (3.0 * M_PI / 2.0)) { rx = i - i1; ry = j - j1; rz = k - k0; }
+        else { rx = i - i0; ry = j - j1; rz = k - k0; }
+    } else if (theta < M_PI && theta > (M_PI / 2.0)) {
+        /* the reference leaves rx = 0 in the second quadrant (src3d/source.c:89-91) */
+        if (phi >= 0 && phi <= (M_PI / 2.0)) { rx = i - i0; ry = j - j0; rz = k - k1; }
+        else if (phi > (M_PI / 2.0) && phi <= M_PI) { ry = j - j0; rz = k - k1; }
+        else if (phi > M_PI && phi <= (3.0 * M_PI / 2.0)) { rx = i - i1; ry = j - j1; rz = k - k1; }
+        else { rx = i - i0; ry = j - j1; rz = k - k1; }
+    } else {
+        return -1;
+    }
+    return 10 + (ax * rx + ay * ry + az * rz);
+}
+
+double sv_gex(double f, double t, double p, double s) { return f * (cos(s) * sin(p) - sin(s) * cos(t) * cos(p)); }
+double sv_gey(double f, double t, double p, double s) { return f * (-cos(s) * cos(p) - sin(s) * cos(t) * sin(p)); }
+double sv_gez(double f, double t, double p, double s) { return f * (sin(s) * sin(t)); }
+double sv_ghx(double f, double t, double p, double s) { return f * (sin(s) * sin(p) + cos(s) * cos(t) * cos(p)); }
+double sv_ghy(double f, double t, double p, double s) { return f * (-sin(s) * cos(p) + cos(s) * cos(t) * sin(p)); }
+double sv_ghz(double f, double t, double p, double s) { return f * (-cos(s) * sin(t)); }
+
+int
+sv_tsf_face_active(const SvPool *mp, const SvSet *set, int i, int j, int k)
+{
+    /* src3d/source.c:253-264 */
+    if (set->ss.tsf.box_boundary_skipdepth == -1)
+        return 1;
+    if (mp->d->sourceskip && mp->d->sourceskip->data[i][j][k])
+        return 0;
+    return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* far-field geometry (src3d/farfield.c:32-54)                         */
+/* ------------------------------------------------------------------ */
+
+double
+sv_ff_dist(int x, int y, int z, int i, int j, int k)
+{
+    return sqrt((double)(x - i) * (x - i) + (double)(y - j) * (y - j) + (double)(z - k) * (z - k));
+}
+
+static double
+angvec(double x1, double y1, double z1, double x2, double y2, double z2)
+{
+    return (x1 * x2 + y1 * y2 + z1 * z2) / (sqrt((x1 * x1 + y1 * y1 + z1 * z1) * (x2 * x2 + y2 * y2 + z2 * z2)));
+}
+
+double
+sv_ff_angle(int x1, int y1, int z1, int x2, int y2, int z2, int side)
+{
+    double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+    switch (side) {
+        case 0: return angvec(dx, dy, dz, 0, -1, 0);
+        case 1: return angvec(dx, dy, dz, 0, 1, 0);
+        case 2: return angvec(dx, dy, dz, -1, 0, 0);
+        case 3: return angvec(dx, dy, dz, 1, 0, 0);
+        case 4: return angvec(dx, dy, dz, 0, 0, -1);
+        default: return angvec(dx, dy, dz, 0, 0, 1);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* sources (src3d/plan.c:1430-1597)                                    */
+/* ------------------------------------------------------------------ */
+
+static int
+read_point_source_file(const char *fname, SvSourcePoint *sp, int nsteps)
+{
+    FILE *fr = fopen(fname, "r");
+    int n, q;
+    if (!fr) {
+        fprintf(stderr, "Error: cannot open file %s for reading\n", fname);
+        return 1;
+    }
+    if (fscanf(fr, "%d", &n) != 1 || n <= 0) {
+        fprintf(stderr, "Error: cannot read number of values from file %s\n", fname);
+        fclose(fr);
+        return 1;
+    }
+    if (n < nsteps)
+        fprintf(stderr, "Warning: less data (%d) in \"%s\" than scheduled computation steps (%d)\n", n, fname, nsteps);
+    sp->ndata = n;
+    sp->layered_zpos = (int *)malloc(n * sizeof(int));
+    sp->ex = (double *)malloc(6 * (size_t)n * sizeof(double));
+    sp->ey = sp->ex + n;
+    sp->ez = sp->ey + n;
+    sp->hx = sp->ez + n;
+    sp->hy = sp->hx + n;
+    sp->hz = sp->hy + n;
+    for (q = 0; q < n; q++) {
+        if (fscanf(fr, "%d %lf %lf %lf %lf %lf %lf", &sp->layered_zpos[q], &sp->ex[q], &sp->ey[q], &sp->ez[q],
+                   &sp->hx[q], &sp->hy[q], &sp->hz[q]) != 7) {
+            fprintf(stderr, "Error: cannot read point source sample %d from %s\n", q, fname);
+            fclose(fr);
+            return 1;
+        }
+    }
+    fclose(fr);
+    return 0;
+}
+
+static int
+load_sources(SvPool *mp, SvSet *set)
+{
+    int i, q;
+    SvSource *src = mp->src;
+
+    src->npnts = set->ss.npnts;
+    if (set->ss.npnts)
+        src->sp = (SvSourcePoint *)calloc(set->ss.npnts, sizeof(SvSourcePoint));
+    for (i = 0; i < set->ss.npnts; i++) {
+        SvSetSourcePoint *p = set->ss.pnts + i;
+        SvSourcePoint *sp = src->sp + i;
+        sp->i = p->point_origin_position_i;
+        sp->j = p->point_origin_position_j;
+        sp->k = p->point_origin_position_k;
+        if (sp->i < 0 || sp->j < 0 || sp->k < 0 || sp->i >= set->sp.xres || sp->j >= set->sp.yres || sp->k >= set->sp.zres) {
+            fprintf(stderr, "Error: point source %d at (%d %d %d) is outside the pool\n", i, sp->i, sp->j, sp->k);
+            return 1;
+        }
+        if (p->source_mode == 0) {
+            if (read_point_source_file(p->source_filename, sp, set->sc.nsteps))
+                return 1;
+        } else {
+            /* generated source written as SRCF_ARB, src3d/settings.c:525-531 */
+            int n = set->sc.nsteps;
+            double *v = (double *)malloc(n * sizeof(double));
+            double ax = sin(p->point_origin_theta) * cos(p->point_origin_phi);
+            double ay = sin(p->point_origin_theta) * sin(p->point_origin_phi);
+            double az = cos(p->point_origin_theta);
+            if (sv_generate_waveform(p->source_mode, n, p->source_wl, p->source_pulsewidth, set->plan.dt,
+                                     p->source_amplitude, v)) {
+                free(v);
+                return 1;
+            }
+            sp->ndata = n;
+            sp->layered_zpos = (int *)malloc(n * sizeof(int));
+            sp->ex = (double *)calloc(6 * (size_t)n, sizeof(double));
+            sp->ey = sp->ex + n;
+            sp->ez = sp->ey + n;
+            sp->hx = sp->ez + n;
+            sp->hy = sp->hx + n;
+            sp->hz = sp->hy + n;
+            for (q = 0; q < n; q++) {
+                sp->layered_zpos[q] = q;
+                sp->ex[q] = sv_round_g6(ax * v[q]);
+                sp->ey[q] = sv_round_g6(ay * v[q]);
+                sp->ez[q] = sv_round_g6(az * v[q]);
+            }
+            free(v);
+        }
+    }
+
+    if (set->ss.tsf.valid) {
+        SvSetSourceTSF *t = &set->ss.tsf;
+        SvSourceTSF *ts = (SvSourceTSF *)calloc(1, sizeof(SvSourceTSF));
+        src->tsf = ts;
+        ts->box_i0 = t->box_i0; ts->box_j0 = t->box_j0; ts->box_k0 = t->box_k0;
+        ts->box_in = t->box_in; ts->box_jn = t->box_jn; ts->box_kn = t->box_kn;
+        ts->skip_i0 = t->box_boundary_skipi0; ts->skip_j0 = t->box_boundary_skipj0; ts->skip_k0 = t->box_boundary_skipk0;
+        ts->skip_in = t->box_boundary_skipin; ts->skip_jn = t->box_boundary_skipjn; ts->skip_kn = t->box_boundary_skipkn;
+        ts->ia_theta = t->ia_theta;
+        ts->ia_phi = t->ia_phi;
+        ts->ia_psi = t->ia_psi;
+        if (ts->box_i0 < 1 || ts->box_j0 < 1 || ts->box_k0 < 1 || ts->box_in >= set->sp.xres
+            || ts->box_jn >= set->sp.yres || ts->box_kn >= set->sp.zres) {
+            fprintf(stderr, "Error: TSF box must lie inside the pool with one cell to spare\n");
+            return 1;
+        }
+        if (t->source_mode == 0) {
+            FILE *fr = fopen(t->source_filename, "r");
+            int n;
+            if (!fr) {
+                fprintf(stderr, "Cannot load source file %s\n", t->source_filename);
+                return 1;
+            }
+            if (fscanf(fr, "%d", &n) != 1 || n <= 0) {
+                fprintf(stderr, "Error: cannot read number of values from file %s\n", t->source_filename);
+                fclose(fr);
+                return 1;
+            }
+            ts->ndata = n;
+            ts->e = (double *)calloc(n > set->sc.nsteps ? n : set->sc.nsteps, sizeof(double));
+            for (q = 0; q < n; q++) {
+                int pos;
+                if (fscanf(fr, "%d %lf", &pos, &ts->e[q]) != 2) {
+                    fprintf(stderr, "Error: cannot read TSF sample %d from %s\n", q, t->source_filename);
+                    fclose(fr);
+                    return 1;
+                }
+            }
+            fclose(fr);
+            if (n < set->sc.nsteps)
+                fprintf(stderr, "Warning: less data (%d) in \"%s\" than scheduled computation steps (%d) in TSF source; "
+                        "missing samples are zero\n", n, t->source_filename, set->sc.nsteps);
+        } else {
+            int n = set->sc.nsteps;
+            ts->ndata = n;
+            ts->e = (double *)calloc(n, sizeof(double));
+            if (sv_generate_waveform(t->source_mode, n, t->source_wl, t->source_pulsewidth, set->plan.dt,
+                                     t->source_amplitude, ts->e))
+                return 1;
+            for (q = 0; q < n; q++)
+                ts->e[q] = sv_round_g6(ts->e[q]);
+        }
+        /* src3d/plan.c:1584-1595 */
+        ts->corr = 1.0 / sqrt(pow(sin(ts->ia_theta), 4) * (pow(cos(ts->ia_phi), 4) + pow(sin(ts->ia_phi), 4))
+                              + pow(cos(ts->ia_theta), 4));
+        ts->jp_res = (int)(sqrt((double)(set->sp.xres * set->sp.xres + set->sp.yres * set->sp.yres
+                                         + set->sp.zres * set->sp.zres)) + 250);
+        ts->jp_dx = set->sp.dx / ts->corr;
+        ts->jp_epsilon = t->layered_epsilon;
+        ts->jp_mu = t->layered_mu;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* materials (src3d/plan.c:2243-2543, 3112-3300)                       */
+/* ------------------------------------------------------------------ */
+
+static void
+calculate_ade(SvMatProp *mat, double epsilon, double sigma, double omegap, double nu, double dt)
+{
+    int i;
+    /* src3d/plan.c:2736-2762 */
+    mat->ade_a0 = (nu * dt - 2) / (nu * dt + 2);
+    mat->ade_a1 = 4 / (nu * dt + 2);
+    mat->ade_a2 = dt * dt * EPS0 * omegap * omegap / (2 * (nu * dt + 2));
+    for (i = 0; i < 2; i++) {
+        double a = mat->cp3_a[i], phi = mat->cp3_phi[i], omega = mat->cp3_omega[i], gamma = mat->cp3_gamma[i];
+        double den = (omega * omega * dt * dt + (2 + gamma * dt) * (2 + gamma * dt));
+        mat->ade_bp0[i] = -(omega * omega * dt * dt + (2 - gamma * dt) * (2 - gamma * dt)) / den;
+        mat->ade_bp1[i] = 2 * (4 - (gamma * gamma + omega * omega) * dt * dt) / den;
+        mat->ade_bp2[i] = 2 * EPS0 * a * omega * dt * (omega * dt * cos(phi) + (2 - gamma * dt) * sin(phi)) / den;
+        mat->ade_bp3[i] = 4 * EPS0 * a * omega * dt * (omega * dt * cos(phi) - gamma * dt * sin(phi)) / den;
+        mat->ade_bp4[i] = 2 * EPS0 * a * omega * dt * (omega * dt * cos(phi) - (2 + gamma * dt) * sin(phi)) / den;
+    }
+    {
+        double den = sigma * dt / 2 + mat->ade_a2 + mat->ade_bp4[0] + mat->ade_bp4[1] + EPS0 * epsilon;
+        mat->ade_c0 = dt / den;
+        mat->ade_c1 = 1 / den;
+        mat->ade_c2 = -(mat->ade_a2 + mat->ade_bp2[0] + mat->ade_bp2[1]) / den;
+        mat->ade_c3 = -(sigma * dt / 2 + 2 * mat->ade_a2 + mat->ade_bp3[0] + mat->ade_bp3[1] - EPS0 * epsilon) / den;
+    }
+}
+
+static int
+scan_floats(FILE *fr, float *v, int n)
+{
+    int q;
+    for (q = 0; q < n; q++)
+        if (fscanf(fr, "%f", v + q) != 1)
+            return 1;
+    return 0;
+}
+
+/* src3d/plan.c:3112-3300; returns 1 on a read error */
+static int
+scan_material(FILE *fr, SvMatProp *mat, double dt)
+{
+    float v[12];
+    int type, q;
+
+    memset(mat, 0, sizeof(*mat));
+    mat->epsilon = mat->mu = 1;
+    mat->pos = -1;
+    if (fscanf(fr, "%d", &type) != 1)
+        return 1;
+    mat->type = type;
+    if (type == SV_MAT_LINEAR || type == SV_MAT_LINTAB) {
+        if (scan_floats(fr, v, 4))
+            return 1;
+        mat->epsilon = v[0];
+        mat->mu = v[1];
+        mat->sigma = v[2];
+        mat->sigast = v[3];
+    } else if (type == SV_MAT_PEC) {
+    } else if (type == SV_MAT_ADE) {
+        if (scan_floats(fr, v, 12))
+            return 1;
+        mat->epsilon = v[0];
+        mat->sigma = v[1];
+        mat->drude_omega_p = v[2];
+        mat->drude_nu = v[3];
+        for (q = 0; q < 2; q++) {
+            mat->cp3_a[q] = v[4 + 4 * q];
+            mat->cp3_phi[q] = v[5 + 4 * q];
+            mat->cp3_omega[q] = v[6 + 4 * q];
+            mat->cp3_gamma[q] = v[7 + 4 * q];
+        }
+        calculate_ade(mat, v[0], v[1], v[2], v[3], dt);
+    } else {
+        fprintf(stderr, "Error: material type %d is not supported by the MI355X engine "
+                "(supported: 0 linear, 1 tabulated linear, 5 ADE Drude + 2 critical points, 10 PEC)\n", type);
+        return 1;
+    }
+    return 0;
+}
+
+static int
+almost_eq(double a, double b)
+{
+    const double TOL = 1e-3;
+    if (fabs(a) >= 1.0 && fabs(b) >= 1.0)
+        return fabs(a / b - 1.0) < TOL;
+    return fabs(a - b) < TOL;
+}
+
+static int
+get_matindex(const SvPool *mp, const SvMatProp *mat)
+{
+    int i;
+    for (i = 0; i < mp->nmat; i++) {
+        const SvMatProp *t = mp->mats + i;
+        if (t->type != mat->type)
+            continue;
+        if (mat->type == SV_MAT_PEC
+            || (almost_eq(t->epsilon, mat->epsilon) && almost_eq(t->mu, mat->mu) && almost_eq(t->sigma, mat->sigma)
+                && almost_eq(t->sigast, mat->sigast) && almost_eq(t->drude_omega_p, mat->drude_omega_p)
+                && almost_eq(t->drude_nu, mat->drude_nu) && almost_eq(t->cp3_a[0], mat->cp3_a[0])
+                && almost_eq(t->cp3_a[1], mat->cp3_a[1]) && almost_eq(t->cp3_phi[2], mat->cp3_a[2])
+                && almost_eq(t->cp3_phi[0], mat->cp3_phi[0]) && almost_eq(t->cp3_phi[1], mat->cp3_phi[1])
+                && almost_eq(t->cp3_phi[2], mat->cp3_phi[2]) && almost_eq(t->cp3_omega[0], mat->cp3_omega[0])
+                && almost_eq(t->cp3_omega[1], mat->cp3_omega[1]) && almost_eq(t->cp3_omega[2], mat->cp3_omega[2])
+                && almost_eq(t->cp3_gamma[0], mat->cp3_gamma[0]) && almost_eq(t->cp3_gamma[1], mat->cp3_gamma[1])
+                && almost_eq(t->cp3_gamma[2], mat->cp3_gamma[2])))
+            return i;
+    }
+    return -1;
+}
+
+static void
+add_material(SvPool *mp, SvMatProp *mat)
+{
+    if (mp->nmat == 0) {
+        mp->mats = (SvMatProp *)calloc(1, sizeof(SvMatProp));
+        mp->mats[0].type = SV_MAT_LINTAB;
+        mp->mats[0].epsilon = 1;
+        mp->mats[0].mu = 1;
+        mp->mats[0].pos = 0;
+        mp->nmat = 1;
+    }
+    mp->mats = (SvMatProp *)realloc(mp->mats, (mp->nmat + 1) * sizeof(SvMatProp));
+    mp->mats[mp->nmat] = *mat;
+    mp->mats[mp->nmat].pos = mp->nmat;
+    mat->pos = mp->nmat;
+    if (mp->set->sc.verbose > 1)
+        printf("material %d of type %d added\n", mp->nmat, mat->type);
+    mp->nmat++;
+}
+
+/* matmode probes, src3d/plan.c:139-200 (voxel file) and :250-360, :573-711 (vector file) */
+static int
+test_material_file(const char *fname)
+{
+    FILE *fr = fopen(fname, "rb");
+    int res[3], is_e = 0, is_m = 0;
+    size_t n, q;
+    float v;
+    if (!fr)
+        return SV_MATMODE_NONE;
+    if (fread(res, sizeof(int), 3, fr) != 3) {
+        fclose(fr);
+        return SV_MATMODE_NONE;
+    }
+    n = (size_t)res[0] * res[1] * res[2];
+    for (q = 0; q < n; q++) {
+        if (fread(&v, sizeof(float), 1, fr) != 1) { fclose(fr); return SV_MATMODE_NONE; }
+        if (v != 1) { is_e = 1; break; }
+    }
+    if (!is_e) {
+        for (q = 0; q < n; q++) {
+            if (fread(&v, sizeof(float), 1, fr) != 1) { fclose(fr); return SV_MATMODE_NONE; }
+            if (v != 0) { is_e = 1; break; }
+        }
+    }
+    /* the magnetic blocks start after both electric blocks */
+    fseek(fr, 12 + 2 * n * sizeof(float), SEEK_SET);
+    for (q = 0; q < n; q++) {
+        if (fread(&v, sizeof(float), 1, fr) != 1) { fclose(fr); return is_e ? SV_MATMODE_ELECTRIC : SV_MATMODE_NONE; }
+        if (v != 1) { is_m = 1; break; }
+    }
+    if (!is_m) {
+        fseek(fr, 12 + 3 * n * sizeof(float), SEEK_SET);
+        for (q = 0; q < n; q++) {
+            if (fread(&v, sizeof(float), 1, fr) != 1) break;
+            if (v != 0) { is_m = 1; break; }
+        }
+    }
+    fclose(fr);
+    if (is_e && is_m) return SV_MATMODE_FULL;
+    if (is_e) return SV_MATMODE_ELECTRIC;
+    if (is_m) return SV_MATMODE_MAGNETIC;
+    return SV_MATMODE_NONE;
+}
+
+static int
+test_script_file(const char *fname)
+{
+    FILE *fr = fopen(fname, "r");
+    int type, mtype, is_e = 0, is_m = 0;
+    float p[12];
+    if (!fr) {
+        fprintf(stderr, "Error: cannot open file %s for reading\n", fname);
+        return SV_MATMODE_ERROR;
+    }
+    while (fscanf(fr, "%d", &type) == 1) {
+        int npts = 0;
+        if (type == SV_ENTITY_SPHERE) npts = 4;
+        else if (type == SV_ENTITY_VOXEL) npts = 6;
+        else {
+            fprintf(stderr, "Error: entity type %d is not supported by the MI355X engine (4 sphere, 8 voxel)\n", type);
+            fclose(fr);
+            return SV_MATMODE_ERROR;
+        }
+        if (scan_floats(fr, p, npts) || fscanf(fr, "%d", &mtype) != 1)
+            goto err;
+        if (mtype == SV_MAT_LINEAR || mtype == SV_MAT_LINTAB) {
+            if (scan_floats(fr, p, 4))
+                goto err;
+            if (mtype == SV_MAT_LINEAR) {
+                if (p[0] != 1.0 || p[2] > 0) is_e = 1;
+                if (p[1] != 1.0 || p[3] > 0) is_m = 1;
+            }
+        } else if (mtype == SV_MAT_ADE) {
+            if (scan_floats(fr, p, 12))
+                goto err;
+            is_e = 1;
+        } else if (mtype == SV_MAT_PEC) {
+        } else {
+            fprintf(stderr, "Error: material type %d is not supported by the MI355X engine\n", mtype);
+            goto err;
+        }
+    }
+    fclose(fr);
+    if ((is_e + is_m) == 0) return SV_MATMODE_NONE;
+    if (is_e == 0 && is_m != 0) return SV_MATMODE_MAGNETIC;
+    if (is_e != 0 && is_m == 0) return SV_MATMODE_ELECTRIC;
+    return SV_MATMODE_NONE;
+err:
+    fclose(fr);
+    return SV_MATMODE_ERROR;
+}
+
+static void
+set_data(SvPool *mp, const SvMatProp *mat, int i, int j, int k)
+{
+    /* src3d/plan.c:2243-2258 */
+    SvYeeData *d = mp->d;
+    int mm = mp->set->plan.matmode;
+    if (mat->type == SV_MAT_LINEAR) {
+        if (mm == SV_MATMODE_ELECTRIC || mm == SV_MATMODE_FULL) {
+            d->epsilon->data[i][j][k] = (float)(mat->epsilon * EPS0);
+            d->sigma->data[i][j][k] = (float)mat->sigma;
+        }
+        if (mm == SV_MATMODE_MAGNETIC || mm == SV_MATMODE_FULL) {
+            d->mu->data[i][j][k] = (float)(mat->mu * MU0);
+            d->sigast->data[i][j][k] = (float)mat->sigast;
+        }
+        if (d->mat)
+            d->mat->data[i][j][k] = 0;
+    } else
+        d->mat->data[i][j][k] = mat->pos;
+}
+
+typedef struct {
+    int type;
+    float p[6];
+    SvMatProp mat;
+} SvEntity;
+
+/* fast reader for the ASCII STRUCTURED_POINTS integer map, src3d/plan.c:3877-3960 */
+static int
+load_vtk(const char *fname, SvPool *mp)
+{
+    FILE *fr = fopen(fname, "r");
+    char line[256], word[64];
+    int xres, yres, zres, nexp, i, j, k;
+    SvSet *set = mp->set;
+    char *buf, *s, *end;
+    long pos, len;
+
+    if (!fr) {
+        fprintf(stderr, "Error: can not open file %s\n", fname);
+        return 1;
+    }
+    if (!fgets(line, sizeof(line), fr) || !fgets(line, sizeof(line), fr))
+        goto bad;
+    if (fscanf(fr, "%63s", word) != 1 || strcmp(word, "ASCII")) {
+        fprintf(stderr, "Error in VTK file, it should be ascii\n");
+        goto bad;
+    }
+    if (fscanf(fr, "%63s %63s", word, word) != 2 || strcmp(word, "STRUCTURED_POINTS")) {
+        fprintf(stderr, "Error in VTK file, it should be structured points dataset\n");
+        goto bad;
+    }
+    if (fscanf(fr, "%63s", word) != 1 || strcmp(word, "DIMENSIONS")
+        || fscanf(fr, "%d %d %d", &xres, &yres, &zres) != 3) {
+        fprintf(stderr, "Error in VTK file, DIMENSIONS keyword is expected here\n");
+        goto bad;
+    }
+    if (xres != set->sp.xres || yres != set->sp.yres || zres != set->sp.zres) {
+        fprintf(stderr, "Error in VTK file, Dimensions do not match expected values\n");
+        goto bad;
+    }
+    for (i = 0; i < 3; i++)
+        if (!fgets(line, sizeof(line), fr))
+            goto bad;
+    if (fscanf(fr, "%63s %d", word, &nexp) != 2 || strcmp(word, "POINT_DATA") || nexp != xres * yres * zres) {
+        fprintf(stderr, "Error in VTK file, POINT_DATA does not match dimensions\n");
+        goto bad;
+    }
+    for (i = 0; i < 3; i++)
+        if (!fgets(line, sizeof(line), fr))
+            goto bad;
+    pos = ftell(fr);
+    fseek(fr, 0, SEEK_END);
+    len = ftell(fr) - pos;
+    fseek(fr, pos, SEEK_SET);
+    buf = (char *)malloc(len + 1);
+    if (!buf || fread(buf, 1, len, fr) != (size_t)len) {
+        free(buf);
+        goto bad;
+    }
+    buf[len] = 0;
+    s = buf;
+    for (k = 0; k < zres; k++)
+        for (j = 0; j < yres; j++)
+            for (i = 0; i < xres; i++) {
+                long v = strtol(s, &end, 10);
+                if (end == s) {
+                    fprintf(stderr, "Error in VTK file, Error while reading datapoints\n");
+                    free(buf);
+                    goto bad;
+                }
+                mp->d->mat->data[i][j][k] = (int)v;
+                s = end;
+            }
+    free(buf);
+    fclose(fr);
+    return 0;
+bad:
+    fclose(fr);
+    return 1;
+}
+
+static int
+load_data(SvPool *mp, SvSet *set)
+{
+    SvYeeData *d = mp->d;
+    int xres = set->sp.xres, yres = set->sp.yres, zres = set->sp.zres;
+    int i, j, k, n;
+
+    if (set->sm.in_voxel_filename[0]) {
+        FILE *fr = fopen(set->sm.in_voxel_filename, "rb");
+        if (!fr)
+            fprintf(stderr, "Error: cannot open material file %s for reading\n", set->sm.in_voxel_filename);
+        if (set->plan.matmode != SV_MATMODE_NONE && fr) {
+            int res[3];
+            size_t plane = (size_t)zres;
+            if (fread(res, sizeof(int), 3, fr) != 3 || res[0] != xres || res[1] != yres || res[2] != zres) {
+                fprintf(stderr, "Error: material dimensions differ from pool settings\n");
+                fclose(fr);
+                return 1;
+            }
+            if (set->plan.matmode == SV_MATMODE_MAGNETIC) {
+                /* src3d/plan.c:4148: absolute seek that ignores the 12-byte header */
+                fseek(fr, 2 * (long)xres * yres * zres * sizeof(float), SEEK_SET);
+            } else {
+                for (i = 0; i < xres; i++)
+                    for (j = 0; j < yres; j++)
+                        if (fread(d->epsilon->data[i][j], sizeof(float), plane, fr) != plane) goto short_read;
+                for (n = 0; n < xres * yres * zres; n++)
+                    d->epsilon->block[n] *= (float)EPS0;
+                for (i = 0; i < xres; i++)
+                    for (j = 0; j < yres; j++)
+                        if (fread(d->sigma->data[i][j], sizeof(float), plane, fr) != plane) goto short_read;
+            }
+            if (set->plan.matmode == SV_MATMODE_ELECTRIC) {
+                /* src3d/plan.c:4162-4165: returns before the vector file and the boundary map */
+                fclose(fr);
+                return 0;
+            }
+            for (i = 0; i < xres; i++)
+                for (j = 0; j < yres; j++)
+                    if (fread(d->mu->data[i][j], sizeof(float), plane, fr) != plane) goto short_read;
+            for (n = 0; n < xres * yres * zres; n++)
+                d->mu->block[n] *= (float)MU0;
+            for (i = 0; i < xres; i++)
+                for (j = 0; j < yres; j++)
+                    if (fread(d->sigast->data[i][j], sizeof(float), plane, fr) != plane) goto short_read;
+            fclose(fr);
+            fr = NULL;
+        }
+        if (fr)
+            fclose(fr);
+    }
+
+    if (set->sm.in_vector_filename[0]) {
+        FILE *fr = fopen(set->sm.in_vector_filename, "r");
+        SvEntity *ents = NULL;
+        int nent = 0, type;
+        if (!fr) {
+            fprintf(stderr, "Error: cannot open material vector file for loading.\n");
+            return 1;
+        }
+        while (fscanf(fr, "%d", &type) == 1) {
+            SvEntity e;
+            memset(&e, 0, sizeof(e));
+            e.type = type;
+            if (type == SV_ENTITY_SPHERE) {
+                if (scan_floats(fr, e.p, 4)) goto vec_err;
+            } else if (type == SV_ENTITY_VOXEL) {
+                if (scan_floats(fr, e.p, 6)) goto vec_err;
+            } else {
+                fprintf(stderr, "Error: entity type %d is not supported by the MI355X engine\n", type);
+                goto vec_err;
+            }
+            if (scan_material(fr, &e.mat, set->plan.dt))
+                goto vec_err;
+            if (e.mat.type != SV_MAT_LINEAR) {
+                int mi = get_matindex(mp, &e.mat);
+                if (mi == -1)
+                    add_material(mp, &e.mat);
+                else
+                    e.mat.pos = mi;
+            }
+            ents = (SvEntity *)realloc(ents, (nent + 1) * sizeof(SvEntity));
+            ents[nent++] = e;
+        }
+        fclose(fr);
+
+        if (mp->nmat != 0)
+            d->mat = sv_icube_new(xres, yres, zres);
+        if (set->sm.in_vtk_data_filename[0]) {
+            if (!d->mat) {
+                fprintf(stderr, "Error: MEDIUM_VTK_DATA needs at least one tabulated material in MEDIUM_VECTOR\n");
+                free(ents);
+                return 1;
+            }
+            if (load_vtk(set->sm.in_vtk_data_filename, mp)) {
+                free(ents);
+                return 1;
+            }
+            for (n = 0; n < xres * yres * zres; n++)
+                if (d->mat->block[n] < 0 || d->mat->block[n] >= mp->nmat) {
+                    fprintf(stderr, "Error: MEDIUM_VTK_DATA holds material index %d, but only %d materials are defined\n",
+                            d->mat->block[n], mp->nmat);
+                    free(ents);
+                    return 1;
+                }
+        }
+
+        for (n = 0; n < nent; n++) {
+            SvEntity *e = ents + n;
+            if (e->type == SV_ENTITY_SPHERE) {
+                /* src3d/plan.c:4378-4391, float arithmetic as GLfloat */
+                float cx = e->p[0], cy = e->p[1], cz = e->p[2], r = e->p[3];
+                for (i = (int)(0 > cx - r - 1 ? 0 : cx - r - 1); i < (xres < cx + r + 1 ? xres : cx + r + 1); i++)
+                    for (j = (int)(0 > cy - r - 1 ? 0 : cy - r - 1); j < (yres < cy + r + 1 ? yres : cy + r + 1); j++)
+                        for (k = (int)(0 > cz - r - 1 ? 0 : cz - r - 1); k < (zres < cz + r + 1 ? zres : cz + r + 1); k++) {
+                            float x = i + 0.5f, y = j + 0.5f, z = k + 0.5f;
+                            float ddx = x - cx, ddy = y - cy, ddz = z - cz;
+                            volatile float s = ddx * ddx + ddy * ddy + ddz * ddz;
+                            volatile float rr = r * r;
+                            if (s <= rr)
+                                set_data(mp, &e->mat, i, j, k);
+                        }
+            } else {
+                /* src3d/plan.c:4467-4479 */
+                for (i = (int)(0 > e->p[0] ? 0 : e->p[0]); i < (xres < e->p[3] ? xres : e->p[3]); i++)
+                    for (j = (int)(0 > e->p[1] ? 0 : e->p[1]); j < (yres < e->p[4] ? yres : e->p[4]); j++)
+                        for (k = (int)(0 > e->p[2] ? 0 : e->p[2]); k < (zres < e->p[5] ? zres : e->p[5]); k++)
+                            set_data(mp, &e->mat, i, j, k);
+            }
+        }
+        free(ents);
+
+        /* src3d/plan.c:4640-4672: relative -> absolute material constants */
+        for (n = 0; n < mp->nmat; n++) {
+            if (sv_material_is_tabulated(mp->mats[n].type)) {
+                mp->mats[n].epsilon *= EPS0;
+                mp->mats[n].mu *= MU0;
+            }
+        }
+        goto after_vector;
+vec_err:
+        fprintf(stderr, "Error: cannot parse material vector file %s\n", set->sm.in_vector_filename);
+        fclose(fr);
+        free(ents);
+        return 1;
+    }
+after_vector:
+
+    /* TSF skip map, src3d/plan.c:5297-5329 */
+    if (mp->src->tsf && set->ss.tsf.box_boundary_skipdepth >= 0) {
+        SvSourceTSF *t = mp->src->tsf;
+        int sd = set->ss.tsf.box_boundary_skipdepth;
+        d->sourceskip = sv_icube_new(xres, yres, zres);
+        #pragma omp parallel for private(j, k) schedule(dynamic, 1)
+        for (i = 0; i < xres; i++)
+            for (j = 0; j < yres; j++)
+                for (k = 0; k < zres; k++) {
+                    int ii, jj, kk, hit = 0;
+                    if (i > (t->box_i0 + sd) && j > (t->box_j0 + sd) && k > (t->box_k0 + sd)
+                        && i < (t->box_in - sd) && j < (t->box_jn - sd) && k < (t->box_kn - sd))
+                        continue;
+                    for (ii = (i - sd > 0 ? i - sd : 0); ii < (i + sd < xres ? i + sd : xres) && !hit; ii++)
+                        for (jj = (j - sd > 0 ? j - sd : 0); jj < (j + sd < yres ? j + sd : yres) && !hit; jj++)
+                            for (kk = (k - sd > 0 ? k - sd : 0); kk < (k + sd < zres ? k + sd : zres) && !hit; kk++) {
+                                /* sv_yee_data_get_tsf_mattype, src3d/yee.c:1684-1705 */
+                                int m = (mp->nmat > 0) ? d->mat->data[ii][jj][kk] : -1, vac = 0;
+                                if (m == 0 || m == -1) {
+                                    if ((set->plan.matmode == SV_MATMODE_FULL || set->plan.matmode == SV_MATMODE_ELECTRIC)
+                                        && fabs(1.0 - d->epsilon->data[ii][jj][kk] / EPS0) < 0.001)
+                                        vac = 1;
+                                    if (set->plan.matmode == SV_MATMODE_NONE)
+                                        vac = 1;
+                                }
+                                if (m > 0 && mp->mats[m].type == SV_MAT_LINTAB && fabs(1.0 - mp->mats[m].epsilon / EPS0) < 0.001)
+                                    vac = 1;
+                                if (!vac)
+                                    hit = 1;
+                            }
+                    if (hit)
+                        d->sourceskip->data[i][j][k] = 1;
+                }
+    }
+
+    /* internal boundaries, src3d/plan.c:5381-5417 */
+    sv_icube_fill(d->bnds, 0);
+    #pragma omp parallel for private(j, k) schedule(static)
+    for (i = 0; i < xres; i++)
+        for (j = 0; j < yres; j++)
+            for (k = 0; k < zres; k++) {
+                int ii, jj, kk, m, nm;
+                double e, s, ne, ns;
+                if (i == 0 || j == 0 || k == 0 || i == xres - 1 || j == yres - 1 || k == zres - 1) {
+                    d->bnds->data[i][j][k] = 1;
+                    continue;
+                }
+                m = sv_pool_epsilon_sigma(mp, i, j, k, &e, &s);
+                for (ii = -1; ii <= 1; ii++)
+                    for (jj = -1; jj <= 1; jj++)
+                        for (kk = -1; kk <= 1; kk++) {
+                            if (d->bnds->data[i][j][k] == 1)
+                                continue;
+                            nm = sv_pool_epsilon_sigma(mp, i + ii, j + jj, k + kk, &ne, &ns);
+                            if (nm != m || ne != e || ns != s)
+                                d->bnds->data[i][j][k] = 1;
+                        }
+            }
+    return 0;
+
+short_read:
+    fprintf(stderr, "Error: material file %s is shorter than the pool\n", set->sm.in_voxel_filename);
+    return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* CPML grading (src3d/boundary.c:33-99, 101-280)                      */
+/* ------------------------------------------------------------------ */
+
+static void
+becebhch(int near, int i, int imax, double sigma, double kappa, double a, int m, double dt,
+         double *be, double *ce, double *bh, double *ch, double *vkappa, double *hkappa)
+{
+    /* near faces use the H offset -0.5 (becebhch0), far faces +0.5 (becebhch) */
+    double off = near ? -0.5 : 0.5;
+    double powdepthe = pow((imax - i) / (double)(imax - 0), m);
+    double powdepthh = pow((imax - i + off) / (double)(imax - 0), m);
+    double sve = sigma * powdepthe;
+    double kve = 1 + (kappa - 1) * powdepthe;
+    double ave = a * pow(((i - 0) / ((double)imax - 0)), 1.0);
+    double svh = sigma * powdepthh;
+    double kvh = 1 + (kappa - 1) * powdepthh;
+    /* alpha grows into the layer: (i - 0.5) on near faces, (i + 0.5) on far faces */
+    double avh = a * pow(((i + off) / ((double)imax - 0)), 1.0);
+
+    *be = exp(-(sve / kve + ave) * dt / EPS0);
+    *ce = (sve == 0) ? 0 : (sve / (sve + kve * ave) / kve) * ((*be) - 1.0);
+    *bh = exp(-(svh / kvh + avh) * dt / EPS0);
+    *ch = (svh == 0) ? 0 : (svh / (svh + kvh * avh) / kvh) * ((*bh) - 1.0);
+    *vkappa = kve;
+    *hkappa = kvh;
+}
+
+static void
+cpml_face(SvCpmlFace *f, int near, int depth, double sigma, double kappa, double a, int m, double dt)
+{
+    int i;
+    f->depth = depth;
+    f->be = (double *)calloc(6 * (size_t)depth, sizeof(double));
+    f->ce = f->be + depth;
+    f->bh = f->ce + depth;
+    f->ch = f->bh + depth;
+    f->kappae = f->ch + depth;
+    f->kappah = f->kappae + depth;
+    for (i = 0; i < depth; i++) {
+        int arg = near ? i : depth - i;
+        becebhch(near, arg, depth, sigma, kappa, a, m, dt, f->be + i, f->ce + i, f->bh + i, f->ch + i,
+                 f->kappae + i, f->kappah + i);
+    }
+}
+
+static SvBoundary *
+boundary_new(SvSet *set)
+{
+    SvBoundary *b = (SvBoundary *)calloc(1, sizeof(SvBoundary));
+    SvSetBoundary *s = &set->sb;
+    double dt = set->plan.dt;
+    if (s->bx0 == SV_BOUNDARY_CPML) cpml_face(&b->x0, 1, s->depth_bx0, s->sigma_bx0, s->kappa_bx0, s->a_bx0, s->m_bx0, dt);
+    if (s->bxn == SV_BOUNDARY_CPML) cpml_face(&b->xn, 0, s->depth_bxn, s->sigma_bxn, s->kappa_bxn, s->a_bxn, s->m_bxn, dt);
+    if (s->by0 == SV_BOUNDARY_CPML) cpml_face(&b->y0, 1, s->depth_by0, s->sigma_by0, s->kappa_by0, s->a_by0, s->m_by0, dt);
+    if (s->byn == SV_BOUNDARY_CPML) cpml_face(&b->yn, 0, s->depth_byn, s->sigma_byn, s->kappa_byn, s->a_byn, s->m_byn, dt);
+    if (s->bz0 == SV_BOUNDARY_CPML) cpml_face(&b->z0, 1, s->depth_bz0, s->sigma_bz0, s->kappa_bz0, s->a_bz0, s->m_bz0, dt);
+    if (s->bzn == SV_BOUNDARY_CPML) cpml_face(&b->zn, 0, s->depth_bzn, s->sigma_bzn, s->kappa_bzn, s->a_bzn, s->m_bzn, dt);
+    return b;
+}
+
+/* ------------------------------------------------------------------ */
+/* far field (src3d/farfield.c:56-172)                                  */
+/* ------------------------------------------------------------------ */
+
+static SvFarfield *
+farfield_new(SvSet *set)
+{
+    SvFarfield *ff = (SvFarfield *)calloc(1, sizeof(SvFarfield));
+    double div = SV_LIGHT_SPEED * set->plan.dt / set->sp.dx;
+    int n;
+
+    ff->ndata = (int)(2000 + 2 * set->sc.nsteps
+                      + sqrt((double)(set->sp.xres * set->sp.xres + set->sp.yres * set->sp.yres + set->sp.zres * set->sp.zres)) / div);
+    ff->nrs = set->sf.nrs;
+    if (ff->nrs)
+        ff->rpoints = (SvRPoint *)calloc(ff->nrs, sizeof(SvRPoint));
+    for (n = 0; n < ff->nrs; n++) {
+        double ri = set->sf.ri[n] / div, rj = set->sf.rj[n] / div, rk = set->sf.rk[n] / div;
+        double v = sqrt(ri * ri + rj * rj + rk * rk) - 500;
+        ff->rpoints[n].istart = (int)(v > 0 ? v : 0);
+        ff->rpoints[n].ex = (double *)calloc(3 * (size_t)ff->ndata, sizeof(double));
+        ff->rpoints[n].ey = ff->rpoints[n].ex + ff->ndata;
+        ff->rpoints[n].ez = ff->rpoints[n].ey + ff->ndata;
+    }
+    ff->ndft = set->sf.ndft;
+    if (ff->ndft) {
+        int q;
+        long ix = set->sf.box_in - set->sf.box_i0, iy = set->sf.box_jn - set->sf.box_j0, iz = set->sf.box_kn - set->sf.box_k0;
+        ff->dft_omega = (double *)malloc(ff->ndft * sizeof(double));
+        for (q = 0; q < ff->ndft; q++)
+            ff->dft_omega[q] = 2 * M_PI * SV_LIGHT_SPEED / set->sf.dft_lambda[q];
+        ff->dft_ncells = 2 * (iy * iz + ix * iz + ix * iy);
+        ff->dft_data = (double *)calloc((size_t)ff->dft_ncells * 4 * ff->ndft * 2, sizeof(double));
+    }
+    return ff;
+}
+
+/* ------------------------------------------------------------------ */
+/* planner entry point                                                  */
+/* ------------------------------------------------------------------ */
+
+SvPool *
+init_and_make_plan(SvSet *set)
+{
+    SvPool *mp = (SvPool *)calloc(1, sizeof(SvPool));
+    SvYeeData *d;
+    int xres = set->sp.xres, yres = set->sp.yres, zres = set->sp.zres;
+    int matmode_mat, matmode_script, n;
+
+    set->plan.gpumode = SV_GPUMODE_NONE;
+    set->plan.dt = 1.0 / SV_LIGHT_SPEED / sqrt(1.0 / set->sp.dx / set->sp.dx + 1.0 / set->sp.dy / set->sp.dy
+                                               + 1.0 / set->sp.dz / set->sp.dz) * set->sc.dtmult;
+    mp->set = set;
+    mp->d = d = (SvYeeData *)calloc(1, sizeof(SvYeeData));
+    d->dt = set->plan.dt;
+    mp->src = (SvSource *)calloc(1, sizeof(SvSource));
+    mp->out = (SvOutput *)calloc(1, sizeof(SvOutput));
+
+    if (xres < 3 || yres < 3 || zres < 3) {
+        fprintf(stderr, "Error: pool must be at least 3 cells along each axis\n");
+        return NULL;
+    }
+    if (load_sources(mp, set))
+        return NULL;
+
+    /* matmode, src3d/plan.c:5489-5518 */
+    if (set->sm.matmode_check) {
+        matmode_mat = SV_MATMODE_NONE;
+        matmode_script = SV_MATMODE_NONE;
+        if (set->sm.in_voxel_filename[0])
+            matmode_mat = test_material_file(set->sm.in_voxel_filename);
+        if (set->sm.in_vector_filename[0])
+            matmode_script = test_script_file(set->sm.in_vector_filename);
+        if (matmode_script == SV_MATMODE_ERROR)
+            return NULL;
+        if (matmode_mat == SV_MATMODE_NONE && matmode_script == SV_MATMODE_NONE)
+            set->plan.matmode = SV_MATMODE_NONE;
+        else if (matmode_mat == SV_MATMODE_ELECTRIC && (matmode_script == SV_MATMODE_NONE || matmode_script == SV_MATMODE_ELECTRIC))
+            set->plan.matmode = SV_MATMODE_ELECTRIC;
+        else if (matmode_mat == SV_MATMODE_MAGNETIC && (matmode_script == SV_MATMODE_NONE || matmode_script == SV_MATMODE_MAGNETIC))
+            set->plan.matmode = SV_MATMODE_MAGNETIC;
+        else if (matmode_script == SV_MATMODE_ELECTRIC && (matmode_mat == SV_MATMODE_NONE || matmode_mat == SV_MATMODE_ELECTRIC))
+            set->plan.matmode = SV_MATMODE_ELECTRIC;
+        else if (matmode_script == SV_MATMODE_MAGNETIC && (matmode_mat == SV_MATMODE_NONE || matmode_mat == SV_MATMODE_MAGNETIC))
+            set->plan.matmode = SV_MATMODE_MAGNETIC;
+        else
+            set->plan.matmode = SV_MATMODE_FULL;
+    } else {
+        if (set->sm.in_vector_filename[0] && test_script_file(set->sm.in_vector_filename) == SV_MATMODE_ERROR)
+            return NULL;
+        set->plan.matmode = SV_MATMODE_FULL;
+    }
+
+    /* field storage, src3d/yee.c:101-141 */
+    d->xres = xres; d->yres = yres; d->zres = zres;
+    d->dx = set->sp.dx; d->dy = set->sp.dy; d->dz = set->sp.dz;
+    d->ex = sv_dcube_new(xres, yres, zres);
+    d->ey = sv_dcube_new(xres, yres, zres);
+    d->ez = sv_dcube_new(xres, yres, zres);
+    d->hx = sv_dcube_new(xres, yres, zres);
+    d->hy = sv_dcube_new(xres, yres, zres);
+    d->hz = sv_dcube_new(xres, yres, zres);
+    d->bnds = sv_icube_new(xres, yres, zres);
+    if (!d->ex || !d->ey || !d->ez || !d->hx || !d->hy || !d->hz || !d->bnds) {
+        fprintf(stderr, "Error: cannot allocate host field storage\n");
+        return NULL;
+    }
+    if (set->plan.matmode == SV_MATMODE_FULL || set->plan.matmode == SV_MATMODE_ELECTRIC) {
+        d->epsilon = sv_fcube_new(xres, yres, zres);
+        d->sigma = sv_fcube_new(xres, yres, zres);
+        sv_fcube_fill(d->epsilon, (float)EPS0);
+    }
+    if (set->plan.matmode == SV_MATMODE_FULL || set->plan.matmode == SV_MATMODE_MAGNETIC) {
+        d->mu = sv_fcube_new(xres, yres, zres);
+        d->sigast = sv_fcube_new(xres, yres, zres);
+        sv_fcube_fill(d->mu, (float)MU0);
+    }
+
+    mp->bnd = boundary_new(set);
+    mp->farfield = farfield_new(set);
+
+    if (set->sm.in_voxel_filename[0] || set->sm.in_vector_filename[0]) {
+        if (load_data(mp, set))
+            return NULL;
+    }
+
+    /* outputs, src3d/output.c (sv_pool_allocate_output) */
+    if (set->so.npnts) {
+        mp->out->outpointdata = (double **)calloc(set->so.npnts, sizeof(double *));
+        for (n = 0; n < set->so.npnts; n++) {
+            SvOutputPar *o = set->so.pnts + n;
+            if (o->i < 0 || o->j < 0 || o->k < 0 || o->i >= xres || o->j >= yres || o->k >= zres) {
+                fprintf(stderr, "Error: output point %d at (%d %d %d) is outside the pool\n", n, o->i, o->j, o->k);
+                return NULL;
+            }
+            mp->out->outpointdata[n] = (double *)calloc(6 * (size_t)(set->sc.nsteps + 1), sizeof(double));
+        }
+    }
+
+    if (set->sf.nrs || set->sf.ndft) {
+        SvSetFarfield *f = &set->sf;
+        if (f->box_i0 < 1 || f->box_j0 < 1 || f->box_k0 < 1 || f->box_in >= xres - 1 || f->box_jn >= yres - 1
+            || f->box_kn >= zres - 1 || f->box_in <= f->box_i0 || f->box_jn <= f->box_j0 || f->box_kn <= f->box_k0) {
+            fprintf(stderr, "Error: NFFF box must lie inside the pool with one cell to spare on each side\n");
+            return NULL;
+        }
+    }
+    return mp;
+}
+
+void
+sv_pool_free(SvPool *mp, SvSet *set)
+{
+    int n;
+    SvYeeData *d;
+    (void)set;
+    if (!mp)
+        return;
+    d = mp->d;
+    if (d) {
+        sv_dcube_free(d->ex); sv_dcube_free(d->ey); sv_dcube_free(d->ez);
+        sv_dcube_free(d->hx); sv_dcube_free(d->hy); sv_dcube_free(d->hz);
+        sv_fcube_free(d->epsilon); sv_fcube_free(d->sigma);
+        sv_fcube_free(d->mu); sv_fcube_free(d->sigast);
+        sv_icube_free(d->mat); sv_icube_free(d->bnds); sv_icube_free(d->sourceskip);
+        free(d);
+    }
+    if (mp->src) {
+        for (n = 0; n < mp->src->npnts; n++) {
+            free(mp->src->sp[n].layered_zpos);
+            free(mp->src->sp[n].ex);
+        }
+        free(mp->src->sp);
+        if (mp->src->tsf) {
+            free(mp->src->tsf->e);
+            free(mp->src->tsf);
+        }
+        free(mp->src);
+    }
+    if (mp->out) {
+        if (mp->out->outpointdata) {
+            for (n = 0; n < mp->set->so.npnts; n++)
+                free(mp->out->outpointdata[n]);
+            free(mp->out->outpointdata);
+        }
+        free(mp->out);
+    }
+    if (mp->bnd) {
+        free(mp->bnd->x0.be); free(mp->bnd->xn.be); free(mp->bnd->y0.be);
+        free(mp->bnd->yn.be); free(mp->bnd->z0.be); free(mp->bnd->zn.be);
+        free(mp->bnd);
+    }
+    if (mp->farfield) {
+        for (n = 0; n < mp->farfield->nrs; n++)
+            free(mp->farfield->rpoints[n].ex);
+        free(mp->farfield->rpoints);
+        free(mp->farfield->dft_omega);
+        free(mp->farfield->dft_data);
+        free(mp->farfield);
+    }
+    free(mp->mats);
+    free(mp);
+}

@@ -1,0 +1,545 @@
+/*
+ * settings.c -- GSvit parameter-file reader for the 3-D time-stepping path.
+ *
+ * Accepts the keyword format of src3d/settings.c:3822-5840 (whitespace separated
+ * keywords, '#' comments, values read with the same integer/double rules as
+ * get_int/get_double, src3d/settings.c:103-135).  Keywords outside the hot path
+ * (images, volumes, sub-gridding, local/focused/layered sources, forces) are
+ * rejected with an error instead of being silently ignored, so a parameter
+ * file never runs with part of its physics dropped.  Keywords that only
+ * request file output unrelated to the step loop (OUT_IMAGE, OUT_FILE,
+ * OUT_PLANE, OUT_VOLUME) are consumed and ignored with a warning.
+ *
+ * Generated sources (types 1 and 2, src3d/settings.c:444-548) are produced here
+ * with the reference's waveform formulas and then rounded to 6 significant
+ * digits, because the reference writes them with "%g" and reads them back
+ * (src3d/settings.c:515-538, src3d/plan.c:1449-1512).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include "gsvit_settings.h"
+#include "gsvit_internal.h"
+
+void
+clear_settings(SvSet *set)
+{
+    memset(set, 0, sizeof(SvSet));
+    /* defaults from src3d/settings.c:570-990 and src3d/defaults.h */
+    set->sp.xres = set->sp.yres = set->sp.zres = 100;
+    set->sp.dx = set->sp.dy = set->sp.dz = 1e-8;
+    set->sc.nsteps = 100;
+    set->sc.dtmult = 1.0;
+    set->sc.nthreads = -1;
+    set->sc.verbose = 0;
+    set->sm.matmode_check = 0;
+    set->ss.tsf.box_i0 = set->ss.tsf.box_j0 = set->ss.tsf.box_k0 = 10;
+    set->ss.tsf.box_in = set->ss.tsf.box_jn = set->ss.tsf.box_kn = 90;
+    set->ss.tsf.layered_epsilon = 1;
+    set->ss.tsf.layered_mu = 1;
+    set->ss.tsf.box_boundary_skipdepth = -1;
+    set->ss.tsf.source_mode = 1;
+    set->ss.tsf.source_wl = 0.633e-6;
+    set->ss.tsf.source_amplitude = 1.0;
+    set->ss.tsf.source_pulsewidth = 100;
+    set->sb.depth_bx0 = set->sb.depth_bxn = set->sb.depth_by0 = 10;
+    set->sb.depth_byn = set->sb.depth_bz0 = set->sb.depth_bzn = 10;
+    set->smb.bxnpos = set->smb.bynpos = set->smb.bznpos = -1;
+    set->sf.sumfrom = 0;
+    set->sf.sumto = 0;
+}
+
+void
+free_settings(SvSet *set)
+{
+    free(set->ss.pnts);
+    free(set->so.pnts);
+    free(set->so.sums);
+    free(set->sf.ri);
+    free(set->sf.rj);
+    free(set->sf.rk);
+    free(set->sf.individual);
+    free(set->sf.source_filename);
+    free(set->sf.dft_lambda);
+    memset(set, 0, sizeof(SvSet));
+}
+
+static int
+get_token(FILE *fr, char *buf, size_t n)
+{
+    char fmt[16];
+    snprintf(fmt, sizeof(fmt), "%%%ds", (int)n - 1);
+    return fscanf(fr, fmt, buf) == 1 ? 0 : 1;
+}
+
+/* src3d/settings.c:103-119: integers may be written in exponential form */
+static int
+get_int(FILE *fr, int *val, const char *key)
+{
+    char value[100], *err;
+    if (get_token(fr, value, sizeof(value))) {
+        fprintf(stderr, "Error: missing integer value (key %s)\n", key);
+        return 1;
+    }
+    *val = (int)strtol(value, &err, 10);
+    if (*err != '\0') {
+        *val = (int)strtod(value, &err);
+        if (*err != '\0') {
+            fprintf(stderr, "Error parsing integer value (key %s)\n", key);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* src3d/settings.c:121-135 */
+static int
+get_double(FILE *fr, double *val, const char *key)
+{
+    char value[100], *err;
+    if (get_token(fr, value, sizeof(value))) {
+        fprintf(stderr, "Error: missing double value (key %s)\n", key);
+        return 1;
+    }
+    *val = strtod(value, &err);
+    if (*err != '\0') {
+        fprintf(stderr, "Error parsing double value (key %s)\n", key);
+        return 1;
+    }
+    return 0;
+}
+
+/* rest of the line, whitespace stripped (the reference's "%100[^\n\r]" + g_strstrip) */
+static int
+get_rest_of_line(FILE *fr, char *out, size_t n)
+{
+    char buf[SV_NAME_LEN];
+    char *s, *e;
+    int c;
+
+    /* skip blanks on this line, then continue onto the next non-empty line,
+     * as fscanf("%[^\n]") would after the previous token */
+    do {
+        c = fgetc(fr);
+    } while (c == ' ' || c == '\t' || c == '\n' || c == '\r');
+    if (c == EOF) {
+        out[0] = 0;
+        return 1;
+    }
+    ungetc(c, fr);
+    if (!fgets(buf, sizeof(buf), fr)) {
+        out[0] = 0;
+        return 1;
+    }
+    s = buf;
+    while (*s == ' ' || *s == '\t')
+        s++;
+    e = s + strlen(s);
+    while (e > s && (e[-1] == '\n' || e[-1] == '\r' || e[-1] == ' ' || e[-1] == '\t'))
+        *--e = 0;
+    snprintf(out, n, "%s", s);
+    return out[0] == 0;
+}
+
+/* src3d/settings.c:137-170 */
+static int
+get_component(FILE *fr, int *val, const char *key)
+{
+    char c[32];
+    if (get_token(fr, c, sizeof(c)))
+        return 1;
+    if (!strcmp(c, "Ex") || !strcmp(c, "ex")) *val = SV_COMP_EX;
+    else if (!strcmp(c, "Ey") || !strcmp(c, "ey")) *val = SV_COMP_EY;
+    else if (!strcmp(c, "Ez") || !strcmp(c, "ez")) *val = SV_COMP_EZ;
+    else if (!strcmp(c, "Hx") || !strcmp(c, "hx")) *val = SV_COMP_HX;
+    else if (!strcmp(c, "Hy") || !strcmp(c, "hy")) *val = SV_COMP_HY;
+    else if (!strcmp(c, "Hz") || !strcmp(c, "hz")) *val = SV_COMP_HZ;
+    else if (!strcmp(c, "All") || !strcmp(c, "all")) *val = SV_COMP_ALL;
+    else {
+        fprintf(stderr, "Error: field component %s is not supported here (key %s)\n", c, key);
+        return 1;
+    }
+    return 0;
+}
+
+static double
+round_g6(double v)
+{
+    char b[64];
+    snprintf(b, sizeof(b), "%g", v);
+    return strtod(b, NULL);
+}
+
+/* Waveform of a generated source, src3d/settings.c:444-512 (SRC_SINE = 1,
+ * SRC_PULSE = 2); values rounded as the reference's "%g" round trip. */
+int
+sv_generate_waveform(int type, int nsteps, double wavelength, int pulse_width, double dt,
+                     double amplitude, double *vals)
+{
+    int i, dampval = 100;
+    double wavefreq = SV_LIGHT_SPEED * 2 * M_PI / wavelength;
+    double dpulse_width = pulse_width * dt;
+
+    if (type != 1 && type != 2) {
+        fprintf(stderr, "Error: generated source type %d is not supported (use 1 = sine, 2 = pulse, or a file)\n", type);
+        return 1;
+    }
+    for (i = 0; i < nsteps; i++) {
+        double v = amplitude * sin(wavefreq * dt * (double)i);
+        if (i < dampval)
+            v *= (1.0 / (double)dampval) * ((double)i);
+        if (type == 2) {
+            double damp = ((double)i * dt - 4.0 * dpulse_width) / sqrt(2.0) / dpulse_width;
+            v *= exp(-damp * damp);
+        }
+        vals[i] = v;
+    }
+    return 0;
+}
+
+static double
+plan_dt(const SvSet *set)
+{
+    /* src3d/plan.c:5452 */
+    return 1.0 / SV_LIGHT_SPEED / sqrt(1.0 / set->sp.dx / set->sp.dx + 1.0 / set->sp.dy / set->sp.dy
+                                       + 1.0 / set->sp.dz / set->sp.dz) * set->sc.dtmult;
+}
+
+static int
+parse_cpml_face(FILE *fr, const char *key, const SvSet *set, int *type, int *depth, int *m,
+                double *sigma, double *a, double *kappa)
+{
+    char value[64];
+    if (get_token(fr, value, sizeof(value)))
+        return 1;
+    if (!strcmp(value, "none"))
+        *type = SV_BOUNDARY_NONE;
+    else if (!strcmp(value, "pec"))
+        *type = SV_BOUNDARY_PEC;
+    else if (!strcmp(value, "liao"))
+        *type = SV_BOUNDARY_LIAO;
+    else if (!strcmp(value, "cpml")) {
+        *type = SV_BOUNDARY_CPML;
+        if (get_int(fr, depth, key) || get_int(fr, m, key) || get_double(fr, sigma, key))
+            return 1;
+        /* src3d/settings.c:5150-5154: sigma == -1 selects 0.8 m / (377 dx) */
+        if (*sigma == -1)
+            *sigma = 0.8 * (double)(*m) / (377 * set->sp.dx);
+        if (get_double(fr, a, key) || get_double(fr, kappa, key))
+            return 1;
+    } else {
+        fprintf(stderr, "Error: unsupported boundary condition (%s)\n", value);
+        return 1;
+    }
+    if (*type == SV_BOUNDARY_PEC) {
+        fprintf(stderr, "Error: PEC outer boundary is not supported on this path\n");
+        return 1;
+    }
+    return 0;
+}
+
+static int
+skip_output_keyword(FILE *fr, const char *key)
+{
+    char buf[SV_NAME_LEN];
+    fprintf(stderr, "Warning: %s output is not produced by this engine, ignored\n", key);
+    return get_rest_of_line(fr, buf, sizeof(buf)) && 0;
+}
+
+int
+parse_settings(const char *filename, SvSet *set)
+{
+    char key[128];
+    char value[64];
+    FILE *fr = fopen(filename, "r");
+
+    if (!fr) {
+        fprintf(stderr, "Error: cannot open parameter file %s\n", filename);
+        return 0;
+    }
+
+    while (fscanf(fr, "%127s", key) == 1) {
+        if (strchr(key, '#')) {
+            char buf[256];
+            if (!fgets(buf, sizeof(buf), fr))
+                break;
+            continue;
+        }
+        if (!strcmp(key, "DIMENSIONALITY")) {
+            int dim;
+            if (get_int(fr, &dim, key))
+                goto fail;
+            if (dim != 3) {
+                fprintf(stderr, "Error: parameter file dimensionality should be 3\n");
+                goto fail;
+            }
+        } else if (!strcmp(key, "VERBOSE")) {
+            if (get_int(fr, &set->sc.verbose, key))
+                goto fail;
+        } else if (!strcmp(key, "POOL")) {
+            if (get_int(fr, &set->sp.xres, key) || get_int(fr, &set->sp.yres, key) || get_int(fr, &set->sp.zres, key)
+                || get_double(fr, &set->sp.dx, key) || get_double(fr, &set->sp.dy, key) || get_double(fr, &set->sp.dz, key))
+                goto fail;
+        } else if (!strcmp(key, "COMP")) {
+            if (get_int(fr, &set->sc.nsteps, key))
+                goto fail;
+        } else if (!strcmp(key, "DT_MULT")) {
+            if (get_double(fr, &set->sc.dtmult, key))
+                goto fail;
+        } else if (!strcmp(key, "GPU")) {
+            if (get_int(fr, &set->sc.usegpu, key))
+                goto fail;
+        } else if (!strcmp(key, "UGPU")) {
+            int v;
+            if (get_int(fr, &v, key))
+                goto fail;
+            if (v >= 0 && v < SV_MAX_GPUS)
+                set->sc.ugpu[v] = 1;
+        } else if (!strcmp(key, "GPU_QUERY")) {
+            if (get_int(fr, &set->sc.devicequery, key))
+                goto fail;
+        } else if (!strcmp(key, "THREADS")) {
+            if (get_int(fr, &set->sc.nthreads, key))
+                goto fail;
+        } else if (!strcmp(key, "MATMODE_CHECK")) {
+            if (get_int(fr, &set->sm.matmode_check, key))
+                goto fail;
+        } else if (!strcmp(key, "MEDIUM_LINEAR")) {
+            if (get_rest_of_line(fr, set->sm.in_voxel_filename, SV_NAME_LEN))
+                goto fail;
+        } else if (!strcmp(key, "MEDIUM_VECTOR")) {
+            if (get_rest_of_line(fr, set->sm.in_vector_filename, SV_NAME_LEN))
+                goto fail;
+        } else if (!strcmp(key, "MEDIUM_VTK_DATA")) {
+            if (get_rest_of_line(fr, set->sm.in_vtk_data_filename, SV_NAME_LEN))
+                goto fail;
+        } else if (!strcmp(key, "SOURCE_POINT")) {
+            SvSetSourcePoint *p;
+            int srctype;
+            set->ss.pnts = (SvSetSourcePoint *)realloc(set->ss.pnts, (set->ss.npnts + 1) * sizeof(SvSetSourcePoint));
+            p = set->ss.pnts + set->ss.npnts;
+            memset(p, 0, sizeof(*p));
+            if (get_int(fr, &p->point_origin_position_i, key) || get_int(fr, &p->point_origin_position_j, key)
+                || get_int(fr, &p->point_origin_position_k, key) || get_int(fr, &srctype, key))
+                goto fail;
+            p->source_mode = srctype;
+            if (srctype == 0) {
+                if (get_rest_of_line(fr, p->source_filename, SV_NAME_LEN))
+                    goto fail;
+            } else if (srctype == 1 || srctype == 2) {
+                /* src3d/settings.c:4090-4140 */
+                p->source_pulsewidth = 100;
+                if (get_double(fr, &p->source_wl, key))
+                    goto fail;
+                if (srctype == 2 && get_int(fr, &p->source_pulsewidth, key))
+                    goto fail;
+                if (get_double(fr, &p->source_amplitude, key) || get_double(fr, &p->point_origin_theta, key)
+                    || get_double(fr, &p->point_origin_phi, key))
+                    goto fail;
+            } else {
+                fprintf(stderr, "Error: point source type %d is not supported\n", srctype);
+                goto fail;
+            }
+            set->ss.npnts++;
+        } else if (!strcmp(key, "SOURCE_TSF")) {
+            SvSetSourceTSF *t = &set->ss.tsf;
+            int srctype;
+            if (get_int(fr, &t->box_i0, key) || get_int(fr, &t->box_j0, key) || get_int(fr, &t->box_k0, key)
+                || get_int(fr, &t->box_in, key) || get_int(fr, &t->box_jn, key) || get_int(fr, &t->box_kn, key)
+                || get_double(fr, &t->ia_theta, key) || get_double(fr, &t->ia_phi, key) || get_double(fr, &t->ia_psi, key)
+                || get_int(fr, &srctype, key))
+                goto fail;
+            t->source_mode = srctype;
+            if (srctype == 0) {
+                if (get_rest_of_line(fr, t->source_filename, SV_NAME_LEN))
+                    goto fail;
+            } else if (srctype == 1 || srctype == 2) {
+                /* src3d/settings.c:4253-4281 */
+                if (get_double(fr, &t->source_wl, key))
+                    goto fail;
+                if (srctype == 2 && get_int(fr, &t->source_pulsewidth, key))
+                    goto fail;
+                if (get_double(fr, &t->source_amplitude, key))
+                    goto fail;
+            } else {
+                fprintf(stderr, "Error: TSF source type %d is not supported\n", srctype);
+                goto fail;
+            }
+            t->valid = 1;
+        } else if (!strcmp(key, "TSF_SKIP")) {
+            SvSetSourceTSF *t = &set->ss.tsf;
+            if (get_token(fr, value, sizeof(value)))
+                goto fail;
+            if (!strcmp(value, "i0")) t->box_boundary_skipi0 = 1;
+            else if (!strcmp(value, "in")) t->box_boundary_skipin = 1;
+            else if (!strcmp(value, "j0")) t->box_boundary_skipj0 = 1;
+            else if (!strcmp(value, "jn")) t->box_boundary_skipjn = 1;
+            else if (!strcmp(value, "k0")) t->box_boundary_skipk0 = 1;
+            else if (!strcmp(value, "kn")) t->box_boundary_skipkn = 1;
+            else if (!strcmp(value, "depth")) {
+                if (get_int(fr, &t->box_boundary_skipdepth, key))
+                    goto fail;
+            }
+        } else if (!strcmp(key, "TSFSOURCE_MATERIAL")) {
+            SvSetSourceTSF *t = &set->ss.tsf;
+            if (get_double(fr, &t->layered_epsilon, key) || get_double(fr, &t->layered_mu, key)
+                || get_double(fr, &t->layered_sigma, key) || get_double(fr, &t->layered_sigast, key))
+                goto fail;
+        } else if (!strcmp(key, "SOURCE_WAVELENGTH")) {
+            if (get_double(fr, &set->ss.lambda_min, key) || get_double(fr, &set->ss.lambda_center, key)
+                || get_double(fr, &set->ss.lambda_max, key))
+                goto fail;
+        } else if (!strcmp(key, "OUT_POINT")) {
+            SvOutputPar *o;
+            set->so.pnts = (SvOutputPar *)realloc(set->so.pnts, (set->so.npnts + 1) * sizeof(SvOutputPar));
+            o = set->so.pnts + set->so.npnts;
+            memset(o, 0, sizeof(*o));
+            if (get_component(fr, &o->component, key) || get_int(fr, &o->step, key) || get_int(fr, &o->i, key)
+                || get_int(fr, &o->j, key) || get_int(fr, &o->k, key))
+                goto fail;
+            if (get_rest_of_line(fr, o->filebase, SV_NAME_LEN))
+                goto fail;
+            set->so.npnts++;
+        } else if (!strcmp(key, "OUT_IMAGE") || !strcmp(key, "OUT_FILE") || !strcmp(key, "OUT_PLANE")
+                   || !strcmp(key, "OUT_VOLUME")) {
+            skip_output_keyword(fr, key);
+        } else if (!strcmp(key, "BOUNDARY_ALL")) {
+            SvSetBoundary *b = &set->sb;
+            if (parse_cpml_face(fr, key, set, &b->bx0, &b->depth_bx0, &b->m_bx0, &b->sigma_bx0, &b->a_bx0, &b->kappa_bx0))
+                goto fail;
+            b->bxn = b->by0 = b->byn = b->bz0 = b->bzn = b->bx0;
+            b->depth_bxn = b->depth_by0 = b->depth_byn = b->depth_bz0 = b->depth_bzn = b->depth_bx0;
+            b->m_bxn = b->m_by0 = b->m_byn = b->m_bz0 = b->m_bzn = b->m_bx0;
+            b->sigma_bxn = b->sigma_by0 = b->sigma_byn = b->sigma_bz0 = b->sigma_bzn = b->sigma_bx0;
+            b->a_bxn = b->a_by0 = b->a_byn = b->a_bz0 = b->a_bzn = b->a_bx0;
+            b->kappa_bxn = b->kappa_by0 = b->kappa_byn = b->kappa_bz0 = b->kappa_bzn = b->kappa_bx0;
+        } else if (!strcmp(key, "BOUNDARY_X0")) {
+            SvSetBoundary *b = &set->sb;
+            if (parse_cpml_face(fr, key, set, &b->bx0, &b->depth_bx0, &b->m_bx0, &b->sigma_bx0, &b->a_bx0, &b->kappa_bx0))
+                goto fail;
+        } else if (!strcmp(key, "BOUNDARY_XN")) {
+            SvSetBoundary *b = &set->sb;
+            if (parse_cpml_face(fr, key, set, &b->bxn, &b->depth_bxn, &b->m_bxn, &b->sigma_bxn, &b->a_bxn, &b->kappa_bxn))
+                goto fail;
+        } else if (!strcmp(key, "BOUNDARY_Y0")) {
+            SvSetBoundary *b = &set->sb;
+            if (parse_cpml_face(fr, key, set, &b->by0, &b->depth_by0, &b->m_by0, &b->sigma_by0, &b->a_by0, &b->kappa_by0))
+                goto fail;
+        } else if (!strcmp(key, "BOUNDARY_YN")) {
+            SvSetBoundary *b = &set->sb;
+            if (parse_cpml_face(fr, key, set, &b->byn, &b->depth_byn, &b->m_byn, &b->sigma_byn, &b->a_byn, &b->kappa_byn))
+                goto fail;
+        } else if (!strcmp(key, "BOUNDARY_Z0")) {
+            SvSetBoundary *b = &set->sb;
+            if (parse_cpml_face(fr, key, set, &b->bz0, &b->depth_bz0, &b->m_bz0, &b->sigma_bz0, &b->a_bz0, &b->kappa_bz0))
+                goto fail;
+        } else if (!strcmp(key, "BOUNDARY_ZN")) {
+            SvSetBoundary *b = &set->sb;
+            if (parse_cpml_face(fr, key, set, &b->bzn, &b->depth_bzn, &b->m_bzn, &b->sigma_bzn, &b->a_bzn, &b->kappa_bzn))
+                goto fail;
+        } else if (!strncmp(key, "MBOUNDARY_", 10)) {
+            /* src3d/settings.c:5047-5128 */
+            int pos, *type = NULL, *ppos = NULL;
+            if (get_token(fr, value, sizeof(value)) || get_int(fr, &pos, key))
+                goto fail;
+            if (strcmp(value, "periodic")) {
+                fprintf(stderr, "Error: unsupported medium boundary condition (%s)\n", value);
+                goto fail;
+            }
+            if (!strcmp(key, "MBOUNDARY_X0")) { type = &set->smb.bx0; ppos = &set->smb.bx0pos; }
+            else if (!strcmp(key, "MBOUNDARY_XN")) { type = &set->smb.bxn; ppos = &set->smb.bxnpos; }
+            else if (!strcmp(key, "MBOUNDARY_Y0")) { type = &set->smb.by0; ppos = &set->smb.by0pos; }
+            else if (!strcmp(key, "MBOUNDARY_YN")) { type = &set->smb.byn; ppos = &set->smb.bynpos; }
+            else if (!strcmp(key, "MBOUNDARY_Z0")) { type = &set->smb.bz0; ppos = &set->smb.bz0pos; }
+            else if (!strcmp(key, "MBOUNDARY_ZN")) { type = &set->smb.bzn; ppos = &set->smb.bznpos; }
+            else {
+                fprintf(stderr, "Error: unsupported keyword %s\n", key);
+                goto fail;
+            }
+            *type = SV_BOUNDARY_PERIODIC;
+            *ppos = pos;
+        } else if (!strcmp(key, "NFFF")) {
+            SvSetFarfield *f = &set->sf;
+            if (get_int(fr, &f->box_i0, key) || get_int(fr, &f->box_j0, key) || get_int(fr, &f->box_k0, key)
+                || get_int(fr, &f->box_in, key) || get_int(fr, &f->box_jn, key) || get_int(fr, &f->box_kn, key)
+                || get_int(fr, &f->sumfrom, key) || get_int(fr, &f->sumto, key))
+                goto fail;
+        } else if (!strcmp(key, "NFFF_SKIP")) {
+            SvSetFarfield *f = &set->sf;
+            int *a = NULL;
+            if (get_token(fr, value, sizeof(value)))
+                goto fail;
+            if (!strcmp(value, "i0")) { f->box_boundary_skipi0 = 1; a = &f->skipi0_jmin; }
+            else if (!strcmp(value, "in")) { f->box_boundary_skipin = 1; a = &f->skipin_jmin; }
+            else if (!strcmp(value, "j0")) { f->box_boundary_skipj0 = 1; a = &f->skipj0_imin; }
+            else if (!strcmp(value, "jn")) { f->box_boundary_skipjn = 1; a = &f->skipjn_imin; }
+            else if (!strcmp(value, "k0")) { f->box_boundary_skipk0 = 1; a = &f->skipk0_imin; }
+            else if (!strcmp(value, "kn")) { f->box_boundary_skipkn = 1; a = &f->skipkn_imin; }
+            if (a && (get_int(fr, a, key) || get_int(fr, a + 1, key) || get_int(fr, a + 2, key) || get_int(fr, a + 3, key)))
+                goto fail;
+        } else if (!strcmp(key, "NFFF_RAMAHI_POINT")) {
+            SvSetFarfield *f = &set->sf;
+            int n = f->nrs + 1;
+            f->ri = (int *)realloc(f->ri, n * sizeof(int));
+            f->rj = (int *)realloc(f->rj, n * sizeof(int));
+            f->rk = (int *)realloc(f->rk, n * sizeof(int));
+            f->individual = (int *)realloc(f->individual, n * sizeof(int));
+            f->source_filename = (char (*)[SV_NAME_LEN])realloc(f->source_filename, n * SV_NAME_LEN);
+            if (get_int(fr, &f->ri[n - 1], key) || get_int(fr, &f->rj[n - 1], key) || get_int(fr, &f->rk[n - 1], key))
+                goto fail;
+            if (get_rest_of_line(fr, f->source_filename[n - 1], SV_NAME_LEN))
+                goto fail;
+            f->individual[n - 1] = 0;
+            f->nrs = n;
+        } else if (!strcmp(key, "NFFF_DFT")) {
+            /* running-DFT accumulation on the NFFF box: NFFF_DFT n lambda_1 .. lambda_n file */
+            SvSetFarfield *f = &set->sf;
+            int n, q;
+            if (get_int(fr, &n, key) || n <= 0 || n > 4096)
+                goto fail;
+            f->dft_lambda = (double *)realloc(f->dft_lambda, n * sizeof(double));
+            for (q = 0; q < n; q++)
+                if (get_double(fr, &f->dft_lambda[q], key))
+                    goto fail;
+            f->ndft = n;
+            if (get_rest_of_line(fr, f->dft_filename, SV_NAME_LEN))
+                goto fail;
+        } else {
+            fprintf(stderr, "Error: keyword %s is not supported by the MI355X engine\n", key);
+            goto fail;
+        }
+    }
+
+    /* src3d/settings.c:5823-5828 */
+    if (set->smb.bxnpos == -1) set->smb.bxnpos = set->sp.xres;
+    if (set->smb.bynpos == -1) set->smb.bynpos = set->sp.yres;
+    if (set->smb.bznpos == -1) set->smb.bznpos = set->sp.zres;
+    set->plan.dt = plan_dt(set);
+
+    fclose(fr);
+    return 1;
+
+fail:
+    fclose(fr);
+    return 0;
+}
+
+/* exported for the planner: generated-source samples after the "%g" round trip */
+int
+sv_generated_source_samples(int type, int nsteps, double wavelength, int pulse_width, double dt,
+                            double amplitude, double *out)
+{
+    int i;
+    if (sv_generate_waveform(type, nsteps, wavelength, pulse_width, dt, amplitude, out))
+        return 1;
+    for (i = 0; i < nsteps; i++)
+        out[i] = round_g6(out[i]);
+    return 0;
+}
+
+double
+sv_round_g6(double v)
+{
+    return round_g6(v);
+}

@@ -1,0 +1,158 @@
+"""Python binding of the MI355X engine's C-ABI session (include/gsvit_session.h).
+
+The compute path is the HIP library gsvit_copy_amd/lib/libgsvit_amd.so; there
+is no CPU fallback: if the library is missing or no GPU is present, opening a
+session raises.
+"""
+import contextlib
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libgsvit_amd.so")
+_lib = None
+
+
+def library():
+    """Loads libgsvit_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgsvit_amd.so is not built; run gsvit_copy_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp, fp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)
+        L.gsv_open.restype = vp
+        L.gsv_open.argtypes = [ctypes.c_char_p]
+        L.gsv_run.restype = ctypes.c_int
+        L.gsv_run.argtypes = [vp, ctypes.c_int]
+        L.gsv_set_write_outputs.argtypes = [vp, ctypes.c_int]
+        L.gsv_sync.restype = ctypes.c_int
+        L.gsv_sync.argtypes = [vp]
+        L.gsv_info.argtypes = [vp, ip]
+        L.gsv_get_field.restype = ctypes.c_int
+        L.gsv_get_field.argtypes = [vp, ctypes.c_int, fp]
+        L.gsv_get_points.restype = ctypes.c_int
+        L.gsv_get_points.argtypes = [vp, dp]
+        L.gsv_get_farfield.restype = ctypes.c_int
+        L.gsv_get_farfield.argtypes = [vp, dp]
+        L.gsv_get_dft.restype = ctypes.c_long
+        L.gsv_get_dft.argtypes = [vp, dp]
+        L.gsv_get_maps.argtypes = [vp, ip, ip]
+        L.gsv_set_timing.argtypes = [vp, ctypes.c_int]
+        L.gsv_kernel_times.argtypes = [vp, dp]
+        L.gsv_close.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+@contextlib.contextmanager
+def _chdir(path):
+    old = os.getcwd()
+    os.chdir(path)
+    try:
+        yield
+    finally:
+        os.chdir(old)
+
+
+INFO_KEYS = ["nx", "ny", "nz", "nsteps", "npts", "nrs", "ndata", "nmat", "step",
+             "e_classes", "e_id_bytes", "h_classes", "h_id_bytes", "ade", "ade_poles",
+             "threads_per_block", "z_chunk"]
+
+
+class Session:
+    """One GPU run of a GSvit parameter file (relative paths resolve against its folder)."""
+
+    def __init__(self, parfile, write_outputs=False):
+        self.L = library()
+        self.dir = os.path.dirname(os.path.abspath(parfile))
+        with _chdir(self.dir):
+            self.h = self.L.gsv_open(os.path.basename(parfile).encode())
+        if not self.h:
+            raise RuntimeError("the MI355X engine could not open %s (see stderr)" % parfile)
+        self.L.gsv_set_write_outputs(self.h, 1 if write_outputs else 0)
+
+    def info(self):
+        v = (ctypes.c_int * 17)()
+        self.L.gsv_info(self.h, v)
+        return dict(zip(INFO_KEYS, list(v)))
+
+    def run(self, nsteps=None):
+        inf = self.info()
+        if nsteps is None:
+            nsteps = inf["nsteps"] - inf["step"]
+        with _chdir(self.dir):
+            r = self.L.gsv_run(self.h, int(nsteps))
+        if r < 0:
+            raise RuntimeError("GPU step failed (see stderr)")
+        return r
+
+    def sync(self):
+        if self.L.gsv_sync(self.h):
+            raise RuntimeError("GPU synchronisation failed")
+
+    def field(self, comp):
+        inf = self.info()
+        out = np.empty((inf["nx"], inf["ny"], inf["nz"]), dtype=np.float32)
+        if self.L.gsv_get_field(self.h, int(comp), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))):
+            raise RuntimeError("field download failed")
+        return out
+
+    def fields(self):
+        return np.stack([self.field(c) for c in range(6)])
+
+    def points(self):
+        inf = self.info()
+        out = np.zeros((max(inf["npts"], 1), inf["nsteps"] + 1, 6), dtype=np.float64)
+        if inf["npts"] and self.L.gsv_get_points(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+            raise RuntimeError("trace download failed")
+        return out[: inf["npts"]]
+
+    def farfield(self):
+        inf = self.info()
+        out = np.zeros((max(inf["nrs"], 1), 3, inf["ndata"]), dtype=np.float64)
+        if inf["nrs"] and self.L.gsv_get_farfield(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+            raise RuntimeError("far-field download failed")
+        return out[: inf["nrs"]]
+
+    def dft(self):
+        n = self.L.gsv_get_dft(self.h, None)
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        if n > 0 and self.L.gsv_get_dft(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) < 0:
+            raise RuntimeError("DFT download failed")
+        return out[:n]
+
+    def maps(self):
+        inf = self.info()
+        mat = np.empty((inf["nx"], inf["ny"], inf["nz"]), dtype=np.int32)
+        bnd = np.empty_like(mat)
+        ptr = ctypes.POINTER(ctypes.c_int)
+        self.L.gsv_get_maps(self.h, mat.ctypes.data_as(ptr), bnd.ctypes.data_as(ptr))
+        return mat, bnd
+
+    def set_timing(self, on):
+        self.L.gsv_set_timing(self.h, 1 if on else 0)
+
+    def kernel_times(self):
+        v = (ctypes.c_double * 4)()
+        self.L.gsv_kernel_times(self.h, v)
+        return {"h_ms": v[0], "h_n": int(v[1]), "e_ms": v[2], "e_n": int(v[3])}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gsv_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

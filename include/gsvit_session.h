@@ -1,0 +1,49 @@
+/*
+ * gsvit_session.h -- plain C-ABI session over the drop-in GPU entry points.
+ *
+ * No counterpart in the reference: GSvit drives the entry points from its own
+ * main loop (src3d/svit.c:203-257).  This header is for callers that only
+ * have a parameter file: the Python bindings, the bench and the tests.  All
+ * arguments are plain pointers and sizes; arrays are caller-allocated.
+ */
+#ifndef GSVIT_SESSION_H
+#define GSVIT_SESSION_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* parse + plan + sv_gpu_new + sv_pool_gpu_init + sv_pool_gpu_copyto(SV_SYNC_ALL);
+ * relative file names inside the parameter file resolve against the current
+ * directory, as in the reference.  NULL on error (message on stderr). */
+void *gsv_open(const char *parfile);
+/* run up to nsteps steps of the reference GPU step order; returns the number
+ * of steps done so far, or -1 on a HIP error */
+int gsv_run(void *session, int nsteps);
+/* write the point / far-field logs on output steps, as src3d/svit.c:253-255 */
+void gsv_set_write_outputs(void *session, int on);
+int gsv_sync(void *session);
+/* info[0..16]: nx ny nz nsteps npts nrs ndata nmat step_act,
+ * E classes, E id bytes, H classes, H id bytes, ADE, ADE poles, threads per block, z chunk */
+void gsv_info(void *session, int *info);
+/* one field component (0..5 = Ex..Hz) as float, [i][j][k] order (k fastest) */
+int gsv_get_field(void *session, int comp, float *out);
+/* point traces as [npts][nsteps+1][6] */
+int gsv_get_points(void *session, double *out);
+/* time-domain far-field bins as [nrs][3][ndata] */
+int gsv_get_farfield(void *session, double *out);
+/* running-DFT accumulators; returns the element count (out may be NULL) */
+long gsv_get_dft(void *session, double *out);
+/* planner maps, [i][j][k]: tabulated material index (-1 without a table) and internal-boundary flags */
+void gsv_get_maps(void *session, int *mat, int *bnds);
+/* per-launch HIP-event timing of the H and E volume kernels */
+void gsv_set_timing(void *session, int on);
+/* out[0..3]: H total ms, H launches, E total ms, E launches */
+void gsv_kernel_times(void *session, double *out);
+void gsv_close(void *session);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

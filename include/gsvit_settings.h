@@ -1,0 +1,250 @@
+/*
+ * gsvit_settings.h -- parameter-file settings (SvSet) for the MI355X FDTD engine.
+ *
+ * Field names and enum values follow GSvit's own settings structure so that
+ * code written against the reference reads the same here:
+ *   enums        src3d/settings.h:38-176  (SV_BOUNDARY_*, SV_COMP_*, SV_MAT_*, SV_MATMODE_*)
+ *   SvSet        src3d/settings.h:866-878 (sp, sc, sb, smb, ss, so, sf, sm, plan)
+ *   SvMatProp    src3d/settings.h:921-964
+ * Only the members that the 3-D time-stepping path reads are present.
+ * The parser that fills this structure lives in gsvit_copy_amd/csrc/settings.c.
+ */
+#ifndef GSVIT_SETTINGS_H
+#define GSVIT_SETTINGS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SV_MAX_GPUS 16
+#define SV_NAME_LEN 512
+
+/* physical constants, src3d/constants.h:28-35 */
+#define SV_LIGHT_SPEED 299792458.0
+#define SV_MU_0 1.256637061435917295e-6
+#define SV_EPSILON_0 8.854187817620389850e-12
+#define SV_EV_TO_J 1.60217646e-19
+
+typedef enum {
+    SV_BOUNDARY_NONE = 0,
+    SV_BOUNDARY_PEC = 1,
+    SV_BOUNDARY_LIAO = 2,
+    SV_BOUNDARY_CPML = 3,
+    SV_BOUNDARY_PERIODIC = 4
+} SvBoundaryType;
+
+typedef enum {
+    SV_COMP_EX = 0,
+    SV_COMP_EY = 1,
+    SV_COMP_EZ = 2,
+    SV_COMP_HX = 3,
+    SV_COMP_HY = 4,
+    SV_COMP_HZ = 5,
+    SV_COMP_ALL = 6
+} SvComponent;
+
+typedef enum {
+    SV_MAT_LINEAR = 0,
+    SV_MAT_LINTAB = 1,
+    SV_MAT_DRUDE = 2,
+    SV_MAT_CP3 = 3,
+    SV_MAT_CP = 4,
+    SV_MAT_ADE = 5,
+    SV_MAT_PLRC = 6,
+    SV_MAT_PEC = 10
+} SvMatType;
+
+typedef enum {
+    SV_MATMODE_NONE = 0,
+    SV_MATMODE_ELECTRIC = 1,
+    SV_MATMODE_MAGNETIC = 2,
+    SV_MATMODE_FULL = 3,
+    SV_MATMODE_ERROR = 999
+} SvMatMode;
+
+typedef enum {
+    SV_GPUMODE_NONE = 0,
+    SV_GPUMODE_FULL = 1
+} SvGpuMode;
+
+/* geometry entities of a MEDIUM_VECTOR file, src3d/settings.h (SV_ENTITY_*) */
+typedef enum {
+    SV_ENTITY_SPHERE = 4,
+    SV_ENTITY_VOXEL = 8
+} SvEntityType;
+
+/* material description, src3d/settings.h:921-964 */
+typedef struct {
+    int type;
+    double epsilon;
+    double mu;
+    double sigma;
+    double sigast;
+    double drude_omega_p;
+    double drude_nu;
+    double cp3_a[3];
+    double cp3_phi[3];
+    double cp3_omega[3];
+    double cp3_gamma[3];
+    double ade_a0;
+    double ade_a1;
+    double ade_a2;
+    double ade_bp0[2];
+    double ade_bp1[2];
+    double ade_bp2[2];
+    double ade_bp3[2];
+    double ade_bp4[2];
+    double ade_c0;
+    double ade_c1;
+    double ade_c2;
+    double ade_c3;
+    int pos;
+} SvMatProp;
+
+typedef struct {
+    int xres, yres, zres;
+    double dx, dy, dz;
+} SvSetPool;
+
+typedef struct {
+    int nsteps;
+    int step_act;
+    int verbose;
+    double dtmult;
+    int usegpu;
+    int ugpu[SV_MAX_GPUS];
+    int devicequery;
+    int nthreads;
+    char killer[SV_NAME_LEN];
+} SvSetComp;
+
+typedef struct {
+    int bx0, bxn, by0, byn, bz0, bzn;
+    int depth_bx0, depth_bxn, depth_by0, depth_byn, depth_bz0, depth_bzn;
+    int m_bx0, m_bxn, m_by0, m_byn, m_bz0, m_bzn;
+    double sigma_bx0, sigma_bxn, sigma_by0, sigma_byn, sigma_bz0, sigma_bzn;
+    double a_bx0, a_bxn, a_by0, a_byn, a_bz0, a_bzn;
+    double kappa_bx0, kappa_bxn, kappa_by0, kappa_byn, kappa_bz0, kappa_bzn;
+} SvSetBoundary;
+
+typedef struct {
+    int bx0, bxn, by0, byn, bz0, bzn;
+    int bx0pos, bxnpos, by0pos, bynpos, bz0pos, bznpos;
+} SvSetMediumBoundary;
+
+typedef struct {
+    int point_origin_position_i;
+    int point_origin_position_j;
+    int point_origin_position_k;
+    int source_mode;              /* 0 = file, 1 = sine, 2 = pulse */
+    double source_wl;
+    double source_amplitude;
+    int source_pulsewidth;
+    double point_origin_theta;
+    double point_origin_phi;
+    char source_filename[SV_NAME_LEN];
+} SvSetSourcePoint;
+
+typedef struct {
+    int valid;
+    int box_i0, box_j0, box_k0, box_in, box_jn, box_kn;
+    double ia_theta, ia_phi, ia_psi;
+    int source_mode;
+    double source_wl;
+    double source_amplitude;
+    int source_pulsewidth;
+    char source_filename[SV_NAME_LEN];
+    double layered_epsilon, layered_mu, layered_sigma, layered_sigast;
+    int box_boundary_skipi0, box_boundary_skipj0, box_boundary_skipk0;
+    int box_boundary_skipin, box_boundary_skipjn, box_boundary_skipkn;
+    int box_boundary_skipdepth;
+} SvSetSourceTSF;
+
+typedef struct {
+    int npnts;
+    SvSetSourcePoint *pnts;
+    SvSetSourceTSF tsf;
+    double lambda_min, lambda_center, lambda_max;
+} SvSetSource;
+
+typedef struct {
+    int component;
+    int step;
+    int i, j, k;
+    char filebase[SV_NAME_LEN];
+} SvOutputPar;
+
+typedef struct {
+    int component;          /* 0 = E^2, 1 = sigma*E^2 (as the reference's sum modes) */
+    int step;
+    int box_i0, box_j0, box_k0, box_in, box_jn, box_kn;
+    double layered_epsilon; /* absolute; -1*EPSILON_0 = every voxel */
+    double layered_mu, layered_sigma, layered_sigast;
+    char filename[SV_NAME_LEN];
+} SvOutputSum;
+
+typedef struct {
+    int npnts;
+    SvOutputPar *pnts;
+    int nsums;
+    SvOutputSum *sums;
+} SvSetOutput;
+
+typedef struct {
+    int box_i0, box_j0, box_k0, box_in, box_jn, box_kn;
+    int sumfrom, sumto;
+    int box_boundary_skipi0, box_boundary_skipin, box_boundary_skipj0;
+    int box_boundary_skipjn, box_boundary_skipk0, box_boundary_skipkn;
+    int skipi0_jmin, skipi0_kmin, skipi0_jmax, skipi0_kmax;
+    int skipin_jmin, skipin_kmin, skipin_jmax, skipin_kmax;
+    int skipj0_imin, skipj0_kmin, skipj0_imax, skipj0_kmax;
+    int skipjn_imin, skipjn_kmin, skipjn_imax, skipjn_kmax;
+    int skipk0_imin, skipk0_jmin, skipk0_imax, skipk0_jmax;
+    int skipkn_imin, skipkn_jmin, skipkn_imax, skipkn_jmax;
+    int nrs;
+    int *ri, *rj, *rk;
+    int *individual;
+    char (*source_filename)[SV_NAME_LEN];
+    /* running-DFT accumulation on the same box (no reference counterpart) */
+    int ndft;
+    double *dft_lambda;
+    char dft_filename[SV_NAME_LEN];
+} SvSetFarfield;
+
+typedef struct {
+    char in_voxel_filename[SV_NAME_LEN];
+    char in_vector_filename[SV_NAME_LEN];
+    char in_vtk_data_filename[SV_NAME_LEN];
+    int matmode_check;
+} SvSetMat;
+
+typedef struct {
+    double dt;
+    int matmode;
+    int gpumode;
+} SvSetPlan;
+
+typedef struct {
+    SvSetPool sp;
+    SvSetComp sc;
+    SvSetBoundary sb;
+    SvSetMediumBoundary smb;
+    SvSetSource ss;
+    SvSetOutput so;
+    SvSetFarfield sf;
+    SvSetMat sm;
+    SvSetPlan plan;
+} SvSet;
+
+/* src3d/settings.c:570 (clear_settings) and :3822 (parse_settings).
+ * parse_settings returns 1 on success and 0 on error, like the reference
+ * (src3d/settings.c:5831-5836, checked by src3d/svit.c:186). */
+void clear_settings(SvSet *set);
+int parse_settings(const char *filename, SvSet *set);
+void free_settings(SvSet *set);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSVIT_SETTINGS_H */

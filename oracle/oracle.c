@@ -1,0 +1,1283 @@
+/*
+ * oracle.c -- CPU restatement of GSvit's 3-D time-stepping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the MI355X engine:
+ * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * load it.  The product (gsvit_copy_amd) never links or calls it.
+ *
+ * The numerics follow the reference CPU path line by line in meaning (not in
+ * text), with the field type `real` selectable at build time (ORACLE_REAL =
+ * double for the FP64 oracle, float for the FP32-storage build):
+ *   Yee H update ............ src3d/yee.c:145-265
+ *   Yee E update ............ src3d/yee.c:528-1247 (linear/tabulated with
+ *                             component averaging :1396-1492, ADE branch
+ *                             :914-1108 incl. its inline CPML, PEC :1139-1237)
+ *   dispersive flags ........ src3d/yee.c:453-526
+ *   CPML H / E .............. src3d/boundary.c:282-520, 521-936 (zeroing of the
+ *                             outer planes, z0 quirk using ztbc for Ex :800-801)
+ *   Liao ABC ................ src3d/boundary.c:549-689, 938-1066
+ *   interior periodic copies  src3d/boundary.c:304-346, 889-932
+ *   point sources ........... src3d/source.c:405-437, 1441-1462
+ *   TSF faces ............... src3d/source.c:547-783 (E), 1464-1657 (H)
+ *   1-D auxiliary line ...... src3d/1dpool.c:83-162
+ *   cubic line interpolation  src3d/dline.c:89-112
+ *   time-domain far field ... src3d/farfield.c:174-407
+ *   point traces ............ src3d/output.c:340-352
+ *   step order .............. src3d/svit.c:217-237
+ * The parameter reader and planner are the product's (settings.c, plan.c);
+ * they are pinned independently by the golden traces and the brute-force
+ * material-map tests in tests/.
+ *
+ * Where the reference reads outside an array (sv_dline_get_dval at the last
+ * two samples of the 1-D line), this restatement returns 0 for the missing
+ * sample; the TSF geometry never reaches those samples for boxes inside the pool.
+ *
+ * Parity status of each part is listed in DESIGN.md of the repository root.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include "gsvit_internal.h"
+
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
+typedef ORACLE_REAL real;
+
+#define EPS0 SV_EPSILON_0
+#define MU0 SV_MU_0
+#define CL SV_LIGHT_SPEED
+
+#define NPLANES 24
+enum { EX00, EY00, EZ00, HX00, HY00, HZ00, EXN0, EYN0, EZN0, HXN0, HYN0, HZN0,
+       EX0P, EY0P, EZ0P, HX0P, HY0P, HZ0P, EXNP, EYNP, EZNP, HXNP, HYNP, HZNP };
+
+typedef struct {
+    real *pe1, *pe2;   /* E-side psi: x faces (peyx, pezx), y faces (pexy, pezy), z faces (peyz, pexz) */
+    real *ph1, *ph2;   /* H-side psi: x faces (phzx, phyx), y faces (phxy, phzy), z faces (phyz, phxz) */
+} OFace;
+
+typedef struct {
+    SvSet set;
+    SvPool *mp;
+    int nx, ny, nz;
+    size_t n;
+    real *ex, *ey, *ez, *hx, *hy, *hz;
+    OFace f[6];                       /* x0, xn, y0, yn, z0, zn */
+    real *liao[3];                    /* [24][ny][nz], [24][nx][nz], [24][nx][ny] */
+    int ade;
+    real *exp_, *eyp, *ezp, *dpx, *dpy, *dpz, *dpxp, *dpyp, *dpzp;
+    real *px[2], *py[2], *pz[2], *pxp[2], *pyp[2], *pzp[2];
+    /* 1-D line */
+    int jres;
+    real *je, *jh;
+    real jb[8];
+    double *points;                   /* [npts][nsteps+1][6] */
+    int no_average;                   /* test option: E coefficients without interface averaging */
+} Oracle;
+
+#define IDX(o, i, j, k) (((size_t)(i) * (o)->ny + (j)) * (o)->nz + (k))
+
+/* --------------------------------------------------------------- */
+
+static void *
+zalloc(size_t n, size_t s)
+{
+    void *p = calloc(n ? n : 1, s);
+    if (!p) {
+        fprintf(stderr, "oracle: out of memory\n");
+        exit(3);
+    }
+    return p;
+}
+
+void oracle_close(void *handle);
+
+void *
+oracle_open(const char *parfile)
+{
+    Oracle *o = (Oracle *)zalloc(1, sizeof(Oracle));
+    SvSet *set = &o->set;
+    SvSetBoundary *b;
+    int q;
+
+    clear_settings(set);
+    if (!parse_settings(parfile, set)) {
+        free(o);
+        return NULL;
+    }
+    o->mp = init_and_make_plan(set);
+    if (!o->mp) {
+        free(o);
+        return NULL;
+    }
+    o->nx = set->sp.xres;
+    o->ny = set->sp.yres;
+    o->nz = set->sp.zres;
+    o->n = (size_t)o->nx * o->ny * o->nz;
+    o->ex = (real *)zalloc(o->n, sizeof(real));
+    o->ey = (real *)zalloc(o->n, sizeof(real));
+    o->ez = (real *)zalloc(o->n, sizeof(real));
+    o->hx = (real *)zalloc(o->n, sizeof(real));
+    o->hy = (real *)zalloc(o->n, sizeof(real));
+    o->hz = (real *)zalloc(o->n, sizeof(real));
+
+    b = &set->sb;
+    {
+        int types[6] = { b->bx0, b->bxn, b->by0, b->byn, b->bz0, b->bzn };
+        int depths[6] = { b->depth_bx0, b->depth_bxn, b->depth_by0, b->depth_byn, b->depth_bz0, b->depth_bzn };
+        for (q = 0; q < 6; q++) {
+            size_t cnt;
+            if (types[q] != SV_BOUNDARY_CPML)
+                continue;
+            if (q < 2) cnt = (size_t)depths[q] * o->ny * o->nz;
+            else if (q < 4) cnt = (size_t)o->nx * depths[q] * o->nz;
+            else cnt = (size_t)o->nx * o->ny * depths[q];
+            o->f[q].pe1 = (real *)zalloc(cnt, sizeof(real));
+            o->f[q].pe2 = (real *)zalloc(cnt, sizeof(real));
+            o->f[q].ph1 = (real *)zalloc(cnt, sizeof(real));
+            o->f[q].ph2 = (real *)zalloc(cnt, sizeof(real));
+        }
+        if (b->bx0 == SV_BOUNDARY_LIAO || b->bxn == SV_BOUNDARY_LIAO)
+            o->liao[0] = (real *)zalloc((size_t)NPLANES * o->ny * o->nz, sizeof(real));
+        if (b->by0 == SV_BOUNDARY_LIAO || b->byn == SV_BOUNDARY_LIAO)
+            o->liao[1] = (real *)zalloc((size_t)NPLANES * o->nx * o->nz, sizeof(real));
+        if (b->bz0 == SV_BOUNDARY_LIAO || b->bzn == SV_BOUNDARY_LIAO)
+            o->liao[2] = (real *)zalloc((size_t)NPLANES * o->nx * o->ny, sizeof(real));
+    }
+
+    for (q = 0; q < o->mp->nmat; q++)
+        if (o->mp->mats[q].type == SV_MAT_ADE)
+            o->ade = 1;
+    if (o->ade) {
+        real **arrs[] = { &o->exp_, &o->eyp, &o->ezp, &o->dpx, &o->dpy, &o->dpz, &o->dpxp, &o->dpyp, &o->dpzp,
+                          &o->px[0], &o->px[1], &o->py[0], &o->py[1], &o->pz[0], &o->pz[1],
+                          &o->pxp[0], &o->pxp[1], &o->pyp[0], &o->pyp[1], &o->pzp[0], &o->pzp[1] };
+        for (q = 0; q < (int)(sizeof(arrs) / sizeof(arrs[0])); q++)
+            *arrs[q] = (real *)zalloc(o->n, sizeof(real));
+    }
+
+    if (o->mp->src->tsf) {
+        o->jres = o->mp->src->tsf->jp_res;
+        o->je = (real *)zalloc(o->jres, sizeof(real));
+        o->jh = (real *)zalloc(o->jres, sizeof(real));
+    }
+    if (set->so.npnts)
+        o->points = (double *)zalloc((size_t)set->so.npnts * (set->sc.nsteps + 1) * 6, sizeof(double));
+    return o;
+}
+
+void
+oracle_close(void *handle)
+{
+    Oracle *o = (Oracle *)handle;
+    int q;
+    if (!o)
+        return;
+    free(o->ex); free(o->ey); free(o->ez); free(o->hx); free(o->hy); free(o->hz);
+    for (q = 0; q < 6; q++) {
+        free(o->f[q].pe1); free(o->f[q].pe2); free(o->f[q].ph1); free(o->f[q].ph2);
+    }
+    for (q = 0; q < 3; q++)
+        free(o->liao[q]);
+    if (o->ade) {
+        free(o->exp_); free(o->eyp); free(o->ezp); free(o->dpx); free(o->dpy); free(o->dpz);
+        free(o->dpxp); free(o->dpyp); free(o->dpzp);
+        for (q = 0; q < 2; q++) {
+            free(o->px[q]); free(o->py[q]); free(o->pz[q]); free(o->pxp[q]); free(o->pyp[q]); free(o->pzp[q]);
+        }
+    }
+    free(o->je); free(o->jh); free(o->points);
+    sv_pool_free(o->mp, &o->set);
+    free_settings(&o->set);
+    free(o);
+}
+
+/* --------------------------------------------------------------- */
+/* material helpers                                                  */
+/* --------------------------------------------------------------- */
+
+static void
+get_da_db(const Oracle *o, int i, int j, int k, double *da, double *dbx, double *dby, double *dbz, double *db)
+{
+    const SvPool *mp = o->mp;
+    double dt = mp->d->dt, mu, sg;
+    sv_pool_mu_sigast(mp, i, j, k, &mu, &sg);
+    if (mp->nmat <= 0 && !(o->set.plan.matmode == SV_MATMODE_FULL || o->set.plan.matmode == SV_MATMODE_MAGNETIC)) {
+        /* src3d/yee.c:228-233: plain vacuum branch */
+        *da = 1;
+        *dbx = dt / MU0 / mp->d->dx;
+        *dby = dt / MU0 / mp->d->dy;
+        *dbz = dt / MU0 / mp->d->dz;
+        *db = dt / MU0 / mp->d->dx;
+        return;
+    }
+    *da = (1 - sg * dt / 2 / mu) / (1 + sg * dt / 2 / mu);
+    *dbx = (dt / mu / mp->d->dx) / (1 + sg * dt / 2 / mu);
+    *dby = (dt / mu / mp->d->dy) / (1 + sg * dt / 2 / mu);
+    *dbz = (dt / mu / mp->d->dz) / (1 + sg * dt / 2 / mu);
+    *db = *dbx;
+}
+
+static double
+get_epsilon(const Oracle *o, int i, int j, int k)
+{
+    double e, s;
+    sv_pool_epsilon_sigma(o->mp, i, j, k, &e, &s);
+    return e;
+}
+
+/* E coefficients, src3d/yee.c:1396-1492.  With the no_average test option the
+ * interface averaging is skipped; some of the reference's golden self-test
+ * traces were produced by a reference version that did not average, and this
+ * option lets the tests pin the rest of the path against them. */
+static void
+e_coeffs(const Oracle *o, int i, int j, int k, double *cax, double *cay, double *caz, double *cbx, double *cby, double *cbz)
+{
+    const SvYeeData *d = o->mp->d;
+    double e, s, dt = d->dt;
+    if (!o->no_average) {
+        sv_pool_get_cabs(o->mp, i, j, k, cax, cay, caz, cbx, cby, cbz);
+        return;
+    }
+    sv_pool_epsilon_sigma(o->mp, i, j, k, &e, &s);
+    *cax = *cay = *caz = (1 - s * dt / 2 / e) / (1 + s * dt / 2 / e);
+    *cbx = (dt / e / d->dx) / (1 + s * dt / 2 / e);
+    *cby = (dt / e / d->dy) / (1 + s * dt / 2 / e);
+    *cbz = (dt / e / d->dz) / (1 + s * dt / 2 / e);
+}
+
+/* kappa at overlapping slabs is the maximum, src3d/yee.c:186-207 */
+static void
+kappas(const Oracle *o, int i, int j, int k, int e_side, double *kx, double *ky, double *kz)
+{
+    const SvSetBoundary *sb = &o->set.sb;
+    const SvBoundary *bnd = o->mp->bnd;
+    *kx = *ky = *kz = 1;
+#define KMAX(v, x) if ((x) > (v)) (v) = (x)
+    if (sb->bx0 == SV_BOUNDARY_CPML && i < sb->depth_bx0) KMAX(*kx, e_side ? bnd->x0.kappae[i] : bnd->x0.kappah[i]);
+    if (sb->bxn == SV_BOUNDARY_CPML && i >= o->nx - sb->depth_bxn) {
+        int r = i - (o->nx - sb->depth_bxn);
+        KMAX(*kx, e_side ? bnd->xn.kappae[r] : bnd->xn.kappah[r]);
+    }
+    if (sb->by0 == SV_BOUNDARY_CPML && j < sb->depth_by0) KMAX(*ky, e_side ? bnd->y0.kappae[j] : bnd->y0.kappah[j]);
+    if (sb->byn == SV_BOUNDARY_CPML && j >= o->ny - sb->depth_byn) {
+        int r = j - (o->ny - sb->depth_byn);
+        KMAX(*ky, e_side ? bnd->yn.kappae[r] : bnd->yn.kappah[r]);
+    }
+    if (sb->bz0 == SV_BOUNDARY_CPML && k < sb->depth_bz0) KMAX(*kz, e_side ? bnd->z0.kappae[k] : bnd->z0.kappah[k]);
+    if (sb->bzn == SV_BOUNDARY_CPML && k >= o->nz - sb->depth_bzn) {
+        int r = k - (o->nz - sb->depth_bzn);
+        KMAX(*kz, e_side ? bnd->zn.kappae[r] : bnd->zn.kappah[r]);
+    }
+#undef KMAX
+}
+
+/* --------------------------------------------------------------- */
+/* Yee H step                                                        */
+/* --------------------------------------------------------------- */
+
+static void
+ystep_h(Oracle *o)
+{
+    int i;
+    const int nx = o->nx, ny = o->ny, nz = o->nz;
+    #pragma omp parallel for schedule(static)
+    for (i = 0; i < nx; i++) {
+        int j, k;
+        for (j = 0; j < ny; j++)
+            for (k = 0; k < nz; k++) {
+                double kx, ky, kz, da, dbx, dby, dbz, db;
+                size_t c = IDX(o, i, j, k);
+                real ex = o->ex[c], ey = o->ey[c], ez = o->ez[c];
+                kappas(o, i, j, k, 0, &kx, &ky, &kz);
+                get_da_db(o, i, j, k, &da, &dbx, &dby, &dbz, &db);
+                if (!(j == ny - 1 || k == nz - 1))
+                    o->hx[c] = da * o->hx[c] + dbx * ((o->ey[c + 1] - ey) / kz - (o->ez[c + nz] - ez) / ky);
+                if (!(i == nx - 1 || k == nz - 1))
+                    o->hy[c] = da * o->hy[c] + dby * ((o->ez[c + (size_t)ny * nz] - ez) / kx - (o->ex[c + 1] - ex) / kz);
+                if (!(i == nx - 1 || j == ny - 1))
+                    o->hz[c] = da * o->hz[c] + dbz * ((o->ex[c + nz] - ex) / ky - (o->ey[c + (size_t)ny * nz] - ey) / kx);
+            }
+    }
+}
+
+/* --------------------------------------------------------------- */
+/* Yee E step with ADE and PEC                                       */
+/* --------------------------------------------------------------- */
+
+/* psi index helpers for the six faces */
+static size_t pidx_x(const Oracle *o, int ir, int j, int k) { return ((size_t)ir * o->ny + j) * o->nz + k; }
+static size_t pidx_y(const Oracle *o, int depth, int i, int jr, int k) { return ((size_t)i * depth + jr) * o->nz + k; }
+static size_t pidx_z(const Oracle *o, int depth, int i, int j, int kr) { return ((size_t)i * o->ny + j) * depth + kr; }
+
+static void
+ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int ztbc, double kx, double ky, double kz)
+{
+    const SvMatProp *m = o->mp->mats + matindex;
+    const SvSetBoundary *sb = &o->set.sb;
+    const SvBoundary *bnd = o->mp->bnd;
+    const int nx = o->nx, ny = o->ny, nz = o->nz;
+    const size_t sx = (size_t)ny * nz, sy = nz;
+    size_t c = IDX(o, i, j, k);
+    double dt = o->mp->d->dt, dx = o->mp->d->dx, dy = o->mp->d->dy, dz = o->mp->d->dz;
+    double vexp = o->ex[c], veyp = o->ey[c], vezp = o->ez[c];
+    double cbx, cby, cbz, cb = 0, be, ce, vdpx, vdpy, vdpz, vpx[2], vpy[2], vpz[2];
+    int n;
+
+    if (j > 0 && k > 0 && xtbc == 0)
+        o->ex[c] = m->ade_c0 * ((o->hz[c] - o->hz[c - sy]) / ky - (o->hy[c] - o->hy[c - 1]) / kz) / dx
+                   + m->ade_c1 * (-m->ade_a0 * o->dpxp[c] + (1 - m->ade_a1) * o->dpx[c])
+                   + m->ade_c1 * (-m->ade_bp0[0] * o->pxp[0][c] + (1 - m->ade_bp1[0]) * o->px[0][c])
+                   + m->ade_c1 * (-m->ade_bp0[1] * o->pxp[1][c] + (1 - m->ade_bp1[1]) * o->px[1][c])
+                   + m->ade_c2 * o->exp_[c] + m->ade_c3 * o->ex[c];
+    if (i > 0 && k > 0 && ytbc == 0)
+        o->ey[c] = m->ade_c0 * ((o->hx[c] - o->hx[c - 1]) / kz - (o->hz[c] - o->hz[c - sx]) / kx) / dy
+                   + m->ade_c1 * (-m->ade_a0 * o->dpyp[c] + (1 - m->ade_a1) * o->dpy[c])
+                   + m->ade_c1 * (-m->ade_bp0[0] * o->pyp[0][c] + (1 - m->ade_bp1[0]) * o->py[0][c])
+                   + m->ade_c1 * (-m->ade_bp0[1] * o->pyp[1][c] + (1 - m->ade_bp1[1]) * o->py[1][c])
+                   + m->ade_c2 * o->eyp[c] + m->ade_c3 * o->ey[c];
+    if (i > 0 && j > 0 && ztbc == 0)
+        o->ez[c] = m->ade_c0 * ((o->hy[c] - o->hy[c - sx]) / kx - (o->hx[c] - o->hx[c - sy]) / ky) / dz
+                   + m->ade_c1 * (-m->ade_a0 * o->dpzp[c] + (1 - m->ade_a1) * o->dpz[c])
+                   + m->ade_c1 * (-m->ade_bp0[0] * o->pzp[0][c] + (1 - m->ade_bp1[0]) * o->pz[0][c])
+                   + m->ade_c1 * (-m->ade_bp0[1] * o->pzp[1][c] + (1 - m->ade_bp1[1]) * o->pz[1][c])
+                   + m->ade_c2 * o->ezp[c] + m->ade_c3 * o->ez[c];
+
+    /* inline CPML of the ADE branch, src3d/yee.c:952-1057 (x0 uses cb, which is 0 here) */
+    {
+        double cbm_x = (dt / m->epsilon / dx) / (1 + m->sigma * dt / 2 / m->epsilon);
+        double cbm_y = (dt / m->epsilon / dy) / (1 + m->sigma * dt / 2 / m->epsilon);
+        double cbm_z = (dt / m->epsilon / dz) / (1 + m->sigma * dt / 2 / m->epsilon);
+        if (sb->bx0 == SV_BOUNDARY_CPML && i > 1 && i < sb->depth_bx0) {
+            size_t p = pidx_x(o, i, j, k);
+            be = bnd->x0.be[i]; ce = bnd->x0.ce[i];
+            o->f[0].pe1[p] = be * o->f[0].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dy;
+            o->f[0].pe2[p] = be * o->f[0].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dz;
+            if (!(j == 0 || k == 0) && ytbc == 0) o->ey[c] -= dy * cb * o->f[0].pe1[p];
+            if (!(j == 0 || k == 0) && ztbc == 0) o->ez[c] += dz * cb * o->f[0].pe2[p];
+        }
+        if (sb->by0 == SV_BOUNDARY_CPML && j > 1 && j < sb->depth_by0) {
+            size_t p = pidx_y(o, sb->depth_by0, i, j, k);
+            cbx = cbm_x; cbz = cbm_z;
+            be = bnd->y0.be[j]; ce = bnd->y0.ce[j];
+            o->f[2].pe1[p] = be * o->f[2].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dz;
+            o->f[2].pe2[p] = be * o->f[2].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dx;
+            if (!(i == 0 || k == 0) && xtbc == 0) o->ex[c] += dx * cbx * o->f[2].pe1[p];
+            if (!(i == 0 || k == 0) && ztbc == 0) o->ez[c] -= dz * cbz * o->f[2].pe2[p];
+        }
+        if (sb->bz0 == SV_BOUNDARY_CPML && k > 1 && k < sb->depth_bz0) {
+            size_t p = pidx_z(o, sb->depth_bz0, i, j, k);
+            cbx = cbm_x; cby = cbm_y;
+            be = bnd->z0.be[k]; ce = bnd->z0.ce[k];
+            o->f[4].pe1[p] = be * o->f[4].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dy;
+            o->f[4].pe2[p] = be * o->f[4].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dx;
+            if (!(i == 0 || j == 0) && ytbc == 0) o->ey[c] += dy * cby * o->f[4].pe1[p];
+            if (!(i == 0 || j == 0) && xtbc == 0) o->ex[c] -= dx * cbx * o->f[4].pe2[p];
+        }
+        if (sb->bxn == SV_BOUNDARY_CPML && i >= nx - sb->depth_bxn + 1 && i < nx) {
+            int ir = i - (nx - sb->depth_bxn);
+            size_t p = pidx_x(o, ir, j, k);
+            cby = cbm_y; cbz = cbm_z;
+            be = bnd->xn.be[ir]; ce = bnd->xn.ce[ir];
+            o->f[1].pe1[p] = be * o->f[1].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dy;
+            o->f[1].pe2[p] = be * o->f[1].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dz;
+            if (!(j == 0 || k == 0) && ytbc == 0) o->ey[c] -= dy * cby * o->f[1].pe1[p];
+            if (!(j == 0 || k == 0) && ztbc == 0) o->ez[c] += dz * cbz * o->f[1].pe2[p];
+        }
+        if (sb->byn == SV_BOUNDARY_CPML && j >= ny - sb->depth_byn + 1 && j < ny) {
+            int jr = j - (ny - sb->depth_byn);
+            size_t p = pidx_y(o, sb->depth_byn, i, jr, k);
+            cbx = cbm_x; cbz = cbm_z;
+            be = bnd->yn.be[jr]; ce = bnd->yn.ce[jr];
+            o->f[3].pe1[p] = be * o->f[3].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dx;
+            o->f[3].pe2[p] = be * o->f[3].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dz;
+            if (!(i == 0 || k == 0) && xtbc == 0) o->ex[c] += dx * cbx * o->f[3].pe1[p];
+            if (!(i == 0 || k == 0) && ztbc == 0) o->ez[c] -= dz * cbz * o->f[3].pe2[p];
+        }
+        if (sb->bzn == SV_BOUNDARY_CPML && k >= nz - sb->depth_bzn + 1 && k < nz) {
+            int kr = k - (nz - sb->depth_bzn);
+            size_t p = pidx_z(o, sb->depth_bzn, i, j, kr);
+            cbx = cbm_x; cby = cbm_y;
+            be = bnd->zn.be[kr]; ce = bnd->zn.ce[kr];
+            o->f[5].pe1[p] = be * o->f[5].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dy;
+            o->f[5].pe2[p] = be * o->f[5].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dx;
+            if (!(i == 0 || j == 0) && ytbc == 0) o->ey[c] += dy * cby * o->f[5].pe1[p];
+            if (!(i == 0 || j == 0) && xtbc == 0) o->ex[c] -= dx * cbx * o->f[5].pe2[p];
+        }
+    }
+
+    /* polarisation updates, src3d/yee.c:1060-1106 */
+    vdpx = o->dpx[c]; vdpy = o->dpy[c]; vdpz = o->dpz[c];
+    for (n = 0; n < 2; n++) {
+        vpx[n] = o->px[n][c];
+        vpy[n] = o->py[n][c];
+        vpz[n] = o->pz[n][c];
+    }
+    o->dpx[c] = m->ade_a0 * o->dpxp[c] + m->ade_a1 * o->dpx[c] + m->ade_a2 * (o->exp_[c] + 2 * vexp + o->ex[c]);
+    o->dpy[c] = m->ade_a0 * o->dpyp[c] + m->ade_a1 * o->dpy[c] + m->ade_a2 * (o->eyp[c] + 2 * veyp + o->ey[c]);
+    o->dpz[c] = m->ade_a0 * o->dpzp[c] + m->ade_a1 * o->dpz[c] + m->ade_a2 * (o->ezp[c] + 2 * vezp + o->ez[c]);
+    for (n = 0; n < 2; n++) {
+        o->px[n][c] = m->ade_bp0[n] * o->pxp[n][c] + m->ade_bp1[n] * o->px[n][c] + m->ade_bp2[n] * o->exp_[c]
+                      + m->ade_bp3[n] * vexp + m->ade_bp4[n] * o->ex[c];
+        o->py[n][c] = m->ade_bp0[n] * o->pyp[n][c] + m->ade_bp1[n] * o->py[n][c] + m->ade_bp2[n] * o->eyp[c]
+                      + m->ade_bp3[n] * veyp + m->ade_bp4[n] * o->ey[c];
+        o->pz[n][c] = m->ade_bp0[n] * o->pzp[n][c] + m->ade_bp1[n] * o->pz[n][c] + m->ade_bp2[n] * o->ezp[c]
+                      + m->ade_bp3[n] * vezp + m->ade_bp4[n] * o->ez[c];
+    }
+    o->exp_[c] = vexp; o->eyp[c] = veyp; o->ezp[c] = vezp;
+    o->dpxp[c] = vdpx; o->dpyp[c] = vdpy; o->dpzp[c] = vdpz;
+    for (n = 0; n < 2; n++) {
+        o->pxp[n][c] = vpx[n];
+        o->pyp[n][c] = vpy[n];
+        o->pzp[n][c] = vpz[n];
+    }
+}
+
+static int
+is_pec_t(const Oracle *o, int i, int j, int k)
+{
+    /* mats[...].type == SV_MAT_PEC for any index, including 0 (vacuum) */
+    return o->mp->mats[o->mp->d->mat->data[i][j][k]].type == SV_MAT_PEC;
+}
+
+static void
+ystep_e(Oracle *o)
+{
+    int i;
+    const int nx = o->nx, ny = o->ny, nz = o->nz;
+    const SvPool *mp = o->mp;
+    #pragma omp parallel for schedule(static)
+    for (i = 0; i < nx; i++) {
+        int j, k;
+        const size_t sx = (size_t)ny * nz, sy = nz;
+        for (j = 0; j < ny; j++)
+            for (k = 0; k < nz; k++) {
+                double kx, ky, kz, cax, cay, caz, cbx, cby, cbz;
+                int xtbc, ytbc, ztbc, matindex;
+                size_t c = IDX(o, i, j, k);
+                real hx, hy, hz;
+                kappas(o, i, j, k, 1, &kx, &ky, &kz);
+                hx = o->hx[c]; hy = o->hy[c]; hz = o->hz[c];
+                matindex = sv_pool_get_tbc(mp, i, j, k, &xtbc, &ytbc, &ztbc);
+                if (mp->nmat > 0 && o->ade && xtbc * ytbc * ztbc == 0 && mp->mats[matindex].type == SV_MAT_ADE)
+                    ade_cell(o, i, j, k, matindex, xtbc, ytbc, ztbc, kx, ky, kz);
+                if ((xtbc + ytbc + ztbc) != 0) {
+                    e_coeffs(o, i, j, k, &cax, &cay, &caz, &cbx, &cby, &cbz);
+                    if (j > 0 && k > 0 && xtbc)
+                        o->ex[c] = cax * o->ex[c] + cbx * ((hz - o->hz[c - sy]) / ky - (hy - o->hy[c - 1]) / kz);
+                    if (i > 0 && k > 0 && ytbc)
+                        o->ey[c] = cay * o->ey[c] + cby * ((hx - o->hx[c - 1]) / kz - (hz - o->hz[c - sx]) / kx);
+                    if (i > 0 && j > 0 && ztbc)
+                        o->ez[c] = caz * o->ez[c] + cbz * ((hy - o->hy[c - sx]) / kx - (hx - o->hx[c - sy]) / ky);
+                }
+                /* PEC, src3d/yee.c:1139-1237 (no scattered-field source on this path) */
+                if (matindex >= 0 && i > 0 && j > 0 && k > 0) {
+                    int p0 = is_pec_t(o, i, j, k);
+                    int pxm = is_pec_t(o, i - 1, j, k), pym = is_pec_t(o, i, j - 1, k), pzm = is_pec_t(o, i, j, k - 1);
+                    int pxy = is_pec_t(o, i - 1, j - 1, k), pyz = is_pec_t(o, i, j - 1, k - 1), pxz = is_pec_t(o, i - 1, j, k - 1);
+                    int pxyz = is_pec_t(o, i - 1, j - 1, k - 1);
+                    /* the entry test uses the material chosen by the dispersive flags (src3d/yee.c:1140) */
+                    int pm = mp->mats[matindex].type == SV_MAT_PEC;
+                    if (pm || pxm || pym || pzm || pxy || pyz || pxz || pxyz) {
+                        if (pxm != p0) { o->ey[c] = 0; o->ez[c] = 0; }
+                        if (pym != p0) { o->ex[c] = 0; o->ez[c] = 0; }
+                        if (pzm != p0) { o->ex[c] = 0; o->ey[c] = 0; }
+                        if (pxy && !p0) o->ez[c] = 0;
+                        if (pxz && !p0) o->ey[c] = 0;
+                        if (pyz && !p0) o->ex[c] = 0;
+                    }
+                }
+            }
+    }
+}
+
+/* --------------------------------------------------------------- */
+/* boundaries                                                        */
+/* --------------------------------------------------------------- */
+
+static void
+boundary_copy(Oracle *o)
+{
+    const SvSetBoundary *sb = &o->set.sb;
+    const int nx = o->nx, ny = o->ny, nz = o->nz;
+    real *F[6] = { o->ex, o->ey, o->ez, o->hx, o->hy, o->hz };
+    int a, b, q;
+    if (o->liao[0]) {
+        size_t pl = (size_t)ny * nz;
+        for (a = 0; a < ny; a++)
+            for (b = 0; b < nz; b++)
+                for (q = 0; q < 6; q++) {
+                    size_t s = (size_t)a * nz + b;
+                    if (sb->bx0 == SV_BOUNDARY_LIAO) {
+                        o->liao[0][(EX00 + q) * pl + s] = F[q][IDX(o, 0, a, b)];
+                        o->liao[0][(EX0P + q) * pl + s] = F[q][IDX(o, 1, a, b)];
+                    }
+                    if (sb->bxn == SV_BOUNDARY_LIAO) {
+                        o->liao[0][(EXN0 + q) * pl + s] = F[q][IDX(o, nx - 1, a, b)];
+                        o->liao[0][(EXNP + q) * pl + s] = F[q][IDX(o, nx - 2, a, b)];
+                    }
+                }
+    }
+    if (o->liao[1]) {
+        size_t pl = (size_t)nx * nz;
+        for (a = 0; a < nx; a++)
+            for (b = 0; b < nz; b++)
+                for (q = 0; q < 6; q++) {
+                    size_t s = (size_t)a * nz + b;
+                    if (sb->by0 == SV_BOUNDARY_LIAO) {
+                        o->liao[1][(EX00 + q) * pl + s] = F[q][IDX(o, a, 0, b)];
+                        o->liao[1][(EX0P + q) * pl + s] = F[q][IDX(o, a, 1, b)];
+                    }
+                    if (sb->byn == SV_BOUNDARY_LIAO) {
+                        o->liao[1][(EXN0 + q) * pl + s] = F[q][IDX(o, a, ny - 1, b)];
+                        o->liao[1][(EXNP + q) * pl + s] = F[q][IDX(o, a, ny - 2, b)];
+                    }
+                }
+    }
+    if (o->liao[2]) {
+        size_t pl = (size_t)nx * ny;
+        for (a = 0; a < nx; a++)
+            for (b = 0; b < ny; b++)
+                for (q = 0; q < 6; q++) {
+                    size_t s = (size_t)a * ny + b;
+                    if (sb->bz0 == SV_BOUNDARY_LIAO) {
+                        o->liao[2][(EX00 + q) * pl + s] = F[q][IDX(o, a, b, 0)];
+                        o->liao[2][(EX0P + q) * pl + s] = F[q][IDX(o, a, b, 1)];
+                    }
+                    if (sb->bzn == SV_BOUNDARY_LIAO) {
+                        o->liao[2][(EXN0 + q) * pl + s] = F[q][IDX(o, a, b, nz - 1)];
+                        o->liao[2][(EXNP + q) * pl + s] = F[q][IDX(o, a, b, nz - 2)];
+                    }
+                }
+    }
+}
+
+static void
+boundary_hstep(Oracle *o)
+{
+    const SvSet *set = &o->set;
+    const SvSetBoundary *sb = &set->sb;
+    const SvSetMediumBoundary *smb = &set->smb;
+    const SvBoundary *bnd = o->mp->bnd;
+    const int nx = o->nx, ny = o->ny, nz = o->nz;
+    const double dx = set->sp.dx, dy = set->sp.dy, dz = set->sp.dz;
+    const size_t sx = (size_t)ny * nz, sy = nz;
+    int i, j, k;
+
+    if (smb->bx0 == SV_BOUNDARY_PERIODIC || smb->bxn == SV_BOUNDARY_PERIODIC)
+        for (j = smb->by0pos; j < smb->bynpos; j++)
+            for (k = smb->bz0pos; k < smb->bznpos; k++) {
+                o->hy[IDX(o, smb->bx0pos - 1, j, k)] = o->hy[IDX(o, smb->bxnpos - 1, j, k)];
+                o->hz[IDX(o, smb->bx0pos - 1, j, k)] = o->hz[IDX(o, smb->bxnpos - 1, j, k)];
+            }
+    if (smb->by0 == SV_BOUNDARY_PERIODIC || smb->byn == SV_BOUNDARY_PERIODIC)
+        for (i = smb->bx0pos; i < smb->bxnpos; i++)
+            for (k = smb->bz0pos; k < smb->bznpos; k++) {
+                o->hx[IDX(o, i, smb->by0pos - 1, k)] = o->hx[IDX(o, i, smb->bynpos - 1, k)];
+                o->hz[IDX(o, i, smb->by0pos - 1, k)] = o->hz[IDX(o, i, smb->bynpos - 1, k)];
+            }
+    if (smb->bz0 == SV_BOUNDARY_PERIODIC || smb->bzn == SV_BOUNDARY_PERIODIC)
+        for (i = smb->bx0pos; i < smb->bxnpos; i++)
+            for (j = smb->by0pos; j < smb->bynpos; j++) {
+                o->hx[IDX(o, i, j, smb->bz0pos - 1)] = o->hx[IDX(o, i, j, smb->bznpos - 1)];
+                o->hy[IDX(o, i, j, smb->bz0pos - 1)] = o->hy[IDX(o, i, j, smb->bznpos - 1)];
+            }
+
+    if (sb->bx0 == SV_BOUNDARY_CPML)
+        for (i = 0; i < sb->depth_bx0 - 1; i++) {
+            #pragma omp parallel for private(k) schedule(static)
+            for (j = 1; j < ny - 1; j++)
+                for (k = 1; k < nz - 1; k++) {
+                    double da, dbx, dby, dbz, db, bh = bnd->x0.bh[i], ch = bnd->x0.ch[i];
+                    size_t c = IDX(o, i, j, k), p = pidx_x(o, i, j, k);
+                    get_da_db(o, i, j, k, &da, &dbx, &dby, &dbz, &db);
+                    o->f[0].ph1[p] = bh * o->f[0].ph1[p] + ch * ((o->ey[c + sx] - o->ey[c])) / dx;
+                    o->f[0].ph2[p] = bh * o->f[0].ph2[p] + ch * ((o->ez[c + sx] - o->ez[c])) / dx;
+                    o->hy[c] += dx * db * o->f[0].ph2[p];
+                    o->hz[c] -= dx * db * o->f[0].ph1[p];
+                }
+        }
+    if (sb->by0 == SV_BOUNDARY_CPML)
+        for (j = 0; j < sb->depth_by0 - 1; j++) {
+            #pragma omp parallel for private(k) schedule(static)
+            for (i = 1; i < nx - 1; i++)
+                for (k = 1; k < nz - 1; k++) {
+                    double da, dbx, dby, dbz, db, bh = bnd->y0.bh[j], ch = bnd->y0.ch[j];
+                    size_t c = IDX(o, i, j, k), p = pidx_y(o, sb->depth_by0, i, j, k);
+                    get_da_db(o, i, j, k, &da, &dbx, &dby, &dbz, &db);
+                    o->f[2].ph1[p] = bh * o->f[2].ph1[p] + ch * ((o->ez[c + sy] - o->ez[c])) / dy;
+                    o->f[2].ph2[p] = bh * o->f[2].ph2[p] + ch * ((o->ex[c + sy] - o->ex[c])) / dy;
+                    o->hx[c] -= dy * db * o->f[2].ph1[p];
+                    o->hz[c] += dy * db * o->f[2].ph2[p];
+                }
+        }
+    if (sb->bz0 == SV_BOUNDARY_CPML)
+        for (k = 0; k < sb->depth_bz0 - 1; k++) {
+            #pragma omp parallel for private(j) schedule(static)
+            for (i = 1; i < nx - 1; i++)
+                for (j = 1; j < ny - 1; j++) {
+                    double da, dbx, dby, dbz, db, bh = bnd->z0.bh[k], ch = bnd->z0.ch[k];
+                    size_t c = IDX(o, i, j, k), p = pidx_z(o, sb->depth_bz0, i, j, k);
+                    get_da_db(o, i, j, k, &da, &dbx, &dby, &dbz, &db);
+                    o->f[4].ph1[p] = bh * o->f[4].ph1[p] + ch * ((o->ex[c + 1] - o->ex[c])) / dz;
+                    o->f[4].ph2[p] = bh * o->f[4].ph2[p] + ch * ((o->ey[c + 1] - o->ey[c])) / dz;
+                    o->hy[c] -= dz * db * o->f[4].ph1[p];
+                    o->hx[c] += dz * db * o->f[4].ph2[p];
+                }
+        }
+    if (sb->bxn == SV_BOUNDARY_CPML) {
+        for (j = 1; j < ny - 1; j++)
+            for (k = 1; k < nz - 1; k++) {
+                size_t c = IDX(o, nx - 1, j, k);
+                o->hx[c] = o->hy[c] = o->hz[c] = 0;
+            }
+        for (i = nx - sb->depth_bxn + 1; i < nx - 1; i++) {
+            int ir = i - (nx - sb->depth_bxn);
+            #pragma omp parallel for private(k) schedule(static)
+            for (j = 1; j < ny - 1; j++)
+                for (k = 1; k < nz - 1; k++) {
+                    double da, dbx, dby, dbz, db, bh = bnd->xn.bh[ir], ch = bnd->xn.ch[ir];
+                    size_t c = IDX(o, i, j, k), p = pidx_x(o, ir, j, k);
+                    get_da_db(o, i, j, k, &da, &dbx, &dby, &dbz, &db);
+                    o->f[1].ph1[p] = bh * o->f[1].ph1[p] + ch * ((o->ey[c + sx] - o->ey[c])) / dx;
+                    o->f[1].ph2[p] = bh * o->f[1].ph2[p] + ch * ((o->ez[c + sx] - o->ez[c])) / dx;
+                    o->hy[c] += dx * db * o->f[1].ph2[p];
+                    o->hz[c] -= dx * db * o->f[1].ph1[p];
+                }
+        }
+    }
+    if (sb->byn == SV_BOUNDARY_CPML) {
+        for (i = 1; i < nx - 1; i++)
+            for (k = 1; k < nz - 1; k++) {
+                size_t c = IDX(o, i, ny - 1, k);
+                o->hx[c] = o->hy[c] = o->hz[c] = 0;
+            }
+        for (j = ny - sb->depth_byn + 1; j < ny - 1; j++) {
+            int jr = j - (ny - sb->depth_byn);
+            #pragma omp parallel for private(k) schedule(static)
+            for (i = 1; i < nx - 1; i++)
+                for (k = 1; k < nz - 1; k++) {
+                    double da, dbx, dby, dbz, db, bh = bnd->yn.bh[jr], ch = bnd->yn.ch[jr];
+                    size_t c = IDX(o, i, j, k), p = pidx_y(o, sb->depth_byn, i, jr, k);
+                    get_da_db(o, i, j, k, &da, &dbx, &dby, &dbz, &db);
+                    o->f[3].ph1[p] = bh * o->f[3].ph1[p] + ch * ((o->ez[c + sy] - o->ez[c])) / dy;
+                    o->f[3].ph2[p] = bh * o->f[3].ph2[p] + ch * ((o->ex[c + sy] - o->ex[c])) / dy;
+                    o->hx[c] -= dy * db * o->f[3].ph1[p];
+                    o->hz[c] += dy * db * o->f[3].ph2[p];
+                }
+        }
+    }
+    if (sb->bzn == SV_BOUNDARY_CPML) {
+        for (i = 1; i < nx - 1; i++)
+            for (j = 1; j < ny - 1; j++) {
+                size_t c = IDX(o, i, j, nz - 1);
+                o->hx[c] = o->hy[c] = o->hz[c] = 0;
+            }
+        for (k = nz - sb->depth_bzn + 1; k < nz - 1; k++) {
+            int kr = k - (nz - sb->depth_bzn);
+            #pragma omp parallel for private(j) schedule(static)
+            for (i = 1; i < nx - 1; i++)
+                for (j = 1; j < ny - 1; j++) {
+                    double da, dbx, dby, dbz, db, bh = bnd->zn.bh[kr], ch = bnd->zn.ch[kr];
+                    size_t c = IDX(o, i, j, k), p = pidx_z(o, sb->depth_bzn, i, j, kr);
+                    get_da_db(o, i, j, k, &da, &dbx, &dby, &dbz, &db);
+                    o->f[5].ph1[p] = bh * o->f[5].ph1[p] + ch * ((o->ex[c + 1] - o->ex[c])) / dz;
+                    o->f[5].ph2[p] = bh * o->f[5].ph2[p] + ch * ((o->ey[c + 1] - o->ey[c])) / dz;
+                    o->hy[c] -= dz * db * o->f[5].ph1[p];
+                    o->hx[c] += dz * db * o->f[5].ph2[p];
+                }
+        }
+    }
+}
+
+static double
+liao_ind(const Oracle *o, int i, int j, int k)
+{
+    int mm = o->set.plan.matmode;
+    if (mm == SV_MATMODE_FULL || mm == SV_MATMODE_ELECTRIC)
+        return sqrt(o->mp->d->epsilon->data[i][j][k] / EPS0);
+    return 1;
+}
+
+/* one Liao face: plane index (of the three axes) a = 0 (x), 1 (y), 2 (z); far = 0/1 */
+static void
+liao_face(Oracle *o, int axis, int far)
+{
+    const SvSet *set = &o->set;
+    const double dt = set->plan.dt, d3[3] = { set->sp.dx, set->sp.dy, set->sp.dz };
+    real *F[6] = { o->ex, o->ey, o->ez, o->hx, o->hy, o->hz };
+    int na, nb, a, b, q;
+    int res[3] = { o->nx, o->ny, o->nz };
+    size_t pl;
+    real *st = o->liao[axis];
+    if (axis == 0) { na = o->ny; nb = o->nz; }
+    else if (axis == 1) { na = o->nx; nb = o->nz; }
+    else { na = o->nx; nb = o->ny; }
+    pl = (size_t)na * nb;
+    for (a = 0; a < na; a++)
+        for (b = 0; b < nb; b++) {
+            int pos[3], ipos[3], epos[3];
+            double ind;
+            size_t s = (size_t)a * nb + b;
+            if (axis == 0) { pos[1] = a; pos[2] = b; }
+            else if (axis == 1) { pos[0] = a; pos[2] = b; }
+            else { pos[0] = a; pos[1] = b; }
+            memcpy(ipos, pos, sizeof(pos));
+            memcpy(epos, pos, sizeof(pos));
+            pos[axis] = far ? res[axis] - 1 : 0;
+            ipos[axis] = far ? res[axis] - 2 : 1;
+            /* the epsilon probe sits on the face plane, except zn which uses zres-2 (src3d/boundary.c:657) */
+            epos[axis] = far ? (axis == 2 ? res[axis] - 2 : res[axis] - 1) : 0;
+            ind = liao_ind(o, epos[0], epos[1], epos[2]);
+            for (q = 0; q < 6; q++) {
+                double dd = d3[q % 3];
+                double coef = (dt * CL / ind - dd) / (dt * CL / ind + dd);
+                int p0 = far ? EXNP + q : EX0P + q, p00 = far ? EXN0 + q : EX00 + q;
+                F[q][IDX(o, pos[0], pos[1], pos[2])] = st[p0 * pl + s]
+                    + coef * (F[q][IDX(o, ipos[0], ipos[1], ipos[2])] - st[p00 * pl + s]);
+            }
+        }
+}
+
+static void
+boundary_estep(Oracle *o)
+{
+    const SvSet *set = &o->set;
+    const SvSetBoundary *sb = &set->sb;
+    const SvSetMediumBoundary *smb = &set->smb;
+    const SvBoundary *bnd = o->mp->bnd;
+    const SvPool *mp = o->mp;
+    const int nx = o->nx, ny = o->ny, nz = o->nz;
+    const double dx = set->sp.dx, dy = set->sp.dy, dz = set->sp.dz;
+    const size_t sx = (size_t)ny * nz, sy = nz;
+    int i, j, k;
+
+    /* Liao faces in the reference order: x (x0 then xn per row), y, z */
+    if (sb->bx0 == SV_BOUNDARY_LIAO || sb->bxn == SV_BOUNDARY_LIAO) {
+        if (sb->bx0 == SV_BOUNDARY_LIAO && sb->bxn == SV_BOUNDARY_LIAO) {
+            liao_face(o, 0, 0);
+            liao_face(o, 0, 1);
+        } else if (sb->bx0 == SV_BOUNDARY_LIAO)
+            liao_face(o, 0, 0);
+        else
+            liao_face(o, 0, 1);
+    }
+    if (sb->by0 == SV_BOUNDARY_LIAO) liao_face(o, 1, 0);
+    if (sb->byn == SV_BOUNDARY_LIAO) liao_face(o, 1, 1);
+    if (sb->bz0 == SV_BOUNDARY_LIAO) liao_face(o, 2, 0);
+    if (sb->bzn == SV_BOUNDARY_LIAO) liao_face(o, 2, 1);
+
+#define CPML_E_PRE(FACE)                                                                 \
+    int xtbc, ytbc, ztbc;                                                                \
+    double cax, cay, caz, cbx, cby, cbz;                                                 \
+    size_t c = IDX(o, i, j, k);                                                          \
+    sv_pool_get_tbc(mp, i, j, k, &xtbc, &ytbc, &ztbc);                                   \
+    e_coeffs(o, i, j, k, &cax, &cay, &caz, &cbx, &cby, &cbz);
+
+    if (sb->bx0 == SV_BOUNDARY_CPML) {
+        for (j = 0; j < ny; j++)
+            for (k = 0; k < nz; k++) {
+                size_t c = IDX(o, 0, j, k);
+                o->ey[c] = o->ez[c] = o->ex[c] = 0;
+            }
+        for (i = 1; i < sb->depth_bx0; i++) {
+            #pragma omp parallel for private(k) schedule(static)
+            for (j = 0; j < ny; j++)
+                for (k = 0; k < nz; k++) {
+                    CPML_E_PRE(0)
+                    size_t p = pidx_x(o, i, j, k);
+                    double be = bnd->x0.be[i], ce = bnd->x0.ce[i];
+                    o->f[0].pe1[p] = be * o->f[0].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dx;
+                    o->f[0].pe2[p] = be * o->f[0].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dx;
+                    if (!(j == 0 || k == 0) && ytbc == 1) o->ey[c] -= dx * cby * o->f[0].pe1[p];
+                    if (!(j == 0 || k == 0) && ztbc == 1) o->ez[c] += dx * cbz * o->f[0].pe2[p];
+                }
+        }
+    }
+    if (sb->by0 == SV_BOUNDARY_CPML) {
+        for (i = 0; i < nx; i++)
+            for (k = 0; k < nz; k++) {
+                size_t c = IDX(o, i, 0, k);
+                o->ey[c] = o->ez[c] = o->ex[c] = 0;
+            }
+        for (j = 1; j < sb->depth_by0; j++) {
+            #pragma omp parallel for private(k) schedule(static)
+            for (i = 0; i < nx; i++)
+                for (k = 0; k < nz; k++) {
+                    CPML_E_PRE(2)
+                    size_t p = pidx_y(o, sb->depth_by0, i, j, k);
+                    double be = bnd->y0.be[j], ce = bnd->y0.ce[j];
+                    o->f[2].pe1[p] = be * o->f[2].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dy;
+                    o->f[2].pe2[p] = be * o->f[2].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dy;
+                    if (!(i == 0 || k == 0) && xtbc == 1) o->ex[c] += dy * cbx * o->f[2].pe1[p];
+                    if (!(i == 0 || k == 0) && ztbc == 1) o->ez[c] -= dy * cbz * o->f[2].pe2[p];
+                }
+        }
+    }
+    if (sb->bz0 == SV_BOUNDARY_CPML) {
+        for (i = 0; i < nx; i++)
+            for (j = 0; j < ny; j++) {
+                size_t c = IDX(o, i, j, 0);
+                o->ey[c] = o->ez[c] = o->ex[c] = 0;
+            }
+        for (k = 1; k < sb->depth_bz0; k++) {
+            #pragma omp parallel for private(j) schedule(static)
+            for (i = 0; i < nx; i++)
+                for (j = 0; j < ny; j++) {
+                    CPML_E_PRE(4)
+                    size_t p = pidx_z(o, sb->depth_bz0, i, j, k);
+                    double be = bnd->z0.be[k], ce = bnd->z0.ce[k];
+                    o->f[4].pe1[p] = be * o->f[4].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dz;
+                    o->f[4].pe2[p] = be * o->f[4].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dz;
+                    if (!(i == 0 || j == 0) && ytbc == 1) o->ey[c] += dx * cby * o->f[4].pe1[p];
+                    /* src3d/boundary.c:800-801 tests ztbc for the Ex correction */
+                    if (!(i == 0 || j == 0) && ztbc == 1) o->ex[c] -= dx * cbx * o->f[4].pe2[p];
+                }
+        }
+    }
+    if (sb->bxn == SV_BOUNDARY_CPML) {
+        for (i = nx - sb->depth_bxn + 1; i < nx; i++) {
+            int ir = i - (nx - sb->depth_bxn);
+            #pragma omp parallel for private(k) schedule(static)
+            for (j = 0; j < ny; j++)
+                for (k = 0; k < nz; k++) {
+                    CPML_E_PRE(1)
+                    size_t p = pidx_x(o, ir, j, k);
+                    double be = bnd->xn.be[ir], ce = bnd->xn.ce[ir];
+                    o->f[1].pe1[p] = be * o->f[1].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dx;
+                    o->f[1].pe2[p] = be * o->f[1].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dx;
+                    if (!(j == 0 || k == 0) && ytbc == 1) o->ey[c] -= dx * cby * o->f[1].pe1[p];
+                    if (!(j == 0 || k == 0) && ztbc == 1) o->ez[c] += dx * cbz * o->f[1].pe2[p];
+                }
+        }
+    }
+    if (sb->byn == SV_BOUNDARY_CPML) {
+        for (j = ny - sb->depth_byn + 1; j < ny; j++) {
+            int jr = j - (ny - sb->depth_byn);
+            #pragma omp parallel for private(k) schedule(static)
+            for (i = 0; i < nx; i++)
+                for (k = 0; k < nz; k++) {
+                    CPML_E_PRE(3)
+                    size_t p = pidx_y(o, sb->depth_byn, i, jr, k);
+                    double be = bnd->yn.be[jr], ce = bnd->yn.ce[jr];
+                    o->f[3].pe1[p] = be * o->f[3].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dy;
+                    o->f[3].pe2[p] = be * o->f[3].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dy;
+                    if (!(i == 0 || k == 0) && xtbc == 1) o->ex[c] += dy * cbx * o->f[3].pe1[p];
+                    if (!(i == 0 || k == 0) && ztbc == 1) o->ez[c] -= dy * cbz * o->f[3].pe2[p];
+                }
+        }
+    }
+    if (sb->bzn == SV_BOUNDARY_CPML) {
+        for (k = nz - sb->depth_bzn + 1; k < nz; k++) {
+            int kr = k - (nz - sb->depth_bzn);
+            #pragma omp parallel for private(j) schedule(static)
+            for (i = 0; i < nx; i++)
+                for (j = 0; j < ny; j++) {
+                    CPML_E_PRE(5)
+                    size_t p = pidx_z(o, sb->depth_bzn, i, j, kr);
+                    double be = bnd->zn.be[kr], ce = bnd->zn.ce[kr];
+                    o->f[5].pe1[p] = be * o->f[5].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dz;
+                    o->f[5].pe2[p] = be * o->f[5].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dz;
+                    if (!(i == 0 || j == 0) && ytbc == 1) o->ey[c] += dx * cby * o->f[5].pe1[p];
+                    if (!(i == 0 || j == 0) && xtbc == 1) o->ex[c] -= dx * cbx * o->f[5].pe2[p];
+                }
+        }
+    }
+#undef CPML_E_PRE
+
+    if (smb->bx0 == SV_BOUNDARY_PERIODIC || smb->bxn == SV_BOUNDARY_PERIODIC)
+        for (j = smb->by0pos; j < smb->bynpos; j++)
+            for (k = smb->bz0pos; k < smb->bznpos; k++) {
+                o->ey[IDX(o, smb->bxnpos, j, k)] = o->ey[IDX(o, smb->bx0pos, j, k)];
+                o->ez[IDX(o, smb->bxnpos, j, k)] = o->ez[IDX(o, smb->bx0pos, j, k)];
+            }
+    if (smb->by0 == SV_BOUNDARY_PERIODIC || smb->byn == SV_BOUNDARY_PERIODIC)
+        for (i = smb->bx0pos; i < smb->bxnpos; i++)
+            for (k = smb->bz0pos; k < smb->bznpos; k++) {
+                o->ex[IDX(o, i, smb->bynpos, k)] = o->ex[IDX(o, i, smb->by0pos, k)];
+                o->ez[IDX(o, i, smb->bynpos, k)] = o->ez[IDX(o, i, smb->by0pos, k)];
+            }
+    if (smb->bz0 == SV_BOUNDARY_PERIODIC || smb->bzn == SV_BOUNDARY_PERIODIC)
+        for (i = smb->bx0pos; i < smb->bxnpos; i++)
+            for (j = smb->by0pos; j < smb->bynpos; j++) {
+                o->ex[IDX(o, i, j, smb->bznpos)] = o->ex[IDX(o, i, j, smb->bz0pos)];
+                o->ey[IDX(o, i, j, smb->bznpos)] = o->ey[IDX(o, i, j, smb->bz0pos)];
+            }
+}
+
+/* --------------------------------------------------------------- */
+/* sources                                                           */
+/* --------------------------------------------------------------- */
+
+static int
+point_source_pos(const SvSourcePoint *sp, int step)
+{
+    int q;
+    /* src3d/source.c:407-418 (the direct probe is limited to valid indices here) */
+    if (step < sp->ndata && sp->layered_zpos[step] == step)
+        return step;
+    for (q = 0; q < sp->ndata; q++)
+        if (sp->layered_zpos[q] == step)
+            return q;
+    return -1;
+}
+
+static double
+dline_val(const real *data, int res, double x)
+{
+    int l = (int)floor(x);
+    double a = x - (double)l;
+#define DV(q) ((q) >= 0 && (q) < res ? (double)data[q] : 0.0)
+    if (x >= 1 && x < (res - 1)) {
+        double w1 = a + 1, w2 = a, w3 = 1 - a, w4 = 2 - a;
+        w1 = 4 - 8 * w1 + 5 * w1 * w1 - w1 * w1 * w1;
+        w2 = 1 - 2 * w2 * w2 + w2 * w2 * w2;
+        w3 = 1 - 2 * w3 * w3 + w3 * w3 * w3;
+        w4 = 4 - 8 * w4 + 5 * w4 * w4 - w4 * w4 * w4;
+        return w1 * DV(l - 1) + w2 * DV(l) + w3 * DV(l + 1) + w4 * DV(l + 2);
+    } else if ((x < 1 && x >= 0) || (x >= (res - 1) && x < res))
+        return (1 - a) * DV(l) + a * DV(l + 1);
+#undef DV
+    return 0;
+}
+
+static void
+jp_step(Oracle *o)
+{
+    /* src3d/source.c:1469-1473 with src3d/1dpool.c:83-162 */
+    const SvSourceTSF *t = o->mp->src->tsf;
+    double dt = o->set.plan.dt, dx = t->jp_dx;
+    double multe = dt / (EPS0 * dx), multh = dt / (MU0 * dx);
+    double coef = (dt * CL - dx) / (dt * CL + dx);
+    int res = o->jres, i;
+    double ev = (o->set.sc.step_act < t->ndata) ? t->e[o->set.sc.step_act] : 0.0;
+
+    for (i = 1; i < res; i++)
+        o->je[i] += 1.0 / t->jp_epsilon * multe * (o->jh[i - 1] - o->jh[i]);
+    o->je[0] = o->jb[1] + coef * (o->je[1] - o->jb[0]);
+    o->jh[0] = o->jb[3] + coef * (o->jh[1] - o->jb[2]);
+    o->je[res - 1] = o->jb[5] + coef * (o->je[res - 2] - o->jb[4]);
+    o->jh[res - 1] = o->jb[7] + coef * (o->jh[res - 2] - o->jb[6]);
+    o->jb[0] = o->je[0]; o->jb[1] = o->je[1]; o->jb[2] = o->jh[0]; o->jb[3] = o->jh[1];
+    o->jb[4] = o->je[res - 1]; o->jb[5] = o->je[res - 2]; o->jb[6] = o->jh[res - 1]; o->jb[7] = o->jh[res - 2];
+    o->je[0] = ev;
+    for (i = 0; i < res - 1; i++)
+        o->jh[i] += 1.0 / t->jp_mu * multh * (o->je[i] - o->je[i + 1]);
+}
+
+static void
+source_hstep(Oracle *o)
+{
+    const SvSet *set = &o->set;
+    const SvSource *src = o->mp->src;
+    int n, i, j, k;
+    for (n = 0; n < src->npnts; n++) {
+        const SvSourcePoint *sp = src->sp + n;
+        int pos = point_source_pos(sp, set->sc.step_act);
+        size_t c = IDX(o, sp->i, sp->j, sp->k);
+        if (pos < 0)
+            continue;
+        if (sp->hx[pos] != 0) o->hx[c] = sp->hx[pos];
+        if (sp->hy[pos] != 0) o->hy[c] = sp->hy[pos];
+        if (sp->hz[pos] != 0) o->hz[c] = sp->hz[pos];
+    }
+    if (src->tsf) {
+        const SvSourceTSF *t = src->tsf;
+        double th = t->ia_theta, ph = t->ia_phi, ps = t->ia_psi, hcor = set->plan.dt / set->sp.dx / MU0;
+        const int X = set->sp.xres, Y = set->sp.yres, Z = set->sp.zres;
+        jp_step(o);
+#define VE(ii, jj, kk) dline_val(o->je, o->jres, t->corr * sv_dcomp(ii, jj, kk, th, ph, 0, X, 0, Y, 0, Z))
+        for (j = t->box_j0; j <= t->box_jn; j++)
+            for (k = t->box_k0; k <= t->box_kn; k++) {
+                if (!t->skip_i0 && sv_tsf_face_active(o->mp, set, t->box_i0, j, k)) {
+                    double ve = VE(t->box_i0, j, k);
+                    if (k < t->box_kn) o->hy[IDX(o, t->box_i0 - 1, j, k)] -= hcor * sv_gez(ve, th, ph, ps);
+                    if (j < t->box_jn) o->hz[IDX(o, t->box_i0 - 1, j, k)] += hcor * sv_gey(ve, th, ph, ps);
+                }
+                if (!t->skip_in && sv_tsf_face_active(o->mp, set, t->box_in, j, k)) {
+                    double ve = VE(t->box_in, j, k);
+                    if (k < t->box_kn) o->hy[IDX(o, t->box_in, j, k)] += hcor * sv_gez(ve, th, ph, ps);
+                    if (j < t->box_jn) o->hz[IDX(o, t->box_in, j, k)] -= hcor * sv_gey(ve, th, ph, ps);
+                }
+            }
+        for (i = t->box_i0; i <= t->box_in; i++)
+            for (k = t->box_k0; k <= t->box_kn; k++) {
+                if (!t->skip_j0 && sv_tsf_face_active(o->mp, set, i, t->box_j0, k)) {
+                    double ve = VE(i, t->box_j0, k);
+                    if (k < t->box_kn) o->hx[IDX(o, i, t->box_j0 - 1, k)] += hcor * sv_gez(ve, th, ph, ps);
+                    if (i < t->box_in) o->hz[IDX(o, i, t->box_j0 - 1, k)] -= hcor * sv_gex(ve, th, ph, ps);
+                }
+                if (!t->skip_jn && sv_tsf_face_active(o->mp, set, i, t->box_jn, k)) {
+                    double ve = VE(i, t->box_jn, k);
+                    if (k < t->box_kn) o->hx[IDX(o, i, t->box_jn, k)] -= hcor * sv_gez(ve, th, ph, ps);
+                    if (i < t->box_in) o->hz[IDX(o, i, t->box_jn, k)] += hcor * sv_gex(ve, th, ph, ps);
+                }
+            }
+        for (i = t->box_i0; i <= t->box_in; i++)
+            for (j = t->box_j0; j <= t->box_jn; j++) {
+                if (!t->skip_k0 && sv_tsf_face_active(o->mp, set, i, j, t->box_k0)) {
+                    double ve = VE(i, j, t->box_k0);
+                    int pec = sv_pool_is_pec(o->mp, i, j, t->box_k0);
+                    if (j < t->box_jn && !pec) o->hx[IDX(o, i, j, t->box_k0 - 1)] -= hcor * sv_gey(ve, th, ph, ps);
+                    if (i < t->box_in && !pec) o->hy[IDX(o, i, j, t->box_k0 - 1)] += hcor * sv_gex(ve, th, ph, ps);
+                }
+                if (!t->skip_kn && sv_tsf_face_active(o->mp, set, i, j, t->box_kn)) {
+                    double ve = VE(i, j, t->box_kn);
+                    if (j < t->box_jn) o->hx[IDX(o, i, j, t->box_kn)] += hcor * sv_gey(ve, th, ph, ps);
+                    if (i < t->box_in) o->hy[IDX(o, i, j, t->box_kn)] -= hcor * sv_gex(ve, th, ph, ps);
+                }
+            }
+#undef VE
+    }
+}
+
+static void
+source_estep(Oracle *o)
+{
+    const SvSet *set = &o->set;
+    const SvSource *src = o->mp->src;
+    int n, i, j, k;
+    for (n = 0; n < src->npnts; n++) {
+        const SvSourcePoint *sp = src->sp + n;
+        int pos = point_source_pos(sp, set->sc.step_act);
+        size_t c = IDX(o, sp->i, sp->j, sp->k);
+        if (pos < 0)
+            continue;
+        if (sp->ex[pos] != 0) o->ex[c] = sp->ex[pos];
+        if (sp->ey[pos] != 0) o->ey[c] = sp->ey[pos];
+        if (sp->ez[pos] != 0) o->ez[c] = sp->ez[pos];
+    }
+    if (src->tsf) {
+        const SvSourceTSF *t = src->tsf;
+        double th = t->ia_theta, ph = t->ia_phi, ps = t->ia_psi, ecor = set->plan.dt / set->sp.dx;
+        const int X = set->sp.xres, Y = set->sp.yres, Z = set->sp.zres;
+#define VH(ii, jj, kk) dline_val(o->jh, o->jres, t->corr * sv_dcomp(ii, jj, kk, th, ph, 0, X, 0, Y, 0, Z))
+        for (j = t->box_j0; j <= t->box_jn; j++)
+            for (k = t->box_k0; k <= t->box_kn; k++) {
+                if (!t->skip_i0 && sv_tsf_face_active(o->mp, set, t->box_i0, j, k)) {
+                    double vh = VH(t->box_i0 - 1, j, k), eps = get_epsilon(o, t->box_i0, j, k);
+                    if (j < t->box_jn) o->ey[IDX(o, t->box_i0, j, k)] += ecor / eps * sv_ghz(vh, th, ph, ps);
+                    if (k < t->box_kn) o->ez[IDX(o, t->box_i0, j, k)] -= ecor / eps * sv_ghy(vh, th, ph, ps);
+                }
+                if (!t->skip_in && sv_tsf_face_active(o->mp, set, t->box_in, j, k)) {
+                    double vh = VH(t->box_in, j, k), eps = get_epsilon(o, t->box_in, j, k);
+                    if (j < t->box_jn) o->ey[IDX(o, t->box_in, j, k)] -= ecor / eps * sv_ghz(vh, th, ph, ps);
+                    if (k < t->box_kn) o->ez[IDX(o, t->box_in, j, k)] += ecor / eps * sv_ghy(vh, th, ph, ps);
+                }
+            }
+        for (i = t->box_i0; i <= t->box_in; i++)
+            for (k = t->box_k0; k <= t->box_kn; k++) {
+                if (!t->skip_j0 && sv_tsf_face_active(o->mp, set, i, t->box_j0, k)) {
+                    double vh = VH(i, t->box_j0 - 1, k), eps = get_epsilon(o, i, t->box_j0, k);
+                    if (i < t->box_in) o->ex[IDX(o, i, t->box_j0, k)] -= ecor / eps * sv_ghz(vh, th, ph, ps);
+                    if (k < t->box_kn) o->ez[IDX(o, i, t->box_j0, k)] += ecor / eps * sv_ghx(vh, th, ph, ps);
+                }
+                if (!t->skip_jn && sv_tsf_face_active(o->mp, set, i, t->box_jn, k)) {
+                    double vh = VH(i, t->box_jn, k), eps = get_epsilon(o, i, t->box_jn, k);
+                    if (i < t->box_in) o->ex[IDX(o, i, t->box_jn, k)] += ecor / eps * sv_ghz(vh, th, ph, ps);
+                    if (k < t->box_kn) o->ez[IDX(o, i, t->box_jn, k)] -= ecor / eps * sv_ghx(vh, th, ph, ps);
+                }
+            }
+        for (i = t->box_i0; i <= t->box_in; i++)
+            for (j = t->box_j0; j <= t->box_jn; j++) {
+                if (!t->skip_k0 && sv_tsf_face_active(o->mp, set, i, j, t->box_k0)) {
+                    double vh = VH(i, j, t->box_k0 - 1), eps = get_epsilon(o, i, j, t->box_k0);
+                    int pec = sv_pool_is_pec(o->mp, i, j, t->box_k0);
+                    if (i < t->box_in && !pec) o->ex[IDX(o, i, j, t->box_k0)] += ecor / eps * sv_ghy(vh, th, ph, ps);
+                    if (j < t->box_jn && !pec) o->ey[IDX(o, i, j, t->box_k0)] -= ecor / eps * sv_ghx(vh, th, ph, ps);
+                }
+                if (!t->skip_kn && sv_tsf_face_active(o->mp, set, i, j, t->box_kn)) {
+                    double vh = VH(i, j, t->box_kn), eps = get_epsilon(o, i, j, t->box_kn);
+                    if (i < t->box_in) o->ex[IDX(o, i, j, t->box_kn)] -= ecor / eps * sv_ghy(vh, th, ph, ps);
+                    if (j < t->box_jn) o->ey[IDX(o, i, j, t->box_kn)] += ecor / eps * sv_ghx(vh, th, ph, ps);
+                }
+            }
+#undef VH
+    }
+}
+
+/* --------------------------------------------------------------- */
+/* far field and traces                                              */
+/* --------------------------------------------------------------- */
+
+static void
+ff_accumulate(Oracle *o, SvRPoint *rp, int x, int y, int z, int i, int j, int k, int face, int sgn,
+              int ni, int nj, int nk, double size)
+{
+    /* one surface cell, src3d/farfield.c:217-240 (sgn = -1 on near faces, +1 on far faces) */
+    const SvSet *set = &o->set;
+    double dt = set->plan.dt, dx = set->sp.dx;
+    double r = sv_ff_dist(x, y, z, i, j, k) * dx;
+    double ct = sv_ff_angle(i, j, k, x, y, z, face);
+    int nn = (int)(set->sc.step_act + 1 + r / (CL * dt)) - rp->istart;
+    double a, b, cc;
+    size_t c = IDX(o, i, j, k), cp = IDX(o, i + ni, j + nj, k + nk), cm = IDX(o, i - ni, j - nj, k - nk);
+    real *F[3] = { o->ex, o->ey, o->ez };
+    double *B[3] = { rp->ex, rp->ey, rp->ez };
+    int q;
+    if (nn < 3 || nn >= o->mp->farfield->ndata)
+        return;
+    a = 1.0 / (4.0 * M_PI * r);
+    b = -1.0 / (4.0 * M_PI) * ct / (r * r);
+    cc = -1.0 / (4.0 * M_PI) * ct / (CL * r);
+    for (q = 0; q < 3; q++) {
+        B[q][nn] += -cc / 2.0 / dt * F[q][c] * size;
+        B[q][nn - 1] += (sgn * a * (F[q][cp] - F[q][cm]) / 2.0 / dx + b * F[q][c]) * size;
+        B[q][nn - 2] += cc / 2.0 / dt * F[q][c] * size;
+    }
+}
+
+static void
+farfield(Oracle *o)
+{
+    const SvSetFarfield *sf = &o->set.sf;
+    const SvSet *set = &o->set;
+    int n;
+    if (sf->nrs == 0)
+        return;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (n = 0; n < sf->nrs; n++) {
+        SvRPoint *rp = o->mp->farfield->rpoints + n;
+        int x = sf->ri[n], y = sf->rj[n], z = sf->rk[n], i, j, k;
+        int i0 = sf->box_i0, i1 = sf->box_in, j0 = sf->box_j0, j1 = sf->box_jn, k0 = sf->box_k0, k1 = sf->box_kn;
+        double dx = set->sp.dx, dy = set->sp.dy, dz = set->sp.dz;
+        for (j = j0; j < j1; j++)
+            for (k = k0; k < k1; k++) {
+                if (!(sf->box_boundary_skipi0 && j >= sf->skipi0_jmin && k >= sf->skipi0_kmin && j <= sf->skipi0_jmax && k <= sf->skipi0_kmax))
+                    ff_accumulate(o, rp, x, y, z, i0, j, k, 2, -1, 1, 0, 0, dy * dz);
+                if (!(sf->box_boundary_skipin && j >= sf->skipin_jmin && k >= sf->skipin_kmin && j <= sf->skipin_jmax && k <= sf->skipin_kmax))
+                    ff_accumulate(o, rp, x, y, z, i1, j, k, 3, 1, 1, 0, 0, dy * dz);
+            }
+        for (i = i0; i < i1; i++)
+            for (k = k0; k < k1; k++) {
+                if (!(sf->box_boundary_skipj0 && i >= sf->skipj0_imin && k >= sf->skipj0_kmin && i <= sf->skipj0_imax && k <= sf->skipj0_kmax))
+                    ff_accumulate(o, rp, x, y, z, i, j0, k, 0, -1, 0, 1, 0, dx * dz);
+                if (!(sf->box_boundary_skipjn && i >= sf->skipjn_imin && k >= sf->skipjn_kmin && i <= sf->skipjn_imax && k <= sf->skipjn_kmax))
+                    ff_accumulate(o, rp, x, y, z, i, j1, k, 1, 1, 0, 1, 0, dx * dz);
+            }
+        for (i = i0; i < i1; i++)
+            for (j = j0; j < j1; j++) {
+                if (!(sf->box_boundary_skipk0 && i >= sf->skipk0_imin && j >= sf->skipk0_jmin && i <= sf->skipk0_imax && j <= sf->skipk0_jmax))
+                    ff_accumulate(o, rp, x, y, z, i, j, k0, 4, -1, 0, 0, 1, dx * dy);
+                if (!(sf->box_boundary_skipkn && i >= sf->skipkn_imin && j >= sf->skipkn_jmin && i <= sf->skipkn_imax && j <= sf->skipkn_jmax))
+                    ff_accumulate(o, rp, x, y, z, i, j, k1, 5, 1, 0, 0, 1, dx * dy);
+            }
+    }
+}
+
+static void
+getpoints(Oracle *o)
+{
+    const SvSet *set = &o->set;
+    int n;
+    for (n = 0; n < set->so.npnts; n++) {
+        const SvOutputPar *p = set->so.pnts + n;
+        size_t c = IDX(o, p->i, p->j, p->k);
+        double *v = o->points + ((size_t)n * (set->sc.nsteps + 1) + set->sc.step_act) * 6;
+        v[0] = o->ex[c]; v[1] = o->ey[c]; v[2] = o->ez[c];
+        v[3] = o->hx[c]; v[4] = o->hy[c]; v[5] = o->hz[c];
+    }
+}
+
+/* --------------------------------------------------------------- */
+/* C-ABI for the tests                                               */
+/* --------------------------------------------------------------- */
+
+/* run up to nsteps steps in the CPU order of src3d/svit.c:217-237 */
+int
+oracle_step(void *handle, int nsteps)
+{
+    Oracle *o = (Oracle *)handle;
+    int s;
+    for (s = 0; s < nsteps && o->set.sc.step_act < o->set.sc.nsteps; s++) {
+        boundary_copy(o);
+        ystep_h(o);
+        source_hstep(o);
+        boundary_hstep(o);
+        ystep_e(o);
+        boundary_estep(o);
+        source_estep(o);
+        farfield(o);
+        getpoints(o);
+        o->set.sc.step_act++;
+    }
+    return o->set.sc.step_act;
+}
+
+/* info[0..8] = nx, ny, nz, nsteps, npts, nrs, ndata, nmat, step_act */
+void
+oracle_info(void *handle, int *info)
+{
+    Oracle *o = (Oracle *)handle;
+    info[0] = o->nx; info[1] = o->ny; info[2] = o->nz;
+    info[3] = o->set.sc.nsteps;
+    info[4] = o->set.so.npnts;
+    info[5] = o->set.sf.nrs;
+    info[6] = o->mp->farfield->ndata;
+    info[7] = o->mp->nmat;
+    info[8] = o->set.sc.step_act;
+}
+
+/* comp 0..5 = Ex..Hz, copied as double in [i][j][k] order */
+int
+oracle_get_field(void *handle, int comp, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    real *F[6] = { o->ex, o->ey, o->ez, o->hx, o->hy, o->hz };
+    size_t q;
+    if (comp < 0 || comp > 5)
+        return 1;
+    for (q = 0; q < o->n; q++)
+        out[q] = F[comp][q];
+    return 0;
+}
+
+/* point traces as [npts][nsteps+1][6] */
+void
+oracle_get_points(void *handle, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    memcpy(out, o->points, (size_t)o->set.so.npnts * (o->set.sc.nsteps + 1) * 6 * sizeof(double));
+}
+
+/* time-domain far-field bins as [nrs][3][ndata] */
+void
+oracle_get_farfield(void *handle, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    int n, nd = o->mp->farfield->ndata;
+    for (n = 0; n < o->set.sf.nrs; n++)
+        memcpy(out + (size_t)n * 3 * nd, o->mp->farfield->rpoints[n].ex, 3 * (size_t)nd * sizeof(double));
+}
+
+/* tabulated material map ([i][j][k], -1 everywhere without a table) and boundary map */
+void
+oracle_get_maps(void *handle, int *mat, int *bnds)
+{
+    Oracle *o = (Oracle *)handle;
+    size_t q;
+    for (q = 0; q < o->n; q++) {
+        mat[q] = o->mp->d->mat ? o->mp->d->mat->block[q] : -1;
+        bnds[q] = o->mp->d->bnds->block[q];
+    }
+}
+
+/* test options: "no_average" (0/1) */
+int
+oracle_set_option(void *handle, const char *name, int value)
+{
+    Oracle *o = (Oracle *)handle;
+    if (!strcmp(name, "no_average")) {
+        o->no_average = value;
+        return 0;
+    }
+    return 1;
+}
+
+int
+oracle_real_size(void)
+{
+    return (int)sizeof(real);
+}

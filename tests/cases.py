@@ -1,0 +1,238 @@
+"""Seeded parameter-file workloads for parity tests and the bench.
+
+Every function writes a GSvit parameter file plus its input files into a
+folder and returns the parameter-file path.  The workloads follow the
+configurations the engine is measured on:
+  config1  plain 3-D FDTD, 40 dielectric spheres, CPML, Ez dipole (128^3)
+  config2  Drude rough-surface scattering, TSF plane wave at 45 degrees,
+           running-DFT near-to-far-field box (512^3)
+and small feature cases (CPML + ADE, TSF oblique, Liao + PEC, periodic, far field).
+"""
+import math
+import os
+
+import numpy as np
+
+C0 = 299792458.0
+ETA0 = 376.73
+
+
+def _dt(dx, dy, dz, dtmult):
+    return 1.0 / C0 / math.sqrt(1.0 / dx ** 2 + 1.0 / dy ** 2 + 1.0 / dz ** 2) * dtmult
+
+
+def gaussian_sine(nsteps, dt, lam, width_periods, delay_widths=3.0, amplitude=1.0):
+    """Gaussian-modulated sine: FWHM of the envelope = width_periods periods."""
+    period = lam / C0
+    sigma = width_periods * period / 2.3548
+    t0 = delay_widths * width_periods * period
+    t = np.arange(nsteps) * dt
+    return amplitude * np.sin(2 * math.pi * t / period) * np.exp(-0.5 * ((t - t0) / sigma) ** 2)
+
+
+def write_point_source(path, vals, comp=2):
+    with open(path, "w") as f:
+        f.write("%d\n" % len(vals))
+        for n, v in enumerate(vals):
+            row = [0.0] * 6
+            row[comp] = v
+            f.write("%d %s\n" % (n, " ".join("%.17g" % x for x in row)))
+
+
+def write_tsf_source(path, vals):
+    with open(path, "w") as f:
+        f.write("%d\n" % len(vals))
+        for n, v in enumerate(vals):
+            f.write("%d %.17g\n" % (n, v))
+
+
+def place_spheres(rng, n, lo, hi, rmin, rmax):
+    """Non-overlapping spheres with centres uniform in [lo+r, hi-r]."""
+    out = []
+    tries = 0
+    while len(out) < n:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError("could not place %d spheres" % n)
+        r = rng.uniform(rmin, rmax)
+        c = rng.uniform(lo + r, hi - r, size=3)
+        if all(np.linalg.norm(c - np.array(o[:3])) > r + o[3] + 1 for o in out):
+            out.append((c[0], c[1], c[2], r))
+    return out
+
+
+def config1(folder, n=128, nsteps=1000, nspheres=40, seed=1, npoints=16):
+    """Plain 3-D FDTD: 10 nm cells, Courant 0.5, CPML 10 (m=3,
+    sigma_max = 0.8 (m+1)/(eta0 dx), kappa 1, alpha 0), 40 LINTAB spheres of
+    radius U[5,15] cells and eps U[1,4], Ez dipole at the centre driven by a
+    Gaussian-modulated sine (lambda0 = 500 nm, width 8 periods)."""
+    os.makedirs(folder, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    dx = 10e-9
+    depth = 10
+    sigma = 0.8 * 4 / (ETA0 * dx)
+    spheres = place_spheres(rng, nspheres, depth, n - depth, 5, 15) if nspheres else []
+    eps = rng.uniform(1, 4, size=len(spheres))
+    with open(os.path.join(folder, "spheres.txt"), "w") as f:
+        for (x, y, z, r), e in zip(spheres, eps):
+            f.write("4 %.6f %.6f %.6f %.6f 1 %.6f 1 0 0\n" % (x, y, z, r, e))
+    dt = _dt(dx, dx, dx, 0.5)
+    write_point_source(os.path.join(folder, "dipole.txt"), gaussian_sine(nsteps, dt, 500e-9, 8), comp=2)
+    pts = rng.integers(depth + 2, n - depth - 2, size=(npoints, 3))
+    c = n // 2
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (n, n, n, dx, dx, dx), "COMP\n%d\n\n" % nsteps, "DT_MULT\n0.5\n\n",
+             "BOUNDARY_ALL\ncpml %d 3 %.10g 0 1\n\n" % (depth, sigma),
+             "SOURCE_POINT\n%d %d %d 0 dipole.txt\n\n" % (c, c, c)]
+    if spheres:
+        lines.append("MEDIUM_VECTOR\nspheres.txt\n\n")
+    for q, p in enumerate(pts):
+        lines.append("OUT_POINT\nAll 1 %d %d %d point%02d\n\n" % (p[0], p[1], p[2], q))
+    path = os.path.join(folder, "config1.par")
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path
+
+
+def rough_height(n, rms, corr, seed):
+    """Gaussian-correlated random height field (cells): white noise filtered with a
+    Gaussian of correlation length `corr` cells, scaled to the requested rms."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((n, n))
+    k = np.fft.fftfreq(n)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    filt = np.exp(-(math.pi * corr) ** 2 * (kx ** 2 + ky ** 2) / 2)
+    h = np.real(np.fft.ifft2(np.fft.fft2(w) * filt))
+    h -= h.mean()
+    return h * (rms / h.std())
+
+
+def write_vtk_ids(path, ids):
+    """ASCII STRUCTURED_POINTS integer map, x fastest (src3d/plan.c:3877-3960)."""
+    nx, ny, nz = ids.shape
+    flat = np.ascontiguousarray(ids.transpose(2, 1, 0)).reshape(-1)
+    if flat.max() > 9 or flat.min() < 0:
+        raise ValueError("write_vtk_ids handles single-digit ids")
+    buf = np.empty(2 * flat.size, dtype=np.uint8)
+    buf[0::2] = flat.astype(np.uint8) + ord("0")
+    buf[1::2] = ord("\n")
+    with open(path, "wb") as f:
+        f.write(b"# vtk DataFile Version 2.0\nmaterial ids\nASCII\nDATASET STRUCTURED_POINTS\n")
+        f.write(("DIMENSIONS %d %d %d\nORIGIN 0 0 0\nSPACING 1 1 1\nPOINT_DATA %d\n" % (nx, ny, nz, flat.size)).encode())
+        f.write(b"SCALARS mat int 1\nLOOKUP_TABLE default\n")
+        f.write(buf.tobytes())
+
+
+DRUDE_SILVER = "5 1 0 1.37e16 2.7e13 0 0 0 0 0 0 0 0"
+
+
+def config2(folder, n=512, nsteps=4000, dx=5e-9, depth=16, seed=2, rms_nm=20.0, corr_nm=100.0,
+            theta_deg=45.0, lam_center=600e-9, dft_lambdas=(500e-9, 600e-9, 700e-9), tsf_margin=4,
+            nfff=True):
+    """Rough-surface scattering: lower half a Drude metal
+    (eps_inf 1, omega_p 1.37e16 rad/s, gamma 2.7e13 1/s, ADE) below a
+    Gaussian-correlated interface (rms 20 nm, correlation 100 nm); CPML 16;
+    a p-polarised TSF plane wave at 45 degrees travelling down onto the metal
+    with a Gaussian pulse centred at 600 nm (bottom TSF face skipped); running
+    DFT of the tangential fields on a closed box at 500/600/700 nm."""
+    os.makedirs(folder, exist_ok=True)
+    h = rough_height(n, rms_nm * 1e-9 / dx, corr_nm * 1e-9 / dx, seed)
+    zc = n / 2.0 + h
+    k = np.arange(n) + 0.5
+    ids = (k[None, None, :] < zc[:, :, None]).astype(np.int32)
+    write_vtk_ids(os.path.join(folder, "surface.vtk"), ids)
+    with open(os.path.join(folder, "metal.txt"), "w") as f:
+        f.write("8 0 0 0 1 1 1 %s\n" % DRUDE_SILVER)
+    dt = _dt(dx, dx, dx, 1.0)
+    write_tsf_source(os.path.join(folder, "pulse.txt"), gaussian_sine(nsteps, dt, lam_center, 4, delay_widths=2.0))
+    b0 = depth + tsf_margin
+    b1 = n - depth - tsf_margin - 1
+    theta = math.pi - math.radians(theta_deg)   # travelling towards -z
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (n, n, n, dx, dx, dx), "COMP\n%d\n\n" % nsteps,
+             "MATMODE_CHECK\n1\n\n",
+             "BOUNDARY_ALL\ncpml %d 3 -1 0 1\n\n" % depth,
+             "MEDIUM_VECTOR\nmetal.txt\n\n", "MEDIUM_VTK_DATA\nsurface.vtk\n\n",
+             "SOURCE_TSF\n%d %d %d %d %d %d %.17g 0 %.17g 0 pulse.txt\n\n" % (b0, b0, b0, b1, b1, b1, theta, math.pi / 2),
+             "TSF_SKIP\nk0\n\n"]
+    if nfff:
+        f0 = b0 + 6
+        f1 = b1 - 6
+        zlow = int(n / 2 + 4 * rms_nm * 1e-9 / dx) + 8
+        lines.append("NFFF\n%d %d %d %d %d %d 0 0\n\n" % (f0, f0, zlow, f1, f1, f1))
+        lines.append("NFFF_DFT\n%d %s dft.txt\n\n" % (len(dft_lambdas), " ".join("%.6g" % l for l in dft_lambdas)))
+    path = os.path.join(folder, "config2.par")
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path
+
+
+def small_case(folder, kind, n=48, nsteps=160, seed=7):
+    """Small feature cases for GPU/oracle parity."""
+    os.makedirs(folder, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    dx = 10e-9
+    dt = _dt(dx, dx, dx, 0.5)
+    c = n // 2
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (n, n, n, dx, dx, dx), "COMP\n%d\n\n" % nsteps, "DT_MULT\n0.5\n\n"]
+    pts = [(c + 7, c - 5, c + 3), (6, c, c), (c, n - 7, 9), (c + 2, c + 2, n - 8)]
+    for q, p in enumerate(pts):
+        lines.append("OUT_POINT\nAll 1 %d %d %d pt%d\n\n" % (p[0], p[1], p[2], q))
+    pulse = gaussian_sine(nsteps, dt, 150e-9, 1.5, delay_widths=1.2)
+    write_point_source(os.path.join(folder, "src.txt"), pulse, comp=2)
+    tsf_pulse = pulse
+    write_tsf_source(os.path.join(folder, "tsf.txt"), tsf_pulse)
+    mats = []
+    if kind == "vacuum_none":
+        lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c))
+    elif kind == "spheres_cpml":
+        sph = place_spheres(rng, 4, 8, n - 8, 3, 6)
+        for (x, y, z, r) in sph:
+            mats.append("4 %.5f %.5f %.5f %.5f 1 %.4f 1 %.4g 0" % (x, y, z, r, rng.uniform(1.5, 4), rng.uniform(0, 2e3)))
+        mats.append("8 %d %d %d %d %d %d 0 2.5 1 0 0" % (4, 4, 4, 12, 12, 10))
+        lines.append("BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\n")
+        lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c))
+    elif kind == "magnetic_liao":
+        mats.append("8 %d %d %d %d %d %d 0 1 2 0 300" % (c + 4, 10, 10, n - 10, n - 10, n - 10))
+        mats.append("4 %d %d %d 5 1 1 3 0 0" % (c - 8, c - 8, c - 8))
+        lines.append("BOUNDARY_ALL\nliao\n\n")
+        lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c))
+    elif kind == "ade_cpml":
+        mats.append("8 %d %d %d %d %d %d %s" % (0, 0, 0, n, n, c - 4, DRUDE_SILVER))
+        mats.append("4 %d %d %d 4 5 2 0 1.2e16 1e14 0.8 -1.2 4e15 4e14 1.5 -1.1 4.2e15 2.4e15" % (c + 8, c + 6, c + 2))
+        lines.append("BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\n")
+        lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c + 6))
+    elif kind == "tsf_pec_liao":
+        mats.append("4 %d %d %d 7 10" % (c, c, c))
+        lines.append("BOUNDARY_ALL\nliao\n\n")
+        lines.append("SOURCE_TSF\n8 8 8 %d %d %d 2.356194490192345 0.6 0.4 0 tsf.txt\n\n" % (n - 9, n - 9, n - 9))
+    elif kind == "tsf_ade_cpml":
+        mats.append("8 %d %d %d %d %d %d %s" % (0, 0, 0, n, n, c - 6, DRUDE_SILVER))
+        lines.append("BOUNDARY_ALL\ncpml 8 3 -1 0 1\n\n")
+        lines.append("SOURCE_TSF\n10 10 10 %d %d %d 2.356194490192345 0.3 1.5707963267948966 0 tsf.txt\n\n" % (n - 11, n - 11, n - 11))
+        lines.append("TSF_SKIP\nk0\n\n")
+    elif kind == "periodic_liao":
+        mats.append("4 %d %d %d 5 1 2.2 1 0 0" % (c, c + 3, c))
+        lines.append("BOUNDARY_ALL\nliao\n\n")
+        lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c))
+        lines.append("MBOUNDARY_X0\nperiodic 8\n\nMBOUNDARY_XN\nperiodic %d\n\n" % (n - 8))
+        lines.append("MBOUNDARY_Y0\nperiodic 8\n\nMBOUNDARY_YN\nperiodic %d\n\n" % (n - 8))
+    elif kind == "farfield_cpml":
+        lines.append("BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\n")
+        lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c))
+        lines.append("NFFF\n12 12 12 %d %d %d 0 0\n\n" % (n - 12, n - 12, n - 12))
+        lines.append("NFFF_RAMAHI_POINT\n%d %d %d ff0\n\n" % (n - 10, n - 10, n - 10))
+        lines.append("NFFF_RAMAHI_POINT\n%d %d %d ff1\n\n" % (2 * n, c, c))
+        lines.append("NFFF_DFT\n2 300e-9 450e-9 dft.txt\n\n")
+    else:
+        raise ValueError(kind)
+    if mats:
+        with open(os.path.join(folder, "mats.txt"), "w") as f:
+            f.write("\n".join(mats) + "\n")
+        lines.append("MEDIUM_VECTOR\nmats.txt\n\n")
+    path = os.path.join(folder, kind + ".par")
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path
+
+
+SMALL_KINDS = ["vacuum_none", "spheres_cpml", "magnetic_liao", "ade_cpml", "tsf_pec_liao", "tsf_ade_cpml",
+               "periodic_liao", "farfield_cpml"]

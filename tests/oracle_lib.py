@@ -1,0 +1,124 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle_{f64,f32}.so).
+
+Test infrastructure only: the oracle is the checker for the MI355X engine and
+is never used by the product path.
+"""
+import ctypes
+import os
+import contextlib
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(REPO, "oracle", "_build")
+
+_libs = {}
+
+
+def lib(precision="f64"):
+    if precision not in _libs:
+        path = os.path.join(BUILD, "liboracle_%s.so" % precision)
+        if not os.path.exists(path):
+            raise FileNotFoundError("oracle library %s is not built (run __graft_entry__.build())" % path)
+        L = ctypes.CDLL(path)
+        L.oracle_open.restype = ctypes.c_void_p
+        L.oracle_open.argtypes = [ctypes.c_char_p]
+        L.oracle_close.argtypes = [ctypes.c_void_p]
+        L.oracle_step.restype = ctypes.c_int
+        L.oracle_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_get_field.restype = ctypes.c_int
+        L.oracle_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_get_points.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_get_farfield.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_get_maps.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.oracle_real_size.restype = ctypes.c_int
+        L.oracle_set_option.restype = ctypes.c_int
+        L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+        _libs[precision] = L
+    return _libs[precision]
+
+
+@contextlib.contextmanager
+def chdir(path):
+    old = os.getcwd()
+    os.chdir(path)
+    try:
+        yield
+    finally:
+        os.chdir(old)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """One CPU run of a parameter file; relative paths resolve against the file's folder."""
+
+    def __init__(self, parfile, precision="f64"):
+        self.L = lib(precision)
+        self.dir = os.path.dirname(os.path.abspath(parfile))
+        with chdir(self.dir):
+            self.h = self.L.oracle_open(os.path.basename(parfile).encode())
+        if not self.h:
+            raise RuntimeError("oracle could not open %s" % parfile)
+
+    def set_option(self, name, value):
+        if self.L.oracle_set_option(self.h, name.encode(), int(value)):
+            raise ValueError("unknown oracle option %s" % name)
+
+    def info(self):
+        v = (ctypes.c_int * 9)()
+        self.L.oracle_info(self.h, v)
+        keys = ["nx", "ny", "nz", "nsteps", "npts", "nrs", "ndata", "nmat", "step"]
+        return dict(zip(keys, list(v)))
+
+    def step(self, n=None):
+        inf = self.info()
+        if n is None:
+            n = inf["nsteps"] - inf["step"]
+        with chdir(self.dir):
+            return self.L.oracle_step(self.h, n)
+
+    def field(self, comp):
+        inf = self.info()
+        out = np.empty((inf["nx"], inf["ny"], inf["nz"]), dtype=np.float64)
+        self.L.oracle_get_field(self.h, comp, _dptr(out))
+        return out
+
+    def fields(self):
+        return np.stack([self.field(c) for c in range(6)])
+
+    def points(self):
+        inf = self.info()
+        out = np.zeros((max(inf["npts"], 1), inf["nsteps"] + 1, 6), dtype=np.float64)
+        if inf["npts"]:
+            self.L.oracle_get_points(self.h, _dptr(out))
+        return out[: inf["npts"]]
+
+    def farfield(self):
+        inf = self.info()
+        out = np.zeros((max(inf["nrs"], 1), 3, inf["ndata"]), dtype=np.float64)
+        if inf["nrs"]:
+            self.L.oracle_get_farfield(self.h, _dptr(out))
+        return out[: inf["nrs"]]
+
+    def maps(self):
+        inf = self.info()
+        mat = np.empty((inf["nx"], inf["ny"], inf["nz"]), dtype=np.int32)
+        bnd = np.empty_like(mat)
+        self.L.oracle_get_maps(self.h, mat.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                               bnd.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return mat, bnd
+
+    def close(self):
+        if self.h:
+            self.L.oracle_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,89 @@
+"""GPU parity: the HIP engine (through the C-ABI session) against the CPU oracle.
+
+Bar (floating point): every field component and every point trace agrees with
+the FP64 oracle to relative L2 <= 1e-5 after the full step count; the error of
+the FP32-storage oracle against the FP64 oracle is printed beside it.
+"""
+import numpy as np
+import pytest
+
+import cases
+from oracle_lib import Oracle
+from gsvit_copy_amd import Session
+from selftest_recipe import write_selftest, read_golden
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def run_both(par, oracle_options=None):
+    o = Oracle(par)
+    for k, v in (oracle_options or {}).items():
+        o.set_option(k, v)
+    o.step()
+    s = Session(par)
+    s.run()
+    return o, s
+
+
+@pytest.mark.parametrize("kind", cases.SMALL_KINDS)
+def test_small_case_fields_and_traces(kind, tmp_path):
+    par = cases.small_case(str(tmp_path / kind), kind)
+    print("case", kind, flush=True)
+    o, s = run_both(par)
+    ref, got = o.fields(), s.fields()
+    errs = [rel_l2(got[c], ref[c]) for c in range(6)]
+    o32 = Oracle(par, "f32")
+    o32.step()
+    f32 = [rel_l2(o32.field(c), ref[c]) for c in range(6)]
+    print(kind, "GPU vs FP64 per component", errs, "FP32 oracle vs FP64", f32)
+    assert max(errs) <= TOL, errs
+    rp, gp = o.points(), s.points()
+    for n in range(rp.shape[0]):
+        e = rel_l2(gp[n], rp[n])
+        assert e <= TOL, (n, e)
+    if kind == "farfield_cpml":
+        rf, gf = o.farfield(), s.farfield()
+        e = rel_l2(gf, rf)
+        print("time-domain far field rel L2", e)
+        assert e <= TOL
+    o.close()
+    s.close()
+
+
+@pytest.mark.parametrize("test_id", ["10000000", "10021100", "10022100", "10024100", "10028000", "10035011"])
+def test_gpu_reproduces_reference_selftest(test_id, tmp_path):
+    """The reference's own golden traces (printed with %g), recipe src3d/tests.c:92-257."""
+    par, nsteps = write_selftest(str(tmp_path / test_id), test_id)
+    ff = int(test_id[6])
+    s = Session(par)
+    s.run()
+    gold = np.array(read_golden(test_id, nsteps, ff))
+    if ff:
+        b = s.farfield()[0]
+        got = np.sqrt(b[0] ** 2 + b[1] ** 2 + b[2] ** 2)[: len(gold)]
+    else:
+        got = s.points()[0][: len(gold), 0]
+    e = rel_l2(got, gold)
+    print(test_id, "GPU vs golden rel L2", e)
+    assert e <= 2e-5
+    s.close()
+
+
+def test_config1_reduced_against_oracle(tmp_path):
+    """configs[0] geometry at 64^3 and 300 steps (the full 128^3 x 1000 run is in test_config1_full)."""
+    par = cases.config1(str(tmp_path / "c1"), n=64, nsteps=300, nspheres=12)
+    o, s = run_both(par)
+    ref, got = o.fields(), s.fields()
+    errs = [rel_l2(got[c], ref[c]) for c in range(6)]
+    print("config1 reduced per component", errs, s.info())
+    assert max(errs) <= TOL
+    mo, bo = o.maps()
+    ms, bs = s.maps()
+    assert np.array_equal(mo, ms) and np.array_equal(bo, bs)
