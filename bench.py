@@ -63,13 +63,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--workdir", default=None)
+    ap.add_argument("--tune-steps", type=int, default=2, help="steps per launch-shape candidate (0 = no tuning)")
+    ap.add_argument("--launch", default=None,
+                    help="fixed launch shape hx,hy,hz,ex,ey,ez (tile x width, rows, z-march depth of the H and E "
+                         "kernels); implies no tuning")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the z-slab decomposition is described in DESIGN.md but not built)", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine has no multi-GPU path (a z-slab decomposition "
+              "with halo exchange is the planned sharding); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import cases
@@ -77,7 +83,8 @@ def main():
 
     n = args.n
     depth = 16
-    nsteps = args.warmup + args.steps
+    ncand = 45
+    nsteps = ncand * args.tune_steps + args.warmup + args.steps + args.kernel_steps
     work = args.workdir or tempfile.mkdtemp(prefix="gsv_bench_")
     t0 = time.time()
     log("bench: writing %d^3 rough-surface workload to %s" % (n, work))
@@ -101,6 +108,13 @@ def main():
     n_ade = int(ade_cells.sum())
     del mat, bnds, metal, ade_cells
 
+    shape = s.get_launch()
+    if args.launch:
+        shape = [int(v) for v in args.launch.split(",")]
+        s.set_launch(shape)
+    elif args.tune_steps > 0:
+        shape, used = s.autotune(args.tune_steps)
+        log("bench: autotuned launch shape %s in %d steps" % (shape, used))
     log("bench: warmup %d steps" % args.warmup)
     s.run(args.warmup)
     s.sync()
@@ -113,14 +127,10 @@ def main():
     gcells = float(n) ** 3 * args.steps / elapsed / 1e9
     log("bench: %.3f ms/step, %.2f Gcell-updates/s" % (ms_per_step, gcells))
 
-    # per-kernel times with HIP events on the engine stream (separate pass: the
-    # event synchronisation would perturb the whole-step timing above)
-    s.close()
-    s = Session(par)
-    s.run(args.warmup)
+    # per-kernel times with HIP events on the engine stream, in steps after the
+    # timed region (the per-launch event synchronisation would perturb it)
     s.set_timing(True)
-    kt_steps = min(args.steps, 100)
-    s.run(kt_steps)
+    s.run(args.kernel_steps)
     kt = s.kernel_times()
     s.close()
     h_ms = kt["h_ms"] / max(kt["h_n"], 1)
@@ -138,18 +148,21 @@ def main():
         try:
             from oracle_lib import Oracle
             nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-            log("bench: CPU baseline (FP64 oracle, %d threads), %d steps at %d^3" % (nthreads, args.cpu_steps, n))
+            cn = args.cpu_n
+            log("bench: CPU baseline (FP64 oracle, %d threads), %d steps at %d^3" % (nthreads, args.cpu_steps, cn))
+            cpar = cases.config2(os.path.join(work, "cpu%d" % cn), n=cn, nsteps=args.cpu_steps + 1)
             t0 = time.time()
-            o = Oracle(par)
+            o = Oracle(cpar)
             log("bench: oracle planned in %.1f s" % (time.time() - t0))
             o.step(1)
             t0 = time.perf_counter()
             o.step(args.cpu_steps)
             ce = time.perf_counter() - t0
             o.close()
-            cpu = {"value": float(n) ** 3 * args.cpu_steps / ce / 1e9, "unit": "Gcell-updates/s", "cores": nthreads,
+            cpu = {"value": float(cn) ** 3 * args.cpu_steps / ce / 1e9, "unit": "Gcell-updates/s", "cores": nthreads,
                    "kind": "port",
-                   "sample": "%d steps (after 1 untimed) of the same %d^3 workload, FP64 fields, OpenMP over x" % (args.cpu_steps, n)}
+                   "sample": "%d steps (after 1 untimed) of the same workload at %d^3 (same physics and layout, "
+                             "grid scaled down to bound the CPU time), FP64 fields, OpenMP over x" % (args.cpu_steps, cn)}
             log("bench: CPU %.4f Gcell-updates/s" % cpu["value"])
         except Exception as exc:  # the baseline is reported, never required
             log("bench: CPU baseline skipped: %r" % (exc,))
@@ -170,7 +183,8 @@ def main():
         "config": {"workload": "rough-surface scattering, Drude ADE half space, CPML 16, TSF 45 deg p-pol, "
                                "running-DFT NFFF at 3 wavelengths",
                    "grid": [n, n, n], "cell_nm": 5, "ade_cells": n_ade,
-                   "e_classes": info["e_classes"], "h_classes": info["h_classes"]},
+                   "e_classes": info["e_classes"], "h_classes": info["h_classes"],
+                   "launch_shape_h": shape[:3], "launch_shape_e": shape[3:]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel_ms": round(dom_ms, 4), "algorithmic_bytes": dom_bytes,

@@ -82,7 +82,7 @@ typedef struct {
     float dx, dy, dz;
     /* ADE */
     int ade, ade_poles;
-    GsvAde adem[GSV_MAX_ADE];
+    const GsvAde *adem;     /* ADE material table in device memory */
     float *exp_, *eyp, *ezp, *dpx, *dpy, *dpz, *dpxp, *dpyp, *dpzp;
     float *px0, *px1, *py0, *py1, *pz0, *pz1, *pxp0, *pxp1, *pyp0, *pyp1, *pzp0, *pzp1;
 } GsvGrid;

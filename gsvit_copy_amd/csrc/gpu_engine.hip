@@ -78,11 +78,12 @@ struct Engine {
     int ready;
     GsvGrid g;
     int idbytes_e, idbytes_h;
-    int bx, by, kchunk_h, kchunk_e;
+    int bx_h, by_h, kchunk_h, bx_e, by_e, kchunk_e;
     std::vector<GsvECoef> etab;
     std::vector<GsvHCoef> htab;
     std::vector<uint8_t> ecls_host, hcls_host;   /* device layout, idbytes per cell */
     int ade_slot_of_mat[256];
+    GsvAde adem_host[GSV_MAX_ADE];
     std::vector<void *> allocs;
     float *fbase[6];
     size_t field_floats;
@@ -303,7 +304,7 @@ build_tables(Engine *E)
             fprintf(stderr, "Error: at most %d ADE materials are supported\n", GSV_MAX_ADE);
             return 1;
         }
-        GsvAde &a = E->g.adem[nslot];
+        GsvAde &a = E->adem_host[nslot];
         double dt = mp->d->dt;
         a.c0 = (float)m->ade_c0; a.c1 = (float)m->ade_c1; a.c2 = (float)m->ade_c2; a.c3 = (float)m->ade_c3;
         a.a0 = (float)m->ade_a0; a.a1 = (float)m->ade_a1; a.a2 = (float)m->ade_a2;
@@ -401,15 +402,10 @@ sv_gpu_new(SvPool *mp, SvSet *set)
     E->g.dx = (float)set->sp.dx;
     E->g.dy = (float)set->sp.dy;
     E->g.dz = (float)set->sp.dz;
-    E->bx = engine_env_int("GSV_BX", 64);
-    E->by = engine_env_int("GSV_BY", 4);
-    E->kchunk_h = engine_env_int("GSV_KCHUNK_H", 16);
-    E->kchunk_e = engine_env_int("GSV_KCHUNK_E", 16);
-    if (E->bx * E->by > 256 || E->bx <= 0 || E->by <= 0 || E->kchunk_h <= 0 || E->kchunk_e <= 0) {
-        fprintf(stderr, "Error: invalid launch shape (%d x %d, k chunk %d/%d)\n", E->bx, E->by, E->kchunk_h, E->kchunk_e);
-        delete E;
-        return NULL;
-    }
+    /* default launch shape; gsv_autotune / gsv_set_launch change it */
+    E->bx_h = E->bx_e = 64;
+    E->by_h = E->by_e = 4;
+    E->kchunk_h = E->kchunk_e = 16;
     if (mp->src->tsf) {
         SvSourceTSF *t = mp->src->tsf;
         if (t->box_in > set->sp.xres - 2 || t->box_jn > set->sp.yres - 2 || t->box_kn > set->sp.zres - 2) {
@@ -868,6 +864,12 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
         g.nhtab = (int)E->htab.size();
     }
     if (g.ade) {
+        GsvAde *am;
+        if (dupload(E, &am, E->adem_host, GSV_MAX_ADE))
+            return 1;
+        g.adem = am;
+    }
+    if (g.ade) {
         size_t n = (size_t)g.plane * g.nz;
         float **arr[] = { &g.exp_, &g.eyp, &g.ezp, &g.dpx, &g.dpy, &g.dpz, &g.dpxp, &g.dpyp, &g.dpzp };
         for (auto a : arr) {
@@ -1038,7 +1040,7 @@ sv_pool_gpu_ystep_h(SvPool *mp, SvSet *set)
     (void)set;
     if (E->timing)
         CK(hipEventRecord(E->ev[0], E->st));
-    CK(gsv_launch_hstep(&E->g, E->idbytes_h, E->bx, E->by, E->kchunk_h, E->st));
+    CK(gsv_launch_hstep(&E->g, E->idbytes_h, E->bx_h, E->by_h, E->kchunk_h, E->st));
     if (E->timing) {
         float ms;
         CK(hipEventRecord(E->ev[1], E->st));
@@ -1080,7 +1082,7 @@ sv_pool_gpu_ystep_e(SvPool *mp, SvSet *set)
     (void)set;
     if (E->timing)
         CK(hipEventRecord(E->ev[2], E->st));
-    CK(gsv_launch_estep(&E->g, E->idbytes_e, E->bx, E->by, E->kchunk_e, E->st));
+    CK(gsv_launch_estep(&E->g, E->idbytes_e, E->bx_e, E->by_e, E->kchunk_e, E->st));
     if (E->timing) {
         float ms;
         CK(hipEventRecord(E->ev[3], E->st));
@@ -1188,6 +1190,35 @@ gsv_engine_sync(void *eng)
     return 0;
 }
 
+/* launch shape of the volume kernels: bx * by <= 512 threads, z chunk >= 1 */
+extern "C" int
+gsv_engine_set_launch(void *eng, const int *shape)
+{
+    Engine *E = (Engine *)eng;
+    for (int q = 0; q < 2; q++)
+        if (shape[3 * q] <= 0 || shape[3 * q + 1] <= 0 || shape[3 * q] * shape[3 * q + 1] > 512 || shape[3 * q + 2] <= 0)
+            return 1;
+    E->bx_h = shape[0];
+    E->by_h = shape[1];
+    E->kchunk_h = shape[2];
+    E->bx_e = shape[3];
+    E->by_e = shape[4];
+    E->kchunk_e = shape[5];
+    return 0;
+}
+
+extern "C" void
+gsv_engine_get_launch(void *eng, int *shape)
+{
+    Engine *E = (Engine *)eng;
+    shape[0] = E->bx_h;
+    shape[1] = E->by_h;
+    shape[2] = E->kchunk_h;
+    shape[3] = E->bx_e;
+    shape[4] = E->by_e;
+    shape[5] = E->kchunk_e;
+}
+
 extern "C" void
 gsv_engine_set_timing(void *eng, int on)
 {
@@ -1219,7 +1250,7 @@ gsv_engine_info(void *eng, int *info)
     info[3] = E->idbytes_h;
     info[4] = E->g.ade;
     info[5] = E->g.ade_poles;
-    info[6] = E->bx * E->by;
+    info[6] = E->bx_e * E->by_e;
     info[7] = E->kchunk_e;
 }
 

@@ -32,7 +32,7 @@
 /* ------------------------------------------------------------------ */
 
 template <typename ID>
-__global__ __launch_bounds__(256) void
+__global__ __launch_bounds__(512) void
 k_hstep(GsvGrid g, int kchunk)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,7 +163,8 @@ k_hstep(GsvGrid g, int kchunk)
 /* ADE branch of one cell (src3d/yee.c:914-1108); updates the components whose
  * flag is clear, runs the branch's own CPML psi update and advances the
  * polarisation state.  Rarely taken: dispersive cells and their +1 neighbours. */
-__device__ __noinline__ void
+template <bool POLES>
+__device__ __forceinline__ void
 ade_cell(const GsvGrid &g, long long c, int i, int j, int k, unsigned fl, float &ex, float &ey, float &ez,
          float hx, float hy, float hz, float hxkm1, float hykm1, float hzi, float hyi, float hzj, float hxj,
          float rkx, float rky, float rkz)
@@ -177,7 +178,7 @@ ade_cell(const GsvGrid &g, long long c, int i, int j, int k, unsigned fl, float 
     const float dpxp = g.dpxp[c], dpyp = g.dpyp[c], dpzp = g.dpzp[c];
     float px0 = 0, px1 = 0, py0 = 0, py1 = 0, pz0 = 0, pz1 = 0;
     float pxp0 = 0, pxp1 = 0, pyp0 = 0, pyp1 = 0, pzp0 = 0, pzp1 = 0;
-    if (m.poles) {
+    if (POLES && m.poles) {
         px0 = g.px0[c]; px1 = g.px1[c]; py0 = g.py0[c]; py1 = g.py1[c]; pz0 = g.pz0[c]; pz1 = g.pz1[c];
         pxp0 = g.pxp0[c]; pxp1 = g.pxp1[c]; pyp0 = g.pyp0[c]; pyp1 = g.pyp1[c]; pzp0 = g.pzp0[c]; pzp1 = g.pzp1[c];
     }
@@ -261,7 +262,7 @@ ade_cell(const GsvGrid &g, long long c, int i, int j, int k, unsigned fl, float 
     g.dpx[c] = m.a0 * dpxp + m.a1 * dpx + m.a2 * (expv + 2.f * vexp + ex);
     g.dpy[c] = m.a0 * dpyp + m.a1 * dpy + m.a2 * (eypv + 2.f * veyp + ey);
     g.dpz[c] = m.a0 * dpzp + m.a1 * dpz + m.a2 * (ezpv + 2.f * vezp + ez);
-    if (m.poles) {
+    if (POLES && m.poles) {
         g.px0[c] = m.bp0[0] * pxp0 + m.bp1[0] * px0 + m.bp2[0] * expv + m.bp3[0] * vexp + m.bp4[0] * ex;
         g.px1[c] = m.bp0[1] * pxp1 + m.bp1[1] * px1 + m.bp2[1] * expv + m.bp3[1] * vexp + m.bp4[1] * ex;
         g.py0[c] = m.bp0[0] * pyp0 + m.bp1[0] * py0 + m.bp2[0] * eypv + m.bp3[0] * veyp + m.bp4[0] * ey;
@@ -278,8 +279,8 @@ ade_cell(const GsvGrid &g, long long c, int i, int j, int k, unsigned fl, float 
     g.dpzp[c] = dpz;
 }
 
-template <typename ID>
-__global__ __launch_bounds__(256) void
+template <typename ID, int ADE>
+__global__ __launch_bounds__(512) void
 k_estep(GsvGrid g, int kchunk)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -315,8 +316,8 @@ k_estep(GsvGrid g, int kchunk)
         const unsigned fl = ec.flags;
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
 
-        if (fl & GSV_E_ADE)
-            ade_cell(g, c, i, j, k, fl, ex, ey, ez, hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
+        if (ADE && (fl & GSV_E_ADE))
+            ade_cell<(ADE == 2)>(g, c, i, j, k, fl, ex, ey, ez, hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
 
         if (tx && j > 0 && k > 0)
             ex = ec.ca[0] * ex + ec.cb[0] * ((hz - hzj) * rky - (hy - hykm1) * rkz);
@@ -846,10 +847,20 @@ extern "C" hipError_t
 gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st)
 {
     dim3 blk(bx, by, 1), grd((g->nx + bx - 1) / bx, (g->ny + by - 1) / by, (g->nz + kchunk - 1) / kchunk);
-    if (idbytes == 1)
-        hipLaunchKernelGGL(k_estep<uint8_t>, grd, blk, 0, st, *g, kchunk);
-    else
-        hipLaunchKernelGGL(k_estep<uint16_t>, grd, blk, 0, st, *g, kchunk);
+    /* the dispersive branch is compiled in only when the pool has ADE cells,
+     * and the critical-point terms only when some material has them */
+    const int ade = g->ade ? (g->ade_poles ? 2 : 1) : 0;
+#define GSV_ESTEP(ID, A) hipLaunchKernelGGL((k_estep<ID, A>), grd, blk, 0, st, *g, kchunk)
+    if (idbytes == 1) {
+        if (ade == 0) GSV_ESTEP(uint8_t, 0);
+        else if (ade == 1) GSV_ESTEP(uint8_t, 1);
+        else GSV_ESTEP(uint8_t, 2);
+    } else {
+        if (ade == 0) GSV_ESTEP(uint16_t, 0);
+        else if (ade == 1) GSV_ESTEP(uint16_t, 1);
+        else GSV_ESTEP(uint16_t, 2);
+    }
+#undef GSV_ESTEP
     return hipGetLastError();
 }
 
@@ -967,4 +978,55 @@ gsv_launch_dft(const GsvGrid *g, const GsvDft *d, int step, hipStream_t st)
 {
     hipLaunchKernelGGL(k_dft_faces, dim3((unsigned)((d->ncells + 255) / 256)), dim3(256), 0, st, *g, *d, step);
     return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ */
+/* HBM probe: a plain streaming copy of known size, used to calibrate the
+ * FETCH_SIZE / WRITE_SIZE counters and to measure attainable bandwidth    */
+/* ------------------------------------------------------------------ */
+
+__global__ __launch_bounds__(256) void
+k_copy_probe(const float4 *__restrict__ a, float4 *__restrict__ b, long long n)
+{
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; q < n; q += stride)
+        b[q] = a[q];
+}
+
+extern "C" int
+gsv_copy_probe(long long nbytes, int reps, double *ms_per_copy)
+{
+    float4 *a = NULL, *b = NULL;
+    hipEvent_t e0, e1;
+    hipStream_t st;
+    float ms = 0;
+    const long long n = nbytes / 16;
+    if (n <= 0 || reps <= 0)
+        return 1;
+    if (hipMalloc(&a, n * 16) != hipSuccess)
+        return 1;
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        hipFree(a);
+        return 1;
+    }
+    hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    hipMemsetAsync(a, 0, n * 16, st);
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    const unsigned blocks = 256 * 32;
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, st, a, b, n);
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, st, a, b, n);
+    hipEventRecord(e1, st);
+    hipEventSynchronize(e1);
+    hipEventElapsedTime(&ms, e0, e1);
+    *ms_per_copy = ms / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(st);
+    hipFree(a);
+    hipFree(b);
+    return hipGetLastError() != hipSuccess;
 }

@@ -19,6 +19,8 @@ void gsv_engine_info(void *eng, int *info);
 int gsv_engine_get_field(void *eng, int comp, float *out);
 int gsv_engine_fetch_aux(void *eng);
 void gsv_engine_free(void *eng);
+int gsv_engine_set_launch(void *eng, const int *shape);
+void gsv_engine_get_launch(void *eng, int *shape);
 
 typedef struct {
     SvSet set;
@@ -182,6 +184,69 @@ void
 gsv_kernel_times(void *h, double *out)
 {
     gsv_engine_kernel_times(((GsvSession *)h)->mp->gpu, out);
+}
+
+int
+gsv_set_launch(void *h, const int *shape)
+{
+    return gsv_engine_set_launch(((GsvSession *)h)->mp->gpu, shape);
+}
+
+void
+gsv_get_launch(void *h, int *shape)
+{
+    gsv_engine_get_launch(((GsvSession *)h)->mp->gpu, shape);
+}
+
+/* Launch-shape autotuner: each candidate tile (x width, rows, z-march depth)
+ * runs `steps_per_candidate` real steps of the simulation with per-launch
+ * event timing; the fastest shape is kept for the H and for the E kernel
+ * independently.  Returns the number of steps consumed, or -1 on error. */
+int
+gsv_autotune(void *h, int steps_per_candidate, int *best)
+{
+    static const int bxs[] = { 64, 128, 256 };
+    static const int bys[] = { 1, 2, 4, 8 };
+    static const int kcs[] = { 8, 16, 32, 64, 128 };
+    GsvSession *s = (GsvSession *)h;
+    void *eng = s->mp->gpu;
+    double bh = 1e30, be = 1e30, t[4];
+    int a, b, c, used = 0, shape[6], start = s->set.sc.step_act;
+    gsv_engine_get_launch(eng, best);
+    for (a = 0; a < 3; a++)
+        for (b = 0; b < 4; b++)
+            for (c = 0; c < 5; c++) {
+                int nthr = bxs[a] * bys[b];
+                if (nthr < 64 || nthr > 512)
+                    continue;
+                if (s->set.sc.step_act + steps_per_candidate > s->set.sc.nsteps)
+                    goto done;
+                shape[0] = shape[3] = bxs[a];
+                shape[1] = shape[4] = bys[b];
+                shape[2] = shape[5] = kcs[c];
+                if (gsv_engine_set_launch(eng, shape))
+                    continue;
+                gsv_engine_set_timing(eng, 1);
+                if (gsv_run(h, steps_per_candidate) < 0)
+                    return -1;
+                gsv_engine_kernel_times(eng, t);
+                gsv_engine_set_timing(eng, 0);
+                if (t[1] > 0 && t[0] / t[1] < bh) {
+                    bh = t[0] / t[1];
+                    best[0] = shape[0]; best[1] = shape[1]; best[2] = shape[2];
+                }
+                if (t[3] > 0 && t[2] / t[3] < be) {
+                    be = t[2] / t[3];
+                    best[3] = shape[3]; best[4] = shape[4]; best[5] = shape[5];
+                }
+            }
+done:
+    gsv_engine_set_launch(eng, best);
+    used = s->set.sc.step_act - start;
+    if (s->set.sc.verbose > 0)
+        printf("autotune: H %dx%d z%d (%.3f ms), E %dx%d z%d (%.3f ms), %d steps\n", best[0], best[1], best[2], bh,
+               best[3], best[4], best[5], be, used);
+    return used;
 }
 
 void

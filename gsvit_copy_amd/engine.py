@@ -43,6 +43,11 @@ def library():
         L.gsv_set_timing.argtypes = [vp, ctypes.c_int]
         L.gsv_kernel_times.argtypes = [vp, dp]
         L.gsv_close.argtypes = [vp]
+        L.gsv_set_launch.restype = ctypes.c_int
+        L.gsv_set_launch.argtypes = [vp, ip]
+        L.gsv_get_launch.argtypes = [vp, ip]
+        L.gsv_autotune.restype = ctypes.c_int
+        L.gsv_autotune.argtypes = [vp, ctypes.c_int, ip]
         _lib = L
     return _lib
 
@@ -139,6 +144,25 @@ class Session:
         v = (ctypes.c_double * 4)()
         self.L.gsv_kernel_times(self.h, v)
         return {"h_ms": v[0], "h_n": int(v[1]), "e_ms": v[2], "e_n": int(v[3])}
+
+    def set_launch(self, shape):
+        v = (ctypes.c_int * 6)(*[int(x) for x in shape])
+        if self.L.gsv_set_launch(self.h, v):
+            raise ValueError("invalid launch shape %r" % (shape,))
+
+    def get_launch(self):
+        v = (ctypes.c_int * 6)()
+        self.L.gsv_get_launch(self.h, v)
+        return list(v)
+
+    def autotune(self, steps_per_candidate=2):
+        """Times candidate launch shapes on real steps; returns (shape, steps consumed)."""
+        v = (ctypes.c_int * 6)()
+        with _chdir(self.dir):
+            used = self.L.gsv_autotune(self.h, int(steps_per_candidate), v)
+        if used < 0:
+            raise RuntimeError("autotune failed")
+        return list(v), used
 
     def close(self):
         if getattr(self, "h", None):

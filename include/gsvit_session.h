@@ -40,7 +40,19 @@ void gsv_get_maps(void *session, int *mat, int *bnds);
 void gsv_set_timing(void *session, int on);
 /* out[0..3]: H total ms, H launches, E total ms, E launches */
 void gsv_kernel_times(void *session, double *out);
+/* launch shape of the volume kernels: shape[0..5] = H tile x width, H rows,
+ * H z-march depth, E tile x width, E rows, E z-march depth (x width * rows <= 512) */
+int gsv_set_launch(void *session, const int *shape);
+void gsv_get_launch(void *session, int *shape);
+/* pick the fastest launch shape for each volume kernel by timing candidate
+ * shapes on real steps (steps_per_candidate each); writes the chosen shape to
+ * best[0..5] and returns the number of steps consumed, -1 on error */
+int gsv_autotune(void *session, int steps_per_candidate, int *best);
 void gsv_close(void *session);
+/* streaming copy of nbytes (read nbytes + write nbytes) repeated reps times on
+ * the current device; *ms_per_copy = mean time of one copy.  Calibrates HBM
+ * traffic counters and attainable bandwidth.  Returns 0 on success. */
+int gsv_copy_probe(long long nbytes, int reps, double *ms_per_copy);
 
 #ifdef __cplusplus
 }
