@@ -34,8 +34,8 @@ gsv_open(const char *parfile)
     GsvSession *s = (GsvSession *)calloc(1, sizeof(GsvSession));
     if (!s)
         return NULL;
-    clear_settings(&s->set);
-    if (!parse_settings(parfile, &s->set)) {
+    clear_settings(&s->set, 0);
+    if (!parse_settings(parfile, &s->set, 1)) {
         fprintf(stderr, "Settings error!\n");
         free(s);
         return NULL;
@@ -43,7 +43,7 @@ gsv_open(const char *parfile)
     s->set.sc.usegpu = 1;
     s->mp = init_and_make_plan(&s->set);
     if (!s->mp) {
-        free_settings(&s->set);
+        clear_settings(&s->set, 1);
         free(s);
         return NULL;
     }
@@ -260,6 +260,6 @@ gsv_close(void *h)
         s->mp->gpu = NULL;
         sv_pool_free(s->mp, &s->set);
     }
-    free_settings(&s->set);
+    clear_settings(&s->set, 1);
     free(s);
 }

@@ -22,9 +22,22 @@
 #include "gsvit_settings.h"
 #include "gsvit_internal.h"
 
+/* remove != 0 frees what an earlier parse allocated, as the reference's
+ * clear_settings(set, TRUE) does; remove == 0 expects an uninitialised SvSet */
 void
-clear_settings(SvSet *set)
+clear_settings(SvSet *set, int remove)
 {
+    if (remove) {
+        free(set->ss.pnts);
+        free(set->so.pnts);
+        free(set->so.sums);
+        free(set->sf.ri);
+        free(set->sf.rj);
+        free(set->sf.rk);
+        free(set->sf.individual);
+        free(set->sf.source_filename);
+        free(set->sf.dft_lambda);
+    }
     memset(set, 0, sizeof(SvSet));
     /* defaults from src3d/settings.c:570-990 and src3d/defaults.h */
     set->sp.xres = set->sp.yres = set->sp.zres = 100;
@@ -48,21 +61,6 @@ clear_settings(SvSet *set)
     set->smb.bxnpos = set->smb.bynpos = set->smb.bznpos = -1;
     set->sf.sumfrom = 0;
     set->sf.sumto = 0;
-}
-
-void
-free_settings(SvSet *set)
-{
-    free(set->ss.pnts);
-    free(set->so.pnts);
-    free(set->so.sums);
-    free(set->sf.ri);
-    free(set->sf.rj);
-    free(set->sf.rk);
-    free(set->sf.individual);
-    free(set->sf.source_filename);
-    free(set->sf.dft_lambda);
-    memset(set, 0, sizeof(SvSet));
 }
 
 static int
@@ -248,8 +246,9 @@ skip_output_keyword(FILE *fr, const char *key)
 }
 
 int
-parse_settings(const char *filename, SvSet *set)
+parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
 {
+    (void)called_from_gsvit;   /* the reference uses it to pick message styles only */
     char key[128];
     char value[64];
     FILE *fr = fopen(filename, "r");

@@ -38,8 +38,8 @@ main(int argc, char *argv[])
         printf("Usage: gsvit_amd parameter_file [--killer file]\n");
         return 1;
     }
-    clear_settings(&set);
-    if (!parse_settings(argv[1], &set)) {
+    clear_settings(&set, 0);
+    if (!parse_settings(argv[1], &set, 1)) {
         printf("Settings error!\n");
         return 1;
     }

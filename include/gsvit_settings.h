@@ -236,12 +236,13 @@ typedef struct {
     SvSetPlan plan;
 } SvSet;
 
-/* src3d/settings.c:570 (clear_settings) and :3822 (parse_settings).
+/* src3d/settings.h:967,974 (clear_settings src3d/settings.c:570, parse_settings :3822).
+ * clear_settings(set, 1) frees what a parse allocated, then resets the defaults.
  * parse_settings returns 1 on success and 0 on error, like the reference
- * (src3d/settings.c:5831-5836, checked by src3d/svit.c:186). */
-void clear_settings(SvSet *set);
-int parse_settings(const char *filename, SvSet *set);
-void free_settings(SvSet *set);
+ * (src3d/settings.c:5831-5836, checked by src3d/svit.c:186); the third
+ * argument is accepted for signature compatibility. */
+void clear_settings(SvSet *set, int remove);
+int parse_settings(const char *filename, SvSet *set, int called_from_gsvit);
 
 #ifdef __cplusplus
 }

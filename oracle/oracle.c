@@ -32,7 +32,11 @@
  * two samples of the 1-D line), this restatement returns 0 for the missing
  * sample; the TSF geometry never reaches those samples for boxes inside the pool.
  *
- * Parity status of each part is listed in DESIGN.md of the repository root.
+ * Pinning: the reference's self-test goldens and example logs pin the Yee,
+ * averaging, ADE, CPML, TSF, point-source and time-domain far-field parts
+ * (tests/test_oracle_goldens.py).  Parity unpinned: the interior periodic
+ * copies (the only reference log with them is from an older version and
+ * drifts); they follow src3d/boundary.c line by line in meaning.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -102,8 +106,8 @@ oracle_open(const char *parfile)
     SvSetBoundary *b;
     int q;
 
-    clear_settings(set);
-    if (!parse_settings(parfile, set)) {
+    clear_settings(set, 0);
+    if (!parse_settings(parfile, set, 1)) {
         free(o);
         return NULL;
     }
@@ -190,7 +194,7 @@ oracle_close(void *handle)
     }
     free(o->je); free(o->jh); free(o->points);
     sv_pool_free(o->mp, &o->set);
-    free_settings(&o->set);
+    clear_settings(&o->set, 1);
     free(o);
 }
 

@@ -11,7 +11,7 @@ import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(REPO, "gsvit_copy_amd", "lib", "libgsvit_amd.so")
-DECL = re.compile(r"^\s*(?:[A-Za-z_][\w\s\*]*?[\s\*])(sv_[a-z_0-9]+|gsv_[a-z_0-9]+|clear_settings|parse_settings|free_settings|init_and_make_plan|is_output)\s*\(", re.M)
+DECL = re.compile(r"^\s*(?:[A-Za-z_][\w\s\*]*?[\s\*])(sv_[a-z_0-9]+|gsv_[a-z_0-9]+|clear_settings|parse_settings|init_and_make_plan|is_output)\s*\(", re.M)
 
 # host-side helpers declared in the headers that are not exported by the library
 NOT_EXPORTED = set()
