@@ -25,11 +25,11 @@ sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(HERE, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s + % HBM peak at 512\u00b3; adjoint-grad time \u00f7 plain forward"
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-METRIC = "Gcell-updates/s + % HBM peak at 512\u00b3; adjoint-grad time \u00f7 plain forward"
 
 
 def algorithmic_bytes(info, n, depth, n_ade_cells):
@@ -80,7 +80,7 @@ def main():
     work = args.workdir or tempfile.mkdtemp(prefix="gsv_bench_")
     t0 = time.time()
     log("bench: writing %d^3 rough-surface workload to %s" % (n, work))
-    par = cases.config2(work, n=n, nsteps=nsteps)
+    par = cases.rough_surface(work, n=n, nsteps=nsteps)
     log("bench: workload written in %.1f s; planning" % (time.time() - t0))
     t0 = time.time()
     s = Session(par)
@@ -142,7 +142,7 @@ def main():
             nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
             cn = args.cpu_n
             log("bench: CPU baseline (FP64 oracle, %d threads), %d steps at %d^3" % (nthreads, args.cpu_steps, cn))
-            cpar = cases.config2(os.path.join(work, "cpu%d" % cn), n=cn, nsteps=args.cpu_steps + 1)
+            cpar = cases.rough_surface(os.path.join(work, "cpu%d" % cn), n=cn, nsteps=args.cpu_steps + 1)
             t0 = time.time()
             o = Oracle(cpar)
             log("bench: oracle planned in %.1f s" % (time.time() - t0))

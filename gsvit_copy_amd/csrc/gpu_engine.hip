@@ -107,6 +107,8 @@ struct Engine {
     GsvFarfieldTD ff;
     int has_dft;
     GsvDft dft;
+    /* box sums */
+    std::vector<GsvBoxSum> sums;
     /* timing of the two volume kernels */
     int timing;
     hipEvent_t ev[4];
@@ -828,6 +830,48 @@ setup_farfield(Engine *E)
     return 0;
 }
 
+/* OUT_SUM boxes: the voxel selection (permittivity within EPSILON_0/100 of
+ * the requested one, src3d/output.c:282) and the sigma weight of the
+ * absorption mode are fixed, so they become one float weight per box cell */
+static int
+setup_sums(Engine *E)
+{
+    SvSet *set = E->set;
+    for (int n = 0; n < set->so.nsums; n++) {
+        const SvOutputSum *o = set->so.sums + n;
+        GsvBoxSum b;
+        memset(&b, 0, sizeof(b));
+        b.i0 = o->box_i0; b.j0 = o->box_j0; b.k0 = o->box_k0;
+        b.ni = o->box_in > o->box_i0 ? o->box_in - o->box_i0 : 0;
+        b.nj = o->box_jn > o->box_j0 ? o->box_jn - o->box_j0 : 0;
+        b.nk = o->box_kn > o->box_k0 ? o->box_kn - o->box_k0 : 0;
+        b.mode = o->component;
+        size_t cnt = (size_t)b.ni * b.nj * b.nk;
+        if (cnt && (o->layered_epsilon != -1 || o->component == SV_SUM_ABS)) {
+            std::vector<float> w(cnt);
+            #pragma omp parallel for schedule(static)
+            for (int k = 0; k < b.nk; k++)
+                for (int j = 0; j < b.nj; j++)
+                    for (int i = 0; i < b.ni; i++) {
+                        double eps, sig;
+                        sv_pool_epsilon_sigma(E->mp, i + b.i0, j + b.j0, k + b.k0, &eps, &sig);
+                        int keep = o->layered_epsilon == -1 || fabs(o->layered_epsilon - eps) < SV_EPSILON_0 / 100.0;
+                        w[(size_t)i + (size_t)b.ni * (j + (size_t)b.nj * k)] =
+                            keep ? (o->component == SV_SUM_ABS ? (float)sig : 1.f) : 0.f;
+                    }
+            float *dw;
+            if (dupload(E, &dw, w.data(), cnt))
+                return 1;
+            b.w = dw;
+        }
+        if (dmalloc(E, &b.out, (size_t)set->sc.nsteps + 1))
+            return 1;
+        CK(hipMemsetAsync(b.out, 0, ((size_t)set->sc.nsteps + 1) * 8, E->st));
+        E->sums.push_back(b);
+    }
+    return 0;
+}
+
 extern "C" int
 sv_pool_gpu_init(SvPool *mp, SvSet *set)
 {
@@ -884,7 +928,8 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
             }
         }
     }
-    if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E))
+    if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E)
+        || setup_sums(E))
         return 1;
     for (int q = 0; q < 4; q++)
         CK(hipEventCreate(&E->ev[q]));
@@ -988,6 +1033,11 @@ fetch_aux(Engine *E, SvPool *mp, SvSet *set)
     if (E->has_dft) {
         size_t cnt = (size_t)E->dft.ncells * E->dft.nw * 8;
         CK(hipMemcpyAsync(mp->farfield->dft_data, E->dft.acc, cnt * 8, hipMemcpyDeviceToHost, E->st));
+        CK(hipStreamSynchronize(E->st));
+    }
+    for (size_t n = 0; n < E->sums.size(); n++) {
+        CK(hipMemcpyAsync(mp->out->outsumdata[n], E->sums[n].out, ((size_t)set->sc.nsteps + 1) * 8,
+                          hipMemcpyDeviceToHost, E->st));
         CK(hipStreamSynchronize(E->st));
     }
     return 0;
@@ -1127,11 +1177,10 @@ extern "C" int
 sv_pool_gpu_getsum(SvPool *mp, SvSet *set)
 {
     ENGINE_OR_FAIL(mp);
-    (void)E;
-    if (set->so.nsums) {
-        fprintf(stderr, "Error: OUT_SUM is not supported by the MI355X engine\n");
+    if (set->sc.step_act > set->sc.nsteps)
         return 1;
-    }
+    for (size_t n = 0; n < E->sums.size(); n++)
+        CK(gsv_launch_box_sum(&E->g, &E->sums[n], set->sc.step_act, E->st));
     return 0;
 }
 

@@ -67,6 +67,16 @@ typedef struct {
     double dt;
 } GsvDft;
 
+/* one OUT_SUM box: per step the sum (or maximum) of selected E components
+ * squared over the box, src3d/output.c:259-306 */
+typedef struct {
+    int i0, j0, k0, ni, nj, nk;   /* box origin and extents */
+    int mode;                     /* SvSumType */
+    const float *w;               /* weight per box cell (i fastest): 0 = excluded voxel,
+                                   * sigma for SV_SUM_ABS, else 1; NULL = 1 everywhere */
+    double *out;                  /* this box's value per step */
+} GsvBoxSum;
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -82,6 +92,7 @@ hipError_t gsv_launch_zero_planes(const GsvGrid *g, int axis, hipStream_t st);
 hipError_t gsv_launch_periodic(const GsvGrid *g, const GsvPeriodic *p, int hside, hipStream_t st);
 hipError_t gsv_launch_farfield_td(const GsvGrid *g, const GsvFarfieldTD *f, int step, hipStream_t st);
 hipError_t gsv_launch_dft(const GsvGrid *g, const GsvDft *d, int step, hipStream_t st);
+hipError_t gsv_launch_box_sum(const GsvGrid *g, const GsvBoxSum *b, int step, hipStream_t st);
 #ifdef __cplusplus
 }
 #endif

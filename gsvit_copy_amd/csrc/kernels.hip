@@ -20,8 +20,10 @@
  *   k_record_points       point traces, src3d/output.c:340-352
  *   k_farfield_td         time-domain Kirchhoff bins, src3d/farfield.c:174-407
  *   k_dft_faces           running DFT of tangential fields on the NFFF box
+ *   k_box_sum             OUT_SUM box sums of E^2, src3d/output.c:259-306
  */
 #include <hip/hip_runtime.h>
+#include "gsvit_settings.h"
 #include "engine.h"
 #include "kernels.h"
 
@@ -828,6 +830,46 @@ k_dft_faces(GsvGrid g, GsvDft d, int step)
     }
 }
 
+/* box sum of E^2 (modes of SvSumType); double accumulation, one atomic per block */
+__global__ __launch_bounds__(256) void
+k_box_sum(GsvGrid g, GsvBoxSum b, int step)
+{
+    __shared__ double red[256];
+    const long long n = (long long)b.ni * b.nj * b.nk;
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (long long q = (long long)blockIdx.x * blockDim.x + t; q < n; q += (long long)gridDim.x * blockDim.x) {
+        const float w = b.w ? b.w[q] : 1.f;
+        if (w == 0.f)
+            continue;
+        const int i = (int)(q % b.ni);
+        const long long r = q / b.ni;
+        const int j = (int)(r % b.nj), k = (int)(r / b.nj);
+        const long long c = (long long)(i + b.i0) + (long long)g.px * (j + b.j0) + g.plane * (k + b.k0);
+        double v = 0.0;
+        if (b.mode != SV_SUM_EY && b.mode != SV_SUM_EZ) { const double e = g.ex[c]; v += e * e; }
+        if (b.mode != SV_SUM_EX && b.mode != SV_SUM_EZ) { const double e = g.ey[c]; v += e * e; }
+        if (b.mode != SV_SUM_EX && b.mode != SV_SUM_EY) { const double e = g.ez[c]; v += e * e; }
+        if (b.mode == SV_SUM_MAX)
+            acc = v > acc ? v : acc;
+        else
+            acc += (double)w * v;
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (t < h)
+            red[t] = (b.mode == SV_SUM_MAX) ? fmax(red[t], red[t + h]) : red[t] + red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (b.mode == SV_SUM_MAX)   /* non-negative doubles order like their bit patterns */
+            atomicMax((unsigned long long *)(b.out + step), (unsigned long long)__double_as_longlong(red[0]));
+        else
+            atomicAdd(b.out + step, red[0]);
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* launchers                                                            */
 /* ------------------------------------------------------------------ */
@@ -1029,4 +1071,17 @@ gsv_copy_probe(long long nbytes, int reps, double *ms_per_copy)
     hipFree(a);
     hipFree(b);
     return hipGetLastError() != hipSuccess;
+}
+
+extern "C" hipError_t
+gsv_launch_box_sum(const GsvGrid *g, const GsvBoxSum *b, int step, hipStream_t st)
+{
+    const long long n = (long long)b->ni * b->nj * b->nk;
+    if (n <= 0)
+        return hipSuccess;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 2048)
+        blocks = 2048;
+    hipLaunchKernelGGL(k_box_sum, dim3((unsigned)blocks), dim3(256), 0, st, *g, *b, step);
+    return hipGetLastError();
 }

@@ -1,7 +1,7 @@
 /*
  * output.c -- text logs of the step loop: point traces and time-domain far
  * field, in the reference formats (src3d/output.c:51-99 is_output,
- * :645-681 point logs "%d %g" / "%d %g %g %g %g %g %g", :1266-1271 far-field
+ * :645-681 point logs "%d %g" / "%d %g %g %g %g %g %g", :690-709 sum logs, :1266-1271 far-field
  * log "%d %g %g %g" of the negated bins).  The running-DFT face data, which
  * has no reference counterpart, is written as one line per face cell.
  */
@@ -18,6 +18,12 @@ is_output(SvPool *mp, SvSet *set)
         if (set->so.pnts[i].step == 0)
             return 0;
         if (set->sc.step_act % set->so.pnts[i].step == 0)
+            return 1;
+    }
+    for (i = 0; i < set->so.nsums; i++) {
+        if (set->so.sums[i].step == 0)
+            return 0;
+        if (set->sc.step_act % set->so.sums[i].step == 0)
             return 1;
     }
     return 0;
@@ -47,6 +53,26 @@ sv_pool_output(SvPool *mp, SvSet *set)
             else
                 fprintf(fh, "%d %g\n", j, v[o->component]);
         }
+        fclose(fh);
+    }
+
+    /* sum logs "%d %g", every step of the run, scaled by the cell volume
+     * except for the maximum (src3d/output.c:690-709; the reference tests
+     * the `step` of a stale index there, the log's own `step` is used here) */
+    for (n = 0; n < set->so.nsums; n++) {
+        SvOutputSum *o = set->so.sums + n;
+        double vol = set->sp.dx * set->sp.dy * set->sp.dz;
+        if (o->step == 0)
+            continue;
+        if (set->sc.step_act % o->step != 0 && set->sc.step_act < (set->sc.nsteps - 1))
+            continue;
+        fh = fopen(o->filename, "w");
+        if (!fh) {
+            fprintf(stderr, "Error: sum: cannot open file %s for writing\n", o->filename);
+            continue;
+        }
+        for (j = 0; j < set->sc.nsteps; j++)
+            fprintf(fh, "%d %g\n", j, mp->out->outsumdata[n][j] * (o->component == SV_SUM_MAX ? 1.0 : vol));
         fclose(fh);
     }
 

@@ -1153,6 +1153,20 @@ init_and_make_plan(SvSet *set)
         }
     }
 
+    if (set->so.nsums) {
+        mp->out->outsumdata = (double **)calloc(set->so.nsums, sizeof(double *));
+        for (n = 0; n < set->so.nsums; n++) {
+            SvOutputSum *o = set->so.sums + n;
+            if (o->box_i0 < 0 || o->box_j0 < 0 || o->box_k0 < 0 || o->box_in > xres || o->box_jn > yres
+                || o->box_kn > zres) {
+                fprintf(stderr, "Error: sum box %d (%d %d %d - %d %d %d) is outside the pool\n", n, o->box_i0,
+                        o->box_j0, o->box_k0, o->box_in, o->box_jn, o->box_kn);
+                return NULL;
+            }
+            mp->out->outsumdata[n] = (double *)calloc((size_t)set->sc.nsteps + 1, sizeof(double));
+        }
+    }
+
     if (set->sf.nrs || set->sf.ndft) {
         SvSetFarfield *f = &set->sf;
         if (f->box_i0 < 1 || f->box_j0 < 1 || f->box_k0 < 1 || f->box_in >= xres - 1 || f->box_jn >= yres - 1
@@ -1198,6 +1212,11 @@ sv_pool_free(SvPool *mp, SvSet *set)
             for (n = 0; n < mp->set->so.npnts; n++)
                 free(mp->out->outpointdata[n]);
             free(mp->out->outpointdata);
+        }
+        if (mp->out->outsumdata) {
+            for (n = 0; n < mp->set->so.nsums; n++)
+                free(mp->out->outsumdata[n]);
+            free(mp->out->outsumdata);
         }
         free(mp->out);
     }

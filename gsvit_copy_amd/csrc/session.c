@@ -162,6 +162,21 @@ gsv_get_dft(void *h, double *out)
     return cnt;
 }
 
+long
+gsv_get_sums(void *h, double *out)
+{
+    GsvSession *s = (GsvSession *)h;
+    int n, nst = s->set.sc.nsteps;
+    long cnt = (long)s->set.so.nsums * nst;
+    if (!out || !cnt)
+        return cnt;
+    if (gsv_engine_fetch_aux(s->mp->gpu))
+        return -1;
+    for (n = 0; n < s->set.so.nsums; n++)
+        memcpy(out + (size_t)n * nst, s->mp->out->outsumdata[n], (size_t)nst * sizeof(double));
+    return cnt;
+}
+
 void
 gsv_get_maps(void *h, int *mat, int *bnds)
 {

@@ -400,6 +400,34 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             if (get_rest_of_line(fr, o->filebase, SV_NAME_LEN))
                 goto fail;
             set->so.npnts++;
+        } else if (!strcmp(key, "OUT_SUM")) {
+            /* src3d/settings.c:4937-4981: component step i0 j0 k0 in jn kn eps mu sigma sigast file */
+            SvOutputSum *o;
+            char comp[16];
+            set->so.sums = (SvOutputSum *)realloc(set->so.sums, (set->so.nsums + 1) * sizeof(SvOutputSum));
+            o = set->so.sums + set->so.nsums;
+            memset(o, 0, sizeof(*o));
+            if (fscanf(fr, "%15s", comp) != 1)
+                goto fail;
+            if (!strcmp(comp, "Ex") || !strcmp(comp, "ex")) o->component = SV_SUM_EX;
+            else if (!strcmp(comp, "Ey") || !strcmp(comp, "ey")) o->component = SV_SUM_EY;
+            else if (!strcmp(comp, "Ez") || !strcmp(comp, "ez")) o->component = SV_SUM_EZ;
+            else if (!strcmp(comp, "All") || !strcmp(comp, "all")) o->component = SV_SUM_ALL;
+            else if (!strcmp(comp, "Abs") || !strcmp(comp, "abs")) o->component = SV_SUM_ABS;
+            else if (!strcmp(comp, "Max") || !strcmp(comp, "max")) o->component = SV_SUM_MAX;
+            else {
+                fprintf(stderr, "Error parsing field component (key %s)\n", key);
+                goto fail;
+            }
+            if (get_int(fr, &o->step, key) || get_int(fr, &o->box_i0, key) || get_int(fr, &o->box_j0, key)
+                || get_int(fr, &o->box_k0, key) || get_int(fr, &o->box_in, key) || get_int(fr, &o->box_jn, key)
+                || get_int(fr, &o->box_kn, key) || get_double(fr, &o->layered_epsilon, key)
+                || get_double(fr, &o->layered_mu, key) || get_double(fr, &o->layered_sigma, key)
+                || get_double(fr, &o->layered_sigast, key))
+                goto fail;
+            if (get_rest_of_line(fr, o->filename, SV_NAME_LEN))
+                goto fail;
+            set->so.nsums++;
         } else if (!strcmp(key, "OUT_IMAGE") || !strcmp(key, "OUT_FILE") || !strcmp(key, "OUT_PLANE")
                    || !strcmp(key, "OUT_VOLUME")) {
             skip_output_keyword(fr, key);

@@ -40,6 +40,8 @@ def library():
         L.gsv_get_dft.restype = ctypes.c_long
         L.gsv_get_dft.argtypes = [vp, dp]
         L.gsv_get_maps.argtypes = [vp, ip, ip]
+        L.gsv_get_sums.restype = ctypes.c_long
+        L.gsv_get_sums.argtypes = [vp, dp]
         L.gsv_set_timing.argtypes = [vp, ctypes.c_int]
         L.gsv_kernel_times.argtypes = [vp, dp]
         L.gsv_close.argtypes = [vp]
@@ -128,6 +130,15 @@ class Session:
         if n > 0 and self.L.gsv_get_dft(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) < 0:
             raise RuntimeError("DFT download failed")
         return out[:n]
+
+    def sums(self):
+        """OUT_SUM values per box and step, [nsums, nsteps]."""
+        inf = self.info()
+        n = self.L.gsv_get_sums(self.h, None)
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        if n > 0 and self.L.gsv_get_sums(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) < 0:
+            raise RuntimeError("sum download failed")
+        return out[:n].reshape(-1, inf["nsteps"]) if n > 0 else np.zeros((0, inf["nsteps"]))
 
     def maps(self):
         inf = self.info()

@@ -34,6 +34,9 @@ int gsv_get_points(void *session, double *out);
 int gsv_get_farfield(void *session, double *out);
 /* running-DFT accumulators; returns the element count (out may be NULL) */
 long gsv_get_dft(void *session, double *out);
+/* OUT_SUM values as [nsums][nsteps] (before the cell-volume scaling of the
+ * log file); returns the element count (out may be NULL), -1 on error */
+long gsv_get_sums(void *session, double *out);
 /* planner maps, [i][j][k]: tabulated material index (-1 without a table) and internal-boundary flags */
 void gsv_get_maps(void *session, int *mat, int *bnds);
 /* per-launch HIP-event timing of the H and E volume kernels */

@@ -174,11 +174,22 @@ typedef struct {
     char filebase[SV_NAME_LEN];
 } SvOutputPar;
 
+/* sum modes, src3d/settings.h:94-99 */
+typedef enum {
+    SV_SUM_EX = 0,          /* sum of Ex^2 over the box */
+    SV_SUM_EY = 1,
+    SV_SUM_EZ = 2,
+    SV_SUM_ALL = 3,         /* sum of |E|^2 */
+    SV_SUM_ABS = 4,         /* sum of sigma |E|^2 */
+    SV_SUM_MAX = 5          /* max of |E|^2 */
+} SvSumType;
+
+/* OUT_SUM, src3d/settings.h:596-610 */
 typedef struct {
-    int component;          /* 0 = E^2, 1 = sigma*E^2 (as the reference's sum modes) */
-    int step;
+    int component;          /* SvSumType */
+    int step;               /* log written every `step` steps (0 = never) */
     int box_i0, box_j0, box_k0, box_in, box_jn, box_kn;
-    double layered_epsilon; /* absolute; -1*EPSILON_0 = every voxel */
+    double layered_epsilon; /* absolute permittivity a voxel must match within EPSILON_0/100; -1 = every voxel */
     double layered_mu, layered_sigma, layered_sigast;
     char filename[SV_NAME_LEN];
 } SvOutputSum;

@@ -22,7 +22,10 @@
  *   1-D auxiliary line ...... src3d/1dpool.c:83-162
  *   cubic line interpolation  src3d/dline.c:89-112
  *   time-domain far field ... src3d/farfield.c:174-407
+ *   running DFT on the NFFF box (no reference counterpart; pinned by a numpy
+ *                             DFT of recorded point traces in the tests)
  *   point traces ............ src3d/output.c:340-352
+ *   box sums ................ src3d/output.c:259-306
  *   step order .............. src3d/svit.c:217-237
  * The parameter reader and planner are the product's (settings.c, plan.c);
  * they are pinned independently by the golden traces and the brute-force
@@ -1171,6 +1174,90 @@ farfield(Oracle *o)
     }
 }
 
+/* box sums of E^2, src3d/output.c:259-306: voxels whose permittivity
+ * matches layered_epsilon within EPSILON_0/100 (or every voxel for -1) */
+static void
+getsum(Oracle *o)
+{
+    const SvSet *set = &o->set;
+    int n;
+    for (n = 0; n < set->so.nsums; n++) {
+        const SvOutputSum *p = set->so.sums + n;
+        double *out = o->mp->out->outsumdata[n] + set->sc.step_act;
+        int i, j, k;
+        for (i = p->box_i0; i < p->box_in; i++)
+            for (j = p->box_j0; j < p->box_jn; j++)
+                for (k = p->box_k0; k < p->box_kn; k++) {
+                    double eps, sig, val = 0;
+                    size_t c = IDX(o, i, j, k);
+                    sv_pool_epsilon_sigma(o->mp, i, j, k, &eps, &sig);
+                    if (!(p->layered_epsilon == -1 || fabs(p->layered_epsilon - eps) < EPS0 / 100.0))
+                        continue;
+                    if (p->component != SV_SUM_EY && p->component != SV_SUM_EZ)
+                        val += (double)o->ex[c] * o->ex[c];
+                    if (p->component != SV_SUM_EX && p->component != SV_SUM_EZ)
+                        val += (double)o->ey[c] * o->ey[c];
+                    if (p->component != SV_SUM_EX && p->component != SV_SUM_EY)
+                        val += (double)o->ez[c] * o->ez[c];
+                    if (p->component == SV_SUM_MAX)
+                        *out = val > *out ? val : *out;
+                    else if (p->component == SV_SUM_ABS)
+                        *out += sig * val;
+                    else
+                        *out += val;
+                }
+    }
+}
+
+/* running DFT of the tangential fields on the NFFF box faces (no reference
+ * counterpart): per face cell and frequency w, acc += F(t) exp(-i w t) for
+ * the two tangential E components at t = (step+1) dt and the two tangential
+ * H components at t = (step+1/2) dt.  Cell order: x faces (j, k loops; i0 then
+ * in), y faces (i, k), z faces (i, j); 8 doubles per cell and frequency. */
+static void
+running_dft(Oracle *o)
+{
+    const SvSetFarfield *sf = &o->set.sf;
+    SvFarfield *ff = o->mp->farfield;
+    const int i0 = sf->box_i0, i1 = sf->box_in, j0 = sf->box_j0, j1 = sf->box_jn, k0 = sf->box_k0, k1 = sf->box_kn;
+    const int ix = i1 - i0, iy = j1 - j0, iz = k1 - k0;
+    const double te = (o->set.sc.step_act + 1) * o->set.plan.dt, th = (o->set.sc.step_act + 0.5) * o->set.plan.dt;
+    long q;
+    if (!ff->ndft)
+        return;
+    #pragma omp parallel for
+    for (q = 0; q < ff->dft_ncells; q++) {
+        long r = q / 2;
+        int far = (int)(q % 2), i, j, k, w;
+        double v[4];
+        size_t c;
+        if (r < (long)iy * iz) {
+            j = j0 + (int)(r / iz); k = k0 + (int)(r % iz); i = far ? i1 : i0;
+            c = IDX(o, i, j, k);
+            v[0] = o->ey[c]; v[1] = o->ez[c]; v[2] = o->hy[c]; v[3] = o->hz[c];
+        } else if (r < (long)iy * iz + (long)ix * iz) {
+            r -= (long)iy * iz;
+            i = i0 + (int)(r / iz); k = k0 + (int)(r % iz); j = far ? j1 : j0;
+            c = IDX(o, i, j, k);
+            v[0] = o->ex[c]; v[1] = o->ez[c]; v[2] = o->hx[c]; v[3] = o->hz[c];
+        } else {
+            r -= (long)iy * iz + (long)ix * iz;
+            i = i0 + (int)(r / iy); j = j0 + (int)(r % iy); k = far ? k1 : k0;
+            c = IDX(o, i, j, k);
+            v[0] = o->ex[c]; v[1] = o->ey[c]; v[2] = o->hx[c]; v[3] = o->hy[c];
+        }
+        for (w = 0; w < ff->ndft; w++) {
+            double *a = ff->dft_data + ((size_t)q * ff->ndft + w) * 8;
+            double ce = cos(ff->dft_omega[w] * te), se = -sin(ff->dft_omega[w] * te);
+            double ch = cos(ff->dft_omega[w] * th), sh = -sin(ff->dft_omega[w] * th);
+            a[0] += v[0] * ce; a[1] += v[0] * se;
+            a[2] += v[1] * ce; a[3] += v[1] * se;
+            a[4] += v[2] * ch; a[5] += v[2] * sh;
+            a[6] += v[3] * ch; a[7] += v[3] * sh;
+        }
+    }
+}
+
 static void
 getpoints(Oracle *o)
 {
@@ -1203,7 +1290,9 @@ oracle_step(void *handle, int nsteps)
         ystep_e(o);
         boundary_estep(o);
         source_estep(o);
+        getsum(o);
         farfield(o);
+        running_dft(o);
         getpoints(o);
         o->set.sc.step_act++;
     }
@@ -1257,6 +1346,28 @@ oracle_get_farfield(void *handle, double *out)
 }
 
 /* tabulated material map ([i][j][k], -1 everywhere without a table) and boundary map */
+/* box sums as [nsums][nsteps] */
+void
+oracle_get_sums(void *handle, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    int n;
+    for (n = 0; n < o->set.so.nsums; n++)
+        memcpy(out + (size_t)n * o->set.sc.nsteps, o->mp->out->outsumdata[n], (size_t)o->set.sc.nsteps * sizeof(double));
+}
+
+/* running-DFT accumulators [ncells][ndft][8]; returns the element count (out may be NULL) */
+long
+oracle_get_dft(void *handle, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    SvFarfield *ff = o->mp->farfield;
+    long cnt = (long)ff->dft_ncells * ff->ndft * 8;
+    if (out && cnt)
+        memcpy(out, ff->dft_data, (size_t)cnt * sizeof(double));
+    return cnt;
+}
+
 void
 oracle_get_maps(void *handle, int *mat, int *bnds)
 {

@@ -1,11 +1,10 @@
 """Seeded parameter-file workloads for parity tests and the bench.
 
 Every function writes a GSvit parameter file plus its input files into a
-folder and returns the parameter-file path.  The workloads follow the
-configurations the engine is measured on:
-  config1  plain 3-D FDTD, 40 dielectric spheres, CPML, Ez dipole (128^3)
-  config2  Drude rough-surface scattering, TSF plane wave at 45 degrees,
-           running-DFT near-to-far-field box (512^3)
+folder and returns the parameter-file path.  The workloads:
+  sphere_scatter  plain 3-D FDTD, 40 dielectric spheres, CPML, Ez dipole (128^3)
+  rough_surface   Drude rough-surface scattering, TSF plane wave at 45 degrees,
+                  running-DFT near-to-far-field box (512^3)
 and small feature cases (CPML + ADE, TSF oblique, Liao + PEC, periodic, far field).
 """
 import math
@@ -61,8 +60,8 @@ def place_spheres(rng, n, lo, hi, rmin, rmax):
     return out
 
 
-def config1(folder, n=128, nsteps=1000, nspheres=40, seed=1, npoints=16):
-    """Plain 3-D FDTD (configs[0]): 10 nm cells, Courant 0.5, CPML 10 (m=3,
+def sphere_scatter(folder, n=128, nsteps=1000, nspheres=40, seed=1, npoints=16):
+    """Plain 3-D FDTD: 10 nm cells, Courant 0.5, CPML 10 (m=3,
     sigma_max = 0.8 (m+1)/(eta0 dx), kappa 1, alpha 0), 40 LINTAB spheres of
     radius U[5,15] cells and eps U[1,4], Ez dipole at the centre driven by a
     Gaussian-modulated sine (lambda0 = 500 nm, width 8 periods)."""
@@ -87,7 +86,7 @@ def config1(folder, n=128, nsteps=1000, nspheres=40, seed=1, npoints=16):
         lines.append("MEDIUM_VECTOR\nspheres.txt\n\n")
     for q, p in enumerate(pts):
         lines.append("OUT_POINT\nAll 1 %d %d %d point%02d\n\n" % (p[0], p[1], p[2], q))
-    path = os.path.join(folder, "config1.par")
+    path = os.path.join(folder, "sphere_scatter.par")
     with open(path, "w") as f:
         f.write("".join(lines))
     return path
@@ -125,10 +124,10 @@ def write_vtk_ids(path, ids):
 DRUDE_SILVER = "5 1 0 1.37e16 2.7e13 0 0 0 0 0 0 0 0"
 
 
-def config2(folder, n=512, nsteps=4000, dx=5e-9, depth=16, seed=2, rms_nm=20.0, corr_nm=100.0,
-            theta_deg=45.0, lam_center=600e-9, dft_lambdas=(500e-9, 600e-9, 700e-9), tsf_margin=4,
-            nfff=True):
-    """Rough-surface scattering (configs[1]): lower half a Drude metal
+def rough_surface(folder, n=512, nsteps=4000, dx=5e-9, depth=16, seed=2, rms_nm=20.0, corr_nm=100.0,
+                  theta_deg=45.0, lam_center=600e-9, dft_lambdas=(500e-9, 600e-9, 700e-9), tsf_margin=4,
+                  nfff=True):
+    """Rough-surface scattering: lower half a Drude metal
     (eps_inf 1, omega_p 1.37e16 rad/s, gamma 2.7e13 1/s, ADE) below a
     Gaussian-correlated interface (rms 20 nm, correlation 100 nm); CPML 16;
     a p-polarised TSF plane wave at 45 degrees travelling down onto the metal
@@ -159,7 +158,7 @@ def config2(folder, n=512, nsteps=4000, dx=5e-9, depth=16, seed=2, rms_nm=20.0, 
         zlow = int(n / 2 + 4 * rms_nm * 1e-9 / dx) + 8
         lines.append("NFFF\n%d %d %d %d %d %d 0 0\n\n" % (f0, f0, zlow, f1, f1, f1))
         lines.append("NFFF_DFT\n%d %s dft.txt\n\n" % (len(dft_lambdas), " ".join("%.6g" % l for l in dft_lambdas)))
-    path = os.path.join(folder, "config2.par")
+    path = os.path.join(folder, "rough_surface.par")
     with open(path, "w") as f:
         f.write("".join(lines))
     return path

@@ -32,6 +32,9 @@ def lib(precision="f64"):
         L.oracle_get_points.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_farfield.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_maps.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.oracle_get_dft.restype = ctypes.c_long
+        L.oracle_get_dft.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_get_sums.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_real_size.restype = ctypes.c_int
         L.oracle_set_option.restype = ctypes.c_int
         L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
@@ -103,6 +106,20 @@ class Oracle:
         if inf["nrs"]:
             self.L.oracle_get_farfield(self.h, _dptr(out))
         return out[: inf["nrs"]]
+
+    def sums(self, nsums):
+        inf = self.info()
+        out = np.zeros((max(nsums, 1), inf["nsteps"]), dtype=np.float64)
+        if nsums:
+            self.L.oracle_get_sums(self.h, _dptr(out))
+        return out[:nsums]
+
+    def dft(self):
+        n = self.L.oracle_get_dft(self.h, None)
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        if n:
+            self.L.oracle_get_dft(self.h, _dptr(out))
+        return out[:n]
 
     def maps(self):
         inf = self.info()

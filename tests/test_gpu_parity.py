@@ -11,6 +11,8 @@ import cases
 from oracle_lib import Oracle
 from gsvit_copy_amd import Session
 from selftest_recipe import write_selftest, read_golden
+from test_running_dft import dft_case
+import test_box_sums
 
 pytestmark = pytest.mark.gpu
 
@@ -100,15 +102,52 @@ def test_gpu_reproduces_reference_selftest(test_id, tmp_path):
     s.close()
 
 
-def test_config1_reduced_against_oracle(tmp_path):
-    """configs[0] geometry at 64^3 and 300 steps (the full 128^3 x 1000 run is in test_config1_full)."""
-    par = cases.config1(str(tmp_path / "c1"), n=64, nsteps=300, nspheres=12)
+def test_sphere_scatter_reduced_against_oracle(tmp_path):
+    """The 40-sphere dipole workload's geometry at 64^3, 12 spheres, 300 steps; maps bit-exact."""
+    par = cases.sphere_scatter(str(tmp_path / "c1"), n=64, nsteps=300, nspheres=12)
     o, s = run_both(par)
     ref, got = o.fields(), s.fields()
     errs = field_errors(got, ref)
-    print("config1 reduced", errs, s.info())
+    print("sphere scatter 64^3", errs, s.info())
     assert max(errs.values()) <= TOL
     assert max(trace_errors(s.points(), o.points())) <= TOL
     mo, bo = o.maps()
     ms, bs = s.maps()
     assert np.array_equal(mo, ms) and np.array_equal(bo, bs)
+
+
+def test_running_dft_against_oracle(tmp_path):
+    """Running-DFT accumulators on all six NFFF box faces at two wavelengths."""
+    par = dft_case(str(tmp_path / "dft"))
+    o, s = run_both(par)
+    ref, got = o.dft(), s.dft()
+    assert ref.shape == got.shape and ref.size > 0
+    e = rel_l2(got, ref)
+    print("running DFT rel L2", e)
+    assert e <= TOL
+    o.close()
+    s.close()
+
+
+def test_box_sums_against_oracle(tmp_path):
+    """OUT_SUM in every mode, every step; the log files in the reference format."""
+    folder = tmp_path / "sums"
+    par = test_box_sums.sums_case(str(folder))
+    o = Oracle(par)
+    o.step()
+    s = Session(par, write_outputs=True)
+    s.run()
+    ref, got = o.sums(len(test_box_sums.SUMS)), s.sums()
+    assert got.shape == ref.shape
+    errs = [rel_l2(got[q], ref[q]) for q in range(ref.shape[0])]
+    print("box sums rel L2", errs)
+    assert max(errs) <= TOL
+    vol = (10e-9) ** 3
+    for q, (mode, _, _) in enumerate(test_box_sums.SUMS):
+        rows = np.loadtxt(str(folder / ("sum%d" % q)))
+        assert rows.shape == (ref.shape[1], 2)
+        assert np.array_equal(rows[:, 0], np.arange(ref.shape[1]))
+        scale = 1.0 if mode == "Max" else vol
+        assert np.allclose(rows[:, 1], got[q] * scale, rtol=1e-5, atol=0)
+    o.close()
+    s.close()
