@@ -13,7 +13,11 @@
  *               the PEC zero mask, computed on the host in double and rounded.
  *   CPML psi    stored only on the boundary slabs (x slabs [k][j][depth],
  *               y slabs [k][depth][px], z slabs [depth][ny][px]).
- *   ADE state   full-grid float arrays, touched only in dispersive cells.
+ *   ADE state   full-grid float arrays with the same guard planes, touched
+ *               only in dispersive cells.
+ * The volume kernels address every per-cell array by one 32-bit byte offset
+ * from the guard-plane start (one offset register shared by all arrays);
+ * grids whose field arrays exceed 4 GiB use 64-bit offsets.
  */
 #ifndef GSV_ENGINE_H
 #define GSV_ENGINE_H
@@ -71,6 +75,7 @@ typedef struct {
     int px;                 /* row pitch (floats) */
     long long plane;        /* px * ny */
     float *ex, *ey, *ez, *hx, *hy, *hz;   /* base (k = 0) pointers */
+    float *exg, *eyg, *ezg, *hxg, *hyg, *hzg;   /* allocation starts (guard plane k = -1) */
     const void *ecls;       /* uint8_t or uint16_t per cell */
     const void *hcls;
     const GsvECoef *etab;
@@ -83,6 +88,8 @@ typedef struct {
     /* ADE */
     int ade, ade_poles;
     const GsvAde *adem;     /* ADE material table in device memory */
+    /* ADE state: same layout as the fields including the guard planes; these
+     * are allocation starts, so cell (i,j,k) is at the fields' guard offset */
     float *exp_, *eyp, *ezp, *dpx, *dpy, *dpz, *dpxp, *dpyp, *dpzp;
     float *px0, *px1, *py0, *py1, *pz0, *pz1, *pxp0, *pxp1, *pyp0, *pyp1, *pzp0, *pzp1;
 } GsvGrid;

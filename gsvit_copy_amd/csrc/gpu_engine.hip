@@ -892,6 +892,8 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
         CK(hipMemsetAsync(E->fbase[q], 0, E->field_floats * 4, E->st));
         *fp[q] = E->fbase[q] + g.plane;
     }
+    g.exg = E->fbase[0]; g.eyg = E->fbase[1]; g.ezg = E->fbase[2];
+    g.hxg = E->fbase[3]; g.hyg = E->fbase[4]; g.hzg = E->fbase[5];
     {
         uint8_t *d;
         if (dupload(E, &d, E->ecls_host.data(), E->ecls_host.size())) return 1;
@@ -914,7 +916,7 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
         g.adem = am;
     }
     if (g.ade) {
-        size_t n = (size_t)g.plane * g.nz;
+        size_t n = E->field_floats;   /* same guard layout as the fields */
         float **arr[] = { &g.exp_, &g.eyp, &g.ezp, &g.dpx, &g.dpy, &g.dpz, &g.dpxp, &g.dpyp, &g.dpzp };
         for (auto a : arr) {
             if (dmalloc(E, a, n)) return 1;

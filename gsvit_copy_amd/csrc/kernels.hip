@@ -30,10 +30,39 @@
 #define LD(p) __builtin_nontemporal_load(p)
 
 /* ------------------------------------------------------------------ */
+/* addressing of the volume kernels                                     */
+/* ------------------------------------------------------------------ */
+
+/* Every per-cell array is addressed by a byte offset from its allocation
+ * start (the guard plane), shared by all arrays: with a 32-bit offset the
+ * loads take the uniform-base + 32-bit-offset form, so the z-march carries
+ * one offset register instead of a 64-bit pointer per array. */
+template <typename OFF>
+__device__ __forceinline__ float
+ldf(const float *base, OFF off)
+{
+    return *(const float *)((const char *)base + off);
+}
+
+template <typename OFF>
+__device__ __forceinline__ void
+stf(float *base, OFF off, float v)
+{
+    *(float *)((char *)base + off) = v;
+}
+
+template <typename ID, typename OFF>
+__device__ __forceinline__ unsigned
+ldid(const void *base, OFF off)
+{
+    return *(const ID *)((const char *)base + off);
+}
+
+/* ------------------------------------------------------------------ */
 /* H update                                                             */
 /* ------------------------------------------------------------------ */
 
-template <typename ID>
+template <typename ID, typename OFF>
 __global__ __launch_bounds__(512) void
 k_hstep(GsvGrid g, int kchunk)
 {
@@ -44,10 +73,12 @@ k_hstep(GsvGrid g, int kchunk)
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int k0 = blockIdx.z * kchunk;
     const int k1 = min(k0 + kchunk, nz);
-    const long long P = g.plane;
     const int px = g.px;
-    const ID *cls = (const ID *)g.hcls;
-    long long c = (long long)i + (long long)px * j + P * k0;
+    const OFF dP = (OFF)g.plane * 4, dJ = (OFF)px * 4;
+    const OFF dPc = (OFF)g.plane * sizeof(ID);
+    /* o: field byte offset of (i, j, k) from the guard plane; oc: class id offset */
+    OFF o = ((OFF)i + (OFF)px * j + (OFF)g.plane * (k0 + 1)) * 4;
+    OFF oc = ((OFF)i + (OFF)px * j + (OFF)g.plane * k0) * sizeof(ID);
 
     const float rkx = g.rkhx[i], rky = g.rkhy[j];
     const bool iin = (i >= 1 && i < nx - 1), jin = (j >= 1 && j < ny - 1);
@@ -62,15 +93,15 @@ k_hstep(GsvGrid g, int kchunk)
     const bool zy = F3.on && j == ny - 1 && iin;
     const bool ijin = iin && jin;
 
-    float exk = g.ex[c], eyk = g.ey[c];
-    for (int k = k0; k < k1; ++k, c += P) {
-        const float ezk = g.ez[c];
-        const float exk1 = g.ex[c + P], eyk1 = g.ey[c + P];
-        const float ezj1 = g.ez[c + px], exj1 = g.ex[c + px];
-        const float ezi1 = g.ez[c + 1], eyi1 = g.ey[c + 1];
-        const GsvHCoef hc = g.htab[cls[c]];
+    float exk = ldf(g.exg, o), eyk = ldf(g.eyg, o);
+    for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
+        const float ezk = ldf(g.ezg, o);
+        const float exk1 = ldf(g.exg, o + dP), eyk1 = ldf(g.eyg, o + dP);
+        const float ezj1 = ldf(g.ezg, o + dJ), exj1 = ldf(g.exg, o + dJ);
+        const float ezi1 = ldf(g.ezg, o + 4), eyi1 = ldf(g.eyg, o + 4);
+        const GsvHCoef hc = g.htab[ldid<ID>(g.hcls, oc)];
         const float rkz = g.rkhz[k];
-        float hx = g.hx[c], hy = g.hy[c], hz = g.hz[c];
+        float hx = ldf(g.hxg, o), hy = ldf(g.hyg, o), hz = ldf(g.hzg, o);
         const bool kin = (k >= 1 && k < nz - 1);
 
         if (!(j == ny - 1 || k == nz - 1))
@@ -80,67 +111,68 @@ k_hstep(GsvGrid g, int kchunk)
         if (!(i == nx - 1 || j == ny - 1))
             hz = hc.da * hz + hc.db[2] * ((exj1 - exk) * rky - (eyi1 - eyk) * rkx);
 
+        /* CPML H psi and correction, src3d/boundary.c:347-518 */
         if (kin) {
             if (sx0) {
-                const long long p = ((long long)k * ny + j) * F0.depth + i;
+                const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 4;
                 const float b = F0.bh[i], cc = F0.ch_d[i];
-                const float a1 = b * F0.ph1[p] + cc * (eyi1 - eyk);
-                const float a2 = b * F0.ph2[p] + cc * (ezi1 - ezk);
-                F0.ph1[p] = a1;
-                F0.ph2[p] = a2;
+                const float a1 = b * ldf(F0.ph1, p) + cc * (eyi1 - eyk);
+                const float a2 = b * ldf(F0.ph2, p) + cc * (ezi1 - ezk);
+                stf(F0.ph1, p, a1);
+                stf(F0.ph2, p, a2);
                 hy += g.dx * hc.dbc * a2;
                 hz -= g.dx * hc.dbc * a1;
             }
             if (sxn) {
-                const long long p = ((long long)k * ny + j) * F1.depth + irn;
+                const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + irn) * 4;
                 const float b = F1.bh[irn], cc = F1.ch_d[irn];
-                const float a1 = b * F1.ph1[p] + cc * (eyi1 - eyk);
-                const float a2 = b * F1.ph2[p] + cc * (ezi1 - ezk);
-                F1.ph1[p] = a1;
-                F1.ph2[p] = a2;
+                const float a1 = b * ldf(F1.ph1, p) + cc * (eyi1 - eyk);
+                const float a2 = b * ldf(F1.ph2, p) + cc * (ezi1 - ezk);
+                stf(F1.ph1, p, a1);
+                stf(F1.ph2, p, a2);
                 hy += g.dx * hc.dbc * a2;
                 hz -= g.dx * hc.dbc * a1;
             }
             if (sy0) {
-                const long long p = ((long long)k * F2.depth + j) * px + i;
+                const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 4;
                 const float b = F2.bh[j], cc = F2.ch_d[j];
-                const float a1 = b * F2.ph1[p] + cc * (ezj1 - ezk);
-                const float a2 = b * F2.ph2[p] + cc * (exj1 - exk);
-                F2.ph1[p] = a1;
-                F2.ph2[p] = a2;
+                const float a1 = b * ldf(F2.ph1, p) + cc * (ezj1 - ezk);
+                const float a2 = b * ldf(F2.ph2, p) + cc * (exj1 - exk);
+                stf(F2.ph1, p, a1);
+                stf(F2.ph2, p, a2);
                 hx -= g.dy * hc.dbc * a1;
                 hz += g.dy * hc.dbc * a2;
             }
             if (syn) {
-                const long long p = ((long long)k * F3.depth + jrn) * px + i;
+                const OFF p = (OFF)(((unsigned)k * F3.depth + jrn) * px + i) * 4;
                 const float b = F3.bh[jrn], cc = F3.ch_d[jrn];
-                const float a1 = b * F3.ph1[p] + cc * (ezj1 - ezk);
-                const float a2 = b * F3.ph2[p] + cc * (exj1 - exk);
-                F3.ph1[p] = a1;
-                F3.ph2[p] = a2;
+                const float a1 = b * ldf(F3.ph1, p) + cc * (ezj1 - ezk);
+                const float a2 = b * ldf(F3.ph2, p) + cc * (exj1 - exk);
+                stf(F3.ph1, p, a1);
+                stf(F3.ph2, p, a2);
                 hx -= g.dy * hc.dbc * a1;
                 hz += g.dy * hc.dbc * a2;
             }
         }
         if (ijin) {
             if (F4.on && k < F4.depth - 1) {
-                const long long p = ((long long)k * ny + j) * px + i;
+                const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 4;
                 const float b = F4.bh[k], cc = F4.ch_d[k];
-                const float a1 = b * F4.ph1[p] + cc * (exk1 - exk);
-                const float a2 = b * F4.ph2[p] + cc * (eyk1 - eyk);
-                F4.ph1[p] = a1;
-                F4.ph2[p] = a2;
+                const float a1 = b * ldf(F4.ph1, p) + cc * (exk1 - exk);
+                const float a2 = b * ldf(F4.ph2, p) + cc * (eyk1 - eyk);
+                stf(F4.ph1, p, a1);
+                stf(F4.ph2, p, a2);
                 hy -= g.dz * hc.dbc * a1;
                 hx += g.dz * hc.dbc * a2;
             }
             if (F5.on && k >= nz - F5.depth + 1 && k < nz - 1) {
                 const int kr = k - (nz - F5.depth);
-                const long long p = ((long long)kr * ny + j) * px + i;
+                const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 4;
                 const float b = F5.bh[kr], cc = F5.ch_d[kr];
-                const float a1 = b * F5.ph1[p] + cc * (exk1 - exk);
-                const float a2 = b * F5.ph2[p] + cc * (eyk1 - eyk);
-                F5.ph1[p] = a1;
-                F5.ph2[p] = a2;
+                const float a1 = b * ldf(F5.ph1, p) + cc * (exk1 - exk);
+                const float a2 = b * ldf(F5.ph2, p) + cc * (eyk1 - eyk);
+                stf(F5.ph1, p, a1);
+                stf(F5.ph2, p, a2);
                 hy -= g.dz * hc.dbc * a1;
                 hx += g.dz * hc.dbc * a2;
             }
@@ -150,9 +182,9 @@ k_hstep(GsvGrid g, int kchunk)
             hy = 0.f;
             hz = 0.f;
         }
-        g.hx[c] = hx;
-        g.hy[c] = hy;
-        g.hz[c] = hz;
+        stf(g.hxg, o, hx);
+        stf(g.hyg, o, hy);
+        stf(g.hzg, o, hz);
         exk = exk1;
         eyk = eyk1;
     }
@@ -164,10 +196,10 @@ k_hstep(GsvGrid g, int kchunk)
 
 /* ADE branch of one cell (src3d/yee.c:914-1108); updates the components whose
  * flag is clear, runs the branch's own CPML psi update and advances the
- * polarisation state.  Rarely taken: dispersive cells and their +1 neighbours. */
-template <bool POLES>
+ * polarisation state.  Taken in dispersive cells and their +1 neighbours. */
+template <bool POLES, typename OFF>
 __device__ __forceinline__ void
-ade_cell(const GsvGrid &g, long long c, int i, int j, int k, unsigned fl, float &ex, float &ey, float &ez,
+ade_cell(const GsvGrid &g, OFF o, int i, int j, int k, unsigned fl, float &ex, float &ey, float &ez,
          float hx, float hy, float hz, float hxkm1, float hykm1, float hzi, float hyi, float hzj, float hxj,
          float rkx, float rky, float rkz)
 {
@@ -175,14 +207,16 @@ ade_cell(const GsvGrid &g, long long c, int i, int j, int k, unsigned fl, float 
     const GsvAde &m = g.adem[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
     const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
     const float vexp = ex, veyp = ey, vezp = ez;
-    const float expv = g.exp_[c], eypv = g.eyp[c], ezpv = g.ezp[c];
-    const float dpx = g.dpx[c], dpy = g.dpy[c], dpz = g.dpz[c];
-    const float dpxp = g.dpxp[c], dpyp = g.dpyp[c], dpzp = g.dpzp[c];
+    const float expv = ldf(g.exp_, o), eypv = ldf(g.eyp, o), ezpv = ldf(g.ezp, o);
+    const float dpx = ldf(g.dpx, o), dpy = ldf(g.dpy, o), dpz = ldf(g.dpz, o);
+    const float dpxp = ldf(g.dpxp, o), dpyp = ldf(g.dpyp, o), dpzp = ldf(g.dpzp, o);
     float px0 = 0, px1 = 0, py0 = 0, py1 = 0, pz0 = 0, pz1 = 0;
     float pxp0 = 0, pxp1 = 0, pyp0 = 0, pyp1 = 0, pzp0 = 0, pzp1 = 0;
     if (POLES && m.poles) {
-        px0 = g.px0[c]; px1 = g.px1[c]; py0 = g.py0[c]; py1 = g.py1[c]; pz0 = g.pz0[c]; pz1 = g.pz1[c];
-        pxp0 = g.pxp0[c]; pxp1 = g.pxp1[c]; pyp0 = g.pyp0[c]; pyp1 = g.pyp1[c]; pzp0 = g.pzp0[c]; pzp1 = g.pzp1[c];
+        px0 = ldf(g.px0, o); px1 = ldf(g.px1, o); py0 = ldf(g.py0, o);
+        py1 = ldf(g.py1, o); pz0 = ldf(g.pz0, o); pz1 = ldf(g.pz1, o);
+        pxp0 = ldf(g.pxp0, o); pxp1 = ldf(g.pxp1, o); pyp0 = ldf(g.pyp0, o);
+        pyp1 = ldf(g.pyp1, o); pzp0 = ldf(g.pzp0, o); pzp1 = ldf(g.pzp1, o);
     }
 
     if (j > 0 && k > 0 && !tx)
@@ -207,81 +241,82 @@ ade_cell(const GsvGrid &g, long long c, int i, int j, int k, unsigned fl, float 
     /* the branch's own CPML update; its x0 correction multiplies by a zero cb */
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
     if (F0.on && i > 1 && i < F0.depth) {
-        const long long p = ((long long)k * ny + j) * F0.depth + i;
-        F0.pe1[p] = F0.be[i] * F0.pe1[p] + F0.ce_raw[i] * (hz - hzi) / g.dy;
-        F0.pe2[p] = F0.be[i] * F0.pe2[p] + F0.ce_raw[i] * (hy - hyi) / g.dz;
+        const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 4;
+        stf(F0.pe1, p, F0.be[i] * ldf(F0.pe1, p) + F0.ce_raw[i] * (hz - hzi) / g.dy);
+        stf(F0.pe2, p, F0.be[i] * ldf(F0.pe2, p) + F0.ce_raw[i] * (hy - hyi) / g.dz);
     }
     if (F2.on && j > 1 && j < F2.depth) {
-        const long long p = ((long long)k * F2.depth + j) * px + i;
-        const float a1 = F2.be[j] * F2.pe1[p] + F2.ce_raw[j] * (hz - hzj) / g.dz;
-        const float a2 = F2.be[j] * F2.pe2[p] + F2.ce_raw[j] * (hx - hxj) / g.dx;
-        F2.pe1[p] = a1;
-        F2.pe2[p] = a2;
+        const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 4;
+        const float a1 = F2.be[j] * ldf(F2.pe1, p) + F2.ce_raw[j] * (hz - hzj) / g.dz;
+        const float a2 = F2.be[j] * ldf(F2.pe2, p) + F2.ce_raw[j] * (hx - hxj) / g.dx;
+        stf(F2.pe1, p, a1);
+        stf(F2.pe2, p, a2);
         if (!(i == 0 || k == 0) && !tx) ex += g.dx * m.cbm[0] * a1;
         if (!(i == 0 || k == 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
     }
     if (F4.on && k > 1 && k < F4.depth) {
-        const long long p = ((long long)k * ny + j) * px + i;
-        const float a1 = F4.be[k] * F4.pe1[p] + F4.ce_raw[k] * (hx - hxkm1) / g.dy;
-        const float a2 = F4.be[k] * F4.pe2[p] + F4.ce_raw[k] * (hy - hykm1) / g.dx;
-        F4.pe1[p] = a1;
-        F4.pe2[p] = a2;
+        const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 4;
+        const float a1 = F4.be[k] * ldf(F4.pe1, p) + F4.ce_raw[k] * (hx - hxkm1) / g.dy;
+        const float a2 = F4.be[k] * ldf(F4.pe2, p) + F4.ce_raw[k] * (hy - hykm1) / g.dx;
+        stf(F4.pe1, p, a1);
+        stf(F4.pe2, p, a2);
         if (!(i == 0 || j == 0) && !ty) ey += g.dy * m.cbm[1] * a1;
         if (!(i == 0 || j == 0) && !tx) ex -= g.dx * m.cbm[0] * a2;
     }
     if (F1.on && i >= nx - F1.depth + 1) {
         const int ir = i - (nx - F1.depth);
-        const long long p = ((long long)k * ny + j) * F1.depth + ir;
-        const float a1 = F1.be[ir] * F1.pe1[p] + F1.ce_raw[ir] * (hz - hzi) / g.dy;
-        const float a2 = F1.be[ir] * F1.pe2[p] + F1.ce_raw[ir] * (hy - hyi) / g.dz;
-        F1.pe1[p] = a1;
-        F1.pe2[p] = a2;
+        const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 4;
+        const float a1 = F1.be[ir] * ldf(F1.pe1, p) + F1.ce_raw[ir] * (hz - hzi) / g.dy;
+        const float a2 = F1.be[ir] * ldf(F1.pe2, p) + F1.ce_raw[ir] * (hy - hyi) / g.dz;
+        stf(F1.pe1, p, a1);
+        stf(F1.pe2, p, a2);
         if (!(j == 0 || k == 0) && !ty) ey -= g.dy * m.cbm[1] * a1;
         if (!(j == 0 || k == 0) && !tz) ez += g.dz * m.cbm[2] * a2;
     }
     if (F3.on && j >= ny - F3.depth + 1) {
         const int jr = j - (ny - F3.depth);
-        const long long p = ((long long)k * F3.depth + jr) * px + i;
-        const float a1 = F3.be[jr] * F3.pe1[p] + F3.ce_raw[jr] * (hz - hzj) / g.dx;
-        const float a2 = F3.be[jr] * F3.pe2[p] + F3.ce_raw[jr] * (hx - hxj) / g.dz;
-        F3.pe1[p] = a1;
-        F3.pe2[p] = a2;
+        const OFF p = (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 4;
+        const float a1 = F3.be[jr] * ldf(F3.pe1, p) + F3.ce_raw[jr] * (hz - hzj) / g.dx;
+        const float a2 = F3.be[jr] * ldf(F3.pe2, p) + F3.ce_raw[jr] * (hx - hxj) / g.dz;
+        stf(F3.pe1, p, a1);
+        stf(F3.pe2, p, a2);
         if (!(i == 0 || k == 0) && !tx) ex += g.dx * m.cbm[0] * a1;
         if (!(i == 0 || k == 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
     }
     if (F5.on && k >= nz - F5.depth + 1) {
         const int kr = k - (nz - F5.depth);
-        const long long p = ((long long)kr * ny + j) * px + i;
-        const float a1 = F5.be[kr] * F5.pe1[p] + F5.ce_raw[kr] * (hx - hxkm1) / g.dy;
-        const float a2 = F5.be[kr] * F5.pe2[p] + F5.ce_raw[kr] * (hy - hykm1) / g.dx;
-        F5.pe1[p] = a1;
-        F5.pe2[p] = a2;
+        const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 4;
+        const float a1 = F5.be[kr] * ldf(F5.pe1, p) + F5.ce_raw[kr] * (hx - hxkm1) / g.dy;
+        const float a2 = F5.be[kr] * ldf(F5.pe2, p) + F5.ce_raw[kr] * (hy - hykm1) / g.dx;
+        stf(F5.pe1, p, a1);
+        stf(F5.pe2, p, a2);
         if (!(i == 0 || j == 0) && !ty) ey += g.dy * m.cbm[1] * a1;
         if (!(i == 0 || j == 0) && !tx) ex -= g.dx * m.cbm[0] * a2;
     }
 
     /* polarisation state (all three components, with the values at this point) */
-    g.dpx[c] = m.a0 * dpxp + m.a1 * dpx + m.a2 * (expv + 2.f * vexp + ex);
-    g.dpy[c] = m.a0 * dpyp + m.a1 * dpy + m.a2 * (eypv + 2.f * veyp + ey);
-    g.dpz[c] = m.a0 * dpzp + m.a1 * dpz + m.a2 * (ezpv + 2.f * vezp + ez);
+    stf(g.dpx, o, m.a0 * dpxp + m.a1 * dpx + m.a2 * (expv + 2.f * vexp + ex));
+    stf(g.dpy, o, m.a0 * dpyp + m.a1 * dpy + m.a2 * (eypv + 2.f * veyp + ey));
+    stf(g.dpz, o, m.a0 * dpzp + m.a1 * dpz + m.a2 * (ezpv + 2.f * vezp + ez));
     if (POLES && m.poles) {
-        g.px0[c] = m.bp0[0] * pxp0 + m.bp1[0] * px0 + m.bp2[0] * expv + m.bp3[0] * vexp + m.bp4[0] * ex;
-        g.px1[c] = m.bp0[1] * pxp1 + m.bp1[1] * px1 + m.bp2[1] * expv + m.bp3[1] * vexp + m.bp4[1] * ex;
-        g.py0[c] = m.bp0[0] * pyp0 + m.bp1[0] * py0 + m.bp2[0] * eypv + m.bp3[0] * veyp + m.bp4[0] * ey;
-        g.py1[c] = m.bp0[1] * pyp1 + m.bp1[1] * py1 + m.bp2[1] * eypv + m.bp3[1] * veyp + m.bp4[1] * ey;
-        g.pz0[c] = m.bp0[0] * pzp0 + m.bp1[0] * pz0 + m.bp2[0] * ezpv + m.bp3[0] * vezp + m.bp4[0] * ez;
-        g.pz1[c] = m.bp0[1] * pzp1 + m.bp1[1] * pz1 + m.bp2[1] * ezpv + m.bp3[1] * vezp + m.bp4[1] * ez;
-        g.pxp0[c] = px0; g.pxp1[c] = px1; g.pyp0[c] = py0; g.pyp1[c] = py1; g.pzp0[c] = pz0; g.pzp1[c] = pz1;
+        stf(g.px0, o, m.bp0[0] * pxp0 + m.bp1[0] * px0 + m.bp2[0] * expv + m.bp3[0] * vexp + m.bp4[0] * ex);
+        stf(g.px1, o, m.bp0[1] * pxp1 + m.bp1[1] * px1 + m.bp2[1] * expv + m.bp3[1] * vexp + m.bp4[1] * ex);
+        stf(g.py0, o, m.bp0[0] * pyp0 + m.bp1[0] * py0 + m.bp2[0] * eypv + m.bp3[0] * veyp + m.bp4[0] * ey);
+        stf(g.py1, o, m.bp0[1] * pyp1 + m.bp1[1] * py1 + m.bp2[1] * eypv + m.bp3[1] * veyp + m.bp4[1] * ey);
+        stf(g.pz0, o, m.bp0[0] * pzp0 + m.bp1[0] * pz0 + m.bp2[0] * ezpv + m.bp3[0] * vezp + m.bp4[0] * ez);
+        stf(g.pz1, o, m.bp0[1] * pzp1 + m.bp1[1] * pz1 + m.bp2[1] * ezpv + m.bp3[1] * vezp + m.bp4[1] * ez);
+        stf(g.pxp0, o, px0); stf(g.pxp1, o, px1); stf(g.pyp0, o, py0);
+        stf(g.pyp1, o, py1); stf(g.pzp0, o, pz0); stf(g.pzp1, o, pz1);
     }
-    g.exp_[c] = vexp;
-    g.eyp[c] = veyp;
-    g.ezp[c] = vezp;
-    g.dpxp[c] = dpx;
-    g.dpyp[c] = dpy;
-    g.dpzp[c] = dpz;
+    stf(g.exp_, o, vexp);
+    stf(g.eyp, o, veyp);
+    stf(g.ezp, o, vezp);
+    stf(g.dpxp, o, dpx);
+    stf(g.dpyp, o, dpy);
+    stf(g.dpzp, o, dpz);
 }
 
-template <typename ID, int ADE>
+template <typename ID, int ADE, typename OFF>
 __global__ __launch_bounds__(512) void
 k_estep(GsvGrid g, int kchunk)
 {
@@ -292,10 +327,11 @@ k_estep(GsvGrid g, int kchunk)
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int k0 = blockIdx.z * kchunk;
     const int k1 = min(k0 + kchunk, nz);
-    const long long P = g.plane;
     const int px = g.px;
-    const ID *cls = (const ID *)g.ecls;
-    long long c = (long long)i + (long long)px * j + P * k0;
+    const OFF dP = (OFF)g.plane * 4, dJ = (OFF)px * 4;
+    const OFF dPc = (OFF)g.plane * sizeof(ID);
+    OFF o = ((OFF)i + (OFF)px * j + (OFF)g.plane * (k0 + 1)) * 4;
+    OFF oc = ((OFF)i + (OFF)px * j + (OFF)g.plane * k0) * sizeof(ID);
 
     const float rkx = g.rkex[i], rky = g.rkey[j];
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
@@ -307,19 +343,20 @@ k_estep(GsvGrid g, int kchunk)
     const bool syn = F3.on && j >= ny - F3.depth + 1;
     const bool zplane_xy = (F0.on && i == 0) || (F2.on && j == 0);
 
-    float hxkm1 = g.hx[c - P], hykm1 = g.hy[c - P];
-    for (int k = k0; k < k1; ++k, c += P) {
-        const float hx = g.hx[c], hy = g.hy[c], hz = g.hz[c];
-        const float hzj = g.hz[c - px], hxj = g.hx[c - px];
-        const float hzi = g.hz[c - 1], hyi = g.hy[c - 1];
+    float hxkm1 = ldf(g.hxg, o - dP), hykm1 = ldf(g.hyg, o - dP);
+    for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
+        const float hx = ldf(g.hxg, o), hy = ldf(g.hyg, o), hz = ldf(g.hzg, o);
+        const float hzj = ldf(g.hzg, o - dJ), hxj = ldf(g.hxg, o - dJ);
+        const float hzi = ldf(g.hzg, o - 4), hyi = ldf(g.hyg, o - 4);
         const float rkz = g.rkez[k];
-        const GsvECoef ec = g.etab[cls[c]];
-        float ex = g.ex[c], ey = g.ey[c], ez = g.ez[c];
+        const GsvECoef ec = g.etab[ldid<ID>(g.ecls, oc)];
+        float ex = ldf(g.exg, o), ey = ldf(g.eyg, o), ez = ldf(g.ezg, o);
         const unsigned fl = ec.flags;
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
 
         if (ADE && (fl & GSV_E_ADE))
-            ade_cell<(ADE == 2)>(g, c, i, j, k, fl, ex, ey, ez, hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
+            ade_cell<(ADE == 2)>(g, o, i, j, k, fl, ex, ey, ez, hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky,
+                                 rkz);
 
         if (tx && j > 0 && k > 0)
             ex = ec.ca[0] * ex + ec.cb[0] * ((hz - hzj) * rky - (hy - hykm1) * rkz);
@@ -336,33 +373,33 @@ k_estep(GsvGrid g, int kchunk)
 
         /* CPML E psi and correction, src3d/boundary.c:691-887 */
         if (sx0) {
-            const long long p = ((long long)k * ny + j) * F0.depth + i;
-            const float a1 = F0.be[i] * F0.pe1[p] + F0.ce_d[i] * (hz - hzi);
-            const float a2 = F0.be[i] * F0.pe2[p] + F0.ce_d[i] * (hy - hyi);
-            F0.pe1[p] = a1;
-            F0.pe2[p] = a2;
+            const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 4;
+            const float a1 = F0.be[i] * ldf(F0.pe1, p) + F0.ce_d[i] * (hz - hzi);
+            const float a2 = F0.be[i] * ldf(F0.pe2, p) + F0.ce_d[i] * (hy - hyi);
+            stf(F0.pe1, p, a1);
+            stf(F0.pe2, p, a2);
             if (j != 0 && k != 0) {
                 if (ty) ey -= g.dx * ec.cb[1] * a1;
                 if (tz) ez += g.dx * ec.cb[2] * a2;
             }
         }
         if (sy0) {
-            const long long p = ((long long)k * F2.depth + j) * px + i;
-            const float a1 = F2.be[j] * F2.pe1[p] + F2.ce_d[j] * (hz - hzj);
-            const float a2 = F2.be[j] * F2.pe2[p] + F2.ce_d[j] * (hx - hxj);
-            F2.pe1[p] = a1;
-            F2.pe2[p] = a2;
+            const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 4;
+            const float a1 = F2.be[j] * ldf(F2.pe1, p) + F2.ce_d[j] * (hz - hzj);
+            const float a2 = F2.be[j] * ldf(F2.pe2, p) + F2.ce_d[j] * (hx - hxj);
+            stf(F2.pe1, p, a1);
+            stf(F2.pe2, p, a2);
             if (i != 0 && k != 0) {
                 if (tx) ex += g.dy * ec.cb[0] * a1;
                 if (tz) ez -= g.dy * ec.cb[2] * a2;
             }
         }
         if (F4.on && k >= 1 && k < F4.depth) {
-            const long long p = ((long long)k * ny + j) * px + i;
-            const float a1 = F4.be[k] * F4.pe1[p] + F4.ce_d[k] * (hx - hxkm1);
-            const float a2 = F4.be[k] * F4.pe2[p] + F4.ce_d[k] * (hy - hykm1);
-            F4.pe1[p] = a1;
-            F4.pe2[p] = a2;
+            const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 4;
+            const float a1 = F4.be[k] * ldf(F4.pe1, p) + F4.ce_d[k] * (hx - hxkm1);
+            const float a2 = F4.be[k] * ldf(F4.pe2, p) + F4.ce_d[k] * (hy - hykm1);
+            stf(F4.pe1, p, a1);
+            stf(F4.pe2, p, a2);
             if (i != 0 && j != 0) {
                 if (ty) ey += g.dx * ec.cb[1] * a1;
                 /* the reference gates this Ex correction on the z flag (src3d/boundary.c:800-801) */
@@ -370,22 +407,22 @@ k_estep(GsvGrid g, int kchunk)
             }
         }
         if (sxn) {
-            const long long p = ((long long)k * ny + j) * F1.depth + irn;
-            const float a1 = F1.be[irn] * F1.pe1[p] + F1.ce_d[irn] * (hz - hzi);
-            const float a2 = F1.be[irn] * F1.pe2[p] + F1.ce_d[irn] * (hy - hyi);
-            F1.pe1[p] = a1;
-            F1.pe2[p] = a2;
+            const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + irn) * 4;
+            const float a1 = F1.be[irn] * ldf(F1.pe1, p) + F1.ce_d[irn] * (hz - hzi);
+            const float a2 = F1.be[irn] * ldf(F1.pe2, p) + F1.ce_d[irn] * (hy - hyi);
+            stf(F1.pe1, p, a1);
+            stf(F1.pe2, p, a2);
             if (j != 0 && k != 0) {
                 if (ty) ey -= g.dx * ec.cb[1] * a1;
                 if (tz) ez += g.dx * ec.cb[2] * a2;
             }
         }
         if (syn) {
-            const long long p = ((long long)k * F3.depth + jrn) * px + i;
-            const float a1 = F3.be[jrn] * F3.pe1[p] + F3.ce_d[jrn] * (hz - hzj);
-            const float a2 = F3.be[jrn] * F3.pe2[p] + F3.ce_d[jrn] * (hx - hxj);
-            F3.pe1[p] = a1;
-            F3.pe2[p] = a2;
+            const OFF p = (OFF)(((unsigned)k * F3.depth + jrn) * px + i) * 4;
+            const float a1 = F3.be[jrn] * ldf(F3.pe1, p) + F3.ce_d[jrn] * (hz - hzj);
+            const float a2 = F3.be[jrn] * ldf(F3.pe2, p) + F3.ce_d[jrn] * (hx - hxj);
+            stf(F3.pe1, p, a1);
+            stf(F3.pe2, p, a2);
             if (i != 0 && k != 0) {
                 if (tx) ex += g.dy * ec.cb[0] * a1;
                 if (tz) ez -= g.dy * ec.cb[2] * a2;
@@ -393,11 +430,11 @@ k_estep(GsvGrid g, int kchunk)
         }
         if (F5.on && k >= nz - F5.depth + 1) {
             const int kr = k - (nz - F5.depth);
-            const long long p = ((long long)kr * ny + j) * px + i;
-            const float a1 = F5.be[kr] * F5.pe1[p] + F5.ce_d[kr] * (hx - hxkm1);
-            const float a2 = F5.be[kr] * F5.pe2[p] + F5.ce_d[kr] * (hy - hykm1);
-            F5.pe1[p] = a1;
-            F5.pe2[p] = a2;
+            const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 4;
+            const float a1 = F5.be[kr] * ldf(F5.pe1, p) + F5.ce_d[kr] * (hx - hxkm1);
+            const float a2 = F5.be[kr] * ldf(F5.pe2, p) + F5.ce_d[kr] * (hy - hykm1);
+            stf(F5.pe1, p, a1);
+            stf(F5.pe2, p, a2);
             if (i != 0 && j != 0) {
                 if (ty) ey += g.dx * ec.cb[1] * a1;
                 if (tx) ex -= g.dx * ec.cb[0] * a2;
@@ -408,9 +445,9 @@ k_estep(GsvGrid g, int kchunk)
             ey = 0.f;
             ez = 0.f;
         }
-        g.ex[c] = ex;
-        g.ey[c] = ey;
-        g.ez[c] = ez;
+        stf(g.exg, o, ex);
+        stf(g.eyg, o, ey);
+        stf(g.ezg, o, ez);
         hxkm1 = hx;
         hykm1 = hy;
     }
@@ -874,15 +911,41 @@ k_box_sum(GsvGrid g, GsvBoxSum b, int step)
 /* launchers                                                            */
 /* ------------------------------------------------------------------ */
 
+/* 32-bit byte offsets cover every per-cell array when one field array
+ * (guard planes included) is below 4 GiB */
+static bool
+offsets32(const GsvGrid *g)
+{
+    return (unsigned long long)g->plane * (g->nz + 2) * 4ull < (1ull << 32);
+}
+
 extern "C" hipError_t
 gsv_launch_hstep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st)
 {
     dim3 blk(bx, by, 1), grd((g->nx + bx - 1) / bx, (g->ny + by - 1) / by, (g->nz + kchunk - 1) / kchunk);
-    if (idbytes == 1)
-        hipLaunchKernelGGL(k_hstep<uint8_t>, grd, blk, 0, st, *g, kchunk);
-    else
-        hipLaunchKernelGGL(k_hstep<uint16_t>, grd, blk, 0, st, *g, kchunk);
+    const bool o32 = offsets32(g);
+#define GSV_HSTEP(ID, OFF) hipLaunchKernelGGL((k_hstep<ID, OFF>), grd, blk, 0, st, *g, kchunk)
+    if (idbytes == 1) {
+        if (o32) GSV_HSTEP(uint8_t, uint32_t);
+        else GSV_HSTEP(uint8_t, uint64_t);
+    } else {
+        if (o32) GSV_HSTEP(uint16_t, uint32_t);
+        else GSV_HSTEP(uint16_t, uint64_t);
+    }
+#undef GSV_HSTEP
     return hipGetLastError();
+}
+
+template <typename ID, typename OFF>
+static void
+launch_estep(const GsvGrid *g, int ade, dim3 grd, dim3 blk, int kchunk, hipStream_t st)
+{
+    if (ade == 0)
+        hipLaunchKernelGGL((k_estep<ID, 0, OFF>), grd, blk, 0, st, *g, kchunk);
+    else if (ade == 1)
+        hipLaunchKernelGGL((k_estep<ID, 1, OFF>), grd, blk, 0, st, *g, kchunk);
+    else
+        hipLaunchKernelGGL((k_estep<ID, 2, OFF>), grd, blk, 0, st, *g, kchunk);
 }
 
 extern "C" hipError_t
@@ -892,17 +955,14 @@ gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipS
     /* the dispersive branch is compiled in only when the pool has ADE cells,
      * and the critical-point terms only when some material has them */
     const int ade = g->ade ? (g->ade_poles ? 2 : 1) : 0;
-#define GSV_ESTEP(ID, A) hipLaunchKernelGGL((k_estep<ID, A>), grd, blk, 0, st, *g, kchunk)
+    const bool o32 = offsets32(g);
     if (idbytes == 1) {
-        if (ade == 0) GSV_ESTEP(uint8_t, 0);
-        else if (ade == 1) GSV_ESTEP(uint8_t, 1);
-        else GSV_ESTEP(uint8_t, 2);
+        if (o32) launch_estep<uint8_t, uint32_t>(g, ade, grd, blk, kchunk, st);
+        else launch_estep<uint8_t, uint64_t>(g, ade, grd, blk, kchunk, st);
     } else {
-        if (ade == 0) GSV_ESTEP(uint16_t, 0);
-        else if (ade == 1) GSV_ESTEP(uint16_t, 1);
-        else GSV_ESTEP(uint16_t, 2);
+        if (o32) launch_estep<uint16_t, uint32_t>(g, ade, grd, blk, kchunk, st);
+        else launch_estep<uint16_t, uint64_t>(g, ade, grd, blk, kchunk, st);
     }
-#undef GSV_ESTEP
     return hipGetLastError();
 }
 
