@@ -13,8 +13,8 @@
  *               the PEC zero mask, computed on the host in double and rounded.
  *   CPML psi    stored only on the boundary slabs (x slabs [k][j][depth],
  *               y slabs [k][depth][px], z slabs [depth][ny][px]).
- *   ADE state   full-grid float arrays with the same guard planes, touched
- *               only in dispersive cells.
+ *   ADE state   blocked per z plane (see GSV_ADE_D), touched only in
+ *               dispersive cells.
  * The volume kernels address every per-cell array by one 32-bit byte offset
  * from the guard-plane start (one offset register shared by all arrays);
  * grids whose field arrays exceed 4 GiB use 64-bit offsets.
@@ -23,6 +23,7 @@
 #define GSV_ENGINE_H
 
 #include <stdint.h>
+#include <hip/hip_runtime.h>
 
 /* flags of an E class */
 #define GSV_E_TBCX 0x1u      /* Ex takes the ordinary (non-dispersive) update */
@@ -59,16 +60,23 @@ typedef struct {
     int poles;       /* nonzero if the critical-point terms are present */
 } GsvAde;
 
-/* one CPML face on the device */
+/* one CPML face on the device: grading coefficients per index along the
+ * stretched axis, packed so that one load brings all of them, and the two
+ * psi values of a slab cell stored as a pair */
 typedef struct {
     int on;
     int depth;
-    float *be, *ce_d;       /* E side: b, c / spacing of the stretched axis (boundary update) */
-    float *bh, *ch_d;       /* H side: b, c / spacing */
-    float *ce_raw;          /* E side c (the ADE inline update divides by another spacing) */
-    float *pe1, *pe2;       /* E psi */
-    float *ph1, *ph2;       /* H psi */
+    const float4 *ce;       /* E side: {b, c / spacing of the stretched axis, c, 0} */
+    const float2 *ch;       /* H side: {b, c / spacing} */
+    float2 *pe;             /* E psi pairs per slab cell */
+    float2 *ph;             /* H psi pairs per slab cell */
 } GsvCpmlFace;
+
+/* ADE state blocks: per z plane [j][i/64][state][64], x pitch pxa (a
+ * multiple of 64), so one base per plane plus the lane's offset reaches
+ * every state of a cell through immediate offsets (state * 256 bytes) */
+#define GSV_ADE_D 9      /* E(n-1) x y z, dP x y z, dP(n-1) x y z */
+#define GSV_ADE_P 12     /* P0 x y z, P1 x y z, P0(n-1) x y z, P1(n-1) x y z (critical points) */
 
 typedef struct {
     int nx, ny, nz;
@@ -88,10 +96,11 @@ typedef struct {
     /* ADE */
     int ade, ade_poles;
     const GsvAde *adem;     /* ADE material table in device memory */
-    /* ADE state: same layout as the fields including the guard planes; these
-     * are allocation starts, so cell (i,j,k) is at the fields' guard offset */
-    float *exp_, *eyp, *ezp, *dpx, *dpy, *dpz, *dpxp, *dpyp, *dpzp;
-    float *px0, *px1, *py0, *py1, *pz0, *pz1, *pxp0, *pxp1, *pyp0, *pyp1, *pzp0, *pzp1;
+    int pxa;                /* x pitch of the ADE state blocks */
+    float *ade_d;           /* Drude state blocks, GSV_ADE_D states */
+    float *ade_p;           /* critical-point state blocks, GSV_ADE_P states (NULL without poles) */
+    long long ade_plane_d;  /* floats per z plane of ade_d = ny * pxa * GSV_ADE_D */
+    long long ade_plane_p;  /* floats per z plane of ade_p */
 } GsvGrid;
 
 #endif

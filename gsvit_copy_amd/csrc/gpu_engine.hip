@@ -493,26 +493,25 @@ setup_cpml(Engine *E)
         }
         F.on = 1;
         F.depth = dep;
-        std::vector<float> be(dep), ced(dep), bh(dep), chd(dep), cer(dep);
+        std::vector<float4> ce(dep);
+        std::vector<float2> ch(dep);
         for (int q = 0; q < dep; q++) {
-            be[q] = (float)h->be[q];
-            ced[q] = (float)(h->ce[q] / dn[axis]);
-            bh[q] = (float)h->bh[q];
-            chd[q] = (float)(h->ch[q] / dn[axis]);
-            cer[q] = (float)h->ce[q];
+            ce[q] = make_float4((float)h->be[q], (float)(h->ce[q] / dn[axis]), (float)h->ce[q], 0.f);
+            ch[q] = make_float2((float)h->bh[q], (float)(h->ch[q] / dn[axis]));
         }
-        if (dupload(E, &F.be, be.data(), dep) || dupload(E, &F.ce_d, ced.data(), dep) || dupload(E, &F.bh, bh.data(), dep)
-            || dupload(E, &F.ch_d, chd.data(), dep) || dupload(E, &F.ce_raw, cer.data(), dep))
+        float4 *dce;
+        float2 *dch;
+        if (dupload(E, &dce, ce.data(), dep) || dupload(E, &dch, ch.data(), dep))
             return 1;
+        F.ce = dce;
+        F.ch = dch;
         if (axis == 0) cnt = (size_t)dep * g.ny * g.nz;
         else if (axis == 1) cnt = (size_t)g.px * dep * g.nz;
         else cnt = (size_t)g.px * g.ny * dep;
-        if (dmalloc(E, &F.pe1, cnt) || dmalloc(E, &F.pe2, cnt) || dmalloc(E, &F.ph1, cnt) || dmalloc(E, &F.ph2, cnt))
+        if (dmalloc(E, &F.pe, cnt) || dmalloc(E, &F.ph, cnt))
             return 1;
-        CK(hipMemsetAsync(F.pe1, 0, cnt * 4, E->st));
-        CK(hipMemsetAsync(F.pe2, 0, cnt * 4, E->st));
-        CK(hipMemsetAsync(F.ph1, 0, cnt * 4, E->st));
-        CK(hipMemsetAsync(F.ph2, 0, cnt * 4, E->st));
+        CK(hipMemsetAsync(F.pe, 0, cnt * 8, E->st));
+        CK(hipMemsetAsync(F.ph, 0, cnt * 8, E->st));
     }
     return 0;
 }
@@ -916,18 +915,18 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
         g.adem = am;
     }
     if (g.ade) {
-        size_t n = E->field_floats;   /* same guard layout as the fields */
-        float **arr[] = { &g.exp_, &g.eyp, &g.ezp, &g.dpx, &g.dpy, &g.dpz, &g.dpxp, &g.dpyp, &g.dpzp };
-        for (auto a : arr) {
-            if (dmalloc(E, a, n)) return 1;
-            CK(hipMemsetAsync(*a, 0, n * 4, E->st));
-        }
+        g.pxa = (g.nx + 63) / 64 * 64;
+        g.ade_plane_d = (long long)g.ny * g.pxa * GSV_ADE_D;
+        size_t n = (size_t)g.ade_plane_d * g.nz;
+        if (dmalloc(E, &g.ade_d, n))
+            return 1;
+        CK(hipMemsetAsync(g.ade_d, 0, n * 4, E->st));
         if (g.ade_poles) {
-            float **par[] = { &g.px0, &g.px1, &g.py0, &g.py1, &g.pz0, &g.pz1, &g.pxp0, &g.pxp1, &g.pyp0, &g.pyp1, &g.pzp0, &g.pzp1 };
-            for (auto a : par) {
-                if (dmalloc(E, a, n)) return 1;
-                CK(hipMemsetAsync(*a, 0, n * 4, E->st));
-            }
+            g.ade_plane_p = (long long)g.ny * g.pxa * GSV_ADE_P;
+            n = (size_t)g.ade_plane_p * g.nz;
+            if (dmalloc(E, &g.ade_p, n))
+                return 1;
+            CK(hipMemsetAsync(g.ade_p, 0, n * 4, E->st));
         }
     }
     if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E)

@@ -51,6 +51,36 @@ stf(float *base, OFF off, float v)
     *(float *)((char *)base + off) = v;
 }
 
+/* state slot of a blocked ADE cell: the 32-bit lane offset is widened before
+ * the constant slot offset is added, so the slot becomes an immediate */
+template <typename OFF>
+__device__ __forceinline__ float
+lds(const float *base, OFF off, int slot_bytes)
+{
+    return *(const float *)((const char *)base + (size_t)off + slot_bytes);
+}
+
+template <typename OFF>
+__device__ __forceinline__ void
+sts(float *base, OFF off, int slot_bytes, float v)
+{
+    *(float *)((char *)base + (size_t)off + slot_bytes) = v;
+}
+
+template <typename OFF>
+__device__ __forceinline__ float2
+ld2(const float2 *base, OFF off)
+{
+    return *(const float2 *)((const char *)base + off);
+}
+
+template <typename OFF>
+__device__ __forceinline__ void
+st2(float2 *base, OFF off, float a, float b)
+{
+    *(float2 *)((char *)base + off) = make_float2(a, b);
+}
+
 template <typename ID, typename OFF>
 __device__ __forceinline__ unsigned
 ldid(const void *base, OFF off)
@@ -114,65 +144,65 @@ k_hstep(GsvGrid g, int kchunk)
         /* CPML H psi and correction, src3d/boundary.c:347-518 */
         if (kin) {
             if (sx0) {
-                const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 4;
-                const float b = F0.bh[i], cc = F0.ch_d[i];
-                const float a1 = b * ldf(F0.ph1, p) + cc * (eyi1 - eyk);
-                const float a2 = b * ldf(F0.ph2, p) + cc * (ezi1 - ezk);
-                stf(F0.ph1, p, a1);
-                stf(F0.ph2, p, a2);
+                const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
+                const float2 bc = F0.ch[i];
+                const float2 ps = ld2(F0.ph, p);
+                const float a1 = bc.x * ps.x + bc.y * (eyi1 - eyk);
+                const float a2 = bc.x * ps.y + bc.y * (ezi1 - ezk);
+                st2(F0.ph, p, a1, a2);
                 hy += g.dx * hc.dbc * a2;
                 hz -= g.dx * hc.dbc * a1;
             }
             if (sxn) {
-                const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + irn) * 4;
-                const float b = F1.bh[irn], cc = F1.ch_d[irn];
-                const float a1 = b * ldf(F1.ph1, p) + cc * (eyi1 - eyk);
-                const float a2 = b * ldf(F1.ph2, p) + cc * (ezi1 - ezk);
-                stf(F1.ph1, p, a1);
-                stf(F1.ph2, p, a2);
+                const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + irn) * 8;
+                const float2 bc = F1.ch[irn];
+                const float2 ps = ld2(F1.ph, p);
+                const float a1 = bc.x * ps.x + bc.y * (eyi1 - eyk);
+                const float a2 = bc.x * ps.y + bc.y * (ezi1 - ezk);
+                st2(F1.ph, p, a1, a2);
                 hy += g.dx * hc.dbc * a2;
                 hz -= g.dx * hc.dbc * a1;
             }
             if (sy0) {
-                const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 4;
-                const float b = F2.bh[j], cc = F2.ch_d[j];
-                const float a1 = b * ldf(F2.ph1, p) + cc * (ezj1 - ezk);
-                const float a2 = b * ldf(F2.ph2, p) + cc * (exj1 - exk);
-                stf(F2.ph1, p, a1);
-                stf(F2.ph2, p, a2);
+                const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
+                const float2 bc = F2.ch[j];
+                const float2 ps = ld2(F2.ph, p);
+                const float a1 = bc.x * ps.x + bc.y * (ezj1 - ezk);
+                const float a2 = bc.x * ps.y + bc.y * (exj1 - exk);
+                st2(F2.ph, p, a1, a2);
                 hx -= g.dy * hc.dbc * a1;
                 hz += g.dy * hc.dbc * a2;
             }
             if (syn) {
-                const OFF p = (OFF)(((unsigned)k * F3.depth + jrn) * px + i) * 4;
-                const float b = F3.bh[jrn], cc = F3.ch_d[jrn];
-                const float a1 = b * ldf(F3.ph1, p) + cc * (ezj1 - ezk);
-                const float a2 = b * ldf(F3.ph2, p) + cc * (exj1 - exk);
-                stf(F3.ph1, p, a1);
-                stf(F3.ph2, p, a2);
+                const OFF p = (OFF)(((unsigned)k * F3.depth + jrn) * px + i) * 8;
+                const float2 bc = F3.ch[jrn];
+                const float2 ps = ld2(F3.ph, p);
+                const float a1 = bc.x * ps.x + bc.y * (ezj1 - ezk);
+                const float a2 = bc.x * ps.y + bc.y * (exj1 - exk);
+                st2(F3.ph, p, a1, a2);
                 hx -= g.dy * hc.dbc * a1;
                 hz += g.dy * hc.dbc * a2;
             }
         }
         if (ijin) {
             if (F4.on && k < F4.depth - 1) {
-                const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 4;
-                const float b = F4.bh[k], cc = F4.ch_d[k];
-                const float a1 = b * ldf(F4.ph1, p) + cc * (exk1 - exk);
-                const float a2 = b * ldf(F4.ph2, p) + cc * (eyk1 - eyk);
-                stf(F4.ph1, p, a1);
-                stf(F4.ph2, p, a2);
+                const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
+                const float2 bc = F4.ch[k];
+                const float2 ps = ld2(F4.ph, p);
+                const float a1 = bc.x * ps.x + bc.y * (exk1 - exk);
+                const float a2 = bc.x * ps.y + bc.y * (eyk1 - eyk);
+                st2(F4.ph, p, a1, a2);
                 hy -= g.dz * hc.dbc * a1;
                 hx += g.dz * hc.dbc * a2;
             }
             if (F5.on && k >= nz - F5.depth + 1 && k < nz - 1) {
                 const int kr = k - (nz - F5.depth);
-                const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 4;
-                const float b = F5.bh[kr], cc = F5.ch_d[kr];
-                const float a1 = b * ldf(F5.ph1, p) + cc * (exk1 - exk);
-                const float a2 = b * ldf(F5.ph2, p) + cc * (eyk1 - eyk);
-                stf(F5.ph1, p, a1);
-                stf(F5.ph2, p, a2);
+                const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
+                const float2 bc = F5.ch[kr];
+                const float2 ps = ld2(F5.ph, p);
+                const float a1 = bc.x * ps.x + bc.y * (exk1 - exk);
+                const float a2 = bc.x * ps.y + bc.y * (eyk1 - eyk);
+                st2(F5.ph, p, a1, a2);
                 hy -= g.dz * hc.dbc * a1;
                 hx += g.dz * hc.dbc * a2;
             }
@@ -196,27 +226,29 @@ k_hstep(GsvGrid g, int kchunk)
 
 /* ADE branch of one cell (src3d/yee.c:914-1108); updates the components whose
  * flag is clear, runs the branch's own CPML psi update and advances the
- * polarisation state.  Taken in dispersive cells and their +1 neighbours. */
+ * polarisation state.  Taken in dispersive cells and their +1 neighbours.
+ * ad/ap: this plane's ADE state blocks, la/lp: the lane's offset in them;
+ * state q of a cell sits q * 256 bytes after its first state. */
 template <bool POLES, typename OFF>
 __device__ __forceinline__ void
-ade_cell(const GsvGrid &g, OFF o, int i, int j, int k, unsigned fl, float &ex, float &ey, float &ez,
-         float hx, float hy, float hz, float hxkm1, float hykm1, float hzi, float hyi, float hzj, float hxj,
+ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, int k, unsigned fl, float &ex,
+         float &ey, float &ez, float hx, float hy, float hz, float hxkm1, float hykm1, float hzi, float hyi, float hzj, float hxj,
          float rkx, float rky, float rkz)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
     const GsvAde &m = g.adem[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
     const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
     const float vexp = ex, veyp = ey, vezp = ez;
-    const float expv = ldf(g.exp_, o), eypv = ldf(g.eyp, o), ezpv = ldf(g.ezp, o);
-    const float dpx = ldf(g.dpx, o), dpy = ldf(g.dpy, o), dpz = ldf(g.dpz, o);
-    const float dpxp = ldf(g.dpxp, o), dpyp = ldf(g.dpyp, o), dpzp = ldf(g.dpzp, o);
+    const float expv = lds(ad, la, 0), eypv = lds(ad, la, 256), ezpv = lds(ad, la, 512);
+    const float dpx = lds(ad, la, 768), dpy = lds(ad, la, 1024), dpz = lds(ad, la, 1280);
+    const float dpxp = lds(ad, la, 1536), dpyp = lds(ad, la, 1792), dpzp = lds(ad, la, 2048);
     float px0 = 0, px1 = 0, py0 = 0, py1 = 0, pz0 = 0, pz1 = 0;
     float pxp0 = 0, pxp1 = 0, pyp0 = 0, pyp1 = 0, pzp0 = 0, pzp1 = 0;
     if (POLES && m.poles) {
-        px0 = ldf(g.px0, o); px1 = ldf(g.px1, o); py0 = ldf(g.py0, o);
-        py1 = ldf(g.py1, o); pz0 = ldf(g.pz0, o); pz1 = ldf(g.pz1, o);
-        pxp0 = ldf(g.pxp0, o); pxp1 = ldf(g.pxp1, o); pyp0 = ldf(g.pyp0, o);
-        pyp1 = ldf(g.pyp1, o); pzp0 = ldf(g.pzp0, o); pzp1 = ldf(g.pzp1, o);
+        px0 = lds(ap, lp, 0); px1 = lds(ap, lp, 768); py0 = lds(ap, lp, 256);
+        py1 = lds(ap, lp, 1024); pz0 = lds(ap, lp, 512); pz1 = lds(ap, lp, 1280);
+        pxp0 = lds(ap, lp, 1536); pxp1 = lds(ap, lp, 2304); pyp0 = lds(ap, lp, 1792);
+        pyp1 = lds(ap, lp, 2560); pzp0 = lds(ap, lp, 2048); pzp1 = lds(ap, lp, 2816);
     }
 
     if (j > 0 && k > 0 && !tx)
@@ -241,79 +273,85 @@ ade_cell(const GsvGrid &g, OFF o, int i, int j, int k, unsigned fl, float &ex, f
     /* the branch's own CPML update; its x0 correction multiplies by a zero cb */
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
     if (F0.on && i > 1 && i < F0.depth) {
-        const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 4;
-        stf(F0.pe1, p, F0.be[i] * ldf(F0.pe1, p) + F0.ce_raw[i] * (hz - hzi) / g.dy);
-        stf(F0.pe2, p, F0.be[i] * ldf(F0.pe2, p) + F0.ce_raw[i] * (hy - hyi) / g.dz);
+        const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
+        const float4 bc = F0.ce[i];
+        const float2 ps = ld2(F0.pe, p);
+        st2(F0.pe, p, bc.x * ps.x + bc.z * (hz - hzi) / g.dy, bc.x * ps.y + bc.z * (hy - hyi) / g.dz);
     }
     if (F2.on && j > 1 && j < F2.depth) {
-        const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 4;
-        const float a1 = F2.be[j] * ldf(F2.pe1, p) + F2.ce_raw[j] * (hz - hzj) / g.dz;
-        const float a2 = F2.be[j] * ldf(F2.pe2, p) + F2.ce_raw[j] * (hx - hxj) / g.dx;
-        stf(F2.pe1, p, a1);
-        stf(F2.pe2, p, a2);
+        const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
+        const float4 bc = F2.ce[j];
+        const float2 ps = ld2(F2.pe, p);
+        const float a1 = bc.x * ps.x + bc.z * (hz - hzj) / g.dz;
+        const float a2 = bc.x * ps.y + bc.z * (hx - hxj) / g.dx;
+        st2(F2.pe, p, a1, a2);
         if (!(i == 0 || k == 0) && !tx) ex += g.dx * m.cbm[0] * a1;
         if (!(i == 0 || k == 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
     }
     if (F4.on && k > 1 && k < F4.depth) {
-        const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 4;
-        const float a1 = F4.be[k] * ldf(F4.pe1, p) + F4.ce_raw[k] * (hx - hxkm1) / g.dy;
-        const float a2 = F4.be[k] * ldf(F4.pe2, p) + F4.ce_raw[k] * (hy - hykm1) / g.dx;
-        stf(F4.pe1, p, a1);
-        stf(F4.pe2, p, a2);
+        const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
+        const float4 bc = F4.ce[k];
+        const float2 ps = ld2(F4.pe, p);
+        const float a1 = bc.x * ps.x + bc.z * (hx - hxkm1) / g.dy;
+        const float a2 = bc.x * ps.y + bc.z * (hy - hykm1) / g.dx;
+        st2(F4.pe, p, a1, a2);
         if (!(i == 0 || j == 0) && !ty) ey += g.dy * m.cbm[1] * a1;
         if (!(i == 0 || j == 0) && !tx) ex -= g.dx * m.cbm[0] * a2;
     }
     if (F1.on && i >= nx - F1.depth + 1) {
         const int ir = i - (nx - F1.depth);
-        const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 4;
-        const float a1 = F1.be[ir] * ldf(F1.pe1, p) + F1.ce_raw[ir] * (hz - hzi) / g.dy;
-        const float a2 = F1.be[ir] * ldf(F1.pe2, p) + F1.ce_raw[ir] * (hy - hyi) / g.dz;
-        stf(F1.pe1, p, a1);
-        stf(F1.pe2, p, a2);
+        const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8;
+        const float4 bc = F1.ce[ir];
+        const float2 ps = ld2(F1.pe, p);
+        const float a1 = bc.x * ps.x + bc.z * (hz - hzi) / g.dy;
+        const float a2 = bc.x * ps.y + bc.z * (hy - hyi) / g.dz;
+        st2(F1.pe, p, a1, a2);
         if (!(j == 0 || k == 0) && !ty) ey -= g.dy * m.cbm[1] * a1;
         if (!(j == 0 || k == 0) && !tz) ez += g.dz * m.cbm[2] * a2;
     }
     if (F3.on && j >= ny - F3.depth + 1) {
         const int jr = j - (ny - F3.depth);
-        const OFF p = (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 4;
-        const float a1 = F3.be[jr] * ldf(F3.pe1, p) + F3.ce_raw[jr] * (hz - hzj) / g.dx;
-        const float a2 = F3.be[jr] * ldf(F3.pe2, p) + F3.ce_raw[jr] * (hx - hxj) / g.dz;
-        stf(F3.pe1, p, a1);
-        stf(F3.pe2, p, a2);
+        const OFF p = (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8;
+        const float4 bc = F3.ce[jr];
+        const float2 ps = ld2(F3.pe, p);
+        const float a1 = bc.x * ps.x + bc.z * (hz - hzj) / g.dx;
+        const float a2 = bc.x * ps.y + bc.z * (hx - hxj) / g.dz;
+        st2(F3.pe, p, a1, a2);
         if (!(i == 0 || k == 0) && !tx) ex += g.dx * m.cbm[0] * a1;
         if (!(i == 0 || k == 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
     }
     if (F5.on && k >= nz - F5.depth + 1) {
         const int kr = k - (nz - F5.depth);
-        const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 4;
-        const float a1 = F5.be[kr] * ldf(F5.pe1, p) + F5.ce_raw[kr] * (hx - hxkm1) / g.dy;
-        const float a2 = F5.be[kr] * ldf(F5.pe2, p) + F5.ce_raw[kr] * (hy - hykm1) / g.dx;
-        stf(F5.pe1, p, a1);
-        stf(F5.pe2, p, a2);
+        const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
+        const float4 bc = F5.ce[kr];
+        const float2 ps = ld2(F5.pe, p);
+        const float a1 = bc.x * ps.x + bc.z * (hx - hxkm1) / g.dy;
+        const float a2 = bc.x * ps.y + bc.z * (hy - hykm1) / g.dx;
+        st2(F5.pe, p, a1, a2);
         if (!(i == 0 || j == 0) && !ty) ey += g.dy * m.cbm[1] * a1;
         if (!(i == 0 || j == 0) && !tx) ex -= g.dx * m.cbm[0] * a2;
     }
 
     /* polarisation state (all three components, with the values at this point) */
-    stf(g.dpx, o, m.a0 * dpxp + m.a1 * dpx + m.a2 * (expv + 2.f * vexp + ex));
-    stf(g.dpy, o, m.a0 * dpyp + m.a1 * dpy + m.a2 * (eypv + 2.f * veyp + ey));
-    stf(g.dpz, o, m.a0 * dpzp + m.a1 * dpz + m.a2 * (ezpv + 2.f * vezp + ez));
+    sts(ad, la, 768, m.a0 * dpxp + m.a1 * dpx + m.a2 * (expv + 2.f * vexp + ex));
+    sts(ad, la, 1024, m.a0 * dpyp + m.a1 * dpy + m.a2 * (eypv + 2.f * veyp + ey));
+    sts(ad, la, 1280, m.a0 * dpzp + m.a1 * dpz + m.a2 * (ezpv + 2.f * vezp + ez));
     if (POLES && m.poles) {
-        stf(g.px0, o, m.bp0[0] * pxp0 + m.bp1[0] * px0 + m.bp2[0] * expv + m.bp3[0] * vexp + m.bp4[0] * ex);
-        stf(g.px1, o, m.bp0[1] * pxp1 + m.bp1[1] * px1 + m.bp2[1] * expv + m.bp3[1] * vexp + m.bp4[1] * ex);
-        stf(g.py0, o, m.bp0[0] * pyp0 + m.bp1[0] * py0 + m.bp2[0] * eypv + m.bp3[0] * veyp + m.bp4[0] * ey);
-        stf(g.py1, o, m.bp0[1] * pyp1 + m.bp1[1] * py1 + m.bp2[1] * eypv + m.bp3[1] * veyp + m.bp4[1] * ey);
-        stf(g.pz0, o, m.bp0[0] * pzp0 + m.bp1[0] * pz0 + m.bp2[0] * ezpv + m.bp3[0] * vezp + m.bp4[0] * ez);
-        stf(g.pz1, o, m.bp0[1] * pzp1 + m.bp1[1] * pz1 + m.bp2[1] * ezpv + m.bp3[1] * vezp + m.bp4[1] * ez);
-        stf(g.pxp0, o, px0); stf(g.pxp1, o, px1); stf(g.pyp0, o, py0);
-        stf(g.pyp1, o, py1); stf(g.pzp0, o, pz0); stf(g.pzp1, o, pz1);
+        sts(ap, lp, 0, m.bp0[0] * pxp0 + m.bp1[0] * px0 + m.bp2[0] * expv + m.bp3[0] * vexp + m.bp4[0] * ex);
+        sts(ap, lp, 768, m.bp0[1] * pxp1 + m.bp1[1] * px1 + m.bp2[1] * expv + m.bp3[1] * vexp + m.bp4[1] * ex);
+        sts(ap, lp, 256, m.bp0[0] * pyp0 + m.bp1[0] * py0 + m.bp2[0] * eypv + m.bp3[0] * veyp + m.bp4[0] * ey);
+        sts(ap, lp, 1024, m.bp0[1] * pyp1 + m.bp1[1] * py1 + m.bp2[1] * eypv + m.bp3[1] * veyp + m.bp4[1] * ey);
+        sts(ap, lp, 512, m.bp0[0] * pzp0 + m.bp1[0] * pz0 + m.bp2[0] * ezpv + m.bp3[0] * vezp + m.bp4[0] * ez);
+        sts(ap, lp, 1280, m.bp0[1] * pzp1 + m.bp1[1] * pz1 + m.bp2[1] * ezpv + m.bp3[1] * vezp + m.bp4[1] * ez);
+        sts(ap, lp, 1536, px0); sts(ap, lp, 2304, px1); sts(ap, lp, 1792, py0);
+        sts(ap, lp, 2560, py1); sts(ap, lp, 2048, pz0); sts(ap, lp, 2816, pz1);
     }
-    stf(g.exp_, o, vexp);
-    stf(g.eyp, o, veyp);
-    stf(g.ezp, o, vezp);
-    stf(g.dpxp, o, dpx);
-    stf(g.dpyp, o, dpy);
-    stf(g.dpzp, o, dpz);
+    sts(ad, la, 0, vexp);
+    sts(ad, la, 256, veyp);
+    sts(ad, la, 512, vezp);
+    sts(ad, la, 1536, dpx);
+    sts(ad, la, 1792, dpy);
+    sts(ad, la, 2048, dpz);
 }
 
 template <typename ID, int ADE, typename OFF>
@@ -332,6 +370,10 @@ k_estep(GsvGrid g, int kchunk)
     const OFF dPc = (OFF)g.plane * sizeof(ID);
     OFF o = ((OFF)i + (OFF)px * j + (OFF)g.plane * (k0 + 1)) * 4;
     OFF oc = ((OFF)i + (OFF)px * j + (OFF)g.plane * k0) * sizeof(ID);
+
+    /* lane offsets in the blocked ADE state planes, [j][i/64][state][64] */
+    const OFF la = ADE ? ((OFF)j * g.pxa * GSV_ADE_D + (OFF)(i >> 6) * 64 * GSV_ADE_D + (i & 63)) * 4 : 0;
+    const OFF lp = ADE == 2 ? ((OFF)j * g.pxa * GSV_ADE_P + (OFF)(i >> 6) * 64 * GSV_ADE_P + (i & 63)) * 4 : 0;
 
     const float rkx = g.rkex[i], rky = g.rkey[j];
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
@@ -355,8 +397,9 @@ k_estep(GsvGrid g, int kchunk)
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
 
         if (ADE && (fl & GSV_E_ADE))
-            ade_cell<(ADE == 2)>(g, o, i, j, k, fl, ex, ey, ez, hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky,
-                                 rkz);
+            ade_cell<(ADE == 2)>(g, g.ade_d + (size_t)k * g.ade_plane_d, la,
+                                 ADE == 2 ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr, lp, i, j, k, fl, ex, ey, ez,
+                                 hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
 
         if (tx && j > 0 && k > 0)
             ex = ec.ca[0] * ex + ec.cb[0] * ((hz - hzj) * rky - (hy - hykm1) * rkz);
@@ -373,33 +416,36 @@ k_estep(GsvGrid g, int kchunk)
 
         /* CPML E psi and correction, src3d/boundary.c:691-887 */
         if (sx0) {
-            const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 4;
-            const float a1 = F0.be[i] * ldf(F0.pe1, p) + F0.ce_d[i] * (hz - hzi);
-            const float a2 = F0.be[i] * ldf(F0.pe2, p) + F0.ce_d[i] * (hy - hyi);
-            stf(F0.pe1, p, a1);
-            stf(F0.pe2, p, a2);
+            const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
+            const float4 bc = F0.ce[i];
+            const float2 ps = ld2(F0.pe, p);
+            const float a1 = bc.x * ps.x + bc.y * (hz - hzi);
+            const float a2 = bc.x * ps.y + bc.y * (hy - hyi);
+            st2(F0.pe, p, a1, a2);
             if (j != 0 && k != 0) {
                 if (ty) ey -= g.dx * ec.cb[1] * a1;
                 if (tz) ez += g.dx * ec.cb[2] * a2;
             }
         }
         if (sy0) {
-            const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 4;
-            const float a1 = F2.be[j] * ldf(F2.pe1, p) + F2.ce_d[j] * (hz - hzj);
-            const float a2 = F2.be[j] * ldf(F2.pe2, p) + F2.ce_d[j] * (hx - hxj);
-            stf(F2.pe1, p, a1);
-            stf(F2.pe2, p, a2);
+            const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
+            const float4 bc = F2.ce[j];
+            const float2 ps = ld2(F2.pe, p);
+            const float a1 = bc.x * ps.x + bc.y * (hz - hzj);
+            const float a2 = bc.x * ps.y + bc.y * (hx - hxj);
+            st2(F2.pe, p, a1, a2);
             if (i != 0 && k != 0) {
                 if (tx) ex += g.dy * ec.cb[0] * a1;
                 if (tz) ez -= g.dy * ec.cb[2] * a2;
             }
         }
         if (F4.on && k >= 1 && k < F4.depth) {
-            const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 4;
-            const float a1 = F4.be[k] * ldf(F4.pe1, p) + F4.ce_d[k] * (hx - hxkm1);
-            const float a2 = F4.be[k] * ldf(F4.pe2, p) + F4.ce_d[k] * (hy - hykm1);
-            stf(F4.pe1, p, a1);
-            stf(F4.pe2, p, a2);
+            const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
+            const float4 bc = F4.ce[k];
+            const float2 ps = ld2(F4.pe, p);
+            const float a1 = bc.x * ps.x + bc.y * (hx - hxkm1);
+            const float a2 = bc.x * ps.y + bc.y * (hy - hykm1);
+            st2(F4.pe, p, a1, a2);
             if (i != 0 && j != 0) {
                 if (ty) ey += g.dx * ec.cb[1] * a1;
                 /* the reference gates this Ex correction on the z flag (src3d/boundary.c:800-801) */
@@ -407,22 +453,24 @@ k_estep(GsvGrid g, int kchunk)
             }
         }
         if (sxn) {
-            const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + irn) * 4;
-            const float a1 = F1.be[irn] * ldf(F1.pe1, p) + F1.ce_d[irn] * (hz - hzi);
-            const float a2 = F1.be[irn] * ldf(F1.pe2, p) + F1.ce_d[irn] * (hy - hyi);
-            stf(F1.pe1, p, a1);
-            stf(F1.pe2, p, a2);
+            const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + irn) * 8;
+            const float4 bc = F1.ce[irn];
+            const float2 ps = ld2(F1.pe, p);
+            const float a1 = bc.x * ps.x + bc.y * (hz - hzi);
+            const float a2 = bc.x * ps.y + bc.y * (hy - hyi);
+            st2(F1.pe, p, a1, a2);
             if (j != 0 && k != 0) {
                 if (ty) ey -= g.dx * ec.cb[1] * a1;
                 if (tz) ez += g.dx * ec.cb[2] * a2;
             }
         }
         if (syn) {
-            const OFF p = (OFF)(((unsigned)k * F3.depth + jrn) * px + i) * 4;
-            const float a1 = F3.be[jrn] * ldf(F3.pe1, p) + F3.ce_d[jrn] * (hz - hzj);
-            const float a2 = F3.be[jrn] * ldf(F3.pe2, p) + F3.ce_d[jrn] * (hx - hxj);
-            stf(F3.pe1, p, a1);
-            stf(F3.pe2, p, a2);
+            const OFF p = (OFF)(((unsigned)k * F3.depth + jrn) * px + i) * 8;
+            const float4 bc = F3.ce[jrn];
+            const float2 ps = ld2(F3.pe, p);
+            const float a1 = bc.x * ps.x + bc.y * (hz - hzj);
+            const float a2 = bc.x * ps.y + bc.y * (hx - hxj);
+            st2(F3.pe, p, a1, a2);
             if (i != 0 && k != 0) {
                 if (tx) ex += g.dy * ec.cb[0] * a1;
                 if (tz) ez -= g.dy * ec.cb[2] * a2;
@@ -430,11 +478,12 @@ k_estep(GsvGrid g, int kchunk)
         }
         if (F5.on && k >= nz - F5.depth + 1) {
             const int kr = k - (nz - F5.depth);
-            const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 4;
-            const float a1 = F5.be[kr] * ldf(F5.pe1, p) + F5.ce_d[kr] * (hx - hxkm1);
-            const float a2 = F5.be[kr] * ldf(F5.pe2, p) + F5.ce_d[kr] * (hy - hykm1);
-            stf(F5.pe1, p, a1);
-            stf(F5.pe2, p, a2);
+            const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
+            const float4 bc = F5.ce[kr];
+            const float2 ps = ld2(F5.pe, p);
+            const float a1 = bc.x * ps.x + bc.y * (hx - hxkm1);
+            const float a2 = bc.x * ps.y + bc.y * (hy - hykm1);
+            st2(F5.pe, p, a1, a2);
             if (i != 0 && j != 0) {
                 if (ty) ey += g.dx * ec.cb[1] * a1;
                 if (tx) ex -= g.dx * ec.cb[0] * a2;
