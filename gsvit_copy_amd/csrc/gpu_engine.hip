@@ -107,6 +107,7 @@ struct Engine {
     GsvFarfieldTD ff;
     int has_dft;
     GsvDft dft;
+    int dft_first, dft_pending;   /* pending ring samples: steps dft_first .. + dft_pending - 1 */
     /* box sums */
     std::vector<GsvBoxSum> sums;
     /* timing of the two volume kernels */
@@ -825,6 +826,26 @@ setup_farfield(Engine *E)
         if (dmalloc(E, &D.acc, cnt))
             return 1;
         CK(hipMemsetAsync(D.acc, 0, cnt * 8, E->st));
+        /* ring depth: the flush tabulates T * nw * 4 phases in LDS (<= 48 KiB) */
+        D.T = 1536 / D.nw;
+        if (D.T > 32) D.T = 32;
+        if (D.T < 1) D.T = 1;
+        if (dmalloc(E, &D.ring, (size_t)D.T * 4 * D.ncells))
+            return 1;
+        E->dft_first = 0;
+        E->dft_pending = 0;
+    }
+    return 0;
+}
+
+/* fold the pending DFT samples into the accumulators */
+static int
+dft_flush(Engine *E)
+{
+    if (E->has_dft && E->dft_pending > 0) {
+        CK(gsv_launch_dft_flush(&E->dft, E->dft_first, E->dft_pending, E->st));
+        E->dft_first += E->dft_pending;
+        E->dft_pending = 0;
     }
     return 0;
 }
@@ -1032,6 +1053,8 @@ fetch_aux(Engine *E, SvPool *mp, SvSet *set)
             memcpy(mp->farfield->rpoints[n].ex, b.data() + (size_t)n * 3 * nd, 3 * (size_t)nd * 8);
     }
     if (E->has_dft) {
+        if (dft_flush(E))
+            return 1;
         size_t cnt = (size_t)E->dft.ncells * E->dft.nw * 8;
         CK(hipMemcpyAsync(mp->farfield->dft_data, E->dft.acc, cnt * 8, hipMemcpyDeviceToHost, E->st));
         CK(hipStreamSynchronize(E->st));
@@ -1191,8 +1214,16 @@ sv_pool_gpu_apply_farfield(SvPool *mp, SvSet *set)
     ENGINE_OR_FAIL(mp);
     if (E->has_ff)
         CK(gsv_launch_farfield_td(&E->g, &E->ff, set->sc.step_act, E->st));
-    if (E->has_dft)
-        CK(gsv_launch_dft(&E->g, &E->dft, set->sc.step_act, E->st));
+    if (E->has_dft) {
+        const int step = set->sc.step_act;
+        if (E->dft_pending > 0 && step != E->dft_first + E->dft_pending && dft_flush(E))
+            return 1;
+        if (E->dft_pending == 0)
+            E->dft_first = step;
+        CK(gsv_launch_dft_sample(&E->g, &E->dft, step % E->dft.T, E->st));
+        if (++E->dft_pending == E->dft.T && dft_flush(E))
+            return 1;
+    }
     return 0;
 }
 

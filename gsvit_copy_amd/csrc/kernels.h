@@ -57,6 +57,10 @@ typedef struct {
     double dx, dy, dz, dt, c;
 } GsvFarfieldTD;
 
+/* running DFT: each step the four tangential components of every face cell
+ * are stored in FP32 into slot `step % T` of a sample ring; every T steps (and
+ * before results are read) the pending samples are folded into the double
+ * accumulators, so the accumulators move once per T steps instead of every step */
 typedef struct {
     long long ncells;
     const long long *cell;    /* cell index of each face cell */
@@ -65,6 +69,8 @@ typedef struct {
     const double *omega;
     double *acc;              /* [ncells][nw][8] */
     double dt;
+    int T;                    /* ring slots */
+    float *ring;              /* [T][4][ncells] */
 } GsvDft;
 
 /* one OUT_SUM box: per step the sum (or maximum) of selected E components
@@ -91,7 +97,8 @@ hipError_t gsv_launch_liao_faces(const GsvGrid *g, const GsvLiao *L, hipStream_t
 hipError_t gsv_launch_zero_planes(const GsvGrid *g, int axis, hipStream_t st);
 hipError_t gsv_launch_periodic(const GsvGrid *g, const GsvPeriodic *p, int hside, hipStream_t st);
 hipError_t gsv_launch_farfield_td(const GsvGrid *g, const GsvFarfieldTD *f, int step, hipStream_t st);
-hipError_t gsv_launch_dft(const GsvGrid *g, const GsvDft *d, int step, hipStream_t st);
+hipError_t gsv_launch_dft_sample(const GsvGrid *g, const GsvDft *d, int slot, hipStream_t st);
+hipError_t gsv_launch_dft_flush(const GsvDft *d, int first_step, int nsamples, hipStream_t st);
 hipError_t gsv_launch_box_sum(const GsvGrid *g, const GsvBoxSum *b, int step, hipStream_t st);
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@
  *   k_periodic_h / _e     interior periodic plane copies, src3d/boundary.c:304-346, 889-932
  *   k_record_points       point traces, src3d/output.c:340-352
  *   k_farfield_td         time-domain Kirchhoff bins, src3d/farfield.c:174-407
- *   k_dft_faces           running DFT of tangential fields on the NFFF box
+ *   k_dft_sample / k_dft_flush   running DFT of tangential fields on the NFFF box
  *   k_box_sum             OUT_SUM box sums of E^2, src3d/output.c:259-306
  */
 #include <hip/hip_runtime.h>
@@ -27,7 +27,7 @@
 #include "engine.h"
 #include "kernels.h"
 
-#define LD(p) __builtin_nontemporal_load(p)
+
 
 /* ------------------------------------------------------------------ */
 /* addressing of the volume kernels                                     */
@@ -49,6 +49,39 @@ __device__ __forceinline__ void
 stf(float *base, OFF off, float v)
 {
     *(float *)((char *)base + off) = v;
+}
+
+
+/* Streams a kernel reads or writes exactly once (the updated field family,
+ * the ADE state) bypass the caches' retention with nontemporal accesses;
+ * the other family, whose neighbours are re-read by adjacent rows, stays
+ * cached. */
+template <typename OFF>
+__device__ __forceinline__ float
+ldn(const float *base, OFF off)
+{
+    return __builtin_nontemporal_load((const float *)((const char *)base + off));
+}
+
+template <typename OFF>
+__device__ __forceinline__ void
+stn(float *base, OFF off, float v)
+{
+    __builtin_nontemporal_store(v, (float *)((char *)base + off));
+}
+
+template <typename OFF>
+__device__ __forceinline__ float
+ldsn(const float *base, OFF off, int slot_bytes)
+{
+    return __builtin_nontemporal_load((const float *)((const char *)base + (size_t)off + slot_bytes));
+}
+
+template <typename OFF>
+__device__ __forceinline__ void
+stsn(float *base, OFF off, int slot_bytes, float v)
+{
+    __builtin_nontemporal_store(v, (float *)((char *)base + (size_t)off + slot_bytes));
 }
 
 /* state slot of a blocked ADE cell: the 32-bit lane offset is widened before
@@ -131,7 +164,7 @@ k_hstep(GsvGrid g, int kchunk)
         const float ezi1 = ldf(g.ezg, o + 4), eyi1 = ldf(g.eyg, o + 4);
         const GsvHCoef hc = g.htab[ldid<ID>(g.hcls, oc)];
         const float rkz = g.rkhz[k];
-        float hx = ldf(g.hxg, o), hy = ldf(g.hyg, o), hz = ldf(g.hzg, o);
+        float hx = ldn(g.hxg, o), hy = ldn(g.hyg, o), hz = ldn(g.hzg, o);
         const bool kin = (k >= 1 && k < nz - 1);
 
         if (!(j == ny - 1 || k == nz - 1))
@@ -212,9 +245,9 @@ k_hstep(GsvGrid g, int kchunk)
             hy = 0.f;
             hz = 0.f;
         }
-        stf(g.hxg, o, hx);
-        stf(g.hyg, o, hy);
-        stf(g.hzg, o, hz);
+        stn(g.hxg, o, hx);
+        stn(g.hyg, o, hy);
+        stn(g.hzg, o, hz);
         exk = exk1;
         eyk = eyk1;
     }
@@ -239,16 +272,16 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     const GsvAde &m = g.adem[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
     const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
     const float vexp = ex, veyp = ey, vezp = ez;
-    const float expv = lds(ad, la, 0), eypv = lds(ad, la, 256), ezpv = lds(ad, la, 512);
-    const float dpx = lds(ad, la, 768), dpy = lds(ad, la, 1024), dpz = lds(ad, la, 1280);
-    const float dpxp = lds(ad, la, 1536), dpyp = lds(ad, la, 1792), dpzp = lds(ad, la, 2048);
+    const float expv = ldsn(ad, la, 0), eypv = ldsn(ad, la, 256), ezpv = ldsn(ad, la, 512);
+    const float dpx = ldsn(ad, la, 768), dpy = ldsn(ad, la, 1024), dpz = ldsn(ad, la, 1280);
+    const float dpxp = ldsn(ad, la, 1536), dpyp = ldsn(ad, la, 1792), dpzp = ldsn(ad, la, 2048);
     float px0 = 0, px1 = 0, py0 = 0, py1 = 0, pz0 = 0, pz1 = 0;
     float pxp0 = 0, pxp1 = 0, pyp0 = 0, pyp1 = 0, pzp0 = 0, pzp1 = 0;
     if (POLES && m.poles) {
-        px0 = lds(ap, lp, 0); px1 = lds(ap, lp, 768); py0 = lds(ap, lp, 256);
-        py1 = lds(ap, lp, 1024); pz0 = lds(ap, lp, 512); pz1 = lds(ap, lp, 1280);
-        pxp0 = lds(ap, lp, 1536); pxp1 = lds(ap, lp, 2304); pyp0 = lds(ap, lp, 1792);
-        pyp1 = lds(ap, lp, 2560); pzp0 = lds(ap, lp, 2048); pzp1 = lds(ap, lp, 2816);
+        px0 = ldsn(ap, lp, 0); px1 = ldsn(ap, lp, 768); py0 = ldsn(ap, lp, 256);
+        py1 = ldsn(ap, lp, 1024); pz0 = ldsn(ap, lp, 512); pz1 = ldsn(ap, lp, 1280);
+        pxp0 = ldsn(ap, lp, 1536); pxp1 = ldsn(ap, lp, 2304); pyp0 = ldsn(ap, lp, 1792);
+        pyp1 = ldsn(ap, lp, 2560); pzp0 = ldsn(ap, lp, 2048); pzp1 = ldsn(ap, lp, 2816);
     }
 
     if (j > 0 && k > 0 && !tx)
@@ -333,25 +366,25 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     }
 
     /* polarisation state (all three components, with the values at this point) */
-    sts(ad, la, 768, m.a0 * dpxp + m.a1 * dpx + m.a2 * (expv + 2.f * vexp + ex));
-    sts(ad, la, 1024, m.a0 * dpyp + m.a1 * dpy + m.a2 * (eypv + 2.f * veyp + ey));
-    sts(ad, la, 1280, m.a0 * dpzp + m.a1 * dpz + m.a2 * (ezpv + 2.f * vezp + ez));
+    stsn(ad, la, 768, m.a0 * dpxp + m.a1 * dpx + m.a2 * (expv + 2.f * vexp + ex));
+    stsn(ad, la, 1024, m.a0 * dpyp + m.a1 * dpy + m.a2 * (eypv + 2.f * veyp + ey));
+    stsn(ad, la, 1280, m.a0 * dpzp + m.a1 * dpz + m.a2 * (ezpv + 2.f * vezp + ez));
     if (POLES && m.poles) {
-        sts(ap, lp, 0, m.bp0[0] * pxp0 + m.bp1[0] * px0 + m.bp2[0] * expv + m.bp3[0] * vexp + m.bp4[0] * ex);
-        sts(ap, lp, 768, m.bp0[1] * pxp1 + m.bp1[1] * px1 + m.bp2[1] * expv + m.bp3[1] * vexp + m.bp4[1] * ex);
-        sts(ap, lp, 256, m.bp0[0] * pyp0 + m.bp1[0] * py0 + m.bp2[0] * eypv + m.bp3[0] * veyp + m.bp4[0] * ey);
-        sts(ap, lp, 1024, m.bp0[1] * pyp1 + m.bp1[1] * py1 + m.bp2[1] * eypv + m.bp3[1] * veyp + m.bp4[1] * ey);
-        sts(ap, lp, 512, m.bp0[0] * pzp0 + m.bp1[0] * pz0 + m.bp2[0] * ezpv + m.bp3[0] * vezp + m.bp4[0] * ez);
-        sts(ap, lp, 1280, m.bp0[1] * pzp1 + m.bp1[1] * pz1 + m.bp2[1] * ezpv + m.bp3[1] * vezp + m.bp4[1] * ez);
-        sts(ap, lp, 1536, px0); sts(ap, lp, 2304, px1); sts(ap, lp, 1792, py0);
-        sts(ap, lp, 2560, py1); sts(ap, lp, 2048, pz0); sts(ap, lp, 2816, pz1);
+        stsn(ap, lp, 0, m.bp0[0] * pxp0 + m.bp1[0] * px0 + m.bp2[0] * expv + m.bp3[0] * vexp + m.bp4[0] * ex);
+        stsn(ap, lp, 768, m.bp0[1] * pxp1 + m.bp1[1] * px1 + m.bp2[1] * expv + m.bp3[1] * vexp + m.bp4[1] * ex);
+        stsn(ap, lp, 256, m.bp0[0] * pyp0 + m.bp1[0] * py0 + m.bp2[0] * eypv + m.bp3[0] * veyp + m.bp4[0] * ey);
+        stsn(ap, lp, 1024, m.bp0[1] * pyp1 + m.bp1[1] * py1 + m.bp2[1] * eypv + m.bp3[1] * veyp + m.bp4[1] * ey);
+        stsn(ap, lp, 512, m.bp0[0] * pzp0 + m.bp1[0] * pz0 + m.bp2[0] * ezpv + m.bp3[0] * vezp + m.bp4[0] * ez);
+        stsn(ap, lp, 1280, m.bp0[1] * pzp1 + m.bp1[1] * pz1 + m.bp2[1] * ezpv + m.bp3[1] * vezp + m.bp4[1] * ez);
+        stsn(ap, lp, 1536, px0); stsn(ap, lp, 2304, px1); stsn(ap, lp, 1792, py0);
+        stsn(ap, lp, 2560, py1); stsn(ap, lp, 2048, pz0); stsn(ap, lp, 2816, pz1);
     }
-    sts(ad, la, 0, vexp);
-    sts(ad, la, 256, veyp);
-    sts(ad, la, 512, vezp);
-    sts(ad, la, 1536, dpx);
-    sts(ad, la, 1792, dpy);
-    sts(ad, la, 2048, dpz);
+    stsn(ad, la, 0, vexp);
+    stsn(ad, la, 256, veyp);
+    stsn(ad, la, 512, vezp);
+    stsn(ad, la, 1536, dpx);
+    stsn(ad, la, 1792, dpy);
+    stsn(ad, la, 2048, dpz);
 }
 
 template <typename ID, int ADE, typename OFF>
@@ -387,12 +420,12 @@ k_estep(GsvGrid g, int kchunk)
 
     float hxkm1 = ldf(g.hxg, o - dP), hykm1 = ldf(g.hyg, o - dP);
     for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
-        const float hx = ldf(g.hxg, o), hy = ldf(g.hyg, o), hz = ldf(g.hzg, o);
+        const float hx = ldn(g.hxg, o), hy = ldn(g.hyg, o), hz = ldn(g.hzg, o);
         const float hzj = ldf(g.hzg, o - dJ), hxj = ldf(g.hxg, o - dJ);
         const float hzi = ldf(g.hzg, o - 4), hyi = ldf(g.hyg, o - 4);
         const float rkz = g.rkez[k];
         const GsvECoef ec = g.etab[ldid<ID>(g.ecls, oc)];
-        float ex = ldf(g.exg, o), ey = ldf(g.eyg, o), ez = ldf(g.ezg, o);
+        float ex = ldn(g.exg, o), ey = ldn(g.eyg, o), ez = ldn(g.ezg, o);
         const unsigned fl = ec.flags;
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
 
@@ -494,9 +527,9 @@ k_estep(GsvGrid g, int kchunk)
             ey = 0.f;
             ez = 0.f;
         }
-        stf(g.exg, o, ex);
-        stf(g.eyg, o, ey);
-        stf(g.ezg, o, ez);
+        stn(g.exg, o, ex);
+        stn(g.eyg, o, ey);
+        stn(g.ezg, o, ez);
         hxkm1 = hx;
         hykm1 = hy;
     }
@@ -888,31 +921,61 @@ k_farfield_td(GsvGrid g, GsvFarfieldTD f, int step)
     }
 }
 
-/* running DFT of the four tangential field components on every NFFF box face
- * cell: acc += F(t) exp(-i w t), E at (step+1) dt, H at (step+1/2) dt */
+/* running DFT, sampling half: the four tangential components of every NFFF
+ * box face cell into ring slot `slot` */
 __global__ __launch_bounds__(256) void
-k_dft_faces(GsvGrid g, GsvDft d, int step)
+k_dft_sample(GsvGrid g, GsvDft d, int slot)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= d.ncells)
         return;
     const long long c = d.cell[q];
     const int face = d.face[q];
-    float v[4];
-    if (face < 2) { v[0] = g.ey[c]; v[1] = g.ez[c]; v[2] = g.hy[c]; v[3] = g.hz[c]; }
-    else if (face < 4) { v[0] = g.ex[c]; v[1] = g.ez[c]; v[2] = g.hx[c]; v[3] = g.hz[c]; }
-    else { v[0] = g.ex[c]; v[1] = g.ey[c]; v[2] = g.hx[c]; v[3] = g.hy[c]; }
-    double *acc = d.acc + q * d.nw * 8;
-    const double te = (step + 1) * d.dt, th = (step + 0.5) * d.dt;
-    for (int w = 0; w < d.nw; w++) {
+    float v0, v1, v2, v3;
+    if (face < 2) { v0 = g.ey[c]; v1 = g.ez[c]; v2 = g.hy[c]; v3 = g.hz[c]; }
+    else if (face < 4) { v0 = g.ex[c]; v1 = g.ez[c]; v2 = g.hx[c]; v3 = g.hz[c]; }
+    else { v0 = g.ex[c]; v1 = g.ey[c]; v2 = g.hx[c]; v3 = g.hy[c]; }
+    float *r = d.ring + (long long)slot * 4 * d.ncells + q;
+    r[0] = v0;
+    r[d.ncells] = v1;
+    r[2 * d.ncells] = v2;
+    r[3 * d.ncells] = v3;
+}
+
+/* running DFT, folding half: acc += F(t) exp(-i w t) for the pending samples
+ * of steps first_step .. first_step + n - 1 (slot = step % T), E sampled at
+ * (step+1) dt and H at (step+1/2) dt; the phases of the block's steps are
+ * tabulated in LDS (T * nw * 4 doubles, at most 48 KiB) */
+__global__ __launch_bounds__(256) void
+k_dft_flush(GsvDft d, int first_step, int n)
+{
+    extern __shared__ double ph[];   /* [n][nw][4]: cos E, sin E, cos H, sin H */
+    for (int t = threadIdx.x; t < n * d.nw; t += blockDim.x) {
+        const int s = t / d.nw, w = t % d.nw;
+        const double te = (first_step + s + 1) * d.dt, th = (first_step + s + 0.5) * d.dt;
         double se, ce, sh, ch;
         sincos(-d.omega[w] * te, &se, &ce);
         sincos(-d.omega[w] * th, &sh, &ch);
-        double *a = acc + w * 8;
-        a[0] += v[0] * ce; a[1] += v[0] * se;
-        a[2] += v[1] * ce; a[3] += v[1] * se;
-        a[4] += v[2] * ch; a[5] += v[2] * sh;
-        a[6] += v[3] * ch; a[7] += v[3] * sh;
+        ph[4 * t] = ce; ph[4 * t + 1] = se; ph[4 * t + 2] = ch; ph[4 * t + 3] = sh;
+    }
+    __syncthreads();
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.ncells)
+        return;
+    for (int w = 0; w < d.nw; w++) {
+        double *a = d.acc + (q * d.nw + w) * 8;
+        double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6], a7 = a[7];
+        for (int s = 0; s < n; s++) {
+            const int slot = (first_step + s) % d.T;
+            const float *r = d.ring + (long long)slot * 4 * d.ncells + q;
+            const double v0 = r[0], v1 = r[d.ncells], v2 = r[2 * d.ncells], v3 = r[3 * d.ncells];
+            const double *p = ph + 4 * (s * d.nw + w);
+            a0 += v0 * p[0]; a1 += v0 * p[1];
+            a2 += v1 * p[0]; a3 += v1 * p[1];
+            a4 += v2 * p[2]; a5 += v2 * p[3];
+            a6 += v3 * p[2]; a7 += v3 * p[3];
+        }
+        a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3; a[4] = a4; a[5] = a5; a[6] = a6; a[7] = a7;
     }
 }
 
@@ -1125,9 +1188,20 @@ gsv_launch_farfield_td(const GsvGrid *g, const GsvFarfieldTD *f, int step, hipSt
 }
 
 extern "C" hipError_t
-gsv_launch_dft(const GsvGrid *g, const GsvDft *d, int step, hipStream_t st)
+gsv_launch_dft_sample(const GsvGrid *g, const GsvDft *d, int slot, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_dft_faces, dim3((unsigned)((d->ncells + 255) / 256)), dim3(256), 0, st, *g, *d, step);
+    hipLaunchKernelGGL(k_dft_sample, dim3((unsigned)((d->ncells + 255) / 256)), dim3(256), 0, st, *g, *d, slot);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_dft_flush(const GsvDft *d, int first_step, int nsamples, hipStream_t st)
+{
+    if (nsamples <= 0)
+        return hipSuccess;
+    const size_t lds = (size_t)nsamples * d->nw * 4 * sizeof(double);
+    hipLaunchKernelGGL(k_dft_flush, dim3((unsigned)((d->ncells + 255) / 256)), dim3(256), lds, st, *d, first_step,
+                       nsamples);
     return hipGetLastError();
 }
 
