@@ -135,6 +135,19 @@ def main():
     log("bench: k_hstep %.3f ms (%.0f GB/s), k_estep %.3f ms (%.0f GB/s)"
         % (h_ms, h_bytes / h_ms / 1e6, e_ms, e_bytes / e_ms / 1e6))
 
+    # HBM traffic of the dominant kernel from the committed counter summary
+    # (rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by the copy-kernel
+    # calibration), used only when it describes this workload and launch shape
+    traffic, traffic_src = None, None
+    try:
+        with open(os.path.join(HERE, "profiles", "hbm_traffic_rough512.json")) as f:
+            tr = json.load(f)
+        if n == 512 and tr["launch_shape"] == list(shape) and dom in tr["kernels"]:
+            traffic = round(tr["kernels"][dom]["traffic_bytes"])
+            traffic_src = "profiles/hbm_traffic_rough512.md"
+    except (OSError, ValueError, KeyError):
+        pass
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -178,7 +191,8 @@ def main():
                    "e_classes": info["e_classes"], "h_classes": info["h_classes"],
                    "launch_shape_h": shape[:3], "launch_shape_e": shape[3:]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel_ms": round(dom_ms, 4), "algorithmic_bytes": dom_bytes,
                      "step_algorithmic_bytes": step_bytes,
                      "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
