@@ -1,0 +1,1008 @@
+/*
+ * adjoint.hip -- gradient of a running-DFT objective with respect to the
+ * relative permittivity of the design voxels, by the discrete adjoint of the
+ * engine's own FP32 time step (no reference counterpart: GSvit has no
+ * gradient mode).
+ *
+ * Objective (OBJECTIVE_DFT box, wavelength lambda, omega = 2 pi c / lambda):
+ *   J = sum_{cells in box} sum_{c = x,y,z} |F_c|^2,
+ *   F_c = sum_{n=0}^{N-1} E_c^{n+1} exp(-i omega t_n),  t_n = (n+1) dt,
+ * with E^{n+1} the field after step n (after its point sources).
+ *
+ * Scope: non-dispersive media, CPML or plain boundaries, hard E point
+ * sources.  One forward step is x^{n+1} = S_n M_E M_H x^n + s_n for the state
+ * x = (E, H, psi_E, psi_H); the backward pass applies the transposed steps in
+ * reverse order (M_H^T M_E^T S_n^T), fused into two kernels per step:
+ *   k_adj_e  (E cells)  adds the H-step transpose of step n+1 (curl^T of r,
+ *            psi_H gather), the objective's source, the point-source
+ *            projection, accumulates the coefficient gradient of the design
+ *            cells and applies the local part of the E-step transpose
+ *            (q = cb m, new adjoint E = ca m, psi_E adjoint);
+ *   k_adj_h  (H cells)  adds the E-step transpose (curl^T of q, psi_E
+ *            gather) and applies the local part of the H-step transpose
+ *            (r = db h, new adjoint H = da h, psi_H adjoint).
+ * The adjoint of psi is stored without its decay factor b, which the next
+ * (earlier) step applies when it reads it.
+ *
+ * The gradient needs, per design cell and step, E^n and the coefficient-free
+ * increment D^n = (E^{n+1} - ca E^n) / cb of the E update.  They are recorded
+ * for the whole run when they fit the history budget; otherwise the forward
+ * pass keeps full checkpoints every K steps and each K-step segment is run
+ * forward again (recording) just before its backward pass.
+ */
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include <omp.h>
+#include "engine_host.h"
+
+/* ------------------------------------------------------------------ */
+/* device parameter block                                               */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    /* objective monitor box */
+    int mi0, mj0, mk0, mnx, mny, mnz;
+    double omega, dt;
+    double *macc;             /* [cell][3][2] running DFT (re, im) of E */
+    /* adjoint state, same layout as the fields (k = 0 pointers) */
+    float *ebx, *eby, *ebz, *hbx, *hby, *hbz;
+    float *qx, *qy, *qz, *rx, *ry, *rz;
+    float2 *te[6], *th[6];    /* psi adjoints per face (E side, H side) */
+    /* design coefficient box: history and gradient accumulators */
+    long long ncoef;
+    float *hist;              /* [slot][6][ncoef]: E^n x y z, D^n x y z */
+    double *gca, *gcb;        /* [3][ncoef] */
+    /* hard E point sources */
+    int nsrc;
+    const long long *spos;
+    const float *svals;       /* [nsteps][nsrc][6] */
+} GsvAdj;
+
+/* ------------------------------------------------------------------ */
+/* forward-side kernels                                                 */
+/* ------------------------------------------------------------------ */
+
+/* objective DFT: acc += E(cell) exp(-i omega (step+1) dt) over the monitor box */
+__global__ __launch_bounds__(256) void
+k_obj_dft(GsvGrid g, GsvAdj a, int step)
+{
+    const long long n = (long long)a.mnx * a.mny * a.mnz;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n)
+        return;
+    const int i = a.mi0 + (int)(q % a.mnx);
+    const int j = a.mj0 + (int)((q / a.mnx) % a.mny);
+    const int k = a.mk0 + (int)(q / ((long long)a.mnx * a.mny));
+    const long long c = i + (long long)g.px * j + g.plane * k;
+    double s, co;
+    sincos(a.omega * (step + 1) * a.dt, &s, &co);
+    double *acc = a.macc + q * 6;
+    const float e[3] = { g.ex[c], g.ey[c], g.ez[c] };
+    for (int comp = 0; comp < 3; comp++) {
+        acc[2 * comp] += e[comp] * co;
+        acc[2 * comp + 1] -= e[comp] * s;
+    }
+}
+
+/* history, before the E update: E^n of the design coefficient cells */
+__global__ __launch_bounds__(256) void
+k_rec_pre(GsvGrid g, GsvAdj a, int slot)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.ncoef)
+        return;
+    const int i = g.pc_i0 + (int)(q % g.pc_nx);
+    const int j = g.pc_j0 + (int)((q / g.pc_nx) % g.pc_ny);
+    const int k = g.pc_k0 + (int)(q / ((long long)g.pc_nx * g.pc_ny));
+    const long long c = i + (long long)g.px * j + g.plane * k;
+    float *h = a.hist + (long long)slot * 6 * a.ncoef + q;
+    h[0] = g.ex[c];
+    h[a.ncoef] = g.ey[c];
+    h[2 * a.ncoef] = g.ez[c];
+}
+
+/* history, after the E update (before the point sources): D^n = (E^{n+1} - ca E^n) / cb */
+__global__ __launch_bounds__(256) void
+k_rec_post(GsvGrid g, GsvAdj a, int slot)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.ncoef)
+        return;
+    const int i = g.pc_i0 + (int)(q % g.pc_nx);
+    const int j = g.pc_j0 + (int)((q / g.pc_nx) % g.pc_ny);
+    const int k = g.pc_k0 + (int)(q / ((long long)g.pc_nx * g.pc_ny));
+    const long long c = i + (long long)g.px * j + g.plane * k;
+    const GsvECoef ec = g.pctab[q];
+    float *h = a.hist + (long long)slot * 6 * a.ncoef + q;
+    const float e1[3] = { g.ex[c], g.ey[c], g.ez[c] };
+    for (int comp = 0; comp < 3; comp++) {
+        const float e0 = h[comp * a.ncoef];
+        h[(3 + comp) * a.ncoef] = ec.cb[comp] != 0.f ? (e1[comp] - ec.ca[comp] * e0) / ec.cb[comp] : 0.f;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* backward kernels                                                     */
+/* ------------------------------------------------------------------ */
+
+template <typename ID>
+__device__ __forceinline__ GsvECoef
+e_coef(const GsvGrid &g, int i, int j, int k, long long c)
+{
+    GsvECoef ec = g.etab[((const ID *)g.ecls)[c]];
+    if (ec.flags & GSV_E_PERCELL)
+        ec = g.pctab[(i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))];
+    return ec;
+}
+
+/* E cells: H-step transpose of step n+1 (gathers of r and the psi_H
+ * adjoints), objective source and point-source projection of step n,
+ * coefficient gradient, local part of the E-step transpose of step n */
+template <typename ID>
+__global__ __launch_bounds__(256) void
+k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int first)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (i >= nx || j >= ny)
+        return;
+    const int px = g.px;
+    const long long P = g.plane;
+    const long long c = i + (long long)px * j + P * k;
+    float ex = a.ebx[c], ey = a.eby[c], ez = a.ebz[c];
+
+    if (!first) {
+        /* transpose of the H curl of step n+1 (H-grid 1/kappa) */
+        const float rxc = a.rx[c], ryc = a.ry[c], rzc = a.rz[c];
+        const float rkx = g.rkhx[i], rky = g.rkhy[j], rkz = g.rkhz[k];
+        ex += rkz * ryc - rky * rzc;
+        ey += rkx * rzc - rkz * rxc;
+        ez += rky * rxc - rkx * ryc;
+        if (k > 0) {
+            const float z1 = g.rkhz[k - 1];
+            ex -= z1 * a.ry[c - P];
+            ey += z1 * a.rx[c - P];
+        }
+        if (j > 0) {
+            const float y1 = g.rkhy[j - 1];
+            ex += y1 * a.rz[c - px];
+            ez -= y1 * a.rx[c - px];
+        }
+        if (i > 0) {
+            const float x1 = g.rkhx[i - 1];
+            ey -= x1 * a.rz[c - 1];
+            ez += x1 * a.ry[c - 1];
+        }
+        /* transpose of the H-side psi updates of step n+1 */
+        const bool jin = j >= 1 && j < ny - 1, kin = k >= 1 && k < nz - 1, iin = i >= 1 && i < nx - 1;
+        const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
+        if (jin && kin) {
+            if (F0.on) {
+                const long long row = ((long long)k * ny + j) * F0.depth;
+                if (i < F0.depth - 1) {
+                    const float cc = F0.ch[i].y;
+                    const float2 t = a.th[0][row + i];
+                    ey -= cc * t.x;
+                    ez -= cc * t.y;
+                }
+                if (i >= 1 && i - 1 < F0.depth - 1) {
+                    const float cc = F0.ch[i - 1].y;
+                    const float2 t = a.th[0][row + i - 1];
+                    ey += cc * t.x;
+                    ez += cc * t.y;
+                }
+            }
+            if (F1.on) {
+                const int lo = nx - F1.depth + 1;
+                const long long row = ((long long)k * ny + j) * F1.depth;
+                if (i >= lo && i < nx - 1) {
+                    const int ir = i - (nx - F1.depth);
+                    const float cc = F1.ch[ir].y;
+                    const float2 t = a.th[1][row + ir];
+                    ey -= cc * t.x;
+                    ez -= cc * t.y;
+                }
+                if (i - 1 >= lo && i - 1 < nx - 1) {
+                    const int ir = i - 1 - (nx - F1.depth);
+                    const float cc = F1.ch[ir].y;
+                    const float2 t = a.th[1][row + ir];
+                    ey += cc * t.x;
+                    ez += cc * t.y;
+                }
+            }
+        }
+        if (iin && kin) {
+            if (F2.on) {
+                if (j < F2.depth - 1) {
+                    const float cc = F2.ch[j].y;
+                    const float2 t = a.th[2][((long long)k * F2.depth + j) * px + i];
+                    ez -= cc * t.x;
+                    ex -= cc * t.y;
+                }
+                if (j >= 1 && j - 1 < F2.depth - 1) {
+                    const float cc = F2.ch[j - 1].y;
+                    const float2 t = a.th[2][((long long)k * F2.depth + j - 1) * px + i];
+                    ez += cc * t.x;
+                    ex += cc * t.y;
+                }
+            }
+            if (F3.on) {
+                const int lo = ny - F3.depth + 1;
+                if (j >= lo && j < ny - 1) {
+                    const int jr = j - (ny - F3.depth);
+                    const float cc = F3.ch[jr].y;
+                    const float2 t = a.th[3][((long long)k * F3.depth + jr) * px + i];
+                    ez -= cc * t.x;
+                    ex -= cc * t.y;
+                }
+                if (j - 1 >= lo && j - 1 < ny - 1) {
+                    const int jr = j - 1 - (ny - F3.depth);
+                    const float cc = F3.ch[jr].y;
+                    const float2 t = a.th[3][((long long)k * F3.depth + jr) * px + i];
+                    ez += cc * t.x;
+                    ex += cc * t.y;
+                }
+            }
+        }
+        if (iin && jin) {
+            if (F4.on) {
+                if (k < F4.depth - 1) {
+                    const float cc = F4.ch[k].y;
+                    const float2 t = a.th[4][((long long)k * ny + j) * px + i];
+                    ex -= cc * t.x;
+                    ey -= cc * t.y;
+                }
+                if (k >= 1 && k - 1 < F4.depth - 1) {
+                    const float cc = F4.ch[k - 1].y;
+                    const float2 t = a.th[4][((long long)(k - 1) * ny + j) * px + i];
+                    ex += cc * t.x;
+                    ey += cc * t.y;
+                }
+            }
+            if (F5.on) {
+                const int lo = nz - F5.depth + 1;
+                if (k >= lo && k < nz - 1) {
+                    const int kr = k - (nz - F5.depth);
+                    const float cc = F5.ch[kr].y;
+                    const float2 t = a.th[5][((long long)kr * ny + j) * px + i];
+                    ex -= cc * t.x;
+                    ey -= cc * t.y;
+                }
+                if (k - 1 >= lo && k - 1 < nz - 1) {
+                    const int kr = k - 1 - (nz - F5.depth);
+                    const float cc = F5.ch[kr].y;
+                    const float2 t = a.th[5][((long long)kr * ny + j) * px + i];
+                    ex += cc * t.x;
+                    ey += cc * t.y;
+                }
+            }
+        }
+    }
+
+    /* objective source: dJ/dE^{n+1} = 2 (Re F cos - Im F sin) */
+    if (i >= a.mi0 && i < a.mi0 + a.mnx && j >= a.mj0 && j < a.mj0 + a.mny && k >= a.mk0 && k < a.mk0 + a.mnz) {
+        const long long q = (i - a.mi0) + (long long)a.mnx * ((j - a.mj0) + (long long)a.mny * (k - a.mk0));
+        const double *F = a.macc + q * 6;
+        double s, co;
+        sincos(a.omega * (step + 1) * a.dt, &s, &co);
+        ex += (float)(2.0 * (F[0] * co - F[1] * s));
+        ey += (float)(2.0 * (F[2] * co - F[3] * s));
+        ez += (float)(2.0 * (F[4] * co - F[5] * s));
+    }
+    /* hard point sources overwrite E: their cells carry no adjoint back */
+    for (int s = 0; s < a.nsrc; s++)
+        if (a.spos[s] == c) {
+            const float *v = a.svals + ((long long)step * a.nsrc + s) * 6;
+            if (v[0] != 0.f) ex = 0.f;
+            if (v[1] != 0.f) ey = 0.f;
+            if (v[2] != 0.f) ez = 0.f;
+        }
+
+    /* E-step transpose: plane zeroing, then the update and the psi_E corrections */
+    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
+    if ((F0.on && i == 0) || (F2.on && j == 0) || (F4.on && k == 0)) {
+        ex = 0.f;
+        ey = 0.f;
+        ez = 0.f;
+    }
+    const GsvECoef ec = e_coef<ID>(g, i, j, k, c);
+    const unsigned fl = ec.flags;
+    if (a.ncoef && i >= g.pc_i0 && i < g.pc_i0 + g.pc_nx && j >= g.pc_j0 && j < g.pc_j0 + g.pc_ny && k >= g.pc_k0
+        && k < g.pc_k0 + g.pc_nz) {
+        /* design coefficient cell: dJ/dca += adj(E^{n+1}) E^n, dJ/dcb += adj(E^{n+1}) D^n */
+        const long long q = (i - g.pc_i0) + (long long)g.pc_nx * ((j - g.pc_j0) + (long long)g.pc_ny * (k - g.pc_k0));
+        const float *h = a.hist + (long long)slot * 6 * a.ncoef + q;
+        a.gca[q] += (double)ex * h[0];
+        a.gca[a.ncoef + q] += (double)ey * h[a.ncoef];
+        a.gca[2 * a.ncoef + q] += (double)ez * h[2 * a.ncoef];
+        a.gcb[q] += (double)ex * h[3 * a.ncoef];
+        a.gcb[a.ncoef + q] += (double)ey * h[4 * a.ncoef];
+        a.gcb[2 * a.ncoef + q] += (double)ez * h[5 * a.ncoef];
+    }
+    const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
+    /* psi_E adjoints: decay of the later step, plus the corrections of this one */
+    if (F0.on && i >= 1 && i < F0.depth) {
+        float2 *t = a.te[0] + ((long long)k * ny + j) * F0.depth + i;
+        const float b = F0.ce[i].x;
+        float t1 = b * t->x, t2 = b * t->y;
+        if (j != 0 && k != 0) {
+            if (ty) t1 -= g.dx * ec.cb[1] * ey;
+            if (tz) t2 += g.dx * ec.cb[2] * ez;
+        }
+        *t = make_float2(t1, t2);
+    }
+    if (F2.on && j >= 1 && j < F2.depth) {
+        float2 *t = a.te[2] + ((long long)k * F2.depth + j) * px + i;
+        const float b = F2.ce[j].x;
+        float t1 = b * t->x, t2 = b * t->y;
+        if (i != 0 && k != 0) {
+            if (tx) t1 += g.dy * ec.cb[0] * ex;
+            if (tz) t2 -= g.dy * ec.cb[2] * ez;
+        }
+        *t = make_float2(t1, t2);
+    }
+    if (F4.on && k >= 1 && k < F4.depth) {
+        float2 *t = a.te[4] + ((long long)k * ny + j) * px + i;
+        const float b = F4.ce[k].x;
+        float t1 = b * t->x, t2 = b * t->y;
+        if (i != 0 && j != 0) {
+            if (ty) t1 += g.dx * ec.cb[1] * ey;
+            if (tz) t2 -= g.dx * ec.cb[0] * ex;
+        }
+        *t = make_float2(t1, t2);
+    }
+    if (F1.on && i >= nx - F1.depth + 1) {
+        const int ir = i - (nx - F1.depth);
+        float2 *t = a.te[1] + ((long long)k * ny + j) * F1.depth + ir;
+        const float b = F1.ce[ir].x;
+        float t1 = b * t->x, t2 = b * t->y;
+        if (j != 0 && k != 0) {
+            if (ty) t1 -= g.dx * ec.cb[1] * ey;
+            if (tz) t2 += g.dx * ec.cb[2] * ez;
+        }
+        *t = make_float2(t1, t2);
+    }
+    if (F3.on && j >= ny - F3.depth + 1) {
+        const int jr = j - (ny - F3.depth);
+        float2 *t = a.te[3] + ((long long)k * F3.depth + jr) * px + i;
+        const float b = F3.ce[jr].x;
+        float t1 = b * t->x, t2 = b * t->y;
+        if (i != 0 && k != 0) {
+            if (tx) t1 += g.dy * ec.cb[0] * ex;
+            if (tz) t2 -= g.dy * ec.cb[2] * ez;
+        }
+        *t = make_float2(t1, t2);
+    }
+    if (F5.on && k >= nz - F5.depth + 1) {
+        const int kr = k - (nz - F5.depth);
+        float2 *t = a.te[5] + ((long long)kr * ny + j) * px + i;
+        const float b = F5.ce[kr].x;
+        float t1 = b * t->x, t2 = b * t->y;
+        if (i != 0 && j != 0) {
+            if (ty) t1 += g.dx * ec.cb[1] * ey;
+            if (tx) t2 -= g.dx * ec.cb[0] * ex;
+        }
+        *t = make_float2(t1, t2);
+    }
+    /* PEC zeroing acts on the curl update only (the corrections come after it) */
+    const float mx = (fl & GSV_E_PECX) ? 0.f : ex, my = (fl & GSV_E_PECY) ? 0.f : ey, mz = (fl & GSV_E_PECZ) ? 0.f : ez;
+    const bool ux = tx && j > 0 && k > 0, uy = ty && i > 0 && k > 0, uz = tz && i > 0 && j > 0;
+    a.qx[c] = ux ? ec.cb[0] * mx : 0.f;
+    a.qy[c] = uy ? ec.cb[1] * my : 0.f;
+    a.qz[c] = uz ? ec.cb[2] * mz : 0.f;
+    a.ebx[c] = ux ? ec.ca[0] * mx : mx;
+    a.eby[c] = uy ? ec.ca[1] * my : my;
+    a.ebz[c] = uz ? ec.ca[2] * mz : mz;
+}
+
+/* H cells: E-step transpose (gathers of q and the psi_E adjoints), then the
+ * local part of the H-step transpose */
+template <typename ID>
+__global__ __launch_bounds__(256) void
+k_adj_h(GsvGrid g, GsvAdj a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    if (i >= nx || j >= ny)
+        return;
+    const int px = g.px;
+    const long long P = g.plane;
+    const long long c = i + (long long)px * j + P * k;
+    float hx = a.hbx[c], hy = a.hby[c], hz = a.hbz[c];
+
+    /* transpose of the E curl (E-grid 1/kappa) */
+    {
+        const float qxc = a.qx[c], qyc = a.qy[c], qzc = a.qz[c];
+        const float rkx = g.rkex[i], rky = g.rkey[j], rkz = g.rkez[k];
+        hx += rkz * qyc - rky * qzc;
+        hy += rkx * qzc - rkz * qxc;
+        hz += rky * qxc - rkx * qyc;
+        if (k + 1 < nz) {
+            const float z1 = g.rkez[k + 1];
+            hx -= z1 * a.qy[c + P];
+            hy += z1 * a.qx[c + P];
+        }
+        if (j + 1 < ny) {
+            const float y1 = g.rkey[j + 1];
+            hx += y1 * a.qz[c + px];
+            hz -= y1 * a.qx[c + px];
+        }
+        if (i + 1 < nx) {
+            const float x1 = g.rkex[i + 1];
+            hy -= x1 * a.qz[c + 1];
+            hz += x1 * a.qy[c + 1];
+        }
+    }
+    /* transpose of the E-side psi updates */
+    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
+    if (F0.on) {
+        const long long row = ((long long)k * ny + j) * F0.depth;
+        if (i >= 1 && i < F0.depth) {
+            const float cc = F0.ce[i].y;
+            const float2 t = a.te[0][row + i];
+            hz += cc * t.x;
+            hy += cc * t.y;
+        }
+        if (i + 1 >= 1 && i + 1 < F0.depth) {
+            const float cc = F0.ce[i + 1].y;
+            const float2 t = a.te[0][row + i + 1];
+            hz -= cc * t.x;
+            hy -= cc * t.y;
+        }
+    }
+    if (F1.on) {
+        const int lo = nx - F1.depth + 1;
+        const long long row = ((long long)k * ny + j) * F1.depth;
+        if (i >= lo) {
+            const int ir = i - (nx - F1.depth);
+            const float cc = F1.ce[ir].y;
+            const float2 t = a.te[1][row + ir];
+            hz += cc * t.x;
+            hy += cc * t.y;
+        }
+        if (i + 1 >= lo && i + 1 < nx) {
+            const int ir = i + 1 - (nx - F1.depth);
+            const float cc = F1.ce[ir].y;
+            const float2 t = a.te[1][row + ir];
+            hz -= cc * t.x;
+            hy -= cc * t.y;
+        }
+    }
+    if (F2.on) {
+        if (j >= 1 && j < F2.depth) {
+            const float cc = F2.ce[j].y;
+            const float2 t = a.te[2][((long long)k * F2.depth + j) * px + i];
+            hz += cc * t.x;
+            hx += cc * t.y;
+        }
+        if (j + 1 >= 1 && j + 1 < F2.depth) {
+            const float cc = F2.ce[j + 1].y;
+            const float2 t = a.te[2][((long long)k * F2.depth + j + 1) * px + i];
+            hz -= cc * t.x;
+            hx -= cc * t.y;
+        }
+    }
+    if (F3.on) {
+        const int lo = ny - F3.depth + 1;
+        if (j >= lo) {
+            const int jr = j - (ny - F3.depth);
+            const float cc = F3.ce[jr].y;
+            const float2 t = a.te[3][((long long)k * F3.depth + jr) * px + i];
+            hz += cc * t.x;
+            hx += cc * t.y;
+        }
+        if (j + 1 >= lo && j + 1 < ny) {
+            const int jr = j + 1 - (ny - F3.depth);
+            const float cc = F3.ce[jr].y;
+            const float2 t = a.te[3][((long long)k * F3.depth + jr) * px + i];
+            hz -= cc * t.x;
+            hx -= cc * t.y;
+        }
+    }
+    if (F4.on) {
+        if (k >= 1 && k < F4.depth) {
+            const float cc = F4.ce[k].y;
+            const float2 t = a.te[4][((long long)k * ny + j) * px + i];
+            hx += cc * t.x;
+            hy += cc * t.y;
+        }
+        if (k + 1 >= 1 && k + 1 < F4.depth) {
+            const float cc = F4.ce[k + 1].y;
+            const float2 t = a.te[4][((long long)(k + 1) * ny + j) * px + i];
+            hx -= cc * t.x;
+            hy -= cc * t.y;
+        }
+    }
+    if (F5.on) {
+        const int lo = nz - F5.depth + 1;
+        if (k >= lo) {
+            const int kr = k - (nz - F5.depth);
+            const float cc = F5.ce[kr].y;
+            const float2 t = a.te[5][((long long)kr * ny + j) * px + i];
+            hx += cc * t.x;
+            hy += cc * t.y;
+        }
+        if (k + 1 >= lo && k + 1 < nz) {
+            const int kr = k + 1 - (nz - F5.depth);
+            const float cc = F5.ce[kr].y;
+            const float2 t = a.te[5][((long long)kr * ny + j) * px + i];
+            hx -= cc * t.x;
+            hy -= cc * t.y;
+        }
+    }
+
+    /* H-step transpose: zeroing, then the update and the psi_H corrections */
+    const bool iin = (i >= 1 && i < nx - 1), jin = (j >= 1 && j < ny - 1), ijin = iin && jin;
+    const bool kin = (k >= 1 && k < nz - 1);
+    const bool zx = F1.on && i == nx - 1 && jin, zy = F3.on && j == ny - 1 && iin;
+    if ((kin && (zx || zy)) || (F5.on && k == nz - 1 && ijin)) {
+        hx = 0.f;
+        hy = 0.f;
+        hz = 0.f;
+    }
+    const GsvHCoef hc = g.htab[((const ID *)g.hcls)[c]];
+    if (kin) {
+        if (F0.on && i < F0.depth - 1 && jin) {
+            float2 *t = a.th[0] + ((long long)k * ny + j) * F0.depth + i;
+            const float b = F0.ch[i].x;
+            *t = make_float2(b * t->x - g.dx * hc.dbc * hz, b * t->y + g.dx * hc.dbc * hy);
+        }
+        if (F1.on && i >= nx - F1.depth + 1 && i < nx - 1 && jin) {
+            const int ir = i - (nx - F1.depth);
+            float2 *t = a.th[1] + ((long long)k * ny + j) * F1.depth + ir;
+            const float b = F1.ch[ir].x;
+            *t = make_float2(b * t->x - g.dx * hc.dbc * hz, b * t->y + g.dx * hc.dbc * hy);
+        }
+        if (F2.on && j < F2.depth - 1 && iin) {
+            float2 *t = a.th[2] + ((long long)k * F2.depth + j) * px + i;
+            const float b = F2.ch[j].x;
+            *t = make_float2(b * t->x - g.dy * hc.dbc * hx, b * t->y + g.dy * hc.dbc * hz);
+        }
+        if (F3.on && j >= ny - F3.depth + 1 && j < ny - 1 && iin) {
+            const int jr = j - (ny - F3.depth);
+            float2 *t = a.th[3] + ((long long)k * F3.depth + jr) * px + i;
+            const float b = F3.ch[jr].x;
+            *t = make_float2(b * t->x - g.dy * hc.dbc * hx, b * t->y + g.dy * hc.dbc * hz);
+        }
+    }
+    if (ijin) {
+        if (F4.on && k < F4.depth - 1) {
+            float2 *t = a.th[4] + ((long long)k * ny + j) * px + i;
+            const float b = F4.ch[k].x;
+            *t = make_float2(b * t->x - g.dz * hc.dbc * hy, b * t->y + g.dz * hc.dbc * hx);
+        }
+        if (F5.on && k >= nz - F5.depth + 1 && k < nz - 1) {
+            const int kr = k - (nz - F5.depth);
+            float2 *t = a.th[5] + ((long long)kr * ny + j) * px + i;
+            const float b = F5.ch[kr].x;
+            *t = make_float2(b * t->x - g.dz * hc.dbc * hy, b * t->y + g.dz * hc.dbc * hx);
+        }
+    }
+    const bool ux = !(j == ny - 1 || k == nz - 1), uy = !(i == nx - 1 || k == nz - 1), uz = !(i == nx - 1 || j == ny - 1);
+    a.rx[c] = ux ? hc.db[0] * hx : 0.f;
+    a.ry[c] = uy ? hc.db[1] * hy : 0.f;
+    a.rz[c] = uz ? hc.db[2] * hz : 0.f;
+    a.hbx[c] = ux ? hc.da * hx : hx;
+    a.hby[c] = uy ? hc.da * hy : hy;
+    a.hbz[c] = uz ? hc.da * hz : hz;
+}
+
+/* ------------------------------------------------------------------ */
+/* host driver                                                          */
+/* ------------------------------------------------------------------ */
+
+struct GsvAdjoint {
+    GsvAdj a;
+    int nsteps;
+    int K;                      /* checkpoint interval (steps per segment) */
+    int history_all;            /* the whole run's history fits: no recomputation */
+    int hist_slots;
+    int recording, record_base; /* recording on, step of history slot 0 */
+    int accumulate;             /* objective DFT accumulation on */
+    std::vector<float *> ckpt;  /* per segment: 6 fields then the psi pairs */
+    size_t ckpt_floats;
+    std::vector<float> src_vals_host;
+    double ms_forward, ms_backward;
+};
+
+static size_t
+psi_pairs(const GsvCpmlFace &F, int axis, const GsvGrid &g)
+{
+    if (!F.on)
+        return 0;
+    if (axis == 0) return (size_t)F.depth * g.ny * g.nz;
+    if (axis == 1) return (size_t)g.px * F.depth * g.nz;
+    return (size_t)g.px * g.ny * F.depth;
+}
+
+void
+gsv_adjoint_free(Engine *E)
+{
+    if (E->adj) {
+        delete E->adj;
+        E->adj = NULL;
+    }
+}
+
+/* device buffers of the gradient state (allocated once per engine) */
+extern "C" int
+gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_interval)
+{
+    Engine *E = (Engine *)eng;
+    SvSet *set = E->set;
+    GsvGrid &g = E->g;
+    if (!E->ready)
+        return 1;
+    if (E->adj)
+        return 0;
+    if (!set->sa.design || !set->sa.objective) {
+        fprintf(stderr, "Error: the gradient needs DESIGN_BOX and OBJECTIVE_DFT in the parameter file\n");
+        return 1;
+    }
+    if (g.ade || E->has_tsf || E->has_liao || E->has_per) {
+        fprintf(stderr, "Error: the gradient supports non-dispersive media, CPML and point sources only\n");
+        return 1;
+    }
+    if (E->has_src)
+        for (size_t q = 0; q < (size_t)E->src.nsteps * E->src.n; q++)
+            for (int c = 3; c < 6; c++)
+                if (E->src_vals_host[q * 6 + c] != 0.f) {
+                    fprintf(stderr, "Error: the gradient supports E point sources only\n");
+                    return 1;
+                }
+    CK(hipSetDevice(E->dev));
+    GsvAdjoint *A = new GsvAdjoint();
+    GsvAdj &a = A->a;
+    memset(&a, 0, sizeof(a));
+    const SvSetAdjoint &sa = set->sa;
+    a.mi0 = sa.obj_i0; a.mj0 = sa.obj_j0; a.mk0 = sa.obj_k0;
+    a.mnx = sa.obj_in - sa.obj_i0; a.mny = sa.obj_jn - sa.obj_j0; a.mnz = sa.obj_kn - sa.obj_k0;
+    a.omega = 2 * M_PI * SV_LIGHT_SPEED / sa.obj_lambda;
+    a.dt = set->plan.dt;
+    a.ncoef = (long long)g.pc_nx * g.pc_ny * g.pc_nz;
+    A->nsteps = set->sc.nsteps;
+    if (dmalloc(E, &a.macc, (size_t)a.mnx * a.mny * a.mnz * 6)) { delete A; return 1; }
+    float **fa[12] = { &a.ebx, &a.eby, &a.ebz, &a.hbx, &a.hby, &a.hbz, &a.qx, &a.qy, &a.qz, &a.rx, &a.ry, &a.rz };
+    for (int q = 0; q < 12; q++) {
+        float *p;
+        if (dmalloc(E, &p, E->field_floats)) { delete A; return 1; }
+        CK(hipMemsetAsync(p, 0, E->field_floats * 4, E->st));
+        *fa[q] = p + g.plane;
+    }
+    for (int f = 0; f < 6; f++) {
+        size_t n = psi_pairs(g.f[f], f / 2, g);
+        if (n) {
+            if (dmalloc(E, &a.te[f], n) || dmalloc(E, &a.th[f], n)) { delete A; return 1; }
+        }
+    }
+    if (dmalloc(E, &a.gca, 3 * (size_t)a.ncoef) || dmalloc(E, &a.gcb, 3 * (size_t)a.ncoef)) { delete A; return 1; }
+    /* history: all steps if they fit the budget, else one segment of K steps */
+    size_t per_step = 6 * (size_t)a.ncoef * 4;
+    A->K = ckpt_interval > 0 ? ckpt_interval : 100;
+    A->history_all = (long long)per_step * A->nsteps <= history_budget_bytes;
+    A->hist_slots = A->history_all ? A->nsteps : (A->K < A->nsteps ? A->K : A->nsteps);
+    if (dmalloc(E, &a.hist, per_step / 4 * A->hist_slots)) { delete A; return 1; }
+    if (!A->history_all) {
+        size_t nf = 6 * E->field_floats;
+        for (int f = 0; f < 6; f++)
+            nf += 2 * 2 * psi_pairs(g.f[f], f / 2, g);
+        A->ckpt_floats = nf;
+        int nseg = (A->nsteps + A->K - 1) / A->K;
+        for (int s = 0; s < nseg; s++) {
+            float *p;
+            if (dmalloc(E, &p, nf)) { delete A; return 1; }
+            A->ckpt.push_back(p);
+        }
+    }
+    if (E->has_src) {
+        a.nsrc = E->src.n;
+        a.spos = E->src.pos;
+        a.svals = E->src.vals;
+    }
+    CK(hipStreamSynchronize(E->st));
+    E->adj = A;
+    return 0;
+}
+
+/* state copy between the live fields / psi and a checkpoint buffer */
+static int
+ckpt_copy(Engine *E, float *buf, int save)
+{
+    GsvGrid &g = E->g;
+    size_t off = 0;
+    for (int q = 0; q < 6; q++) {
+        if (save)
+            CK(hipMemcpyAsync(buf + off, E->fbase[q], E->field_floats * 4, hipMemcpyDeviceToDevice, E->st));
+        else
+            CK(hipMemcpyAsync(E->fbase[q], buf + off, E->field_floats * 4, hipMemcpyDeviceToDevice, E->st));
+        off += E->field_floats;
+    }
+    for (int f = 0; f < 6; f++) {
+        size_t n = psi_pairs(g.f[f], f / 2, g);
+        if (!n)
+            continue;
+        float2 *ps[2] = { g.f[f].pe, g.f[f].ph };
+        for (int s = 0; s < 2; s++) {
+            if (save)
+                CK(hipMemcpyAsync(buf + off, ps[s], n * 8, hipMemcpyDeviceToDevice, E->st));
+            else
+                CK(hipMemcpyAsync(ps[s], buf + off, n * 8, hipMemcpyDeviceToDevice, E->st));
+            off += 2 * n;
+        }
+    }
+    return 0;
+}
+
+/* hooks called by the step entry points (gpu_engine.hip) */
+int
+gsv_adjoint_pre_e(Engine *E, int step)
+{
+    GsvAdjoint *A = E->adj;
+    if (!A || !A->recording)
+        return 0;
+    const long long n = A->a.ncoef;
+    hipLaunchKernelGGL(k_rec_pre, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, E->st, E->g, A->a,
+                       step - A->record_base);
+    CK(hipGetLastError());
+    return 0;
+}
+
+int
+gsv_adjoint_post_e(Engine *E, int step)
+{
+    GsvAdjoint *A = E->adj;
+    if (!A || !A->recording)
+        return 0;
+    const long long n = A->a.ncoef;
+    hipLaunchKernelGGL(k_rec_post, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, E->st, E->g, A->a,
+                       step - A->record_base);
+    CK(hipGetLastError());
+    return 0;
+}
+
+int
+gsv_adjoint_after_step(Engine *E, int step)
+{
+    GsvAdjoint *A = E->adj;
+    if (!A || !A->accumulate)
+        return 0;
+    const long long n = (long long)A->a.mnx * A->a.mny * A->a.mnz;
+    hipLaunchKernelGGL(k_obj_dft, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, E->st, E->g, A->a, step);
+    CK(hipGetLastError());
+    return 0;
+}
+
+/* zero fields, psi and the step counter (a fresh run of the same plan) */
+int
+gsv_engine_reset_state(Engine *E)
+{
+    GsvGrid &g = E->g;
+    for (int q = 0; q < 6; q++)
+        CK(hipMemsetAsync(E->fbase[q], 0, E->field_floats * 4, E->st));
+    for (int f = 0; f < 6; f++) {
+        size_t n = psi_pairs(g.f[f], f / 2, g);
+        if (n) {
+            CK(hipMemsetAsync(g.f[f].pe, 0, n * 8, E->st));
+            CK(hipMemsetAsync(g.f[f].ph, 0, n * 8, E->st));
+        }
+    }
+    CK(hipStreamSynchronize(E->st));
+    return 0;
+}
+
+/* the step sequence of one forward step (src3d/svit.c:239-257) */
+typedef int (*GsvStepFn)(void *ctx);
+
+/* Forward pass for the gradient: fresh state, every step, objective DFT,
+ * checkpoints (or the full history); returns J. */
+extern "C" int
+gsv_engine_adjoint_forward(void *eng, int (*step_fn)(void *), void *ctx, double *J)
+{
+    Engine *E = (Engine *)eng;
+    GsvAdjoint *A = E->adj;
+    SvSet *set = E->set;
+    if (!A)
+        return 1;
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    if (gsv_engine_reset_state(E))
+        return 1;
+    const size_t nm = (size_t)A->a.mnx * A->a.mny * A->a.mnz * 6;
+    CK(hipMemsetAsync(A->a.macc, 0, nm * 8, E->st));
+    set->sc.step_act = 0;
+    A->accumulate = 1;
+    A->recording = A->history_all;
+    A->record_base = 0;
+    CK(hipEventRecord(e0, E->st));
+    for (int n = 0; n < A->nsteps; n++) {
+        if (!A->history_all && n % A->K == 0 && ckpt_copy(E, A->ckpt[n / A->K], 1))
+            return 1;
+        if (step_fn(ctx))
+            return 1;
+    }
+    CK(hipEventRecord(e1, E->st));
+    A->accumulate = 0;
+    A->recording = 0;
+    std::vector<double> F(nm);
+    CK(hipMemcpyAsync(F.data(), A->a.macc, nm * 8, hipMemcpyDeviceToHost, E->st));
+    CK(hipStreamSynchronize(E->st));
+    float ms;
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    A->ms_forward = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    double j = 0;
+    for (size_t q = 0; q < nm; q++)
+        j += F[q] * F[q];
+    *J = j;
+    return 0;
+}
+
+template <typename ID>
+static void
+launch_adj(Engine *E, int step, int slot, int first)
+{
+    GsvGrid &g = E->g;
+    dim3 blk(64, 4, 1), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+    hipLaunchKernelGGL(k_adj_e<ID>, grd, blk, 0, E->st, g, E->adj->a, step, slot, first);
+    hipLaunchKernelGGL(k_adj_h<ID>, grd, blk, 0, E->st, g, E->adj->a);
+}
+
+/* Backward pass: coefficient gradients accumulated on the device; the host
+ * chains them through the permittivity averaging to the design voxels.
+ * grad: dJ / d eps_r per design voxel, [i][j][k] order over the design box. */
+extern "C" int
+gsv_engine_adjoint_backward(void *eng, int (*step_fn)(void *), void *ctx, float *grad)
+{
+    Engine *E = (Engine *)eng;
+    GsvAdjoint *A = E->adj;
+    SvSet *set = E->set;
+    GsvGrid &g = E->g;
+    if (!A)
+        return 1;
+    if (E->idbytes_e != E->idbytes_h) {
+        fprintf(stderr, "Error: the gradient needs E and H class ids of the same width\n");
+        return 1;
+    }
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    GsvAdj &a = A->a;
+    for (float *p : { a.ebx, a.eby, a.ebz, a.hbx, a.hby, a.hbz, a.qx, a.qy, a.qz, a.rx, a.ry, a.rz })
+        CK(hipMemsetAsync(p - g.plane, 0, E->field_floats * 4, E->st));
+    for (int f = 0; f < 6; f++) {
+        size_t n = psi_pairs(g.f[f], f / 2, g);
+        if (n) {
+            CK(hipMemsetAsync(a.te[f], 0, n * 8, E->st));
+            CK(hipMemsetAsync(a.th[f], 0, n * 8, E->st));
+        }
+    }
+    CK(hipMemsetAsync(a.gca, 0, 3 * a.ncoef * 8, E->st));
+    CK(hipMemsetAsync(a.gcb, 0, 3 * a.ncoef * 8, E->st));
+    CK(hipEventRecord(e0, E->st));
+    int first = 1;
+    const int nseg = A->history_all ? 1 : (A->nsteps + A->K - 1) / A->K;
+    for (int s = nseg - 1; s >= 0; s--) {
+        int n0 = 0, n1 = A->nsteps;
+        if (!A->history_all) {
+            /* rebuild the segment's history from its checkpoint */
+            n0 = s * A->K;
+            n1 = n0 + A->K < A->nsteps ? n0 + A->K : A->nsteps;
+            if (ckpt_copy(E, A->ckpt[s], 0))
+                return 1;
+            A->recording = 1;
+            A->record_base = n0;
+            set->sc.step_act = n0;
+            for (int n = n0; n < n1; n++)
+                if (step_fn(ctx))
+                    return 1;
+            A->recording = 0;
+        }
+        for (int n = n1 - 1; n >= n0; n--) {
+            const int slot = A->history_all ? n : n - n0;
+            if (E->idbytes_e == 1)
+                launch_adj<uint8_t>(E, n, slot, first);
+            else
+                launch_adj<uint16_t>(E, n, slot, first);
+            CK(hipGetLastError());
+            first = 0;
+        }
+    }
+    CK(hipEventRecord(e1, E->st));
+    std::vector<double> gca(3 * a.ncoef), gcb(3 * a.ncoef);
+    CK(hipMemcpyAsync(gca.data(), a.gca, gca.size() * 8, hipMemcpyDeviceToHost, E->st));
+    CK(hipMemcpyAsync(gcb.data(), a.gcb, gcb.size() * 8, hipMemcpyDeviceToHost, E->st));
+    CK(hipStreamSynchronize(E->st));
+    float ms;
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    A->ms_backward = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    set->sc.step_act = A->nsteps;
+
+    /* chain rule through ca, cb of each coefficient cell to the averaged
+     * permittivity and on to the voxels (sv_pool_cabs_vjp, plan.c) */
+    const SvSetAdjoint &sa = set->sa;
+    const int dnx = sa.design_in - sa.design_i0, dny = sa.design_jn - sa.design_j0, dnz = sa.design_kn - sa.design_k0;
+    std::vector<double> gv((size_t)dnx * dny * dnz, 0.0);
+    for (int k = 0; k < g.pc_nz; k++)
+        for (int j = 0; j < g.pc_ny; j++)
+            for (int i = 0; i < g.pc_nx; i++) {
+                const size_t q = (size_t)i + (size_t)g.pc_nx * (j + (size_t)g.pc_ny * k);
+                double ga[3] = { gca[q], gca[a.ncoef + q], gca[2 * a.ncoef + q] };
+                double gb[3] = { gcb[q], gcb[a.ncoef + q], gcb[2 * a.ncoef + q] };
+                sv_pool_cabs_vjp(E->mp, i + g.pc_i0, j + g.pc_j0, k + g.pc_k0, ga, gb, sa.design_i0, sa.design_j0,
+                                 sa.design_k0, dnx, dny, dnz, gv.data());
+            }
+    for (size_t q = 0; q < gv.size(); q++)
+        grad[q] = (float)(gv[q] * SV_EPSILON_0);
+    return 0;
+}
+
+/* relative permittivity of the design voxels, [i][j][k] over the design box */
+extern "C" int
+gsv_engine_set_design_eps(void *eng, const float *eps_r)
+{
+    Engine *E = (Engine *)eng;
+    SvSet *set = E->set;
+    const SvSetAdjoint &sa = set->sa;
+    if (!sa.design || !E->ready)
+        return 1;
+    const int dnx = sa.design_in - sa.design_i0, dny = sa.design_jn - sa.design_j0, dnz = sa.design_kn - sa.design_k0;
+    SvFCube *ep = E->mp->d->epsilon;
+    for (int i = 0; i < dnx; i++)
+        for (int j = 0; j < dny; j++)
+            for (int k = 0; k < dnz; k++) {
+                const float v = eps_r[((size_t)i * dny + j) * dnz + k];
+                if (!(v > 0)) {
+                    fprintf(stderr, "Error: design permittivity must be positive\n");
+                    return 1;
+                }
+                /* the same float product as the MEDIUM_LINEAR reader */
+                ep->data[i + sa.design_i0][j + sa.design_j0][k + sa.design_k0] = v * (float)SV_EPSILON_0;
+            }
+    if (gsv_build_pctab(E))
+        return 1;
+    CK(hipMemcpyAsync((void *)E->g.pctab, E->pctab_host.data(), E->pctab_host.size() * sizeof(GsvECoef),
+                      hipMemcpyHostToDevice, E->st));
+    CK(hipStreamSynchronize(E->st));
+    return 0;
+}
+
+extern "C" int
+gsv_engine_get_design_eps(void *eng, float *eps_r)
+{
+    Engine *E = (Engine *)eng;
+    const SvSetAdjoint &sa = E->set->sa;
+    if (!sa.design)
+        return 1;
+    const int dnx = sa.design_in - sa.design_i0, dny = sa.design_jn - sa.design_j0, dnz = sa.design_kn - sa.design_k0;
+    SvFCube *ep = E->mp->d->epsilon;
+    for (int i = 0; i < dnx; i++)
+        for (int j = 0; j < dny; j++)
+            for (int k = 0; k < dnz; k++)
+                eps_r[((size_t)i * dny + j) * dnz + k] =
+                    (float)(ep->data[i + sa.design_i0][j + sa.design_j0][k + sa.design_k0] / SV_EPSILON_0);
+    return 0;
+}
+
+extern "C" void
+gsv_engine_adjoint_info(void *eng, double *out)
+{
+    Engine *E = (Engine *)eng;
+    GsvAdjoint *A = E->adj;
+    out[0] = A ? A->ms_forward : 0;
+    out[1] = A ? A->ms_backward : 0;
+    out[2] = A ? A->history_all : 0;
+    out[3] = A ? A->K : 0;
+    out[4] = A ? (double)A->ckpt.size() : 0;
+}

@@ -34,6 +34,7 @@
 #define GSV_E_PECX 0x100u    /* PEC zeroing of Ex after the update */
 #define GSV_E_PECY 0x200u
 #define GSV_E_PECZ 0x400u
+#define GSV_E_PERCELL 0x800u /* coefficients come from the per-cell table of the design region */
 
 #define GSV_MAX_ADE 16
 
@@ -93,6 +94,10 @@ typedef struct {
     const float *rkhx, *rkhy, *rkhz;      /* 1/kappa on the H grid, per axis */
     GsvCpmlFace f[6];                     /* x0, xn, y0, yn, z0, zn */
     float dx, dy, dz;
+    /* per-cell E coefficients of the design region: cells whose class carries
+     * GSV_E_PERCELL read pctab[(i-pc_i0) + pc_nx*((j-pc_j0) + pc_ny*(k-pc_k0))] */
+    int pc_i0, pc_j0, pc_k0, pc_nx, pc_ny, pc_nz;
+    const GsvECoef *pctab;
     /* ADE */
     int ade, ade_poles;
     const GsvAde *adem;     /* ADE material table in device memory */

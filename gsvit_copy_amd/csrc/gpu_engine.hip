@@ -26,18 +26,10 @@
 #include "gsvit_internal.h"
 #include "engine.h"
 #include "kernels.h"
+#include "engine_host.h"
 
 #define EPS0 SV_EPSILON_0
 #define MU0 SV_MU_0
-
-#define CK(x)                                                                                 \
-    do {                                                                                      \
-        hipError_t e_ = (x);                                                                  \
-        if (e_ != hipSuccess) {                                                               \
-            fprintf(stderr, "HIP error: %s (%s) at %s:%d\n", hipGetErrorString(e_), #x, __FILE__, __LINE__); \
-            return 1;                                                                         \
-        }                                                                                     \
-    } while (0)
 
 struct EKey {
     uint32_t w[7];
@@ -68,78 +60,6 @@ fbits(double v)
     uint32_t u;
     memcpy(&u, &f, 4);
     return u;
-}
-
-struct Engine {
-    SvPool *mp;
-    SvSet *set;
-    int dev;
-    hipStream_t st;
-    int ready;
-    GsvGrid g;
-    int idbytes_e, idbytes_h;
-    int bx_h, by_h, kchunk_h, bx_e, by_e, kchunk_e;
-    std::vector<GsvECoef> etab;
-    std::vector<GsvHCoef> htab;
-    std::vector<uint8_t> ecls_host, hcls_host;   /* device layout, idbytes per cell */
-    int ade_slot_of_mat[256];
-    GsvAde adem_host[GSV_MAX_ADE];
-    std::vector<void *> allocs;
-    float *fbase[6];
-    size_t field_floats;
-    /* sources */
-    int has_tsf;
-    GsvTsf tsf;
-    int has_src;
-    GsvPointSrc src;
-    /* traces */
-    int npts;
-    long long *d_ppos;
-    float *d_rec;
-    /* boundaries */
-    int has_liao;
-    GsvLiao liao;
-    int rezero[3];
-    int has_per;
-    GsvPeriodic per;
-    /* far field */
-    int has_ff;
-    GsvFarfieldTD ff;
-    int has_dft;
-    GsvDft dft;
-    int dft_first, dft_pending;   /* pending ring samples: steps dft_first .. + dft_pending - 1 */
-    /* box sums */
-    std::vector<GsvBoxSum> sums;
-    /* timing of the two volume kernels */
-    int timing;
-    hipEvent_t ev[4];
-    double t_h_ms, t_e_ms;
-    long n_h, n_e;
-};
-
-template <typename T>
-static int
-dmalloc(Engine *E, T **p, size_t count)
-{
-    void *q = NULL;
-    if (count == 0)
-        count = 1;
-    CK(hipMalloc(&q, count * sizeof(T)));
-    E->allocs.push_back(q);
-    *p = (T *)q;
-    return 0;
-}
-
-template <typename T>
-static int
-dupload(Engine *E, T **p, const T *h, size_t count)
-{
-    if (dmalloc(E, p, count))
-        return 1;
-    /* synchronous: the host staging vectors of the callers are temporaries */
-    CK(hipMemcpyAsync(*p, h, count * sizeof(T), hipMemcpyHostToDevice, E->st));
-    CK(hipStreamSynchronize(E->st));
-    return 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -175,8 +95,29 @@ pec_type(const SvPool *mp, int i, int j, int k)
     return mp->mats[mp->d->mat->data[i][j][k]].type == SV_MAT_PEC;
 }
 
+/* cell inside the design region's per-cell coefficient box */
+static int
+in_pc_box(const GsvGrid &g, int i, int j, int k)
+{
+    return g.pc_nx > 0 && i >= g.pc_i0 && i < g.pc_i0 + g.pc_nx && j >= g.pc_j0 && j < g.pc_j0 + g.pc_ny
+           && k >= g.pc_k0 && k < g.pc_k0 + g.pc_nz;
+}
+
+static void e_key_raw(const Engine *E, int i, int j, int k, EKey *key);
+
 static void
 e_key(const Engine *E, int i, int j, int k, EKey *key)
+{
+    if (in_pc_box(E->g, i, j, k)) {
+        memset(key->w, 0, sizeof(key->w));
+        key->w[6] = GSV_E_PERCELL;
+        return;
+    }
+    e_key_raw(E, i, j, k, key);
+}
+
+static void
+e_key_raw(const Engine *E, int i, int j, int k, EKey *key)
 {
     const SvPool *mp = E->mp;
     int xt, yt, zt, mi;
@@ -208,6 +149,37 @@ e_key(const Engine *E, int i, int j, int k, EKey *key)
     key->w[0] = fbits(ca[0]); key->w[1] = fbits(ca[1]); key->w[2] = fbits(ca[2]);
     key->w[3] = fbits(cb[0]); key->w[4] = fbits(cb[1]); key->w[5] = fbits(cb[2]);
     key->w[6] = fl;
+}
+
+/* per-cell coefficients of the design region's cells, from the current
+ * voxel permittivities; the region must be free of dispersive and PEC cells */
+int
+gsv_build_pctab(Engine *E)
+{
+    const GsvGrid &g = E->g;
+    std::vector<GsvECoef> &t = E->pctab_host;
+    int bad = 0;
+    t.resize((size_t)g.pc_nx * g.pc_ny * g.pc_nz);
+    #pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int k = 0; k < g.pc_nz; k++)
+        for (int j = 0; j < g.pc_ny; j++)
+            for (int i = 0; i < g.pc_nx; i++) {
+                EKey key;
+                e_key_raw(E, i + g.pc_i0, j + g.pc_j0, k + g.pc_k0, &key);
+                GsvECoef &c = t[(size_t)i + (size_t)g.pc_nx * (j + (size_t)g.pc_ny * k)];
+                memcpy(c.ca, key.w, 12);
+                memcpy(c.cb, key.w + 3, 12);
+                c.flags = key.w[6];
+                c.pad = 0;
+                if ((c.flags & (GSV_E_ADE | GSV_E_PECX | GSV_E_PECY | GSV_E_PECZ))
+                    || (c.flags & (GSV_E_TBCX | GSV_E_TBCY | GSV_E_TBCZ)) != (GSV_E_TBCX | GSV_E_TBCY | GSV_E_TBCZ))
+                    bad = 1;
+            }
+    if (bad) {
+        fprintf(stderr, "Error: the design region touches dispersive or PEC cells\n");
+        return 1;
+    }
+    return 0;
 }
 
 /* classes for every cell; ids in device layout (x fastest, row pitch px) */
@@ -418,6 +390,19 @@ sv_gpu_new(SvPool *mp, SvSet *set)
         }
         if (!(t->ia_theta >= 0 && t->ia_theta < M_PI)) {
             fprintf(stderr, "Error: TSF theta outside [0, pi) is not supported\n");
+            delete E;
+            return NULL;
+        }
+    }
+    if (set->sa.design) {
+        /* voxel (i,j,k) feeds the averaged coefficients of cells up to (i+1, j+1, k+1) */
+        E->g.pc_i0 = set->sa.design_i0;
+        E->g.pc_j0 = set->sa.design_j0;
+        E->g.pc_k0 = set->sa.design_k0;
+        E->g.pc_nx = set->sa.design_in + 1 - set->sa.design_i0;
+        E->g.pc_ny = set->sa.design_jn + 1 - set->sa.design_j0;
+        E->g.pc_nz = set->sa.design_kn + 1 - set->sa.design_k0;
+        if (gsv_build_pctab(E)) {
             delete E;
             return NULL;
         }
@@ -648,6 +633,7 @@ setup_sources_and_outputs(Engine *E)
         E->src.nsteps = nst;
         E->src.pos = dp;
         E->src.vals = dv;
+        E->src_vals_host.swap(vals);
     }
     /* point traces */
     E->npts = set->so.npnts;
@@ -928,6 +914,11 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
         g.htab = ht;
         g.netab = (int)E->etab.size();
         g.nhtab = (int)E->htab.size();
+        if (!E->pctab_host.empty()) {
+            GsvECoef *pt;
+            if (dupload(E, &pt, E->pctab_host.data(), E->pctab_host.size())) return 1;
+            g.pctab = pt;
+        }
     }
     if (g.ade) {
         GsvAde *am;
@@ -1153,7 +1144,8 @@ extern "C" int
 sv_pool_gpu_ystep_e(SvPool *mp, SvSet *set)
 {
     ENGINE_OR_FAIL(mp);
-    (void)set;
+    if (E->adj && gsv_adjoint_pre_e(E, set->sc.step_act))
+        return 1;
     if (E->timing)
         CK(hipEventRecord(E->ev[2], E->st));
     CK(gsv_launch_estep(&E->g, E->idbytes_e, E->bx_e, E->by_e, E->kchunk_e, E->st));
@@ -1174,6 +1166,8 @@ sv_pool_gpu_ystep_e(SvPool *mp, SvSet *set)
     }
     if (E->has_per)
         CK(gsv_launch_periodic(&E->g, &E->per, 0, E->st));
+    if (E->adj && gsv_adjoint_post_e(E, set->sc.step_act))
+        return 1;
     return 0;
 }
 
@@ -1212,6 +1206,8 @@ extern "C" int
 sv_pool_gpu_apply_farfield(SvPool *mp, SvSet *set)
 {
     ENGINE_OR_FAIL(mp);
+    if (E->adj && gsv_adjoint_after_step(E, set->sc.step_act))
+        return 1;
     if (E->has_ff)
         CK(gsv_launch_farfield_td(&E->g, &E->ff, set->sc.step_act, E->st));
     if (E->has_dft) {
@@ -1373,6 +1369,7 @@ gsv_engine_free(void *eng)
     hipSetDevice(E->dev);
     if (E->st)
         hipStreamSynchronize(E->st);
+    gsv_adjoint_free(E);
     for (void *p : E->allocs)
         hipFree(p);
     if (E->ready)

@@ -23,6 +23,8 @@ int sv_material_is_tabulated(int type);   /* LINTAB, DRUDE, PLRC, ADE, CP, CP3 *
 int sv_material_is_dispersive(int type);  /* DRUDE, CP3, CP, ADE, PLRC */
 /* per-voxel epsilon/sigma as seen by the E update, src3d/yee.c:1317-1350 */
 int sv_pool_epsilon_sigma(const SvPool *mp, int i, int j, int k, double *eps, double *sig);
+void sv_pool_cabs_vjp(const SvPool *mp, int i, int j, int k, const double gca[3], const double gcb[3], int b_i0, int b_j0,
+                      int b_k0, int bnx, int bny, int bnz, double *grad);
 /* per-voxel mu/sigast as seen by the H update, src3d/yee.c:1615-1645 */
 int sv_pool_mu_sigast(const SvPool *mp, int i, int j, int k, double *mu, double *sigast);
 /* component-wise averaged E coefficients, src3d/yee.c:1396-1492 */

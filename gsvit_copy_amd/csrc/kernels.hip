@@ -424,7 +424,9 @@ k_estep(GsvGrid g, int kchunk)
         const float hzj = ldf(g.hzg, o - dJ), hxj = ldf(g.hxg, o - dJ);
         const float hzi = ldf(g.hzg, o - 4), hyi = ldf(g.hyg, o - 4);
         const float rkz = g.rkez[k];
-        const GsvECoef ec = g.etab[ldid<ID>(g.ecls, oc)];
+        GsvECoef ec = g.etab[ldid<ID>(g.ecls, oc)];
+        if (ec.flags & GSV_E_PERCELL)
+            ec = g.pctab[(i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))];
         float ex = ldn(g.exg, o), ey = ldn(g.eyg, o), ez = ldn(g.ezg, o);
         const unsigned fl = ec.flags;
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;

@@ -136,6 +136,66 @@ sv_pool_get_cabs(const SvPool *mp, int i, int j, int k,
     }
 }
 
+/* Vector-Jacobian product of sv_pool_get_cabs with respect to the voxel
+ * permittivities (sigma held fixed): given dJ/dca and dJ/dcb of cell
+ * (i,j,k) per component, adds dJ/d(eps_v) to grad[v] for the voxels v of the
+ * box (b_i0, b_j0, b_k0) + (bnx, bny, bnz), [i][j][k] order.  Voxels outside
+ * the box are not parameters and are skipped. */
+void
+sv_pool_cabs_vjp(const SvPool *mp, int i, int j, int k, const double gca[3], const double gcb[3], int b_i0, int b_j0,
+                 int b_k0, int bnx, int bny, int bnz, double *grad)
+{
+    const SvYeeData *d = mp->d;
+    const double dt = d->dt;
+    const double dn[3] = { d->dx, d->dy, d->dz };
+    /* voxels averaged per component (Ex: self, j-1, k-1, j-1 k-1; ...), or self only */
+    int vox[3][4][3], nv[3];
+    double eavg[3], savg[3];
+    int c, q;
+    if (d->bnds->data[i][j][k] > 0) {
+        static const int offs[3][4][3] = {
+            { { 0, 0, 0 }, { 0, -1, 0 }, { 0, 0, -1 }, { 0, -1, -1 } },
+            { { 0, 0, 0 }, { -1, 0, 0 }, { 0, 0, -1 }, { -1, 0, -1 } },
+            { { 0, 0, 0 }, { -1, 0, 0 }, { 0, -1, 0 }, { -1, -1, 0 } },
+        };
+        for (c = 0; c < 3; c++) {
+            nv[c] = 0;
+            for (q = 0; q < 4; q++) {
+                int vi = i + offs[c][q][0], vj = j + offs[c][q][1], vk = k + offs[c][q][2];
+                if (vi < 0 || vj < 0 || vk < 0)
+                    continue;
+                vox[c][nv[c]][0] = vi; vox[c][nv[c]][1] = vj; vox[c][nv[c]][2] = vk;
+                nv[c]++;
+            }
+        }
+    } else
+        for (c = 0; c < 3; c++) {
+            nv[c] = 1;
+            vox[c][0][0] = i; vox[c][0][1] = j; vox[c][0][2] = k;
+        }
+    for (c = 0; c < 3; c++) {
+        double e, s, es = 0, ss = 0, half, dca, dcb, g;
+        for (q = 0; q < nv[c]; q++) {
+            sv_pool_epsilon_sigma(mp, vox[c][q][0], vox[c][q][1], vox[c][q][2], &e, &s);
+            es += e;
+            ss += s;
+        }
+        eavg[c] = es / nv[c];
+        savg[c] = ss / nv[c];
+        /* ca = (eps - h)/(eps + h), cb = dt / (d (eps + h)), h = sigma dt / 2 */
+        half = savg[c] * dt / 2;
+        dca = 2 * half / ((eavg[c] + half) * (eavg[c] + half));
+        dcb = -dt / (dn[c] * (eavg[c] + half) * (eavg[c] + half));
+        g = (gca[c] * dca + gcb[c] * dcb) / nv[c];
+        for (q = 0; q < nv[c]; q++) {
+            int vi = vox[c][q][0] - b_i0, vj = vox[c][q][1] - b_j0, vk = vox[c][q][2] - b_k0;
+            if (vi < 0 || vj < 0 || vk < 0 || vi >= bnx || vj >= bny || vk >= bnz)
+                continue;
+            grad[((size_t)vi * bny + vj) * bnz + vk] += g;
+        }
+    }
+}
+
 static int
 disp_at(const SvPool *mp, int i, int j, int k)
 {
@@ -1172,6 +1232,39 @@ init_and_make_plan(SvSet *set)
         if (f->box_i0 < 1 || f->box_j0 < 1 || f->box_k0 < 1 || f->box_in >= xres - 1 || f->box_jn >= yres - 1
             || f->box_kn >= zres - 1 || f->box_in <= f->box_i0 || f->box_jn <= f->box_j0 || f->box_kn <= f->box_k0) {
             fprintf(stderr, "Error: NFFF box must lie inside the pool with one cell to spare on each side\n");
+            return NULL;
+        }
+    }
+
+    /* gradient extension: the design voxels carry per-voxel permittivity
+     * (MEDIUM_LINEAR data) and the coefficients they feed must stay off the
+     * zeroed boundary planes */
+    if (set->sa.design) {
+        SvSetAdjoint *a = &set->sa;
+        int i, j, k;
+        if (a->design_i0 < 1 || a->design_j0 < 1 || a->design_k0 < 1 || a->design_in > xres - 2
+            || a->design_jn > yres - 2 || a->design_kn > zres - 2 || a->design_in <= a->design_i0
+            || a->design_jn <= a->design_j0 || a->design_kn <= a->design_k0) {
+            fprintf(stderr, "Error: DESIGN_BOX must lie inside the pool with two cells to spare on each side\n");
+            return NULL;
+        }
+        if (!mp->d->epsilon) {
+            fprintf(stderr, "Error: DESIGN_BOX needs per-voxel permittivity (MEDIUM_LINEAR)\n");
+            return NULL;
+        }
+        for (i = a->design_i0; i < a->design_in; i++)
+            for (j = a->design_j0; j < a->design_jn; j++)
+                for (k = a->design_k0; k < a->design_kn; k++)
+                    if (mp->nmat > 0 && mp->d->mat->data[i][j][k] > 0) {
+                        fprintf(stderr, "Error: DESIGN_BOX voxel (%d %d %d) holds a tabulated material\n", i, j, k);
+                        return NULL;
+                    }
+    }
+    if (set->sa.objective) {
+        SvSetAdjoint *a = &set->sa;
+        if (a->obj_i0 < 0 || a->obj_j0 < 0 || a->obj_k0 < 0 || a->obj_in > xres || a->obj_jn > yres || a->obj_kn > zres
+            || a->obj_in <= a->obj_i0 || a->obj_jn <= a->obj_j0 || a->obj_kn <= a->obj_k0) {
+            fprintf(stderr, "Error: OBJECTIVE_DFT box must lie inside the pool\n");
             return NULL;
         }
     }

@@ -21,6 +21,12 @@ int gsv_engine_fetch_aux(void *eng);
 void gsv_engine_free(void *eng);
 int gsv_engine_set_launch(void *eng, const int *shape);
 void gsv_engine_get_launch(void *eng, int *shape);
+int gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_interval);
+int gsv_engine_adjoint_forward(void *eng, int (*step_fn)(void *), void *ctx, double *J);
+int gsv_engine_adjoint_backward(void *eng, int (*step_fn)(void *), void *ctx, float *grad);
+int gsv_engine_set_design_eps(void *eng, const float *eps_r);
+int gsv_engine_get_design_eps(void *eng, float *eps_r);
+void gsv_engine_adjoint_info(void *eng, double *out);
 
 typedef struct {
     SvSet set;
@@ -262,6 +268,64 @@ done:
         printf("autotune: H %dx%d z%d (%.3f ms), E %dx%d z%d (%.3f ms), %d steps\n", best[0], best[1], best[2], bh,
                best[3], best[4], best[5], be, used);
     return used;
+}
+
+/* ---- gradient extension ---- */
+
+static int
+one_step(void *h)
+{
+    GsvSession *s = (GsvSession *)h;
+    int save = s->write_outputs, r;
+    s->write_outputs = 0;
+    r = gsv_run(h, 1) < 0;
+    s->write_outputs = save;
+    return r;
+}
+
+void
+gsv_design_dims(void *h, int *dims)
+{
+    const SvSetAdjoint *a = &((GsvSession *)h)->set.sa;
+    dims[0] = a->design ? a->design_in - a->design_i0 : 0;
+    dims[1] = a->design ? a->design_jn - a->design_j0 : 0;
+    dims[2] = a->design ? a->design_kn - a->design_k0 : 0;
+}
+
+int
+gsv_adjoint_setup(void *h, long long history_budget_bytes, int ckpt_interval)
+{
+    return gsv_engine_adjoint_setup(((GsvSession *)h)->mp->gpu, history_budget_bytes, ckpt_interval);
+}
+
+int
+gsv_set_design_eps(void *h, const float *eps_r)
+{
+    return gsv_engine_set_design_eps(((GsvSession *)h)->mp->gpu, eps_r);
+}
+
+int
+gsv_get_design_eps(void *h, float *eps_r)
+{
+    return gsv_engine_get_design_eps(((GsvSession *)h)->mp->gpu, eps_r);
+}
+
+int
+gsv_objective(void *h, double *J)
+{
+    return gsv_engine_adjoint_forward(((GsvSession *)h)->mp->gpu, one_step, h, J);
+}
+
+int
+gsv_objective_gradient(void *h, float *grad)
+{
+    return gsv_engine_adjoint_backward(((GsvSession *)h)->mp->gpu, one_step, h, grad);
+}
+
+void
+gsv_adjoint_info(void *h, double *out)
+{
+    gsv_engine_adjoint_info(((GsvSession *)h)->mp->gpu, out);
 }
 
 void

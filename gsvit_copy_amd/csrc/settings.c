@@ -532,6 +532,19 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             f->ndft = n;
             if (get_rest_of_line(fr, f->dft_filename, SV_NAME_LEN))
                 goto fail;
+        } else if (!strcmp(key, "DESIGN_BOX")) {
+            SvSetAdjoint *a = &set->sa;
+            if (get_int(fr, &a->design_i0, key) || get_int(fr, &a->design_j0, key) || get_int(fr, &a->design_k0, key)
+                || get_int(fr, &a->design_in, key) || get_int(fr, &a->design_jn, key) || get_int(fr, &a->design_kn, key))
+                goto fail;
+            a->design = 1;
+        } else if (!strcmp(key, "OBJECTIVE_DFT")) {
+            SvSetAdjoint *a = &set->sa;
+            if (get_int(fr, &a->obj_i0, key) || get_int(fr, &a->obj_j0, key) || get_int(fr, &a->obj_k0, key)
+                || get_int(fr, &a->obj_in, key) || get_int(fr, &a->obj_jn, key) || get_int(fr, &a->obj_kn, key)
+                || get_double(fr, &a->obj_lambda, key) || a->obj_lambda <= 0)
+                goto fail;
+            a->objective = 1;
         } else {
             fprintf(stderr, "Error: keyword %s is not supported by the MI355X engine\n", key);
             goto fail;

@@ -50,6 +50,18 @@ def library():
         L.gsv_get_launch.argtypes = [vp, ip]
         L.gsv_autotune.restype = ctypes.c_int
         L.gsv_autotune.argtypes = [vp, ctypes.c_int, ip]
+        L.gsv_design_dims.argtypes = [vp, ip]
+        L.gsv_adjoint_setup.restype = ctypes.c_int
+        L.gsv_adjoint_setup.argtypes = [vp, ctypes.c_longlong, ctypes.c_int]
+        L.gsv_set_design_eps.restype = ctypes.c_int
+        L.gsv_set_design_eps.argtypes = [vp, fp]
+        L.gsv_get_design_eps.restype = ctypes.c_int
+        L.gsv_get_design_eps.argtypes = [vp, fp]
+        L.gsv_objective.restype = ctypes.c_int
+        L.gsv_objective.argtypes = [vp, dp]
+        L.gsv_objective_gradient.restype = ctypes.c_int
+        L.gsv_objective_gradient.argtypes = [vp, fp]
+        L.gsv_adjoint_info.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -174,6 +186,54 @@ class Session:
         if used < 0:
             raise RuntimeError("autotune failed")
         return list(v), used
+
+    # ---- gradient extension (DESIGN_BOX / OBJECTIVE_DFT parameter files) ----
+
+    def design_dims(self):
+        v = (ctypes.c_int * 3)()
+        self.L.gsv_design_dims(self.h, v)
+        return tuple(v)
+
+    def adjoint_setup(self, history_budget_bytes=16 << 30, ckpt_interval=100):
+        if self.L.gsv_adjoint_setup(self.h, int(history_budget_bytes), int(ckpt_interval)):
+            raise RuntimeError("gradient setup failed (see stderr)")
+
+    def set_design_eps(self, eps_r):
+        a = np.ascontiguousarray(eps_r, dtype=np.float32)
+        if a.shape != self.design_dims():
+            raise ValueError("design array shape %r, expected %r" % (a.shape, self.design_dims()))
+        if self.L.gsv_set_design_eps(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))):
+            raise RuntimeError("setting the design permittivity failed")
+
+    def design_eps(self):
+        out = np.empty(self.design_dims(), dtype=np.float32)
+        if self.L.gsv_get_design_eps(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))):
+            raise RuntimeError("no design region")
+        return out
+
+    def objective(self):
+        """Fresh forward run over all steps; returns J."""
+        J = ctypes.c_double()
+        with _chdir(self.dir):
+            r = self.L.gsv_objective(self.h, ctypes.byref(J))
+        if r:
+            raise RuntimeError("objective run failed (see stderr)")
+        return J.value
+
+    def objective_gradient(self):
+        """dJ/d eps_r per design voxel, after objective()."""
+        out = np.empty(self.design_dims(), dtype=np.float32)
+        with _chdir(self.dir):
+            r = self.L.gsv_objective_gradient(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        if r:
+            raise RuntimeError("adjoint run failed (see stderr)")
+        return out
+
+    def adjoint_info(self):
+        v = (ctypes.c_double * 5)()
+        self.L.gsv_adjoint_info(self.h, v)
+        return {"forward_ms": v[0], "backward_ms": v[1], "history_all": bool(v[2]), "ckpt_interval": int(v[3]),
+                "checkpoints": int(v[4])}
 
     def close(self):
         if getattr(self, "h", None):

@@ -51,6 +51,24 @@ void gsv_get_launch(void *session, int *shape);
  * shapes on real steps (steps_per_candidate each); writes the chosen shape to
  * best[0..5] and returns the number of steps consumed, -1 on error */
 int gsv_autotune(void *session, int steps_per_candidate, int *best);
+/* Gradient extension (no reference counterpart).  The parameter file
+ * declares DESIGN_BOX i0 j0 k0 in jn kn (voxels with per-voxel permittivity
+ * from MEDIUM_LINEAR) and OBJECTIVE_DFT i0 j0 k0 in jn kn lambda, the
+ * objective J = sum over the box of |DFT_lambda(E)|^2 over the whole run.
+ * Design arrays are [i][j][k] (k fastest) over the design box. */
+void gsv_design_dims(void *session, int *dims);
+/* allocate the gradient state: the whole run's design-region history is kept
+ * when it fits history_budget_bytes, otherwise full checkpoints every
+ * ckpt_interval steps with recomputation */
+int gsv_adjoint_setup(void *session, long long history_budget_bytes, int ckpt_interval);
+int gsv_set_design_eps(void *session, const float *eps_r);
+int gsv_get_design_eps(void *session, float *eps_r);
+/* fresh forward run over all steps; *J = objective */
+int gsv_objective(void *session, double *J);
+/* after gsv_objective: grad = dJ / d eps_r per design voxel */
+int gsv_objective_gradient(void *session, float *grad);
+/* out[0..4]: forward ms, backward ms, whole history kept, checkpoint interval, checkpoints */
+void gsv_adjoint_info(void *session, double *out);
 void gsv_close(void *session);
 /* streaming copy of nbytes (read nbytes + write nbytes) repeated reps times on
  * the current device; *ms_per_copy = mean time of one copy.  Calibrates HBM

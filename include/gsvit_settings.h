@@ -235,6 +235,19 @@ typedef struct {
     int gpumode;
 } SvSetPlan;
 
+/* Gradient extension (no reference counterpart): DESIGN_BOX i0 j0 k0 in jn kn
+ * marks voxels whose relative permittivity is a design parameter (given per
+ * voxel by MEDIUM_LINEAR); OBJECTIVE_DFT i0 j0 k0 in jn kn lambda defines
+ * J = sum over the box cells and the three E components of |DFT_lambda(E)|^2,
+ * the DFT running over every step with E sampled at (step+1) dt. */
+typedef struct {
+    int design;
+    int design_i0, design_j0, design_k0, design_in, design_jn, design_kn;
+    int objective;
+    int obj_i0, obj_j0, obj_k0, obj_in, obj_jn, obj_kn;
+    double obj_lambda;
+} SvSetAdjoint;
+
 typedef struct {
     SvSetPool sp;
     SvSetComp sc;
@@ -245,6 +258,7 @@ typedef struct {
     SvSetFarfield sf;
     SvSetMat sm;
     SvSetPlan plan;
+    SvSetAdjoint sa;
 } SvSet;
 
 /* src3d/settings.h:967,974 (clear_settings src3d/settings.c:570, parse_settings :3822).

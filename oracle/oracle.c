@@ -26,6 +26,8 @@
  *                             DFT of recorded point traces in the tests)
  *   point traces ............ src3d/output.c:340-352
  *   box sums ................ src3d/output.c:259-306
+ *   gradient-extension objective (OBJECTIVE_DFT, no reference counterpart):
+ *                             running DFT of E over a box, J = sum |F|^2
  *   step order .............. src3d/svit.c:217-237
  * The parameter reader and planner are the product's (settings.c, plan.c);
  * they are pinned independently by the golden traces and the brute-force
@@ -82,6 +84,7 @@ typedef struct {
     real jb[8];
     double *points;                   /* [npts][nsteps+1][6] */
     int no_average;                   /* test option: E coefficients without interface averaging */
+    double *objacc;                   /* OBJECTIVE_DFT box: [cell][3][re, im] */
 } Oracle;
 
 #define IDX(o, i, j, k) (((size_t)(i) * (o)->ny + (j)) * (o)->nz + (k))
@@ -172,6 +175,10 @@ oracle_open(const char *parfile)
     }
     if (set->so.npnts)
         o->points = (double *)zalloc((size_t)set->so.npnts * (set->sc.nsteps + 1) * 6, sizeof(double));
+    if (set->sa.objective)
+        o->objacc = (double *)zalloc((size_t)(set->sa.obj_in - set->sa.obj_i0) * (set->sa.obj_jn - set->sa.obj_j0)
+                                         * (set->sa.obj_kn - set->sa.obj_k0) * 6,
+                                     sizeof(double));
     return o;
 }
 
@@ -195,7 +202,7 @@ oracle_close(void *handle)
             free(o->px[q]); free(o->py[q]); free(o->pz[q]); free(o->pxp[q]); free(o->pyp[q]); free(o->pzp[q]);
         }
     }
-    free(o->je); free(o->jh); free(o->points);
+    free(o->je); free(o->jh); free(o->points); free(o->objacc);
     sv_pool_free(o->mp, &o->set);
     clear_settings(&o->set, 1);
     free(o);
@@ -1258,6 +1265,30 @@ running_dft(Oracle *o)
     }
 }
 
+/* gradient-extension objective: running DFT of E over the OBJECTIVE_DFT box,
+ * E after step n sampled at t = (n+1) dt (box i fastest, then j, then k) */
+static void
+objective_dft(Oracle *o)
+{
+    const SvSetAdjoint *a = &o->set.sa;
+    const double w = 2 * M_PI * SV_LIGHT_SPEED / a->obj_lambda;
+    const double t = (o->set.sc.step_act + 1) * o->set.plan.dt;
+    const double co = cos(w * t), si = sin(w * t);
+    int i, j, k;
+    size_t q = 0;
+    if (!a->objective)
+        return;
+    for (k = a->obj_k0; k < a->obj_kn; k++)
+        for (j = a->obj_j0; j < a->obj_jn; j++)
+            for (i = a->obj_i0; i < a->obj_in; i++, q++) {
+                size_t c = IDX(o, i, j, k);
+                double *acc = o->objacc + q * 6;
+                acc[0] += o->ex[c] * co; acc[1] -= o->ex[c] * si;
+                acc[2] += o->ey[c] * co; acc[3] -= o->ey[c] * si;
+                acc[4] += o->ez[c] * co; acc[5] -= o->ez[c] * si;
+            }
+}
+
 static void
 getpoints(Oracle *o)
 {
@@ -1293,6 +1324,7 @@ oracle_step(void *handle, int nsteps)
         getsum(o);
         farfield(o);
         running_dft(o);
+        objective_dft(o);
         getpoints(o);
         o->set.sc.step_act++;
     }
@@ -1346,6 +1378,22 @@ oracle_get_farfield(void *handle, double *out)
 }
 
 /* tabulated material map ([i][j][k], -1 everywhere without a table) and boundary map */
+/* OBJECTIVE_DFT value J = sum |DFT(E)|^2 over the box so far */
+double
+oracle_objective(void *handle)
+{
+    Oracle *o = (Oracle *)handle;
+    const SvSetAdjoint *a = &o->set.sa;
+    size_t q, n;
+    double J = 0;
+    if (!o->objacc)
+        return 0;
+    n = (size_t)(a->obj_in - a->obj_i0) * (a->obj_jn - a->obj_j0) * (a->obj_kn - a->obj_k0) * 6;
+    for (q = 0; q < n; q++)
+        J += o->objacc[q] * o->objacc[q];
+    return J;
+}
+
 /* box sums as [nsums][nsteps] */
 void
 oracle_get_sums(void *handle, double *out)

@@ -5,6 +5,8 @@ folder and returns the parameter-file path.  The workloads:
   sphere_scatter  plain 3-D FDTD, 40 dielectric spheres, CPML, Ez dipole (128^3)
   rough_surface   Drude rough-surface scattering, TSF plane wave at 45 degrees,
                   running-DFT near-to-far-field box (512^3)
+  inverse_design  continuous-permittivity design region, CW Ez dipole below
+                  it, DFT objective on a plane above it (gradient extension)
 and small feature cases (CPML + ADE, TSF oblique, Liao + PEC, periodic, far field).
 """
 import math
@@ -235,3 +237,67 @@ def small_case(folder, kind, n=48, nsteps=160, seed=7):
 
 SMALL_KINDS = ["vacuum_none", "spheres_cpml", "magnetic_liao", "ade_cpml", "tsf_pec_liao", "tsf_ade_cpml",
                "periodic_liao", "farfield_cpml"]
+
+
+def write_linear_medium(path, eps_r, sigma=None, mu_r=None, sigast=None):
+    """MEDIUM_LINEAR file: 3 int32 dims, then float32 eps_r, sigma, mu_r, sigma* ([i][j][k], k fastest)."""
+    eps_r = np.asarray(eps_r, dtype=np.float32)
+    shape = eps_r.shape
+    parts = [eps_r,
+             np.zeros(shape, np.float32) if sigma is None else np.asarray(sigma, np.float32),
+             np.ones(shape, np.float32) if mu_r is None else np.asarray(mu_r, np.float32),
+             np.zeros(shape, np.float32) if sigast is None else np.asarray(sigast, np.float32)]
+    with open(path, "wb") as f:
+        f.write(np.array(shape, dtype=np.int32).tobytes())
+        for a in parts:
+            f.write(np.ascontiguousarray(a).tobytes())
+
+
+def cw_ramp(nsteps, dt, lam, ramp_periods=3.0, amplitude=1.0):
+    """CW sine with a smooth (raised-cosine) turn-on over ramp_periods periods."""
+    period = lam / C0
+    t = np.arange(nsteps) * dt
+    ramp = np.clip(t / (ramp_periods * period), 0.0, 1.0)
+    env = 0.5 - 0.5 * np.cos(math.pi * ramp)
+    return amplitude * env * np.sin(2 * math.pi * t / period)
+
+
+def inverse_design(folder, n=256, nsteps=3000, design=(64, 64, 32), dx=10e-9, cpml=12, seed=5, lam=1.0e-6,
+                   eps_range=(1.0, 12.0), gap=8, magnetic_voxel=False, dtmult=1.0):
+    """Differentiable design problem: a design box of per-voxel eps_r ~ U[eps_range]
+    (MEDIUM_LINEAR) centred in x, y; an Ez dipole `gap` cells below it driven by
+    a CW ramp at lam; the objective is the DFT |E|^2 at lam summed over a plane
+    of the design's x-y extent `gap` cells above it.  CPML on all faces.
+    magnetic_voxel puts mu_r = 2 in one voxel far from the design, which makes
+    the planner average the E coefficients at material interfaces."""
+    os.makedirs(folder, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    dnx, dny, dnz = design
+    c = n // 2
+    i0, j0 = c - dnx // 2, c - dny // 2
+    k0 = c - dnz // 2
+    box = (i0, j0, k0, i0 + dnx, j0 + dny, k0 + dnz)
+    eps = np.ones((n, n, n), dtype=np.float32)
+    eps[box[0]:box[3], box[1]:box[4], box[2]:box[5]] = rng.uniform(eps_range[0], eps_range[1], size=design)
+    mu = None
+    if magnetic_voxel:
+        mu = np.ones((n, n, n), dtype=np.float32)
+        mu[cpml + 2, cpml + 2, cpml + 2] = 2.0
+    write_linear_medium(os.path.join(folder, "design.lin"), eps, mu_r=mu)
+    dt = _dt(dx, dx, dx, dtmult)
+    write_point_source(os.path.join(folder, "dipole.txt"), cw_ramp(nsteps, dt, lam), comp=2)
+    sk = box[2] - gap
+    mk = box[5] + gap
+    if sk < cpml + 1 or mk > n - cpml - 2:
+        raise ValueError("design, source and monitor do not fit between the CPML slabs")
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (n, n, n, dx, dx, dx), "COMP\n%d\n\n" % nsteps,
+             "DT_MULT\n%g\n\n" % dtmult,
+             "MEDIUM_LINEAR\ndesign.lin\n\n",
+             "BOUNDARY_ALL\ncpml %d 3 %.9g 0 1\n\n" % (cpml, 0.8 * 4 / (ETA0 * dx)),
+             "SOURCE_POINT\n%d %d %d 0 dipole.txt\n\n" % (c, c, sk),
+             "DESIGN_BOX\n%d %d %d %d %d %d\n\n" % box,
+             "OBJECTIVE_DFT\n%d %d %d %d %d %d %.9g\n\n" % (box[0], box[1], mk, box[3], box[4], mk + 1, lam)]
+    path = os.path.join(folder, "inverse_design.par")
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path, box, eps[box[0]:box[3], box[1]:box[4], box[2]:box[5]].copy()

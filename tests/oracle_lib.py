@@ -35,6 +35,8 @@ def lib(precision="f64"):
         L.oracle_get_dft.restype = ctypes.c_long
         L.oracle_get_dft.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_sums.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_objective.restype = ctypes.c_double
+        L.oracle_objective.argtypes = [ctypes.c_void_p]
         L.oracle_real_size.restype = ctypes.c_int
         L.oracle_set_option.restype = ctypes.c_int
         L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
@@ -113,6 +115,9 @@ class Oracle:
         if nsums:
             self.L.oracle_get_sums(self.h, _dptr(out))
         return out[:nsums]
+
+    def objective(self):
+        return self.L.oracle_objective(self.h)
 
     def dft(self):
         n = self.L.oracle_get_dft(self.h, None)
