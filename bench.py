@@ -48,6 +48,51 @@ def algorithmic_bytes(info, n, depth, n_ade_cells):
     return h_bytes, e_bytes
 
 
+def adjoint_bench(args, work, shape):
+    """Adjoint-gradient cost on the inverse-design workload: 256^3 cells of
+    10 nm, CPML 12, a 64x64x32 design region of eps_r ~ U[1,12], CW Ez dipole
+    at 1 um, 3000 steps, DFT |E|^2 objective on a 64x64 plane.  Ratio = (forward
+    run with history + backward run) / one plain forward run of the same
+    problem, both wall-clock with all data resident on the GPU."""
+    import cases
+    from gsvit_copy_amd import Session
+    n, nsteps = args.adjoint_n, args.adjoint_steps
+    design = (n // 4, n // 4, n // 8)
+    t0 = time.time()
+    par, box, eps = cases.inverse_design(os.path.join(work, "inverse"), n=n, nsteps=nsteps, design=design)
+    log("bench: inverse-design workload %d^3 written in %.1f s" % (n, time.time() - t0))
+    s = Session(par)
+    s.set_launch(shape)
+    s.run(5)
+    s.sync()
+    s.close()
+    s = Session(par)
+    s.set_launch(shape)
+    t0 = time.perf_counter()
+    s.run(nsteps)
+    s.sync()
+    plain = time.perf_counter() - t0
+    s.close()
+    s = Session(par)
+    s.set_launch(shape)
+    s.adjoint_setup()
+    t0 = time.perf_counter()
+    J = s.objective()
+    grad = s.objective_gradient()
+    total = time.perf_counter() - t0
+    info = s.adjoint_info()
+    s.close()
+    log("bench: plain forward %.3f s, forward+gradient %.3f s (ratio %.2f), J %.6g, |grad| %.4g"
+        % (plain, total, total / plain, J, float(np.linalg.norm(grad))))
+    return {"workload": "inverse design: %d^3, 10 nm cells, CPML 12, %dx%dx%d eps_r design region, CW Ez dipole "
+                        "1 um, DFT |E|^2 objective on a %dx%d plane" % (n, design[0], design[1], design[2],
+                                                                        design[0], design[1]),
+            "steps": nsteps, "plain_forward_s": round(plain, 4), "gradient_s": round(total, 4),
+            "ratio": round(total / plain, 3), "forward_with_history_ms": round(info["forward_ms"], 2),
+            "backward_ms": round(info["backward_ms"], 2), "history_all": info["history_all"],
+            "metric": "adjoint-grad time / plain forward", "higher_is_better": False}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +105,9 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=50)
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--tune-steps", type=int, default=2, help="steps per launch-shape candidate (0 = no tuning)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient measurement")
+    ap.add_argument("--adjoint-n", type=int, default=256)
+    ap.add_argument("--adjoint-steps", type=int, default=3000)
     ap.add_argument("--launch", default=None,
                     help="fixed launch shape hx,hy,hz,ex,ey,ez (tile x width, rows, z-march depth of the H and E "
                          "kernels); implies no tuning")
@@ -148,6 +196,10 @@ def main():
     except (OSError, ValueError, KeyError):
         pass
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_bench(args, work, shape)
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -197,6 +249,7 @@ def main():
                      "step_algorithmic_bytes": step_bytes,
                      "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

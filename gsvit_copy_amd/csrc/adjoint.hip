@@ -38,6 +38,7 @@
 #include <vector>
 #include <omp.h>
 #include "engine_host.h"
+#include "device_access.h"
 
 /* ------------------------------------------------------------------ */
 /* device parameter block                                               */
@@ -48,7 +49,7 @@ typedef struct {
     int mi0, mj0, mk0, mnx, mny, mnz;
     double omega, dt;
     double *macc;             /* [cell][3][2] running DFT (re, im) of E */
-    /* adjoint state, same layout as the fields (k = 0 pointers) */
+    /* adjoint state, same layout as the fields; allocation starts (guard plane k = -1) */
     float *ebx, *eby, *ebz, *hbx, *hby, *hbz;
     float *qx, *qy, *qz, *rx, *ry, *rz;
     float2 *te[6], *th[6];    /* psi adjoints per face (E side, H side) */
@@ -129,11 +130,15 @@ k_rec_post(GsvGrid g, GsvAdj a, int slot)
 /* backward kernels                                                     */
 /* ------------------------------------------------------------------ */
 
-template <typename ID>
+/* Both kernels march a chunk of z planes per (i, j) column like the forward
+ * volume kernels, address every per-cell array by one 32-bit byte offset from
+ * its guard plane and carry the neighbour plane they share in registers. */
+
+template <typename ID, typename OFF>
 __device__ __forceinline__ GsvECoef
-e_coef(const GsvGrid &g, int i, int j, int k, long long c)
+e_coef(const GsvGrid &g, OFF oc, int i, int j, int k)
 {
-    GsvECoef ec = g.etab[((const ID *)g.ecls)[c]];
+    GsvECoef ec = g.etab[ldid<ID>(g.ecls, oc)];
     if (ec.flags & GSV_E_PERCELL)
         ec = g.pctab[(i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))];
     return ec;
@@ -142,457 +147,472 @@ e_coef(const GsvGrid &g, int i, int j, int k, long long c)
 /* E cells: H-step transpose of step n+1 (gathers of r and the psi_H
  * adjoints), objective source and point-source projection of step n,
  * coefficient gradient, local part of the E-step transpose of step n */
-template <typename ID>
-__global__ __launch_bounds__(256) void
-k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int first)
+template <typename ID, typename OFF>
+__global__ __launch_bounds__(512) void
+k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int kchunk)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (i >= nx || j >= ny)
         return;
+    const int k0 = blockIdx.z * kchunk;
+    const int k1 = min(k0 + kchunk, nz);
     const int px = g.px;
-    const long long P = g.plane;
-    const long long c = i + (long long)px * j + P * k;
-    float ex = a.ebx[c], ey = a.eby[c], ez = a.ebz[c];
+    const OFF dP = (OFF)g.plane * 4, dJ = (OFF)px * 4, dPc = (OFF)g.plane * sizeof(ID);
+    OFF o = ((OFF)i + (OFF)px * j + (OFF)g.plane * (k0 + 1)) * 4;
+    OFF oc = ((OFF)i + (OFF)px * j + (OFF)g.plane * k0) * sizeof(ID);
 
-    if (!first) {
+    const float rhx = g.rkhx[i], rhy = g.rkhy[j];
+    const float rhx1 = i > 0 ? g.rkhx[i - 1] : 0.f, rhy1 = j > 0 ? g.rkhy[j - 1] : 0.f;
+    const bool iin = i >= 1 && i < nx - 1, jin = j >= 1 && j < ny - 1;
+    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
+    /* H-side psi cells whose E neighbours include this column (x and y faces) */
+    const bool hx0a = F0.on && jin && i < F0.depth - 1, hx0b = F0.on && jin && i >= 1 && i - 1 < F0.depth - 1;
+    const int lo1 = nx - F1.depth + 1;
+    const bool hxna = F1.on && jin && i >= lo1 && i < nx - 1, hxnb = F1.on && jin && i - 1 >= lo1 && i - 1 < nx - 1;
+    const bool hy0a = F2.on && iin && j < F2.depth - 1, hy0b = F2.on && iin && j >= 1 && j - 1 < F2.depth - 1;
+    const int lo3 = ny - F3.depth + 1;
+    const bool hyna = F3.on && iin && j >= lo3 && j < ny - 1, hynb = F3.on && iin && j - 1 >= lo3 && j - 1 < ny - 1;
+    const bool ijin = iin && jin;
+    /* E-side psi cells of this column */
+    const bool ex0 = F0.on && i >= 1 && i < F0.depth, exn = F1.on && i >= nx - F1.depth + 1;
+    const bool ey0 = F2.on && j >= 1 && j < F2.depth, eyn = F3.on && j >= ny - F3.depth + 1;
+    const bool zplane_xy = (F0.on && i == 0) || (F2.on && j == 0);
+    const bool in_mon_ij = i >= a.mi0 && i < a.mi0 + a.mnx && j >= a.mj0 && j < a.mj0 + a.mny;
+    const bool in_pc_ij = a.ncoef && i >= g.pc_i0 && i < g.pc_i0 + g.pc_nx && j >= g.pc_j0 && j < g.pc_j0 + g.pc_ny;
+
+    /* r of plane k-1 (the guard plane below k = 0 is zero) */
+    float rxm = ldf(a.rx, o - dP), rym = ldf(a.ry, o - dP);
+    for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
+        float ex = ldn(a.ebx, o), ey = ldn(a.eby, o), ez = ldn(a.ebz, o);
+        const float rxc = ldf(a.rx, o), ryc = ldf(a.ry, o), rzc = ldf(a.rz, o);
+        const float rhz = g.rkhz[k], rhz1 = k > 0 ? g.rkhz[k - 1] : 0.f;
         /* transpose of the H curl of step n+1 (H-grid 1/kappa) */
-        const float rxc = a.rx[c], ryc = a.ry[c], rzc = a.rz[c];
-        const float rkx = g.rkhx[i], rky = g.rkhy[j], rkz = g.rkhz[k];
-        ex += rkz * ryc - rky * rzc;
-        ey += rkx * rzc - rkz * rxc;
-        ez += rky * rxc - rkx * ryc;
-        if (k > 0) {
-            const float z1 = g.rkhz[k - 1];
-            ex -= z1 * a.ry[c - P];
-            ey += z1 * a.rx[c - P];
-        }
+        ex += rhz * ryc - rhy * rzc - rhz1 * rym;
+        ey += rhx * rzc - rhz * rxc + rhz1 * rxm;
+        ez += rhy * rxc - rhx * ryc;
         if (j > 0) {
-            const float y1 = g.rkhy[j - 1];
-            ex += y1 * a.rz[c - px];
-            ez -= y1 * a.rx[c - px];
+            ex += rhy1 * ldf(a.rz, o - dJ);
+            ez -= rhy1 * ldf(a.rx, o - dJ);
         }
         if (i > 0) {
-            const float x1 = g.rkhx[i - 1];
-            ey -= x1 * a.rz[c - 1];
-            ez += x1 * a.ry[c - 1];
+            ey -= rhx1 * ldf(a.rz, o - 4);
+            ez += rhx1 * ldf(a.ry, o - 4);
         }
+        rxm = rxc;
+        rym = ryc;
         /* transpose of the H-side psi updates of step n+1 */
-        const bool jin = j >= 1 && j < ny - 1, kin = k >= 1 && k < nz - 1, iin = i >= 1 && i < nx - 1;
-        const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
-        if (jin && kin) {
-            if (F0.on) {
-                const long long row = ((long long)k * ny + j) * F0.depth;
-                if (i < F0.depth - 1) {
-                    const float cc = F0.ch[i].y;
-                    const float2 t = a.th[0][row + i];
-                    ey -= cc * t.x;
-                    ez -= cc * t.y;
-                }
-                if (i >= 1 && i - 1 < F0.depth - 1) {
-                    const float cc = F0.ch[i - 1].y;
-                    const float2 t = a.th[0][row + i - 1];
-                    ey += cc * t.x;
-                    ez += cc * t.y;
-                }
+        const bool kin = k >= 1 && k < nz - 1;
+        if (kin) {
+            if (hx0a) {
+                const float cc = F0.ch[i].y;
+                const float2 t = ld2(a.th[0], (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8);
+                ey -= cc * t.x;
+                ez -= cc * t.y;
             }
-            if (F1.on) {
-                const int lo = nx - F1.depth + 1;
-                const long long row = ((long long)k * ny + j) * F1.depth;
-                if (i >= lo && i < nx - 1) {
-                    const int ir = i - (nx - F1.depth);
-                    const float cc = F1.ch[ir].y;
-                    const float2 t = a.th[1][row + ir];
-                    ey -= cc * t.x;
-                    ez -= cc * t.y;
-                }
-                if (i - 1 >= lo && i - 1 < nx - 1) {
-                    const int ir = i - 1 - (nx - F1.depth);
-                    const float cc = F1.ch[ir].y;
-                    const float2 t = a.th[1][row + ir];
-                    ey += cc * t.x;
-                    ez += cc * t.y;
-                }
+            if (hx0b) {
+                const float cc = F0.ch[i - 1].y;
+                const float2 t = ld2(a.th[0], (OFF)(((unsigned)k * ny + j) * F0.depth + i - 1) * 8);
+                ey += cc * t.x;
+                ez += cc * t.y;
             }
-        }
-        if (iin && kin) {
-            if (F2.on) {
-                if (j < F2.depth - 1) {
-                    const float cc = F2.ch[j].y;
-                    const float2 t = a.th[2][((long long)k * F2.depth + j) * px + i];
-                    ez -= cc * t.x;
-                    ex -= cc * t.y;
-                }
-                if (j >= 1 && j - 1 < F2.depth - 1) {
-                    const float cc = F2.ch[j - 1].y;
-                    const float2 t = a.th[2][((long long)k * F2.depth + j - 1) * px + i];
-                    ez += cc * t.x;
-                    ex += cc * t.y;
-                }
+            if (hxna) {
+                const int ir = i - (nx - F1.depth);
+                const float cc = F1.ch[ir].y;
+                const float2 t = ld2(a.th[1], (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8);
+                ey -= cc * t.x;
+                ez -= cc * t.y;
             }
-            if (F3.on) {
-                const int lo = ny - F3.depth + 1;
-                if (j >= lo && j < ny - 1) {
-                    const int jr = j - (ny - F3.depth);
-                    const float cc = F3.ch[jr].y;
-                    const float2 t = a.th[3][((long long)k * F3.depth + jr) * px + i];
-                    ez -= cc * t.x;
-                    ex -= cc * t.y;
-                }
-                if (j - 1 >= lo && j - 1 < ny - 1) {
-                    const int jr = j - 1 - (ny - F3.depth);
-                    const float cc = F3.ch[jr].y;
-                    const float2 t = a.th[3][((long long)k * F3.depth + jr) * px + i];
-                    ez += cc * t.x;
-                    ex += cc * t.y;
-                }
+            if (hxnb) {
+                const int ir = i - 1 - (nx - F1.depth);
+                const float cc = F1.ch[ir].y;
+                const float2 t = ld2(a.th[1], (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8);
+                ey += cc * t.x;
+                ez += cc * t.y;
+            }
+            if (hy0a) {
+                const float cc = F2.ch[j].y;
+                const float2 t = ld2(a.th[2], (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8);
+                ez -= cc * t.x;
+                ex -= cc * t.y;
+            }
+            if (hy0b) {
+                const float cc = F2.ch[j - 1].y;
+                const float2 t = ld2(a.th[2], (OFF)(((unsigned)k * F2.depth + j - 1) * px + i) * 8);
+                ez += cc * t.x;
+                ex += cc * t.y;
+            }
+            if (hyna) {
+                const int jr = j - (ny - F3.depth);
+                const float cc = F3.ch[jr].y;
+                const float2 t = ld2(a.th[3], (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8);
+                ez -= cc * t.x;
+                ex -= cc * t.y;
+            }
+            if (hynb) {
+                const int jr = j - 1 - (ny - F3.depth);
+                const float cc = F3.ch[jr].y;
+                const float2 t = ld2(a.th[3], (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8);
+                ez += cc * t.x;
+                ex += cc * t.y;
             }
         }
-        if (iin && jin) {
-            if (F4.on) {
-                if (k < F4.depth - 1) {
-                    const float cc = F4.ch[k].y;
-                    const float2 t = a.th[4][((long long)k * ny + j) * px + i];
-                    ex -= cc * t.x;
-                    ey -= cc * t.y;
-                }
-                if (k >= 1 && k - 1 < F4.depth - 1) {
-                    const float cc = F4.ch[k - 1].y;
-                    const float2 t = a.th[4][((long long)(k - 1) * ny + j) * px + i];
-                    ex += cc * t.x;
-                    ey += cc * t.y;
-                }
+        if (ijin) {
+            if (F4.on && k < F4.depth - 1) {
+                const float cc = F4.ch[k].y;
+                const float2 t = ld2(a.th[4], (OFF)(((unsigned)k * ny + j) * px + i) * 8);
+                ex -= cc * t.x;
+                ey -= cc * t.y;
             }
-            if (F5.on) {
-                const int lo = nz - F5.depth + 1;
-                if (k >= lo && k < nz - 1) {
-                    const int kr = k - (nz - F5.depth);
-                    const float cc = F5.ch[kr].y;
-                    const float2 t = a.th[5][((long long)kr * ny + j) * px + i];
-                    ex -= cc * t.x;
-                    ey -= cc * t.y;
-                }
-                if (k - 1 >= lo && k - 1 < nz - 1) {
-                    const int kr = k - 1 - (nz - F5.depth);
-                    const float cc = F5.ch[kr].y;
-                    const float2 t = a.th[5][((long long)kr * ny + j) * px + i];
-                    ex += cc * t.x;
-                    ey += cc * t.y;
-                }
+            if (F4.on && k >= 1 && k - 1 < F4.depth - 1) {
+                const float cc = F4.ch[k - 1].y;
+                const float2 t = ld2(a.th[4], (OFF)(((unsigned)(k - 1) * ny + j) * px + i) * 8);
+                ex += cc * t.x;
+                ey += cc * t.y;
             }
-        }
-    }
-
-    /* objective source: dJ/dE^{n+1} = 2 (Re F cos - Im F sin) */
-    if (i >= a.mi0 && i < a.mi0 + a.mnx && j >= a.mj0 && j < a.mj0 + a.mny && k >= a.mk0 && k < a.mk0 + a.mnz) {
-        const long long q = (i - a.mi0) + (long long)a.mnx * ((j - a.mj0) + (long long)a.mny * (k - a.mk0));
-        const double *F = a.macc + q * 6;
-        double s, co;
-        sincos(a.omega * (step + 1) * a.dt, &s, &co);
-        ex += (float)(2.0 * (F[0] * co - F[1] * s));
-        ey += (float)(2.0 * (F[2] * co - F[3] * s));
-        ez += (float)(2.0 * (F[4] * co - F[5] * s));
-    }
-    /* hard point sources overwrite E: their cells carry no adjoint back */
-    for (int s = 0; s < a.nsrc; s++)
-        if (a.spos[s] == c) {
-            const float *v = a.svals + ((long long)step * a.nsrc + s) * 6;
-            if (v[0] != 0.f) ex = 0.f;
-            if (v[1] != 0.f) ey = 0.f;
-            if (v[2] != 0.f) ez = 0.f;
+            const int lo5 = nz - F5.depth + 1;
+            if (F5.on && k >= lo5 && k < nz - 1) {
+                const int kr = k - (nz - F5.depth);
+                const float cc = F5.ch[kr].y;
+                const float2 t = ld2(a.th[5], (OFF)(((unsigned)kr * ny + j) * px + i) * 8);
+                ex -= cc * t.x;
+                ey -= cc * t.y;
+            }
+            if (F5.on && k - 1 >= lo5 && k - 1 < nz - 1) {
+                const int kr = k - 1 - (nz - F5.depth);
+                const float cc = F5.ch[kr].y;
+                const float2 t = ld2(a.th[5], (OFF)(((unsigned)kr * ny + j) * px + i) * 8);
+                ex += cc * t.x;
+                ey += cc * t.y;
+            }
         }
 
-    /* E-step transpose: plane zeroing, then the update and the psi_E corrections */
-    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
-    if ((F0.on && i == 0) || (F2.on && j == 0) || (F4.on && k == 0)) {
-        ex = 0.f;
-        ey = 0.f;
-        ez = 0.f;
-    }
-    const GsvECoef ec = e_coef<ID>(g, i, j, k, c);
-    const unsigned fl = ec.flags;
-    if (a.ncoef && i >= g.pc_i0 && i < g.pc_i0 + g.pc_nx && j >= g.pc_j0 && j < g.pc_j0 + g.pc_ny && k >= g.pc_k0
-        && k < g.pc_k0 + g.pc_nz) {
-        /* design coefficient cell: dJ/dca += adj(E^{n+1}) E^n, dJ/dcb += adj(E^{n+1}) D^n */
-        const long long q = (i - g.pc_i0) + (long long)g.pc_nx * ((j - g.pc_j0) + (long long)g.pc_ny * (k - g.pc_k0));
-        const float *h = a.hist + (long long)slot * 6 * a.ncoef + q;
-        a.gca[q] += (double)ex * h[0];
-        a.gca[a.ncoef + q] += (double)ey * h[a.ncoef];
-        a.gca[2 * a.ncoef + q] += (double)ez * h[2 * a.ncoef];
-        a.gcb[q] += (double)ex * h[3 * a.ncoef];
-        a.gcb[a.ncoef + q] += (double)ey * h[4 * a.ncoef];
-        a.gcb[2 * a.ncoef + q] += (double)ez * h[5 * a.ncoef];
-    }
-    const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
-    /* psi_E adjoints: decay of the later step, plus the corrections of this one */
-    if (F0.on && i >= 1 && i < F0.depth) {
-        float2 *t = a.te[0] + ((long long)k * ny + j) * F0.depth + i;
-        const float b = F0.ce[i].x;
-        float t1 = b * t->x, t2 = b * t->y;
-        if (j != 0 && k != 0) {
-            if (ty) t1 -= g.dx * ec.cb[1] * ey;
-            if (tz) t2 += g.dx * ec.cb[2] * ez;
+        /* objective source: dJ/dE^{n+1} = 2 (Re F cos - Im F sin) */
+        if (in_mon_ij && k >= a.mk0 && k < a.mk0 + a.mnz) {
+            const long long q = (i - a.mi0) + (long long)a.mnx * ((j - a.mj0) + (long long)a.mny * (k - a.mk0));
+            const double *F = a.macc + q * 6;
+            double s, co;
+            sincos(a.omega * (step + 1) * a.dt, &s, &co);
+            ex += (float)(2.0 * (F[0] * co - F[1] * s));
+            ey += (float)(2.0 * (F[2] * co - F[3] * s));
+            ez += (float)(2.0 * (F[4] * co - F[5] * s));
         }
-        *t = make_float2(t1, t2);
-    }
-    if (F2.on && j >= 1 && j < F2.depth) {
-        float2 *t = a.te[2] + ((long long)k * F2.depth + j) * px + i;
-        const float b = F2.ce[j].x;
-        float t1 = b * t->x, t2 = b * t->y;
-        if (i != 0 && k != 0) {
-            if (tx) t1 += g.dy * ec.cb[0] * ex;
-            if (tz) t2 -= g.dy * ec.cb[2] * ez;
+        /* hard point sources overwrite E: their cells carry no adjoint back */
+        if (a.nsrc) {
+            const long long c = i + (long long)px * j + g.plane * k;
+            for (int s = 0; s < a.nsrc; s++)
+                if (a.spos[s] == c) {
+                    const float *v = a.svals + ((long long)step * a.nsrc + s) * 6;
+                    if (v[0] != 0.f) ex = 0.f;
+                    if (v[1] != 0.f) ey = 0.f;
+                    if (v[2] != 0.f) ez = 0.f;
+                }
         }
-        *t = make_float2(t1, t2);
-    }
-    if (F4.on && k >= 1 && k < F4.depth) {
-        float2 *t = a.te[4] + ((long long)k * ny + j) * px + i;
-        const float b = F4.ce[k].x;
-        float t1 = b * t->x, t2 = b * t->y;
-        if (i != 0 && j != 0) {
-            if (ty) t1 += g.dx * ec.cb[1] * ey;
-            if (tz) t2 -= g.dx * ec.cb[0] * ex;
+
+        /* E-step transpose: plane zeroing, then the update and the psi_E corrections */
+        if (zplane_xy || (F4.on && k == 0)) {
+            ex = 0.f;
+            ey = 0.f;
+            ez = 0.f;
         }
-        *t = make_float2(t1, t2);
-    }
-    if (F1.on && i >= nx - F1.depth + 1) {
-        const int ir = i - (nx - F1.depth);
-        float2 *t = a.te[1] + ((long long)k * ny + j) * F1.depth + ir;
-        const float b = F1.ce[ir].x;
-        float t1 = b * t->x, t2 = b * t->y;
-        if (j != 0 && k != 0) {
-            if (ty) t1 -= g.dx * ec.cb[1] * ey;
-            if (tz) t2 += g.dx * ec.cb[2] * ez;
+        const GsvECoef ec = e_coef<ID>(g, oc, i, j, k);
+        const unsigned fl = ec.flags;
+        if (in_pc_ij && k >= g.pc_k0 && k < g.pc_k0 + g.pc_nz) {
+            /* design coefficient cell: dJ/dca += adj(E^{n+1}) E^n, dJ/dcb += adj(E^{n+1}) D^n */
+            const long long q = (i - g.pc_i0) + (long long)g.pc_nx * ((j - g.pc_j0) + (long long)g.pc_ny * (k - g.pc_k0));
+            const float *h = a.hist + (long long)slot * 6 * a.ncoef + q;
+            a.gca[q] += (double)ex * h[0];
+            a.gca[a.ncoef + q] += (double)ey * h[a.ncoef];
+            a.gca[2 * a.ncoef + q] += (double)ez * h[2 * a.ncoef];
+            a.gcb[q] += (double)ex * h[3 * a.ncoef];
+            a.gcb[a.ncoef + q] += (double)ey * h[4 * a.ncoef];
+            a.gcb[2 * a.ncoef + q] += (double)ez * h[5 * a.ncoef];
         }
-        *t = make_float2(t1, t2);
-    }
-    if (F3.on && j >= ny - F3.depth + 1) {
-        const int jr = j - (ny - F3.depth);
-        float2 *t = a.te[3] + ((long long)k * F3.depth + jr) * px + i;
-        const float b = F3.ce[jr].x;
-        float t1 = b * t->x, t2 = b * t->y;
-        if (i != 0 && k != 0) {
-            if (tx) t1 += g.dy * ec.cb[0] * ex;
-            if (tz) t2 -= g.dy * ec.cb[2] * ez;
+        const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
+        /* psi_E adjoints: decay of the later step, plus the corrections of this one */
+        if (ex0) {
+            const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
+            const float b = F0.ce[i].x;
+            const float2 t = ld2(a.te[0], p);
+            float t1 = b * t.x, t2 = b * t.y;
+            if (j != 0 && k != 0) {
+                if (ty) t1 -= g.dx * ec.cb[1] * ey;
+                if (tz) t2 += g.dx * ec.cb[2] * ez;
+            }
+            st2(a.te[0], p, t1, t2);
         }
-        *t = make_float2(t1, t2);
-    }
-    if (F5.on && k >= nz - F5.depth + 1) {
-        const int kr = k - (nz - F5.depth);
-        float2 *t = a.te[5] + ((long long)kr * ny + j) * px + i;
-        const float b = F5.ce[kr].x;
-        float t1 = b * t->x, t2 = b * t->y;
-        if (i != 0 && j != 0) {
-            if (ty) t1 += g.dx * ec.cb[1] * ey;
-            if (tx) t2 -= g.dx * ec.cb[0] * ex;
+        if (ey0) {
+            const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
+            const float b = F2.ce[j].x;
+            const float2 t = ld2(a.te[2], p);
+            float t1 = b * t.x, t2 = b * t.y;
+            if (i != 0 && k != 0) {
+                if (tx) t1 += g.dy * ec.cb[0] * ex;
+                if (tz) t2 -= g.dy * ec.cb[2] * ez;
+            }
+            st2(a.te[2], p, t1, t2);
         }
-        *t = make_float2(t1, t2);
+        if (F4.on && k >= 1 && k < F4.depth) {
+            const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
+            const float b = F4.ce[k].x;
+            const float2 t = ld2(a.te[4], p);
+            float t1 = b * t.x, t2 = b * t.y;
+            if (i != 0 && j != 0) {
+                if (ty) t1 += g.dx * ec.cb[1] * ey;
+                if (tz) t2 -= g.dx * ec.cb[0] * ex;
+            }
+            st2(a.te[4], p, t1, t2);
+        }
+        if (exn) {
+            const int ir = i - (nx - F1.depth);
+            const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8;
+            const float b = F1.ce[ir].x;
+            const float2 t = ld2(a.te[1], p);
+            float t1 = b * t.x, t2 = b * t.y;
+            if (j != 0 && k != 0) {
+                if (ty) t1 -= g.dx * ec.cb[1] * ey;
+                if (tz) t2 += g.dx * ec.cb[2] * ez;
+            }
+            st2(a.te[1], p, t1, t2);
+        }
+        if (eyn) {
+            const int jr = j - (ny - F3.depth);
+            const OFF p = (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8;
+            const float b = F3.ce[jr].x;
+            const float2 t = ld2(a.te[3], p);
+            float t1 = b * t.x, t2 = b * t.y;
+            if (i != 0 && k != 0) {
+                if (tx) t1 += g.dy * ec.cb[0] * ex;
+                if (tz) t2 -= g.dy * ec.cb[2] * ez;
+            }
+            st2(a.te[3], p, t1, t2);
+        }
+        if (F5.on && k >= nz - F5.depth + 1) {
+            const int kr = k - (nz - F5.depth);
+            const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
+            const float b = F5.ce[kr].x;
+            const float2 t = ld2(a.te[5], p);
+            float t1 = b * t.x, t2 = b * t.y;
+            if (i != 0 && j != 0) {
+                if (ty) t1 += g.dx * ec.cb[1] * ey;
+                if (tx) t2 -= g.dx * ec.cb[0] * ex;
+            }
+            st2(a.te[5], p, t1, t2);
+        }
+        /* PEC zeroing acts on the curl update only (the corrections come after it) */
+        const float mx = (fl & GSV_E_PECX) ? 0.f : ex, my = (fl & GSV_E_PECY) ? 0.f : ey;
+        const float mz = (fl & GSV_E_PECZ) ? 0.f : ez;
+        const bool ux = tx && j > 0 && k > 0, uy = ty && i > 0 && k > 0, uz = tz && i > 0 && j > 0;
+        stf(a.qx, o, ux ? ec.cb[0] * mx : 0.f);
+        stf(a.qy, o, uy ? ec.cb[1] * my : 0.f);
+        stf(a.qz, o, uz ? ec.cb[2] * mz : 0.f);
+        stn(a.ebx, o, ux ? ec.ca[0] * mx : mx);
+        stn(a.eby, o, uy ? ec.ca[1] * my : my);
+        stn(a.ebz, o, uz ? ec.ca[2] * mz : mz);
     }
-    /* PEC zeroing acts on the curl update only (the corrections come after it) */
-    const float mx = (fl & GSV_E_PECX) ? 0.f : ex, my = (fl & GSV_E_PECY) ? 0.f : ey, mz = (fl & GSV_E_PECZ) ? 0.f : ez;
-    const bool ux = tx && j > 0 && k > 0, uy = ty && i > 0 && k > 0, uz = tz && i > 0 && j > 0;
-    a.qx[c] = ux ? ec.cb[0] * mx : 0.f;
-    a.qy[c] = uy ? ec.cb[1] * my : 0.f;
-    a.qz[c] = uz ? ec.cb[2] * mz : 0.f;
-    a.ebx[c] = ux ? ec.ca[0] * mx : mx;
-    a.eby[c] = uy ? ec.ca[1] * my : my;
-    a.ebz[c] = uz ? ec.ca[2] * mz : mz;
 }
 
 /* H cells: E-step transpose (gathers of q and the psi_E adjoints), then the
  * local part of the H-step transpose */
-template <typename ID>
-__global__ __launch_bounds__(256) void
-k_adj_h(GsvGrid g, GsvAdj a)
+template <typename ID, typename OFF>
+__global__ __launch_bounds__(512) void
+k_adj_h(GsvGrid g, GsvAdj a, int kchunk)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (i >= nx || j >= ny)
         return;
+    const int k0 = blockIdx.z * kchunk;
+    const int k1 = min(k0 + kchunk, nz);
     const int px = g.px;
-    const long long P = g.plane;
-    const long long c = i + (long long)px * j + P * k;
-    float hx = a.hbx[c], hy = a.hby[c], hz = a.hbz[c];
+    const OFF dP = (OFF)g.plane * 4, dJ = (OFF)px * 4, dPc = (OFF)g.plane * sizeof(ID);
+    OFF o = ((OFF)i + (OFF)px * j + (OFF)g.plane * (k0 + 1)) * 4;
+    OFF oc = ((OFF)i + (OFF)px * j + (OFF)g.plane * k0) * sizeof(ID);
 
-    /* transpose of the E curl (E-grid 1/kappa) */
-    {
-        const float qxc = a.qx[c], qyc = a.qy[c], qzc = a.qz[c];
-        const float rkx = g.rkex[i], rky = g.rkey[j], rkz = g.rkez[k];
-        hx += rkz * qyc - rky * qzc;
-        hy += rkx * qzc - rkz * qxc;
-        hz += rky * qxc - rkx * qyc;
-        if (k + 1 < nz) {
-            const float z1 = g.rkez[k + 1];
-            hx -= z1 * a.qy[c + P];
-            hy += z1 * a.qx[c + P];
-        }
-        if (j + 1 < ny) {
-            const float y1 = g.rkey[j + 1];
-            hx += y1 * a.qz[c + px];
-            hz -= y1 * a.qx[c + px];
-        }
-        if (i + 1 < nx) {
-            const float x1 = g.rkex[i + 1];
-            hy -= x1 * a.qz[c + 1];
-            hz += x1 * a.qy[c + 1];
-        }
-    }
-    /* transpose of the E-side psi updates */
+    const float rex = g.rkex[i], rey = g.rkey[j];
+    const float rex1 = i + 1 < nx ? g.rkex[i + 1] : 0.f, rey1 = j + 1 < ny ? g.rkey[j + 1] : 0.f;
+    const bool ip = i + 1 < nx, jp = j + 1 < ny;
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
-    if (F0.on) {
-        const long long row = ((long long)k * ny + j) * F0.depth;
-        if (i >= 1 && i < F0.depth) {
+    /* E-side psi cells whose H neighbours include this column */
+    const bool ex0a = F0.on && i >= 1 && i < F0.depth, ex0b = F0.on && i + 1 < F0.depth;
+    const int lo1 = nx - F1.depth + 1;
+    const bool exna = F1.on && i >= lo1, exnb = F1.on && i + 1 >= lo1 && i + 1 < nx;
+    const bool ey0a = F2.on && j >= 1 && j < F2.depth, ey0b = F2.on && j + 1 < F2.depth;
+    const int lo3 = ny - F3.depth + 1;
+    const bool eyna = F3.on && j >= lo3, eynb = F3.on && j + 1 >= lo3 && j + 1 < ny;
+    const bool iin = (i >= 1 && i < nx - 1), jin = (j >= 1 && j < ny - 1), ijin = iin && jin;
+    const bool zx = F1.on && i == nx - 1 && jin, zy = F3.on && j == ny - 1 && iin;
+    const bool hx0 = F0.on && i < F0.depth - 1 && jin, hxn = F1.on && i >= nx - F1.depth + 1 && i < nx - 1 && jin;
+    const bool hy0 = F2.on && j < F2.depth - 1 && iin, hyn = F3.on && j >= ny - F3.depth + 1 && j < ny - 1 && iin;
+
+    /* q of plane k (carried), the plane above is loaded each step (guard plane above nz-1 is zero) */
+    float qxk = ldf(a.qx, o), qyk = ldf(a.qy, o);
+    for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
+        float hx = ldn(a.hbx, o), hy = ldn(a.hby, o), hz = ldn(a.hbz, o);
+        const float qzc = ldf(a.qz, o);
+        const float qxk1 = ldf(a.qx, o + dP), qyk1 = ldf(a.qy, o + dP);
+        const float rez = g.rkez[k], rez1 = k + 1 < nz ? g.rkez[k + 1] : 0.f;
+        /* transpose of the E curl (E-grid 1/kappa) */
+        hx += rez * qyk - rey * qzc - rez1 * qyk1;
+        hy += rex * qzc - rez * qxk + rez1 * qxk1;
+        hz += rey * qxk - rex * qyk;
+        if (jp) {
+            hx += rey1 * ldf(a.qz, o + dJ);
+            hz -= rey1 * ldf(a.qx, o + dJ);
+        }
+        if (ip) {
+            hy -= rex1 * ldf(a.qz, o + 4);
+            hz += rex1 * ldf(a.qy, o + 4);
+        }
+        qxk = qxk1;
+        qyk = qyk1;
+        /* transpose of the E-side psi updates */
+        if (ex0a) {
             const float cc = F0.ce[i].y;
-            const float2 t = a.te[0][row + i];
+            const float2 t = ld2(a.te[0], (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8);
             hz += cc * t.x;
             hy += cc * t.y;
         }
-        if (i + 1 >= 1 && i + 1 < F0.depth) {
+        if (ex0b) {
             const float cc = F0.ce[i + 1].y;
-            const float2 t = a.te[0][row + i + 1];
+            const float2 t = ld2(a.te[0], (OFF)(((unsigned)k * ny + j) * F0.depth + i + 1) * 8);
             hz -= cc * t.x;
             hy -= cc * t.y;
         }
-    }
-    if (F1.on) {
-        const int lo = nx - F1.depth + 1;
-        const long long row = ((long long)k * ny + j) * F1.depth;
-        if (i >= lo) {
+        if (exna) {
             const int ir = i - (nx - F1.depth);
             const float cc = F1.ce[ir].y;
-            const float2 t = a.te[1][row + ir];
+            const float2 t = ld2(a.te[1], (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8);
             hz += cc * t.x;
             hy += cc * t.y;
         }
-        if (i + 1 >= lo && i + 1 < nx) {
+        if (exnb) {
             const int ir = i + 1 - (nx - F1.depth);
             const float cc = F1.ce[ir].y;
-            const float2 t = a.te[1][row + ir];
+            const float2 t = ld2(a.te[1], (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8);
             hz -= cc * t.x;
             hy -= cc * t.y;
         }
-    }
-    if (F2.on) {
-        if (j >= 1 && j < F2.depth) {
+        if (ey0a) {
             const float cc = F2.ce[j].y;
-            const float2 t = a.te[2][((long long)k * F2.depth + j) * px + i];
+            const float2 t = ld2(a.te[2], (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8);
             hz += cc * t.x;
             hx += cc * t.y;
         }
-        if (j + 1 >= 1 && j + 1 < F2.depth) {
+        if (ey0b) {
             const float cc = F2.ce[j + 1].y;
-            const float2 t = a.te[2][((long long)k * F2.depth + j + 1) * px + i];
+            const float2 t = ld2(a.te[2], (OFF)(((unsigned)k * F2.depth + j + 1) * px + i) * 8);
             hz -= cc * t.x;
             hx -= cc * t.y;
         }
-    }
-    if (F3.on) {
-        const int lo = ny - F3.depth + 1;
-        if (j >= lo) {
+        if (eyna) {
             const int jr = j - (ny - F3.depth);
             const float cc = F3.ce[jr].y;
-            const float2 t = a.te[3][((long long)k * F3.depth + jr) * px + i];
+            const float2 t = ld2(a.te[3], (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8);
             hz += cc * t.x;
             hx += cc * t.y;
         }
-        if (j + 1 >= lo && j + 1 < ny) {
+        if (eynb) {
             const int jr = j + 1 - (ny - F3.depth);
             const float cc = F3.ce[jr].y;
-            const float2 t = a.te[3][((long long)k * F3.depth + jr) * px + i];
+            const float2 t = ld2(a.te[3], (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8);
             hz -= cc * t.x;
             hx -= cc * t.y;
         }
-    }
-    if (F4.on) {
-        if (k >= 1 && k < F4.depth) {
-            const float cc = F4.ce[k].y;
-            const float2 t = a.te[4][((long long)k * ny + j) * px + i];
-            hx += cc * t.x;
-            hy += cc * t.y;
+        if (F4.on) {
+            if (k >= 1 && k < F4.depth) {
+                const float cc = F4.ce[k].y;
+                const float2 t = ld2(a.te[4], (OFF)(((unsigned)k * ny + j) * px + i) * 8);
+                hx += cc * t.x;
+                hy += cc * t.y;
+            }
+            if (k + 1 < F4.depth) {
+                const float cc = F4.ce[k + 1].y;
+                const float2 t = ld2(a.te[4], (OFF)(((unsigned)(k + 1) * ny + j) * px + i) * 8);
+                hx -= cc * t.x;
+                hy -= cc * t.y;
+            }
         }
-        if (k + 1 >= 1 && k + 1 < F4.depth) {
-            const float cc = F4.ce[k + 1].y;
-            const float2 t = a.te[4][((long long)(k + 1) * ny + j) * px + i];
-            hx -= cc * t.x;
-            hy -= cc * t.y;
+        if (F5.on) {
+            const int lo5 = nz - F5.depth + 1;
+            if (k >= lo5) {
+                const int kr = k - (nz - F5.depth);
+                const float cc = F5.ce[kr].y;
+                const float2 t = ld2(a.te[5], (OFF)(((unsigned)kr * ny + j) * px + i) * 8);
+                hx += cc * t.x;
+                hy += cc * t.y;
+            }
+            if (k + 1 >= lo5 && k + 1 < nz) {
+                const int kr = k + 1 - (nz - F5.depth);
+                const float cc = F5.ce[kr].y;
+                const float2 t = ld2(a.te[5], (OFF)(((unsigned)kr * ny + j) * px + i) * 8);
+                hx -= cc * t.x;
+                hy -= cc * t.y;
+            }
         }
-    }
-    if (F5.on) {
-        const int lo = nz - F5.depth + 1;
-        if (k >= lo) {
-            const int kr = k - (nz - F5.depth);
-            const float cc = F5.ce[kr].y;
-            const float2 t = a.te[5][((long long)kr * ny + j) * px + i];
-            hx += cc * t.x;
-            hy += cc * t.y;
-        }
-        if (k + 1 >= lo && k + 1 < nz) {
-            const int kr = k + 1 - (nz - F5.depth);
-            const float cc = F5.ce[kr].y;
-            const float2 t = a.te[5][((long long)kr * ny + j) * px + i];
-            hx -= cc * t.x;
-            hy -= cc * t.y;
-        }
-    }
 
-    /* H-step transpose: zeroing, then the update and the psi_H corrections */
-    const bool iin = (i >= 1 && i < nx - 1), jin = (j >= 1 && j < ny - 1), ijin = iin && jin;
-    const bool kin = (k >= 1 && k < nz - 1);
-    const bool zx = F1.on && i == nx - 1 && jin, zy = F3.on && j == ny - 1 && iin;
-    if ((kin && (zx || zy)) || (F5.on && k == nz - 1 && ijin)) {
-        hx = 0.f;
-        hy = 0.f;
-        hz = 0.f;
+        /* H-step transpose: zeroing, then the update and the psi_H corrections */
+        const bool kin = (k >= 1 && k < nz - 1);
+        if ((kin && (zx || zy)) || (F5.on && k == nz - 1 && ijin)) {
+            hx = 0.f;
+            hy = 0.f;
+            hz = 0.f;
+        }
+        const GsvHCoef hc = g.htab[ldid<ID>(g.hcls, oc)];
+        if (kin) {
+            if (hx0) {
+                const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
+                const float b = F0.ch[i].x;
+                const float2 t = ld2(a.th[0], p);
+                st2(a.th[0], p, b * t.x - g.dx * hc.dbc * hz, b * t.y + g.dx * hc.dbc * hy);
+            }
+            if (hxn) {
+                const int ir = i - (nx - F1.depth);
+                const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8;
+                const float b = F1.ch[ir].x;
+                const float2 t = ld2(a.th[1], p);
+                st2(a.th[1], p, b * t.x - g.dx * hc.dbc * hz, b * t.y + g.dx * hc.dbc * hy);
+            }
+            if (hy0) {
+                const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
+                const float b = F2.ch[j].x;
+                const float2 t = ld2(a.th[2], p);
+                st2(a.th[2], p, b * t.x - g.dy * hc.dbc * hx, b * t.y + g.dy * hc.dbc * hz);
+            }
+            if (hyn) {
+                const int jr = j - (ny - F3.depth);
+                const OFF p = (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8;
+                const float b = F3.ch[jr].x;
+                const float2 t = ld2(a.th[3], p);
+                st2(a.th[3], p, b * t.x - g.dy * hc.dbc * hx, b * t.y + g.dy * hc.dbc * hz);
+            }
+        }
+        if (ijin) {
+            if (F4.on && k < F4.depth - 1) {
+                const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
+                const float b = F4.ch[k].x;
+                const float2 t = ld2(a.th[4], p);
+                st2(a.th[4], p, b * t.x - g.dz * hc.dbc * hy, b * t.y + g.dz * hc.dbc * hx);
+            }
+            if (F5.on && k >= nz - F5.depth + 1 && k < nz - 1) {
+                const int kr = k - (nz - F5.depth);
+                const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
+                const float b = F5.ch[kr].x;
+                const float2 t = ld2(a.th[5], p);
+                st2(a.th[5], p, b * t.x - g.dz * hc.dbc * hy, b * t.y + g.dz * hc.dbc * hx);
+            }
+        }
+        const bool ux = !(j == ny - 1 || k == nz - 1), uy = !(i == nx - 1 || k == nz - 1);
+        const bool uz = !(i == nx - 1 || j == ny - 1);
+        stf(a.rx, o, ux ? hc.db[0] * hx : 0.f);
+        stf(a.ry, o, uy ? hc.db[1] * hy : 0.f);
+        stf(a.rz, o, uz ? hc.db[2] * hz : 0.f);
+        stn(a.hbx, o, ux ? hc.da * hx : hx);
+        stn(a.hby, o, uy ? hc.da * hy : hy);
+        stn(a.hbz, o, uz ? hc.da * hz : hz);
     }
-    const GsvHCoef hc = g.htab[((const ID *)g.hcls)[c]];
-    if (kin) {
-        if (F0.on && i < F0.depth - 1 && jin) {
-            float2 *t = a.th[0] + ((long long)k * ny + j) * F0.depth + i;
-            const float b = F0.ch[i].x;
-            *t = make_float2(b * t->x - g.dx * hc.dbc * hz, b * t->y + g.dx * hc.dbc * hy);
-        }
-        if (F1.on && i >= nx - F1.depth + 1 && i < nx - 1 && jin) {
-            const int ir = i - (nx - F1.depth);
-            float2 *t = a.th[1] + ((long long)k * ny + j) * F1.depth + ir;
-            const float b = F1.ch[ir].x;
-            *t = make_float2(b * t->x - g.dx * hc.dbc * hz, b * t->y + g.dx * hc.dbc * hy);
-        }
-        if (F2.on && j < F2.depth - 1 && iin) {
-            float2 *t = a.th[2] + ((long long)k * F2.depth + j) * px + i;
-            const float b = F2.ch[j].x;
-            *t = make_float2(b * t->x - g.dy * hc.dbc * hx, b * t->y + g.dy * hc.dbc * hz);
-        }
-        if (F3.on && j >= ny - F3.depth + 1 && j < ny - 1 && iin) {
-            const int jr = j - (ny - F3.depth);
-            float2 *t = a.th[3] + ((long long)k * F3.depth + jr) * px + i;
-            const float b = F3.ch[jr].x;
-            *t = make_float2(b * t->x - g.dy * hc.dbc * hx, b * t->y + g.dy * hc.dbc * hz);
-        }
-    }
-    if (ijin) {
-        if (F4.on && k < F4.depth - 1) {
-            float2 *t = a.th[4] + ((long long)k * ny + j) * px + i;
-            const float b = F4.ch[k].x;
-            *t = make_float2(b * t->x - g.dz * hc.dbc * hy, b * t->y + g.dz * hc.dbc * hx);
-        }
-        if (F5.on && k >= nz - F5.depth + 1 && k < nz - 1) {
-            const int kr = k - (nz - F5.depth);
-            float2 *t = a.th[5] + ((long long)kr * ny + j) * px + i;
-            const float b = F5.ch[kr].x;
-            *t = make_float2(b * t->x - g.dz * hc.dbc * hy, b * t->y + g.dz * hc.dbc * hx);
-        }
-    }
-    const bool ux = !(j == ny - 1 || k == nz - 1), uy = !(i == nx - 1 || k == nz - 1), uz = !(i == nx - 1 || j == ny - 1);
-    a.rx[c] = ux ? hc.db[0] * hx : 0.f;
-    a.ry[c] = uy ? hc.db[1] * hy : 0.f;
-    a.rz[c] = uz ? hc.db[2] * hz : 0.f;
-    a.hbx[c] = ux ? hc.da * hx : hx;
-    a.hby[c] = uy ? hc.da * hy : hy;
-    a.hbz[c] = uz ? hc.da * hz : hz;
 }
 
 /* ------------------------------------------------------------------ */
@@ -675,7 +695,7 @@ gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_int
         float *p;
         if (dmalloc(E, &p, E->field_floats)) { delete A; return 1; }
         CK(hipMemsetAsync(p, 0, E->field_floats * 4, E->st));
-        *fa[q] = p + g.plane;
+        *fa[q] = p;
     }
     for (int f = 0; f < 6; f++) {
         size_t n = psi_pairs(g.f[f], f / 2, g);
@@ -847,14 +867,17 @@ gsv_engine_adjoint_forward(void *eng, int (*step_fn)(void *), void *ctx, double 
     return 0;
 }
 
+/* one backward step, with the E kernel's launch shape (32-bit offsets: the
+ * design region already requires field arrays below 4 GiB) */
 template <typename ID>
 static void
-launch_adj(Engine *E, int step, int slot, int first)
+launch_adj(Engine *E, int step, int slot)
 {
     GsvGrid &g = E->g;
-    dim3 blk(64, 4, 1), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
-    hipLaunchKernelGGL(k_adj_e<ID>, grd, blk, 0, E->st, g, E->adj->a, step, slot, first);
-    hipLaunchKernelGGL(k_adj_h<ID>, grd, blk, 0, E->st, g, E->adj->a);
+    const int bx = E->bx_e, by = E->by_e, kc = E->kchunk_e;
+    dim3 blk(bx, by, 1), grd((g.nx + bx - 1) / bx, (g.ny + by - 1) / by, (g.nz + kc - 1) / kc);
+    hipLaunchKernelGGL((k_adj_e<ID, uint32_t>), grd, blk, 0, E->st, g, E->adj->a, step, slot, kc);
+    hipLaunchKernelGGL((k_adj_h<ID, uint32_t>), grd, blk, 0, E->st, g, E->adj->a, kc);
 }
 
 /* Backward pass: coefficient gradients accumulated on the device; the host
@@ -878,7 +901,7 @@ gsv_engine_adjoint_backward(void *eng, int (*step_fn)(void *), void *ctx, float 
     CK(hipEventCreate(&e1));
     GsvAdj &a = A->a;
     for (float *p : { a.ebx, a.eby, a.ebz, a.hbx, a.hby, a.hbz, a.qx, a.qy, a.qz, a.rx, a.ry, a.rz })
-        CK(hipMemsetAsync(p - g.plane, 0, E->field_floats * 4, E->st));
+        CK(hipMemsetAsync(p, 0, E->field_floats * 4, E->st));
     for (int f = 0; f < 6; f++) {
         size_t n = psi_pairs(g.f[f], f / 2, g);
         if (n) {
@@ -889,7 +912,6 @@ gsv_engine_adjoint_backward(void *eng, int (*step_fn)(void *), void *ctx, float 
     CK(hipMemsetAsync(a.gca, 0, 3 * a.ncoef * 8, E->st));
     CK(hipMemsetAsync(a.gcb, 0, 3 * a.ncoef * 8, E->st));
     CK(hipEventRecord(e0, E->st));
-    int first = 1;
     const int nseg = A->history_all ? 1 : (A->nsteps + A->K - 1) / A->K;
     for (int s = nseg - 1; s >= 0; s--) {
         int n0 = 0, n1 = A->nsteps;
@@ -910,11 +932,10 @@ gsv_engine_adjoint_backward(void *eng, int (*step_fn)(void *), void *ctx, float 
         for (int n = n1 - 1; n >= n0; n--) {
             const int slot = A->history_all ? n : n - n0;
             if (E->idbytes_e == 1)
-                launch_adj<uint8_t>(E, n, slot, first);
+                launch_adj<uint8_t>(E, n, slot);
             else
-                launch_adj<uint16_t>(E, n, slot, first);
+                launch_adj<uint16_t>(E, n, slot);
             CK(hipGetLastError());
-            first = 0;
         }
     }
     CK(hipEventRecord(e1, E->st));

@@ -394,6 +394,12 @@ sv_gpu_new(SvPool *mp, SvSet *set)
             return NULL;
         }
     }
+    if (set->sa.design && (unsigned long long)E->g.plane * (E->g.nz + 2) * 4ull >= (1ull << 32)) {
+        fprintf(stderr, "Error: a design region needs field arrays below 4 GiB (%d x %d x %d is too large)\n",
+                E->g.nx, E->g.ny, E->g.nz);
+        delete E;
+        return NULL;
+    }
     if (set->sa.design) {
         /* voxel (i,j,k) feeds the averaged coefficients of cells up to (i+1, j+1, k+1) */
         E->g.pc_i0 = set->sa.design_i0;

@@ -26,100 +26,9 @@
 #include "gsvit_settings.h"
 #include "engine.h"
 #include "kernels.h"
+#include "device_access.h"
 
 
-
-/* ------------------------------------------------------------------ */
-/* addressing of the volume kernels                                     */
-/* ------------------------------------------------------------------ */
-
-/* Every per-cell array is addressed by a byte offset from its allocation
- * start (the guard plane), shared by all arrays: with a 32-bit offset the
- * loads take the uniform-base + 32-bit-offset form, so the z-march carries
- * one offset register instead of a 64-bit pointer per array. */
-template <typename OFF>
-__device__ __forceinline__ float
-ldf(const float *base, OFF off)
-{
-    return *(const float *)((const char *)base + off);
-}
-
-template <typename OFF>
-__device__ __forceinline__ void
-stf(float *base, OFF off, float v)
-{
-    *(float *)((char *)base + off) = v;
-}
-
-
-/* Streams a kernel reads or writes exactly once (the updated field family,
- * the ADE state) bypass the caches' retention with nontemporal accesses;
- * the other family, whose neighbours are re-read by adjacent rows, stays
- * cached. */
-template <typename OFF>
-__device__ __forceinline__ float
-ldn(const float *base, OFF off)
-{
-    return __builtin_nontemporal_load((const float *)((const char *)base + off));
-}
-
-template <typename OFF>
-__device__ __forceinline__ void
-stn(float *base, OFF off, float v)
-{
-    __builtin_nontemporal_store(v, (float *)((char *)base + off));
-}
-
-template <typename OFF>
-__device__ __forceinline__ float
-ldsn(const float *base, OFF off, int slot_bytes)
-{
-    return __builtin_nontemporal_load((const float *)((const char *)base + (size_t)off + slot_bytes));
-}
-
-template <typename OFF>
-__device__ __forceinline__ void
-stsn(float *base, OFF off, int slot_bytes, float v)
-{
-    __builtin_nontemporal_store(v, (float *)((char *)base + (size_t)off + slot_bytes));
-}
-
-/* state slot of a blocked ADE cell: the 32-bit lane offset is widened before
- * the constant slot offset is added, so the slot becomes an immediate */
-template <typename OFF>
-__device__ __forceinline__ float
-lds(const float *base, OFF off, int slot_bytes)
-{
-    return *(const float *)((const char *)base + (size_t)off + slot_bytes);
-}
-
-template <typename OFF>
-__device__ __forceinline__ void
-sts(float *base, OFF off, int slot_bytes, float v)
-{
-    *(float *)((char *)base + (size_t)off + slot_bytes) = v;
-}
-
-template <typename OFF>
-__device__ __forceinline__ float2
-ld2(const float2 *base, OFF off)
-{
-    return *(const float2 *)((const char *)base + off);
-}
-
-template <typename OFF>
-__device__ __forceinline__ void
-st2(float2 *base, OFF off, float a, float b)
-{
-    *(float2 *)((char *)base + off) = make_float2(a, b);
-}
-
-template <typename ID, typename OFF>
-__device__ __forceinline__ unsigned
-ldid(const void *base, OFF off)
-{
-    return *(const ID *)((const char *)base + off);
-}
 
 /* ------------------------------------------------------------------ */
 /* H update                                                             */
@@ -387,7 +296,7 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     stsn(ad, la, 2048, dpz);
 }
 
-template <typename ID, int ADE, typename OFF>
+template <typename ID, int ADE, typename OFF, bool PC>
 __global__ __launch_bounds__(512) void
 k_estep(GsvGrid g, int kchunk)
 {
@@ -425,7 +334,7 @@ k_estep(GsvGrid g, int kchunk)
         const float hzi = ldf(g.hzg, o - 4), hyi = ldf(g.hyg, o - 4);
         const float rkz = g.rkez[k];
         GsvECoef ec = g.etab[ldid<ID>(g.ecls, oc)];
-        if (ec.flags & GSV_E_PERCELL)
+        if (PC && (ec.flags & GSV_E_PERCELL))
             ec = g.pctab[(i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))];
         float ex = ldn(g.exg, o), ey = ldn(g.eyg, o), ez = ldn(g.ezg, o);
         const unsigned fl = ec.flags;
@@ -1050,16 +959,18 @@ gsv_launch_hstep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipS
     return hipGetLastError();
 }
 
-template <typename ID, typename OFF>
+/* the per-cell coefficient branch (design region) is compiled in only for
+ * pools that have one, so the production kernels carry none of its registers */
+template <typename ID, typename OFF, bool PC>
 static void
 launch_estep(const GsvGrid *g, int ade, dim3 grd, dim3 blk, int kchunk, hipStream_t st)
 {
     if (ade == 0)
-        hipLaunchKernelGGL((k_estep<ID, 0, OFF>), grd, blk, 0, st, *g, kchunk);
+        hipLaunchKernelGGL((k_estep<ID, 0, OFF, PC>), grd, blk, 0, st, *g, kchunk);
     else if (ade == 1)
-        hipLaunchKernelGGL((k_estep<ID, 1, OFF>), grd, blk, 0, st, *g, kchunk);
+        hipLaunchKernelGGL((k_estep<ID, 1, OFF, PC>), grd, blk, 0, st, *g, kchunk);
     else
-        hipLaunchKernelGGL((k_estep<ID, 2, OFF>), grd, blk, 0, st, *g, kchunk);
+        hipLaunchKernelGGL((k_estep<ID, 2, OFF, PC>), grd, blk, 0, st, *g, kchunk);
 }
 
 extern "C" hipError_t
@@ -1069,13 +980,17 @@ gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipS
     /* the dispersive branch is compiled in only when the pool has ADE cells,
      * and the critical-point terms only when some material has them */
     const int ade = g->ade ? (g->ade_poles ? 2 : 1) : 0;
-    const bool o32 = offsets32(g);
+    const bool o32 = offsets32(g), pc = g->pc_nx > 0;
+    if (pc && !o32)
+        return hipErrorInvalidValue;   /* design regions need 32-bit offsets (field arrays below 4 GiB) */
     if (idbytes == 1) {
-        if (o32) launch_estep<uint8_t, uint32_t>(g, ade, grd, blk, kchunk, st);
-        else launch_estep<uint8_t, uint64_t>(g, ade, grd, blk, kchunk, st);
+        if (pc) launch_estep<uint8_t, uint32_t, true>(g, ade, grd, blk, kchunk, st);
+        else if (o32) launch_estep<uint8_t, uint32_t, false>(g, ade, grd, blk, kchunk, st);
+        else launch_estep<uint8_t, uint64_t, false>(g, ade, grd, blk, kchunk, st);
     } else {
-        if (o32) launch_estep<uint16_t, uint32_t>(g, ade, grd, blk, kchunk, st);
-        else launch_estep<uint16_t, uint64_t>(g, ade, grd, blk, kchunk, st);
+        if (pc) launch_estep<uint16_t, uint32_t, true>(g, ade, grd, blk, kchunk, st);
+        else if (o32) launch_estep<uint16_t, uint32_t, false>(g, ade, grd, blk, kchunk, st);
+        else launch_estep<uint16_t, uint64_t, false>(g, ade, grd, blk, kchunk, st);
     }
     return hipGetLastError();
 }
