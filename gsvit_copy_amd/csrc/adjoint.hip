@@ -12,15 +12,16 @@
  * Scope: non-dispersive media, CPML or plain boundaries, hard E point
  * sources.  One forward step is x^{n+1} = S_n M_E M_H x^n + s_n for the state
  * x = (E, H, psi_E, psi_H); the backward pass applies the transposed steps in
- * reverse order (M_H^T M_E^T S_n^T), fused into two kernels per step:
- *   k_adj_e  (E cells)  adds the H-step transpose of step n+1 (curl^T of r,
- *            psi_H gather), the objective's source, the point-source
+ * reverse order (M_H^T M_E^T S_n^T), in two bulk kernels per step (with the
+ * CPML psi terms in four slab kernels around them, see k_slab_*):
+ *   k_adj_e  (E cells)  adds the H-step transpose of step n+1 (curl^T of r),
+ *            the objective's source, the point-source
  *            projection, accumulates the coefficient gradient of the design
  *            cells and applies the local part of the E-step transpose
- *            (q = cb m, new adjoint E = ca m, psi_E adjoint);
- *   k_adj_h  (H cells)  adds the E-step transpose (curl^T of q, psi_E
- *            gather) and applies the local part of the H-step transpose
- *            (r = db h, new adjoint H = da h, psi_H adjoint).
+ *            (q = cb m, new adjoint E = ca m);
+ *   k_adj_h  (H cells)  adds the E-step transpose (curl^T of q) and applies
+ *            the local part of the H-step transpose (r = db h, new adjoint
+ *            H = da h).
  * The adjoint of psi is stored without its decay factor b, which the next
  * (earlier) step applies when it reads it.
  *
@@ -165,20 +166,7 @@ k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int kchunk)
 
     const float rhx = g.rkhx[i], rhy = g.rkhy[j];
     const float rhx1 = i > 0 ? g.rkhx[i - 1] : 0.f, rhy1 = j > 0 ? g.rkhy[j - 1] : 0.f;
-    const bool iin = i >= 1 && i < nx - 1, jin = j >= 1 && j < ny - 1;
-    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
-    /* H-side psi cells whose E neighbours include this column (x and y faces) */
-    const bool hx0a = F0.on && jin && i < F0.depth - 1, hx0b = F0.on && jin && i >= 1 && i - 1 < F0.depth - 1;
-    const int lo1 = nx - F1.depth + 1;
-    const bool hxna = F1.on && jin && i >= lo1 && i < nx - 1, hxnb = F1.on && jin && i - 1 >= lo1 && i - 1 < nx - 1;
-    const bool hy0a = F2.on && iin && j < F2.depth - 1, hy0b = F2.on && iin && j >= 1 && j - 1 < F2.depth - 1;
-    const int lo3 = ny - F3.depth + 1;
-    const bool hyna = F3.on && iin && j >= lo3 && j < ny - 1, hynb = F3.on && iin && j - 1 >= lo3 && j - 1 < ny - 1;
-    const bool ijin = iin && jin;
-    /* E-side psi cells of this column */
-    const bool ex0 = F0.on && i >= 1 && i < F0.depth, exn = F1.on && i >= nx - F1.depth + 1;
-    const bool ey0 = F2.on && j >= 1 && j < F2.depth, eyn = F3.on && j >= ny - F3.depth + 1;
-    const bool zplane_xy = (F0.on && i == 0) || (F2.on && j == 0);
+    const bool zplane_xy = (g.f[0].on && i == 0) || (g.f[2].on && j == 0), zplane_z = g.f[4].on;
     const bool in_mon_ij = i >= a.mi0 && i < a.mi0 + a.mnx && j >= a.mj0 && j < a.mj0 + a.mny;
     const bool in_pc_ij = a.ncoef && i >= g.pc_i0 && i < g.pc_i0 + g.pc_nx && j >= g.pc_j0 && j < g.pc_j0 + g.pc_ny;
 
@@ -202,92 +190,6 @@ k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int kchunk)
         }
         rxm = rxc;
         rym = ryc;
-        /* transpose of the H-side psi updates of step n+1 */
-        const bool kin = k >= 1 && k < nz - 1;
-        if (kin) {
-            if (hx0a) {
-                const float cc = F0.ch[i].y;
-                const float2 t = ld2(a.th[0], (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8);
-                ey -= cc * t.x;
-                ez -= cc * t.y;
-            }
-            if (hx0b) {
-                const float cc = F0.ch[i - 1].y;
-                const float2 t = ld2(a.th[0], (OFF)(((unsigned)k * ny + j) * F0.depth + i - 1) * 8);
-                ey += cc * t.x;
-                ez += cc * t.y;
-            }
-            if (hxna) {
-                const int ir = i - (nx - F1.depth);
-                const float cc = F1.ch[ir].y;
-                const float2 t = ld2(a.th[1], (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8);
-                ey -= cc * t.x;
-                ez -= cc * t.y;
-            }
-            if (hxnb) {
-                const int ir = i - 1 - (nx - F1.depth);
-                const float cc = F1.ch[ir].y;
-                const float2 t = ld2(a.th[1], (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8);
-                ey += cc * t.x;
-                ez += cc * t.y;
-            }
-            if (hy0a) {
-                const float cc = F2.ch[j].y;
-                const float2 t = ld2(a.th[2], (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8);
-                ez -= cc * t.x;
-                ex -= cc * t.y;
-            }
-            if (hy0b) {
-                const float cc = F2.ch[j - 1].y;
-                const float2 t = ld2(a.th[2], (OFF)(((unsigned)k * F2.depth + j - 1) * px + i) * 8);
-                ez += cc * t.x;
-                ex += cc * t.y;
-            }
-            if (hyna) {
-                const int jr = j - (ny - F3.depth);
-                const float cc = F3.ch[jr].y;
-                const float2 t = ld2(a.th[3], (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8);
-                ez -= cc * t.x;
-                ex -= cc * t.y;
-            }
-            if (hynb) {
-                const int jr = j - 1 - (ny - F3.depth);
-                const float cc = F3.ch[jr].y;
-                const float2 t = ld2(a.th[3], (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8);
-                ez += cc * t.x;
-                ex += cc * t.y;
-            }
-        }
-        if (ijin) {
-            if (F4.on && k < F4.depth - 1) {
-                const float cc = F4.ch[k].y;
-                const float2 t = ld2(a.th[4], (OFF)(((unsigned)k * ny + j) * px + i) * 8);
-                ex -= cc * t.x;
-                ey -= cc * t.y;
-            }
-            if (F4.on && k >= 1 && k - 1 < F4.depth - 1) {
-                const float cc = F4.ch[k - 1].y;
-                const float2 t = ld2(a.th[4], (OFF)(((unsigned)(k - 1) * ny + j) * px + i) * 8);
-                ex += cc * t.x;
-                ey += cc * t.y;
-            }
-            const int lo5 = nz - F5.depth + 1;
-            if (F5.on && k >= lo5 && k < nz - 1) {
-                const int kr = k - (nz - F5.depth);
-                const float cc = F5.ch[kr].y;
-                const float2 t = ld2(a.th[5], (OFF)(((unsigned)kr * ny + j) * px + i) * 8);
-                ex -= cc * t.x;
-                ey -= cc * t.y;
-            }
-            if (F5.on && k - 1 >= lo5 && k - 1 < nz - 1) {
-                const int kr = k - 1 - (nz - F5.depth);
-                const float cc = F5.ch[kr].y;
-                const float2 t = ld2(a.th[5], (OFF)(((unsigned)kr * ny + j) * px + i) * 8);
-                ex += cc * t.x;
-                ey += cc * t.y;
-            }
-        }
-
         /* objective source: dJ/dE^{n+1} = 2 (Re F cos - Im F sin) */
         if (in_mon_ij && k >= a.mk0 && k < a.mk0 + a.mnz) {
             const long long q = (i - a.mi0) + (long long)a.mnx * ((j - a.mj0) + (long long)a.mny * (k - a.mk0));
@@ -311,7 +213,7 @@ k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int kchunk)
         }
 
         /* E-step transpose: plane zeroing, then the update and the psi_E corrections */
-        if (zplane_xy || (F4.on && k == 0)) {
+        if (zplane_xy || (zplane_z && k == 0)) {
             ex = 0.f;
             ey = 0.f;
             ez = 0.f;
@@ -330,76 +232,6 @@ k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int kchunk)
             a.gcb[2 * a.ncoef + q] += (double)ez * h[5 * a.ncoef];
         }
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
-        /* psi_E adjoints: decay of the later step, plus the corrections of this one */
-        if (ex0) {
-            const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
-            const float b = F0.ce[i].x;
-            const float2 t = ld2(a.te[0], p);
-            float t1 = b * t.x, t2 = b * t.y;
-            if (j != 0 && k != 0) {
-                if (ty) t1 -= g.dx * ec.cb[1] * ey;
-                if (tz) t2 += g.dx * ec.cb[2] * ez;
-            }
-            st2(a.te[0], p, t1, t2);
-        }
-        if (ey0) {
-            const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
-            const float b = F2.ce[j].x;
-            const float2 t = ld2(a.te[2], p);
-            float t1 = b * t.x, t2 = b * t.y;
-            if (i != 0 && k != 0) {
-                if (tx) t1 += g.dy * ec.cb[0] * ex;
-                if (tz) t2 -= g.dy * ec.cb[2] * ez;
-            }
-            st2(a.te[2], p, t1, t2);
-        }
-        if (F4.on && k >= 1 && k < F4.depth) {
-            const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
-            const float b = F4.ce[k].x;
-            const float2 t = ld2(a.te[4], p);
-            float t1 = b * t.x, t2 = b * t.y;
-            if (i != 0 && j != 0) {
-                if (ty) t1 += g.dx * ec.cb[1] * ey;
-                if (tz) t2 -= g.dx * ec.cb[0] * ex;
-            }
-            st2(a.te[4], p, t1, t2);
-        }
-        if (exn) {
-            const int ir = i - (nx - F1.depth);
-            const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8;
-            const float b = F1.ce[ir].x;
-            const float2 t = ld2(a.te[1], p);
-            float t1 = b * t.x, t2 = b * t.y;
-            if (j != 0 && k != 0) {
-                if (ty) t1 -= g.dx * ec.cb[1] * ey;
-                if (tz) t2 += g.dx * ec.cb[2] * ez;
-            }
-            st2(a.te[1], p, t1, t2);
-        }
-        if (eyn) {
-            const int jr = j - (ny - F3.depth);
-            const OFF p = (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8;
-            const float b = F3.ce[jr].x;
-            const float2 t = ld2(a.te[3], p);
-            float t1 = b * t.x, t2 = b * t.y;
-            if (i != 0 && k != 0) {
-                if (tx) t1 += g.dy * ec.cb[0] * ex;
-                if (tz) t2 -= g.dy * ec.cb[2] * ez;
-            }
-            st2(a.te[3], p, t1, t2);
-        }
-        if (F5.on && k >= nz - F5.depth + 1) {
-            const int kr = k - (nz - F5.depth);
-            const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
-            const float b = F5.ce[kr].x;
-            const float2 t = ld2(a.te[5], p);
-            float t1 = b * t.x, t2 = b * t.y;
-            if (i != 0 && j != 0) {
-                if (ty) t1 += g.dx * ec.cb[1] * ey;
-                if (tx) t2 -= g.dx * ec.cb[0] * ex;
-            }
-            st2(a.te[5], p, t1, t2);
-        }
         /* PEC zeroing acts on the curl update only (the corrections come after it) */
         const float mx = (fl & GSV_E_PECX) ? 0.f : ex, my = (fl & GSV_E_PECY) ? 0.f : ey;
         const float mz = (fl & GSV_E_PECZ) ? 0.f : ez;
@@ -434,18 +266,8 @@ k_adj_h(GsvGrid g, GsvAdj a, int kchunk)
     const float rex = g.rkex[i], rey = g.rkey[j];
     const float rex1 = i + 1 < nx ? g.rkex[i + 1] : 0.f, rey1 = j + 1 < ny ? g.rkey[j + 1] : 0.f;
     const bool ip = i + 1 < nx, jp = j + 1 < ny;
-    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
-    /* E-side psi cells whose H neighbours include this column */
-    const bool ex0a = F0.on && i >= 1 && i < F0.depth, ex0b = F0.on && i + 1 < F0.depth;
-    const int lo1 = nx - F1.depth + 1;
-    const bool exna = F1.on && i >= lo1, exnb = F1.on && i + 1 >= lo1 && i + 1 < nx;
-    const bool ey0a = F2.on && j >= 1 && j < F2.depth, ey0b = F2.on && j + 1 < F2.depth;
-    const int lo3 = ny - F3.depth + 1;
-    const bool eyna = F3.on && j >= lo3, eynb = F3.on && j + 1 >= lo3 && j + 1 < ny;
     const bool iin = (i >= 1 && i < nx - 1), jin = (j >= 1 && j < ny - 1), ijin = iin && jin;
-    const bool zx = F1.on && i == nx - 1 && jin, zy = F3.on && j == ny - 1 && iin;
-    const bool hx0 = F0.on && i < F0.depth - 1 && jin, hxn = F1.on && i >= nx - F1.depth + 1 && i < nx - 1 && jin;
-    const bool hy0 = F2.on && j < F2.depth - 1 && iin, hyn = F3.on && j >= ny - F3.depth + 1 && j < ny - 1 && iin;
+    const bool zx = g.f[1].on && i == nx - 1 && jin, zy = g.f[3].on && j == ny - 1 && iin, zzn = g.f[5].on && ijin;
 
     /* q of plane k (carried), the plane above is loaded each step (guard plane above nz-1 is zero) */
     float qxk = ldf(a.qx, o), qyk = ldf(a.qy, o);
@@ -468,142 +290,14 @@ k_adj_h(GsvGrid g, GsvAdj a, int kchunk)
         }
         qxk = qxk1;
         qyk = qyk1;
-        /* transpose of the E-side psi updates */
-        if (ex0a) {
-            const float cc = F0.ce[i].y;
-            const float2 t = ld2(a.te[0], (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8);
-            hz += cc * t.x;
-            hy += cc * t.y;
-        }
-        if (ex0b) {
-            const float cc = F0.ce[i + 1].y;
-            const float2 t = ld2(a.te[0], (OFF)(((unsigned)k * ny + j) * F0.depth + i + 1) * 8);
-            hz -= cc * t.x;
-            hy -= cc * t.y;
-        }
-        if (exna) {
-            const int ir = i - (nx - F1.depth);
-            const float cc = F1.ce[ir].y;
-            const float2 t = ld2(a.te[1], (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8);
-            hz += cc * t.x;
-            hy += cc * t.y;
-        }
-        if (exnb) {
-            const int ir = i + 1 - (nx - F1.depth);
-            const float cc = F1.ce[ir].y;
-            const float2 t = ld2(a.te[1], (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8);
-            hz -= cc * t.x;
-            hy -= cc * t.y;
-        }
-        if (ey0a) {
-            const float cc = F2.ce[j].y;
-            const float2 t = ld2(a.te[2], (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8);
-            hz += cc * t.x;
-            hx += cc * t.y;
-        }
-        if (ey0b) {
-            const float cc = F2.ce[j + 1].y;
-            const float2 t = ld2(a.te[2], (OFF)(((unsigned)k * F2.depth + j + 1) * px + i) * 8);
-            hz -= cc * t.x;
-            hx -= cc * t.y;
-        }
-        if (eyna) {
-            const int jr = j - (ny - F3.depth);
-            const float cc = F3.ce[jr].y;
-            const float2 t = ld2(a.te[3], (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8);
-            hz += cc * t.x;
-            hx += cc * t.y;
-        }
-        if (eynb) {
-            const int jr = j + 1 - (ny - F3.depth);
-            const float cc = F3.ce[jr].y;
-            const float2 t = ld2(a.te[3], (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8);
-            hz -= cc * t.x;
-            hx -= cc * t.y;
-        }
-        if (F4.on) {
-            if (k >= 1 && k < F4.depth) {
-                const float cc = F4.ce[k].y;
-                const float2 t = ld2(a.te[4], (OFF)(((unsigned)k * ny + j) * px + i) * 8);
-                hx += cc * t.x;
-                hy += cc * t.y;
-            }
-            if (k + 1 < F4.depth) {
-                const float cc = F4.ce[k + 1].y;
-                const float2 t = ld2(a.te[4], (OFF)(((unsigned)(k + 1) * ny + j) * px + i) * 8);
-                hx -= cc * t.x;
-                hy -= cc * t.y;
-            }
-        }
-        if (F5.on) {
-            const int lo5 = nz - F5.depth + 1;
-            if (k >= lo5) {
-                const int kr = k - (nz - F5.depth);
-                const float cc = F5.ce[kr].y;
-                const float2 t = ld2(a.te[5], (OFF)(((unsigned)kr * ny + j) * px + i) * 8);
-                hx += cc * t.x;
-                hy += cc * t.y;
-            }
-            if (k + 1 >= lo5 && k + 1 < nz) {
-                const int kr = k + 1 - (nz - F5.depth);
-                const float cc = F5.ce[kr].y;
-                const float2 t = ld2(a.te[5], (OFF)(((unsigned)kr * ny + j) * px + i) * 8);
-                hx -= cc * t.x;
-                hy -= cc * t.y;
-            }
-        }
-
         /* H-step transpose: zeroing, then the update and the psi_H corrections */
         const bool kin = (k >= 1 && k < nz - 1);
-        if ((kin && (zx || zy)) || (F5.on && k == nz - 1 && ijin)) {
+        if ((kin && (zx || zy)) || (zzn && k == nz - 1)) {
             hx = 0.f;
             hy = 0.f;
             hz = 0.f;
         }
         const GsvHCoef hc = g.htab[ldid<ID>(g.hcls, oc)];
-        if (kin) {
-            if (hx0) {
-                const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
-                const float b = F0.ch[i].x;
-                const float2 t = ld2(a.th[0], p);
-                st2(a.th[0], p, b * t.x - g.dx * hc.dbc * hz, b * t.y + g.dx * hc.dbc * hy);
-            }
-            if (hxn) {
-                const int ir = i - (nx - F1.depth);
-                const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8;
-                const float b = F1.ch[ir].x;
-                const float2 t = ld2(a.th[1], p);
-                st2(a.th[1], p, b * t.x - g.dx * hc.dbc * hz, b * t.y + g.dx * hc.dbc * hy);
-            }
-            if (hy0) {
-                const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
-                const float b = F2.ch[j].x;
-                const float2 t = ld2(a.th[2], p);
-                st2(a.th[2], p, b * t.x - g.dy * hc.dbc * hx, b * t.y + g.dy * hc.dbc * hz);
-            }
-            if (hyn) {
-                const int jr = j - (ny - F3.depth);
-                const OFF p = (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8;
-                const float b = F3.ch[jr].x;
-                const float2 t = ld2(a.th[3], p);
-                st2(a.th[3], p, b * t.x - g.dy * hc.dbc * hx, b * t.y + g.dy * hc.dbc * hz);
-            }
-        }
-        if (ijin) {
-            if (F4.on && k < F4.depth - 1) {
-                const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
-                const float b = F4.ch[k].x;
-                const float2 t = ld2(a.th[4], p);
-                st2(a.th[4], p, b * t.x - g.dz * hc.dbc * hy, b * t.y + g.dz * hc.dbc * hx);
-            }
-            if (F5.on && k >= nz - F5.depth + 1 && k < nz - 1) {
-                const int kr = k - (nz - F5.depth);
-                const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
-                const float b = F5.ch[kr].x;
-                const float2 t = ld2(a.th[5], p);
-                st2(a.th[5], p, b * t.x - g.dz * hc.dbc * hy, b * t.y + g.dz * hc.dbc * hx);
-            }
-        }
         const bool ux = !(j == ny - 1 || k == nz - 1), uy = !(i == nx - 1 || k == nz - 1);
         const bool uz = !(i == nx - 1 || j == ny - 1);
         stf(a.rx, o, ux ? hc.db[0] * hx : 0.f);
@@ -613,6 +307,247 @@ k_adj_h(GsvGrid g, GsvAdj a, int kchunk)
         stn(a.hby, o, uy ? hc.da * hy : hy);
         stn(a.hbz, o, uz ? hc.da * hz : hz);
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* CPML slabs of the backward step                                      */
+/* ------------------------------------------------------------------ */
+
+/* The psi terms of the transposed step run in four slab kernels around the
+ * two bulk kernels above, so that the bulk kernels carry no CPML state:
+ *   k_slab_e_pre   adds the psi_H transpose of step n+1 to the adjoint E;
+ *   k_slab_e_post  psi_E adjoint of step n from the new adjoint E (which holds
+ *                  ca m, or m where the update does not apply: m = E'' without PEC);
+ *   k_slab_h_pre   adds the psi_E transpose to the adjoint H;
+ *   k_slab_h_post  psi_H adjoint from the new adjoint H (da h'' or h'').
+ * Per face axis: psi pair 1 / 2 correct E components ec and are fed by H
+ * components es (E side); they correct H components hc and are fed by E
+ * components hs (H side); 0, 1, 2 = x, y, z.  The pre kernels run once per
+ * axis (near and far faces of one axis never share a cell). */
+
+__device__ __forceinline__ float *
+fcomp(float *x, float *y, float *z, int c)
+{
+    return c == 0 ? x : (c == 1 ? y : z);
+}
+
+/* cell of thread q in the windows [0, dn) and [n - df, n) along AXIS */
+template <int AXIS>
+__device__ __forceinline__ bool
+window_cell(const GsvGrid &g, long long q, int dn, int df, int &i, int &j, int &k, int &pos)
+{
+    const int w = dn + df;
+    int t;
+    if (AXIS == 0) {
+        t = (int)(q % w);
+        const long long rest = q / w;
+        j = (int)(rest % g.ny);
+        k = (int)(rest / g.ny);
+        if (k >= g.nz) return false;
+    } else if (AXIS == 1) {
+        i = (int)(q % g.nx);
+        const long long rest = q / g.nx;
+        t = (int)(rest % w);
+        k = (int)(rest / w);
+        if (k >= g.nz) return false;
+    } else {
+        i = (int)(q % g.nx);
+        const long long rest = q / g.nx;
+        j = (int)(rest % g.ny);
+        t = (int)(rest / g.ny);
+        if (t >= w) return false;
+    }
+    const int n = AXIS == 0 ? g.nx : (AXIS == 1 ? g.ny : g.nz);
+    pos = t < dn ? t : n - df + (t - dn);
+    if (AXIS == 0) i = pos; else if (AXIS == 1) j = pos; else k = pos;
+    return true;
+}
+
+/* psi pair index of slab cell r of a face along AXIS: x [k][j][r], y [k][r][i], z [r][j][i] */
+template <int AXIS>
+__device__ __forceinline__ long long
+psi_idx(const GsvGrid &g, int depth, int r, int i, int j, int k)
+{
+    if (AXIS == 0) return ((long long)k * g.ny + j) * depth + r;
+    if (AXIS == 1) return ((long long)k * depth + r) * g.px + i;
+    return ((long long)r * g.ny + j) * g.px + i;
+}
+
+template <int AXIS>
+__global__ __launch_bounds__(256) void
+k_slab_e_pre(GsvGrid g, GsvAdj a)
+{
+    const GsvCpmlFace &Fn = g.f[2 * AXIS], &Ff = g.f[2 * AXIS + 1];
+    const int dn = Fn.on ? Fn.depth : 0, df = Ff.on ? Ff.depth : 0;
+    int i = 0, j = 0, k = 0, pos;
+    if (!window_cell<AXIS>(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, dn, df, i, j, k, pos))
+        return;
+    const bool iin = i >= 1 && i < g.nx - 1, jin = j >= 1 && j < g.ny - 1, kin = k >= 1 && k < g.nz - 1;
+    if (!(AXIS == 0 ? (jin && kin) : (AXIS == 1 ? (iin && kin) : (iin && jin))))
+        return;
+    const int n = AXIS == 0 ? g.nx : (AXIS == 1 ? g.ny : g.nz);
+    /* E components feeding psi_H 1 / 2 */
+    const int c1 = AXIS == 0 ? 1 : (AXIS == 1 ? 2 : 0), c2 = AXIS == 0 ? 2 : (AXIS == 1 ? 0 : 1);
+    float v1 = 0.f, v2 = 0.f;
+    for (int d = 0; d < 2; d++) {
+        const int pp = pos - d;   /* psi_H cell at pos (sign -) and pos - 1 (sign +) */
+        const float sg = d ? 1.f : -1.f;
+        if (Fn.on && pp >= 0 && pp < Fn.depth - 1) {
+            const float cc = sg * Fn.ch[pp].y;
+            const float2 t = a.th[2 * AXIS][psi_idx<AXIS>(g, Fn.depth, pp, i, j, k)];
+            v1 += cc * t.x;
+            v2 += cc * t.y;
+        }
+        if (Ff.on && pp >= n - Ff.depth + 1 && pp < n - 1) {
+            const int r = pp - (n - Ff.depth);
+            const float cc = sg * Ff.ch[r].y;
+            const float2 t = a.th[2 * AXIS + 1][psi_idx<AXIS>(g, Ff.depth, r, i, j, k)];
+            v1 += cc * t.x;
+            v2 += cc * t.y;
+        }
+    }
+    const long long c = i + (long long)g.px * j + g.plane * k;
+    float *e1 = fcomp(a.ebx, a.eby, a.ebz, c1) + g.plane, *e2 = fcomp(a.ebx, a.eby, a.ebz, c2) + g.plane;
+    e1[c] += v1;
+    e2[c] += v2;
+}
+
+template <int AXIS>
+__global__ __launch_bounds__(256) void
+k_slab_h_pre(GsvGrid g, GsvAdj a)
+{
+    const GsvCpmlFace &Fn = g.f[2 * AXIS], &Ff = g.f[2 * AXIS + 1];
+    const int dn = Fn.on ? Fn.depth : 0, df = Ff.on ? Ff.depth : 0;
+    int i = 0, j = 0, k = 0, pos;
+    if (!window_cell<AXIS>(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, dn, df, i, j, k, pos))
+        return;
+    const int n = AXIS == 0 ? g.nx : (AXIS == 1 ? g.ny : g.nz);
+    /* H components feeding psi_E 1 / 2 */
+    const int c1 = AXIS == 0 ? 2 : (AXIS == 1 ? 2 : 0), c2 = AXIS == 0 ? 1 : (AXIS == 1 ? 0 : 1);
+    float v1 = 0.f, v2 = 0.f;
+    for (int d = 0; d < 2; d++) {
+        const int pp = pos + d;   /* psi_E cell at pos (sign +) and pos + 1 (sign -) */
+        const float sg = d ? -1.f : 1.f;
+        if (Fn.on && pp >= 1 && pp < Fn.depth) {
+            const float cc = sg * Fn.ce[pp].y;
+            const float2 t = a.te[2 * AXIS][psi_idx<AXIS>(g, Fn.depth, pp, i, j, k)];
+            v1 += cc * t.x;
+            v2 += cc * t.y;
+        }
+        if (Ff.on && pp >= n - Ff.depth + 1 && pp < n) {
+            const int r = pp - (n - Ff.depth);
+            const float cc = sg * Ff.ce[r].y;
+            const float2 t = a.te[2 * AXIS + 1][psi_idx<AXIS>(g, Ff.depth, r, i, j, k)];
+            v1 += cc * t.x;
+            v2 += cc * t.y;
+        }
+    }
+    const long long c = i + (long long)g.px * j + g.plane * k;
+    float *h1 = fcomp(a.hbx, a.hby, a.hbz, c1) + g.plane, *h2 = fcomp(a.hbx, a.hby, a.hbz, c2) + g.plane;
+    h1[c] += v1;
+    h2[c] += v2;
+}
+
+/* slab cell (r, i, j, k) of face f from a flat psi index q; false for padding */
+__device__ __forceinline__ bool
+face_cell(const GsvGrid &g, int f, long long q, int &r, int &i, int &j, int &k)
+{
+    const int axis = f / 2, depth = g.f[f].depth;
+    if (axis == 0) {
+        r = (int)(q % depth);
+        j = (int)((q / depth) % g.ny);
+        k = (int)(q / ((long long)depth * g.ny));
+        if (k >= g.nz) return false;
+        i = (f & 1) ? g.nx - depth + r : r;
+    } else if (axis == 1) {
+        i = (int)(q % g.px);
+        r = (int)((q / g.px) % depth);
+        k = (int)(q / ((long long)g.px * depth));
+        if (i >= g.nx || k >= g.nz) return false;
+        j = (f & 1) ? g.ny - depth + r : r;
+    } else {
+        i = (int)(q % g.px);
+        j = (int)((q / g.px) % g.ny);
+        r = (int)(q / ((long long)g.px * g.ny));
+        if (i >= g.nx || r >= depth) return false;
+        k = (f & 1) ? g.nz - depth + r : r;
+    }
+    return true;
+}
+
+/* psi_E adjoint of the step: T = b T(later) + P^T E'' at every E-side slab cell */
+template <typename ID>
+__global__ __launch_bounds__(256) void
+k_slab_e_post(GsvGrid g, GsvAdj a)
+{
+    const int f = blockIdx.y;
+    const GsvCpmlFace &F = g.f[f];
+    int r, i, j, k;
+    if (!F.on || !face_cell(g, f, (long long)blockIdx.x * blockDim.x + threadIdx.x, r, i, j, k) || r < 1)
+        return;
+    const int axis = f / 2;
+    const long long c = i + (long long)g.px * j + g.plane * k;
+    GsvECoef ec = g.etab[((const ID *)g.ecls)[c]];
+    if (ec.flags & GSV_E_PERCELL)
+        ec = g.pctab[(i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))];
+    const unsigned fl = ec.flags;
+    /* E'' of the components this face corrects (m = E'' here: no PEC in gradient mode) */
+    const int c1 = axis == 1 ? 0 : 1, c2 = axis == 0 ? 2 : (axis == 1 ? 2 : 0);
+    const bool u[3] = { (fl & GSV_E_TBCX) && j > 0 && k > 0, (fl & GSV_E_TBCY) && i > 0 && k > 0,
+                        (fl & GSV_E_TBCZ) && i > 0 && j > 0 };
+    const float e1 = fcomp(a.ebx, a.eby, a.ebz, c1)[g.plane + c], e2 = fcomp(a.ebx, a.eby, a.ebz, c2)[g.plane + c];
+    const float m1 = u[c1] ? e1 / ec.ca[c1] : e1, m2 = u[c2] ? e2 / ec.ca[c2] : e2;
+    /* the forward corrections: x faces -dx cb_y, +dx cb_z; y faces +dy cb_x, -dy cb_z;
+     * z faces +dx cb_y, -dx cb_x, gated on the flags of src3d/boundary.c:691-887 */
+    const bool nz2 = axis == 0 ? (j != 0 && k != 0) : (axis == 1 ? (i != 0 && k != 0) : (i != 0 && j != 0));
+    const unsigned g1 = axis == 1 ? GSV_E_TBCX : GSV_E_TBCY;
+    const unsigned g2 = axis == 2 ? (f == 4 ? GSV_E_TBCZ : GSV_E_TBCX) : GSV_E_TBCZ;
+    const float d = axis == 1 ? g.dy : g.dx;
+    const float s1 = axis == 0 ? -d : d, s2 = axis == 0 ? d : -d;
+    const long long p = axis == 0 ? psi_idx<0>(g, F.depth, r, i, j, k)
+                                  : (axis == 1 ? psi_idx<1>(g, F.depth, r, i, j, k) : psi_idx<2>(g, F.depth, r, i, j, k));
+    const float b = F.ce[r].x;
+    float2 t = a.te[f][p];
+    t.x *= b;
+    t.y *= b;
+    if (nz2) {
+        if (fl & g1) t.x += s1 * ec.cb[c1] * m1;
+        if (fl & g2) t.y += s2 * ec.cb[c2] * m2;
+    }
+    a.te[f][p] = t;
+}
+
+/* psi_H adjoint of the step: T = b T(later) + P_h^T H'' at every H-side slab cell */
+template <typename ID>
+__global__ __launch_bounds__(256) void
+k_slab_h_post(GsvGrid g, GsvAdj a)
+{
+    const int f = blockIdx.y;
+    const GsvCpmlFace &F = g.f[f];
+    int r, i, j, k;
+    if (!F.on || !face_cell(g, f, (long long)blockIdx.x * blockDim.x + threadIdx.x, r, i, j, k))
+        return;
+    const int axis = f / 2;
+    const int n = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+    const int pos = axis == 0 ? i : (axis == 1 ? j : k);
+    const bool inside = (f & 1) ? (pos >= n - F.depth + 1 && pos < n - 1) : (pos < F.depth - 1);
+    const bool iin = i >= 1 && i < g.nx - 1, jin = j >= 1 && j < g.ny - 1, kin = k >= 1 && k < g.nz - 1;
+    if (!inside || !(axis == 0 ? (jin && kin) : (axis == 1 ? (iin && kin) : (iin && jin))))
+        return;
+    const long long c = i + (long long)g.px * j + g.plane * k;
+    const GsvHCoef hc = g.htab[((const ID *)g.hcls)[c]];
+    /* H'' of the components this face corrects: x faces z, y; y faces x, z; z faces y, x */
+    const int c1 = axis == 0 ? 2 : (axis == 1 ? 0 : 1), c2 = axis == 0 ? 1 : (axis == 1 ? 2 : 0);
+    const bool u[3] = { !(j == g.ny - 1 || k == g.nz - 1), !(i == g.nx - 1 || k == g.nz - 1),
+                        !(i == g.nx - 1 || j == g.ny - 1) };
+    const float h1 = fcomp(a.hbx, a.hby, a.hbz, c1)[g.plane + c], h2 = fcomp(a.hbx, a.hby, a.hbz, c2)[g.plane + c];
+    const float m1 = u[c1] ? h1 / hc.da : h1, m2 = u[c2] ? h2 / hc.da : h2;
+    const float d = axis == 0 ? g.dx : (axis == 1 ? g.dy : g.dz);
+    const long long p = axis == 0 ? psi_idx<0>(g, F.depth, r, i, j, k)
+                                  : (axis == 1 ? psi_idx<1>(g, F.depth, r, i, j, k) : psi_idx<2>(g, F.depth, r, i, j, k));
+    const float b = F.ch[r].x;
+    float2 t = a.th[f][p];
+    a.th[f][p] = make_float2(b * t.x - d * hc.dbc * m1, b * t.y + d * hc.dbc * m2);
 }
 
 /* ------------------------------------------------------------------ */
@@ -678,6 +613,26 @@ gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_int
                     fprintf(stderr, "Error: the gradient supports E point sources only\n");
                     return 1;
                 }
+    /* the slab kernels recover E'' = (adjoint E) / ca and H'' = (adjoint H) / da */
+    for (const GsvECoef &c : E->etab)
+        if (!(c.flags & GSV_E_PERCELL)
+            && ((c.flags & (GSV_E_PECX | GSV_E_PECY | GSV_E_PECZ)) || c.ca[0] == 0.f || c.ca[1] == 0.f || c.ca[2] == 0.f)) {
+            fprintf(stderr, "Error: the gradient supports no PEC cells and needs nonzero E update factors\n");
+            return 1;
+        }
+    for (const GsvHCoef &c : E->htab)
+        if (c.da == 0.f) {
+            fprintf(stderr, "Error: the gradient needs nonzero H update factors\n");
+            return 1;
+        }
+    for (int ax = 0; ax < 3; ax++) {
+        const int n = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
+        const int w = (g.f[2 * ax].on ? g.f[2 * ax].depth : 0) + (g.f[2 * ax + 1].on ? g.f[2 * ax + 1].depth : 0);
+        if (w > n) {
+            fprintf(stderr, "Error: the CPML slabs of one axis overlap\n");
+            return 1;
+        }
+    }
     CK(hipSetDevice(E->dev));
     GsvAdjoint *A = new GsvAdjoint();
     GsvAdj &a = A->a;
@@ -867,17 +822,42 @@ gsv_engine_adjoint_forward(void *eng, int (*step_fn)(void *), void *ctx, double 
     return 0;
 }
 
-/* one backward step, with the E kernel's launch shape (32-bit offsets: the
- * design region already requires field arrays below 4 GiB) */
+/* one backward step: slab terms around the two bulk kernels (the bulk
+ * kernels use the E kernel's launch shape and 32-bit offsets, which the design
+ * region already requires) */
 template <typename ID>
-static void
+static int
 launch_adj(Engine *E, int step, int slot)
 {
     GsvGrid &g = E->g;
+    const GsvAdj &a = E->adj->a;
     const int bx = E->bx_e, by = E->by_e, kc = E->kchunk_e;
     dim3 blk(bx, by, 1), grd((g.nx + bx - 1) / bx, (g.ny + by - 1) / by, (g.nz + kc - 1) / kc);
-    hipLaunchKernelGGL((k_adj_e<ID, uint32_t>), grd, blk, 0, E->st, g, E->adj->a, step, slot, kc);
-    hipLaunchKernelGGL((k_adj_h<ID, uint32_t>), grd, blk, 0, E->st, g, E->adj->a, kc);
+    long long win[3], fmax = 0;
+    for (int ax = 0; ax < 3; ax++) {
+        const int w = (g.f[2 * ax].on ? g.f[2 * ax].depth : 0) + (g.f[2 * ax + 1].on ? g.f[2 * ax + 1].depth : 0);
+        win[ax] = ax == 0 ? (long long)w * g.ny * g.nz : (ax == 1 ? (long long)g.nx * w * g.nz : (long long)g.nx * g.ny * w);
+    }
+    for (int f = 0; f < 6; f++)
+        if (g.f[f].on) {
+            const long long d = g.f[f].depth;
+            const long long n = f < 2 ? d * g.ny * g.nz : (f < 4 ? (long long)g.px * d * g.nz : (long long)g.px * g.ny * d);
+            fmax = n > fmax ? n : fmax;
+        }
+#define GSV_WIN(n) dim3((unsigned)(((n) + 255) / 256)), dim3(256), 0, E->st
+    if (win[0]) hipLaunchKernelGGL(k_slab_e_pre<0>, GSV_WIN(win[0]), g, a);
+    if (win[1]) hipLaunchKernelGGL(k_slab_e_pre<1>, GSV_WIN(win[1]), g, a);
+    if (win[2]) hipLaunchKernelGGL(k_slab_e_pre<2>, GSV_WIN(win[2]), g, a);
+    hipLaunchKernelGGL((k_adj_e<ID, uint32_t>), grd, blk, 0, E->st, g, a, step, slot, kc);
+    if (fmax) hipLaunchKernelGGL(k_slab_e_post<ID>, dim3((unsigned)((fmax + 255) / 256), 6), dim3(256), 0, E->st, g, a);
+    if (win[0]) hipLaunchKernelGGL(k_slab_h_pre<0>, GSV_WIN(win[0]), g, a);
+    if (win[1]) hipLaunchKernelGGL(k_slab_h_pre<1>, GSV_WIN(win[1]), g, a);
+    if (win[2]) hipLaunchKernelGGL(k_slab_h_pre<2>, GSV_WIN(win[2]), g, a);
+    hipLaunchKernelGGL((k_adj_h<ID, uint32_t>), grd, blk, 0, E->st, g, a, kc);
+    if (fmax) hipLaunchKernelGGL(k_slab_h_post<ID>, dim3((unsigned)((fmax + 255) / 256), 6), dim3(256), 0, E->st, g, a);
+#undef GSV_WIN
+    CK(hipGetLastError());
+    return 0;
 }
 
 /* Backward pass: coefficient gradients accumulated on the device; the host
@@ -931,11 +911,8 @@ gsv_engine_adjoint_backward(void *eng, int (*step_fn)(void *), void *ctx, float 
         }
         for (int n = n1 - 1; n >= n0; n--) {
             const int slot = A->history_all ? n : n - n0;
-            if (E->idbytes_e == 1)
-                launch_adj<uint8_t>(E, n, slot);
-            else
-                launch_adj<uint16_t>(E, n, slot);
-            CK(hipGetLastError());
+            if (E->idbytes_e == 1 ? launch_adj<uint8_t>(E, n, slot) : launch_adj<uint16_t>(E, n, slot))
+                return 1;
         }
     }
     CK(hipEventRecord(e1, E->st));
