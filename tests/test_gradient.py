@@ -150,3 +150,61 @@ def test_set_design_eps_equals_file(tmp_path):
     J_file = s.objective()
     s.close()
     assert J_api == J_file
+
+
+def torch_problem(folder, n=24, nsteps=200, design=(6, 6, 4), cpml=5, gap=3):
+    """The inverse-design problem at a size the pure-torch FDTD differentiates quickly."""
+    import torch_fdtd
+    par, box, eps = cases.inverse_design(folder, n=n, nsteps=nsteps, design=design, cpml=cpml, gap=gap)
+    c = n // 2
+    dx = 10e-9
+    dt = 1.0 / C0 / np.sqrt(3.0 / dx ** 2)
+    src = cases.cw_ramp(nsteps, dt, 1.0e-6)
+    # the source file is written with %.17g and read back as double
+    src = [float("%.17g" % v) for v in src]
+    mk = box[5] + gap
+    sigma = float("%.9g" % (0.8 * 4 / (376.73 * dx)))   # as written in the parameter file
+    prob = torch_fdtd.Problem(n, dx, cpml, 3, sigma, 1.0, box, (c, c, box[2] - gap), src,
+                              (box[0], box[1], mk, box[3], box[4], mk + 1), 1.0e-6)
+    return par, box, eps, prob
+
+
+def test_torch_fdtd_reproduces_oracle_objective(tmp_path):
+    """The pure-torch FDTD is the same discrete scheme: J agrees with the FP64 oracle."""
+    import torch
+    par, box, eps, prob = torch_problem(str(tmp_path / "t"))
+    J_oracle = oracle_J(par)
+    # the CPU path stores float32(eps_r) * float32(eps0) per voxel
+    e32 = (eps.astype(np.float32) * np.float32(8.854187817620389850e-12)).astype(np.float64) / 8.854187817620389850e-12
+    bg = float(np.float32(1.0) * np.float32(8.854187817620389850e-12))
+    with torch.no_grad():
+        J_torch = float(prob.objective(torch.tensor(e32, dtype=torch.float64), background_eps=bg))
+    assert J_oracle > 0
+    # fields agree to ~1e-10; the DFT over ~1.2 periods cancels, so J is held to 1e-7
+    # (leaving out the interface averaging, for instance, moves J by 5e-4)
+    assert abs(J_torch - J_oracle) / J_oracle < 1e-7, (J_torch, J_oracle)
+
+
+@pytest.mark.gpu
+def test_adjoint_gradient_against_torch_autograd(tmp_path):
+    """Engine adjoint (FP32) vs torch autograd through the pure-torch FDTD (FP64), whole design box."""
+    import torch
+    from gsvit_copy_amd import Session
+    from gsvit_copy_amd.autograd import DesignObjective
+    par, box, eps, prob = torch_problem(str(tmp_path / "ta"))
+    e = torch.tensor(eps.astype(np.float64), requires_grad=True)
+    J_t = prob.objective(e)
+    J_t.backward()
+    g_torch = e.grad.numpy()
+    s = Session(par)
+    s.adjoint_setup()
+    e_gpu = torch.tensor(eps, dtype=torch.float32, requires_grad=True)
+    J_g = DesignObjective.apply(e_gpu, s)
+    J_g.backward()
+    g_gpu = e_gpu.grad.numpy().astype(np.float64)
+    s.close()
+    print("J torch", float(J_t), "gpu", float(J_g))
+    err = np.linalg.norm(g_gpu - g_torch) / np.linalg.norm(g_torch)
+    print("gradient rel L2 vs torch autograd", err)
+    assert abs(float(J_g) - float(J_t)) / float(J_t) < 1e-5
+    assert err < 1e-3, err
