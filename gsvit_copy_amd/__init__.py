@@ -4,6 +4,6 @@ The product is the C-ABI library lib/libgsvit_amd.so (drop-in sv_gpu_new /
 sv_pool_gpu_* entry points plus a plain session API) and the gsvit_amd driver;
 this package only locates and binds them.
 """
-from .engine import Session, library, LIB_PATH  # noqa: F401
+from .engine import Batch, Session, library, LIB_PATH  # noqa: F401
 
-__all__ = ["Session", "library", "LIB_PATH"]
+__all__ = ["Batch", "Session", "library", "LIB_PATH"]

@@ -52,11 +52,26 @@ typedef struct {
     float pad[3];
 } GsvHCoef;
 
-/* ADE material (Drude + 2 critical points), src3d/plan.c:2736-2762 */
+/* ADE material (Drude + 2 critical points), src3d/plan.c:2736-2762 and the
+ * ADE branch of src3d/yee.c:914-1108, carried in increment form.  With
+ * J^n = P^n - P^(n-1), the reference's polarisation recursion
+ *     P^(n+1) = a0 P^(n-1) + a1 P^n + a2 E^(n-1) + a3 E^n + a4 E^(n+1)
+ * is  J^(n+1) = -a0 J^n + s P^n + a2 E^(n-1) + a3 E^n + a4 E^(n+1),  s = a0 + a1 - 1,
+ * and its E-update term -a0 P^(n-1) + (1 - a1) P^n is a0 J^n - s P^n: the same
+ * update in exact arithmetic, but in FP32 the small increment is stored and
+ * multiplied directly instead of arising from the cancellation of two nearly
+ * equal P values with separately rounded coefficients (whose rounding, with
+ * a0 + a1 - 1 no longer zero, also acts as a spurious restoring term).  The
+ * Drude term has s = 0 identically (a0 = (nu dt - 2)/(nu dt + 2), a1 = 4/(nu
+ * dt + 2)), so its state is J alone. */
 typedef struct {
-    float c0, c1, c2, c3;
-    float a0, a1, a2;
-    float bp0[2], bp1[2], bp2[2], bp3[2], bp4[2];
+    float c0, c2, c3;  /* E update: c0 curl / d + ... + c2 E^(n-1) + c3 E^n */
+    float cj;          /* E-update coefficient of the Drude J: c1 a0 */
+    float dj, da;      /* Drude: J' = dj J + da (E^(n-1) + 2 E^n + E^(n+1)); dj = -a0, da = a2 */
+    float pcj[2];      /* E-update coefficients of a pole's J and P: c1 bp0, -c1 s */
+    float pcp[2];
+    float pj[2], ps[2];           /* pole J recursion coefficients: -bp0, s */
+    float b2[2], b3[2], b4[2];    /* pole E coefficients: bp2, bp3, bp4 */
     float cbm[3];    /* material cb used by the ADE branch's inline CPML, src3d/yee.c:952-1057 */
     int poles;       /* nonzero if the critical-point terms are present */
 } GsvAde;
@@ -76,8 +91,8 @@ typedef struct {
 /* ADE state blocks: per z plane [j][i/64][state][64], x pitch pxa (a
  * multiple of 64), so one base per plane plus the lane's offset reaches
  * every state of a cell through immediate offsets (state * 256 bytes) */
-#define GSV_ADE_D 9      /* E(n-1) x y z, dP x y z, dP(n-1) x y z */
-#define GSV_ADE_P 12     /* P0 x y z, P1 x y z, P0(n-1) x y z, P1(n-1) x y z (critical points) */
+#define GSV_ADE_D 6      /* E(n-1) x y z, Drude J x y z */
+#define GSV_ADE_P 12     /* P0 x y z, P1 x y z, J0 x y z, J1 x y z (critical points) */
 
 typedef struct {
     int nx, ny, nz;

@@ -54,6 +54,7 @@ struct Engine {
     /* sources */
     int has_tsf;
     GsvTsf tsf;
+    int tsf_face_on[6];       /* face has at least one nonzero correction coefficient */
     int has_src;
     GsvPointSrc src;
     /* traces */

@@ -281,11 +281,22 @@ build_tables(Engine *E)
         }
         GsvAde &a = E->adem_host[nslot];
         double dt = mp->d->dt;
-        a.c0 = (float)m->ade_c0; a.c1 = (float)m->ade_c1; a.c2 = (float)m->ade_c2; a.c3 = (float)m->ade_c3;
-        a.a0 = (float)m->ade_a0; a.a1 = (float)m->ade_a1; a.a2 = (float)m->ade_a2;
+        /* increment form (engine.h), coefficients combined in double */
+        if (fabs(m->ade_a0 + m->ade_a1 - 1) > 1e-12) {
+            fprintf(stderr, "Error: Drude coefficients of material %d do not satisfy a0 + a1 = 1\n", n);
+            return 1;
+        }
+        a.c0 = (float)m->ade_c0; a.c2 = (float)m->ade_c2; a.c3 = (float)m->ade_c3;
+        a.cj = (float)(m->ade_c1 * m->ade_a0);
+        a.dj = (float)(-m->ade_a0);
+        a.da = (float)m->ade_a2;
         for (int q = 0; q < 2; q++) {
-            a.bp0[q] = (float)m->ade_bp0[q]; a.bp1[q] = (float)m->ade_bp1[q]; a.bp2[q] = (float)m->ade_bp2[q];
-            a.bp3[q] = (float)m->ade_bp3[q]; a.bp4[q] = (float)m->ade_bp4[q];
+            const double sq = m->ade_bp0[q] + m->ade_bp1[q] - 1;
+            a.pcj[q] = (float)(m->ade_c1 * m->ade_bp0[q]);
+            a.pcp[q] = (float)(-m->ade_c1 * sq);
+            a.pj[q] = (float)(-m->ade_bp0[q]);
+            a.ps[q] = (float)sq;
+            a.b2[q] = (float)m->ade_bp2[q]; a.b3[q] = (float)m->ade_bp3[q]; a.b4[q] = (float)m->ade_bp4[q];
         }
         a.cbm[0] = (float)((dt / m->epsilon / mp->d->dx) / (1 + m->sigma * dt / 2 / m->epsilon));
         a.cbm[1] = (float)((dt / m->epsilon / mp->d->dy) / (1 + m->sigma * dt / 2 / m->epsilon));
@@ -585,6 +596,10 @@ setup_tsf(Engine *E)
                 ce[(size_t)a + (size_t)na * b] = (float)(ecor / eps);
                 chh[(size_t)a + (size_t)na * b] = (float)hcor;
             }
+        E->tsf_face_on[f] = 0;
+        for (size_t q = 0; q < ce.size(); q++)
+            if (ce[q] != 0.f || chh[q] != 0.f)
+                E->tsf_face_on[f] = 1;
         float *dce, *dch;
         if (dupload(E, &dce, ce.data(), ce.size()) || dupload(E, &dch, chh.data(), chh.size()))
             return 1;

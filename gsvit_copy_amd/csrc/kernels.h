@@ -83,6 +83,18 @@ typedef struct {
     double *out;                  /* this box's value per step */
 } GsvBoxSum;
 
+/* batched stepping of independent grids, one launch per kernel type for all
+ * of them: a volume tile is one block of a grid's columns (tx, ty) over z
+ * chunk kz; a face tile is one 64 x 4 block (ta, tb) of a TSF face
+ * (face = 2 * axis + far) */
+typedef struct {
+    uint16_t run, tx, ty, kz;
+} GsvTile;
+
+typedef struct {
+    uint16_t run, face, ta, tb;
+} GsvFaceTile;
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -100,6 +112,16 @@ hipError_t gsv_launch_farfield_td(const GsvGrid *g, const GsvFarfieldTD *f, int 
 hipError_t gsv_launch_dft_sample(const GsvGrid *g, const GsvDft *d, int slot, hipStream_t st);
 hipError_t gsv_launch_dft_flush(const GsvDft *d, int first_step, int nsamples, hipStream_t st);
 hipError_t gsv_launch_box_sum(const GsvGrid *g, const GsvBoxSum *b, int step, hipStream_t st);
+/* batched launches: grids / tsfs / tiles / runs / steps are device arrays;
+ * nblocks 0 launches one workgroup per tile, otherwise nblocks workgroups
+ * stride over the tile list */
+hipError_t gsv_launch_hstep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int bx, int by,
+                                  int kchunk, int nblocks, hipStream_t st);
+hipError_t gsv_launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int ade, int bx,
+                                  int by, int kchunk, int nblocks, hipStream_t st);
+hipError_t gsv_launch_tsf_line_batch(const GsvTsf *tsfs, const int *runs, int nruns, int *steps, hipStream_t st);
+hipError_t gsv_launch_tsf_faces_batch(const GsvGrid *grids, const GsvTsf *tsfs, const GsvFaceTile *tiles, int ntiles,
+                                      int hside, hipStream_t st);
 #ifdef __cplusplus
 }
 #endif

@@ -21,6 +21,8 @@
  *   k_farfield_td         time-domain Kirchhoff bins, src3d/farfield.c:174-407
  *   k_dft_sample / k_dft_flush   running DFT of tangential fields on the NFFF box
  *   k_box_sum             OUT_SUM box sums of E^2, src3d/output.c:259-306
+ * Batched variants (k_*_batch) run the same column and face-cell functions
+ * over a tile list that covers every grid of a batch of independent runs.
  */
 #include <hip/hip_runtime.h>
 #include "gsvit_settings.h"
@@ -34,16 +36,15 @@
 /* H update                                                             */
 /* ------------------------------------------------------------------ */
 
+/* column (i, j) of the H update over the planes of z chunk kz (kchunk planes each) */
 template <typename ID, typename OFF>
-__global__ __launch_bounds__(512) void
-k_hstep(GsvGrid g, int kchunk)
+__device__ __forceinline__ void
+hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= g.nx || j >= g.ny)
         return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int k0 = blockIdx.z * kchunk;
+    const int k0 = kz * kchunk;
     const int k1 = min(k0 + kchunk, nz);
     const int px = g.px;
     const OFF dP = (OFF)g.plane * 4, dJ = (OFF)px * 4;
@@ -162,15 +163,24 @@ k_hstep(GsvGrid g, int kchunk)
     }
 }
 
+template <typename ID, typename OFF>
+__global__ __launch_bounds__(512) void
+k_hstep(GsvGrid g, int kchunk)
+{
+    hstep_col<ID, OFF>(g, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+                       blockIdx.z, kchunk);
+}
+
 /* ------------------------------------------------------------------ */
 /* E update                                                             */
 /* ------------------------------------------------------------------ */
 
 /* ADE branch of one cell (src3d/yee.c:914-1108); updates the components whose
  * flag is clear, runs the branch's own CPML psi update and advances the
- * polarisation state.  Taken in dispersive cells and their +1 neighbours.
- * ad/ap: this plane's ADE state blocks, la/lp: the lane's offset in them;
- * state q of a cell sits q * 256 bytes after its first state. */
+ * polarisation state (increment form, see GsvAde).  Taken in dispersive cells
+ * and their +1 neighbours.  ad/ap: this plane's ADE state blocks, la/lp: the
+ * lane's offset in them; state q of a cell sits q * 256 bytes after its first
+ * state. */
 template <bool POLES, typename OFF>
 __device__ __forceinline__ void
 ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, int k, unsigned fl, float &ex,
@@ -182,34 +192,28 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
     const float vexp = ex, veyp = ey, vezp = ez;
     const float expv = ldsn(ad, la, 0), eypv = ldsn(ad, la, 256), ezpv = ldsn(ad, la, 512);
-    const float dpx = ldsn(ad, la, 768), dpy = ldsn(ad, la, 1024), dpz = ldsn(ad, la, 1280);
-    const float dpxp = ldsn(ad, la, 1536), dpyp = ldsn(ad, la, 1792), dpzp = ldsn(ad, la, 2048);
-    float px0 = 0, px1 = 0, py0 = 0, py1 = 0, pz0 = 0, pz1 = 0;
-    float pxp0 = 0, pxp1 = 0, pyp0 = 0, pyp1 = 0, pzp0 = 0, pzp1 = 0;
+    const float jx = ldsn(ad, la, 768), jy = ldsn(ad, la, 1024), jz = ldsn(ad, la, 1280);
+    /* pole state: P0 x y z, P1 x y z, J0 x y z, J1 x y z */
+    float p0x = 0, p0y = 0, p0z = 0, p1x = 0, p1y = 0, p1z = 0;
+    float j0x = 0, j0y = 0, j0z = 0, j1x = 0, j1y = 0, j1z = 0;
     if (POLES && m.poles) {
-        px0 = ldsn(ap, lp, 0); px1 = ldsn(ap, lp, 768); py0 = ldsn(ap, lp, 256);
-        py1 = ldsn(ap, lp, 1024); pz0 = ldsn(ap, lp, 512); pz1 = ldsn(ap, lp, 1280);
-        pxp0 = ldsn(ap, lp, 1536); pxp1 = ldsn(ap, lp, 2304); pyp0 = ldsn(ap, lp, 1792);
-        pyp1 = ldsn(ap, lp, 2560); pzp0 = ldsn(ap, lp, 2048); pzp1 = ldsn(ap, lp, 2816);
+        p0x = ldsn(ap, lp, 0); p0y = ldsn(ap, lp, 256); p0z = ldsn(ap, lp, 512);
+        p1x = ldsn(ap, lp, 768); p1y = ldsn(ap, lp, 1024); p1z = ldsn(ap, lp, 1280);
+        j0x = ldsn(ap, lp, 1536); j0y = ldsn(ap, lp, 1792); j0z = ldsn(ap, lp, 2048);
+        j1x = ldsn(ap, lp, 2304); j1y = ldsn(ap, lp, 2560); j1z = ldsn(ap, lp, 2816);
     }
 
     if (j > 0 && k > 0 && !tx)
-        ex = m.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx
-             + m.c1 * (-m.a0 * dpxp + (1.f - m.a1) * dpx)
-             + m.c1 * (-m.bp0[0] * pxp0 + (1.f - m.bp1[0]) * px0)
-             + m.c1 * (-m.bp0[1] * pxp1 + (1.f - m.bp1[1]) * px1)
+        ex = m.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx + m.cj * jx
+             + (m.pcj[0] * j0x + m.pcp[0] * p0x) + (m.pcj[1] * j1x + m.pcp[1] * p1x)
              + m.c2 * expv + m.c3 * ex;
     if (i > 0 && k > 0 && !ty)
-        ey = m.c0 * ((hx - hxkm1) * rkz - (hz - hzi) * rkx) / g.dy
-             + m.c1 * (-m.a0 * dpyp + (1.f - m.a1) * dpy)
-             + m.c1 * (-m.bp0[0] * pyp0 + (1.f - m.bp1[0]) * py0)
-             + m.c1 * (-m.bp0[1] * pyp1 + (1.f - m.bp1[1]) * py1)
+        ey = m.c0 * ((hx - hxkm1) * rkz - (hz - hzi) * rkx) / g.dy + m.cj * jy
+             + (m.pcj[0] * j0y + m.pcp[0] * p0y) + (m.pcj[1] * j1y + m.pcp[1] * p1y)
              + m.c2 * eypv + m.c3 * ey;
     if (i > 0 && j > 0 && !tz)
-        ez = m.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz
-             + m.c1 * (-m.a0 * dpzp + (1.f - m.a1) * dpz)
-             + m.c1 * (-m.bp0[0] * pzp0 + (1.f - m.bp1[0]) * pz0)
-             + m.c1 * (-m.bp0[1] * pzp1 + (1.f - m.bp1[1]) * pz1)
+        ez = m.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz + m.cj * jz
+             + (m.pcj[0] * j0z + m.pcp[0] * p0z) + (m.pcj[1] * j1z + m.pcp[1] * p1z)
              + m.c2 * ezpv + m.c3 * ez;
 
     /* the branch's own CPML update; its x0 correction multiplies by a zero cb */
@@ -275,37 +279,34 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     }
 
     /* polarisation state (all three components, with the values at this point) */
-    stsn(ad, la, 768, m.a0 * dpxp + m.a1 * dpx + m.a2 * (expv + 2.f * vexp + ex));
-    stsn(ad, la, 1024, m.a0 * dpyp + m.a1 * dpy + m.a2 * (eypv + 2.f * veyp + ey));
-    stsn(ad, la, 1280, m.a0 * dpzp + m.a1 * dpz + m.a2 * (ezpv + 2.f * vezp + ez));
+    stsn(ad, la, 768, m.dj * jx + m.da * (expv + 2.f * vexp + ex));
+    stsn(ad, la, 1024, m.dj * jy + m.da * (eypv + 2.f * veyp + ey));
+    stsn(ad, la, 1280, m.dj * jz + m.da * (ezpv + 2.f * vezp + ez));
     if (POLES && m.poles) {
-        stsn(ap, lp, 0, m.bp0[0] * pxp0 + m.bp1[0] * px0 + m.bp2[0] * expv + m.bp3[0] * vexp + m.bp4[0] * ex);
-        stsn(ap, lp, 768, m.bp0[1] * pxp1 + m.bp1[1] * px1 + m.bp2[1] * expv + m.bp3[1] * vexp + m.bp4[1] * ex);
-        stsn(ap, lp, 256, m.bp0[0] * pyp0 + m.bp1[0] * py0 + m.bp2[0] * eypv + m.bp3[0] * veyp + m.bp4[0] * ey);
-        stsn(ap, lp, 1024, m.bp0[1] * pyp1 + m.bp1[1] * py1 + m.bp2[1] * eypv + m.bp3[1] * veyp + m.bp4[1] * ey);
-        stsn(ap, lp, 512, m.bp0[0] * pzp0 + m.bp1[0] * pz0 + m.bp2[0] * ezpv + m.bp3[0] * vezp + m.bp4[0] * ez);
-        stsn(ap, lp, 1280, m.bp0[1] * pzp1 + m.bp1[1] * pz1 + m.bp2[1] * ezpv + m.bp3[1] * vezp + m.bp4[1] * ez);
-        stsn(ap, lp, 1536, px0); stsn(ap, lp, 2304, px1); stsn(ap, lp, 1792, py0);
-        stsn(ap, lp, 2560, py1); stsn(ap, lp, 2048, pz0); stsn(ap, lp, 2816, pz1);
+        const float n0x = m.pj[0] * j0x + m.ps[0] * p0x + m.b2[0] * expv + m.b3[0] * vexp + m.b4[0] * ex;
+        const float n1x = m.pj[1] * j1x + m.ps[1] * p1x + m.b2[1] * expv + m.b3[1] * vexp + m.b4[1] * ex;
+        const float n0y = m.pj[0] * j0y + m.ps[0] * p0y + m.b2[0] * eypv + m.b3[0] * veyp + m.b4[0] * ey;
+        const float n1y = m.pj[1] * j1y + m.ps[1] * p1y + m.b2[1] * eypv + m.b3[1] * veyp + m.b4[1] * ey;
+        const float n0z = m.pj[0] * j0z + m.ps[0] * p0z + m.b2[0] * ezpv + m.b3[0] * vezp + m.b4[0] * ez;
+        const float n1z = m.pj[1] * j1z + m.ps[1] * p1z + m.b2[1] * ezpv + m.b3[1] * vezp + m.b4[1] * ez;
+        stsn(ap, lp, 0, p0x + n0x); stsn(ap, lp, 256, p0y + n0y); stsn(ap, lp, 512, p0z + n0z);
+        stsn(ap, lp, 768, p1x + n1x); stsn(ap, lp, 1024, p1y + n1y); stsn(ap, lp, 1280, p1z + n1z);
+        stsn(ap, lp, 1536, n0x); stsn(ap, lp, 1792, n0y); stsn(ap, lp, 2048, n0z);
+        stsn(ap, lp, 2304, n1x); stsn(ap, lp, 2560, n1y); stsn(ap, lp, 2816, n1z);
     }
     stsn(ad, la, 0, vexp);
     stsn(ad, la, 256, veyp);
     stsn(ad, la, 512, vezp);
-    stsn(ad, la, 1536, dpx);
-    stsn(ad, la, 1792, dpy);
-    stsn(ad, la, 2048, dpz);
 }
 
 template <typename ID, int ADE, typename OFF, bool PC>
-__global__ __launch_bounds__(512) void
-k_estep(GsvGrid g, int kchunk)
+__device__ __forceinline__ void
+estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= g.nx || j >= g.ny)
         return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int k0 = blockIdx.z * kchunk;
+    const int k0 = kz * kchunk;
     const int k1 = min(k0 + kchunk, nz);
     const int px = g.px;
     const OFF dP = (OFF)g.plane * 4, dJ = (OFF)px * 4;
@@ -446,6 +447,14 @@ k_estep(GsvGrid g, int kchunk)
     }
 }
 
+template <typename ID, int ADE, typename OFF, bool PC>
+__global__ __launch_bounds__(512) void
+k_estep(GsvGrid g, int kchunk)
+{
+    estep_col<ID, ADE, OFF, PC>(g, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+                                blockIdx.z, kchunk);
+}
+
 /* ------------------------------------------------------------------ */
 /* TSF plane wave                                                       */
 /* ------------------------------------------------------------------ */
@@ -476,10 +485,11 @@ tsf_d(const GsvTsf &t, int i, int j, int k)
     return t.corr * (10.0 + t.ax * (double)(i - t.oi) + t.ay * (double)(j - t.oj) + t.az * (double)(k - t.ok));
 }
 
-__global__ __launch_bounds__(1024) void
-k_tsf_line(GsvTsf t, int step)
+/* one step of the 1-D line (double precision), src3d/1dpool.c:83-162; one
+ * block, `step` is read by thread 0 only */
+__device__ __forceinline__ void
+tsf_line_step(const GsvTsf &t, int step)
 {
-    /* one step of the 1-D line (double precision), src3d/1dpool.c:83-162 */
     const int res = t.res;
     double *e = t.je, *h = t.jh, *b = t.jb;
     for (int q = threadIdx.x + 1; q < res; q += blockDim.x)
@@ -499,13 +509,16 @@ k_tsf_line(GsvTsf t, int step)
         h[q] += t.multh * (e[q] - e[q + 1]);
 }
 
-/* H corrections on the two faces normal to `axis` (blockIdx.z: 0 near, 1 far) */
-__global__ __launch_bounds__(256) void
-k_tsf_h(GsvGrid g, GsvTsf t, int axis)
+__global__ __launch_bounds__(1024) void
+k_tsf_line(GsvTsf t, int step)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y * blockDim.y + threadIdx.y;
-    const int far = blockIdx.z;
+    tsf_line_step(t, step);
+}
+
+/* H correction of face cell (a, b) of the face normal to `axis` (far: 0 near, 1 far) */
+__device__ __forceinline__ void
+tsf_h_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
+{
     int na, nb;
     if (axis == 0) { na = t.jn - t.j0 + 1; nb = t.kn - t.k0 + 1; }
     else if (axis == 1) { na = t.in - t.i0 + 1; nb = t.kn - t.k0 + 1; }
@@ -547,13 +560,10 @@ k_tsf_h(GsvGrid g, GsvTsf t, int axis)
     }
 }
 
-/* E corrections on the two faces normal to `axis` */
-__global__ __launch_bounds__(256) void
-k_tsf_e(GsvGrid g, GsvTsf t, int axis)
+/* E correction of face cell (a, b) of the face normal to `axis` */
+__device__ __forceinline__ void
+tsf_e_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y * blockDim.y + threadIdx.y;
-    const int far = blockIdx.z;
     int na, nb;
     if (axis == 0) { na = t.jn - t.j0 + 1; nb = t.kn - t.k0 + 1; }
     else if (axis == 1) { na = t.in - t.i0 + 1; nb = t.kn - t.k0 + 1; }
@@ -588,6 +598,20 @@ k_tsf_e(GsvGrid g, GsvTsf t, int axis)
         if (i < t.in) g.ex[c] += s * (float)(cf * t.ghy * vh);
         if (j < t.jn) g.ey[c] -= s * (float)(cf * t.ghx * vh);
     }
+}
+
+/* H corrections on the two faces normal to `axis` (blockIdx.z: 0 near, 1 far) */
+__global__ __launch_bounds__(256) void
+k_tsf_h(GsvGrid g, GsvTsf t, int axis)
+{
+    tsf_h_cell(g, t, axis, blockIdx.z, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y);
+}
+
+/* E corrections on the two faces normal to `axis` */
+__global__ __launch_bounds__(256) void
+k_tsf_e(GsvGrid g, GsvTsf t, int axis)
+{
+    tsf_e_cell(g, t, axis, blockIdx.z, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1119,6 +1143,130 @@ gsv_launch_dft_flush(const GsvDft *d, int first_step, int nsamples, hipStream_t 
     const size_t lds = (size_t)nsamples * d->nw * 4 * sizeof(double);
     hipLaunchKernelGGL(k_dft_flush, dim3((unsigned)((d->ncells + 255) / 256)), dim3(256), lds, st, *d, first_step,
                        nsamples);
+    return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ */
+/* batched stepping: the same column and face-cell updates, one launch per */
+/* kernel type for every grid of a batch; a workgroup reads its tile, then */
+/* the tile's grid parameters (uniform, scalar loads)                       */
+/* ------------------------------------------------------------------ */
+
+template <typename ID, typename OFF>
+__global__ __launch_bounds__(512) void
+k_hstep_batch(const GsvGrid *__restrict__ grids, const GsvTile *__restrict__ tiles, int ntiles, int kchunk)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const GsvTile e = tiles[t];
+        hstep_col<ID, OFF>(grids[e.run], e.tx * blockDim.x + threadIdx.x, e.ty * blockDim.y + threadIdx.y, e.kz, kchunk);
+    }
+}
+
+template <typename ID, int ADE, typename OFF>
+__global__ __launch_bounds__(512) void
+k_estep_batch(const GsvGrid *__restrict__ grids, const GsvTile *__restrict__ tiles, int ntiles, int kchunk)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const GsvTile e = tiles[t];
+        estep_col<ID, ADE, OFF, false>(grids[e.run], e.tx * blockDim.x + threadIdx.x, e.ty * blockDim.y + threadIdx.y,
+                                       e.kz, kchunk);
+    }
+}
+
+/* one block per grid with a TSF line; the grid's step counter lives in device
+ * memory so that a captured step sequence replays with advancing steps */
+__global__ __launch_bounds__(1024) void
+k_tsf_line_batch(const GsvTsf *__restrict__ tsfs, const int *__restrict__ runs, int *steps)
+{
+    const int r = runs[blockIdx.x];
+    const int step = threadIdx.x == 0 ? steps[r] : 0;
+    tsf_line_step(tsfs[r], step);
+    if (threadIdx.x == 0)
+        steps[r] = step + 1;
+}
+
+__global__ __launch_bounds__(256) void
+k_tsf_h_batch(const GsvGrid *__restrict__ grids, const GsvTsf *__restrict__ tsfs, const GsvFaceTile *__restrict__ tiles)
+{
+    const GsvFaceTile e = tiles[blockIdx.x];
+    tsf_h_cell(grids[e.run], tsfs[e.run], e.face >> 1, e.face & 1, e.ta * blockDim.x + threadIdx.x,
+               e.tb * blockDim.y + threadIdx.y);
+}
+
+__global__ __launch_bounds__(256) void
+k_tsf_e_batch(const GsvGrid *__restrict__ grids, const GsvTsf *__restrict__ tsfs, const GsvFaceTile *__restrict__ tiles)
+{
+    const GsvFaceTile e = tiles[blockIdx.x];
+    tsf_e_cell(grids[e.run], tsfs[e.run], e.face >> 1, e.face & 1, e.ta * blockDim.x + threadIdx.x,
+               e.tb * blockDim.y + threadIdx.y);
+}
+
+static dim3
+batch_grid(int ntiles, int nblocks)
+{
+    return dim3((unsigned)(nblocks > 0 && nblocks < ntiles ? nblocks : ntiles));
+}
+
+extern "C" hipError_t
+gsv_launch_hstep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int bx, int by, int kchunk,
+                       int nblocks, hipStream_t st)
+{
+    if (ntiles <= 0)
+        return hipSuccess;
+    const dim3 blk(bx, by, 1), grd = batch_grid(ntiles, nblocks);
+    if (idbytes == 1)
+        hipLaunchKernelGGL((k_hstep_batch<uint8_t, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
+    else
+        hipLaunchKernelGGL((k_hstep_batch<uint16_t, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
+    return hipGetLastError();
+}
+
+template <typename ID>
+static void
+launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int ade, dim3 grd, dim3 blk, int kchunk,
+                   hipStream_t st)
+{
+    if (ade == 0)
+        hipLaunchKernelGGL((k_estep_batch<ID, 0, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
+    else if (ade == 1)
+        hipLaunchKernelGGL((k_estep_batch<ID, 1, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
+    else
+        hipLaunchKernelGGL((k_estep_batch<ID, 2, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
+}
+
+extern "C" hipError_t
+gsv_launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int ade, int bx, int by,
+                       int kchunk, int nblocks, hipStream_t st)
+{
+    if (ntiles <= 0)
+        return hipSuccess;
+    const dim3 blk(bx, by, 1), grd = batch_grid(ntiles, nblocks);
+    if (idbytes == 1)
+        launch_estep_batch<uint8_t>(grids, tiles, ntiles, ade, grd, blk, kchunk, st);
+    else
+        launch_estep_batch<uint16_t>(grids, tiles, ntiles, ade, grd, blk, kchunk, st);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_tsf_line_batch(const GsvTsf *tsfs, const int *runs, int nruns, int *steps, hipStream_t st)
+{
+    if (nruns <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_tsf_line_batch, dim3(nruns), dim3(1024), 0, st, tsfs, runs, steps);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_tsf_faces_batch(const GsvGrid *grids, const GsvTsf *tsfs, const GsvFaceTile *tiles, int ntiles, int hside,
+                           hipStream_t st)
+{
+    if (ntiles <= 0)
+        return hipSuccess;
+    if (hside)
+        hipLaunchKernelGGL(k_tsf_h_batch, dim3(ntiles), dim3(64, 4), 0, st, grids, tsfs, tiles);
+    else
+        hipLaunchKernelGGL(k_tsf_e_batch, dim3(ntiles), dim3(64, 4), 0, st, grids, tsfs, tiles);
     return hipGetLastError();
 }
 

@@ -27,6 +27,14 @@ int gsv_engine_adjoint_backward(void *eng, int (*step_fn)(void *), void *ctx, fl
 int gsv_engine_set_design_eps(void *eng, const float *eps_r);
 int gsv_engine_get_design_eps(void *eng, float *eps_r);
 void gsv_engine_adjoint_info(void *eng, double *out);
+void *gsv_engine_batch_new(void *const *engines, int n);
+int gsv_engine_batch_run(void *b, int nsteps, const int *steps);
+int gsv_engine_batch_set_launch(void *b, const int *shape);
+void gsv_engine_batch_get_launch(void *b, int *shape);
+int gsv_engine_batch_set_graph(void *b, int steps_per_graph);
+void gsv_engine_batch_set_timing(void *b, int on);
+void gsv_engine_batch_info(void *b, double *out);
+void gsv_engine_batch_free(void *b);
 
 typedef struct {
     SvSet set;
@@ -341,4 +349,203 @@ gsv_close(void *h)
     }
     clear_settings(&s->set, 1);
     free(s);
+}
+
+/* ---- batches of independent runs ---- */
+
+typedef struct {
+    int n;
+    GsvSession **s;
+    int *steps;
+    void *eng;
+} GsvBatchSession;
+
+void
+gsv_batch_close(void *h)
+{
+    GsvBatchSession *b = (GsvBatchSession *)h;
+    if (!b)
+        return;
+    gsv_engine_batch_free(b->eng);
+    free(b->s);
+    free(b->steps);
+    free(b);
+}
+
+void *
+gsv_batch_open(void *const *sessions, int n)
+{
+    GsvBatchSession *b;
+    void **eng;
+    int r, q;
+    if (n <= 0) {
+        fprintf(stderr, "Error: empty batch\n");
+        return NULL;
+    }
+    for (r = 0; r < n; r++) {
+        if (!sessions[r]) {
+            fprintf(stderr, "Error: batch run %d is not an open session\n", r);
+            return NULL;
+        }
+        if (((GsvSession *)sessions[r])->write_outputs) {
+            fprintf(stderr, "Error: batch run %d writes outputs; batched runs are read with gsv_get_field\n", r);
+            return NULL;
+        }
+        for (q = 0; q < r; q++)
+            if (sessions[q] == sessions[r]) {
+                fprintf(stderr, "Error: session of batch run %d repeats run %d\n", r, q);
+                return NULL;
+            }
+    }
+    b = (GsvBatchSession *)calloc(1, sizeof(GsvBatchSession));
+    eng = (void **)calloc((size_t)n, sizeof(void *));
+    if (b) {
+        b->s = (GsvSession **)calloc((size_t)n, sizeof(GsvSession *));
+        b->steps = (int *)calloc((size_t)n, sizeof(int));
+    }
+    if (!b || !eng || !b->s || !b->steps) {
+        free(eng);
+        gsv_batch_close(b);
+        return NULL;
+    }
+    b->n = n;
+    for (r = 0; r < n; r++) {
+        b->s[r] = (GsvSession *)sessions[r];
+        eng[r] = b->s[r]->mp->gpu;
+    }
+    b->eng = gsv_engine_batch_new(eng, n);
+    free(eng);
+    if (!b->eng) {
+        gsv_batch_close(b);
+        return NULL;
+    }
+    return b;
+}
+
+int
+gsv_batch_run(void *h, int nsteps)
+{
+    GsvBatchSession *b = (GsvBatchSession *)h;
+    int r;
+    for (r = 0; r < b->n; r++) {
+        int left = b->s[r]->set.sc.nsteps - b->s[r]->set.sc.step_act;
+        if (left < nsteps)
+            nsteps = left;
+    }
+    if (nsteps <= 0)
+        return 0;
+    for (r = 0; r < b->n; r++)
+        b->steps[r] = b->s[r]->set.sc.step_act;
+    if (gsv_engine_batch_run(b->eng, nsteps, b->steps))
+        return -1;
+    for (r = 0; r < b->n; r++)
+        b->s[r]->set.sc.step_act += nsteps;
+    return nsteps;
+}
+
+int
+gsv_batch_set_launch(void *h, const int *shape)
+{
+    return gsv_engine_batch_set_launch(((GsvBatchSession *)h)->eng, shape);
+}
+
+void
+gsv_batch_get_launch(void *h, int *shape)
+{
+    gsv_engine_batch_get_launch(((GsvBatchSession *)h)->eng, shape);
+}
+
+int
+gsv_batch_set_graph(void *h, int steps_per_graph)
+{
+    return gsv_engine_batch_set_graph(((GsvBatchSession *)h)->eng, steps_per_graph);
+}
+
+void
+gsv_batch_set_timing(void *h, int on)
+{
+    gsv_engine_batch_set_timing(((GsvBatchSession *)h)->eng, on);
+}
+
+void
+gsv_batch_info(void *h, double *out)
+{
+    gsv_engine_batch_info(((GsvBatchSession *)h)->eng, out);
+}
+
+/* Batch autotuner: the single-run candidate tiles timed on real batched steps
+ * (H and E chosen independently), then the workgroup count of each kernel
+ * (one per tile, or a resident set striding over the tile list). */
+int
+gsv_batch_autotune(void *h, int steps_per_candidate, int *best)
+{
+    static const int bxs[] = { 64, 128, 256 };
+    static const int bys[] = { 1, 2, 4, 8 };
+    static const int kcs[] = { 8, 16, 32, 64, 128 };
+    static const int blocks[] = { 1024, 2048, 4096, 8192 };
+    GsvBatchSession *b = (GsvBatchSession *)h;
+    double bh = 1e30, be = 1e30, t[8];
+    int a, c, d, used = 0, shape[8], left = 1 << 30;
+    for (a = 0; a < b->n; a++)
+        if (b->s[a]->set.sc.nsteps - b->s[a]->set.sc.step_act < left)
+            left = b->s[a]->set.sc.nsteps - b->s[a]->set.sc.step_act;
+    gsv_engine_batch_get_launch(b->eng, best);
+    for (a = 0; a < 3; a++)
+        for (c = 0; c < 4; c++)
+            for (d = 0; d < 5; d++) {
+                int nthr = bxs[a] * bys[c];
+                if (nthr > 512)
+                    continue;
+                if (used + steps_per_candidate > left)
+                    goto shapes_done;
+                shape[0] = shape[3] = bxs[a];
+                shape[1] = shape[4] = bys[c];
+                shape[2] = shape[5] = kcs[d];
+                shape[6] = shape[7] = 0;
+                if (gsv_engine_batch_set_launch(b->eng, shape))
+                    continue;
+                gsv_engine_batch_set_timing(b->eng, 1);
+                if (gsv_batch_run(h, steps_per_candidate) < 0)
+                    return -1;
+                used += steps_per_candidate;
+                gsv_engine_batch_info(b->eng, t);
+                gsv_engine_batch_set_timing(b->eng, 0);
+                if (t[1] > 0 && t[0] < bh) {
+                    bh = t[0];
+                    best[0] = shape[0]; best[1] = shape[1]; best[2] = shape[2]; best[6] = 0;
+                }
+                if (t[3] > 0 && t[2] < be) {
+                    be = t[2];
+                    best[3] = shape[3]; best[4] = shape[4]; best[5] = shape[5]; best[7] = 0;
+                }
+            }
+shapes_done:
+    memcpy(shape, best, sizeof(shape));
+    for (a = 0; a < 4; a++) {
+        if (used + steps_per_candidate > left)
+            break;
+        shape[6] = shape[7] = blocks[a];
+        if (gsv_engine_batch_set_launch(b->eng, shape))
+            continue;
+        gsv_engine_batch_set_timing(b->eng, 1);
+        if (gsv_batch_run(h, steps_per_candidate) < 0)
+            return -1;
+        used += steps_per_candidate;
+        gsv_engine_batch_info(b->eng, t);
+        gsv_engine_batch_set_timing(b->eng, 0);
+        if (t[1] > 0 && t[0] < bh) {
+            bh = t[0];
+            best[6] = blocks[a];
+        }
+        if (t[3] > 0 && t[2] < be) {
+            be = t[2];
+            best[7] = blocks[a];
+        }
+    }
+    if (gsv_engine_batch_set_launch(b->eng, best))
+        return -1;
+    if (b->s[0]->set.sc.verbose > 0)
+        printf("batch autotune: H %dx%d z%d wg %d (%.3f ms), E %dx%d z%d wg %d (%.3f ms), %d steps\n", best[0], best[1],
+               best[2], best[6], bh, best[3], best[4], best[5], best[7], be, used);
+    return used;
 }

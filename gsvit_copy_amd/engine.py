@@ -62,6 +62,20 @@ def library():
         L.gsv_objective_gradient.restype = ctypes.c_int
         L.gsv_objective_gradient.argtypes = [vp, fp]
         L.gsv_adjoint_info.argtypes = [vp, dp]
+        L.gsv_batch_open.restype = vp
+        L.gsv_batch_open.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+        L.gsv_batch_run.restype = ctypes.c_int
+        L.gsv_batch_run.argtypes = [vp, ctypes.c_int]
+        L.gsv_batch_set_launch.restype = ctypes.c_int
+        L.gsv_batch_set_launch.argtypes = [vp, ip]
+        L.gsv_batch_get_launch.argtypes = [vp, ip]
+        L.gsv_batch_set_graph.restype = ctypes.c_int
+        L.gsv_batch_set_graph.argtypes = [vp, ctypes.c_int]
+        L.gsv_batch_autotune.restype = ctypes.c_int
+        L.gsv_batch_autotune.argtypes = [vp, ctypes.c_int, ip]
+        L.gsv_batch_set_timing.argtypes = [vp, ctypes.c_int]
+        L.gsv_batch_info.argtypes = [vp, dp]
+        L.gsv_batch_close.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -238,6 +252,75 @@ class Session:
     def close(self):
         if getattr(self, "h", None):
             self.L.gsv_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+BATCH_INFO_KEYS = ["h_ms", "h_launches", "e_ms", "e_launches", "h_tiles", "e_tiles", "face_tiles", "tsf_runs"]
+
+
+class Batch:
+    """Independent sessions stepped together, one launch per kernel type for
+    all of them (gsv_batch_* in include/gsvit_session.h).  The sessions stay
+    open while the batch lives; read results with Session.field()."""
+
+    def __init__(self, sessions):
+        self.L = library()
+        self.sessions = list(sessions)
+        arr = (ctypes.c_void_p * len(self.sessions))(*[s.h for s in self.sessions])
+        self.h = self.L.gsv_batch_open(arr, len(self.sessions))
+        if not self.h:
+            raise RuntimeError("the sessions could not be batched (see stderr)")
+
+    def run(self, nsteps):
+        r = self.L.gsv_batch_run(self.h, int(nsteps))
+        if r < 0:
+            raise RuntimeError("batched step failed (see stderr)")
+        return r
+
+    def set_launch(self, shape):
+        v = (ctypes.c_int * 8)(*[int(x) for x in shape])
+        if self.L.gsv_batch_set_launch(self.h, v):
+            raise ValueError("invalid batch launch shape %r" % (list(shape),))
+
+    def get_launch(self):
+        v = (ctypes.c_int * 8)()
+        self.L.gsv_batch_get_launch(self.h, v)
+        return list(v)
+
+    def set_graph(self, steps_per_graph):
+        if self.L.gsv_batch_set_graph(self.h, int(steps_per_graph)):
+            raise ValueError("invalid steps per graph")
+
+    def autotune(self, steps_per_candidate=2):
+        v = (ctypes.c_int * 8)()
+        r = self.L.gsv_batch_autotune(self.h, int(steps_per_candidate), v)
+        if r < 0:
+            raise RuntimeError("batch autotune failed (see stderr)")
+        return r, list(v)
+
+    def set_timing(self, on):
+        self.L.gsv_batch_set_timing(self.h, 1 if on else 0)
+
+    def info(self):
+        v = (ctypes.c_double * 8)()
+        self.L.gsv_batch_info(self.h, v)
+        return dict(zip(BATCH_INFO_KEYS, list(v)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gsv_batch_close(self.h)
             self.h = None
 
     def __enter__(self):

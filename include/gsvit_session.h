@@ -70,6 +70,34 @@ int gsv_objective_gradient(void *session, float *grad);
 /* out[0..4]: forward ms, backward ms, whole history kept, checkpoint interval, checkpoints */
 void gsv_adjoint_info(void *session, double *out);
 void gsv_close(void *session);
+/* Batched runs (no reference counterpart: GSvit runs one simulation per
+ * process).  A batch steps n open sessions together, one launch per kernel
+ * type for all of them (runs of uneven grid sizes, e.g. an incidence-angle,
+ * wavelength or roughness sweep); each run's results equal those of stepping
+ * it alone.  A batched session has CPML (or no absorbing boundary) and at
+ * most a TSF source, no point sources, point or sum outputs, far field, DFT,
+ * design region or adjoint, and its output writing off; the sessions stay
+ * open (and are not stepped on their own) while the batch exists, and their
+ * fields are read with gsv_get_field.  NULL on error (message on stderr). */
+void *gsv_batch_open(void *const *sessions, int n);
+/* run nsteps steps of every session (fewer if a session reaches its step
+ * count); returns the number of steps run, -1 on error */
+int gsv_batch_run(void *batch, int nsteps);
+/* shape[0..7]: H x width, rows, z chunk, E x width, rows, z chunk, H and E
+ * workgroups (0: one per tile, otherwise that many stride over the tiles);
+ * x widths are multiples of 64 */
+int gsv_batch_set_launch(void *batch, const int *shape);
+void gsv_batch_get_launch(void *batch, int *shape);
+/* capture steps_per_graph steps as a HIP graph and replay it (0: launch
+ * every kernel from the host) */
+int gsv_batch_set_graph(void *batch, int steps_per_graph);
+/* as gsv_autotune over the batch's tile lists, then the workgroup count;
+ * writes best[0..7] and returns the number of steps consumed, -1 on error */
+int gsv_batch_autotune(void *batch, int steps_per_candidate, int *best);
+/* out[0..7]: mean H ms, timed H launches, mean E ms, timed E launches, H tiles, E tiles, TSF face tiles, TSF runs */
+void gsv_batch_set_timing(void *batch, int on);
+void gsv_batch_info(void *batch, double *out);
+void gsv_batch_close(void *batch);
 /* streaming copy of nbytes (read nbytes + write nbytes) repeated reps times on
  * the current device; *ms_per_copy = mean time of one copy.  Calibrates HBM
  * traffic counters and attainable bandwidth.  Returns 0 on success. */

@@ -7,6 +7,7 @@ folder and returns the parameter-file path.  The workloads:
                   running-DFT near-to-far-field box (512^3)
   inverse_design  continuous-permittivity design region, CW Ez dipole below
                   it, DFT objective on a plane above it (gradient extension)
+  sweep           64 seeded rough-surface runs of uneven grid sizes (batching)
 and small feature cases (CPML + ADE, TSF oblique, Liao + PEC, periodic, far field).
 """
 import math
@@ -94,13 +95,14 @@ def sphere_scatter(folder, n=128, nsteps=1000, nspheres=40, seed=1, npoints=16):
     return path
 
 
-def rough_height(n, rms, corr, seed):
-    """Gaussian-correlated random height field (cells): white noise filtered with a
-    Gaussian of correlation length `corr` cells, scaled to the requested rms."""
+def rough_height(n, rms, corr, seed, ny=None):
+    """Gaussian-correlated random height field (cells) of n x ny (default n x n):
+    white noise filtered with a Gaussian of correlation length `corr` cells,
+    scaled to the requested rms."""
+    ny = n if ny is None else ny
     rng = np.random.default_rng(seed)
-    w = rng.standard_normal((n, n))
-    k = np.fft.fftfreq(n)
-    kx, ky = np.meshgrid(k, k, indexing="ij")
+    w = rng.standard_normal((n, ny))
+    kx, ky = np.meshgrid(np.fft.fftfreq(n), np.fft.fftfreq(ny), indexing="ij")
     filt = np.exp(-(math.pi * corr) ** 2 * (kx ** 2 + ky ** 2) / 2)
     h = np.real(np.fft.ifft2(np.fft.fft2(w) * filt))
     h -= h.mean()
@@ -164,6 +166,63 @@ def rough_surface(folder, n=512, nsteps=4000, dx=5e-9, depth=16, seed=2, rms_nm=
     with open(path, "w") as f:
         f.write("".join(lines))
     return path
+
+
+SWEEP_EDGES = (64, 96, 128, 192, 256)
+
+
+def sweep_params(nruns=64, seed=11, edges=SWEEP_EDGES):
+    """Seeded parameters of a batched sweep: per run and per axis a grid edge
+    drawn from `edges`, a roughness seed, rms U[5, 30] nm, incidence angle
+    U[0, 70] degrees and centre wavelength U[450, 750] nm."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(nruns):
+        shape = tuple(int(e) for e in rng.choice(edges, size=3))
+        out.append(dict(shape=shape, seed=int(rng.integers(1, 1 << 30)), rms_nm=float(rng.uniform(5, 30)),
+                        theta_deg=float(rng.uniform(0, 70)), lam_center=float(rng.uniform(450e-9, 750e-9))))
+    return out
+
+
+def sweep_run(folder, shape, seed, rms_nm, theta_deg, lam_center, nsteps=2000, dx=5e-9, depth=10,
+              corr_nm=100.0, tsf_margin=4):
+    """One run of a batched sweep: the rough-surface geometry (Drude metal below
+    a Gaussian-correlated interface at mid height) on an nx x ny x nz grid,
+    CPML `depth` on all faces, a p-polarised TSF plane wave at theta_deg from
+    the normal travelling down onto the metal (bottom face skipped) with a
+    Gaussian pulse centred at lam_center.  No far-field box."""
+    os.makedirs(folder, exist_ok=True)
+    nx, ny, nz = shape
+    h = rough_height(nx, rms_nm * 1e-9 / dx, corr_nm * 1e-9 / dx, seed, ny=ny)
+    zc = nz / 2.0 + h
+    k = np.arange(nz) + 0.5
+    ids = (k[None, None, :] < zc[:, :, None]).astype(np.int32)
+    write_vtk_ids(os.path.join(folder, "surface.vtk"), ids)
+    with open(os.path.join(folder, "metal.txt"), "w") as f:
+        f.write("8 0 0 0 1 1 1 %s\n" % DRUDE_SILVER)
+    dt = _dt(dx, dx, dx, 1.0)
+    write_tsf_source(os.path.join(folder, "pulse.txt"), gaussian_sine(nsteps, dt, lam_center, 4, delay_widths=2.0))
+    b0 = [depth + tsf_margin] * 3
+    b1 = [n - depth - tsf_margin - 1 for n in shape]
+    theta = math.pi - math.radians(theta_deg)
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (nx, ny, nz, dx, dx, dx), "COMP\n%d\n\n" % nsteps,
+             "MATMODE_CHECK\n1\n\n",
+             "BOUNDARY_ALL\ncpml %d 3 -1 0 1\n\n" % depth,
+             "MEDIUM_VECTOR\nmetal.txt\n\n", "MEDIUM_VTK_DATA\nsurface.vtk\n\n",
+             "SOURCE_TSF\n%d %d %d %d %d %d %.17g 0 %.17g 0 pulse.txt\n\n" % (*b0, *b1, theta, math.pi / 2),
+             "TSF_SKIP\nk0\n\n"]
+    path = os.path.join(folder, "sweep_run.par")
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path
+
+
+def sweep(folder, nruns=64, seed=11, nsteps=2000, edges=SWEEP_EDGES):
+    """Write the runs of a seeded sweep into folder/runNN; returns the
+    parameter-file paths and the per-run parameters."""
+    params = sweep_params(nruns, seed, edges)
+    paths = [sweep_run(os.path.join(folder, "run%02d" % r), nsteps=nsteps, **p) for r, p in enumerate(params)]
+    return paths, params
 
 
 def small_case(folder, kind, n=48, nsteps=160, seed=7):

@@ -116,6 +116,26 @@ def test_sphere_scatter_reduced_against_oracle(tmp_path):
     assert np.array_equal(mo, ms) and np.array_equal(bo, bs)
 
 
+def test_rough_surface_reduced_against_oracle(tmp_path):
+    """The rough-surface workload (Drude metal below a rough interface, oblique
+    p-polarised TSF plane wave, CPML) at 64^3 with CPML 8, 400 steps; maps
+    bit-exact.  The FP32-storage oracle (the reference's own update order with
+    float state) is far less accurate here than the engine, whose ADE state is
+    kept in increment form; both errors against FP64 are printed."""
+    par = cases.rough_surface(str(tmp_path / "c2"), n=64, nsteps=400, depth=8, nfff=False)
+    o, s = run_both(par)
+    ref, got = o.fields(), s.fields()
+    errs = field_errors(got, ref)
+    o32 = Oracle(par, "f32")
+    o32.step()
+    f32 = field_errors(o32.fields(), ref)
+    print("rough surface 64^3: GPU vs FP64", errs, "FP32-storage oracle vs FP64", f32)
+    assert max(errs.values()) <= TOL
+    mo, bo = o.maps()
+    ms, bs = s.maps()
+    assert np.array_equal(mo, ms) and np.array_equal(bo, bs)
+
+
 def test_running_dft_against_oracle(tmp_path):
     """Running-DFT accumulators on all six NFFF box faces at two wavelengths."""
     par = dft_case(str(tmp_path / "dft"))
