@@ -53,7 +53,7 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     OFF o = ((OFF)i + (OFF)px * j + (OFF)g.plane * (k0 + 1)) * 4;
     OFF oc = ((OFF)i + (OFF)px * j + (OFF)g.plane * k0) * sizeof(ID);
 
-    const float rkx = g.rkhx[i], rky = g.rkhy[j];
+    const float rkx = gp(g.rkhx)[i], rky = gp(g.rkhy)[j];
     const bool iin = (i >= 1 && i < nx - 1), jin = (j >= 1 && j < ny - 1);
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
     const bool sx0 = F0.on && i < F0.depth - 1 && jin;
@@ -72,8 +72,8 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         const float exk1 = ldf(g.exg, o + dP), eyk1 = ldf(g.eyg, o + dP);
         const float ezj1 = ldf(g.ezg, o + dJ), exj1 = ldf(g.exg, o + dJ);
         const float ezi1 = ldf(g.ezg, o + 4), eyi1 = ldf(g.eyg, o + 4);
-        const GsvHCoef hc = g.htab[ldid<ID>(g.hcls, oc)];
-        const float rkz = g.rkhz[k];
+        const GsvHCoef hc = ldgs(g.htab + ldid<ID>(g.hcls, oc));
+        const float rkz = gp(g.rkhz)[k];
         float hx = ldn(g.hxg, o), hy = ldn(g.hyg, o), hz = ldn(g.hzg, o);
         const bool kin = (k >= 1 && k < nz - 1);
 
@@ -88,7 +88,7 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         if (kin) {
             if (sx0) {
                 const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
-                const float2 bc = F0.ch[i];
+                const float2 bc = ldg2(F0.ch + i);
                 const float2 ps = ld2(F0.ph, p);
                 const float a1 = bc.x * ps.x + bc.y * (eyi1 - eyk);
                 const float a2 = bc.x * ps.y + bc.y * (ezi1 - ezk);
@@ -98,7 +98,7 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             }
             if (sxn) {
                 const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + irn) * 8;
-                const float2 bc = F1.ch[irn];
+                const float2 bc = ldg2(F1.ch + irn);
                 const float2 ps = ld2(F1.ph, p);
                 const float a1 = bc.x * ps.x + bc.y * (eyi1 - eyk);
                 const float a2 = bc.x * ps.y + bc.y * (ezi1 - ezk);
@@ -108,7 +108,7 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             }
             if (sy0) {
                 const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
-                const float2 bc = F2.ch[j];
+                const float2 bc = ldg2(F2.ch + j);
                 const float2 ps = ld2(F2.ph, p);
                 const float a1 = bc.x * ps.x + bc.y * (ezj1 - ezk);
                 const float a2 = bc.x * ps.y + bc.y * (exj1 - exk);
@@ -118,7 +118,7 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             }
             if (syn) {
                 const OFF p = (OFF)(((unsigned)k * F3.depth + jrn) * px + i) * 8;
-                const float2 bc = F3.ch[jrn];
+                const float2 bc = ldg2(F3.ch + jrn);
                 const float2 ps = ld2(F3.ph, p);
                 const float a1 = bc.x * ps.x + bc.y * (ezj1 - ezk);
                 const float a2 = bc.x * ps.y + bc.y * (exj1 - exk);
@@ -130,7 +130,7 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         if (ijin) {
             if (F4.on && k < F4.depth - 1) {
                 const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
-                const float2 bc = F4.ch[k];
+                const float2 bc = ldg2(F4.ch + k);
                 const float2 ps = ld2(F4.ph, p);
                 const float a1 = bc.x * ps.x + bc.y * (exk1 - exk);
                 const float a2 = bc.x * ps.y + bc.y * (eyk1 - eyk);
@@ -141,7 +141,7 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             if (F5.on && k >= nz - F5.depth + 1 && k < nz - 1) {
                 const int kr = k - (nz - F5.depth);
                 const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
-                const float2 bc = F5.ch[kr];
+                const float2 bc = ldg2(F5.ch + kr);
                 const float2 ps = ld2(F5.ph, p);
                 const float a1 = bc.x * ps.x + bc.y * (exk1 - exk);
                 const float a2 = bc.x * ps.y + bc.y * (eyk1 - eyk);
@@ -188,7 +188,7 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
          float rkx, float rky, float rkz)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
-    const GsvAde &m = g.adem[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
+    const GSV_GLOBAL GsvAde &m = gp(g.adem)[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
     const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
     const float vexp = ex, veyp = ey, vezp = ez;
     const float expv = ldsn(ad, la, 0), eypv = ldsn(ad, la, 256), ezpv = ldsn(ad, la, 512);
@@ -220,13 +220,13 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
     if (F0.on && i > 1 && i < F0.depth) {
         const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
-        const float4 bc = F0.ce[i];
+        const float4 bc = ldg4(F0.ce + i);
         const float2 ps = ld2(F0.pe, p);
         st2(F0.pe, p, bc.x * ps.x + bc.z * (hz - hzi) / g.dy, bc.x * ps.y + bc.z * (hy - hyi) / g.dz);
     }
     if (F2.on && j > 1 && j < F2.depth) {
         const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
-        const float4 bc = F2.ce[j];
+        const float4 bc = ldg4(F2.ce + j);
         const float2 ps = ld2(F2.pe, p);
         const float a1 = bc.x * ps.x + bc.z * (hz - hzj) / g.dz;
         const float a2 = bc.x * ps.y + bc.z * (hx - hxj) / g.dx;
@@ -236,7 +236,7 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     }
     if (F4.on && k > 1 && k < F4.depth) {
         const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
-        const float4 bc = F4.ce[k];
+        const float4 bc = ldg4(F4.ce + k);
         const float2 ps = ld2(F4.pe, p);
         const float a1 = bc.x * ps.x + bc.z * (hx - hxkm1) / g.dy;
         const float a2 = bc.x * ps.y + bc.z * (hy - hykm1) / g.dx;
@@ -247,7 +247,7 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     if (F1.on && i >= nx - F1.depth + 1) {
         const int ir = i - (nx - F1.depth);
         const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8;
-        const float4 bc = F1.ce[ir];
+        const float4 bc = ldg4(F1.ce + ir);
         const float2 ps = ld2(F1.pe, p);
         const float a1 = bc.x * ps.x + bc.z * (hz - hzi) / g.dy;
         const float a2 = bc.x * ps.y + bc.z * (hy - hyi) / g.dz;
@@ -258,7 +258,7 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     if (F3.on && j >= ny - F3.depth + 1) {
         const int jr = j - (ny - F3.depth);
         const OFF p = (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8;
-        const float4 bc = F3.ce[jr];
+        const float4 bc = ldg4(F3.ce + jr);
         const float2 ps = ld2(F3.pe, p);
         const float a1 = bc.x * ps.x + bc.z * (hz - hzj) / g.dx;
         const float a2 = bc.x * ps.y + bc.z * (hx - hxj) / g.dz;
@@ -269,7 +269,7 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     if (F5.on && k >= nz - F5.depth + 1) {
         const int kr = k - (nz - F5.depth);
         const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
-        const float4 bc = F5.ce[kr];
+        const float4 bc = ldg4(F5.ce + kr);
         const float2 ps = ld2(F5.pe, p);
         const float a1 = bc.x * ps.x + bc.z * (hx - hxkm1) / g.dy;
         const float a2 = bc.x * ps.y + bc.z * (hy - hykm1) / g.dx;
@@ -318,7 +318,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     const OFF la = ADE ? ((OFF)j * g.pxa * GSV_ADE_D + (OFF)(i >> 6) * 64 * GSV_ADE_D + (i & 63)) * 4 : 0;
     const OFF lp = ADE == 2 ? ((OFF)j * g.pxa * GSV_ADE_P + (OFF)(i >> 6) * 64 * GSV_ADE_P + (i & 63)) * 4 : 0;
 
-    const float rkx = g.rkex[i], rky = g.rkey[j];
+    const float rkx = gp(g.rkex)[i], rky = gp(g.rkey)[j];
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
     const bool sx0 = F0.on && i >= 1 && i < F0.depth;
     const int irn = i - (nx - F1.depth);
@@ -333,10 +333,10 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         const float hx = ldn(g.hxg, o), hy = ldn(g.hyg, o), hz = ldn(g.hzg, o);
         const float hzj = ldf(g.hzg, o - dJ), hxj = ldf(g.hxg, o - dJ);
         const float hzi = ldf(g.hzg, o - 4), hyi = ldf(g.hyg, o - 4);
-        const float rkz = g.rkez[k];
-        GsvECoef ec = g.etab[ldid<ID>(g.ecls, oc)];
+        const float rkz = gp(g.rkez)[k];
+        GsvECoef ec = ldgs(g.etab + ldid<ID>(g.ecls, oc));
         if (PC && (ec.flags & GSV_E_PERCELL))
-            ec = g.pctab[(i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))];
+            ec = ldgs(g.pctab + ((i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))));
         float ex = ldn(g.exg, o), ey = ldn(g.eyg, o), ez = ldn(g.ezg, o);
         const unsigned fl = ec.flags;
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
@@ -362,7 +362,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         /* CPML E psi and correction, src3d/boundary.c:691-887 */
         if (sx0) {
             const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
-            const float4 bc = F0.ce[i];
+            const float4 bc = ldg4(F0.ce + i);
             const float2 ps = ld2(F0.pe, p);
             const float a1 = bc.x * ps.x + bc.y * (hz - hzi);
             const float a2 = bc.x * ps.y + bc.y * (hy - hyi);
@@ -374,7 +374,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         }
         if (sy0) {
             const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
-            const float4 bc = F2.ce[j];
+            const float4 bc = ldg4(F2.ce + j);
             const float2 ps = ld2(F2.pe, p);
             const float a1 = bc.x * ps.x + bc.y * (hz - hzj);
             const float a2 = bc.x * ps.y + bc.y * (hx - hxj);
@@ -386,7 +386,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         }
         if (F4.on && k >= 1 && k < F4.depth) {
             const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
-            const float4 bc = F4.ce[k];
+            const float4 bc = ldg4(F4.ce + k);
             const float2 ps = ld2(F4.pe, p);
             const float a1 = bc.x * ps.x + bc.y * (hx - hxkm1);
             const float a2 = bc.x * ps.y + bc.y * (hy - hykm1);
@@ -399,7 +399,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         }
         if (sxn) {
             const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + irn) * 8;
-            const float4 bc = F1.ce[irn];
+            const float4 bc = ldg4(F1.ce + irn);
             const float2 ps = ld2(F1.pe, p);
             const float a1 = bc.x * ps.x + bc.y * (hz - hzi);
             const float a2 = bc.x * ps.y + bc.y * (hy - hyi);
@@ -411,7 +411,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         }
         if (syn) {
             const OFF p = (OFF)(((unsigned)k * F3.depth + jrn) * px + i) * 8;
-            const float4 bc = F3.ce[jrn];
+            const float4 bc = ldg4(F3.ce + jrn);
             const float2 ps = ld2(F3.pe, p);
             const float a1 = bc.x * ps.x + bc.y * (hz - hzj);
             const float a2 = bc.x * ps.y + bc.y * (hx - hxj);
@@ -424,7 +424,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         if (F5.on && k >= nz - F5.depth + 1) {
             const int kr = k - (nz - F5.depth);
             const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
-            const float4 bc = F5.ce[kr];
+            const float4 bc = ldg4(F5.ce + kr);
             const float2 ps = ld2(F5.pe, p);
             const float a1 = bc.x * ps.x + bc.y * (hx - hxkm1);
             const float a2 = bc.x * ps.y + bc.y * (hy - hykm1);
@@ -465,7 +465,7 @@ line_val(const double *data, int res, double x)
     /* cubic interpolation of the 1-D line, src3d/dline.c:89-112; samples outside the line are 0 */
     const int l = (int)floor(x);
     const double a = x - (double)l;
-#define DV(q) (((q) >= 0 && (q) < res) ? data[q] : 0.0)
+#define DV(q) (((q) >= 0 && (q) < res) ? gp(data)[q] : 0.0)
     if (x >= 1 && x < (res - 1)) {
         double w1 = a + 1, w2 = a, w3 = 1 - a, w4 = 2 - a;
         w1 = 4 - 8 * w1 + 5 * w1 * w1 - w1 * w1 * w1;
@@ -491,7 +491,7 @@ __device__ __forceinline__ void
 tsf_line_step(const GsvTsf &t, int step)
 {
     const int res = t.res;
-    double *e = t.je, *h = t.jh, *b = t.jb;
+    GSV_GLOBAL double *e = gp(t.je), *h = gp(t.jh), *b = gp(t.jb);
     for (int q = threadIdx.x + 1; q < res; q += blockDim.x)
         e[q] += t.multe * (h[q - 1] - h[q]);
     __syncthreads();
@@ -502,7 +502,7 @@ tsf_line_step(const GsvTsf &t, int step)
         h[res - 1] = b[7] + t.coef * (h[res - 2] - b[6]);
         b[0] = e[0]; b[1] = e[1]; b[2] = h[0]; b[3] = h[1];
         b[4] = e[res - 1]; b[5] = e[res - 2]; b[6] = h[res - 1]; b[7] = h[res - 2];
-        e[0] = (step < t.ndata) ? t.wave[step] : 0.0;
+        e[0] = (step < t.ndata) ? gp(t.wave)[step] : 0.0;
     }
     __syncthreads();
     for (int q = threadIdx.x; q < res - 1; q += blockDim.x)
@@ -525,7 +525,7 @@ tsf_h_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
     else { na = t.in - t.i0 + 1; nb = t.jn - t.j0 + 1; }
     if (a >= na || b >= nb)
         return;
-    const float cf = t.coefH[axis * 2 + far][a + (long long)na * b];
+    const float cf = gp(t.coefH[axis * 2 + far])[a + (long long)na * b];
     if (cf == 0.f)
         return;
     const long long P = g.plane;
@@ -537,8 +537,8 @@ tsf_h_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
         const int iw = far ? t.in : t.i0 - 1;
         const long long c = iw + (long long)px * j + P * k;
         const float s = far ? 1.f : -1.f;
-        if (k < t.kn) g.hy[c] += s * (float)(cf * t.gez * ve);
-        if (j < t.jn) g.hz[c] -= s * (float)(cf * t.gey * ve);
+        if (k < t.kn) gp(g.hy)[c] += s * (float)(cf * t.gez * ve);
+        if (j < t.jn) gp(g.hz)[c] -= s * (float)(cf * t.gey * ve);
     } else if (axis == 1) {
         const int i = t.i0 + a, k = t.k0 + b;
         const int jp = far ? t.jn : t.j0;
@@ -546,8 +546,8 @@ tsf_h_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
         const int jw = far ? t.jn : t.j0 - 1;
         const long long c = i + (long long)px * jw + P * k;
         const float s = far ? 1.f : -1.f;
-        if (k < t.kn) g.hx[c] -= s * (float)(cf * t.gez * ve);
-        if (i < t.in) g.hz[c] += s * (float)(cf * t.gex * ve);
+        if (k < t.kn) gp(g.hx)[c] -= s * (float)(cf * t.gez * ve);
+        if (i < t.in) gp(g.hz)[c] += s * (float)(cf * t.gex * ve);
     } else {
         const int i = t.i0 + a, j = t.j0 + b;
         const int kp = far ? t.kn : t.k0;
@@ -555,8 +555,8 @@ tsf_h_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
         const int kw = far ? t.kn : t.k0 - 1;
         const long long c = i + (long long)px * j + P * kw;
         const float s = far ? 1.f : -1.f;
-        if (j < t.jn) g.hx[c] += s * (float)(cf * t.gey * ve);
-        if (i < t.in) g.hy[c] -= s * (float)(cf * t.gex * ve);
+        if (j < t.jn) gp(g.hx)[c] += s * (float)(cf * t.gey * ve);
+        if (i < t.in) gp(g.hy)[c] -= s * (float)(cf * t.gex * ve);
     }
 }
 
@@ -570,7 +570,7 @@ tsf_e_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
     else { na = t.in - t.i0 + 1; nb = t.jn - t.j0 + 1; }
     if (a >= na || b >= nb)
         return;
-    const float cf = t.coefE[axis * 2 + far][a + (long long)na * b];
+    const float cf = gp(t.coefE[axis * 2 + far])[a + (long long)na * b];
     if (cf == 0.f)
         return;
     const long long P = g.plane;
@@ -581,22 +581,22 @@ tsf_e_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
         const int ip = far ? t.in : t.i0 - 1;
         const double vh = line_val(t.jh, t.res, tsf_d(t, ip, j, k));
         const long long c = (far ? t.in : t.i0) + (long long)px * j + P * k;
-        if (j < t.jn) g.ey[c] += s * (float)(cf * t.ghz * vh);
-        if (k < t.kn) g.ez[c] -= s * (float)(cf * t.ghy * vh);
+        if (j < t.jn) gp(g.ey)[c] += s * (float)(cf * t.ghz * vh);
+        if (k < t.kn) gp(g.ez)[c] -= s * (float)(cf * t.ghy * vh);
     } else if (axis == 1) {
         const int i = t.i0 + a, k = t.k0 + b;
         const int jp = far ? t.jn : t.j0 - 1;
         const double vh = line_val(t.jh, t.res, tsf_d(t, i, jp, k));
         const long long c = i + (long long)px * (far ? t.jn : t.j0) + P * k;
-        if (i < t.in) g.ex[c] -= s * (float)(cf * t.ghz * vh);
-        if (k < t.kn) g.ez[c] += s * (float)(cf * t.ghx * vh);
+        if (i < t.in) gp(g.ex)[c] -= s * (float)(cf * t.ghz * vh);
+        if (k < t.kn) gp(g.ez)[c] += s * (float)(cf * t.ghx * vh);
     } else {
         const int i = t.i0 + a, j = t.j0 + b;
         const int kp = far ? t.kn : t.k0 - 1;
         const double vh = line_val(t.jh, t.res, tsf_d(t, i, j, kp));
         const long long c = i + (long long)px * j + P * (far ? t.kn : t.k0);
-        if (i < t.in) g.ex[c] += s * (float)(cf * t.ghy * vh);
-        if (j < t.jn) g.ey[c] -= s * (float)(cf * t.ghx * vh);
+        if (i < t.in) gp(g.ex)[c] += s * (float)(cf * t.ghy * vh);
+        if (j < t.jn) gp(g.ey)[c] -= s * (float)(cf * t.ghx * vh);
     }
 }
 
