@@ -122,6 +122,7 @@ batch_reject(const Engine *E)
     if (E->has_per) return "periodic boundaries";
     if (E->has_ff) return "time-domain far field";
     if (E->has_dft) return "running DFT";
+    if (E->has_flux) return "flux spectra";
     if (!E->sums.empty()) return "box sums";
     if (E->npts) return "point outputs";
     if (E->adj) return "adjoint state";
@@ -159,7 +160,7 @@ gsv_engine_batch_new(void *const *engines, int n)
      * cells (or without critical-point terms) never takes it */
     b->ade = 0;
     for (Engine *E : b->eng)
-        b->ade = std::max(b->ade, E->g.ade ? (E->g.ade_poles ? 2 : 1) : 0);
+        b->ade = std::max(b->ade, E->g.ade ? (E->g.ade_poles ? E->g.ade_poles : 1) : 0);
 
     bool ok = hipSetDevice(b->eng[0]->dev) == hipSuccess && hipStreamCreate(&b->st) == hipSuccess;
     std::vector<GsvGrid> grids(n);
@@ -184,8 +185,8 @@ gsv_engine_batch_new(void *const *engines, int n)
         for (int r : truns) {
             const Engine *E = b->eng[r];
             const GsvTsf &T = E->tsf;
-            const int na = axis == 0 ? T.jn - T.j0 + 1 : T.in - T.i0 + 1;
-            const int nb = axis == 2 ? T.jn - T.j0 + 1 : T.kn - T.k0 + 1;
+            int na, nb;
+            tsf_face_extent(T, E->g.nx, E->g.ny, axis, &na, &nb);
             for (int far = 0; far < 2; far++) {
                 if (!E->tsf_face_on[2 * axis + far])
                     continue;

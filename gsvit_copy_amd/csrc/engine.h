@@ -68,12 +68,12 @@ typedef struct {
     float c0, c2, c3;  /* E update: c0 curl / d + ... + c2 E^(n-1) + c3 E^n */
     float cj;          /* E-update coefficient of the Drude J: c1 a0 */
     float dj, da;      /* Drude: J' = dj J + da (E^(n-1) + 2 E^n + E^(n+1)); dj = -a0, da = a2 */
-    float pcj[2];      /* E-update coefficients of a pole's J and P: c1 bp0, -c1 s */
-    float pcp[2];
-    float pj[2], ps[2];           /* pole J recursion coefficients: -bp0, s */
-    float b2[2], b3[2], b4[2];    /* pole E coefficients: bp2, bp3, bp4 */
+    float pcj[3];      /* E-update coefficients of a pole's J and P: c1 bp0, -c1 s */
+    float pcp[3];
+    float pj[3], ps[3];           /* pole J recursion coefficients: -bp0, s */
+    float b2[3], b3[3], b4[3];    /* pole E coefficients: bp2, bp3, bp4 */
     float cbm[3];    /* material cb used by the ADE branch's inline CPML, src3d/yee.c:952-1057 */
-    int poles;       /* nonzero if the critical-point terms are present */
+    int poles;       /* critical-point poles with nonzero terms: 0, 2 or 3 (the third has no reference counterpart) */
 } GsvAde;
 
 /* one CPML face on the device: grading coefficients per index along the
@@ -92,7 +92,7 @@ typedef struct {
  * multiple of 64), so one base per plane plus the lane's offset reaches
  * every state of a cell through immediate offsets (state * 256 bytes) */
 #define GSV_ADE_D 6      /* E(n-1) x y z, Drude J x y z */
-#define GSV_ADE_P 12     /* P0 x y z, P1 x y z, J0 x y z, J1 x y z (critical points) */
+#define GSV_ADE_P(np) (6 * (np))   /* critical points: P of each pole x y z, then J of each pole x y z */
 
 typedef struct {
     int nx, ny, nz;
@@ -109,16 +109,17 @@ typedef struct {
     const float *rkhx, *rkhy, *rkhz;      /* 1/kappa on the H grid, per axis */
     GsvCpmlFace f[6];                     /* x0, xn, y0, yn, z0, zn */
     float dx, dy, dz;
+    int perx, pery;         /* domain-periodic x / y: the far plane's neighbour is the near plane */
     /* per-cell E coefficients of the design region: cells whose class carries
      * GSV_E_PERCELL read pctab[(i-pc_i0) + pc_nx*((j-pc_j0) + pc_ny*(k-pc_k0))] */
     int pc_i0, pc_j0, pc_k0, pc_nx, pc_ny, pc_nz;
     const GsvECoef *pctab;
     /* ADE */
-    int ade, ade_poles;
+    int ade, ade_poles;     /* ADE cells present; poles of the widest ADE material (0, 2 or 3) */
     const GsvAde *adem;     /* ADE material table in device memory */
     int pxa;                /* x pitch of the ADE state blocks */
     float *ade_d;           /* Drude state blocks, GSV_ADE_D states */
-    float *ade_p;           /* critical-point state blocks, GSV_ADE_P states (NULL without poles) */
+    float *ade_p;           /* critical-point state blocks, GSV_ADE_P(ade_poles) states (NULL without poles) */
     long long ade_plane_d;  /* floats per z plane of ade_d = ny * pxa * GSV_ADE_D */
     long long ade_plane_p;  /* floats per z plane of ade_p */
 } GsvGrid;

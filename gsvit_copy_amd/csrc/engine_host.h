@@ -73,6 +73,9 @@ struct Engine {
     int has_dft;
     GsvDft dft;
     int dft_first, dft_pending;   /* pending ring samples: steps dft_first .. + dft_pending - 1 */
+    int has_flux;                 /* FLUX_DFT planes: the same ring scheme, face code 6 */
+    GsvDft flx;
+    int flx_first, flx_pending;
     /* box sums */
     std::vector<GsvBoxSum> sums;
     /* timing of the two volume kernels */

@@ -290,7 +290,7 @@ build_tables(Engine *E)
         a.cj = (float)(m->ade_c1 * m->ade_a0);
         a.dj = (float)(-m->ade_a0);
         a.da = (float)m->ade_a2;
-        for (int q = 0; q < 2; q++) {
+        for (int q = 0; q < 3; q++) {
             const double sq = m->ade_bp0[q] + m->ade_bp1[q] - 1;
             a.pcj[q] = (float)(m->ade_c1 * m->ade_bp0[q]);
             a.pcp[q] = (float)(-m->ade_c1 * sq);
@@ -301,10 +301,13 @@ build_tables(Engine *E)
         a.cbm[0] = (float)((dt / m->epsilon / mp->d->dx) / (1 + m->sigma * dt / 2 / m->epsilon));
         a.cbm[1] = (float)((dt / m->epsilon / mp->d->dy) / (1 + m->sigma * dt / 2 / m->epsilon));
         a.cbm[2] = (float)((dt / m->epsilon / mp->d->dz) / (1 + m->sigma * dt / 2 / m->epsilon));
-        a.poles = (m->cp3_a[0] != 0 || m->cp3_a[1] != 0 || m->ade_bp0[0] != -1 || m->ade_bp0[1] != -1
-                   || m->ade_bp1[0] != 2 || m->ade_bp1[1] != 2);
-        if (a.poles)
-            E->g.ade_poles = 1;
+        /* a pole with zero amplitude and zero frequency and damping contributes nothing */
+        a.poles = 0;
+        for (int q = 0; q < 3; q++)
+            if (m->cp3_a[q] != 0 || m->ade_bp0[q] != -1 || m->ade_bp1[q] != 2)
+                a.poles = q < 2 ? 2 : 3;
+        if (a.poles > E->g.ade_poles)
+            E->g.ade_poles = a.poles;
         E->ade_slot_of_mat[n] = nslot++;
         E->g.ade = 1;
     }
@@ -385,6 +388,8 @@ sv_gpu_new(SvPool *mp, SvSet *set)
     E->g.nz = set->sp.zres;
     E->g.px = (set->sp.xres + 15) / 16 * 16;
     E->g.plane = (long long)E->g.px * E->g.ny;
+    E->g.perx = set->sb.bx0 == SV_BOUNDARY_PERIODIC;
+    E->g.pery = set->sb.by0 == SV_BOUNDARY_PERIODIC;
     E->g.dx = (float)set->sp.dx;
     E->g.dy = (float)set->sp.dy;
     E->g.dz = (float)set->sp.dz;
@@ -394,7 +399,9 @@ sv_gpu_new(SvPool *mp, SvSet *set)
     E->kchunk_h = E->kchunk_e = 16;
     if (mp->src->tsf) {
         SvSourceTSF *t = mp->src->tsf;
-        if (t->box_in > set->sp.xres - 2 || t->box_jn > set->sp.yres - 2 || t->box_kn > set->sp.zres - 2) {
+        const int perx = set->sb.bx0 == SV_BOUNDARY_PERIODIC, pery = set->sb.by0 == SV_BOUNDARY_PERIODIC;
+        if ((!perx && t->box_in > set->sp.xres - 2) || (!pery && t->box_jn > set->sp.yres - 2)
+            || t->box_kn > set->sp.zres - 2) {
             fprintf(stderr, "Error: the TSF box must end at least two cells before the far faces\n");
             delete E;
             return NULL;
@@ -577,9 +584,7 @@ setup_tsf(Engine *E)
     int skips[6] = { t->skip_i0, t->skip_in, t->skip_j0, t->skip_jn, t->skip_k0, t->skip_kn };
     for (int f = 0; f < 6; f++) {
         int axis = f / 2, far = f % 2, na, nb;
-        if (axis == 0) { na = T.jn - T.j0 + 1; nb = T.kn - T.k0 + 1; }
-        else if (axis == 1) { na = T.in - T.i0 + 1; nb = T.kn - T.k0 + 1; }
-        else { na = T.in - T.i0 + 1; nb = T.jn - T.j0 + 1; }
+        tsf_face_extent(T, E->g.nx, E->g.ny, axis, &na, &nb);
         std::vector<float> ce((size_t)na * nb, 0.f), chh((size_t)na * nb, 0.f);
         for (int b = 0; b < nb; b++)
             for (int a = 0; a < na; a++) {
@@ -597,9 +602,14 @@ setup_tsf(Engine *E)
                 chh[(size_t)a + (size_t)na * b] = (float)hcor;
             }
         E->tsf_face_on[f] = 0;
+        long active = 0;
         for (size_t q = 0; q < ce.size(); q++)
-            if (ce[q] != 0.f || chh[q] != 0.f)
+            if (ce[q] != 0.f || chh[q] != 0.f) {
                 E->tsf_face_on[f] = 1;
+                active++;
+            }
+        if (set->sc.verbose > 1)
+            printf("TSF face %d: %d x %d cells, %ld active\n", f, na, nb, active);
         float *dce, *dch;
         if (dupload(E, &dce, ce.data(), ce.size()) || dupload(E, &dch, chh.data(), chh.size()))
             return 1;
@@ -749,6 +759,14 @@ setup_liao_periodic(Engine *E)
     return 0;
 }
 
+/* ring depth of a running DFT: the flush tabulates T * nw * 4 phases in LDS (<= 48 KiB) */
+static int
+dft_ring_depth(int nw)
+{
+    int T = 1536 / nw;
+    return T > 32 ? 32 : (T < 1 ? 1 : T);
+}
+
 static int
 setup_farfield(Engine *E)
 {
@@ -833,10 +851,7 @@ setup_farfield(Engine *E)
         if (dmalloc(E, &D.acc, cnt))
             return 1;
         CK(hipMemsetAsync(D.acc, 0, cnt * 8, E->st));
-        /* ring depth: the flush tabulates T * nw * 4 phases in LDS (<= 48 KiB) */
-        D.T = 1536 / D.nw;
-        if (D.T > 32) D.T = 32;
-        if (D.T < 1) D.T = 1;
+        D.T = dft_ring_depth(D.nw);
         if (dmalloc(E, &D.ring, (size_t)D.T * 4 * D.ncells))
             return 1;
         E->dft_first = 0;
@@ -845,16 +860,79 @@ setup_farfield(Engine *E)
     return 0;
 }
 
-/* fold the pending DFT samples into the accumulators */
+/* FLUX_DFT planes: one face-6 cell per (plane, j, i) of the rectangle */
+static int
+setup_flux(Engine *E)
+{
+    SvSet *set = E->set;
+    SvPool *mp = E->mp;
+    const SvSetFlux *x = &set->sx;
+    memset(&E->flx, 0, sizeof(E->flx));
+    E->has_flux = x->nplanes > 0;
+    if (!E->has_flux)
+        return 0;
+    GsvDft &D = E->flx;
+    std::vector<long long> cells;
+    for (int p = 0; p < x->nplanes; p++)
+        for (int j = x->j0; j < x->j1; j++)
+            for (int i = x->i0; i < x->i1; i++)
+                cells.push_back(cell_index(E->g, i, j, x->k[p]));
+    std::vector<int> faces(cells.size(), 6);
+    D.ncells = (long long)cells.size();
+    D.nw = mp->farfield->nflux;
+    D.dt = set->plan.dt;
+    long long *pc;
+    int *pf;
+    double *po;
+    if (dupload(E, &pc, cells.data(), cells.size()) || dupload(E, &pf, faces.data(), faces.size())
+        || dupload(E, &po, mp->farfield->flux_omega, D.nw))
+        return 1;
+    D.cell = pc;
+    D.face = pf;
+    D.omega = po;
+    size_t cnt = (size_t)D.ncells * D.nw * 8;
+    if (dmalloc(E, &D.acc, cnt))
+        return 1;
+    CK(hipMemsetAsync(D.acc, 0, cnt * 8, E->st));
+    D.T = dft_ring_depth(D.nw);
+    if (dmalloc(E, &D.ring, (size_t)D.T * 4 * D.ncells))
+        return 1;
+    E->flx_first = 0;
+    E->flx_pending = 0;
+    return 0;
+}
+
+/* fold the pending samples of one running DFT into its accumulators */
+static int
+ring_flush(Engine *E, int on, const GsvDft &D, int &first, int &pending)
+{
+    if (on && pending > 0) {
+        CK(gsv_launch_dft_flush(&D, first, pending, E->st));
+        first += pending;
+        pending = 0;
+    }
+    return 0;
+}
+
+/* record this step's samples (folding first if the ring is not contiguous with them) */
+static int
+ring_sample(Engine *E, const GsvDft &D, int &first, int &pending, int step)
+{
+    if (pending > 0 && step != first + pending && ring_flush(E, 1, D, first, pending))
+        return 1;
+    if (pending == 0)
+        first = step;
+    CK(gsv_launch_dft_sample(&E->g, &D, step % D.T, E->st));
+    if (++pending == D.T && ring_flush(E, 1, D, first, pending))
+        return 1;
+    return 0;
+}
+
 static int
 dft_flush(Engine *E)
 {
-    if (E->has_dft && E->dft_pending > 0) {
-        CK(gsv_launch_dft_flush(&E->dft, E->dft_first, E->dft_pending, E->st));
-        E->dft_first += E->dft_pending;
-        E->dft_pending = 0;
-    }
-    return 0;
+    return ring_flush(E, E->has_dft, E->dft, E->dft_first, E->dft_pending)
+           || ring_flush(E, E->has_flux, E->flx, E->flx_first, E->flx_pending);
 }
 
 /* OUT_SUM boxes: the voxel selection (permittivity within EPSILON_0/100 of
@@ -955,7 +1033,7 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
             return 1;
         CK(hipMemsetAsync(g.ade_d, 0, n * 4, E->st));
         if (g.ade_poles) {
-            g.ade_plane_p = (long long)g.ny * g.pxa * GSV_ADE_P;
+            g.ade_plane_p = (long long)g.ny * g.pxa * GSV_ADE_P(g.ade_poles);
             n = (size_t)g.ade_plane_p * g.nz;
             if (dmalloc(E, &g.ade_p, n))
                 return 1;
@@ -963,7 +1041,7 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
         }
     }
     if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E)
-        || setup_sums(E))
+        || setup_flux(E) || setup_sums(E))
         return 1;
     for (int q = 0; q < 4; q++)
         CK(hipEventCreate(&E->ev[q]));
@@ -1064,11 +1142,15 @@ fetch_aux(Engine *E, SvPool *mp, SvSet *set)
         for (int n = 0; n < E->ff.nrs; n++)
             memcpy(mp->farfield->rpoints[n].ex, b.data() + (size_t)n * 3 * nd, 3 * (size_t)nd * 8);
     }
-    if (E->has_dft) {
+    if (E->has_dft || E->has_flux) {
         if (dft_flush(E))
             return 1;
-        size_t cnt = (size_t)E->dft.ncells * E->dft.nw * 8;
-        CK(hipMemcpyAsync(mp->farfield->dft_data, E->dft.acc, cnt * 8, hipMemcpyDeviceToHost, E->st));
+        if (E->has_dft)
+            CK(hipMemcpyAsync(mp->farfield->dft_data, E->dft.acc, (size_t)E->dft.ncells * E->dft.nw * 64,
+                              hipMemcpyDeviceToHost, E->st));
+        if (E->has_flux)
+            CK(hipMemcpyAsync(mp->farfield->flux_data, E->flx.acc, (size_t)E->flx.ncells * E->flx.nw * 64,
+                              hipMemcpyDeviceToHost, E->st));
         CK(hipStreamSynchronize(E->st));
     }
     for (size_t n = 0; n < E->sums.size(); n++) {
@@ -1231,16 +1313,10 @@ sv_pool_gpu_apply_farfield(SvPool *mp, SvSet *set)
         return 1;
     if (E->has_ff)
         CK(gsv_launch_farfield_td(&E->g, &E->ff, set->sc.step_act, E->st));
-    if (E->has_dft) {
-        const int step = set->sc.step_act;
-        if (E->dft_pending > 0 && step != E->dft_first + E->dft_pending && dft_flush(E))
-            return 1;
-        if (E->dft_pending == 0)
-            E->dft_first = step;
-        CK(gsv_launch_dft_sample(&E->g, &E->dft, step % E->dft.T, E->st));
-        if (++E->dft_pending == E->dft.T && dft_flush(E))
-            return 1;
-    }
+    if (E->has_dft && ring_sample(E, E->dft, E->dft_first, E->dft_pending, set->sc.step_act))
+        return 1;
+    if (E->has_flux && ring_sample(E, E->flx, E->flx_first, E->flx_pending, set->sc.step_act))
+        return 1;
     return 0;
 }
 

@@ -47,6 +47,8 @@ int sv_tsf_face_active(const SvPool *mp, const SvSet *set, int i, int j, int k);
 /* far-field geometry helpers, src3d/farfield.c (dist, angle) */
 double sv_ff_dist(int x, int y, int z, int i, int j, int k);
 double sv_ff_angle(int i, int j, int k, int x, int y, int z, int face);
+/* FLUX_DFT spectra [plane][wavelength] from the accumulators (output.c) */
+void sv_flux_spectra(const SvPool *mp, const SvSet *set, double *out);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,20 @@ typedef struct {
     const float *coefH[6];
 } GsvTsf;
 
+/* cells of a TSF face along its two in-plane axes; a box that spans a
+ * periodic axis ends at the pool's last plane */
+static __host__ __device__ __forceinline__ void
+tsf_face_extent(const GsvTsf &t, int nx, int ny, int axis, int *na, int *nb)
+{
+    /* written as selects: the if/else-if form of this function was compiled
+     * for gfx950 (ROCm 7.2 hipcc, kernel-argument GsvTsf) into code that read
+     * k0 in place of j0 for axis 2 */
+    const int in = t.in < nx - 1 ? t.in : nx - 1, jn = t.jn < ny - 1 ? t.jn : ny - 1;
+    const int i0 = t.i0, j0 = t.j0, k0 = t.k0, kn = t.kn;
+    *na = (axis == 0) ? jn - j0 + 1 : in - i0 + 1;
+    *nb = (axis == 2) ? jn - j0 + 1 : kn - k0 + 1;
+}
+
 typedef struct {
     int n;                    /* number of point sources */
     int nsteps;
@@ -58,13 +72,16 @@ typedef struct {
 } GsvFarfieldTD;
 
 /* running DFT: each step the four tangential components of every face cell
+ * (face 0..5 = x0, xn, y0, yn, z0, zn of the NFFF box; face 6 = a FLUX_DFT
+ * plane cell: Ex, Ey and the means of Hx, Hy over the cell and the cell
+ * below)
  * are stored in FP32 into slot `step % T` of a sample ring; every T steps (and
  * before results are read) the pending samples are folded into the double
  * accumulators, so the accumulators move once per T steps instead of every step */
 typedef struct {
     long long ncells;
     const long long *cell;    /* cell index of each face cell */
-    const int *face;          /* 0/1 x faces, 2/3 y faces, 4/5 z faces */
+    const int *face;          /* 0/1 x faces, 2/3 y faces, 4/5 z faces, 6 flux plane */
     int nw;
     const double *omega;
     double *acc;              /* [ncells][nw][8] */

@@ -54,7 +54,12 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     OFF oc = ((OFF)i + (OFF)px * j + (OFF)g.plane * k0) * sizeof(ID);
 
     const float rkx = gp(g.rkhx)[i], rky = gp(g.rkhy)[j];
-    const bool iin = (i >= 1 && i < nx - 1), jin = (j >= 1 && j < ny - 1);
+    /* a periodic axis has no boundary planes: every column is interior, and the
+     * +x / +y neighbour of the far plane is the near plane */
+    const bool iin = (i >= 1 && i < nx - 1) || g.perx, jin = (j >= 1 && j < ny - 1) || g.pery;
+    const bool xe = i == nx - 1 && !g.perx, ye = j == ny - 1 && !g.pery;
+    const OFF dIp = (g.perx && i == nx - 1) ? (OFF)0 - (OFF)(nx - 1) * 4 : (OFF)4;
+    const OFF dJp = (g.pery && j == ny - 1) ? (OFF)0 - (OFF)(ny - 1) * dJ : dJ;
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
     const bool sx0 = F0.on && i < F0.depth - 1 && jin;
     const int irn = i - (nx - F1.depth);
@@ -70,18 +75,18 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
         const float ezk = ldf(g.ezg, o);
         const float exk1 = ldf(g.exg, o + dP), eyk1 = ldf(g.eyg, o + dP);
-        const float ezj1 = ldf(g.ezg, o + dJ), exj1 = ldf(g.exg, o + dJ);
-        const float ezi1 = ldf(g.ezg, o + 4), eyi1 = ldf(g.eyg, o + 4);
+        const float ezj1 = ldf(g.ezg, o + dJp), exj1 = ldf(g.exg, o + dJp);
+        const float ezi1 = ldf(g.ezg, o + dIp), eyi1 = ldf(g.eyg, o + dIp);
         const GsvHCoef hc = ldgs(g.htab + ldid<ID>(g.hcls, oc));
         const float rkz = gp(g.rkhz)[k];
         float hx = ldn(g.hxg, o), hy = ldn(g.hyg, o), hz = ldn(g.hzg, o);
         const bool kin = (k >= 1 && k < nz - 1);
 
-        if (!(j == ny - 1 || k == nz - 1))
+        if (!(ye || k == nz - 1))
             hx = hc.da * hx + hc.db[0] * ((eyk1 - eyk) * rkz - (ezj1 - ezk) * rky);
-        if (!(i == nx - 1 || k == nz - 1))
+        if (!(xe || k == nz - 1))
             hy = hc.da * hy + hc.db[1] * ((ezi1 - ezk) * rkx - (exk1 - exk) * rkz);
-        if (!(i == nx - 1 || j == ny - 1))
+        if (!(xe || ye))
             hz = hc.da * hz + hc.db[2] * ((exj1 - exk) * rky - (eyi1 - eyk) * rkx);
 
         /* CPML H psi and correction, src3d/boundary.c:347-518 */
@@ -181,7 +186,7 @@ k_hstep(GsvGrid g, int kchunk)
  * and their +1 neighbours.  ad/ap: this plane's ADE state blocks, la/lp: the
  * lane's offset in them; state q of a cell sits q * 256 bytes after its first
  * state. */
-template <bool POLES, typename OFF>
+template <int NP, typename OFF>
 __device__ __forceinline__ void
 ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, int k, unsigned fl, float &ex,
          float &ey, float &ez, float hx, float hy, float hz, float hxkm1, float hykm1, float hzi, float hyi, float hzj, float hxj,
@@ -190,31 +195,35 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
     const GSV_GLOBAL GsvAde &m = gp(g.adem)[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
     const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
+    const bool xi = i > 0 || g.perx, yi = j > 0 || g.pery;   /* not on a (non-periodic) near plane */
     const float vexp = ex, veyp = ey, vezp = ez;
     const float expv = ldsn(ad, la, 0), eypv = ldsn(ad, la, 256), ezpv = ldsn(ad, la, 512);
     const float jx = ldsn(ad, la, 768), jy = ldsn(ad, la, 1024), jz = ldsn(ad, la, 1280);
-    /* pole state: P0 x y z, P1 x y z, J0 x y z, J1 x y z */
-    float p0x = 0, p0y = 0, p0z = 0, p1x = 0, p1y = 0, p1z = 0;
-    float j0x = 0, j0y = 0, j0z = 0, j1x = 0, j1y = 0, j1z = 0;
-    if (POLES && m.poles) {
-        p0x = ldsn(ap, lp, 0); p0y = ldsn(ap, lp, 256); p0z = ldsn(ap, lp, 512);
-        p1x = ldsn(ap, lp, 768); p1y = ldsn(ap, lp, 1024); p1z = ldsn(ap, lp, 1280);
-        j0x = ldsn(ap, lp, 1536); j0y = ldsn(ap, lp, 1792); j0z = ldsn(ap, lp, 2048);
-        j1x = ldsn(ap, lp, 2304); j1y = ldsn(ap, lp, 2560); j1z = ldsn(ap, lp, 2816);
+    /* pole state (NP poles): P of pole q, component c at slot 3q + c, J at slot 3 NP + 3q + c */
+    float pp[3][3] = {}, jj[3][3] = {};
+    if (NP && m.poles) {
+#pragma unroll
+        for (int q = 0; q < NP; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                pp[q][c] = ldsn(ap, lp, (3 * q + c) * 256);
+                jj[q][c] = ldsn(ap, lp, (3 * NP + 3 * q + c) * 256);
+            }
+    }
+    float cx = m.cj * jx, cy = m.cj * jy, cz = m.cj * jz;   /* dispersive current terms of the E update */
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+        cx += m.pcj[q] * jj[q][0] + m.pcp[q] * pp[q][0];
+        cy += m.pcj[q] * jj[q][1] + m.pcp[q] * pp[q][1];
+        cz += m.pcj[q] * jj[q][2] + m.pcp[q] * pp[q][2];
     }
 
-    if (j > 0 && k > 0 && !tx)
-        ex = m.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx + m.cj * jx
-             + (m.pcj[0] * j0x + m.pcp[0] * p0x) + (m.pcj[1] * j1x + m.pcp[1] * p1x)
-             + m.c2 * expv + m.c3 * ex;
-    if (i > 0 && k > 0 && !ty)
-        ey = m.c0 * ((hx - hxkm1) * rkz - (hz - hzi) * rkx) / g.dy + m.cj * jy
-             + (m.pcj[0] * j0y + m.pcp[0] * p0y) + (m.pcj[1] * j1y + m.pcp[1] * p1y)
-             + m.c2 * eypv + m.c3 * ey;
-    if (i > 0 && j > 0 && !tz)
-        ez = m.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz + m.cj * jz
-             + (m.pcj[0] * j0z + m.pcp[0] * p0z) + (m.pcj[1] * j1z + m.pcp[1] * p1z)
-             + m.c2 * ezpv + m.c3 * ez;
+    if (yi && k > 0 && !tx)
+        ex = m.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx + cx + m.c2 * expv + m.c3 * ex;
+    if (xi && k > 0 && !ty)
+        ey = m.c0 * ((hx - hxkm1) * rkz - (hz - hzi) * rkx) / g.dy + cy + m.c2 * eypv + m.c3 * ey;
+    if (xi && yi && !tz)
+        ez = m.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz + cz + m.c2 * ezpv + m.c3 * ez;
 
     /* the branch's own CPML update; its x0 correction multiplies by a zero cb */
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
@@ -231,8 +240,8 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
         const float a1 = bc.x * ps.x + bc.z * (hz - hzj) / g.dz;
         const float a2 = bc.x * ps.y + bc.z * (hx - hxj) / g.dx;
         st2(F2.pe, p, a1, a2);
-        if (!(i == 0 || k == 0) && !tx) ex += g.dx * m.cbm[0] * a1;
-        if (!(i == 0 || k == 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
+        if ((xi && k != 0) && !tx) ex += g.dx * m.cbm[0] * a1;
+        if ((xi && k != 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
     }
     if (F4.on && k > 1 && k < F4.depth) {
         const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
@@ -241,8 +250,8 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
         const float a1 = bc.x * ps.x + bc.z * (hx - hxkm1) / g.dy;
         const float a2 = bc.x * ps.y + bc.z * (hy - hykm1) / g.dx;
         st2(F4.pe, p, a1, a2);
-        if (!(i == 0 || j == 0) && !ty) ey += g.dy * m.cbm[1] * a1;
-        if (!(i == 0 || j == 0) && !tx) ex -= g.dx * m.cbm[0] * a2;
+        if ((xi && yi) && !ty) ey += g.dy * m.cbm[1] * a1;
+        if ((xi && yi) && !tx) ex -= g.dx * m.cbm[0] * a2;
     }
     if (F1.on && i >= nx - F1.depth + 1) {
         const int ir = i - (nx - F1.depth);
@@ -252,8 +261,8 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
         const float a1 = bc.x * ps.x + bc.z * (hz - hzi) / g.dy;
         const float a2 = bc.x * ps.y + bc.z * (hy - hyi) / g.dz;
         st2(F1.pe, p, a1, a2);
-        if (!(j == 0 || k == 0) && !ty) ey -= g.dy * m.cbm[1] * a1;
-        if (!(j == 0 || k == 0) && !tz) ez += g.dz * m.cbm[2] * a2;
+        if ((yi && k != 0) && !ty) ey -= g.dy * m.cbm[1] * a1;
+        if ((yi && k != 0) && !tz) ez += g.dz * m.cbm[2] * a2;
     }
     if (F3.on && j >= ny - F3.depth + 1) {
         const int jr = j - (ny - F3.depth);
@@ -263,8 +272,8 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
         const float a1 = bc.x * ps.x + bc.z * (hz - hzj) / g.dx;
         const float a2 = bc.x * ps.y + bc.z * (hx - hxj) / g.dz;
         st2(F3.pe, p, a1, a2);
-        if (!(i == 0 || k == 0) && !tx) ex += g.dx * m.cbm[0] * a1;
-        if (!(i == 0 || k == 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
+        if ((xi && k != 0) && !tx) ex += g.dx * m.cbm[0] * a1;
+        if ((xi && k != 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
     }
     if (F5.on && k >= nz - F5.depth + 1) {
         const int kr = k - (nz - F5.depth);
@@ -274,25 +283,25 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
         const float a1 = bc.x * ps.x + bc.z * (hx - hxkm1) / g.dy;
         const float a2 = bc.x * ps.y + bc.z * (hy - hykm1) / g.dx;
         st2(F5.pe, p, a1, a2);
-        if (!(i == 0 || j == 0) && !ty) ey += g.dy * m.cbm[1] * a1;
-        if (!(i == 0 || j == 0) && !tx) ex -= g.dx * m.cbm[0] * a2;
+        if ((xi && yi) && !ty) ey += g.dy * m.cbm[1] * a1;
+        if ((xi && yi) && !tx) ex -= g.dx * m.cbm[0] * a2;
     }
 
     /* polarisation state (all three components, with the values at this point) */
     stsn(ad, la, 768, m.dj * jx + m.da * (expv + 2.f * vexp + ex));
     stsn(ad, la, 1024, m.dj * jy + m.da * (eypv + 2.f * veyp + ey));
     stsn(ad, la, 1280, m.dj * jz + m.da * (ezpv + 2.f * vezp + ez));
-    if (POLES && m.poles) {
-        const float n0x = m.pj[0] * j0x + m.ps[0] * p0x + m.b2[0] * expv + m.b3[0] * vexp + m.b4[0] * ex;
-        const float n1x = m.pj[1] * j1x + m.ps[1] * p1x + m.b2[1] * expv + m.b3[1] * vexp + m.b4[1] * ex;
-        const float n0y = m.pj[0] * j0y + m.ps[0] * p0y + m.b2[0] * eypv + m.b3[0] * veyp + m.b4[0] * ey;
-        const float n1y = m.pj[1] * j1y + m.ps[1] * p1y + m.b2[1] * eypv + m.b3[1] * veyp + m.b4[1] * ey;
-        const float n0z = m.pj[0] * j0z + m.ps[0] * p0z + m.b2[0] * ezpv + m.b3[0] * vezp + m.b4[0] * ez;
-        const float n1z = m.pj[1] * j1z + m.ps[1] * p1z + m.b2[1] * ezpv + m.b3[1] * vezp + m.b4[1] * ez;
-        stsn(ap, lp, 0, p0x + n0x); stsn(ap, lp, 256, p0y + n0y); stsn(ap, lp, 512, p0z + n0z);
-        stsn(ap, lp, 768, p1x + n1x); stsn(ap, lp, 1024, p1y + n1y); stsn(ap, lp, 1280, p1z + n1z);
-        stsn(ap, lp, 1536, n0x); stsn(ap, lp, 1792, n0y); stsn(ap, lp, 2048, n0z);
-        stsn(ap, lp, 2304, n1x); stsn(ap, lp, 2560, n1y); stsn(ap, lp, 2816, n1z);
+    if (NP && m.poles) {
+        const float ev[3] = { ex, ey, ez }, epv[3] = { expv, eypv, ezpv }, evp[3] = { vexp, veyp, vezp };
+#pragma unroll
+        for (int q = 0; q < NP; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float nj = m.pj[q] * jj[q][c] + m.ps[q] * pp[q][c] + m.b2[q] * epv[c] + m.b3[q] * evp[c]
+                                 + m.b4[q] * ev[c];
+                stsn(ap, lp, (3 * q + c) * 256, pp[q][c] + nj);
+                stsn(ap, lp, (3 * NP + 3 * q + c) * 256, nj);
+            }
     }
     stsn(ad, la, 0, vexp);
     stsn(ad, la, 256, veyp);
@@ -316,7 +325,8 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
 
     /* lane offsets in the blocked ADE state planes, [j][i/64][state][64] */
     const OFF la = ADE ? ((OFF)j * g.pxa * GSV_ADE_D + (OFF)(i >> 6) * 64 * GSV_ADE_D + (i & 63)) * 4 : 0;
-    const OFF lp = ADE == 2 ? ((OFF)j * g.pxa * GSV_ADE_P + (OFF)(i >> 6) * 64 * GSV_ADE_P + (i & 63)) * 4 : 0;
+    constexpr int NP = ADE >= 2 ? ADE : 0;   /* critical-point poles: ADE 2 -> 2, ADE 3 -> 3 */
+    const OFF lp = NP ? ((OFF)j * g.pxa * GSV_ADE_P(NP) + (OFF)(i >> 6) * 64 * GSV_ADE_P(NP) + (i & 63)) * 4 : 0;
 
     const float rkx = gp(g.rkex)[i], rky = gp(g.rkey)[j];
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
@@ -327,12 +337,16 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     const int jrn = j - (ny - F3.depth);
     const bool syn = F3.on && j >= ny - F3.depth + 1;
     const bool zplane_xy = (F0.on && i == 0) || (F2.on && j == 0);
+    /* a periodic axis has no near plane: the -x / -y neighbour of the near plane is the far plane */
+    const bool xi = i > 0 || g.perx, yi = j > 0 || g.pery;
+    const OFF dIm = (g.perx && i == 0) ? (OFF)(nx - 1) * 4 : (OFF)0 - (OFF)4;
+    const OFF dJm = (g.pery && j == 0) ? (OFF)(ny - 1) * dJ : (OFF)0 - dJ;
 
     float hxkm1 = ldf(g.hxg, o - dP), hykm1 = ldf(g.hyg, o - dP);
     for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
         const float hx = ldn(g.hxg, o), hy = ldn(g.hyg, o), hz = ldn(g.hzg, o);
-        const float hzj = ldf(g.hzg, o - dJ), hxj = ldf(g.hxg, o - dJ);
-        const float hzi = ldf(g.hzg, o - 4), hyi = ldf(g.hyg, o - 4);
+        const float hzj = ldf(g.hzg, o + dJm), hxj = ldf(g.hxg, o + dJm);
+        const float hzi = ldf(g.hzg, o + dIm), hyi = ldf(g.hyg, o + dIm);
         const float rkz = gp(g.rkez)[k];
         GsvECoef ec = ldgs(g.etab + ldid<ID>(g.ecls, oc));
         if (PC && (ec.flags & GSV_E_PERCELL))
@@ -342,15 +356,15 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
 
         if (ADE && (fl & GSV_E_ADE))
-            ade_cell<(ADE == 2)>(g, g.ade_d + (size_t)k * g.ade_plane_d, la,
-                                 ADE == 2 ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr, lp, i, j, k, fl, ex, ey, ez,
-                                 hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
+            ade_cell<NP>(g, g.ade_d + (size_t)k * g.ade_plane_d, la,
+                         NP ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr, lp, i, j, k, fl, ex, ey, ez,
+                         hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
 
-        if (tx && j > 0 && k > 0)
+        if (tx && yi && k > 0)
             ex = ec.ca[0] * ex + ec.cb[0] * ((hz - hzj) * rky - (hy - hykm1) * rkz);
-        if (ty && i > 0 && k > 0)
+        if (ty && xi && k > 0)
             ey = ec.ca[1] * ey + ec.cb[1] * ((hx - hxkm1) * rkz - (hz - hzi) * rkx);
-        if (tz && i > 0 && j > 0)
+        if (tz && xi && yi)
             ez = ec.ca[2] * ez + ec.cb[2] * ((hy - hyi) * rkx - (hx - hxj) * rky);
 
         if (fl & (GSV_E_PECX | GSV_E_PECY | GSV_E_PECZ)) {
@@ -367,7 +381,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             const float a1 = bc.x * ps.x + bc.y * (hz - hzi);
             const float a2 = bc.x * ps.y + bc.y * (hy - hyi);
             st2(F0.pe, p, a1, a2);
-            if (j != 0 && k != 0) {
+            if (yi && k != 0) {
                 if (ty) ey -= g.dx * ec.cb[1] * a1;
                 if (tz) ez += g.dx * ec.cb[2] * a2;
             }
@@ -379,7 +393,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             const float a1 = bc.x * ps.x + bc.y * (hz - hzj);
             const float a2 = bc.x * ps.y + bc.y * (hx - hxj);
             st2(F2.pe, p, a1, a2);
-            if (i != 0 && k != 0) {
+            if (xi && k != 0) {
                 if (tx) ex += g.dy * ec.cb[0] * a1;
                 if (tz) ez -= g.dy * ec.cb[2] * a2;
             }
@@ -391,7 +405,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             const float a1 = bc.x * ps.x + bc.y * (hx - hxkm1);
             const float a2 = bc.x * ps.y + bc.y * (hy - hykm1);
             st2(F4.pe, p, a1, a2);
-            if (i != 0 && j != 0) {
+            if (xi && yi) {
                 if (ty) ey += g.dx * ec.cb[1] * a1;
                 /* the reference gates this Ex correction on the z flag (src3d/boundary.c:800-801) */
                 if (tz) ex -= g.dx * ec.cb[0] * a2;
@@ -404,7 +418,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             const float a1 = bc.x * ps.x + bc.y * (hz - hzi);
             const float a2 = bc.x * ps.y + bc.y * (hy - hyi);
             st2(F1.pe, p, a1, a2);
-            if (j != 0 && k != 0) {
+            if (yi && k != 0) {
                 if (ty) ey -= g.dx * ec.cb[1] * a1;
                 if (tz) ez += g.dx * ec.cb[2] * a2;
             }
@@ -416,7 +430,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             const float a1 = bc.x * ps.x + bc.y * (hz - hzj);
             const float a2 = bc.x * ps.y + bc.y * (hx - hxj);
             st2(F3.pe, p, a1, a2);
-            if (i != 0 && k != 0) {
+            if (xi && k != 0) {
                 if (tx) ex += g.dy * ec.cb[0] * a1;
                 if (tz) ez -= g.dy * ec.cb[2] * a2;
             }
@@ -429,7 +443,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             const float a1 = bc.x * ps.x + bc.y * (hx - hxkm1);
             const float a2 = bc.x * ps.y + bc.y * (hy - hykm1);
             st2(F5.pe, p, a1, a2);
-            if (i != 0 && j != 0) {
+            if (xi && yi) {
                 if (ty) ey += g.dx * ec.cb[1] * a1;
                 if (tx) ex -= g.dx * ec.cb[0] * a2;
             }
@@ -520,9 +534,7 @@ __device__ __forceinline__ void
 tsf_h_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
 {
     int na, nb;
-    if (axis == 0) { na = t.jn - t.j0 + 1; nb = t.kn - t.k0 + 1; }
-    else if (axis == 1) { na = t.in - t.i0 + 1; nb = t.kn - t.k0 + 1; }
-    else { na = t.in - t.i0 + 1; nb = t.jn - t.j0 + 1; }
+    tsf_face_extent(t, g.nx, g.ny, axis, &na, &nb);
     if (a >= na || b >= nb)
         return;
     const float cf = gp(t.coefH[axis * 2 + far])[a + (long long)na * b];
@@ -565,9 +577,7 @@ __device__ __forceinline__ void
 tsf_e_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
 {
     int na, nb;
-    if (axis == 0) { na = t.jn - t.j0 + 1; nb = t.kn - t.k0 + 1; }
-    else if (axis == 1) { na = t.in - t.i0 + 1; nb = t.kn - t.k0 + 1; }
-    else { na = t.in - t.i0 + 1; nb = t.jn - t.j0 + 1; }
+    tsf_face_extent(t, g.nx, g.ny, axis, &na, &nb);
     if (a >= na || b >= nb)
         return;
     const float cf = gp(t.coefE[axis * 2 + far])[a + (long long)na * b];
@@ -869,7 +879,13 @@ k_dft_sample(GsvGrid g, GsvDft d, int slot)
     float v0, v1, v2, v3;
     if (face < 2) { v0 = g.ey[c]; v1 = g.ez[c]; v2 = g.hy[c]; v3 = g.hz[c]; }
     else if (face < 4) { v0 = g.ex[c]; v1 = g.ez[c]; v2 = g.hx[c]; v3 = g.hz[c]; }
-    else { v0 = g.ex[c]; v1 = g.ey[c]; v2 = g.hx[c]; v3 = g.hy[c]; }
+    else if (face < 6) { v0 = g.ex[c]; v1 = g.ey[c]; v2 = g.hx[c]; v3 = g.hy[c]; }
+    else {
+        v0 = g.ex[c];
+        v1 = g.ey[c];
+        v2 = 0.5f * (g.hx[c] + g.hx[c - g.plane]);
+        v3 = 0.5f * (g.hy[c] + g.hy[c - g.plane]);
+    }
     float *r = d.ring + (long long)slot * 4 * d.ncells + q;
     r[0] = v0;
     r[d.ncells] = v1;
@@ -993,8 +1009,17 @@ launch_estep(const GsvGrid *g, int ade, dim3 grd, dim3 blk, int kchunk, hipStrea
         hipLaunchKernelGGL((k_estep<ID, 0, OFF, PC>), grd, blk, 0, st, *g, kchunk);
     else if (ade == 1)
         hipLaunchKernelGGL((k_estep<ID, 1, OFF, PC>), grd, blk, 0, st, *g, kchunk);
-    else
+    else if (ade == 2)
         hipLaunchKernelGGL((k_estep<ID, 2, OFF, PC>), grd, blk, 0, st, *g, kchunk);
+    else
+        hipLaunchKernelGGL((k_estep<ID, 3, OFF, PC>), grd, blk, 0, st, *g, kchunk);
+}
+
+/* dispersive build of a grid: 0 none, 1 Drude only, 2 / 3 Drude + 2 / 3 critical-point poles */
+static int
+ade_mode(const GsvGrid *g)
+{
+    return g->ade ? (g->ade_poles ? g->ade_poles : 1) : 0;
 }
 
 extern "C" hipError_t
@@ -1003,7 +1028,7 @@ gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipS
     dim3 blk(bx, by, 1), grd((g->nx + bx - 1) / bx, (g->ny + by - 1) / by, (g->nz + kchunk - 1) / kchunk);
     /* the dispersive branch is compiled in only when the pool has ADE cells,
      * and the critical-point terms only when some material has them */
-    const int ade = g->ade ? (g->ade_poles ? 2 : 1) : 0;
+    const int ade = ade_mode(g);
     const bool o32 = offsets32(g), pc = g->pc_nx > 0;
     if (pc && !o32)
         return hipErrorInvalidValue;   /* design regions need 32-bit offsets (field arrays below 4 GiB) */
@@ -1026,20 +1051,12 @@ gsv_launch_tsf_line(const GsvTsf *t, int step, hipStream_t st)
     return hipGetLastError();
 }
 
-static void
-tsf_face_dims(const GsvTsf *t, int axis, int *na, int *nb)
-{
-    if (axis == 0) { *na = t->jn - t->j0 + 1; *nb = t->kn - t->k0 + 1; }
-    else if (axis == 1) { *na = t->in - t->i0 + 1; *nb = t->kn - t->k0 + 1; }
-    else { *na = t->in - t->i0 + 1; *nb = t->jn - t->j0 + 1; }
-}
-
 extern "C" hipError_t
 gsv_launch_tsf_faces(const GsvGrid *g, const GsvTsf *t, int hside, hipStream_t st)
 {
     for (int axis = 0; axis < 3; axis++) {
         int na, nb;
-        tsf_face_dims(t, axis, &na, &nb);
+        tsf_face_extent(*t, g->nx, g->ny, axis, &na, &nb);
         dim3 blk(64, 4, 1), grd((na + 63) / 64, (nb + 3) / 4, 2);
         if (hside)
             hipLaunchKernelGGL(k_tsf_h, grd, blk, 0, st, *g, *t, axis);
@@ -1230,8 +1247,10 @@ launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int a
         hipLaunchKernelGGL((k_estep_batch<ID, 0, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
     else if (ade == 1)
         hipLaunchKernelGGL((k_estep_batch<ID, 1, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
-    else
+    else if (ade == 2)
         hipLaunchKernelGGL((k_estep_batch<ID, 2, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
+    else
+        hipLaunchKernelGGL((k_estep_batch<ID, 3, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
 }
 
 extern "C" hipError_t

@@ -29,6 +29,28 @@ is_output(SvPool *mp, SvSet *set)
     return 0;
 }
 
+/* FLUX_DFT spectra from the accumulators: out[p * nw + w] = 1/2 Re sum over the
+ * plane's cells of (Ex Hy* - Ey Hx*) dx dy, the DFTs scaled by dt */
+void
+sv_flux_spectra(const SvPool *mp, const SvSet *set, double *out)
+{
+    const SvFarfield *ff = mp->farfield;
+    const SvSetFlux *x = &set->sx;
+    const long per = (long)(x->i1 - x->i0) * (x->j1 - x->j0);
+    const double scale = 0.5 * set->sp.dx * set->sp.dy * set->plan.dt * set->plan.dt;
+    int p, w;
+    for (p = 0; p < x->nplanes; p++)
+        for (w = 0; w < ff->nflux; w++) {
+            double s = 0;
+            long c;
+            for (c = 0; c < per; c++) {
+                const double *a = ff->flux_data + ((size_t)(p * per + c) * ff->nflux + w) * 8;
+                s += a[0] * a[6] + a[1] * a[7] - a[2] * a[4] - a[3] * a[5];
+            }
+            out[p * ff->nflux + w] = scale * s;
+        }
+}
+
 void
 sv_pool_output(SvPool *mp, SvSet *set)
 {
@@ -108,5 +130,31 @@ sv_pool_output(SvPool *mp, SvSet *set)
             fprintf(fh, "\n");
         }
         fclose(fh);
+    }
+    if (set->sx.nplanes && set->sx.filename[0]) {
+        SvFarfield *ff = mp->farfield;
+        double *fl = (double *)malloc((size_t)set->sx.nplanes * ff->nflux * sizeof(double));
+        int w, p;
+        fh = fopen(set->sx.filename, "w");
+        if (!fh || !fl) {
+            fprintf(stderr, "Error: flux DFT: cannot open file %s for writing\n", set->sx.filename);
+            free(fl);
+            if (fh)
+                fclose(fh);
+            return;
+        }
+        sv_flux_spectra(mp, set, fl);
+        fprintf(fh, "# net z flux spectrum through each plane: lambda, then one column per plane (k =");
+        for (p = 0; p < set->sx.nplanes; p++)
+            fprintf(fh, " %d", set->sx.k[p]);
+        fprintf(fh, ")\n");
+        for (w = 0; w < ff->nflux; w++) {
+            fprintf(fh, "%g", set->sx.lambda[w]);
+            for (p = 0; p < set->sx.nplanes; p++)
+                fprintf(fh, " %g", fl[p * ff->nflux + w]);
+            fprintf(fh, "\n");
+        }
+        fclose(fh);
+        free(fl);
     }
 }

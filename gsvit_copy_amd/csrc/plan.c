@@ -417,8 +417,26 @@ load_sources(SvPool *mp, SvSet *set)
         ts->ia_theta = t->ia_theta;
         ts->ia_phi = t->ia_phi;
         ts->ia_psi = t->ia_psi;
-        if (ts->box_i0 < 1 || ts->box_j0 < 1 || ts->box_k0 < 1 || ts->box_in >= set->sp.xres
-            || ts->box_jn >= set->sp.yres || ts->box_kn >= set->sp.zres) {
+        /* a domain-periodic axis (no reference counterpart): the box spans the
+         * whole axis, its faces normal to it are off, and the wave must not
+         * travel along it (the incident field has to be periodic) */
+        if (set->sb.bx0 == SV_BOUNDARY_PERIODIC || set->sb.by0 == SV_BOUNDARY_PERIODIC) {
+            double ax = sin(t->ia_theta) * cos(t->ia_phi), ay = sin(t->ia_theta) * sin(t->ia_phi);
+            if ((set->sb.bx0 == SV_BOUNDARY_PERIODIC && fabs(ax) > 1e-12)
+                || (set->sb.by0 == SV_BOUNDARY_PERIODIC && fabs(ay) > 1e-12)) {
+                fprintf(stderr, "Error: a TSF wave in a periodic pool must travel normal to the periodic axes\n");
+                return 1;
+            }
+            if (set->sb.bx0 == SV_BOUNDARY_PERIODIC) {
+                ts->box_i0 = 0; ts->box_in = set->sp.xres; ts->skip_i0 = ts->skip_in = 1;
+            }
+            if (set->sb.by0 == SV_BOUNDARY_PERIODIC) {
+                ts->box_j0 = 0; ts->box_jn = set->sp.yres; ts->skip_j0 = ts->skip_jn = 1;
+            }
+        }
+        if ((set->sb.bx0 != SV_BOUNDARY_PERIODIC && (ts->box_i0 < 1 || ts->box_in >= set->sp.xres))
+            || (set->sb.by0 != SV_BOUNDARY_PERIODIC && (ts->box_j0 < 1 || ts->box_jn >= set->sp.yres))
+            || ts->box_k0 < 1 || ts->box_kn >= set->sp.zres) {
             fprintf(stderr, "Error: TSF box must lie inside the pool with one cell to spare\n");
             return 1;
         }
@@ -482,7 +500,9 @@ calculate_ade(SvMatProp *mat, double epsilon, double sigma, double omegap, doubl
     mat->ade_a0 = (nu * dt - 2) / (nu * dt + 2);
     mat->ade_a1 = 4 / (nu * dt + 2);
     mat->ade_a2 = dt * dt * EPS0 * omegap * omegap / (2 * (nu * dt + 2));
-    for (i = 0; i < 2; i++) {
+    /* a third pole (type 7, no reference counterpart) follows the same
+     * formulas; for type 5 its parameters are zero and its terms vanish */
+    for (i = 0; i < 3; i++) {
         double a = mat->cp3_a[i], phi = mat->cp3_phi[i], omega = mat->cp3_omega[i], gamma = mat->cp3_gamma[i];
         double den = (omega * omega * dt * dt + (2 + gamma * dt) * (2 + gamma * dt));
         mat->ade_bp0[i] = -(omega * omega * dt * dt + (2 - gamma * dt) * (2 - gamma * dt)) / den;
@@ -492,11 +512,12 @@ calculate_ade(SvMatProp *mat, double epsilon, double sigma, double omegap, doubl
         mat->ade_bp4[i] = 2 * EPS0 * a * omega * dt * (omega * dt * cos(phi) - (2 + gamma * dt) * sin(phi)) / den;
     }
     {
-        double den = sigma * dt / 2 + mat->ade_a2 + mat->ade_bp4[0] + mat->ade_bp4[1] + EPS0 * epsilon;
+        double den = sigma * dt / 2 + mat->ade_a2 + mat->ade_bp4[0] + mat->ade_bp4[1] + mat->ade_bp4[2] + EPS0 * epsilon;
         mat->ade_c0 = dt / den;
         mat->ade_c1 = 1 / den;
-        mat->ade_c2 = -(mat->ade_a2 + mat->ade_bp2[0] + mat->ade_bp2[1]) / den;
-        mat->ade_c3 = -(sigma * dt / 2 + 2 * mat->ade_a2 + mat->ade_bp3[0] + mat->ade_bp3[1] - EPS0 * epsilon) / den;
+        mat->ade_c2 = -(mat->ade_a2 + mat->ade_bp2[0] + mat->ade_bp2[1] + mat->ade_bp2[2]) / den;
+        mat->ade_c3 = -(sigma * dt / 2 + 2 * mat->ade_a2 + mat->ade_bp3[0] + mat->ade_bp3[1] + mat->ade_bp3[2]
+                        - EPS0 * epsilon) / den;
     }
 }
 
@@ -514,8 +535,8 @@ scan_floats(FILE *fr, float *v, int n)
 static int
 scan_material(FILE *fr, SvMatProp *mat, double dt)
 {
-    float v[12];
-    int type, q;
+    float v[16];
+    int type, q, np;
 
     memset(mat, 0, sizeof(*mat));
     mat->epsilon = mat->mu = 1;
@@ -531,14 +552,17 @@ scan_material(FILE *fr, SvMatProp *mat, double dt)
         mat->sigma = v[2];
         mat->sigast = v[3];
     } else if (type == SV_MAT_PEC) {
-    } else if (type == SV_MAT_ADE) {
-        if (scan_floats(fr, v, 12))
+    } else if (type == SV_MAT_ADE || type == SV_MAT_ADE3) {
+        np = type == SV_MAT_ADE3 ? 3 : 2;
+        if (scan_floats(fr, v, 4 + 4 * np))
             return 1;
+        mat->type = SV_MAT_ADE;
+        mat->ade_npoles = np;
         mat->epsilon = v[0];
         mat->sigma = v[1];
         mat->drude_omega_p = v[2];
         mat->drude_nu = v[3];
-        for (q = 0; q < 2; q++) {
+        for (q = 0; q < np; q++) {
             mat->cp3_a[q] = v[4 + 4 * q];
             mat->cp3_phi[q] = v[5 + 4 * q];
             mat->cp3_omega[q] = v[6 + 4 * q];
@@ -547,7 +571,8 @@ scan_material(FILE *fr, SvMatProp *mat, double dt)
         calculate_ade(mat, v[0], v[1], v[2], v[3], dt);
     } else {
         fprintf(stderr, "Error: material type %d is not supported by the MI355X engine "
-                "(supported: 0 linear, 1 tabulated linear, 5 ADE Drude + 2 critical points, 10 PEC)\n", type);
+                "(supported: 0 linear, 1 tabulated linear, 5 ADE Drude + 2 critical points, 7 ADE Drude + 3 "
+                "critical points, 10 PEC)\n", type);
         return 1;
     }
     return 0;
@@ -579,7 +604,9 @@ get_matindex(const SvPool *mp, const SvMatProp *mat)
                 && almost_eq(t->cp3_phi[2], mat->cp3_phi[2]) && almost_eq(t->cp3_omega[0], mat->cp3_omega[0])
                 && almost_eq(t->cp3_omega[1], mat->cp3_omega[1]) && almost_eq(t->cp3_omega[2], mat->cp3_omega[2])
                 && almost_eq(t->cp3_gamma[0], mat->cp3_gamma[0]) && almost_eq(t->cp3_gamma[1], mat->cp3_gamma[1])
-                && almost_eq(t->cp3_gamma[2], mat->cp3_gamma[2])))
+                && almost_eq(t->cp3_gamma[2], mat->cp3_gamma[2])
+                /* three-pole materials (no reference counterpart) also compare the third amplitude */
+                && (t->ade_npoles < 3 && mat->ade_npoles < 3 ? 1 : almost_eq(t->cp3_a[2], mat->cp3_a[2]))))
             return i;
     }
     return -1;
@@ -655,7 +682,7 @@ test_script_file(const char *fname)
 {
     FILE *fr = fopen(fname, "r");
     int type, mtype, is_e = 0, is_m = 0;
-    float p[12];
+    float p[16];
     if (!fr) {
         fprintf(stderr, "Error: cannot open file %s for reading\n", fname);
         return SV_MATMODE_ERROR;
@@ -678,8 +705,8 @@ test_script_file(const char *fname)
                 if (p[0] != 1.0 || p[2] > 0) is_e = 1;
                 if (p[1] != 1.0 || p[3] > 0) is_m = 1;
             }
-        } else if (mtype == SV_MAT_ADE) {
-            if (scan_floats(fr, p, 12))
+        } else if (mtype == SV_MAT_ADE || mtype == SV_MAT_ADE3) {
+            if (scan_floats(fr, p, mtype == SV_MAT_ADE3 ? 16 : 12))
                 goto err;
             is_e = 1;
         } else if (mtype == SV_MAT_PEC) {
@@ -1108,6 +1135,15 @@ farfield_new(SvSet *set)
         ff->dft_ncells = 2 * (iy * iz + ix * iz + ix * iy);
         ff->dft_data = (double *)calloc((size_t)ff->dft_ncells * 4 * ff->ndft * 2, sizeof(double));
     }
+    ff->nflux = set->sx.nplanes ? set->sx.nw : 0;
+    if (ff->nflux) {
+        int q;
+        ff->flux_omega = (double *)malloc(ff->nflux * sizeof(double));
+        for (q = 0; q < ff->nflux; q++)
+            ff->flux_omega[q] = 2 * M_PI * SV_LIGHT_SPEED / set->sx.lambda[q];
+        ff->flux_ncells = (long)set->sx.nplanes * (set->sx.i1 - set->sx.i0) * (set->sx.j1 - set->sx.j0);
+        ff->flux_data = (double *)calloc((size_t)ff->flux_ncells * ff->nflux * 8, sizeof(double));
+    }
     return ff;
 }
 
@@ -1134,6 +1170,19 @@ init_and_make_plan(SvSet *set)
 
     if (xres < 3 || yres < 3 || zres < 3) {
         fprintf(stderr, "Error: pool must be at least 3 cells along each axis\n");
+        return NULL;
+    }
+    /* domain-periodic axes (no reference counterpart): x and y only, both faces,
+     * not combined with the interior periodic planes of MBOUNDARY */
+    if ((set->sb.bx0 == SV_BOUNDARY_PERIODIC) != (set->sb.bxn == SV_BOUNDARY_PERIODIC)
+        || (set->sb.by0 == SV_BOUNDARY_PERIODIC) != (set->sb.byn == SV_BOUNDARY_PERIODIC)
+        || set->sb.bz0 == SV_BOUNDARY_PERIODIC || set->sb.bzn == SV_BOUNDARY_PERIODIC) {
+        fprintf(stderr, "Error: periodic boundaries are supported on both x faces and/or both y faces only\n");
+        return NULL;
+    }
+    if ((set->sb.bx0 == SV_BOUNDARY_PERIODIC || set->sb.by0 == SV_BOUNDARY_PERIODIC)
+        && (set->smb.bx0 || set->smb.bxn || set->smb.by0 || set->smb.byn || set->smb.bz0 || set->smb.bzn)) {
+        fprintf(stderr, "Error: periodic pool boundaries cannot be combined with MBOUNDARY planes\n");
         return NULL;
     }
     if (load_sources(mp, set))
@@ -1236,6 +1285,20 @@ init_and_make_plan(SvSet *set)
         }
     }
 
+    if (set->sx.nplanes) {
+        const SvSetFlux *x = &set->sx;
+        int q;
+        if (x->i0 < 0 || x->j0 < 0 || x->i1 > xres || x->j1 > yres || x->i1 <= x->i0 || x->j1 <= x->j0) {
+            fprintf(stderr, "Error: FLUX_DFT rectangle must lie inside the pool\n");
+            return NULL;
+        }
+        for (q = 0; q < x->nplanes; q++)
+            if (x->k[q] < 1 || x->k[q] > zres - 1) {
+                fprintf(stderr, "Error: FLUX_DFT plane %d outside 1 .. %d\n", x->k[q], zres - 1);
+                return NULL;
+            }
+    }
+
     /* gradient extension: the design voxels carry per-voxel permittivity
      * (MEDIUM_LINEAR data) and the coefficients they feed must stay off the
      * zeroed boundary planes */
@@ -1324,6 +1387,8 @@ sv_pool_free(SvPool *mp, SvSet *set)
         free(mp->farfield->rpoints);
         free(mp->farfield->dft_omega);
         free(mp->farfield->dft_data);
+        free(mp->farfield->flux_omega);
+        free(mp->farfield->flux_data);
         free(mp->farfield);
     }
     free(mp->mats);

@@ -192,6 +192,32 @@ gsv_get_sums(void *h, double *out)
 }
 
 void
+gsv_flux_dims(void *h, int *dims)
+{
+    const SvSetFlux *x = &((GsvSession *)h)->set.sx;
+    dims[0] = x->nplanes;
+    dims[1] = x->nplanes ? x->nw : 0;
+    dims[2] = x->nplanes ? (x->i1 - x->i0) * (x->j1 - x->j0) : 0;
+}
+
+long
+gsv_get_flux(void *h, double *flux, double *acc)
+{
+    GsvSession *s = (GsvSession *)h;
+    SvFarfield *ff = s->mp->farfield;
+    long cnt = ff ? ff->flux_ncells * ff->nflux * 8 : 0;
+    if (!cnt || (!flux && !acc))
+        return cnt;
+    if (gsv_engine_fetch_aux(s->mp->gpu))
+        return -1;
+    if (flux)
+        sv_flux_spectra(s->mp, &s->set, flux);
+    if (acc)
+        memcpy(acc, ff->flux_data, (size_t)cnt * sizeof(double));
+    return cnt;
+}
+
+void
 gsv_get_maps(void *h, int *mat, int *bnds)
 {
     GsvSession *s = (GsvSession *)h;

@@ -37,6 +37,8 @@ clear_settings(SvSet *set, int remove)
         free(set->sf.individual);
         free(set->sf.source_filename);
         free(set->sf.dft_lambda);
+        free(set->sx.k);
+        free(set->sx.lambda);
     }
     memset(set, 0, sizeof(SvSet));
     /* defaults from src3d/settings.c:570-990 and src3d/defaults.h */
@@ -217,6 +219,10 @@ parse_cpml_face(FILE *fr, const char *key, const SvSet *set, int *type, int *dep
         *type = SV_BOUNDARY_PEC;
     else if (!strcmp(value, "liao"))
         *type = SV_BOUNDARY_LIAO;
+    else if (!strcmp(value, "periodic"))
+        /* domain-periodic axis (no reference counterpart): the far plane's
+         * neighbour is the near plane; both faces of the axis must say so */
+        *type = SV_BOUNDARY_PERIODIC;
     else if (!strcmp(value, "cpml")) {
         *type = SV_BOUNDARY_CPML;
         if (get_int(fr, depth, key) || get_int(fr, m, key) || get_double(fr, sigma, key))
@@ -531,6 +537,28 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
                     goto fail;
             f->ndft = n;
             if (get_rest_of_line(fr, f->dft_filename, SV_NAME_LEN))
+                goto fail;
+        } else if (!strcmp(key, "FLUX_DFT")) {
+            /* flux spectra through planes (no reference counterpart):
+             * FLUX_DFT i0 j0 i1 j1 n k_1 .. k_n m lambda_1 .. lambda_m file */
+            SvSetFlux *x = &set->sx;
+            int n, q;
+            if (get_int(fr, &x->i0, key) || get_int(fr, &x->j0, key) || get_int(fr, &x->i1, key)
+                || get_int(fr, &x->j1, key) || get_int(fr, &n, key) || n <= 0 || n > 256)
+                goto fail;
+            x->k = (int *)realloc(x->k, n * sizeof(int));
+            for (q = 0; q < n; q++)
+                if (get_int(fr, &x->k[q], key))
+                    goto fail;
+            x->nplanes = n;
+            if (get_int(fr, &n, key) || n <= 0 || n > 1024)
+                goto fail;
+            x->lambda = (double *)realloc(x->lambda, n * sizeof(double));
+            for (q = 0; q < n; q++)
+                if (get_double(fr, &x->lambda[q], key) || x->lambda[q] <= 0)
+                    goto fail;
+            x->nw = n;
+            if (get_rest_of_line(fr, x->filename, SV_NAME_LEN))
                 goto fail;
         } else if (!strcmp(key, "DESIGN_BOX")) {
             SvSetAdjoint *a = &set->sa;

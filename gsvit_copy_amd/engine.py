@@ -40,6 +40,9 @@ def library():
         L.gsv_get_dft.restype = ctypes.c_long
         L.gsv_get_dft.argtypes = [vp, dp]
         L.gsv_get_maps.argtypes = [vp, ip, ip]
+        L.gsv_get_flux.restype = ctypes.c_long
+        L.gsv_get_flux.argtypes = [vp, dp, dp]
+        L.gsv_flux_dims.argtypes = [vp, ip]
         L.gsv_get_sums.restype = ctypes.c_long
         L.gsv_get_sums.argtypes = [vp, dp]
         L.gsv_set_timing.argtypes = [vp, ctypes.c_int]
@@ -156,6 +159,21 @@ class Session:
         if n > 0 and self.L.gsv_get_dft(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) < 0:
             raise RuntimeError("DFT download failed")
         return out[:n]
+
+    def flux(self, with_acc=False):
+        """FLUX_DFT spectra [nplanes, nwavelengths] (with with_acc also the raw
+        accumulators [nplanes, cells per plane, nwavelengths, 8])."""
+        d = (ctypes.c_int * 3)()
+        self.L.gsv_flux_dims(self.h, d)
+        npl, nw, per = d[0], d[1], d[2]
+        if npl == 0:
+            raise RuntimeError("the session has no FLUX_DFT planes")
+        dp = ctypes.POINTER(ctypes.c_double)
+        fl = np.zeros((npl, nw), dtype=np.float64)
+        acc = np.zeros((npl, per, nw, 8), dtype=np.float64) if with_acc else None
+        if self.L.gsv_get_flux(self.h, fl.ctypes.data_as(dp), acc.ctypes.data_as(dp) if with_acc else None) < 0:
+            raise RuntimeError("flux download failed")
+        return (fl, acc) if with_acc else fl
 
     def sums(self):
         """OUT_SUM values per box and step, [nsums, nsteps]."""

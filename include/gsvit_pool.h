@@ -111,6 +111,12 @@ typedef struct {
     double *dft_omega;
     double *dft_data;
     long dft_ncells;
+    /* FLUX_DFT planes (no reference counterpart): per plane cell (plane, j, i
+     * order), Ex, Ey, mean Hx, mean Hy x nflux wavelengths x (re, im) */
+    int nflux;
+    double *flux_omega;
+    double *flux_data;
+    long flux_ncells;
 } SvFarfield;
 
 typedef struct _SvPool {

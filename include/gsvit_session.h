@@ -37,6 +37,13 @@ long gsv_get_dft(void *session, double *out);
 /* OUT_SUM values as [nsums][nsteps] (before the cell-volume scaling of the
  * log file); returns the element count (out may be NULL), -1 on error */
 long gsv_get_sums(void *session, double *out);
+/* FLUX_DFT spectra (no reference counterpart) as [nplanes][nwavelengths] into
+ * flux and, when acc is not NULL, the raw accumulators [cell][wavelength][8]
+ * (cells plane, j, i order; Ex, Ey, mean Hx, mean Hy as re, im); returns the
+ * accumulator count, -1 on error */
+long gsv_get_flux(void *session, double *flux, double *acc);
+/* dims[0..2]: FLUX_DFT planes, wavelengths, cells per plane (0 without FLUX_DFT) */
+void gsv_flux_dims(void *session, int *dims);
 /* planner maps, [i][j][k]: tabulated material index (-1 without a table) and internal-boundary flags */
 void gsv_get_maps(void *session, int *mat, int *bnds);
 /* per-launch HIP-event timing of the H and E volume kernels */

@@ -51,6 +51,11 @@ typedef enum {
     SV_MAT_CP = 4,
     SV_MAT_ADE = 5,
     SV_MAT_PLRC = 6,
+    /* no reference counterpart: ADE with a third critical-point pole (e.g. a
+     * Lorentz term, which is a critical point with phi = 0); a type-7 line reads
+     * as type 5 plus four more numbers and is stored as an SV_MAT_ADE material
+     * with ade_npoles = 3 */
+    SV_MAT_ADE3 = 7,
     SV_MAT_PEC = 10
 } SvMatType;
 
@@ -89,11 +94,12 @@ typedef struct {
     double ade_a0;
     double ade_a1;
     double ade_a2;
-    double ade_bp0[2];
-    double ade_bp1[2];
-    double ade_bp2[2];
-    double ade_bp3[2];
-    double ade_bp4[2];
+    double ade_bp0[3];
+    double ade_bp1[3];
+    double ade_bp2[3];
+    double ade_bp3[3];
+    double ade_bp4[3];
+    int ade_npoles;       /* critical-point poles of an ADE material: 2 (type 5) or 3 (type 7) */
     double ade_c0;
     double ade_c1;
     double ade_c2;
@@ -248,6 +254,22 @@ typedef struct {
     double obj_lambda;
 } SvSetAdjoint;
 
+/* Flux spectra (no reference counterpart): FLUX_DFT i0 j0 i1 j1 n k_1 .. k_n
+ * m lambda_1 .. lambda_m file -- the net z-directed power flux spectrum
+ * through the rectangle [i0, i1) x [j0, j1) of each plane k (E grid; H taken
+ * as the mean of planes k-1 and k), F = 1/2 Re sum (Ex Hy* - Ey Hx*) dx dy with
+ * the DFTs of E at (step+1) dt and of H at (step+1/2) dt times dt.  The
+ * difference of two planes is the power absorbed between them, e.g. per
+ * layer of a stack. */
+typedef struct {
+    int i0, j0, i1, j1;
+    int nplanes;
+    int *k;
+    int nw;
+    double *lambda;
+    char filename[SV_NAME_LEN];
+} SvSetFlux;
+
 typedef struct {
     SvSetPool sp;
     SvSetComp sc;
@@ -259,6 +281,7 @@ typedef struct {
     SvSetMat sm;
     SvSetPlan plan;
     SvSetAdjoint sa;
+    SvSetFlux sx;
 } SvSet;
 
 /* src3d/settings.h:967,974 (clear_settings src3d/settings.c:570, parse_settings :3822).

@@ -12,6 +12,11 @@
  *   Yee E update ............ src3d/yee.c:528-1247 (linear/tabulated with
  *                             component averaging :1396-1492, ADE branch
  *                             :914-1108 incl. its inline CPML, PEC :1139-1237)
+ *   third ADE pole .......... (material type 7, no reference counterpart) the
+ *                             pole formulas of src3d/plan.c:2736-2762 and the
+ *                             ADE branch applied to a third pole; pinned by the
+ *                             pole-splitting identity (a pole of amplitude A
+ *                             equals two equal poles of A/2) in the tests
  *   dispersive flags ........ src3d/yee.c:453-526
  *   CPML H / E .............. src3d/boundary.c:282-520, 521-936 (zeroing of the
  *                             outer planes, z0 quirk using ztbc for Ex :800-801)
@@ -26,6 +31,10 @@
  *                             DFT of recorded point traces in the tests)
  *   point traces ............ src3d/output.c:340-352
  *   box sums ................ src3d/output.c:259-306
+ *   flux spectra (FLUX_DFT, no reference counterpart): running DFT of Ex, Ey
+ *                             and the mean of Hx, Hy over planes k-1 and k on
+ *                             each flux plane (pinned by a numpy DFT of point
+ *                             traces and by the plane-wave flux identities)
  *   gradient-extension objective (OBJECTIVE_DFT, no reference counterpart):
  *                             running DFT of E over a box, J = sum |F|^2
  *   step order .............. src3d/svit.c:217-237
@@ -77,7 +86,9 @@ typedef struct {
     real *liao[3];                    /* [24][ny][nz], [24][nx][nz], [24][nx][ny] */
     int ade;
     real *exp_, *eyp, *ezp, *dpx, *dpy, *dpz, *dpxp, *dpyp, *dpzp;
-    real *px[2], *py[2], *pz[2], *pxp[2], *pyp[2], *pzp[2];
+    real *px[3], *py[3], *pz[3], *pxp[3], *pyp[3], *pzp[3];
+    int npoles;                       /* critical-point poles carried: 2, or 3 with a type-7 material */
+    int perx, pery;                   /* domain-periodic x / y (no reference counterpart) */
     /* 1-D line */
     int jres;
     real *je, *jh;
@@ -157,15 +168,26 @@ oracle_open(const char *parfile)
             o->liao[2] = (real *)zalloc((size_t)NPLANES * o->nx * o->ny, sizeof(real));
     }
 
+    o->npoles = 2;
+    o->perx = set->sb.bx0 == SV_BOUNDARY_PERIODIC;
+    o->pery = set->sb.by0 == SV_BOUNDARY_PERIODIC;
     for (q = 0; q < o->mp->nmat; q++)
-        if (o->mp->mats[q].type == SV_MAT_ADE)
+        if (o->mp->mats[q].type == SV_MAT_ADE) {
             o->ade = 1;
+            if (o->mp->mats[q].ade_npoles == 3)
+                o->npoles = 3;
+        }
     if (o->ade) {
         real **arrs[] = { &o->exp_, &o->eyp, &o->ezp, &o->dpx, &o->dpy, &o->dpz, &o->dpxp, &o->dpyp, &o->dpzp,
                           &o->px[0], &o->px[1], &o->py[0], &o->py[1], &o->pz[0], &o->pz[1],
                           &o->pxp[0], &o->pxp[1], &o->pyp[0], &o->pyp[1], &o->pzp[0], &o->pzp[1] };
         for (q = 0; q < (int)(sizeof(arrs) / sizeof(arrs[0])); q++)
             *arrs[q] = (real *)zalloc(o->n, sizeof(real));
+        if (o->npoles == 3) {
+            real **a3[] = { &o->px[2], &o->py[2], &o->pz[2], &o->pxp[2], &o->pyp[2], &o->pzp[2] };
+            for (q = 0; q < 6; q++)
+                *a3[q] = (real *)zalloc(o->n, sizeof(real));
+        }
     }
 
     if (o->mp->src->tsf) {
@@ -198,7 +220,7 @@ oracle_close(void *handle)
     if (o->ade) {
         free(o->exp_); free(o->eyp); free(o->ezp); free(o->dpx); free(o->dpy); free(o->dpz);
         free(o->dpxp); free(o->dpyp); free(o->dpzp);
-        for (q = 0; q < 2; q++) {
+        for (q = 0; q < o->npoles; q++) {
             free(o->px[q]); free(o->py[q]); free(o->pz[q]); free(o->pxp[q]); free(o->pyp[q]); free(o->pzp[q]);
         }
     }
@@ -307,12 +329,16 @@ ystep_h(Oracle *o)
                 real ex = o->ex[c], ey = o->ey[c], ez = o->ez[c];
                 kappas(o, i, j, k, 0, &kx, &ky, &kz);
                 get_da_db(o, i, j, k, &da, &dbx, &dby, &dbz, &db);
-                if (!(j == ny - 1 || k == nz - 1))
-                    o->hx[c] = da * o->hx[c] + dbx * ((o->ey[c + 1] - ey) / kz - (o->ez[c + nz] - ez) / ky);
-                if (!(i == nx - 1 || k == nz - 1))
-                    o->hy[c] = da * o->hy[c] + dby * ((o->ez[c + (size_t)ny * nz] - ez) / kx - (o->ex[c + 1] - ex) / kz);
-                if (!(i == nx - 1 || j == ny - 1))
-                    o->hz[c] = da * o->hz[c] + dbz * ((o->ex[c + nz] - ex) / ky - (o->ey[c + (size_t)ny * nz] - ey) / kx);
+                /* +x / +y neighbours; on a periodic axis the far plane's is the near plane */
+                const size_t cxp = (i == nx - 1 && o->perx) ? c - (size_t)(nx - 1) * ny * nz : c + (size_t)ny * nz;
+                const size_t cyp = (j == ny - 1 && o->pery) ? c - (size_t)(ny - 1) * nz : c + nz;
+                const int xe = i == nx - 1 && !o->perx, ye = j == ny - 1 && !o->pery;
+                if (!(ye || k == nz - 1))
+                    o->hx[c] = da * o->hx[c] + dbx * ((o->ey[c + 1] - ey) / kz - (o->ez[cyp] - ez) / ky);
+                if (!(xe || k == nz - 1))
+                    o->hy[c] = da * o->hy[c] + dby * ((o->ez[cxp] - ez) / kx - (o->ex[c + 1] - ex) / kz);
+                if (!(xe || ye))
+                    o->hz[c] = da * o->hz[c] + dbz * ((o->ex[cyp] - ex) / ky - (o->ey[cxp] - ey) / kx);
             }
     }
 }
@@ -337,27 +363,34 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
     size_t c = IDX(o, i, j, k);
     double dt = o->mp->d->dt, dx = o->mp->d->dx, dy = o->mp->d->dy, dz = o->mp->d->dz;
     double vexp = o->ex[c], veyp = o->ey[c], vezp = o->ez[c];
-    double cbx, cby, cbz, cb = 0, be, ce, vdpx, vdpy, vdpz, vpx[2], vpy[2], vpz[2];
+    double cbx, cby, cbz, cb = 0, be, ce, vdpx, vdpy, vdpz, vpx[3], vpy[3], vpz[3];
     int n;
+    /* -x / -y neighbours and interior tests of a periodic axis, as in ystep_e */
+    const size_t cxm = (i == 0 && o->perx) ? c + (size_t)(nx - 1) * sx : c - sx;
+    const size_t cym = (j == 0 && o->pery) ? c + (size_t)(ny - 1) * sy : c - sy;
+    const int xi = i > 0 || o->perx, yi = j > 0 || o->pery;
 
-    if (j > 0 && k > 0 && xtbc == 0)
-        o->ex[c] = m->ade_c0 * ((o->hz[c] - o->hz[c - sy]) / ky - (o->hy[c] - o->hy[c - 1]) / kz) / dx
+    if (yi && k > 0 && xtbc == 0)
+        o->ex[c] = m->ade_c0 * ((o->hz[c] - o->hz[cym]) / ky - (o->hy[c] - o->hy[c - 1]) / kz) / dx
                    + m->ade_c1 * (-m->ade_a0 * o->dpxp[c] + (1 - m->ade_a1) * o->dpx[c])
                    + m->ade_c1 * (-m->ade_bp0[0] * o->pxp[0][c] + (1 - m->ade_bp1[0]) * o->px[0][c])
                    + m->ade_c1 * (-m->ade_bp0[1] * o->pxp[1][c] + (1 - m->ade_bp1[1]) * o->px[1][c])
-                   + m->ade_c2 * o->exp_[c] + m->ade_c3 * o->ex[c];
-    if (i > 0 && k > 0 && ytbc == 0)
-        o->ey[c] = m->ade_c0 * ((o->hx[c] - o->hx[c - 1]) / kz - (o->hz[c] - o->hz[c - sx]) / kx) / dy
+                   + m->ade_c2 * o->exp_[c] + m->ade_c3 * o->ex[c]
+                   + (o->npoles == 3 ? m->ade_c1 * (-m->ade_bp0[2] * o->pxp[2][c] + (1 - m->ade_bp1[2]) * o->px[2][c]) : 0);
+    if (xi && k > 0 && ytbc == 0)
+        o->ey[c] = m->ade_c0 * ((o->hx[c] - o->hx[c - 1]) / kz - (o->hz[c] - o->hz[cxm]) / kx) / dy
                    + m->ade_c1 * (-m->ade_a0 * o->dpyp[c] + (1 - m->ade_a1) * o->dpy[c])
                    + m->ade_c1 * (-m->ade_bp0[0] * o->pyp[0][c] + (1 - m->ade_bp1[0]) * o->py[0][c])
                    + m->ade_c1 * (-m->ade_bp0[1] * o->pyp[1][c] + (1 - m->ade_bp1[1]) * o->py[1][c])
-                   + m->ade_c2 * o->eyp[c] + m->ade_c3 * o->ey[c];
-    if (i > 0 && j > 0 && ztbc == 0)
-        o->ez[c] = m->ade_c0 * ((o->hy[c] - o->hy[c - sx]) / kx - (o->hx[c] - o->hx[c - sy]) / ky) / dz
+                   + m->ade_c2 * o->eyp[c] + m->ade_c3 * o->ey[c]
+                   + (o->npoles == 3 ? m->ade_c1 * (-m->ade_bp0[2] * o->pyp[2][c] + (1 - m->ade_bp1[2]) * o->py[2][c]) : 0);
+    if (xi && yi && ztbc == 0)
+        o->ez[c] = m->ade_c0 * ((o->hy[c] - o->hy[cxm]) / kx - (o->hx[c] - o->hx[cym]) / ky) / dz
                    + m->ade_c1 * (-m->ade_a0 * o->dpzp[c] + (1 - m->ade_a1) * o->dpz[c])
                    + m->ade_c1 * (-m->ade_bp0[0] * o->pzp[0][c] + (1 - m->ade_bp1[0]) * o->pz[0][c])
                    + m->ade_c1 * (-m->ade_bp0[1] * o->pzp[1][c] + (1 - m->ade_bp1[1]) * o->pz[1][c])
-                   + m->ade_c2 * o->ezp[c] + m->ade_c3 * o->ez[c];
+                   + m->ade_c2 * o->ezp[c] + m->ade_c3 * o->ez[c]
+                   + (o->npoles == 3 ? m->ade_c1 * (-m->ade_bp0[2] * o->pzp[2][c] + (1 - m->ade_bp1[2]) * o->pz[2][c]) : 0);
 
     /* inline CPML of the ADE branch, src3d/yee.c:952-1057 (x0 uses cb, which is 0 here) */
     {
@@ -369,8 +402,8 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
             be = bnd->x0.be[i]; ce = bnd->x0.ce[i];
             o->f[0].pe1[p] = be * o->f[0].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dy;
             o->f[0].pe2[p] = be * o->f[0].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dz;
-            if (!(j == 0 || k == 0) && ytbc == 0) o->ey[c] -= dy * cb * o->f[0].pe1[p];
-            if (!(j == 0 || k == 0) && ztbc == 0) o->ez[c] += dz * cb * o->f[0].pe2[p];
+            if (yi && k != 0 && ytbc == 0) o->ey[c] -= dy * cb * o->f[0].pe1[p];
+            if (yi && k != 0 && ztbc == 0) o->ez[c] += dz * cb * o->f[0].pe2[p];
         }
         if (sb->by0 == SV_BOUNDARY_CPML && j > 1 && j < sb->depth_by0) {
             size_t p = pidx_y(o, sb->depth_by0, i, j, k);
@@ -378,8 +411,8 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
             be = bnd->y0.be[j]; ce = bnd->y0.ce[j];
             o->f[2].pe1[p] = be * o->f[2].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dz;
             o->f[2].pe2[p] = be * o->f[2].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dx;
-            if (!(i == 0 || k == 0) && xtbc == 0) o->ex[c] += dx * cbx * o->f[2].pe1[p];
-            if (!(i == 0 || k == 0) && ztbc == 0) o->ez[c] -= dz * cbz * o->f[2].pe2[p];
+            if (xi && k != 0 && xtbc == 0) o->ex[c] += dx * cbx * o->f[2].pe1[p];
+            if (xi && k != 0 && ztbc == 0) o->ez[c] -= dz * cbz * o->f[2].pe2[p];
         }
         if (sb->bz0 == SV_BOUNDARY_CPML && k > 1 && k < sb->depth_bz0) {
             size_t p = pidx_z(o, sb->depth_bz0, i, j, k);
@@ -387,8 +420,8 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
             be = bnd->z0.be[k]; ce = bnd->z0.ce[k];
             o->f[4].pe1[p] = be * o->f[4].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dy;
             o->f[4].pe2[p] = be * o->f[4].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dx;
-            if (!(i == 0 || j == 0) && ytbc == 0) o->ey[c] += dy * cby * o->f[4].pe1[p];
-            if (!(i == 0 || j == 0) && xtbc == 0) o->ex[c] -= dx * cbx * o->f[4].pe2[p];
+            if (xi && yi && ytbc == 0) o->ey[c] += dy * cby * o->f[4].pe1[p];
+            if (xi && yi && xtbc == 0) o->ex[c] -= dx * cbx * o->f[4].pe2[p];
         }
         if (sb->bxn == SV_BOUNDARY_CPML && i >= nx - sb->depth_bxn + 1 && i < nx) {
             int ir = i - (nx - sb->depth_bxn);
@@ -397,8 +430,8 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
             be = bnd->xn.be[ir]; ce = bnd->xn.ce[ir];
             o->f[1].pe1[p] = be * o->f[1].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dy;
             o->f[1].pe2[p] = be * o->f[1].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dz;
-            if (!(j == 0 || k == 0) && ytbc == 0) o->ey[c] -= dy * cby * o->f[1].pe1[p];
-            if (!(j == 0 || k == 0) && ztbc == 0) o->ez[c] += dz * cbz * o->f[1].pe2[p];
+            if (yi && k != 0 && ytbc == 0) o->ey[c] -= dy * cby * o->f[1].pe1[p];
+            if (yi && k != 0 && ztbc == 0) o->ez[c] += dz * cbz * o->f[1].pe2[p];
         }
         if (sb->byn == SV_BOUNDARY_CPML && j >= ny - sb->depth_byn + 1 && j < ny) {
             int jr = j - (ny - sb->depth_byn);
@@ -407,8 +440,8 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
             be = bnd->yn.be[jr]; ce = bnd->yn.ce[jr];
             o->f[3].pe1[p] = be * o->f[3].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dx;
             o->f[3].pe2[p] = be * o->f[3].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dz;
-            if (!(i == 0 || k == 0) && xtbc == 0) o->ex[c] += dx * cbx * o->f[3].pe1[p];
-            if (!(i == 0 || k == 0) && ztbc == 0) o->ez[c] -= dz * cbz * o->f[3].pe2[p];
+            if (xi && k != 0 && xtbc == 0) o->ex[c] += dx * cbx * o->f[3].pe1[p];
+            if (xi && k != 0 && ztbc == 0) o->ez[c] -= dz * cbz * o->f[3].pe2[p];
         }
         if (sb->bzn == SV_BOUNDARY_CPML && k >= nz - sb->depth_bzn + 1 && k < nz) {
             int kr = k - (nz - sb->depth_bzn);
@@ -417,14 +450,14 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
             be = bnd->zn.be[kr]; ce = bnd->zn.ce[kr];
             o->f[5].pe1[p] = be * o->f[5].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dy;
             o->f[5].pe2[p] = be * o->f[5].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dx;
-            if (!(i == 0 || j == 0) && ytbc == 0) o->ey[c] += dy * cby * o->f[5].pe1[p];
-            if (!(i == 0 || j == 0) && xtbc == 0) o->ex[c] -= dx * cbx * o->f[5].pe2[p];
+            if (xi && yi && ytbc == 0) o->ey[c] += dy * cby * o->f[5].pe1[p];
+            if (xi && yi && xtbc == 0) o->ex[c] -= dx * cbx * o->f[5].pe2[p];
         }
     }
 
     /* polarisation updates, src3d/yee.c:1060-1106 */
     vdpx = o->dpx[c]; vdpy = o->dpy[c]; vdpz = o->dpz[c];
-    for (n = 0; n < 2; n++) {
+    for (n = 0; n < o->npoles; n++) {
         vpx[n] = o->px[n][c];
         vpy[n] = o->py[n][c];
         vpz[n] = o->pz[n][c];
@@ -432,7 +465,7 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
     o->dpx[c] = m->ade_a0 * o->dpxp[c] + m->ade_a1 * o->dpx[c] + m->ade_a2 * (o->exp_[c] + 2 * vexp + o->ex[c]);
     o->dpy[c] = m->ade_a0 * o->dpyp[c] + m->ade_a1 * o->dpy[c] + m->ade_a2 * (o->eyp[c] + 2 * veyp + o->ey[c]);
     o->dpz[c] = m->ade_a0 * o->dpzp[c] + m->ade_a1 * o->dpz[c] + m->ade_a2 * (o->ezp[c] + 2 * vezp + o->ez[c]);
-    for (n = 0; n < 2; n++) {
+    for (n = 0; n < o->npoles; n++) {
         o->px[n][c] = m->ade_bp0[n] * o->pxp[n][c] + m->ade_bp1[n] * o->px[n][c] + m->ade_bp2[n] * o->exp_[c]
                       + m->ade_bp3[n] * vexp + m->ade_bp4[n] * o->ex[c];
         o->py[n][c] = m->ade_bp0[n] * o->pyp[n][c] + m->ade_bp1[n] * o->py[n][c] + m->ade_bp2[n] * o->eyp[c]
@@ -442,7 +475,7 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
     }
     o->exp_[c] = vexp; o->eyp[c] = veyp; o->ezp[c] = vezp;
     o->dpxp[c] = vdpx; o->dpyp[c] = vdpy; o->dpzp[c] = vdpz;
-    for (n = 0; n < 2; n++) {
+    for (n = 0; n < o->npoles; n++) {
         o->pxp[n][c] = vpx[n];
         o->pyp[n][c] = vpy[n];
         o->pzp[n][c] = vpz[n];
@@ -478,13 +511,17 @@ ystep_e(Oracle *o)
                 if (mp->nmat > 0 && o->ade && xtbc * ytbc * ztbc == 0 && mp->mats[matindex].type == SV_MAT_ADE)
                     ade_cell(o, i, j, k, matindex, xtbc, ytbc, ztbc, kx, ky, kz);
                 if ((xtbc + ytbc + ztbc) != 0) {
+                    /* -x / -y neighbours; on a periodic axis the near plane's is the far plane */
+                    const size_t cxm = (i == 0 && o->perx) ? c + (size_t)(nx - 1) * sx : c - sx;
+                    const size_t cym = (j == 0 && o->pery) ? c + (size_t)(ny - 1) * sy : c - sy;
+                    const int xi = i > 0 || o->perx, yi = j > 0 || o->pery;
                     e_coeffs(o, i, j, k, &cax, &cay, &caz, &cbx, &cby, &cbz);
-                    if (j > 0 && k > 0 && xtbc)
-                        o->ex[c] = cax * o->ex[c] + cbx * ((hz - o->hz[c - sy]) / ky - (hy - o->hy[c - 1]) / kz);
-                    if (i > 0 && k > 0 && ytbc)
-                        o->ey[c] = cay * o->ey[c] + cby * ((hx - o->hx[c - 1]) / kz - (hz - o->hz[c - sx]) / kx);
-                    if (i > 0 && j > 0 && ztbc)
-                        o->ez[c] = caz * o->ez[c] + cbz * ((hy - o->hy[c - sx]) / kx - (hx - o->hx[c - sy]) / ky);
+                    if (yi && k > 0 && xtbc)
+                        o->ex[c] = cax * o->ex[c] + cbx * ((hz - o->hz[cym]) / ky - (hy - o->hy[c - 1]) / kz);
+                    if (xi && k > 0 && ytbc)
+                        o->ey[c] = cay * o->ey[c] + cby * ((hx - o->hx[c - 1]) / kz - (hz - o->hz[cxm]) / kx);
+                    if (xi && yi && ztbc)
+                        o->ez[c] = caz * o->ez[c] + cbz * ((hy - o->hy[cxm]) / kx - (hx - o->hx[cym]) / ky);
                 }
                 /* PEC, src3d/yee.c:1139-1237 (no scattered-field source on this path) */
                 if (matindex >= 0 && i > 0 && j > 0 && k > 0) {
@@ -578,6 +615,8 @@ boundary_hstep(Oracle *o)
     const int nx = o->nx, ny = o->ny, nz = o->nz;
     const double dx = set->sp.dx, dy = set->sp.dy, dz = set->sp.dz;
     const size_t sx = (size_t)ny * nz, sy = nz;
+    /* interior columns of the slab loops: a periodic axis has no boundary planes */
+    const int ilo = o->perx ? 0 : 1, ihi = o->perx ? nx : nx - 1, jlo = o->pery ? 0 : 1, jhi = o->pery ? ny : ny - 1;
     int i, j, k;
 
     if (smb->bx0 == SV_BOUNDARY_PERIODIC || smb->bxn == SV_BOUNDARY_PERIODIC)
@@ -602,7 +641,7 @@ boundary_hstep(Oracle *o)
     if (sb->bx0 == SV_BOUNDARY_CPML)
         for (i = 0; i < sb->depth_bx0 - 1; i++) {
             #pragma omp parallel for private(k) schedule(static)
-            for (j = 1; j < ny - 1; j++)
+            for (j = jlo; j < jhi; j++)
                 for (k = 1; k < nz - 1; k++) {
                     double da, dbx, dby, dbz, db, bh = bnd->x0.bh[i], ch = bnd->x0.ch[i];
                     size_t c = IDX(o, i, j, k), p = pidx_x(o, i, j, k);
@@ -616,7 +655,7 @@ boundary_hstep(Oracle *o)
     if (sb->by0 == SV_BOUNDARY_CPML)
         for (j = 0; j < sb->depth_by0 - 1; j++) {
             #pragma omp parallel for private(k) schedule(static)
-            for (i = 1; i < nx - 1; i++)
+            for (i = ilo; i < ihi; i++)
                 for (k = 1; k < nz - 1; k++) {
                     double da, dbx, dby, dbz, db, bh = bnd->y0.bh[j], ch = bnd->y0.ch[j];
                     size_t c = IDX(o, i, j, k), p = pidx_y(o, sb->depth_by0, i, j, k);
@@ -630,8 +669,8 @@ boundary_hstep(Oracle *o)
     if (sb->bz0 == SV_BOUNDARY_CPML)
         for (k = 0; k < sb->depth_bz0 - 1; k++) {
             #pragma omp parallel for private(j) schedule(static)
-            for (i = 1; i < nx - 1; i++)
-                for (j = 1; j < ny - 1; j++) {
+            for (i = ilo; i < ihi; i++)
+                for (j = jlo; j < jhi; j++) {
                     double da, dbx, dby, dbz, db, bh = bnd->z0.bh[k], ch = bnd->z0.ch[k];
                     size_t c = IDX(o, i, j, k), p = pidx_z(o, sb->depth_bz0, i, j, k);
                     get_da_db(o, i, j, k, &da, &dbx, &dby, &dbz, &db);
@@ -642,7 +681,7 @@ boundary_hstep(Oracle *o)
                 }
         }
     if (sb->bxn == SV_BOUNDARY_CPML) {
-        for (j = 1; j < ny - 1; j++)
+        for (j = jlo; j < jhi; j++)
             for (k = 1; k < nz - 1; k++) {
                 size_t c = IDX(o, nx - 1, j, k);
                 o->hx[c] = o->hy[c] = o->hz[c] = 0;
@@ -650,7 +689,7 @@ boundary_hstep(Oracle *o)
         for (i = nx - sb->depth_bxn + 1; i < nx - 1; i++) {
             int ir = i - (nx - sb->depth_bxn);
             #pragma omp parallel for private(k) schedule(static)
-            for (j = 1; j < ny - 1; j++)
+            for (j = jlo; j < jhi; j++)
                 for (k = 1; k < nz - 1; k++) {
                     double da, dbx, dby, dbz, db, bh = bnd->xn.bh[ir], ch = bnd->xn.ch[ir];
                     size_t c = IDX(o, i, j, k), p = pidx_x(o, ir, j, k);
@@ -663,7 +702,7 @@ boundary_hstep(Oracle *o)
         }
     }
     if (sb->byn == SV_BOUNDARY_CPML) {
-        for (i = 1; i < nx - 1; i++)
+        for (i = ilo; i < ihi; i++)
             for (k = 1; k < nz - 1; k++) {
                 size_t c = IDX(o, i, ny - 1, k);
                 o->hx[c] = o->hy[c] = o->hz[c] = 0;
@@ -671,7 +710,7 @@ boundary_hstep(Oracle *o)
         for (j = ny - sb->depth_byn + 1; j < ny - 1; j++) {
             int jr = j - (ny - sb->depth_byn);
             #pragma omp parallel for private(k) schedule(static)
-            for (i = 1; i < nx - 1; i++)
+            for (i = ilo; i < ihi; i++)
                 for (k = 1; k < nz - 1; k++) {
                     double da, dbx, dby, dbz, db, bh = bnd->yn.bh[jr], ch = bnd->yn.ch[jr];
                     size_t c = IDX(o, i, j, k), p = pidx_y(o, sb->depth_byn, i, jr, k);
@@ -684,16 +723,16 @@ boundary_hstep(Oracle *o)
         }
     }
     if (sb->bzn == SV_BOUNDARY_CPML) {
-        for (i = 1; i < nx - 1; i++)
-            for (j = 1; j < ny - 1; j++) {
+        for (i = ilo; i < ihi; i++)
+            for (j = jlo; j < jhi; j++) {
                 size_t c = IDX(o, i, j, nz - 1);
                 o->hx[c] = o->hy[c] = o->hz[c] = 0;
             }
         for (k = nz - sb->depth_bzn + 1; k < nz - 1; k++) {
             int kr = k - (nz - sb->depth_bzn);
             #pragma omp parallel for private(j) schedule(static)
-            for (i = 1; i < nx - 1; i++)
-                for (j = 1; j < ny - 1; j++) {
+            for (i = ilo; i < ihi; i++)
+                for (j = jlo; j < jhi; j++) {
                     double da, dbx, dby, dbz, db, bh = bnd->zn.bh[kr], ch = bnd->zn.ch[kr];
                     size_t c = IDX(o, i, j, k), p = pidx_z(o, sb->depth_bzn, i, j, kr);
                     get_da_db(o, i, j, k, &da, &dbx, &dby, &dbz, &db);
@@ -805,8 +844,8 @@ boundary_estep(Oracle *o)
                     double be = bnd->x0.be[i], ce = bnd->x0.ce[i];
                     o->f[0].pe1[p] = be * o->f[0].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dx;
                     o->f[0].pe2[p] = be * o->f[0].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dx;
-                    if (!(j == 0 || k == 0) && ytbc == 1) o->ey[c] -= dx * cby * o->f[0].pe1[p];
-                    if (!(j == 0 || k == 0) && ztbc == 1) o->ez[c] += dx * cbz * o->f[0].pe2[p];
+                    if (((j != 0 || o->pery) && k != 0) && ytbc == 1) o->ey[c] -= dx * cby * o->f[0].pe1[p];
+                    if (((j != 0 || o->pery) && k != 0) && ztbc == 1) o->ez[c] += dx * cbz * o->f[0].pe2[p];
                 }
         }
     }
@@ -825,8 +864,8 @@ boundary_estep(Oracle *o)
                     double be = bnd->y0.be[j], ce = bnd->y0.ce[j];
                     o->f[2].pe1[p] = be * o->f[2].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dy;
                     o->f[2].pe2[p] = be * o->f[2].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dy;
-                    if (!(i == 0 || k == 0) && xtbc == 1) o->ex[c] += dy * cbx * o->f[2].pe1[p];
-                    if (!(i == 0 || k == 0) && ztbc == 1) o->ez[c] -= dy * cbz * o->f[2].pe2[p];
+                    if (((i != 0 || o->perx) && k != 0) && xtbc == 1) o->ex[c] += dy * cbx * o->f[2].pe1[p];
+                    if (((i != 0 || o->perx) && k != 0) && ztbc == 1) o->ez[c] -= dy * cbz * o->f[2].pe2[p];
                 }
         }
     }
@@ -845,9 +884,9 @@ boundary_estep(Oracle *o)
                     double be = bnd->z0.be[k], ce = bnd->z0.ce[k];
                     o->f[4].pe1[p] = be * o->f[4].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dz;
                     o->f[4].pe2[p] = be * o->f[4].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dz;
-                    if (!(i == 0 || j == 0) && ytbc == 1) o->ey[c] += dx * cby * o->f[4].pe1[p];
+                    if (((i != 0 || o->perx) && (j != 0 || o->pery)) && ytbc == 1) o->ey[c] += dx * cby * o->f[4].pe1[p];
                     /* src3d/boundary.c:800-801 tests ztbc for the Ex correction */
-                    if (!(i == 0 || j == 0) && ztbc == 1) o->ex[c] -= dx * cbx * o->f[4].pe2[p];
+                    if (((i != 0 || o->perx) && (j != 0 || o->pery)) && ztbc == 1) o->ex[c] -= dx * cbx * o->f[4].pe2[p];
                 }
         }
     }
@@ -862,8 +901,8 @@ boundary_estep(Oracle *o)
                     double be = bnd->xn.be[ir], ce = bnd->xn.ce[ir];
                     o->f[1].pe1[p] = be * o->f[1].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dx;
                     o->f[1].pe2[p] = be * o->f[1].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dx;
-                    if (!(j == 0 || k == 0) && ytbc == 1) o->ey[c] -= dx * cby * o->f[1].pe1[p];
-                    if (!(j == 0 || k == 0) && ztbc == 1) o->ez[c] += dx * cbz * o->f[1].pe2[p];
+                    if (((j != 0 || o->pery) && k != 0) && ytbc == 1) o->ey[c] -= dx * cby * o->f[1].pe1[p];
+                    if (((j != 0 || o->pery) && k != 0) && ztbc == 1) o->ez[c] += dx * cbz * o->f[1].pe2[p];
                 }
         }
     }
@@ -878,8 +917,8 @@ boundary_estep(Oracle *o)
                     double be = bnd->yn.be[jr], ce = bnd->yn.ce[jr];
                     o->f[3].pe1[p] = be * o->f[3].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dy;
                     o->f[3].pe2[p] = be * o->f[3].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dy;
-                    if (!(i == 0 || k == 0) && xtbc == 1) o->ex[c] += dy * cbx * o->f[3].pe1[p];
-                    if (!(i == 0 || k == 0) && ztbc == 1) o->ez[c] -= dy * cbz * o->f[3].pe2[p];
+                    if (((i != 0 || o->perx) && k != 0) && xtbc == 1) o->ex[c] += dy * cbx * o->f[3].pe1[p];
+                    if (((i != 0 || o->perx) && k != 0) && ztbc == 1) o->ez[c] -= dy * cbz * o->f[3].pe2[p];
                 }
         }
     }
@@ -894,8 +933,8 @@ boundary_estep(Oracle *o)
                     double be = bnd->zn.be[kr], ce = bnd->zn.ce[kr];
                     o->f[5].pe1[p] = be * o->f[5].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dz;
                     o->f[5].pe2[p] = be * o->f[5].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dz;
-                    if (!(i == 0 || j == 0) && ytbc == 1) o->ey[c] += dx * cby * o->f[5].pe1[p];
-                    if (!(i == 0 || j == 0) && xtbc == 1) o->ex[c] -= dx * cbx * o->f[5].pe2[p];
+                    if (((i != 0 || o->perx) && (j != 0 || o->pery)) && ytbc == 1) o->ey[c] += dx * cby * o->f[5].pe1[p];
+                    if (((i != 0 || o->perx) && (j != 0 || o->pery)) && xtbc == 1) o->ex[c] -= dx * cbx * o->f[5].pe2[p];
                 }
         }
     }
@@ -1003,7 +1042,7 @@ source_hstep(Oracle *o)
         const int X = set->sp.xres, Y = set->sp.yres, Z = set->sp.zres;
         jp_step(o);
 #define VE(ii, jj, kk) dline_val(o->je, o->jres, t->corr * sv_dcomp(ii, jj, kk, th, ph, 0, X, 0, Y, 0, Z))
-        for (j = t->box_j0; j <= t->box_jn; j++)
+        for (j = t->box_j0; j <= t->box_jn && j < Y; j++)
             for (k = t->box_k0; k <= t->box_kn; k++) {
                 if (!t->skip_i0 && sv_tsf_face_active(o->mp, set, t->box_i0, j, k)) {
                     double ve = VE(t->box_i0, j, k);
@@ -1016,7 +1055,7 @@ source_hstep(Oracle *o)
                     if (j < t->box_jn) o->hz[IDX(o, t->box_in, j, k)] -= hcor * sv_gey(ve, th, ph, ps);
                 }
             }
-        for (i = t->box_i0; i <= t->box_in; i++)
+        for (i = t->box_i0; i <= t->box_in && i < X; i++)
             for (k = t->box_k0; k <= t->box_kn; k++) {
                 if (!t->skip_j0 && sv_tsf_face_active(o->mp, set, i, t->box_j0, k)) {
                     double ve = VE(i, t->box_j0, k);
@@ -1029,8 +1068,8 @@ source_hstep(Oracle *o)
                     if (i < t->box_in) o->hz[IDX(o, i, t->box_jn, k)] += hcor * sv_gex(ve, th, ph, ps);
                 }
             }
-        for (i = t->box_i0; i <= t->box_in; i++)
-            for (j = t->box_j0; j <= t->box_jn; j++) {
+        for (i = t->box_i0; i <= t->box_in && i < X; i++)
+            for (j = t->box_j0; j <= t->box_jn && j < Y; j++) {
                 if (!t->skip_k0 && sv_tsf_face_active(o->mp, set, i, j, t->box_k0)) {
                     double ve = VE(i, j, t->box_k0);
                     int pec = sv_pool_is_pec(o->mp, i, j, t->box_k0);
@@ -1068,7 +1107,7 @@ source_estep(Oracle *o)
         double th = t->ia_theta, ph = t->ia_phi, ps = t->ia_psi, ecor = set->plan.dt / set->sp.dx;
         const int X = set->sp.xres, Y = set->sp.yres, Z = set->sp.zres;
 #define VH(ii, jj, kk) dline_val(o->jh, o->jres, t->corr * sv_dcomp(ii, jj, kk, th, ph, 0, X, 0, Y, 0, Z))
-        for (j = t->box_j0; j <= t->box_jn; j++)
+        for (j = t->box_j0; j <= t->box_jn && j < Y; j++)
             for (k = t->box_k0; k <= t->box_kn; k++) {
                 if (!t->skip_i0 && sv_tsf_face_active(o->mp, set, t->box_i0, j, k)) {
                     double vh = VH(t->box_i0 - 1, j, k), eps = get_epsilon(o, t->box_i0, j, k);
@@ -1081,7 +1120,7 @@ source_estep(Oracle *o)
                     if (k < t->box_kn) o->ez[IDX(o, t->box_in, j, k)] += ecor / eps * sv_ghy(vh, th, ph, ps);
                 }
             }
-        for (i = t->box_i0; i <= t->box_in; i++)
+        for (i = t->box_i0; i <= t->box_in && i < X; i++)
             for (k = t->box_k0; k <= t->box_kn; k++) {
                 if (!t->skip_j0 && sv_tsf_face_active(o->mp, set, i, t->box_j0, k)) {
                     double vh = VH(i, t->box_j0 - 1, k), eps = get_epsilon(o, i, t->box_j0, k);
@@ -1094,8 +1133,8 @@ source_estep(Oracle *o)
                     if (k < t->box_kn) o->ez[IDX(o, i, t->box_jn, k)] -= ecor / eps * sv_ghx(vh, th, ph, ps);
                 }
             }
-        for (i = t->box_i0; i <= t->box_in; i++)
-            for (j = t->box_j0; j <= t->box_jn; j++) {
+        for (i = t->box_i0; i <= t->box_in && i < X; i++)
+            for (j = t->box_j0; j <= t->box_jn && j < Y; j++) {
                 if (!t->skip_k0 && sv_tsf_face_active(o->mp, set, i, j, t->box_k0)) {
                     double vh = VH(i, j, t->box_k0 - 1), eps = get_epsilon(o, i, j, t->box_k0);
                     int pec = sv_pool_is_pec(o->mp, i, j, t->box_k0);
@@ -1265,6 +1304,38 @@ running_dft(Oracle *o)
     }
 }
 
+/* FLUX_DFT planes (no reference counterpart): per plane cell (plane, j, i
+ * order) and wavelength, acc += v exp(-i w t) for Ex, Ey at t = (step+1) dt and
+ * the means of Hx, Hy over planes k-1 and k at t = (step+1/2) dt */
+static void
+flux_dft(Oracle *o)
+{
+    const SvSetFlux *x = &o->set.sx;
+    SvFarfield *ff = o->mp->farfield;
+    const double te = (o->set.sc.step_act + 1) * o->set.plan.dt, th = (o->set.sc.step_act + 0.5) * o->set.plan.dt;
+    const long ni = x->i1 - x->i0, per = ni * (x->j1 - x->j0);
+    long q;
+    if (!ff->nflux)
+        return;
+    #pragma omp parallel for
+    for (q = 0; q < ff->flux_ncells; q++) {
+        const int p = (int)(q / per), j = x->j0 + (int)((q % per) / ni), i = x->i0 + (int)(q % ni);
+        const size_t c = IDX(o, i, j, x->k[p]), cb = IDX(o, i, j, x->k[p] - 1);
+        const double v0 = o->ex[c], v1 = o->ey[c];
+        const double v2 = 0.5 * ((double)o->hx[c] + o->hx[cb]), v3 = 0.5 * ((double)o->hy[c] + o->hy[cb]);
+        int w;
+        for (w = 0; w < ff->nflux; w++) {
+            double *a = ff->flux_data + ((size_t)q * ff->nflux + w) * 8;
+            const double ce = cos(ff->flux_omega[w] * te), se = -sin(ff->flux_omega[w] * te);
+            const double ch = cos(ff->flux_omega[w] * th), sh = -sin(ff->flux_omega[w] * th);
+            a[0] += v0 * ce; a[1] += v0 * se;
+            a[2] += v1 * ce; a[3] += v1 * se;
+            a[4] += v2 * ch; a[5] += v2 * sh;
+            a[6] += v3 * ch; a[7] += v3 * sh;
+        }
+    }
+}
+
 /* gradient-extension objective: running DFT of E over the OBJECTIVE_DFT box,
  * E after step n sampled at t = (n+1) dt (box i fastest, then j, then k) */
 static void
@@ -1324,6 +1395,7 @@ oracle_step(void *handle, int nsteps)
         getsum(o);
         farfield(o);
         running_dft(o);
+        flux_dft(o);
         objective_dft(o);
         getpoints(o);
         o->set.sc.step_act++;
@@ -1413,6 +1485,18 @@ oracle_get_dft(void *handle, double *out)
     long cnt = (long)ff->dft_ncells * ff->ndft * 8;
     if (out && cnt)
         memcpy(out, ff->dft_data, (size_t)cnt * sizeof(double));
+    return cnt;
+}
+
+/* FLUX_DFT accumulators [cell][nflux][8] (out may be NULL); returns the count */
+long
+oracle_get_flux(void *handle, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    SvFarfield *ff = o->mp->farfield;
+    long cnt = ff->flux_ncells * ff->nflux * 8;
+    if (out && cnt)
+        memcpy(out, ff->flux_data, (size_t)cnt * sizeof(double));
     return cnt;
 }
 

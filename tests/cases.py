@@ -8,6 +8,8 @@ folder and returns the parameter-file path.  The workloads:
   inverse_design  continuous-permittivity design region, CW Ez dipole below
                   it, DFT objective on a plane above it (gradient extension)
   sweep           64 seeded rough-surface runs of uneven grid sizes (batching)
+  textured_film   dispersive textured thin-film stack, periodic in x and y,
+                  three-pole silicon, flux spectra per layer (768 x 768 x 384)
 and small feature cases (CPML + ADE, TSF oblique, Liao + PEC, periodic, far field).
 """
 import math
@@ -225,6 +227,119 @@ def sweep(folder, nruns=64, seed=11, nsteps=2000, edges=SWEEP_EDGES):
     return paths, params
 
 
+def plane_wave_flux(folder, n=16, nz=120, nsteps=2200, dx=5e-9, depth=10, lam=500e-9,
+                    planes=(40, 70), lambdas=(450e-9, 500e-9, 550e-9), slab=None, points=()):
+    """Normal-incidence plane wave (theta = 0, travelling +z) in an n x n cell
+    periodic in x and y (domain-periodic boundaries), CPML in z; the TSF box
+    spans the cell from k = 20 up to nz - 20 and injects through its bottom
+    face only; FLUX_DFT on `planes`.  `slab` = (k0, k1, material line) fills
+    k0 <= k < k1 with a MEDIUM_VECTOR material; `points` adds OUT_POINT traces
+    at (i, j, k)."""
+    os.makedirs(folder, exist_ok=True)
+    dt = _dt(dx, dx, dx, 1.0)
+    write_tsf_source(os.path.join(folder, "pulse.txt"), gaussian_sine(nsteps, dt, lam, 2, delay_widths=3.0))
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (n, n, nz, dx, dx, dx), "COMP\n%d\n\n" % nsteps,
+             "BOUNDARY_X0\nperiodic\n\nBOUNDARY_XN\nperiodic\n\nBOUNDARY_Y0\nperiodic\n\nBOUNDARY_YN\nperiodic\n\n",
+             "BOUNDARY_Z0\ncpml %d 3 -1 0 1\n\nBOUNDARY_ZN\ncpml %d 3 -1 0 1\n\n" % (depth, depth),
+             "SOURCE_TSF\n0 0 %d %d %d %d 0 0 0 0 pulse.txt\n\n" % (20, n, n, nz - 20),
+             "TSF_SKIP\nkn\n\n",
+             "FLUX_DFT\n0 0 %d %d %d %s %d %s flux.txt\n\n"
+             % (n, n, len(planes), " ".join(str(k) for k in planes), len(lambdas),
+                " ".join("%.9g" % l for l in lambdas))]
+    for q, (i, j, k) in enumerate(points):
+        lines.append("OUT_POINT\nAll 1 %d %d %d pt%d\n\n" % (i, j, k, q))
+    if slab:
+        with open(os.path.join(folder, "slab.txt"), "w") as f:
+            f.write("8 0 0 %d %d %d %d %s\n" % (slab[0], n, n, slab[1], slab[2]))
+        lines.append("MEDIUM_VECTOR\nslab.txt\n\n")
+    path = os.path.join(folder, "plane_wave_flux.par")
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path
+
+
+AU_ADE = "5 1.1 0 1.32e16 1.2e14 0.8 -1.2 4e15 6e14 1.2 -1.0 5.2e15 1.6e15"
+
+
+def silicon_3pole(rng):
+    """Seeded three-pole silicon-like ADE line (type 7): two critical points and
+    a Lorentz term (a critical point with phi = 0), no Drude term."""
+    p1 = (rng.uniform(2.5, 3.5), rng.uniform(-0.7, -0.4), rng.uniform(5.0e15, 5.4e15), rng.uniform(1.5e14, 3e14))
+    p2 = (rng.uniform(1.5, 2.5), rng.uniform(-1.1, -0.8), rng.uniform(6.2e15, 6.6e15), rng.uniform(5e14, 8e14))
+    p3 = (rng.uniform(0.5, 1.0), 0.0, rng.uniform(3.0e15, 3.5e15), rng.uniform(1e14, 3e14))
+    return "7 1 0 0 0 " + " ".join("%.9g" % v for q in (p1, p2, p3) for v in q)
+
+
+def pyramid_height(nx, ny, npyr, base_lo, base_hi, rng):
+    """Height field (cells) of random upright square pyramids with 54.7 degree
+    facets (height = base / 2 * tan 54.7), positions uniform, wrapped
+    periodically in x and y; the texture is the maximum over the pyramids."""
+    t = math.tan(math.radians(54.7))
+    x = np.arange(nx)[:, None] + 0.5
+    y = np.arange(ny)[None, :] + 0.5
+    h = np.zeros((nx, ny))
+    for _ in range(npyr):
+        b = rng.uniform(base_lo, base_hi)
+        cx, cy = rng.uniform(0, nx), rng.uniform(0, ny)
+        dx = np.abs((x - cx + nx / 2) % nx - nx / 2)
+        dy = np.abs((y - cy + ny / 2) % ny - ny / 2)
+        h = np.maximum(h, b / 2 * t - t * np.maximum(dx, dy))
+    return h
+
+
+def textured_film(folder, nx=768, ny=768, nz=384, dx=4e-9, nsteps=6000, seed=3, nw=32, depth=16,
+                  au=25, si=75, sio2=20, npyr=None):
+    """Dispersive textured thin-film stack, periodic in x and y, CPML in z.
+    From the illuminated side: SiO2 (eps 2.13, `sio2` cells thick, conformal
+    on the texture), silicon (three-pole ADE, `si` cells plus random
+    pyramids of base U[100, 300] nm with 54.7 degree facets) and a gold-like
+    Drude + 2 critical-point back contact (`au` cells, continuing into the
+    CPML).  A normally incident broadband pulse (400-1100 nm) enters through
+    the TSF box's near-z face (the stack is stored upside down: the wave
+    travels +z); FLUX_DFT planes in air before the stack, in the silicon just
+    before the back contact and inside the contact give the reflected,
+    silicon-absorbed and gold-absorbed power at nw wavelengths."""
+    os.makedirs(folder, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    if npyr is None:
+        npyr = int(1.2 * nx * ny * dx * dx / (200e-9) ** 2)
+    h = pyramid_height(nx, ny, npyr, 100e-9 / dx, 300e-9 / dx, rng)
+    # natural orientation: z up, light from the top; k_nat = nz - 1 - k
+    kn = np.arange(nz)[None, None, :] + 0.5
+    z_au = depth + au
+    z_si = z_au + si
+    surf = z_si + h[:, :, None]
+    ids = np.zeros((nx, ny, nz), dtype=np.int32)
+    ids[(kn >= surf) & (kn < surf + sio2)] = 1
+    ids[(kn >= z_au) & (kn < surf)] = 2
+    ids[:, :, :z_au] = 3
+    ids = ids[:, :, ::-1]
+    write_vtk_ids(os.path.join(folder, "film.vtk"), ids)
+    with open(os.path.join(folder, "mats.txt"), "w") as f:
+        f.write("8 0 0 0 1 1 1 1 2.13 1 0 0\n")
+        f.write("8 1 0 0 2 1 1 %s\n" % silicon_3pole(rng))
+        f.write("8 2 0 0 3 1 1 %s\n" % AU_ADE)
+    dt = _dt(dx, dx, dx, 1.0)
+    write_tsf_source(os.path.join(folder, "pulse.txt"), gaussian_sine(nsteps, dt, 600e-9, 1.0, delay_widths=3.0))
+    top = nz - 1 - int(math.ceil(z_si + h.max() + sio2))     # lowest stack cell after the flip
+    k_tsf = depth + 8
+    planes = (max(k_tsf + 4, (k_tsf + top) // 2), nz - 1 - z_au - 2, nz - 1 - z_au + au // 2)
+    lambdas = np.linspace(400e-9, 1100e-9, nw)
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (nx, ny, nz, dx, dx, dx), "COMP\n%d\n\n" % nsteps,
+             "MATMODE_CHECK\n1\n\n",
+             "BOUNDARY_X0\nperiodic\n\nBOUNDARY_XN\nperiodic\n\nBOUNDARY_Y0\nperiodic\n\nBOUNDARY_YN\nperiodic\n\n",
+             "BOUNDARY_Z0\ncpml %d 3 -1 0 1\n\nBOUNDARY_ZN\ncpml %d 3 -1 0 1\n\n" % (depth, depth),
+             "MEDIUM_VECTOR\nmats.txt\n\n", "MEDIUM_VTK_DATA\nfilm.vtk\n\n",
+             "SOURCE_TSF\n0 0 %d %d %d %d 0 0 0 0 pulse.txt\n\n" % (k_tsf, nx, ny, nz - depth - 4),
+             "TSF_SKIP\nkn\n\n",
+             "FLUX_DFT\n0 0 %d %d %d %s %d %s flux.txt\n\n"
+             % (nx, ny, len(planes), " ".join(str(k) for k in planes), nw, " ".join("%.6g" % l for l in lambdas))]
+    path = os.path.join(folder, "textured_film.par")
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path
+
+
 def small_case(folder, kind, n=48, nsteps=160, seed=7):
     """Small feature cases for GPU/oracle parity."""
     os.makedirs(folder, exist_ok=True)
@@ -258,6 +373,23 @@ def small_case(folder, kind, n=48, nsteps=160, seed=7):
     elif kind == "ade_cpml":
         mats.append("8 %d %d %d %d %d %d %s" % (0, 0, 0, n, n, c - 4, DRUDE_SILVER))
         mats.append("4 %d %d %d 4 5 2 0 1.2e16 1e14 0.8 -1.2 4e15 4e14 1.5 -1.1 4.2e15 2.4e15" % (c + 8, c + 6, c + 2))
+        lines.append("BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\n")
+        lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c + 6))
+    elif kind.startswith("ade3_"):
+        # a sphere of a three-pole ADE material (type 7) over a Drude slab; the
+        # variants pin the third pole: "ade3_split" splits pole 1 into two equal
+        # poles of half amplitude (poles 1 and 3), "ade3_two" is the same medium as
+        # a two-pole type-5 material, "ade3_idle" has a third pole of zero
+        # amplitude, "ade3_lorentz" a genuine third pole (a Lorentz term: phi = 0)
+        p1, p2 = (0.8, -1.2, 4e15, 4e14), (1.5, -1.1, 4.2e15, 2.4e15)
+        half = (p1[0] / 2,) + p1[1:]
+        fmt = lambda poles: " ".join("%.9g" % v for q in poles for v in q)
+        med = {"ade3_split": "7 2 0 1.2e16 1e14 " + fmt((half, p2, half)),
+               "ade3_two": "5 2 0 1.2e16 1e14 " + fmt((p1, p2)),
+               "ade3_idle": "7 2 0 1.2e16 1e14 " + fmt((p1, p2, (0.0, 0.3, 3e15, 5e14))),
+               "ade3_lorentz": "7 2 0 1.2e16 1e14 " + fmt((p1, p2, (1.1, 0.0, 3.3e15, 5e14)))}[kind]
+        mats.append("8 %d %d %d %d %d %d %s" % (0, 0, 0, n, n, c - 4, DRUDE_SILVER))
+        mats.append("4 %d %d %d 5 %s" % (c + 8, c + 6, c + 2, med))
         lines.append("BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\n")
         lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c + 6))
     elif kind == "tsf_pec_liao":

@@ -35,6 +35,8 @@ def lib(precision="f64"):
         L.oracle_get_dft.restype = ctypes.c_long
         L.oracle_get_dft.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_sums.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_get_flux.restype = ctypes.c_long
+        L.oracle_get_flux.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_objective.restype = ctypes.c_double
         L.oracle_objective.argtypes = [ctypes.c_void_p]
         L.oracle_real_size.restype = ctypes.c_int
@@ -124,6 +126,14 @@ class Oracle:
         out = np.zeros(max(n, 1), dtype=np.float64)
         if n:
             self.L.oracle_get_dft(self.h, _dptr(out))
+        return out[:n]
+
+    def flux_acc(self):
+        """FLUX_DFT accumulators, flat [cell][wavelength][8]."""
+        n = self.L.oracle_get_flux(self.h, None)
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        if n:
+            self.L.oracle_get_flux(self.h, _dptr(out))
         return out[:n]
 
     def maps(self):
