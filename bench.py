@@ -155,6 +155,109 @@ def batch_bench(args, work):
             "metric": "batched / sequential throughput", "higher_is_better": True}
 
 
+def dilate(mask):
+    """cells that take a material's dispersive branch: the material's own
+    cells and the +1 neighbours whose averaged coefficients it feeds"""
+    out = mask.copy()
+    out[1:, :, :] |= mask[:-1, :, :]
+    out[:, 1:, :] |= mask[:, :-1, :]
+    out[:, :, 1:] |= mask[:, :, :-1]
+    out[1:, 1:, :] |= mask[:-1, :-1, :]
+    out[:, 1:, 1:] |= mask[:, :-1, :-1]
+    out[1:, :, 1:] |= mask[:-1, :, :-1]
+    return out
+
+
+def film_bench(args, work):
+    """Dispersive textured thin film: 768 x 768 x 384 cells of 4 nm, periodic
+    in x and y, CPML 16 in z; SiO2 / three-pole silicon with seeded pyramids /
+    Drude + 2 CP gold; normally incident 400-1100 nm pulse; flux DFT on three
+    planes at 32 wavelengths.  Timed: `film_steps` steps after autotuning and
+    20 untimed steps.  Algorithmic bytes by the reference formulation: 74 B
+    per cell and step, 32 B per CPML cell, and per dispersive cell the
+    reference's state arrays (Drude + 2 CP: 21 arrays = +168 B; three poles:
+    27 arrays = +216 B).  With `film_full` the whole 6000-step run is made and
+    the flux spectra are turned into reflectance, the fraction absorbed in
+    the silicon (and the lossless SiO2) between the first two planes, the
+    fraction absorbed in the gold contact and the fraction transmitted behind
+    it, normalised by the incident power of the same source through the same
+    planes in vacuum."""
+    import cases
+    from gsvit_copy_amd import Session
+    nx, ny, nz, depth, dx = 768, 768, 384, 16, 4e-9
+    t0 = time.time()
+    par = cases.textured_film(os.path.join(work, "film"), nx=nx, ny=ny, nz=nz, dx=dx, nsteps=6000)
+    log("bench: textured film written in %.1f s" % (time.time() - t0))
+    t0 = time.time()
+    s = Session(par)
+    info = s.info()
+    t_open = time.time() - t0
+    mat, _ = s.maps()
+    si = dilate(mat == 2)
+    au = dilate(mat == 3) & ~si
+    n_si, n_au = int(si.sum()), int(au.sum())
+    del mat, si, au
+    cells = float(nx * ny * nz)
+    cpml_cells = 2.0 * depth * nx * ny
+    h_bytes = 37.0 * cells + 16.0 * cpml_cells
+    e_bytes = 37.0 * cells + 16.0 * cpml_cells + 216.0 * n_si + 168.0 * n_au
+    shape, used = s.autotune(2)
+    s.run(20)
+    s.sync()
+    S = args.film_steps
+    t0 = time.perf_counter()
+    s.run(S)
+    s.sync()
+    el = time.perf_counter() - t0
+    s.set_timing(True)
+    s.run(10)
+    kt = s.kernel_times()
+    s.set_timing(False)
+    h_ms = kt["h_ms"] / max(kt["h_n"], 1)
+    e_ms = kt["e_ms"] / max(kt["e_n"], 1)
+    gc = cells * S / el / 1e9
+    log("bench: textured film %.3f ms/step, %.2f Gcell-updates/s; k_hstep %.3f ms, k_estep %.3f ms"
+        % (1e3 * el / S, gc, h_ms, e_ms))
+    out = {"workload": "dispersive textured thin film 768x768x384, 4 nm, periodic x/y, CPML 16 z, SiO2 / 3-pole Si "
+                       "with pyramids / Drude+2CP Au, TSF normal 400-1100 nm, flux DFT 3 planes x 32 wavelengths",
+           "cells": int(cells), "si_ade_cells": n_si, "au_ade_cells": n_au, "ade_poles": info["ade_poles"],
+           "timed_steps": S, "ms_per_step": round(1e3 * el / S, 4), "gcells": round(gc, 3),
+           "open_s": round(t_open, 2), "autotune_steps": used, "launch": shape,
+           "kernels": {"k_hstep": {"ms": round(h_ms, 4), "algorithmic_bytes": h_bytes,
+                                   "frac": round(h_bytes / (h_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                       "k_estep": {"ms": round(e_ms, 4), "algorithmic_bytes": e_bytes,
+                                   "frac": round(e_bytes / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+           "metric": "Gcell-updates/s", "higher_is_better": True}
+    if args.film_full:
+        t0 = time.perf_counter()
+        s.run()
+        s.sync()
+        t_rest = time.perf_counter() - t0
+        steps = s.info()["step"]
+        fl = s.flux()
+        f = s.fields()
+        finite = bool(np.isfinite(f).all())
+        del f
+        # incident power per wavelength: the same source and planes in vacuum;
+        # the normally incident field is uniform in x and y, so a 16 x 16 column
+        # of the periodic cell scaled by area gives it
+        vpar = cases.textured_film(os.path.join(work, "film_vacuum"), nx=16, ny=16, nz=nz, dx=dx, nsteps=6000,
+                                   media=False)
+        with Session(vpar) as v:
+            v.run()
+            pin = v.flux()[0] * (nx * ny) / (16.0 * 16.0)
+        lam = np.linspace(400e-9, 1100e-9, fl.shape[1])
+        out["full_run"] = {"steps": steps, "rest_s": round(t_rest, 2), "fields_finite": finite,
+                           "wavelength_nm": [round(float(v) * 1e9, 1) for v in lam],
+                           "reflectance": [round(float(v), 4) for v in 1 - fl[0] / pin],
+                           "absorbed_si": [round(float(v), 4) for v in (fl[0] - fl[1]) / pin],
+                           "absorbed_au": [round(float(v), 4) for v in (fl[1] - fl[3]) / pin],
+                           "transmitted": [round(float(v), 6) for v in fl[3] / pin]}
+        log("bench: textured film full run: %d steps, %.1f s after the timed part, finite %s" % (steps, t_rest, finite))
+    s.close()
+    return out
+
+
 _BATCH_E2E = """
 import sys
 sys.path.insert(0, sys.argv[1])
@@ -214,6 +317,10 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=400, help="timed steps of the batched-sweep comparison")
     ap.add_argument("--batch-e2e", action="store_true",
                     help="also time the whole 64-run sweep end to end (one process per run vs one batch)")
+    ap.add_argument("--no-film", action="store_true", help="skip the textured thin-film measurement")
+    ap.add_argument("--film-steps", type=int, default=300, help="timed steps of the textured thin film")
+    ap.add_argument("--film-full", action="store_true",
+                    help="also run the textured thin film to its 6000 steps and report its spectra")
     ap.add_argument("--launch", default=None,
                     help="fixed launch shape hx,hy,hz,ex,ey,ez (tile x width, rows, z-march depth of the H and E "
                          "kernels); implies no tuning")
@@ -243,16 +350,8 @@ def main():
 
     # ADE cells (dispersive cells and their +1 neighbours take the ADE branch)
     mat, bnds = s.maps()
-    metal = (mat == 1)
-    ade_cells = metal.copy()
-    ade_cells[1:, :, :] |= metal[:-1, :, :]
-    ade_cells[:, 1:, :] |= metal[:, :-1, :]
-    ade_cells[:, :, 1:] |= metal[:, :, :-1]
-    ade_cells[1:, 1:, :] |= metal[:-1, :-1, :]
-    ade_cells[:, 1:, 1:] |= metal[:, :-1, :-1]
-    ade_cells[1:, :, 1:] |= metal[:-1, :, :-1]
-    n_ade = int(ade_cells.sum())
-    del mat, bnds, metal, ade_cells
+    n_ade = int(dilate(mat == 1).sum())
+    del mat, bnds
 
     shape = s.get_launch()
     if args.launch:
@@ -312,6 +411,10 @@ def main():
         if args.batch_e2e:
             batch["end_to_end"] = batch_e2e(work)
 
+    film = None
+    if not args.no_film:
+        film = film_bench(args, work)
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -367,6 +470,7 @@ def main():
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "batch": batch,
+        "film": film,
     }
     print(json.dumps(out), flush=True)
 

@@ -29,6 +29,7 @@ struct GsvBatch {
     std::vector<Engine *> eng;
     hipStream_t st;
     int idb_h, idb_e, ade;
+    int per;                  /* some run has a periodic axis */
     int shape[8];              /* bx_h by_h kchunk_h bx_e by_e kchunk_e blocks_h blocks_e */
     GsvGrid *d_grids;
     GsvTsf *d_tsfs;
@@ -148,6 +149,9 @@ gsv_engine_batch_new(void *const *engines, int n)
             why = "class-id width differing from run 0";
         if (!why && b->eng[r]->dev != b->eng[0]->dev)
             why = "a different GPU than run 0";
+        for (int q = 0; !why && q < r; q++)   /* the pole state layout follows the pole count */
+            if (b->eng[r]->g.ade_poles && b->eng[q]->g.ade_poles && b->eng[r]->g.ade_poles != b->eng[q]->g.ade_poles)
+                why = "a critical-point pole count differing from another run's";
         if (why) {
             fprintf(stderr, "Error: run %d cannot be batched: %s\n", r, why);
             delete b;
@@ -159,8 +163,11 @@ gsv_engine_batch_new(void *const *engines, int n)
     /* the widest dispersive branch covers the others: a grid without ADE
      * cells (or without critical-point terms) never takes it */
     b->ade = 0;
-    for (Engine *E : b->eng)
+    b->per = 0;
+    for (Engine *E : b->eng) {
         b->ade = std::max(b->ade, E->g.ade ? (E->g.ade_poles ? E->g.ade_poles : 1) : 0);
+        b->per |= E->g.perx || E->g.pery;
+    }
 
     bool ok = hipSetDevice(b->eng[0]->dev) == hipSuccess && hipStreamCreate(&b->st) == hipSuccess;
     std::vector<GsvGrid> grids(n);
@@ -227,7 +234,7 @@ enqueue_step(GsvBatch *b)
     const int *s = b->shape;
     if (b->timing)
         CK(hipEventRecord(b->ev[0], b->st));
-    CK(gsv_launch_hstep_batch(b->d_grids, b->d_th, b->nth, b->idb_h, s[0], s[1], s[2], s[6], b->st));
+    CK(gsv_launch_hstep_batch(b->d_grids, b->d_th, b->nth, b->idb_h, b->per, s[0], s[1], s[2], s[6], b->st));
     if (b->timing)
         CK(hipEventRecord(b->ev[1], b->st));
     if (b->ntsf) {
@@ -236,7 +243,7 @@ enqueue_step(GsvBatch *b)
     }
     if (b->timing)
         CK(hipEventRecord(b->ev[2], b->st));
-    CK(gsv_launch_estep_batch(b->d_grids, b->d_te, b->nte, b->idb_e, b->ade, s[3], s[4], s[5], s[7], b->st));
+    CK(gsv_launch_estep_batch(b->d_grids, b->d_te, b->nte, b->idb_e, b->ade, b->per, s[3], s[4], s[5], s[7], b->st));
     if (b->timing)
         CK(hipEventRecord(b->ev[3], b->st));
     for (int axis = 0; b->ntsf && axis < 3; axis++)

@@ -74,6 +74,7 @@ typedef struct {
     float b2[3], b3[3], b4[3];    /* pole E coefficients: bp2, bp3, bp4 */
     float cbm[3];    /* material cb used by the ADE branch's inline CPML, src3d/yee.c:952-1057 */
     int poles;       /* critical-point poles with nonzero terms: 0, 2 or 3 (the third has no reference counterpart) */
+    int drude;       /* the Drude current is driven (da != 0); without it J stays 0 and is neither read nor written */
 } GsvAde;
 
 /* one CPML face on the device: grading coefficients per index along the

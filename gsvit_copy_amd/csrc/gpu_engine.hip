@@ -306,6 +306,7 @@ build_tables(Engine *E)
         for (int q = 0; q < 3; q++)
             if (m->cp3_a[q] != 0 || m->ade_bp0[q] != -1 || m->ade_bp1[q] != 2)
                 a.poles = q < 2 ? 2 : 3;
+        a.drude = a.da != 0.f;
         if (a.poles > E->g.ade_poles)
             E->g.ade_poles = a.poles;
         E->ade_slot_of_mat[n] = nslot++;
@@ -411,6 +412,11 @@ sv_gpu_new(SvPool *mp, SvSet *set)
             delete E;
             return NULL;
         }
+    }
+    if (set->sa.design && (E->g.perx || E->g.pery)) {
+        fprintf(stderr, "Error: design regions are not supported in pools with periodic boundaries\n");
+        delete E;
+        return NULL;
     }
     if (set->sa.design && (unsigned long long)E->g.plane * (E->g.nz + 2) * 4ull >= (1ull << 32)) {
         fprintf(stderr, "Error: a design region needs field arrays below 4 GiB (%d x %d x %d is too large)\n",

@@ -132,10 +132,10 @@ hipError_t gsv_launch_box_sum(const GsvGrid *g, const GsvBoxSum *b, int step, hi
 /* batched launches: grids / tsfs / tiles / runs / steps are device arrays;
  * nblocks 0 launches one workgroup per tile, otherwise nblocks workgroups
  * stride over the tile list */
-hipError_t gsv_launch_hstep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int bx, int by,
-                                  int kchunk, int nblocks, hipStream_t st);
-hipError_t gsv_launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int ade, int bx,
+hipError_t gsv_launch_hstep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int per, int bx,
                                   int by, int kchunk, int nblocks, hipStream_t st);
+hipError_t gsv_launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int ade, int per,
+                                  int bx, int by, int kchunk, int nblocks, hipStream_t st);
 hipError_t gsv_launch_tsf_line_batch(const GsvTsf *tsfs, const int *runs, int nruns, int *steps, hipStream_t st);
 hipError_t gsv_launch_tsf_faces_batch(const GsvGrid *grids, const GsvTsf *tsfs, const GsvFaceTile *tiles, int ntiles,
                                       int hside, hipStream_t st);

@@ -37,7 +37,7 @@
 /* ------------------------------------------------------------------ */
 
 /* column (i, j) of the H update over the planes of z chunk kz (kchunk planes each) */
-template <typename ID, typename OFF>
+template <typename ID, typename OFF, bool PER>
 __device__ __forceinline__ void
 hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
 {
@@ -55,11 +55,14 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
 
     const float rkx = gp(g.rkhx)[i], rky = gp(g.rkhy)[j];
     /* a periodic axis has no boundary planes: every column is interior, and the
-     * +x / +y neighbour of the far plane is the near plane */
-    const bool iin = (i >= 1 && i < nx - 1) || g.perx, jin = (j >= 1 && j < ny - 1) || g.pery;
-    const bool xe = i == nx - 1 && !g.perx, ye = j == ny - 1 && !g.pery;
-    const OFF dIp = (g.perx && i == nx - 1) ? (OFF)0 - (OFF)(nx - 1) * 4 : (OFF)4;
-    const OFF dJp = (g.pery && j == ny - 1) ? (OFF)0 - (OFF)(ny - 1) * dJ : dJ;
+     * +x / +y neighbour of the far plane is the near plane.  Pools without
+     * periodic axes use the PER = false build, whose neighbour offsets are the
+     * constants 4 and dJ (immediate and scalar operands of the loads). */
+    const bool perx = PER && g.perx, pery = PER && g.pery;
+    const bool iin = (i >= 1 && i < nx - 1) || perx, jin = (j >= 1 && j < ny - 1) || pery;
+    const bool xe = i == nx - 1 && !perx, ye = j == ny - 1 && !pery;
+    const OFF dIp = (perx && i == nx - 1) ? (OFF)0 - (OFF)(nx - 1) * 4 : (OFF)4;
+    const OFF dJp = (pery && j == ny - 1) ? (OFF)0 - (OFF)(ny - 1) * dJ : dJ;
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
     const bool sx0 = F0.on && i < F0.depth - 1 && jin;
     const int irn = i - (nx - F1.depth);
@@ -168,11 +171,11 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     }
 }
 
-template <typename ID, typename OFF>
+template <typename ID, typename OFF, bool PER>
 __global__ __launch_bounds__(512) void
 k_hstep(GsvGrid g, int kchunk)
 {
-    hstep_col<ID, OFF>(g, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+    hstep_col<ID, OFF, PER>(g, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
                        blockIdx.z, kchunk);
 }
 
@@ -186,7 +189,7 @@ k_hstep(GsvGrid g, int kchunk)
  * and their +1 neighbours.  ad/ap: this plane's ADE state blocks, la/lp: the
  * lane's offset in them; state q of a cell sits q * 256 bytes after its first
  * state. */
-template <int NP, typename OFF>
+template <int NP, bool PER, typename OFF>
 __device__ __forceinline__ void
 ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, int k, unsigned fl, float &ex,
          float &ey, float &ez, float hx, float hy, float hz, float hxkm1, float hykm1, float hzi, float hyi, float hzj, float hxj,
@@ -195,21 +198,30 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
     const GSV_GLOBAL GsvAde &m = gp(g.adem)[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
     const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
-    const bool xi = i > 0 || g.perx, yi = j > 0 || g.pery;   /* not on a (non-periodic) near plane */
+    const bool xi = i > 0 || (PER && g.perx), yi = j > 0 || (PER && g.pery);   /* not on a (non-periodic) near plane */
     const float vexp = ex, veyp = ey, vezp = ez;
     const float expv = ldsn(ad, la, 0), eypv = ldsn(ad, la, 256), ezpv = ldsn(ad, la, 512);
-    const float jx = ldsn(ad, la, 768), jy = ldsn(ad, la, 1024), jz = ldsn(ad, la, 1280);
+    /* state that stays zero (an undriven Drude current, the poles beyond the
+     * material's own in a pool of wider materials) is neither read nor written */
+    const bool dr = NP == 0 || m.drude;   /* Drude-only builds always carry J */
+    const int npl = NP ? m.poles : 0;
+    float jx = 0.f, jy = 0.f, jz = 0.f;
+    if (dr) {
+        jx = ldsn(ad, la, 768);
+        jy = ldsn(ad, la, 1024);
+        jz = ldsn(ad, la, 1280);
+    }
     /* pole state (NP poles): P of pole q, component c at slot 3q + c, J at slot 3 NP + 3q + c */
     float pp[3][3] = {}, jj[3][3] = {};
-    if (NP && m.poles) {
 #pragma unroll
-        for (int q = 0; q < NP; q++)
+    for (int q = 0; q < NP; q++)
+        if (q < npl) {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 pp[q][c] = ldsn(ap, lp, (3 * q + c) * 256);
                 jj[q][c] = ldsn(ap, lp, (3 * NP + 3 * q + c) * 256);
             }
-    }
+        }
     float cx = m.cj * jx, cy = m.cj * jy, cz = m.cj * jz;   /* dispersive current terms of the E update */
 #pragma unroll
     for (int q = 0; q < NP; q++) {
@@ -288,19 +300,23 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     }
 
     /* polarisation state (all three components, with the values at this point) */
-    stsn(ad, la, 768, m.dj * jx + m.da * (expv + 2.f * vexp + ex));
-    stsn(ad, la, 1024, m.dj * jy + m.da * (eypv + 2.f * veyp + ey));
-    stsn(ad, la, 1280, m.dj * jz + m.da * (ezpv + 2.f * vezp + ez));
-    if (NP && m.poles) {
+    if (dr) {
+        stsn(ad, la, 768, m.dj * jx + m.da * (expv + 2.f * vexp + ex));
+        stsn(ad, la, 1024, m.dj * jy + m.da * (eypv + 2.f * veyp + ey));
+        stsn(ad, la, 1280, m.dj * jz + m.da * (ezpv + 2.f * vezp + ez));
+    }
+    {
         const float ev[3] = { ex, ey, ez }, epv[3] = { expv, eypv, ezpv }, evp[3] = { vexp, veyp, vezp };
 #pragma unroll
         for (int q = 0; q < NP; q++)
+            if (q < npl) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const float nj = m.pj[q] * jj[q][c] + m.ps[q] * pp[q][c] + m.b2[q] * epv[c] + m.b3[q] * evp[c]
-                                 + m.b4[q] * ev[c];
-                stsn(ap, lp, (3 * q + c) * 256, pp[q][c] + nj);
-                stsn(ap, lp, (3 * NP + 3 * q + c) * 256, nj);
+                for (int c = 0; c < 3; c++) {
+                    const float nj = m.pj[q] * jj[q][c] + m.ps[q] * pp[q][c] + m.b2[q] * epv[c]
+                                     + m.b3[q] * evp[c] + m.b4[q] * ev[c];
+                    stsn(ap, lp, (3 * q + c) * 256, pp[q][c] + nj);
+                    stsn(ap, lp, (3 * NP + 3 * q + c) * 256, nj);
+                }
             }
     }
     stsn(ad, la, 0, vexp);
@@ -308,7 +324,7 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     stsn(ad, la, 512, vezp);
 }
 
-template <typename ID, int ADE, typename OFF, bool PC>
+template <typename ID, int ADE, typename OFF, bool PC, bool PER>
 __device__ __forceinline__ void
 estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
 {
@@ -338,9 +354,10 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     const bool syn = F3.on && j >= ny - F3.depth + 1;
     const bool zplane_xy = (F0.on && i == 0) || (F2.on && j == 0);
     /* a periodic axis has no near plane: the -x / -y neighbour of the near plane is the far plane */
-    const bool xi = i > 0 || g.perx, yi = j > 0 || g.pery;
-    const OFF dIm = (g.perx && i == 0) ? (OFF)(nx - 1) * 4 : (OFF)0 - (OFF)4;
-    const OFF dJm = (g.pery && j == 0) ? (OFF)(ny - 1) * dJ : (OFF)0 - dJ;
+    const bool perx = PER && g.perx, pery = PER && g.pery;
+    const bool xi = i > 0 || perx, yi = j > 0 || pery;
+    const OFF dIm = (perx && i == 0) ? (OFF)(nx - 1) * 4 : (OFF)0 - (OFF)4;
+    const OFF dJm = (pery && j == 0) ? (OFF)(ny - 1) * dJ : (OFF)0 - dJ;
 
     float hxkm1 = ldf(g.hxg, o - dP), hykm1 = ldf(g.hyg, o - dP);
     for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
@@ -356,7 +373,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
 
         if (ADE && (fl & GSV_E_ADE))
-            ade_cell<NP>(g, g.ade_d + (size_t)k * g.ade_plane_d, la,
+            ade_cell<NP, PER>(g, g.ade_d + (size_t)k * g.ade_plane_d, la,
                          NP ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr, lp, i, j, k, fl, ex, ey, ez,
                          hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
 
@@ -461,11 +478,11 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     }
 }
 
-template <typename ID, int ADE, typename OFF, bool PC>
+template <typename ID, int ADE, typename OFF, bool PC, bool PER>
 __global__ __launch_bounds__(512) void
 k_estep(GsvGrid g, int kchunk)
 {
-    estep_col<ID, ADE, OFF, PC>(g, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+    estep_col<ID, ADE, OFF, PC, PER>(g, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
                                 blockIdx.z, kchunk);
 }
 
@@ -986,8 +1003,12 @@ extern "C" hipError_t
 gsv_launch_hstep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st)
 {
     dim3 blk(bx, by, 1), grd((g->nx + bx - 1) / bx, (g->ny + by - 1) / by, (g->nz + kchunk - 1) / kchunk);
-    const bool o32 = offsets32(g);
-#define GSV_HSTEP(ID, OFF) hipLaunchKernelGGL((k_hstep<ID, OFF>), grd, blk, 0, st, *g, kchunk)
+    const bool o32 = offsets32(g), per = g->perx || g->pery;
+#define GSV_HSTEP(ID, OFF)                                                                      \
+    do {                                                                                         \
+        if (per) hipLaunchKernelGGL((k_hstep<ID, OFF, true>), grd, blk, 0, st, *g, kchunk);    \
+        else hipLaunchKernelGGL((k_hstep<ID, OFF, false>), grd, blk, 0, st, *g, kchunk);       \
+    } while (0)
     if (idbytes == 1) {
         if (o32) GSV_HSTEP(uint8_t, uint32_t);
         else GSV_HSTEP(uint8_t, uint64_t);
@@ -999,20 +1020,21 @@ gsv_launch_hstep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipS
     return hipGetLastError();
 }
 
-/* the per-cell coefficient branch (design region) is compiled in only for
- * pools that have one, so the production kernels carry none of its registers */
-template <typename ID, typename OFF, bool PC>
+/* the per-cell coefficient branch (design region) and the periodic
+ * neighbour offsets are compiled in only for pools that need them, so the
+ * production kernels carry none of their registers */
+template <typename ID, typename OFF, bool PC, bool PER>
 static void
 launch_estep(const GsvGrid *g, int ade, dim3 grd, dim3 blk, int kchunk, hipStream_t st)
 {
     if (ade == 0)
-        hipLaunchKernelGGL((k_estep<ID, 0, OFF, PC>), grd, blk, 0, st, *g, kchunk);
+        hipLaunchKernelGGL((k_estep<ID, 0, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
     else if (ade == 1)
-        hipLaunchKernelGGL((k_estep<ID, 1, OFF, PC>), grd, blk, 0, st, *g, kchunk);
+        hipLaunchKernelGGL((k_estep<ID, 1, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
     else if (ade == 2)
-        hipLaunchKernelGGL((k_estep<ID, 2, OFF, PC>), grd, blk, 0, st, *g, kchunk);
+        hipLaunchKernelGGL((k_estep<ID, 2, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
     else
-        hipLaunchKernelGGL((k_estep<ID, 3, OFF, PC>), grd, blk, 0, st, *g, kchunk);
+        hipLaunchKernelGGL((k_estep<ID, 3, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
 }
 
 /* dispersive build of a grid: 0 none, 1 Drude only, 2 / 3 Drude + 2 / 3 critical-point poles */
@@ -1029,18 +1051,22 @@ gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipS
     /* the dispersive branch is compiled in only when the pool has ADE cells,
      * and the critical-point terms only when some material has them */
     const int ade = ade_mode(g);
-    const bool o32 = offsets32(g), pc = g->pc_nx > 0;
-    if (pc && !o32)
-        return hipErrorInvalidValue;   /* design regions need 32-bit offsets (field arrays below 4 GiB) */
-    if (idbytes == 1) {
-        if (pc) launch_estep<uint8_t, uint32_t, true>(g, ade, grd, blk, kchunk, st);
-        else if (o32) launch_estep<uint8_t, uint32_t, false>(g, ade, grd, blk, kchunk, st);
-        else launch_estep<uint8_t, uint64_t, false>(g, ade, grd, blk, kchunk, st);
-    } else {
-        if (pc) launch_estep<uint16_t, uint32_t, true>(g, ade, grd, blk, kchunk, st);
-        else if (o32) launch_estep<uint16_t, uint32_t, false>(g, ade, grd, blk, kchunk, st);
-        else launch_estep<uint16_t, uint64_t, false>(g, ade, grd, blk, kchunk, st);
-    }
+    const bool o32 = offsets32(g), pc = g->pc_nx > 0, per = g->perx || g->pery;
+    if (pc && (!o32 || per))
+        return hipErrorInvalidValue;   /* design regions need 32-bit offsets (field arrays below 4 GiB), no periodic axes */
+#define GSV_ESTEP(ID)                                                                               \
+    do {                                                                                            \
+        if (pc) launch_estep<ID, uint32_t, true, false>(g, ade, grd, blk, kchunk, st);             \
+        else if (o32 && per) launch_estep<ID, uint32_t, false, true>(g, ade, grd, blk, kchunk, st); \
+        else if (o32) launch_estep<ID, uint32_t, false, false>(g, ade, grd, blk, kchunk, st);      \
+        else if (per) launch_estep<ID, uint64_t, false, true>(g, ade, grd, blk, kchunk, st);       \
+        else launch_estep<ID, uint64_t, false, false>(g, ade, grd, blk, kchunk, st);               \
+    } while (0)
+    if (idbytes == 1)
+        GSV_ESTEP(uint8_t);
+    else
+        GSV_ESTEP(uint16_t);
+#undef GSV_ESTEP
     return hipGetLastError();
 }
 
@@ -1169,24 +1195,24 @@ gsv_launch_dft_flush(const GsvDft *d, int first_step, int nsamples, hipStream_t 
 /* the tile's grid parameters (uniform, scalar loads)                       */
 /* ------------------------------------------------------------------ */
 
-template <typename ID, typename OFF>
+template <typename ID, typename OFF, bool PER>
 __global__ __launch_bounds__(512) void
 k_hstep_batch(const GsvGrid *__restrict__ grids, const GsvTile *__restrict__ tiles, int ntiles, int kchunk)
 {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const GsvTile e = tiles[t];
-        hstep_col<ID, OFF>(grids[e.run], e.tx * blockDim.x + threadIdx.x, e.ty * blockDim.y + threadIdx.y, e.kz, kchunk);
+        hstep_col<ID, OFF, PER>(grids[e.run], e.tx * blockDim.x + threadIdx.x, e.ty * blockDim.y + threadIdx.y, e.kz, kchunk);
     }
 }
 
-template <typename ID, int ADE, typename OFF>
+template <typename ID, int ADE, typename OFF, bool PER>
 __global__ __launch_bounds__(512) void
 k_estep_batch(const GsvGrid *__restrict__ grids, const GsvTile *__restrict__ tiles, int ntiles, int kchunk)
 {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const GsvTile e = tiles[t];
-        estep_col<ID, ADE, OFF, false>(grids[e.run], e.tx * blockDim.x + threadIdx.x, e.ty * blockDim.y + threadIdx.y,
-                                       e.kz, kchunk);
+        estep_col<ID, ADE, OFF, false, PER>(grids[e.run], e.tx * blockDim.x + threadIdx.x,
+                                            e.ty * blockDim.y + threadIdx.y, e.kz, kchunk);
     }
 }
 
@@ -1225,45 +1251,53 @@ batch_grid(int ntiles, int nblocks)
 }
 
 extern "C" hipError_t
-gsv_launch_hstep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int bx, int by, int kchunk,
-                       int nblocks, hipStream_t st)
-{
-    if (ntiles <= 0)
-        return hipSuccess;
-    const dim3 blk(bx, by, 1), grd = batch_grid(ntiles, nblocks);
-    if (idbytes == 1)
-        hipLaunchKernelGGL((k_hstep_batch<uint8_t, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
-    else
-        hipLaunchKernelGGL((k_hstep_batch<uint16_t, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
-    return hipGetLastError();
-}
-
-template <typename ID>
-static void
-launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int ade, dim3 grd, dim3 blk, int kchunk,
-                   hipStream_t st)
-{
-    if (ade == 0)
-        hipLaunchKernelGGL((k_estep_batch<ID, 0, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
-    else if (ade == 1)
-        hipLaunchKernelGGL((k_estep_batch<ID, 1, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
-    else if (ade == 2)
-        hipLaunchKernelGGL((k_estep_batch<ID, 2, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
-    else
-        hipLaunchKernelGGL((k_estep_batch<ID, 3, uint32_t>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
-}
-
-extern "C" hipError_t
-gsv_launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int ade, int bx, int by,
+gsv_launch_hstep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int per, int bx, int by,
                        int kchunk, int nblocks, hipStream_t st)
 {
     if (ntiles <= 0)
         return hipSuccess;
     const dim3 blk(bx, by, 1), grd = batch_grid(ntiles, nblocks);
-    if (idbytes == 1)
-        launch_estep_batch<uint8_t>(grids, tiles, ntiles, ade, grd, blk, kchunk, st);
+#define GSV_HB(ID, PER) hipLaunchKernelGGL((k_hstep_batch<ID, uint32_t, PER>), grd, blk, 0, st, grids, tiles, ntiles, kchunk)
+    if (idbytes == 1) {
+        if (per) GSV_HB(uint8_t, true);
+        else GSV_HB(uint8_t, false);
+    } else {
+        if (per) GSV_HB(uint16_t, true);
+        else GSV_HB(uint16_t, false);
+    }
+#undef GSV_HB
+    return hipGetLastError();
+}
+
+template <typename ID, bool PER>
+static void
+launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int ade, dim3 grd, dim3 blk, int kchunk,
+                   hipStream_t st)
+{
+    if (ade == 0)
+        hipLaunchKernelGGL((k_estep_batch<ID, 0, uint32_t, PER>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
+    else if (ade == 1)
+        hipLaunchKernelGGL((k_estep_batch<ID, 1, uint32_t, PER>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
+    else if (ade == 2)
+        hipLaunchKernelGGL((k_estep_batch<ID, 2, uint32_t, PER>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
     else
-        launch_estep_batch<uint16_t>(grids, tiles, ntiles, ade, grd, blk, kchunk, st);
+        hipLaunchKernelGGL((k_estep_batch<ID, 3, uint32_t, PER>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
+}
+
+extern "C" hipError_t
+gsv_launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int idbytes, int ade, int per, int bx,
+                       int by, int kchunk, int nblocks, hipStream_t st)
+{
+    if (ntiles <= 0)
+        return hipSuccess;
+    const dim3 blk(bx, by, 1), grd = batch_grid(ntiles, nblocks);
+    if (idbytes == 1) {
+        if (per) launch_estep_batch<uint8_t, true>(grids, tiles, ntiles, ade, grd, blk, kchunk, st);
+        else launch_estep_batch<uint8_t, false>(grids, tiles, ntiles, ade, grd, blk, kchunk, st);
+    } else {
+        if (per) launch_estep_batch<uint16_t, true>(grids, tiles, ntiles, ade, grd, blk, kchunk, st);
+        else launch_estep_batch<uint16_t, false>(grids, tiles, ntiles, ade, grd, blk, kchunk, st);
+    }
     return hipGetLastError();
 }
 

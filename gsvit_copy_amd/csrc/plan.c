@@ -484,6 +484,17 @@ load_sources(SvPool *mp, SvSet *set)
         ts->jp_dx = set->sp.dx / ts->corr;
         ts->jp_epsilon = t->layered_epsilon;
         ts->jp_mu = t->layered_mu;
+        /* The line's update (Mur condition on H at the source end, then the Yee
+         * update of that H, src3d/1dpool.c:83-162 in the order of
+         * src3d/source.c:1469-1473) grows without bound when its Courant number
+         * exceeds 0.75 (e.g. theta = 45 deg at the default time step: 0.816).
+         * The results stay the reference's; the warning names the remedy. */
+        {
+            const double s = SV_LIGHT_SPEED * set->plan.dt / ts->jp_dx;
+            if (s > 0.75)
+                fprintf(stderr, "Warning: the TSF auxiliary line runs at Courant number %.4f; its update diverges "
+                                "above 0.75 (DT_MULT <= %.4g keeps it stable)\n", s, set->sc.dtmult * 0.75 / s);
+        }
     }
     return 0;
 }

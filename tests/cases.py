@@ -129,6 +129,14 @@ def write_vtk_ids(path, ids):
 
 DRUDE_SILVER = "5 1 0 1.37e16 2.7e13 0 0 0 0 0 0 0 0"
 
+# Time-step multiplier of the oblique-incidence workloads.  The TSF auxiliary
+# line (src3d/1dpool.c: Mur condition on H at the source end, then the Yee
+# update of the same H) diverges when its Courant number c dt / (dx / corr)
+# exceeds 0.75; at the default dt that number is corr / sqrt(3), 0.816 for
+# theta = 45 deg (corr = sqrt 2), so the line grows by ~1e10 per 250 steps.
+# 0.9 keeps every incidence angle with phi = 0 below 0.75 (worst: 0.735).
+OBLIQUE_DT_MULT = 0.9
+
 
 def rough_surface(folder, n=512, nsteps=4000, dx=5e-9, depth=16, seed=2, rms_nm=20.0, corr_nm=100.0,
                   theta_deg=45.0, lam_center=600e-9, dft_lambdas=(500e-9, 600e-9, 700e-9), tsf_margin=4,
@@ -147,13 +155,13 @@ def rough_surface(folder, n=512, nsteps=4000, dx=5e-9, depth=16, seed=2, rms_nm=
     write_vtk_ids(os.path.join(folder, "surface.vtk"), ids)
     with open(os.path.join(folder, "metal.txt"), "w") as f:
         f.write("8 0 0 0 1 1 1 %s\n" % DRUDE_SILVER)
-    dt = _dt(dx, dx, dx, 1.0)
+    dt = _dt(dx, dx, dx, OBLIQUE_DT_MULT)
     write_tsf_source(os.path.join(folder, "pulse.txt"), gaussian_sine(nsteps, dt, lam_center, 4, delay_widths=2.0))
     b0 = depth + tsf_margin
     b1 = n - depth - tsf_margin - 1
     theta = math.pi - math.radians(theta_deg)   # travelling towards -z
     lines = ["POOL\n%d %d %d %g %g %g\n\n" % (n, n, n, dx, dx, dx), "COMP\n%d\n\n" % nsteps,
-             "MATMODE_CHECK\n1\n\n",
+             "DT_MULT\n%g\n\n" % OBLIQUE_DT_MULT, "MATMODE_CHECK\n1\n\n",
              "BOUNDARY_ALL\ncpml %d 3 -1 0 1\n\n" % depth,
              "MEDIUM_VECTOR\nmetal.txt\n\n", "MEDIUM_VTK_DATA\nsurface.vtk\n\n",
              "SOURCE_TSF\n%d %d %d %d %d %d %.17g 0 %.17g 0 pulse.txt\n\n" % (b0, b0, b0, b1, b1, b1, theta, math.pi / 2),
@@ -202,13 +210,13 @@ def sweep_run(folder, shape, seed, rms_nm, theta_deg, lam_center, nsteps=2000, d
     write_vtk_ids(os.path.join(folder, "surface.vtk"), ids)
     with open(os.path.join(folder, "metal.txt"), "w") as f:
         f.write("8 0 0 0 1 1 1 %s\n" % DRUDE_SILVER)
-    dt = _dt(dx, dx, dx, 1.0)
+    dt = _dt(dx, dx, dx, OBLIQUE_DT_MULT)
     write_tsf_source(os.path.join(folder, "pulse.txt"), gaussian_sine(nsteps, dt, lam_center, 4, delay_widths=2.0))
     b0 = [depth + tsf_margin] * 3
     b1 = [n - depth - tsf_margin - 1 for n in shape]
     theta = math.pi - math.radians(theta_deg)
     lines = ["POOL\n%d %d %d %g %g %g\n\n" % (nx, ny, nz, dx, dx, dx), "COMP\n%d\n\n" % nsteps,
-             "MATMODE_CHECK\n1\n\n",
+             "DT_MULT\n%g\n\n" % OBLIQUE_DT_MULT, "MATMODE_CHECK\n1\n\n",
              "BOUNDARY_ALL\ncpml %d 3 -1 0 1\n\n" % depth,
              "MEDIUM_VECTOR\nmetal.txt\n\n", "MEDIUM_VTK_DATA\nsurface.vtk\n\n",
              "SOURCE_TSF\n%d %d %d %d %d %d %.17g 0 %.17g 0 pulse.txt\n\n" % (*b0, *b1, theta, math.pi / 2),
@@ -232,7 +240,7 @@ def plane_wave_flux(folder, n=16, nz=120, nsteps=2200, dx=5e-9, depth=10, lam=50
     """Normal-incidence plane wave (theta = 0, travelling +z) in an n x n cell
     periodic in x and y (domain-periodic boundaries), CPML in z; the TSF box
     spans the cell from k = 20 up to nz - 20 and injects through its bottom
-    face only; FLUX_DFT on `planes`.  `slab` = (k0, k1, material line) fills
+    face only; FLUX_DFT on `planes` (none if empty).  `slab` = (k0, k1, material line) fills
     k0 <= k < k1 with a MEDIUM_VECTOR material; `points` adds OUT_POINT traces
     at (i, j, k)."""
     os.makedirs(folder, exist_ok=True)
@@ -242,10 +250,11 @@ def plane_wave_flux(folder, n=16, nz=120, nsteps=2200, dx=5e-9, depth=10, lam=50
              "BOUNDARY_X0\nperiodic\n\nBOUNDARY_XN\nperiodic\n\nBOUNDARY_Y0\nperiodic\n\nBOUNDARY_YN\nperiodic\n\n",
              "BOUNDARY_Z0\ncpml %d 3 -1 0 1\n\nBOUNDARY_ZN\ncpml %d 3 -1 0 1\n\n" % (depth, depth),
              "SOURCE_TSF\n0 0 %d %d %d %d 0 0 0 0 pulse.txt\n\n" % (20, n, n, nz - 20),
-             "TSF_SKIP\nkn\n\n",
-             "FLUX_DFT\n0 0 %d %d %d %s %d %s flux.txt\n\n"
-             % (n, n, len(planes), " ".join(str(k) for k in planes), len(lambdas),
-                " ".join("%.9g" % l for l in lambdas))]
+             "TSF_SKIP\nkn\n\n"]
+    if planes:
+        lines.append("FLUX_DFT\n0 0 %d %d %d %s %d %s flux.txt\n\n"
+                     % (n, n, len(planes), " ".join(str(k) for k in planes), len(lambdas),
+                        " ".join("%.9g" % l for l in lambdas)))
     for q, (i, j, k) in enumerate(points):
         lines.append("OUT_POINT\nAll 1 %d %d %d pt%d\n\n" % (i, j, k, q))
     if slab:
@@ -258,16 +267,49 @@ def plane_wave_flux(folder, n=16, nz=120, nsteps=2200, dx=5e-9, depth=10, lam=50
     return path
 
 
-AU_ADE = "5 1.1 0 1.32e16 1.2e14 0.8 -1.2 4e15 6e14 1.2 -1.0 5.2e15 1.6e15"
+# gold-like Drude + 2 critical points (eps_inf, sigma, omega_D, gamma_D, then
+# A, phi, Omega, Gamma per pole; rad/s), in the range of published Drude-CP
+# fits of gold; passive (Im eps > 0 at every frequency)
+AU_ADE = "5 1.1431 0 1.3202e16 1.0805e14 0.26698 -1.2371 3.8711e15 4.4642e14 3.0834 -1.0968 4.1684e15 2.3555e15"
+
+PASSIVITY_OMEGA = np.logspace(11, 18, 20000)
+
+
+def ade_susceptibility(line, omega):
+    """chi(omega) of an ADE material line (type 5 or 7) for exp(-i omega t):
+    -omega_D^2 / (omega^2 + i gamma_D omega) plus, per pole,
+    A Omega (e^{i phi} / (Omega - omega - i Gamma) + e^{-i phi} / (Omega + omega + i Gamma))."""
+    v = [float(x) for x in line.split()]
+    wd, gd = v[3], v[4]
+    w = np.asarray(omega, dtype=float)
+    chi = np.zeros(w.shape, dtype=complex)
+    if wd:
+        chi -= wd ** 2 / (w ** 2 + 1j * gd * w)
+    for q in range((len(v) - 5) // 4):
+        a, phi, om, g = v[5 + 4 * q: 9 + 4 * q]
+        chi += a * om * (np.exp(1j * phi) / (om - w - 1j * g) + np.exp(-1j * phi) / (om + w + 1j * g))
+    return chi
 
 
 def silicon_3pole(rng):
-    """Seeded three-pole silicon-like ADE line (type 7): two critical points and
-    a Lorentz term (a critical point with phi = 0), no Drude term."""
-    p1 = (rng.uniform(2.5, 3.5), rng.uniform(-0.7, -0.4), rng.uniform(5.0e15, 5.4e15), rng.uniform(1.5e14, 3e14))
-    p2 = (rng.uniform(1.5, 2.5), rng.uniform(-1.1, -0.8), rng.uniform(6.2e15, 6.6e15), rng.uniform(5e14, 8e14))
-    p3 = (rng.uniform(0.5, 1.0), 0.0, rng.uniform(3.0e15, 3.5e15), rng.uniform(1e14, 3e14))
-    return "7 1 0 0 0 " + " ".join("%.9g" % v for q in (p1, p2, p3) for v in q)
+    """Seeded three-pole silicon-like ADE line (type 7): two critical points in
+    the near UV and a Lorentz term (a critical point with phi = 0) at the blue
+    edge, no Drude term; eps ~ 12 in the near infrared.  A critical point with
+    phase phi has a 1/detuning tail of sign -phi above and +phi below its
+    resonance, which only the Lorentzian part (Gamma / detuning^2) offsets, so
+    the phases are drawn small and negative (|phi| <= Gamma / Omega) and the
+    draw is repeated until Im chi >= 0 at every frequency up to 1e18 rad/s: an
+    active medium would grow without bound over a long run."""
+    for _ in range(100):
+        p = []
+        for a, om, g in ((rng.uniform(2.5, 3.5), rng.uniform(5.0e15, 5.4e15), rng.uniform(1.5e14, 3e14)),
+                         (rng.uniform(1.5, 2.5), rng.uniform(6.2e15, 6.6e15), rng.uniform(5e14, 8e14))):
+            p.append((a, -rng.uniform(0.0, 1.0) * g / om, om, g))
+        p.append((rng.uniform(0.5, 1.0), 0.0, rng.uniform(3.8e15, 4.2e15), rng.uniform(3e14, 6e14)))
+        line = "7 1 0 0 0 " + " ".join("%.9g" % v for q in p for v in q)
+        if ade_susceptibility(line, PASSIVITY_OMEGA).imag.min() >= 0:
+            return line
+    raise RuntimeError("no passive draw")
 
 
 def pyramid_height(nx, ny, npyr, base_lo, base_hi, rng):
@@ -288,17 +330,22 @@ def pyramid_height(nx, ny, npyr, base_lo, base_hi, rng):
 
 
 def textured_film(folder, nx=768, ny=768, nz=384, dx=4e-9, nsteps=6000, seed=3, nw=32, depth=16,
-                  au=25, si=75, sio2=20, npyr=None):
+                  au=25, si=75, sio2=20, npyr=None, media=True, gap=8):
     """Dispersive textured thin-film stack, periodic in x and y, CPML in z.
     From the illuminated side: SiO2 (eps 2.13, `sio2` cells thick, conformal
     on the texture), silicon (three-pole ADE, `si` cells plus random
     pyramids of base U[100, 300] nm with 54.7 degree facets) and a gold-like
-    Drude + 2 critical-point back contact (`au` cells, continuing into the
-    CPML).  A normally incident broadband pulse (400-1100 nm) enters through
+    Drude + 2 critical-point back contact (`au` cells), followed by `gap` air
+    cells before the CPML (a dispersive medium continued into the CPML takes
+    the reference's ADE-in-CPML update, which lets this metal's fields grow
+    over thousands of steps).  A normally incident broadband pulse (400-1100 nm) enters through
     the TSF box's near-z face (the stack is stored upside down: the wave
     travels +z); FLUX_DFT planes in air before the stack, in the silicon just
-    before the back contact and inside the contact give the reflected,
-    silicon-absorbed and gold-absorbed power at nw wavelengths."""
+    before the back contact, inside the contact and in the air behind it give
+    the reflected, silicon-absorbed, gold-absorbed and transmitted power at
+    nw wavelengths.  media=False
+    writes the same cell, source and planes in vacuum (the incident power
+    that normalises the spectra)."""
     os.makedirs(folder, exist_ok=True)
     rng = np.random.default_rng(seed)
     if npyr is None:
@@ -306,30 +353,34 @@ def textured_film(folder, nx=768, ny=768, nz=384, dx=4e-9, nsteps=6000, seed=3, 
     h = pyramid_height(nx, ny, npyr, 100e-9 / dx, 300e-9 / dx, rng)
     # natural orientation: z up, light from the top; k_nat = nz - 1 - k
     kn = np.arange(nz)[None, None, :] + 0.5
-    z_au = depth + au
+    z_au = depth + gap + au
     z_si = z_au + si
     surf = z_si + h[:, :, None]
     ids = np.zeros((nx, ny, nz), dtype=np.int32)
     ids[(kn >= surf) & (kn < surf + sio2)] = 1
     ids[(kn >= z_au) & (kn < surf)] = 2
-    ids[:, :, :z_au] = 3
+    ids[:, :, depth + gap:z_au] = 3
     ids = ids[:, :, ::-1]
-    write_vtk_ids(os.path.join(folder, "film.vtk"), ids)
-    with open(os.path.join(folder, "mats.txt"), "w") as f:
-        f.write("8 0 0 0 1 1 1 1 2.13 1 0 0\n")
-        f.write("8 1 0 0 2 1 1 %s\n" % silicon_3pole(rng))
-        f.write("8 2 0 0 3 1 1 %s\n" % AU_ADE)
+    si_line = silicon_3pole(rng)
+    if media:
+        write_vtk_ids(os.path.join(folder, "film.vtk"), ids)
+        with open(os.path.join(folder, "mats.txt"), "w") as f:
+            f.write("8 0 0 0 1 1 1 1 2.13 1 0 0\n")
+            f.write("8 1 0 0 2 1 1 %s\n" % si_line)
+            f.write("8 2 0 0 3 1 1 %s\n" % AU_ADE)
+    del ids
     dt = _dt(dx, dx, dx, 1.0)
     write_tsf_source(os.path.join(folder, "pulse.txt"), gaussian_sine(nsteps, dt, 600e-9, 1.0, delay_widths=3.0))
     top = nz - 1 - int(math.ceil(z_si + h.max() + sio2))     # lowest stack cell after the flip
     k_tsf = depth + 8
-    planes = (max(k_tsf + 4, (k_tsf + top) // 2), nz - 1 - z_au - 2, nz - 1 - z_au + au // 2)
+    planes = (max(k_tsf + 4, (k_tsf + top) // 2), nz - 1 - z_au - 2, nz - 1 - z_au + au // 2,
+              nz - 1 - depth - gap // 2)
     lambdas = np.linspace(400e-9, 1100e-9, nw)
     lines = ["POOL\n%d %d %d %g %g %g\n\n" % (nx, ny, nz, dx, dx, dx), "COMP\n%d\n\n" % nsteps,
              "MATMODE_CHECK\n1\n\n",
              "BOUNDARY_X0\nperiodic\n\nBOUNDARY_XN\nperiodic\n\nBOUNDARY_Y0\nperiodic\n\nBOUNDARY_YN\nperiodic\n\n",
              "BOUNDARY_Z0\ncpml %d 3 -1 0 1\n\nBOUNDARY_ZN\ncpml %d 3 -1 0 1\n\n" % (depth, depth),
-             "MEDIUM_VECTOR\nmats.txt\n\n", "MEDIUM_VTK_DATA\nfilm.vtk\n\n",
+             "MEDIUM_VECTOR\nmats.txt\n\nMEDIUM_VTK_DATA\nfilm.vtk\n\n" if media else "",
              "SOURCE_TSF\n0 0 %d %d %d %d 0 0 0 0 pulse.txt\n\n" % (k_tsf, nx, ny, nz - depth - 4),
              "TSF_SKIP\nkn\n\n",
              "FLUX_DFT\n0 0 %d %d %d %s %d %s flux.txt\n\n"

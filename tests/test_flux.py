@@ -92,6 +92,35 @@ def test_periodic_pool_keeps_plane_wave_uniform_and_translates(tmp_path):
     assert np.array_equal(moved, fields[1])
 
 
+def test_film_media_are_passive_and_absorb(tmp_path):
+    """The textured film's dispersive media: Im chi >= 0 at every frequency up
+    to 1e18 rad/s (chi as the ADE line defines it; a critical point with phi = 0
+    reduces to the textbook Lorentz form), and a 12-cell slab of each in the
+    periodic plane-wave cell absorbs (0 < flux behind < flux before) with the
+    fields decayed once the pulse has passed.  Media with a gain band fail the
+    last two: earlier synthetic lines with large critical-point phases left
+    |E| ~ 0.75 after 4000 steps and a negative flux."""
+    w = cases.PASSIVITY_OMEGA
+    a, om, g = 1.3, 4e15, 3e14
+    lor = cases.ade_susceptibility("7 1 0 0 0 %g 0 %g %g 0 0 0 0 0 0 0 0" % (a, om, g), w)
+    assert np.allclose(lor, 2 * a * om ** 2 / (om ** 2 + g ** 2 - w ** 2 - 2j * g * w), rtol=1e-10)
+    cases.textured_film(str(tmp_path / "tf"), nx=8, ny=8, nz=150, nsteps=10, nw=4, au=10, si=30, sio2=8)
+    lines = [ln.split(None, 7)[7] for ln in open(str(tmp_path / "tf" / "mats.txt"))]
+    ade = [ln for ln in lines if ln.split()[0] in ("5", "7")]
+    assert len(ade) == 2
+    for q, line in enumerate(ade):
+        assert cases.ade_susceptibility(line, w).imag.min() >= 0, line
+        par = cases.plane_wave_flux(str(tmp_path / ("s%d" % q)), n=4, nsteps=4000, slab=(50, 62, line),
+                                    planes=(40, 70))
+        o = Oracle(par)
+        o.step()
+        F = flux_from_acc(o.flux_acc(), 2, 3)
+        tail = float(np.abs(o.fields()[:3]).max())
+        print(line.split()[0], "flux before / behind the slab", F, "max |E| at the end", tail)
+        assert np.all(F[1] > 0) and np.all(F[1] < F[0]) and tail < 1e-2
+        o.close()
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("slab", [None, (50, 62, "7 2 0 1.2e16 1e14 0.8 -1.2 4e15 4e14 1.5 -1.1 4.2e15 2.4e15 "
                                               "1.1 0 3.3e15 5e14")])
