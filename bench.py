@@ -172,8 +172,8 @@ def film_bench(args, work):
     """Dispersive textured thin film: 768 x 768 x 384 cells of 4 nm, periodic
     in x and y, CPML 16 in z; SiO2 / three-pole silicon with seeded pyramids /
     Drude + 2 CP gold; normally incident 400-1100 nm pulse; flux DFT on three
-    planes at 32 wavelengths.  Timed: `film_steps` steps after autotuning and
-    20 untimed steps.  Algorithmic bytes by the reference formulation: 74 B
+    planes at 32 wavelengths.  Timed: `film_steps` steps after 20 untimed ones,
+    at the fixed launch shape `film_launch` (or autotuned with `film_tune`).  Algorithmic bytes by the reference formulation: 74 B
     per cell and step, 32 B per CPML cell, and per dispersive cell the
     reference's state arrays (Drude + 2 CP: 21 arrays = +168 B; three poles:
     27 arrays = +216 B).  With `film_full` the whole 6000-step run is made and
@@ -201,7 +201,11 @@ def film_bench(args, work):
     cpml_cells = 2.0 * depth * nx * ny
     h_bytes = 37.0 * cells + 16.0 * cpml_cells
     e_bytes = 37.0 * cells + 16.0 * cpml_cells + 216.0 * n_si + 168.0 * n_au
-    shape, used = s.autotune(2)
+    if args.film_tune:
+        shape, used = s.autotune(2)
+    else:
+        shape, used = [int(v) for v in args.film_launch.split(",")], 0
+        s.set_launch(shape)
     s.run(20)
     s.sync()
     S = args.film_steps
@@ -218,15 +222,26 @@ def film_bench(args, work):
     gc = cells * S / el / 1e9
     log("bench: textured film %.3f ms/step, %.2f Gcell-updates/s; k_hstep %.3f ms, k_estep %.3f ms"
         % (1e3 * el / S, gc, h_ms, e_ms))
+    traffic = {}
+    try:
+        with open(os.path.join(HERE, "profiles", "hbm_traffic_film768.json")) as f:
+            tr = json.load(f)
+        if tr["launch_shape"] == list(shape):
+            traffic = {k: round(v["traffic_bytes"]) for k, v in tr["kernels"].items()}
+    except (OSError, ValueError, KeyError):
+        pass
     out = {"workload": "dispersive textured thin film 768x768x384, 4 nm, periodic x/y, CPML 16 z, SiO2 / 3-pole Si "
                        "with pyramids / Drude+2CP Au, TSF normal 400-1100 nm, flux DFT 3 planes x 32 wavelengths",
            "cells": int(cells), "si_ade_cells": n_si, "au_ade_cells": n_au, "ade_poles": info["ade_poles"],
            "timed_steps": S, "ms_per_step": round(1e3 * el / S, 4), "gcells": round(gc, 3),
            "open_s": round(t_open, 2), "autotune_steps": used, "launch": shape,
            "kernels": {"k_hstep": {"ms": round(h_ms, 4), "algorithmic_bytes": h_bytes,
-                                   "frac": round(h_bytes / (h_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                                   "frac": round(h_bytes / (h_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                   "traffic": traffic.get("k_hstep")},
                        "k_estep": {"ms": round(e_ms, 4), "algorithmic_bytes": e_bytes,
-                                   "frac": round(e_bytes / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+                                   "frac": round(e_bytes / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                   "traffic": traffic.get("k_estep")}},
+           "traffic_source": "profiles/hbm_traffic_film768.md" if traffic else None,
            "metric": "Gcell-updates/s", "higher_is_better": True}
     if args.film_full:
         t0 = time.perf_counter()
@@ -319,6 +334,10 @@ def main():
                     help="also time the whole 64-run sweep end to end (one process per run vs one batch)")
     ap.add_argument("--no-film", action="store_true", help="skip the textured thin-film measurement")
     ap.add_argument("--film-steps", type=int, default=300, help="timed steps of the textured thin film")
+    ap.add_argument("--film-launch", default="256,2,8,256,1,8",
+                    help="launch shape of the textured film (as --launch; the autotuner's usual choice, and the "
+                         "shape of the committed counter summary)")
+    ap.add_argument("--film-tune", action="store_true", help="autotune the textured film's launch shape instead")
     ap.add_argument("--film-full", action="store_true",
                     help="also run the textured thin film to its 6000 steps and report its spectra")
     ap.add_argument("--launch", default=None,
