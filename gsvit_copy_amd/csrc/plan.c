@@ -688,6 +688,23 @@ test_material_file(const char *fname)
     return SV_MATMODE_NONE;
 }
 
+/* numbers of an entity line before its material (TetGen: after the file
+ * name and the two attribute integers), -1 for an unsupported type */
+static int
+entity_nfloats(int type)
+{
+    switch (type) {
+    case SV_ENTITY_SPHERE: return 4;
+    case SV_ENTITY_CYLINDER: return 7;
+    case SV_ENTITY_CONE: return 7;
+    case SV_ENTITY_RCONE: return 8;
+    case SV_ENTITY_VOXEL: return 6;
+    case SV_ENTITY_TETRAHEDRON: return 12;
+    case SV_ENTITY_TETGEN: return 6;
+    default: return -1;
+    }
+}
+
 static int
 test_script_file(const char *fname)
 {
@@ -699,13 +716,19 @@ test_script_file(const char *fname)
         return SV_MATMODE_ERROR;
     }
     while (fscanf(fr, "%d", &type) == 1) {
-        int npts = 0;
-        if (type == SV_ENTITY_SPHERE) npts = 4;
-        else if (type == SV_ENTITY_VOXEL) npts = 6;
-        else {
-            fprintf(stderr, "Error: entity type %d is not supported by the MI355X engine (4 sphere, 8 voxel)\n", type);
+        /* src3d/plan.c:590-692: the entity's numbers, then its material */
+        int npts = entity_nfloats(type);
+        if (npts < 0) {
+            fprintf(stderr, "Error: entity type %d is not supported by the MI355X engine (4 sphere, 7 cylinder, "
+                            "8 voxel, 12 tetrahedron, 21 TetGen mesh, 107 cone, 108 rounded cone)\n", type);
             fclose(fr);
             return SV_MATMODE_ERROR;
+        }
+        if (type == SV_ENTITY_TETGEN) {
+            char base[SV_NAME_LEN];
+            int a, b;
+            if (fscanf(fr, "%511s %d %d", base, &a, &b) != 3)
+                goto err;
         }
         if (scan_floats(fr, p, npts) || fscanf(fr, "%d", &mtype) != 1)
             goto err;
@@ -759,9 +782,245 @@ set_data(SvPool *mp, const SvMatProp *mat, int i, int j, int k)
 
 typedef struct {
     int type;
-    float p[6];
+    float p[12];
     SvMatProp mat;
 } SvEntity;
+
+/* ------------------------------------------------------------------ */
+/* entity geometry, src3d/plan.c:2062-2240: inside tests at voxel      */
+/* centres in float arithmetic (the reference's GLfloat), evaluated in */
+/* the reference's operation order (the C build has contraction off)   */
+/* ------------------------------------------------------------------ */
+
+static float
+geo_dist(const float *a, const float *b)
+{
+    return (float)sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+}
+
+/* (b - a) . (c - b) */
+static float
+geo_dot(const float *a, const float *b, const float *c)
+{
+    return (b[0] - a[0]) * (c[0] - b[0]) + (b[1] - a[1]) * (c[1] - b[1]) + (b[2] - a[2]) * (c[2] - b[2]);
+}
+
+static float
+geo_norm(float u, float v, float w)
+{
+    return (float)sqrt(u * u + v * v + w * w);
+}
+
+/* distance of point a from the segment b-c, 1e6 beyond either end's
+ * perpendicular plane */
+static float
+geo_segdist(const float *a, const float *b, const float *c)
+{
+    float u1, u2, u3, v1, v2, v3;
+    if (geo_dot(b, c, a) > 0 || geo_dot(c, b, a) > 0)
+        return 1e6f;
+    u1 = a[0] - b[0]; u2 = a[1] - b[1]; u3 = a[2] - b[2];
+    v1 = a[0] - c[0]; v2 = a[1] - c[1]; v3 = a[2] - c[2];
+    return (float)fabs(geo_norm(u2 * v3 - u3 * v2, u3 * v1 - u1 * v3, u1 * v2 - u2 * v1)
+                       / geo_norm(c[0] - b[0], c[1] - b[1], c[2] - b[2]));
+}
+
+static float
+geo_det3(float a11, float a12, float a13, float a21, float a22, float a23, float a31, float a32, float a33)
+{
+    return a11 * a22 * a33 - a11 * a23 * a32 + a12 * a23 * a31 - a12 * a21 * a33 + a13 * a21 * a32 - a13 * a22 * a31;
+}
+
+/* 4 x 4 determinant of rows a, b, c, d by expansion along a */
+static float
+geo_det4(const float *a, const float *b, const float *c, const float *d)
+{
+    const float m0 = geo_det3(b[1], b[2], b[3], c[1], c[2], c[3], d[1], d[2], d[3]);
+    const float m1 = geo_det3(b[0], b[2], b[3], c[0], c[2], c[3], d[0], d[2], d[3]);
+    const float m2 = geo_det3(b[0], b[1], b[3], c[0], c[1], c[3], d[0], d[1], d[3]);
+    const float m3 = geo_det3(b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]);
+    return a[0] * m0 - a[1] * m1 + a[2] * m2 - a[3] * m3;
+}
+
+static int
+geo_in_cylinder(const float *x, const float *p1, const float *p2, float r)
+{
+    return geo_segdist(x, p1, p2) < r;
+}
+
+/* radius grows from 0 at p1 to r at p2, measured by the distance from p1 */
+static int
+geo_in_cone(const float *x, const float *p1, const float *p2, float r)
+{
+    const float length = geo_dist(p1, p2), pos = geo_dist(x, p1);
+    return geo_segdist(x, p1, p2) < pos * r / length;
+}
+
+/* truncated cone, radius r1 at p1 and r2 at p2 along the axis */
+static int
+geo_in_rcone(const float *x, const float *p1, const float *p2, float r1, float r2)
+{
+    const float length = geo_dist(p1, p2);
+    float n[3], q[3], t;
+    n[0] = (p2[0] - p1[0]) / length;
+    n[1] = (p2[1] - p1[1]) / length;
+    n[2] = (p2[2] - p1[2]) / length;
+    t = (x[0] - p1[0]) * n[0] + (x[1] - p1[1]) * n[1] + (x[2] - p1[2]) * n[2];
+    q[0] = p1[0] + t * n[0];
+    q[1] = p1[1] + t * n[1];
+    q[2] = p1[2] + t * n[2];
+    return t >= 0 && t <= length && geo_dist(x, q) <= r1 + t * (r2 - r1) / length;
+}
+
+/* the point and each vertex-replaced determinant have one sign (zero counts
+ * as either), src3d/plan.c:2213-2240 */
+static int
+geo_in_tetrahedron(const float *x, const float *p)
+{
+    float v[4][4], xp[4], d[5];
+    int q;
+    for (q = 0; q < 4; q++) {
+        v[q][0] = p[3 * q];
+        v[q][1] = p[3 * q + 1];
+        v[q][2] = p[3 * q + 2];
+        v[q][3] = 1;
+    }
+    xp[0] = x[0]; xp[1] = x[1]; xp[2] = x[2]; xp[3] = 1;
+    d[0] = geo_det4(v[0], v[1], v[2], v[3]);
+    d[1] = geo_det4(xp, v[1], v[2], v[3]);
+    d[2] = geo_det4(v[0], xp, v[2], v[3]);
+    d[3] = geo_det4(v[0], v[1], xp, v[3]);
+    d[4] = geo_det4(v[0], v[1], v[2], xp);
+    return (d[0] >= 0 && d[1] >= 0 && d[2] >= 0 && d[3] >= 0 && d[4] >= 0)
+           || (d[0] <= 0 && d[1] <= 0 && d[2] <= 0 && d[3] <= 0 && d[4] <= 0);
+}
+
+static float
+fmin2(float a, float b)
+{
+    return a < b ? a : b;
+}
+
+static float
+fmax2(float a, float b)
+{
+    return a > b ? a : b;
+}
+
+/* TetGen mesh (base.node, base.ele) as tetrahedron entities appended to
+ * *ents, src3d/plan.c:3410-3626: node coordinates are read as float, scaled
+ * and shifted in double and rounded to float; comment and blank lines are
+ * skipped only before the two headers; nodes may be numbered from 0 or 1;
+ * node attributes are read as integers and boundary markers are not read.
+ * An element is kept when the files have no attributes or the filter
+ * (position, value) is off (-1); the reference then compares an attribute
+ * past the ones it read, which keeps nothing here. */
+static void
+tetgen_skip_comments(FILE *fr)
+{
+    char line[256];
+    int c;
+    while ((c = getc(fr)) != EOF) {
+        ungetc(c, fr);
+        if (c != '#' && c != '\n')
+            break;
+        if (!fgets(line, sizeof(line), fr))
+            break;
+    }
+}
+
+static int
+load_tetgen(const char *base, int att_pos, int att_val, const float *shift, const float *mult, const SvMatProp *mat,
+            SvEntity **ents, int *nent)
+{
+    char fname[SV_NAME_LEN + 8];
+    FILE *fr;
+    int nn, dim, natt, nbm, i, j, nt, npe, nattt, buf, corr = 0, kept = 0;
+    float *xyz;
+    snprintf(fname, sizeof(fname), "%s.node", base);
+    if (!(fr = fopen(fname, "r"))) {
+        fprintf(stderr, "Error: node file %s cannot be opened for reading\n", fname);
+        return 1;
+    }
+    tetgen_skip_comments(fr);
+    if (fscanf(fr, "%d %d %d %d", &nn, &dim, &natt, &nbm) != 4 || nn <= 0) {
+        fclose(fr);
+        fprintf(stderr, "Error: cannot read the header of %s\n", fname);
+        return 1;
+    }
+    tetgen_skip_comments(fr);
+    xyz = (float *)malloc(3 * (size_t)nn * sizeof(float));
+    for (i = 0; i < nn; i++) {
+        float v[3];
+        if (fscanf(fr, "%d %f %f %f", &buf, v, v + 1, v + 2) != 4) {
+            fclose(fr);
+            free(xyz);
+            fprintf(stderr, "Error: cannot read node %d of %s\n", i, fname);
+            return 1;
+        }
+        if (i == 0 && buf == 1)
+            corr = 1;
+        for (j = 0; j < 3; j++)
+            xyz[3 * i + j] = (float)((double)v[j] * (double)mult[j] + (double)shift[j]);
+        for (j = 0; j < natt; j++)
+            if (fscanf(fr, "%d", &buf) != 1)
+                break;
+    }
+    fclose(fr);
+
+    snprintf(fname, sizeof(fname), "%s.ele", base);
+    if (!(fr = fopen(fname, "r"))) {
+        free(xyz);
+        fprintf(stderr, "Error: element file %s cannot be opened for reading\n", fname);
+        return 1;
+    }
+    tetgen_skip_comments(fr);
+    if (fscanf(fr, "%d %d %d", &nt, &npe, &nattt) != 3 || nt < 0 || (npe != 4 && npe != 10)) {
+        fclose(fr);
+        free(xyz);
+        fprintf(stderr, "Error: cannot read the header of %s\n", fname);
+        return 1;
+    }
+    tetgen_skip_comments(fr);
+    for (i = 0; i < nt; i++) {
+        int tp[4];
+        if (fscanf(fr, "%d %d %d %d %d", &buf, tp, tp + 1, tp + 2, tp + 3) != 5)
+            goto bad;
+        for (j = 4; j < npe; j++)
+            if (fscanf(fr, "%d", &buf) != 1)
+                goto bad;
+        for (j = 0; j < nattt; j++)
+            if (fscanf(fr, "%d", &buf) != 1)
+                goto bad;
+        if (nattt == 0 || att_pos == -1 || att_val == -1) {
+            SvEntity e;
+            memset(&e, 0, sizeof(e));
+            e.type = SV_ENTITY_TETRAHEDRON;
+            for (j = 0; j < 4; j++) {
+                const int node = tp[j] - corr;
+                if (node < 0 || node >= nn)
+                    goto bad;
+                e.p[3 * j] = xyz[3 * node];
+                e.p[3 * j + 1] = xyz[3 * node + 1];
+                e.p[3 * j + 2] = xyz[3 * node + 2];
+            }
+            e.mat = *mat;
+            *ents = (SvEntity *)realloc(*ents, (*nent + 1) * sizeof(SvEntity));
+            (*ents)[(*nent)++] = e;
+            kept++;
+        }
+    }
+    fclose(fr);
+    free(xyz);
+    if (!kept && nt > 0)
+        fprintf(stderr, "Warning: no element of TetGen mesh %s passes its attribute filter (reference behaviour)\n", base);
+    return 0;
+bad:
+    fclose(fr);
+    free(xyz);
+    fprintf(stderr, "Error: cannot read element %d of %s\n", i, fname);
+    return 1;
+}
 
 /* fast reader for the ASCII STRUCTURED_POINTS integer map, src3d/plan.c:3877-3960 */
 static int
@@ -902,14 +1161,29 @@ load_data(SvPool *mp, SvSet *set)
             SvEntity e;
             memset(&e, 0, sizeof(e));
             e.type = type;
-            if (type == SV_ENTITY_SPHERE) {
-                if (scan_floats(fr, e.p, 4)) goto vec_err;
-            } else if (type == SV_ENTITY_VOXEL) {
-                if (scan_floats(fr, e.p, 6)) goto vec_err;
-            } else {
+            if (type == SV_ENTITY_TETGEN) {
+                char base[SV_NAME_LEN];
+                int att_pos, att_val;
+                if (fscanf(fr, "%511s %d %d", base, &att_pos, &att_val) != 3 || scan_floats(fr, e.p, 6)
+                    || scan_material(fr, &e.mat, set->plan.dt))
+                    goto vec_err;
+                if (e.mat.type != SV_MAT_LINEAR) {
+                    int mi = get_matindex(mp, &e.mat);
+                    if (mi == -1)
+                        add_material(mp, &e.mat);
+                    else
+                        e.mat.pos = mi;
+                }
+                if (load_tetgen(base, att_pos, att_val, e.p, e.p + 3, &e.mat, &ents, &nent))
+                    goto vec_err;
+                continue;
+            }
+            if (entity_nfloats(type) < 0) {
                 fprintf(stderr, "Error: entity type %d is not supported by the MI355X engine\n", type);
                 goto vec_err;
             }
+            if (scan_floats(fr, e.p, entity_nfloats(type)))
+                goto vec_err;
             if (scan_material(fr, &e.mat, set->plan.dt))
                 goto vec_err;
             if (e.mat.type != SV_MAT_LINEAR) {
@@ -960,12 +1234,54 @@ load_data(SvPool *mp, SvSet *set)
                             if (s <= rr)
                                 set_data(mp, &e->mat, i, j, k);
                         }
-            } else {
+            } else if (e->type == SV_ENTITY_VOXEL) {
                 /* src3d/plan.c:4467-4479 */
                 for (i = (int)(0 > e->p[0] ? 0 : e->p[0]); i < (xres < e->p[3] ? xres : e->p[3]); i++)
                     for (j = (int)(0 > e->p[1] ? 0 : e->p[1]); j < (yres < e->p[4] ? yres : e->p[4]); j++)
                         for (k = (int)(0 > e->p[2] ? 0 : e->p[2]); k < (zres < e->p[5] ? zres : e->p[5]); k++)
                             set_data(mp, &e->mat, i, j, k);
+            } else if (e->type == SV_ENTITY_TETRAHEDRON) {
+                /* src3d/plan.c:4418-4446: bounding box truncated to integers and clamped, inclusive */
+                int lo[3], hi[3], a;
+                for (a = 0; a < 3; a++) {
+                    const int res = a == 0 ? xres : a == 1 ? yres : zres;
+                    lo[a] = (int)fmin2(fmin2(e->p[a], e->p[3 + a]), fmin2(e->p[6 + a], e->p[9 + a]));
+                    hi[a] = (int)fmax2(fmax2(e->p[a], e->p[3 + a]), fmax2(e->p[6 + a], e->p[9 + a]));
+                    lo[a] = lo[a] < 0 ? 0 : lo[a] > res - 1 ? res - 1 : lo[a];
+                    hi[a] = hi[a] < 0 ? 0 : hi[a] > res - 1 ? res - 1 : hi[a];
+                }
+                if (geo_det4((const float[4]){ e->p[0], e->p[1], e->p[2], 1 }, (const float[4]){ e->p[3], e->p[4], e->p[5], 1 },
+                             (const float[4]){ e->p[6], e->p[7], e->p[8], 1 }, (const float[4]){ e->p[9], e->p[10], e->p[11], 1 }) == 0)
+                    fprintf(stderr, "Warning: degenerated tetrahedron\n");
+                for (i = lo[0]; i <= hi[0]; i++)
+                    for (j = lo[1]; j <= hi[1]; j++)
+                        for (k = lo[2]; k <= hi[2]; k++) {
+                            const float x[3] = { i + 0.5f, j + 0.5f, k + 0.5f };
+                            if (geo_in_tetrahedron(x, e->p))
+                                set_data(mp, &e->mat, i, j, k);
+                        }
+            } else {
+                /* cylinder, cone, rounded cone, src3d/plan.c:4516-4587: the box of both end points
+                 * widened by the (largest) radius + 1, float bounds as in the sphere loop */
+                const float *p1 = e->p, *p2 = e->p + 3;
+                const float r = e->type == SV_ENTITY_RCONE ? fmax2(e->p[6], e->p[7]) : e->p[6];
+                const float x0 = fmin2(p1[0], p2[0]) - r - 1, x1 = fmax2(p1[0], p2[0]) + r + 1;
+                const float y0 = fmin2(p1[1], p2[1]) - r - 1, y1 = fmax2(p1[1], p2[1]) + r + 1;
+                const float z0 = fmin2(p1[2], p2[2]) - r - 1, z1 = fmax2(p1[2], p2[2]) + r + 1;
+                for (i = (int)(0 > x0 ? 0 : x0); i < (xres < x1 ? xres : x1); i++)
+                    for (j = (int)(0 > y0 ? 0 : y0); j < (yres < y1 ? yres : y1); j++)
+                        for (k = (int)(0 > z0 ? 0 : z0); k < (zres < z1 ? zres : z1); k++) {
+                            const float x[3] = { i + 0.5f, j + 0.5f, k + 0.5f };
+                            int in;
+                            if (e->type == SV_ENTITY_CYLINDER)
+                                in = geo_in_cylinder(x, p1, p2, e->p[6]);
+                            else if (e->type == SV_ENTITY_CONE)
+                                in = geo_in_cone(x, p1, p2, e->p[6]);
+                            else
+                                in = geo_in_rcone(x, p1, p2, e->p[6], e->p[7]);
+                            if (in)
+                                set_data(mp, &e->mat, i, j, k);
+                        }
             }
         }
         free(ents);

@@ -72,10 +72,15 @@ typedef enum {
     SV_GPUMODE_FULL = 1
 } SvGpuMode;
 
-/* geometry entities of a MEDIUM_VECTOR file, src3d/settings.h (SV_ENTITY_*) */
+/* geometry entities of a MEDIUM_VECTOR file, src3d/plan.h:41-51 (SV_ENTITY_*) */
 typedef enum {
     SV_ENTITY_SPHERE = 4,
-    SV_ENTITY_VOXEL = 8
+    SV_ENTITY_CYLINDER = 7,
+    SV_ENTITY_VOXEL = 8,
+    SV_ENTITY_TETRAHEDRON = 12,
+    SV_ENTITY_TETGEN = 21,
+    SV_ENTITY_CONE = 107,
+    SV_ENTITY_RCONE = 108
 } SvEntityType;
 
 /* material description, src3d/settings.h:921-964 */

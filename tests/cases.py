@@ -416,6 +416,16 @@ def small_case(folder, kind, n=48, nsteps=160, seed=7):
         mats.append("8 %d %d %d %d %d %d 0 2.5 1 0 0" % (4, 4, 4, 12, 12, 10))
         lines.append("BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\n")
         lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c))
+    elif kind == "solids_cpml":
+        # every supported solid: cylinder, cone, rounded cone and tetrahedron of
+        # tabulated dielectrics, a Drude + 2 CP rounded cone, a PEC cylinder
+        mats.append("7 10.3 12.1 9.2 18.6 15.4 11.7 3.6 1 3.1 1 40 0")
+        mats.append("107 34.2 11.5 30.3 27.7 20.4 36.1 5.2 1 2.2 1 0 0")
+        mats.append("108 12.4 33.6 12.2 20.8 36.3 21.5 1.5 4.1 %s" % AU_ADE)
+        mats.append("12 28.3 30.1 9.4 38.6 34.2 13.3 31.7 39.1 18.8 35.2 31.4 20.6 1 4.5 1 0 0")
+        mats.append("7 30.5 8.5 36.5 36.5 8.5 36.5 2.4 10")
+        lines.append("BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\n")
+        lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c))
     elif kind == "magnetic_liao":
         mats.append("8 %d %d %d %d %d %d 0 1 2 0 300" % (c + 4, 10, 10, n - 10, n - 10, n - 10))
         mats.append("4 %d %d %d 5 1 1 3 0 0" % (c - 8, c - 8, c - 8))
@@ -478,7 +488,7 @@ def small_case(folder, kind, n=48, nsteps=160, seed=7):
 
 
 SMALL_KINDS = ["vacuum_none", "spheres_cpml", "magnetic_liao", "ade_cpml", "tsf_pec_liao", "tsf_ade_cpml",
-               "periodic_liao", "farfield_cpml"]
+               "periodic_liao", "farfield_cpml", "solids_cpml"]
 
 
 def write_linear_medium(path, eps_r, sigma=None, mu_r=None, sigast=None):
