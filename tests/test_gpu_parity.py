@@ -116,6 +116,26 @@ def test_sphere_scatter_reduced_against_oracle(tmp_path):
     assert np.array_equal(mo, ms) and np.array_equal(bo, bs)
 
 
+def test_sphere_scatter_full_size_against_oracle(tmp_path):
+    """The plain-FDTD correctness workload at its full size: 128^3, 10 nm cells,
+    Courant 0.5, CPML 10, 40 dielectric spheres, Ez dipole, 1000 steps, all six
+    field components and 16 point traces against the FP64 oracle; maps bit-exact."""
+    par = cases.sphere_scatter(str(tmp_path / "c1full"))
+    print("sphere scatter 128^3 x 1000 steps", flush=True)
+    o, s = run_both(par)
+    ref, got = o.fields(), s.fields()
+    errs = field_errors(got, ref)
+    terr = trace_errors(s.points(), o.points())
+    print("sphere scatter 128^3: fields", errs, "traces max", max(terr), "points", len(terr) // 2)
+    assert max(errs.values()) <= TOL
+    assert len(terr) >= 16 and max(terr) <= TOL
+    mo, bo = o.maps()
+    ms, bs = s.maps()
+    assert np.array_equal(mo, ms) and np.array_equal(bo, bs)
+    o.close()
+    s.close()
+
+
 def test_rough_surface_reduced_against_oracle(tmp_path):
     """The rough-surface workload (Drude metal below a rough interface, oblique
     p-polarised TSF plane wave, CPML) at 64^3 with CPML 8, 400 steps; maps
