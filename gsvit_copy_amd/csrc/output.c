@@ -2,11 +2,14 @@
  * output.c -- text logs of the step loop: point traces and time-domain far
  * field, in the reference formats (src3d/output.c:51-99 is_output,
  * :645-681 point logs "%d %g" / "%d %g %g %g %g %g %g", :690-709 sum logs, :1266-1271 far-field
- * log "%d %g %g %g" of the negated bins).  The running-DFT face data, which
- * has no reference counterpart, is written as one line per face cell.
+ * log "%d %g %g %g" of the negated bins), plane and volume outputs
+ * (:712-905 volumes: ASCII "%g", VTK STRUCTURED_POINTS, raw doubles;
+ * :1100-1263 planes: ASCII "%10.10g", raw doubles).  The running-DFT face data,
+ * which has no reference counterpart, is written as one line per face cell.
  */
 #include <stdio.h>
 #include <stdlib.h>
+#include <math.h>
 #include "gsvit_internal.h"
 
 int
@@ -26,7 +29,251 @@ is_output(SvPool *mp, SvSet *set)
         if (set->sc.step_act % set->so.sums[i].step == 0)
             return 1;
     }
+    for (i = 0; i < set->so.ncubs; i++) {
+        const SvOutputRegion *o = set->so.cubs + i;
+        if (o->step == 0)
+            return 0;
+        if (set->sc.step_act % o->step == 0 && !(set->sc.step_act < o->start || set->sc.step_act > o->stop))
+            return 1;
+    }
+    for (i = 0; i < set->so.nplns; i++) {
+        const SvOutputRegion *o = set->so.plns + i;
+        if (o->step == 0)
+            return 0;
+        if (set->sc.step_act % o->step == 0 && !(set->sc.step_act < o->start || set->sc.step_act > o->stop))
+            return 1;
+    }
     return 0;
+}
+
+static void
+vtk_header(FILE *fw, int xres, int yres, int zres, const char *description)
+{
+    /* src3d/output.c:139-151 */
+    fprintf(fw, "# vtk DataFile Version 2.0\n%s\nASCII\nDATASET STRUCTURED_POINTS\n", description);
+    fprintf(fw, "DIMENSIONS %d %d %d\nASPECT_RATIO 1 1 1\nORIGIN 0 0 0\nPOINT_DATA %d\n", xres, yres, zres,
+            xres * yres * zres);
+    fprintf(fw, "SCALARS volume_scalars double 1\nLOOKUP_TABLE default\n");
+}
+
+/* value of volume component c at (i, j, k); *is_int for integer outputs */
+static double
+volume_value(const SvPool *mp, const SvOutputRegion *o, int i, int j, int k, int *is_int)
+{
+    const SvYeeData *d = mp->d;
+    double eps, sig, mu, sigast;
+    int mi = d->mat ? d->mat->data[i][j][k] : 0;
+    *is_int = 0;
+    switch (o->component) {
+    case SV_OVOLUME_EX: return d->ex->data[i][j][k];
+    case SV_OVOLUME_EY: return d->ey->data[i][j][k];
+    case SV_OVOLUME_EZ: return d->ez->data[i][j][k];
+    case SV_OVOLUME_HX: return d->hx->data[i][j][k];
+    case SV_OVOLUME_HY: return d->hy->data[i][j][k];
+    case SV_OVOLUME_HZ: return d->hz->data[i][j][k];
+    case SV_OVOLUME_ALL:
+        return sqrt(d->ex->data[i][j][k] * d->ex->data[i][j][k] + d->ey->data[i][j][k] * d->ey->data[i][j][k]
+                    + d->ez->data[i][j][k] * d->ez->data[i][j][k]);
+    case SV_OVOLUME_EPSILON:
+        sv_pool_epsilon_sigma(mp, i, j, k, &eps, &sig);
+        return eps / SV_EPSILON_0;
+    case SV_OVOLUME_SIGMA:
+        sv_pool_epsilon_sigma(mp, i, j, k, &eps, &sig);
+        return sig;
+    case SV_OVOLUME_MU:
+        sv_pool_mu_sigast(mp, i, j, k, &mu, &sigast);
+        return mu / SV_MU_0;
+    case SV_OVOLUME_SIGAST:
+        sv_pool_mu_sigast(mp, i, j, k, &mu, &sigast);
+        return sigast;
+    case SV_OVOLUME_MAT:
+        *is_int = 1;
+        return mi;
+    case SV_OVOLUME_MATTYPE:
+        *is_int = 1;
+        return mp->mats[mi].type;
+    default:
+        return mp->out->outabsdata[o->k]->data[i][j][k];
+    }
+}
+
+/* OUT_VOLUME at this step, src3d/output.c:712-905 */
+static void
+write_volumes(SvPool *mp, SvSet *set)
+{
+    const SvYeeData *d = mp->d;
+    const int X = set->sp.xres, Y = set->sp.yres, Z = set->sp.zres;
+    static const char *vtk_names[] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "All", "Epsilon", "Sigma", "Mu",
+                                       "Sigast", "Material", "Mattype", "Abs", "Sumall", "Maxall" };
+    char filename[SV_NAME_LEN + 32];
+    int n, i, j, k, is_int;
+    for (n = 0; n < set->so.ncubs; n++) {
+        const SvOutputRegion *o = set->so.cubs + n;
+        const int acc = o->component >= SV_OVOLUME_ABS;
+        const int table = o->component == SV_OVOLUME_MAT || o->component == SV_OVOLUME_MATTYPE;
+        FILE *fh;
+        if (o->step == 0)
+            continue;
+        if (set->sc.step_act % o->step != 0 && set->sc.step_act < (set->sc.nsteps - 1))
+            continue;
+        if (set->sc.step_act < o->start || set->sc.step_act > o->stop)
+            continue;
+        if (acc) {
+            /* accumulated over the output steps: sigma |E|^2, |E|^2 or max |E|^2 */
+            SvDCube *a = mp->out->outabsdata[o->k];
+            for (i = 0; i < X; i++)
+                for (j = 0; j < Y; j++)
+                    for (k = 0; k < Z; k++) {
+                        const double e2 = d->ex->data[i][j][k] * d->ex->data[i][j][k]
+                                          + d->ey->data[i][j][k] * d->ey->data[i][j][k]
+                                          + d->ez->data[i][j][k] * d->ez->data[i][j][k];
+                        double w = 1, eps;
+                        if (o->component == SV_OVOLUME_ABS)
+                            sv_pool_epsilon_sigma(mp, i, j, k, &eps, &w);
+                        if (o->component == SV_OVOLUME_MAXALL)
+                            a->data[i][j][k] = a->data[i][j][k] > e2 ? a->data[i][j][k] : e2;
+                        else
+                            a->data[i][j][k] += w * e2;
+                    }
+        }
+        if (o->format == 1 || o->format == 2) {
+            if (acc)
+                snprintf(filename, sizeof(filename), o->format == 2 ? "%s.vtk" : "%s", o->filebase);
+            else
+                snprintf(filename, sizeof(filename), o->format == 2 ? "%s_%.4d.vtk" : "%s_%.4d", o->filebase,
+                         set->sc.step_act);
+            fh = fopen(filename, "w");
+            if (!fh) {
+                fprintf(stderr, "Error: volume: cannot open file %s for writing\n", filename);
+                continue;
+            }
+            if (o->format == 1) {
+                /* ASCII: the reference prints the E components for Hx, Hy, Hz here */
+                SvOutputRegion oe = *o;
+                if (oe.component >= SV_OVOLUME_HX && oe.component <= SV_OVOLUME_HZ)
+                    oe.component -= 3;
+                fprintf(fh, "%d %d %d %d\n", X, Y, Z, o->component);
+                if (table && mp->nmat == 0)
+                    fprintf(fh, "No data available\n");
+                else {
+                    for (k = 0; k < Z; k++) {
+                        for (j = 0; j < Y; j++) {
+                            for (i = 0; i < X; i++) {
+                                const double v = volume_value(mp, &oe, i, j, k, &is_int);
+                                if (is_int)
+                                    fprintf(fh, "%d ", (int)v);
+                                else
+                                    fprintf(fh, "%g ", v);
+                            }
+                            fprintf(fh, "\n");
+                        }
+                        fprintf(fh, "\n");
+                    }
+                }
+            } else {
+                vtk_header(fh, X, Y, Z, vtk_names[o->component]);
+                for (k = 0; k < Z; k++)
+                    for (j = 0; j < Y; j++)
+                        for (i = 0; i < X; i++) {
+                            double v;
+                            if (table && !d->mat) {
+                                fprintf(fh, "0\n");
+                                continue;
+                            }
+                            v = volume_value(mp, o, i, j, k, &is_int);
+                            if (is_int)
+                                fprintf(fh, "%d\n", (int)v);
+                            else
+                                fprintf(fh, "%g\n", v);
+                        }
+            }
+            fclose(fh);
+        } else {
+            /* raw doubles, x fastest; the accumulated outputs only once their window closes */
+            if (acc && !(set->sc.step_act > (o->stop - o->step)))
+                continue;
+            if (acc)
+                snprintf(filename, sizeof(filename), "%s.raw", o->filebase);
+            else
+                snprintf(filename, sizeof(filename), "%s_%.4d.raw", o->filebase, set->sc.step_act);
+            fh = fopen(filename, "wb");
+            if (!fh) {
+                fprintf(stderr, "Error: volume: cannot open file %s for writing\n", filename);
+                continue;
+            }
+            if (table && mp->nmat == 0)
+                fprintf(stderr, "No data available for binary output.\n");
+            else
+                for (k = 0; k < Z; k++)
+                    for (j = 0; j < Y; j++)
+                        for (i = 0; i < X; i++) {
+                            const double v = volume_value(mp, o, i, j, k, &is_int);
+                            fwrite(&v, sizeof(double), 1, fh);
+                        }
+            fclose(fh);
+        }
+    }
+}
+
+/* OUT_PLANE at this step, src3d/output.c:1100-1263: a plane of one field
+ * component, the plane's fixed index the one that is not -1 */
+static void
+write_planes(SvPool *mp, SvSet *set)
+{
+    const SvYeeData *d = mp->d;
+    char filename[SV_NAME_LEN + 32];
+    int n;
+    for (n = 0; n < set->so.nplns; n++) {
+        const SvOutputRegion *o = set->so.plns + n;
+        const SvDCube *c = NULL;
+        int a, b, na, nb, axis;
+        FILE *fh;
+        if (o->step == 0)
+            continue;
+        if (set->sc.step_act % o->step != 0 && set->sc.step_act < (set->sc.nsteps - 1))
+            continue;
+        if (set->sc.step_act <= o->start || set->sc.step_act > o->stop)
+            continue;
+        if (o->format)
+            snprintf(filename, sizeof(filename), "%s_%.4d", o->filebase, set->sc.step_act);
+        else
+            snprintf(filename, sizeof(filename), "%s_%.4d.raw", o->filebase, set->sc.step_act);
+        fh = fopen(filename, o->format ? "w" : "wb");
+        if (!fh) {
+            fprintf(stderr, "Error: plane: cannot open file %s for writing\n", filename);
+            continue;
+        }
+        switch (o->component) {
+        case SV_COMP_EX: c = d->ex; break;
+        case SV_COMP_EY: c = d->ey; break;
+        case SV_COMP_EZ: c = d->ez; break;
+        case SV_COMP_HX: c = d->hx; break;
+        case SV_COMP_HY: c = d->hy; break;
+        case SV_COMP_HZ: c = d->hz; break;
+        default:
+            fprintf(stderr, "Error: Unsupported output type for plane output\n");
+        }
+        for (axis = 0; axis < 3 && c; axis++) {
+            /* i fixed: rows k, columns j; j fixed: rows k, columns i; k fixed: rows j, columns i */
+            if ((axis == 0 && !(o->j == -1 && o->k == -1)) || (axis == 1 && !(o->i == -1 && o->k == -1))
+                || (axis == 2 && !(o->i == -1 && o->j == -1)))
+                continue;
+            nb = axis == 2 ? set->sp.yres : set->sp.zres;
+            na = axis == 0 ? set->sp.yres : set->sp.xres;
+            for (b = 0; b < nb; b++) {
+                for (a = 0; a < na; a++) {
+                    const double v = axis == 0 ? c->data[o->i][a][b] : axis == 1 ? c->data[a][o->j][b] : c->data[a][b][o->k];
+                    if (o->format)
+                        fprintf(fh, "%10.10g ", v);
+                    else
+                        fwrite(&v, sizeof(double), 1, fh);
+                }
+                if (o->format)
+                    fprintf(fh, "\n");
+            }
+        }
+        fclose(fh);
+    }
 }
 
 /* FLUX_DFT spectra from the accumulators: out[p * nw + w] = 1/2 Re sum over the
@@ -97,6 +344,9 @@ sv_pool_output(SvPool *mp, SvSet *set)
             fprintf(fh, "%d %g\n", j, mp->out->outsumdata[n][j] * (o->component == SV_SUM_MAX ? 1.0 : vol));
         fclose(fh);
     }
+
+    write_volumes(mp, set);
+    write_planes(mp, set);
 
     /* far-field logs are written at the last step only */
     if (set->sc.step_act != set->sc.nsteps - 1)

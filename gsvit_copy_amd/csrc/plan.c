@@ -1577,6 +1577,23 @@ init_and_make_plan(SvSet *set)
     }
 
     /* outputs, src3d/output.c (sv_pool_allocate_output) */
+    for (n = 0; n < set->so.nplns; n++) {
+        const SvOutputRegion *o = set->so.plns + n;
+        const int fixed = (o->i >= 0) + (o->j >= 0) + (o->k >= 0);
+        if (fixed != 1 || o->i >= xres || o->j >= yres || o->k >= zres) {
+            fprintf(stderr, "Error: output plane %d needs one index inside the pool and -1 for the other two\n", n);
+            return NULL;
+        }
+    }
+    mp->out->nabs = 0;
+    for (n = 0; n < set->so.ncubs; n++)
+        if (set->so.cubs[n].component >= SV_OVOLUME_ABS)
+            mp->out->nabs++;
+    if (mp->out->nabs) {
+        mp->out->outabsdata = (SvDCube **)calloc(mp->out->nabs, sizeof(SvDCube *));
+        for (n = 0; n < mp->out->nabs; n++)
+            mp->out->outabsdata[n] = sv_dcube_new(xres, yres, zres);
+    }
     if (set->so.npnts) {
         mp->out->outpointdata = (double **)calloc(set->so.npnts, sizeof(double *));
         for (n = 0; n < set->so.npnts; n++) {
@@ -1701,6 +1718,9 @@ sv_pool_free(SvPool *mp, SvSet *set)
                 free(mp->out->outsumdata[n]);
             free(mp->out->outsumdata);
         }
+        for (n = 0; n < mp->out->nabs; n++)
+            sv_dcube_free(mp->out->outabsdata[n]);
+        free(mp->out->outabsdata);
         free(mp->out);
     }
     if (mp->bnd) {

@@ -31,6 +31,8 @@ clear_settings(SvSet *set, int remove)
         free(set->ss.pnts);
         free(set->so.pnts);
         free(set->so.sums);
+        free(set->so.plns);
+        free(set->so.cubs);
         free(set->sf.ri);
         free(set->sf.rj);
         free(set->sf.rk);
@@ -161,6 +163,31 @@ get_component(FILE *fr, int *val, const char *key)
         return 1;
     }
     return 0;
+}
+
+/* OUT_VOLUME component names, src3d/settings.c:180-221 */
+static int
+get_volume_component(FILE *fr, int *val, const char *key)
+{
+    static const char *names[] = { "Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "All", "Epsilon", "Sigma", "Mu", "Sigast",
+                                   "Material", "Mattype", "Abs", "Sumall", "Maxall" };
+    char c[32];
+    int q;
+    if (get_token(fr, c, sizeof(c)))
+        return 1;
+    for (q = 0; q < (int)(sizeof(names) / sizeof(names[0])); q++) {
+        char lower[32];
+        int n;
+        for (n = 0; names[q][n]; n++)
+            lower[n] = (char)(names[q][n] | (n == 0 ? 0x20 : 0));
+        lower[n] = 0;
+        if (!strcmp(c, names[q]) || !strcmp(c, lower)) {
+            *val = q;
+            return 0;
+        }
+    }
+    fprintf(stderr, "Error: volume component %s is not supported (key %s)\n", c, key);
+    return 1;
 }
 
 static double
@@ -434,8 +461,43 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             if (get_rest_of_line(fr, o->filename, SV_NAME_LEN))
                 goto fail;
             set->so.nsums++;
-        } else if (!strcmp(key, "OUT_IMAGE") || !strcmp(key, "OUT_FILE") || !strcmp(key, "OUT_PLANE")
-                   || !strcmp(key, "OUT_VOLUME")) {
+        } else if (!strcmp(key, "OUT_PLANE")) {
+            /* src3d/settings.c:4869-4901: component step start stop format i j k file */
+            SvOutputRegion *o;
+            set->so.plns = (SvOutputRegion *)realloc(set->so.plns, (set->so.nplns + 1) * sizeof(SvOutputRegion));
+            o = set->so.plns + set->so.nplns;
+            memset(o, 0, sizeof(*o));
+            if (get_component(fr, &o->component, key) || get_int(fr, &o->step, key) || get_int(fr, &o->start, key)
+                || get_int(fr, &o->stop, key) || get_int(fr, &o->format, key) || get_int(fr, &o->i, key)
+                || get_int(fr, &o->j, key) || get_int(fr, &o->k, key))
+                goto fail;
+            if (get_rest_of_line(fr, o->filebase, SV_NAME_LEN))
+                goto fail;
+            set->so.nplns++;
+        } else if (!strcmp(key, "OUT_VOLUME")) {
+            /* src3d/settings.c:4902-4936: component step start stop format file; start -1 = 0,
+             * stop -1 = the step count read so far */
+            SvOutputRegion *o;
+            int q, nabs = 0;
+            set->so.cubs = (SvOutputRegion *)realloc(set->so.cubs, (set->so.ncubs + 1) * sizeof(SvOutputRegion));
+            o = set->so.cubs + set->so.ncubs;
+            memset(o, 0, sizeof(*o));
+            if (get_volume_component(fr, &o->component, key) || get_int(fr, &o->step, key)
+                || get_int(fr, &o->start, key) || get_int(fr, &o->stop, key) || get_int(fr, &o->format, key))
+                goto fail;
+            if (o->start == -1)
+                o->start = 0;
+            if (o->stop == -1)
+                o->stop = set->sc.nsteps;
+            if (get_rest_of_line(fr, o->filebase, SV_NAME_LEN))
+                goto fail;
+            for (q = 0; q < set->so.ncubs; q++)
+                if (set->so.cubs[q].component >= SV_OVOLUME_ABS)
+                    nabs++;
+            if (o->component >= SV_OVOLUME_ABS)
+                o->k = nabs;
+            set->so.ncubs++;
+        } else if (!strcmp(key, "OUT_IMAGE") || !strcmp(key, "OUT_FILE")) {
             skip_output_keyword(fr, key);
         } else if (!strcmp(key, "BOUNDARY_ALL")) {
             SvSetBoundary *b = &set->sb;

@@ -83,6 +83,8 @@ typedef struct {
 typedef struct {
     double **outpointdata;
     double **outsumdata;
+    SvDCube **outabsdata;   /* OUT_VOLUME ABS / SUMALL / MAXALL accumulators, one per such output */
+    int nabs;
 } SvOutput;
 
 /* CPML grading per face, src3d/boundary.h:70-152 (be, ce, bh, ch, kappae, kappah) */

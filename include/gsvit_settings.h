@@ -205,11 +205,46 @@ typedef struct {
     char filename[SV_NAME_LEN];
 } SvOutputSum;
 
+/* OUT_VOLUME components, src3d/settings.h:73-90 */
+typedef enum {
+    SV_OVOLUME_EX = 0,
+    SV_OVOLUME_EY = 1,
+    SV_OVOLUME_EZ = 2,
+    SV_OVOLUME_HX = 3,
+    SV_OVOLUME_HY = 4,
+    SV_OVOLUME_HZ = 5,
+    SV_OVOLUME_ALL = 6,        /* |E| */
+    SV_OVOLUME_EPSILON = 7,    /* relative */
+    SV_OVOLUME_SIGMA = 8,
+    SV_OVOLUME_MU = 9,         /* relative */
+    SV_OVOLUME_SIGAST = 10,
+    SV_OVOLUME_MAT = 11,       /* material index */
+    SV_OVOLUME_MATTYPE = 12,   /* material type */
+    SV_OVOLUME_ABS = 13,       /* sum over output steps of sigma |E|^2 */
+    SV_OVOLUME_SUMALL = 14,    /* sum over output steps of |E|^2 */
+    SV_OVOLUME_MAXALL = 15     /* maximum over output steps of |E|^2 */
+} SvOutputVolumeType;
+
+/* OUT_PLANE and OUT_VOLUME, src3d/settings.c:4869-4936 */
+typedef struct {
+    int component;          /* plane: SV_COMP_EX..HZ; volume: SvOutputVolumeType */
+    int step;               /* every `step` steps (and the last step) */
+    int start, stop;        /* step window */
+    int format;             /* plane: 0 raw doubles, else ASCII; volume: 0 raw doubles, 1 ASCII, 2 VTK */
+    int i, j, k;            /* plane: the fixed index, -1 on the two in-plane axes;
+                             * volume: k = accumulator slot of ABS / SUMALL / MAXALL */
+    char filebase[SV_NAME_LEN];
+} SvOutputRegion;
+
 typedef struct {
     int npnts;
     SvOutputPar *pnts;
     int nsums;
     SvOutputSum *sums;
+    int nplns;
+    SvOutputRegion *plns;
+    int ncubs;
+    SvOutputRegion *cubs;
 } SvSetOutput;
 
 typedef struct {
