@@ -70,6 +70,8 @@ struct Engine {
     /* far field */
     int has_ff;
     GsvFarfieldTD ff;
+    int has_pff;                  /* periodic far field */
+    GsvPFarfield pff;
     int has_dft;
     GsvDft dft;
     int dft_first, dft_pending;   /* pending ring samples: steps dft_first .. + dft_pending - 1 */

@@ -777,6 +777,59 @@ dft_ring_depth(int nw)
     return T > 32 ? 32 : (T < 1 ? 1 : T);
 }
 
+/* periodic far field, src3d/pfarfield.c:36-62; bins zeroed, point lists and
+ * istart uploaded once */
+static int
+setup_pfarfield(Engine *E)
+{
+    SvSet *set = E->set;
+    SvPool *mp = E->mp;
+    const SvSetPFarfield *sf = &set->spf;
+    GsvPFarfield &F = E->pff;
+    memset(&F, 0, sizeof(F));
+    E->has_pff = sf->nrs > 0;
+    if (!E->has_pff)
+        return 0;
+    F.i0 = sf->box_i0; F.j0 = sf->box_j0; F.k0 = sf->box_k0;
+    F.i1 = sf->box_in; F.j1 = sf->box_jn; F.k1 = sf->box_kn;
+    F.ix = F.i1 - F.i0; F.iy = F.j1 - F.j0;
+    F.pimin = sf->pimin; F.pimax = sf->pimax; F.pjmin = sf->pjmin; F.pjmax = sf->pjmax;
+    const long long nimg = (F.pimax > F.pimin && F.pjmax > F.pjmin)
+                           ? (long long)(F.pimax - F.pimin) * (F.pjmax - F.pjmin) : 0;
+    F.nwork = nimg * 2 * F.ix * F.iy;
+    if (F.nwork >= (1LL << 31) * 256) {
+        fprintf(stderr, "Error: PERIODIC_NFFF: too many images x plane cells\n");
+        return 1;
+    }
+    int sk[2][5] = { { sf->box_boundary_skipk0, sf->skipk0_imin, sf->skipk0_jmin, sf->skipk0_imax, sf->skipk0_jmax },
+                     { sf->box_boundary_skipkn, sf->skipkn_imin, sf->skipkn_jmin, sf->skipkn_imax, sf->skipkn_jmax } };
+    memcpy(F.skip, sk, sizeof(sk));
+    F.nrs = sf->nrs;
+    F.ndata = mp->pfarfield->ndata;
+    std::vector<int> ist(F.nrs);
+    for (int n = 0; n < F.nrs; n++)
+        ist[n] = mp->pfarfield->prpoints[n].istart;
+    int *p;
+    if (dupload(E, &p, sf->ri, F.nrs)) return 1;
+    F.ri = p;
+    if (dupload(E, &p, sf->rj, F.nrs)) return 1;
+    F.rj = p;
+    if (dupload(E, &p, sf->rk, F.nrs)) return 1;
+    F.rk = p;
+    if (dupload(E, &p, ist.data(), F.nrs)) return 1;
+    F.istart = p;
+    size_t cnt = (size_t)F.nrs * 3 * F.ndata;
+    if (dmalloc(E, &F.bins, cnt))
+        return 1;
+    CK(hipMemsetAsync(F.bins, 0, cnt * 8, E->st));
+    F.dx = set->sp.dx; F.dy = set->sp.dy;
+    F.dt = set->plan.dt;
+    F.c = SV_LIGHT_SPEED;
+    F.post = sf->postprocess != 0;
+    F.ppstart = sf->ppstart;
+    return 0;
+}
+
 static int
 setup_farfield(Engine *E)
 {
@@ -1050,7 +1103,7 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
             CK(hipMemsetAsync(g.ade_p, 0, n * 4, E->st));
         }
     }
-    if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E)
+    if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E)
         || setup_flux(E) || setup_sums(E))
         return 1;
     for (int q = 0; q < 4; q++)
@@ -1168,6 +1221,14 @@ fetch_aux(Engine *E, SvPool *mp, SvSet *set)
         CK(hipStreamSynchronize(E->st));
         for (int n = 0; n < E->ff.nrs; n++)
             memcpy(mp->farfield->rpoints[n].ex, b.data() + (size_t)n * 3 * nd, 3 * (size_t)nd * 8);
+    }
+    if (E->has_pff) {
+        int nd = E->pff.ndata;
+        std::vector<double> b((size_t)E->pff.nrs * 3 * nd);
+        CK(hipMemcpyAsync(b.data(), E->pff.bins, b.size() * 8, hipMemcpyDeviceToHost, E->st));
+        CK(hipStreamSynchronize(E->st));
+        for (int n = 0; n < E->pff.nrs; n++)
+            memcpy(mp->pfarfield->prpoints[n].ex, b.data() + (size_t)n * 3 * nd, 3 * (size_t)nd * 8);
     }
     if (E->has_dft || E->has_flux) {
         if (dft_flush(E))
@@ -1375,8 +1436,8 @@ extern "C" int
 sv_pool_gpu_apply_pfarfield(SvPool *mp, SvSet *set)
 {
     ENGINE_OR_FAIL(mp);
-    (void)E;
-    (void)set;
+    if (E->has_pff)
+        CK(gsv_launch_pfarfield(&E->g, &E->pff, set->sc.step_act, E->st));
     return 0;
 }
 

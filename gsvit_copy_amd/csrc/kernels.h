@@ -71,6 +71,23 @@ typedef struct {
     double dx, dy, dz, dt, c;
 } GsvFarfieldTD;
 
+/* periodic far field: the z0 / zn planes of the box, (pimax - pimin) x
+ * (pjmax - pjmin) images shifted by the box width; post != 0 bins only steps
+ * >= ppstart with the postprocess mode's operation order */
+typedef struct {
+    int i0, j0, k0, i1, j1, k1;
+    int ix, iy;
+    int pimin, pimax, pjmin, pjmax;
+    int skip[2][5];           /* z0, zn: on, imin, jmin, imax, jmax */
+    long long nwork;          /* images x 2 planes x ix x iy */
+    int nrs;
+    const int *ri, *rj, *rk, *istart;
+    int ndata;
+    double *bins;             /* [nrs][3][ndata] */
+    double dx, dy, dt, c;
+    int post, ppstart;
+} GsvPFarfield;
+
 /* running DFT: each step the four tangential components of every face cell
  * (face 0..5 = x0, xn, y0, yn, z0, zn of the NFFF box; face 6 = a FLUX_DFT
  * plane cell: Ex, Ey and the means of Hx, Hy over the cell and the cell
@@ -147,6 +164,7 @@ hipError_t gsv_launch_liao_faces(const GsvGrid *g, const GsvLiao *L, hipStream_t
 hipError_t gsv_launch_zero_planes(const GsvGrid *g, int axis, hipStream_t st);
 hipError_t gsv_launch_periodic(const GsvGrid *g, const GsvPeriodic *p, int hside, hipStream_t st);
 hipError_t gsv_launch_farfield_td(const GsvGrid *g, const GsvFarfieldTD *f, int step, hipStream_t st);
+hipError_t gsv_launch_pfarfield(const GsvGrid *g, const GsvPFarfield *f, int step, hipStream_t st);
 hipError_t gsv_launch_dft_sample(const GsvGrid *g, const GsvDft *d, int slot, hipStream_t st);
 hipError_t gsv_launch_dft_flush(const GsvDft *d, int first_step, int nsamples, hipStream_t st);
 hipError_t gsv_launch_box_sum(const GsvGrid *g, const GsvBoxSum *b, int step, hipStream_t st);

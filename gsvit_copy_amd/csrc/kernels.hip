@@ -19,6 +19,7 @@
  *   k_periodic_h / _e     interior periodic plane copies, src3d/boundary.c:304-346, 889-932
  *   k_record_points       point traces, src3d/output.c:340-352
  *   k_farfield_td         time-domain Kirchhoff bins, src3d/farfield.c:174-407
+ *   k_pfarfield           periodic Kirchhoff bins (z planes, images), src3d/pfarfield.c:120-401
  *   k_dft_sample / k_dft_flush   running DFT of tangential fields on the NFFF box
  *   k_box_sum             OUT_SUM box sums of E^2, src3d/output.c:259-306
  * Batched variants (k_*_batch) run the same column and face-cell functions
@@ -1316,6 +1317,69 @@ k_farfield_td(GsvGrid g, GsvFarfieldTD f, int step)
     }
 }
 
+/* periodic far field, src3d/pfarfield.c:120-401: one thread per (image,
+ * plane, i, j) and point (blockIdx.y), i fastest so that a wave reads 64
+ * consecutive cells.  The distance is formed in float with explicit
+ * round-to-nearest operations (no contraction), as the reference's
+ * ((float)ri - (i + pri)) squares are; everything after it in double. */
+__global__ __launch_bounds__(256) void
+k_pfarfield(GsvGrid g, GsvPFarfield f, int step)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = blockIdx.y;
+    if (q >= f.nwork || (f.post && step < f.ppstart))
+        return;
+    const long long pl = (long long)f.ix * f.iy;
+    const int i = f.i0 + (int)(q % f.ix);
+    const int j = f.j0 + (int)((q / f.ix) % f.iy);
+    const int s = (int)((q / pl) & 1);
+    const long long img = q / (2 * pl);
+    const int npj = f.pjmax - f.pjmin;
+    const int pi = f.pimin + (int)(img / npj), pj = f.pjmin + (int)(img % npj);
+    const int *sk = f.skip[s];
+    if (sk[0] && i >= sk[1] && j >= sk[2] && i <= sk[3] && j <= sk[4])
+        return;
+    const int k = s ? f.k1 : f.k0;
+    const int rk = f.rk[n];
+    const float ddx = __fsub_rn((float)f.ri[n], (float)(i + f.ix * pi));
+    const float ddy = __fsub_rn((float)f.rj[n], (float)(j + f.iy * pj));
+    const float ddz = __fsub_rn((float)rk, (float)k);
+    const float rs = __fadd_rn(__fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy)), __fmul_rn(ddz, ddz));
+    double r = sqrt((double)rs);
+    const double ct = (s ? (double)(-(f.k1 - rk)) : (double)(f.k0 - rk)) / r;
+    r *= f.dx;
+    const double mult = 1.0 / (4.0 * M_PI), size = f.dx * f.dy;
+    int nn;
+    double b, cvala, aval;
+    if (f.post) {
+        const double nred = f.ppstart + 1 + r / (f.c * f.dt);
+        nn = (int)((step - f.ppstart) + nred) - f.istart[n];
+        b = -mult * ct / (r * r) * size;
+        aval = mult / r / 2.0 / f.dx * size;
+    } else {
+        nn = (int)(step + 1 + r / (f.c * f.dt)) - f.istart[n];
+        b = -mult * ct / (r * r);
+        aval = mult / r / 2.0 / f.dx;
+    }
+    if (nn < 3 || nn >= f.ndata)
+        return;
+    cvala = -mult * ct / (f.c * r) / 2.0 / f.dt * size;
+    if (!s)
+        aval = -aval;
+    const long long c = i + (long long)g.px * j + g.plane * k;
+    const float *FF[3] = { g.ex, g.ey, g.ez };
+    double *bins = f.bins + (long long)n * 3 * f.ndata;
+    for (int qq = 0; qq < 3; qq++) {
+        const double e = FF[qq][c];
+        const double de = (double)FF[qq][c + g.plane] - (double)FF[qq][c - g.plane];
+        const double cvalb = cvala * e;
+        double *bq = bins + (long long)qq * f.ndata;
+        atomicAdd(bq + nn, -cvalb);
+        atomicAdd(bq + nn - 1, f.post ? aval * de + b * e : (aval * de + b * e) * size);
+        atomicAdd(bq + nn - 2, cvalb);
+    }
+}
+
 /* running DFT, sampling half: the four tangential components of every NFFF
  * box face cell into ring slot `slot` */
 __global__ __launch_bounds__(256) void
@@ -1593,6 +1657,16 @@ gsv_launch_periodic(const GsvGrid *g, const GsvPeriodic *p, int hside, hipStream
         hipLaunchKernelGGL(k_periodic_h, grd, blk, 0, st, *g, *p);
     else
         hipLaunchKernelGGL(k_periodic_e, grd, blk, 0, st, *g, *p);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_pfarfield(const GsvGrid *g, const GsvPFarfield *f, int step, hipStream_t st)
+{
+    if (f->nwork <= 0 || f->nrs <= 0)
+        return hipSuccess;
+    dim3 grd((unsigned)((f->nwork + 255) / 256), (unsigned)f->nrs);
+    hipLaunchKernelGGL(k_pfarfield, grd, dim3(256), 0, st, *g, *f, step);
     return hipGetLastError();
 }
 

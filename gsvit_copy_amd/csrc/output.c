@@ -362,6 +362,21 @@ sv_pool_output(SvPool *mp, SvSet *set)
             fprintf(fh, "%d %g %g %g\n", i, -r->ex[i], -r->ey[i], -r->ez[i]);
         fclose(fh);
     }
+    /* periodic far field, src3d/output.c:1335-1354 (the reference rewrites these
+     * files on every output step; the content of the last one is the result) */
+    for (n = 0; n < set->spf.nrs; n++) {
+        SvRPoint *r = mp->pfarfield->prpoints + n;
+        if (!set->spf.source_filename[n][0])
+            continue;
+        fh = fopen(set->spf.source_filename[n], "w");
+        if (!fh) {
+            fprintf(stderr, "Error: periodic farfield: cannot open file %s for writing\n", set->spf.source_filename[n]);
+            continue;
+        }
+        for (i = 0; i < mp->pfarfield->ndata; i++)
+            fprintf(fh, "%d %g %g %g\n", i, -r->ex[i], -r->ey[i], -r->ez[i]);
+        fclose(fh);
+    }
     if (set->sf.ndft && set->sf.dft_filename[0]) {
         SvFarfield *ff = mp->farfield;
         long c;

@@ -1474,6 +1474,32 @@ farfield_new(SvSet *set)
     return ff;
 }
 
+/* periodic far field, src3d/pfarfield.c:36-62 (the postprocess plane store
+ * is not allocated here: the engine integrates every step from ppstart on) */
+static SvPFarfield *
+pfarfield_new(SvSet *set)
+{
+    SvPFarfield *ff = (SvPFarfield *)calloc(1, sizeof(SvPFarfield));
+    double div = SV_LIGHT_SPEED * set->plan.dt / set->sp.dx;
+    int n;
+
+    ff->ndata = 1000 + 2 * set->sc.nsteps
+                + (int)(sqrt((double)(set->sp.xres * set->sp.xres + set->sp.yres * set->sp.yres + set->sp.zres * set->sp.zres))
+                        / div);
+    ff->nrs = set->spf.nrs;
+    if (ff->nrs)
+        ff->prpoints = (SvRPoint *)calloc(ff->nrs, sizeof(SvRPoint));
+    for (n = 0; n < ff->nrs; n++) {
+        double ri = set->spf.ri[n] / div, rj = set->spf.rj[n] / div, rk = set->spf.rk[n] / div;
+        double v = sqrt(ri * ri + rj * rj + rk * rk) - 500;
+        ff->prpoints[n].istart = (int)(v > 0 ? v : 0);
+        ff->prpoints[n].ex = (double *)calloc(3 * (size_t)ff->ndata, sizeof(double));
+        ff->prpoints[n].ey = ff->prpoints[n].ex + ff->ndata;
+        ff->prpoints[n].ez = ff->prpoints[n].ey + ff->ndata;
+    }
+    return ff;
+}
+
 /* ------------------------------------------------------------------ */
 /* planner entry point                                                  */
 /* ------------------------------------------------------------------ */
@@ -1570,6 +1596,7 @@ init_and_make_plan(SvSet *set)
 
     mp->bnd = boundary_new(set);
     mp->farfield = farfield_new(set);
+    mp->pfarfield = pfarfield_new(set);
 
     if (set->sm.in_voxel_filename[0] || set->sm.in_vector_filename[0]) {
         if (load_data(mp, set))
@@ -1625,6 +1652,20 @@ init_and_make_plan(SvSet *set)
         if (f->box_i0 < 1 || f->box_j0 < 1 || f->box_k0 < 1 || f->box_in >= xres - 1 || f->box_jn >= yres - 1
             || f->box_kn >= zres - 1 || f->box_in <= f->box_i0 || f->box_jn <= f->box_j0 || f->box_kn <= f->box_k0) {
             fprintf(stderr, "Error: NFFF box must lie inside the pool with one cell to spare on each side\n");
+            return NULL;
+        }
+    }
+
+    if (set->spf.nrs) {
+        /* the plane integrals read E at k0 +- 1 and kn +- 1 */
+        SvSetPFarfield *f = &set->spf;
+        if (f->box_i0 < 0 || f->box_j0 < 0 || f->box_k0 < 1 || f->box_in > xres || f->box_jn > yres
+            || f->box_kn > zres - 2 || f->box_in <= f->box_i0 || f->box_jn <= f->box_j0 || f->box_kn <= f->box_k0) {
+            fprintf(stderr, "Error: PERIODIC_NFFF box must lie inside the pool, its z planes one cell inside\n");
+            return NULL;
+        }
+        if (f->postprocess && (f->ppstart < 0 || f->ppstart >= set->sc.nsteps)) {
+            fprintf(stderr, "Error: PERIODIC_NFFF_POSTPROCESS start step must lie in 0 .. nsteps-1\n");
             return NULL;
         }
     }
@@ -1737,6 +1778,12 @@ sv_pool_free(SvPool *mp, SvSet *set)
         free(mp->farfield->flux_omega);
         free(mp->farfield->flux_data);
         free(mp->farfield);
+    }
+    if (mp->pfarfield) {
+        for (n = 0; n < mp->pfarfield->nrs; n++)
+            free(mp->pfarfield->prpoints[n].ex);
+        free(mp->pfarfield->prpoints);
+        free(mp->pfarfield);
     }
     free(mp->mats);
     free(mp);

@@ -164,6 +164,26 @@ gsv_get_farfield(void *h, double *out)
     return 0;
 }
 
+void
+gsv_pfarfield_dims(void *h, int *dims)
+{
+    GsvSession *s = (GsvSession *)h;
+    dims[0] = s->mp->pfarfield->nrs;
+    dims[1] = s->mp->pfarfield->ndata;
+}
+
+int
+gsv_get_pfarfield(void *h, double *out)
+{
+    GsvSession *s = (GsvSession *)h;
+    int n, nd = s->mp->pfarfield->ndata;
+    if (gsv_engine_fetch_aux(s->mp->gpu))
+        return 1;
+    for (n = 0; n < s->mp->pfarfield->nrs; n++)
+        memcpy(out + (size_t)n * 3 * nd, s->mp->pfarfield->prpoints[n].ex, 3 * (size_t)nd * sizeof(double));
+    return 0;
+}
+
 long
 gsv_get_dft(void *h, double *out)
 {

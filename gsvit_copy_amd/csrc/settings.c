@@ -6,9 +6,10 @@
  * get_int/get_double, src3d/settings.c:103-135).  Keywords outside the hot path
  * (images, volumes, sub-gridding, local/focused/layered sources, forces) are
  * rejected with an error instead of being silently ignored, so a parameter
- * file never runs with part of its physics dropped.  Keywords that only
- * request file output unrelated to the step loop (OUT_IMAGE, OUT_FILE,
- * OUT_PLANE, OUT_VOLUME) are consumed and ignored with a warning.
+ * file never runs with part of its physics dropped.  Keywords whose only
+ * product is a Gwyddion container (OUT_IMAGE, OUT_FILE, the far-field
+ * spherical areas NFFF_SPHERICAL_AREA and PERIODIC_NFFF_SPHERICAL_AREA) are
+ * consumed and ignored with a warning.
  *
  * Generated sources (types 1 and 2, src3d/settings.c:444-548) are produced here
  * with the reference's waveform formulas and then rounded to 6 significant
@@ -39,6 +40,10 @@ clear_settings(SvSet *set, int remove)
         free(set->sf.individual);
         free(set->sf.source_filename);
         free(set->sf.dft_lambda);
+        free(set->spf.ri);
+        free(set->spf.rj);
+        free(set->spf.rk);
+        free(set->spf.source_filename);
         free(set->sx.k);
         free(set->sx.lambda);
     }
@@ -65,6 +70,11 @@ clear_settings(SvSet *set, int remove)
     set->smb.bxnpos = set->smb.bynpos = set->smb.bznpos = -1;
     set->sf.sumfrom = 0;
     set->sf.sumto = 0;
+    /* src3d/settings.c:945-957, src3d/defaults.h:167-179 */
+    set->spf.box_i0 = set->spf.box_j0 = set->spf.box_k0 = 10;
+    set->spf.box_in = set->spf.box_jn = set->spf.box_kn = 90;
+    set->spf.pimin = set->spf.pjmin = 0;
+    set->spf.pimax = set->spf.pjmax = 1;
 }
 
 static int
@@ -497,7 +507,8 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             if (o->component >= SV_OVOLUME_ABS)
                 o->k = nabs;
             set->so.ncubs++;
-        } else if (!strcmp(key, "OUT_IMAGE") || !strcmp(key, "OUT_FILE")) {
+        } else if (!strcmp(key, "OUT_IMAGE") || !strcmp(key, "OUT_FILE") || !strcmp(key, "NFFF_SPHERICAL_AREA")
+                   || !strcmp(key, "PERIODIC_NFFF_SPHERICAL_AREA")) {
             skip_output_keyword(fr, key);
         } else if (!strcmp(key, "BOUNDARY_ALL")) {
             SvSetBoundary *b = &set->sb;
@@ -586,6 +597,40 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             if (get_rest_of_line(fr, f->source_filename[n - 1], SV_NAME_LEN))
                 goto fail;
             f->individual[n - 1] = 0;
+            f->nrs = n;
+        } else if (!strcmp(key, "PERIODIC_NFFF")) {
+            /* src3d/settings.c:3424-3446: box vertices, then pimin pjmin pimax pjmax */
+            SvSetPFarfield *f = &set->spf;
+            if (get_int(fr, &f->box_i0, key) || get_int(fr, &f->box_j0, key) || get_int(fr, &f->box_k0, key)
+                || get_int(fr, &f->box_in, key) || get_int(fr, &f->box_jn, key) || get_int(fr, &f->box_kn, key)
+                || get_int(fr, &f->pimin, key) || get_int(fr, &f->pjmin, key) || get_int(fr, &f->pimax, key)
+                || get_int(fr, &f->pjmax, key))
+                goto fail;
+        } else if (!strcmp(key, "PERIODIC_NFFF_POSTPROCESS")) {
+            if (get_int(fr, &set->spf.postprocess, key) || get_int(fr, &set->spf.ppstart, key))
+                goto fail;
+        } else if (!strcmp(key, "PERIODIC_NFFF_SKIP")) {
+            /* src3d/settings.c:3452-3474: k0 or kn, then imin jmin imax jmax */
+            SvSetPFarfield *f = &set->spf;
+            int *a = NULL;
+            if (get_token(fr, value, sizeof(value)))
+                goto fail;
+            if (!strcmp(value, "k0")) { f->box_boundary_skipk0 = 1; a = &f->skipk0_imin; }
+            else if (!strcmp(value, "kn")) { f->box_boundary_skipkn = 1; a = &f->skipkn_imin; }
+            if (a && (get_int(fr, a, key) || get_int(fr, a + 1, key) || get_int(fr, a + 2, key) || get_int(fr, a + 3, key)))
+                goto fail;
+        } else if (!strcmp(key, "PERIODIC_NFFF_RAMAHI_POINT")) {
+            /* src3d/settings.c:3475-3499 */
+            SvSetPFarfield *f = &set->spf;
+            int n = f->nrs + 1;
+            f->ri = (int *)realloc(f->ri, n * sizeof(int));
+            f->rj = (int *)realloc(f->rj, n * sizeof(int));
+            f->rk = (int *)realloc(f->rk, n * sizeof(int));
+            f->source_filename = (char (*)[SV_NAME_LEN])realloc(f->source_filename, n * SV_NAME_LEN);
+            if (get_int(fr, &f->ri[n - 1], key) || get_int(fr, &f->rj[n - 1], key) || get_int(fr, &f->rk[n - 1], key))
+                goto fail;
+            if (get_rest_of_line(fr, f->source_filename[n - 1], SV_NAME_LEN))
+                goto fail;
             f->nrs = n;
         } else if (!strcmp(key, "NFFF_DFT")) {
             /* running-DFT accumulation on the NFFF box: NFFF_DFT n lambda_1 .. lambda_n file */

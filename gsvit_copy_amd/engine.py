@@ -37,6 +37,10 @@ def library():
         L.gsv_get_points.argtypes = [vp, dp]
         L.gsv_get_farfield.restype = ctypes.c_int
         L.gsv_get_farfield.argtypes = [vp, dp]
+        L.gsv_pfarfield_dims.restype = None
+        L.gsv_pfarfield_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.gsv_get_pfarfield.restype = ctypes.c_int
+        L.gsv_get_pfarfield.argtypes = [vp, dp]
         L.gsv_get_dft.restype = ctypes.c_long
         L.gsv_get_dft.argtypes = [vp, dp]
         L.gsv_get_maps.argtypes = [vp, ip, ip]
@@ -156,6 +160,15 @@ class Session:
         if inf["nrs"] and self.L.gsv_get_farfield(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
             raise RuntimeError("far-field download failed")
         return out[: inf["nrs"]]
+
+    def pfarfield(self):
+        """periodic far-field bins [nrs][3][ndata] (Ex, Ey, Ez, not negated)"""
+        d = (ctypes.c_int * 2)()
+        self.L.gsv_pfarfield_dims(self.h, d)
+        out = np.zeros((max(d[0], 1), 3, d[1]), dtype=np.float64)
+        if d[0] and self.L.gsv_get_pfarfield(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+            raise RuntimeError("periodic far-field download failed")
+        return out[: d[0]]
 
     def dft(self):
         n = self.L.gsv_get_dft(self.h, None)

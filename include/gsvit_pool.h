@@ -121,6 +121,15 @@ typedef struct {
     long flux_ncells;
 } SvFarfield;
 
+/* periodic far field, src3d/pfarfield.h and src3d/pfarfield.c:36-62: per
+ * point E bins, ndata = 1000 + 2 nsteps + diagonal / (c dt / dx), istart as
+ * the plain far field */
+typedef struct {
+    int ndata;
+    int nrs;
+    SvRPoint *prpoints;
+} SvPFarfield;
+
 typedef struct _SvPool {
     SvYeeData *d;
     SvMatProp *mats;
@@ -129,6 +138,7 @@ typedef struct _SvPool {
     SvOutput *out;
     SvBoundary *bnd;
     SvFarfield *farfield;
+    SvPFarfield *pfarfield;
     void *gpu;                  /* device engine, created by sv_gpu_new */
     SvSet *set;
 } SvPool;

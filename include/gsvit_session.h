@@ -32,6 +32,11 @@ int gsv_get_field(void *session, int comp, float *out);
 int gsv_get_points(void *session, double *out);
 /* time-domain far-field bins as [nrs][3][ndata] */
 int gsv_get_farfield(void *session, double *out);
+/* periodic far field (PERIODIC_NFFF, src3d/pfarfield.c): dims[0..1] = number of
+ * points, bins per component; bins as [nrs][3][ndata] (Ex, Ey, Ez; the log file
+ * holds them negated, src3d/output.c:1347-1352) */
+void gsv_pfarfield_dims(void *session, int *dims);
+int gsv_get_pfarfield(void *session, double *out);
 /* running-DFT accumulators; returns the element count (out may be NULL) */
 long gsv_get_dft(void *session, double *out);
 /* OUT_SUM values as [nsums][nsteps] (before the cell-volume scaling of the

@@ -310,6 +310,23 @@ typedef struct {
     char filename[SV_NAME_LEN];
 } SvSetFlux;
 
+/* periodic near-to-far-field transform, src3d/settings.h (SvSetPFarfield) and
+ * src3d/settings.c:3424-3499: the Kirchhoff integral of E over the z0 and zn
+ * planes of the box, repeated over the periodic images pi in [pimin, pimax),
+ * pj in [pjmin, pjmax) shifted by the box width; postprocess != 0 stores the
+ * plane fields from step ppstart on and integrates them at the last step */
+typedef struct {
+    int box_i0, box_j0, box_k0, box_in, box_jn, box_kn;
+    int pimin, pjmin, pimax, pjmax;
+    int postprocess, ppstart;
+    int box_boundary_skipk0, box_boundary_skipkn;
+    int skipk0_imin, skipk0_jmin, skipk0_imax, skipk0_jmax;
+    int skipkn_imin, skipkn_jmin, skipkn_imax, skipkn_jmax;
+    int nrs;
+    int *ri, *rj, *rk;
+    char (*source_filename)[SV_NAME_LEN];
+} SvSetPFarfield;
+
 typedef struct {
     SvSetPool sp;
     SvSetComp sc;
@@ -318,6 +335,7 @@ typedef struct {
     SvSetSource ss;
     SvSetOutput so;
     SvSetFarfield sf;
+    SvSetPFarfield spf;
     SvSetMat sm;
     SvSetPlan plan;
     SvSetAdjoint sa;

@@ -27,6 +27,7 @@
  *   1-D auxiliary line ...... src3d/1dpool.c:83-162
  *   cubic line interpolation  src3d/dline.c:89-112
  *   time-domain far field ... src3d/farfield.c:174-407
+ *   periodic far field ...... src3d/pfarfield.c:120-401
  *   running DFT on the NFFF box (no reference counterpart; pinned by a numpy
  *                             DFT of recorded point traces in the tests)
  *   point traces ............ src3d/output.c:340-352
@@ -96,6 +97,7 @@ typedef struct {
     double *points;                   /* [npts][nsteps+1][6] */
     int no_average;                   /* test option: E coefficients without interface averaging */
     double *objacc;                   /* OBJECTIVE_DFT box: [cell][3][re, im] */
+    double *pfstore;                  /* periodic far field, postprocess mode: [plane][6][ix][iy][nt] */
 } Oracle;
 
 #define IDX(o, i, j, k) (((size_t)(i) * (o)->ny + (j)) * (o)->nz + (k))
@@ -224,7 +226,7 @@ oracle_close(void *handle)
             free(o->px[q]); free(o->py[q]); free(o->pz[q]); free(o->pxp[q]); free(o->pyp[q]); free(o->pzp[q]);
         }
     }
-    free(o->je); free(o->jh); free(o->points); free(o->objacc);
+    free(o->je); free(o->jh); free(o->points); free(o->objacc); free(o->pfstore);
     sv_pool_free(o->mp, &o->set);
     clear_settings(&o->set, 1);
     free(o);
@@ -1220,6 +1222,146 @@ farfield(Oracle *o)
     }
 }
 
+/* periodic far field, src3d/pfarfield.c:120-401: Kirchhoff integral of E
+ * over the z0 / zn planes of the box and their periodic images, in the
+ * reference's loop order (point, i, j, pi, pj, z0 then zn, step).  The
+ * distance is formed in float as the reference does ((float)ri - (i + pri),
+ * squares summed in float), then widened for sqrt.  Direct mode bins every
+ * step; postprocess mode stores E and its k+1 - k-1 difference at both
+ * planes from step ppstart on and integrates them at the last step with the
+ * size factor folded into b and aval (src3d/pfarfield.c:152-251). */
+static void
+pfarfield(Oracle *o)
+{
+    const SvSetPFarfield *sf = &o->set.spf;
+    const SvSet *set = &o->set;
+    SvPFarfield *pf = o->mp->pfarfield;
+    const int i0 = sf->box_i0, i1 = sf->box_in, j0 = sf->box_j0, j1 = sf->box_jn, k0 = sf->box_k0, k1 = sf->box_kn;
+    const int ix = i1 - i0, iy = j1 - j0, step = set->sc.step_act;
+    const double dx = set->sp.dx, dy = set->sp.dy, dt = set->plan.dt, mult = 1.0 / (4.0 * M_PI), size = dx * dy;
+    real *F[3] = { o->ex, o->ey, o->ez };
+    int n;
+
+    if (sf->nrs == 0)
+        return;
+    if (sf->postprocess) {
+        const int nt = set->sc.nsteps - sf->ppstart;
+        const size_t pl = (size_t)ix * iy * nt;
+        if (step >= sf->ppstart) {
+            int i, j, q, s;
+            const int tit = step - sf->ppstart;
+            if (!o->pfstore)
+                o->pfstore = (double *)zalloc(2 * 6 * pl, sizeof(double));
+            for (s = 0; s < 2; s++) {
+                const int k = s ? k1 : k0;
+                for (i = i0; i < i1; i++)
+                    for (j = j0; j < j1; j++)
+                        for (q = 0; q < 3; q++) {
+                            const size_t at = ((size_t)(i - i0) * iy + (j - j0)) * nt + tit;
+                            o->pfstore[(size_t)(s * 6 + 2 * q) * pl + at] = F[q][IDX(o, i, j, k)];
+                            o->pfstore[(size_t)(s * 6 + 2 * q + 1) * pl + at] = F[q][IDX(o, i, j, k + 1)] - F[q][IDX(o, i, j, k - 1)];
+                        }
+            }
+        }
+        if (step != set->sc.nsteps - 1)
+            return;
+        #pragma omp parallel for schedule(dynamic, 1)
+        for (n = 0; n < sf->nrs; n++) {
+            SvRPoint *rp = pf->prpoints + n;
+            double *B[3] = { rp->ex, rp->ey, rp->ez };
+            int i, j, pi, pj, s, tit, q;
+            for (i = i0; i < i1; i++)
+                for (j = j0; j < j1; j++)
+                    for (pi = sf->pimin; pi < sf->pimax; pi++)
+                        for (pj = sf->pjmin; pj < sf->pjmax; pj++)
+                            for (s = 0; s < 2; s++) {
+                                const int k = s ? k1 : k0;
+                                const int sk = s ? sf->box_boundary_skipkn : sf->box_boundary_skipk0;
+                                const int *lim = s ? &sf->skipkn_imin : &sf->skipk0_imin;
+                                const int pri = ix * pi, prj = iy * pj;
+                                float ddx, ddy, ddz, rs;
+                                double r, ct, a, b, c, cvala, aval, nred;
+                                const size_t at = ((size_t)(i - i0) * iy + (j - j0)) * nt;
+                                if (sk && i >= lim[0] && j >= lim[1] && i <= lim[2] && j <= lim[3])
+                                    continue;
+                                ddx = (float)sf->ri[n] - (float)(i + pri);
+                                ddy = (float)sf->rj[n] - (float)(j + prj);
+                                ddz = (float)sf->rk[n] - (float)k;
+                                rs = ddx * ddx + ddy * ddy;
+                                rs = rs + ddz * ddz;
+                                r = sqrt((double)rs);
+                                ct = (s ? (double)(-(k1 - sf->rk[n])) : (double)(k0 - sf->rk[n])) / r;
+                                r *= dx;
+                                a = mult / r;
+                                b = -mult * ct / (r * r) * size;
+                                c = -mult * ct / (CL * r);
+                                cvala = c / 2.0 / dt * size;
+                                aval = a / 2.0 / dx * size;
+                                nred = sf->ppstart + 1 + r / (CL * dt);
+                                for (tit = 0; tit < nt; tit++) {
+                                    const int nn = (int)(tit + nred) - rp->istart;
+                                    if (nn < 3 || nn >= pf->ndata)
+                                        continue;
+                                    for (q = 0; q < 3; q++) {
+                                        const double e = o->pfstore[(size_t)(s * 6 + 2 * q) * pl + at + tit];
+                                        const double de = o->pfstore[(size_t)(s * 6 + 2 * q + 1) * pl + at + tit];
+                                        const double cvalb = cvala * e;
+                                        B[q][nn] += -cvalb;
+                                        B[q][nn - 1] += ((s ? aval : -aval) * de + b * e);
+                                        B[q][nn - 2] += cvalb;
+                                    }
+                                }
+                            }
+        }
+        return;
+    }
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (n = 0; n < sf->nrs; n++) {
+        SvRPoint *rp = pf->prpoints + n;
+        double *B[3] = { rp->ex, rp->ey, rp->ez };
+        int i, j, pi, pj, s, q;
+        for (i = i0; i < i1; i++)
+            for (j = j0; j < j1; j++)
+                for (pi = sf->pimin; pi < sf->pimax; pi++)
+                    for (pj = sf->pjmin; pj < sf->pjmax; pj++)
+                        for (s = 0; s < 2; s++) {
+                            const int k = s ? k1 : k0;
+                            const int sk = s ? sf->box_boundary_skipkn : sf->box_boundary_skipk0;
+                            const int *lim = s ? &sf->skipkn_imin : &sf->skipk0_imin;
+                            const int pri = ix * pi, prj = iy * pj;
+                            float ddx, ddy, ddz, rs;
+                            double r, ct, a, b, c, cvala, aval;
+                            int nn;
+                            if (sk && i >= lim[0] && j >= lim[1] && i <= lim[2] && j <= lim[3])
+                                continue;
+                            ddx = (float)sf->ri[n] - (float)(i + pri);
+                            ddy = (float)sf->rj[n] - (float)(j + prj);
+                            ddz = (float)sf->rk[n] - (float)k;
+                            rs = ddx * ddx + ddy * ddy;
+                            rs = rs + ddz * ddz;
+                            r = sqrt((double)rs);
+                            ct = (s ? (double)(-(k1 - sf->rk[n])) : (double)(k0 - sf->rk[n])) / r;
+                            r *= dx;
+                            nn = (int)(step + 1 + r / (CL * dt)) - rp->istart;
+                            if (nn < 3 || nn >= pf->ndata)
+                                continue;
+                            a = mult / r;
+                            b = -mult * ct / (r * r);
+                            c = -mult * ct / (CL * r);
+                            cvala = c / 2.0 / dt * size;
+                            aval = a / 2.0 / dx;
+                            for (q = 0; q < 3; q++) {
+                                const double e = F[q][IDX(o, i, j, k)];
+                                const double de = F[q][IDX(o, i, j, k + 1)] - F[q][IDX(o, i, j, k - 1)];
+                                const double cvalb = cvala * e;
+                                B[q][nn] += -cvalb;
+                                B[q][nn - 1] += ((s ? aval : -aval) * de + b * e) * size;
+                                B[q][nn - 2] += cvalb;
+                            }
+                        }
+    }
+}
+
 /* box sums of E^2, src3d/output.c:259-306: voxels whose permittivity
  * matches layered_epsilon within EPSILON_0/100 (or every voxel for -1) */
 static void
@@ -1394,6 +1536,7 @@ oracle_step(void *handle, int nsteps)
         source_estep(o);
         getsum(o);
         farfield(o);
+        pfarfield(o);
         running_dft(o);
         flux_dft(o);
         objective_dft(o);
@@ -1447,6 +1590,24 @@ oracle_get_farfield(void *handle, double *out)
     int n, nd = o->mp->farfield->ndata;
     for (n = 0; n < o->set.sf.nrs; n++)
         memcpy(out + (size_t)n * 3 * nd, o->mp->farfield->rpoints[n].ex, 3 * (size_t)nd * sizeof(double));
+}
+
+/* periodic far field: dims[0..1] = nrs, ndata; bins as [nrs][3][ndata] */
+void
+oracle_pfarfield_dims(void *handle, int *dims)
+{
+    Oracle *o = (Oracle *)handle;
+    dims[0] = o->mp->pfarfield->nrs;
+    dims[1] = o->mp->pfarfield->ndata;
+}
+
+void
+oracle_get_pfarfield(void *handle, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    int n, nd = o->mp->pfarfield->ndata;
+    for (n = 0; n < o->mp->pfarfield->nrs; n++)
+        memcpy(out + (size_t)n * 3 * nd, o->mp->pfarfield->prpoints[n].ex, 3 * (size_t)nd * sizeof(double));
 }
 
 /* tabulated material map ([i][j][k], -1 everywhere without a table) and boundary map */

@@ -491,6 +491,30 @@ SMALL_KINDS = ["vacuum_none", "spheres_cpml", "magnetic_liao", "ade_cpml", "tsf_
                "periodic_liao", "farfield_cpml", "solids_cpml"]
 
 
+def pfarfield_case(folder, blocks, n=40, nsteps=200, seed=3):
+    """A point source (Ez, centre) among two dielectric spheres, CPML 8,
+    with the given parameter-file blocks (PERIODIC_NFFF*, NFFF*) appended."""
+    os.makedirs(folder, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    dx = 10e-9
+    dt = _dt(dx, dx, dx, 0.5)
+    c = n // 2
+    pulse = gaussian_sine(nsteps, dt, 150e-9, 1.5, delay_widths=1.2)
+    write_point_source(os.path.join(folder, "src.txt"), pulse, comp=2)
+    mats = ["4 %.3f %.3f %.3f 3.5 1 %.3f 1 0 0" % (c + 5, c - 4, c + 2, rng.uniform(2, 4)),
+            "4 %.3f %.3f %.3f 2.5 1 %.3f 1 0 0" % (c - 5, c + 3, c - 3, rng.uniform(2, 4))]
+    with open(os.path.join(folder, "mats.txt"), "w") as f:
+        f.write("\n".join(mats) + "\n")
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (n, n, n, dx, dx, dx), "COMP\n%d\n\n" % nsteps, "DT_MULT\n0.5\n\n",
+             "BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\n", "SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c),
+             "MEDIUM_VECTOR\nmats.txt\n\n"]
+    lines += [b.rstrip("\n") + "\n\n" for b in blocks]
+    path = os.path.join(folder, "pff.par")
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path
+
+
 def write_linear_medium(path, eps_r, sigma=None, mu_r=None, sigast=None):
     """MEDIUM_LINEAR file: 3 int32 dims, then float32 eps_r, sigma, mu_r, sigma* ([i][j][k], k fastest)."""
     eps_r = np.asarray(eps_r, dtype=np.float32)

@@ -31,6 +31,8 @@ def lib(precision="f64"):
         L.oracle_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_points.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_farfield.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_pfarfield_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_get_pfarfield.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_maps.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.oracle_get_dft.restype = ctypes.c_long
         L.oracle_get_dft.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
@@ -110,6 +112,15 @@ class Oracle:
         if inf["nrs"]:
             self.L.oracle_get_farfield(self.h, _dptr(out))
         return out[: inf["nrs"]]
+
+    def pfarfield(self):
+        """periodic far-field bins [nrs][3][ndata] (Ex, Ey, Ez, not negated)"""
+        d = (ctypes.c_int * 2)()
+        self.L.oracle_pfarfield_dims(self.h, d)
+        out = np.zeros((max(d[0], 1), 3, d[1]), dtype=np.float64)
+        if d[0]:
+            self.L.oracle_get_pfarfield(self.h, _dptr(out))
+        return out[: d[0]]
 
     def sums(self, nsums):
         inf = self.info()
