@@ -70,6 +70,8 @@ struct Engine {
     /* far field */
     int has_ff;
     GsvFarfieldTD ff;
+    int has_ltsf;                 /* layered TSF: 1-D model history and face tables */
+    GsvLtsf ltsf;
     int has_pff;                  /* periodic far field */
     GsvPFarfield pff;
     int has_dft;

@@ -777,6 +777,140 @@ dft_ring_depth(int nw)
     return T > 32 ? 32 : (T < 1 ? 1 : T);
 }
 
+/* layered TSF, src3d/plan.c:1600-1685 and src3d/source.c:231-322: the face
+ * tables (lateral delay and coefficient of every face cell, E and H sides),
+ * then the 1-D model run once for its whole history (the reference runs it at
+ * the first H source step; nothing reads it earlier) */
+static double
+ltsf_delay(const SvSourceLTSF *ls, const SvSet *set, int i, int j)
+{
+    /* src3d/source.c:286-322: distance from the box corner the wave enters
+     * first along the in-plane direction, as a time in steps */
+    const double phi = ls->ia_phi;
+    int rx, ry;
+    if (phi >= 0 && phi <= M_PI / 2.0) { rx = i - ls->box_i0; ry = j - ls->box_j0; }
+    else if (phi > M_PI / 2.0 && phi <= M_PI) { rx = i - ls->box_in; ry = j - ls->box_j0; }
+    else if (phi > M_PI && phi <= 3.0 * M_PI / 2.0) { rx = i - ls->box_in; ry = j - ls->box_jn; }
+    else { rx = i - ls->box_i0; ry = j - ls->box_jn; }
+    double radius = sqrt((double)(rx * rx + ry * ry));
+    const double alpha = atan2((double)ry, (double)rx);
+    radius *= cos(alpha - phi);
+    return radius * set->sp.dx * sin(ls->ia_theta) / (SV_LIGHT_SPEED * set->plan.dt);
+}
+
+static double
+ltsf_profile(const SvSet *set, int x, int y, int h_far_x)
+{
+    /* src3d/source.c:231-251; the H face at in drops the Gaussian unless the
+     * fiber profile is on as well (its "if / if-else" there, :1734-1741) */
+    const SvSetSourceLTSF *t = &set->ss.ltsf;
+    double gm = 1.0, fm = 1.0;
+    if (t->gaussian)
+        gm = exp(-((double)(x - t->gaussian_fxpos)) * ((double)(x - t->gaussian_fxpos)) / t->gaussian_rx / t->gaussian_rx / 2.0)
+             * exp(-((double)(y - t->gaussian_fypos)) * ((double)(y - t->gaussian_fypos)) / t->gaussian_ry / t->gaussian_ry / 2.0);
+    if (t->fiber) {
+        const double dx = set->sp.dx, lambda = set->ss.lambda_center;
+        const double v = 2 * M_PI / lambda * t->fiber_radius * dx * (t->fiber_epsilon_core - t->fiber_epsilon_cladding);
+        const double w0 = (0.65 + 1.619 / v / sqrt(v) + 2.879 / pow(v, 6)) * t->fiber_radius * dx;
+        const double rv = (x - t->fiber_fxpos) * (x - t->fiber_fxpos) + (y - t->fiber_fypos) * (y - t->fiber_fypos);
+        fm = rv < t->fiber_cutoff * t->fiber_cutoff ? exp(-rv * dx * dx / w0 / w0) : 0.0;
+    }
+    if (h_far_x)
+        return t->fiber ? fm : 1.0;
+    return t->gaussian ? gm : (t->fiber ? fm : 1.0);
+}
+
+static int
+setup_ltsf(Engine *E)
+{
+    SvSet *set = E->set;
+    SvPool *mp = E->mp;
+    const SvSourceLTSF *ls = mp->src->ltsf;
+    GsvLtsf &L = E->ltsf;
+    memset(&L, 0, sizeof(L));
+    E->has_ltsf = ls != NULL;
+    if (!ls)
+        return 0;
+    const SvSetSourceLTSF *t = &set->ss.ltsf;
+    const int res = set->sp.zres;
+    if (8 * (size_t)(res + 2) * sizeof(double) > 160 * 1024) {
+        fprintf(stderr, "Error: the layered TSF model needs z resolution <= 2046 (one workgroup's LDS)\n");
+        return 1;
+    }
+    L.i0 = ls->box_i0; L.j0 = ls->box_j0; L.k0 = ls->box_k0;
+    L.i1 = ls->box_in; L.j1 = ls->box_jn; L.k1 = ls->box_kn;
+    const int sk[6] = { t->box_boundary_skipi0, t->box_boundary_skipin, t->box_boundary_skipj0,
+                        t->box_boundary_skipjn, t->box_boundary_skipk0, t->box_boundary_skipkn };
+    memcpy(L.skip, sk, sizeof(sk));
+    L.res = res;
+    L.nrows = ls->nrows;
+    L.sph = sin(ls->ia_phi);
+    L.cph = cos(ls->ia_phi);
+    const double ecor = set->plan.dt / set->sp.dx, hcor = set->plan.dt / set->sp.dx / SV_MU_0;
+    for (int hs = 0; hs < 2; hs++)
+        for (int axis = 0; axis < 3; axis++) {
+            const int na = axis == 0 ? L.j1 - L.j0 + 1 : L.i1 - L.i0 + 1;
+            const int nb = axis == 2 ? L.j1 - L.j0 + 1 : L.k1 - L.k0 + 1;
+            std::vector<double> q(2 * (size_t)na * nb), cf(q.size());
+            for (int f = 0; f < 2; f++)
+                for (int a = 0; a < na; a++)
+                    for (int b = 0; b < nb; b++) {
+                        int i, j, k;
+                        if (axis == 0) { i = f ? L.i1 : L.i0; j = L.j0 + a; k = L.k0 + b; }
+                        else if (axis == 1) { i = L.i0 + a; j = f ? L.j1 : L.j0; k = L.k0 + b; }
+                        else { i = L.i0 + a; j = L.j0 + b; k = f ? L.k1 : L.k0; }
+                        /* E faces sample outside the near faces, H faces at the face cells */
+                        const int si = (!hs && axis == 0 && !f) ? i - 1 : i, sj = (!hs && axis == 1 && !f) ? j - 1 : j;
+                        const size_t at = ((size_t)f * na + a) * nb + b;
+                        q[at] = ltsf_delay(ls, set, si, sj);
+                        if (hs) {
+                            cf[at] = hcor * ltsf_profile(set, si, sj, axis == 0 && f);
+                        } else {
+                            double eps, sig;
+                            sv_pool_epsilon_sigma(mp, i, j, k, &eps, &sig);
+                            cf[at] = ecor * ltsf_profile(set, si, sj, 0) / eps;
+                        }
+                    }
+            double *p;
+            if (dupload(E, &p, q.data(), q.size())) return 1;
+            L.q[hs][axis] = p;
+            if (dupload(E, &p, cf.data(), cf.size())) return 1;
+            L.coef[hs][axis] = p;
+        }
+    /* the 1-D model, src3d/zpool.c:38-338 */
+    GsvLtsfLine M;
+    memset(&M, 0, sizeof(M));
+    const double zdt = ls->zdt, zdx = ls->zdx, corr = ls->corr, st = sin(ls->ia_theta);
+    const double i0 = sqrt(ls->zeps[0]), i1 = sqrt(ls->zeps[res - 1]);
+    M.res = res;
+    M.nrows = ls->nrows;
+    M.ndata = ls->ndata;
+    M.multe = zdt / (SV_EPSILON_0 * zdx) / corr;
+    M.multh = zdt / (SV_MU_0 * zdx) / corr;
+    M.m0 = (zdt * SV_LIGHT_SPEED / i0 - zdx) / (zdt * SV_LIGHT_SPEED / i0 + zdx);
+    M.m1 = (zdt * SV_LIGHT_SPEED / i1 - zdx) / (zdt * SV_LIGHT_SPEED / i1 + zdx);
+    M.st = st;
+    M.inj_e_s = zdt / zdx / (SV_EPSILON_0 * ls->zeps[10]) * sin(ls->ia_psi);
+    M.inj_e_c = zdt / zdx / (SV_EPSILON_0 * ls->zeps[10]) * cos(ls->ia_psi);
+    M.inj_h_s = zdt / zdx / SV_MU_0 * sin(ls->ia_psi);
+    M.inj_h_c = zdt / zdx / SV_MU_0 * cos(ls->ia_psi);
+    M.cez = 2 * sqrt(SV_MU_0 / SV_EPSILON_0) * st;
+    M.chz = 2 * sqrt(SV_EPSILON_0 / SV_MU_0) * st;
+    double *p;
+    if (dupload(E, &p, ls->zeps, res)) return 1;
+    M.eps = p;
+    if (dupload(E, &p, ls->zmu, res)) return 1;
+    M.mu = p;
+    if (dupload(E, &p, ls->e, ls->ndata)) return 1;
+    M.wave = p;
+    if (dmalloc(E, &M.hist, 6 * (size_t)M.nrows * res))
+        return 1;
+    CK(gsv_launch_ltsf_line(&M, E->st));
+    CK(hipStreamSynchronize(E->st));
+    L.hist = M.hist;
+    return 0;
+}
+
 /* periodic far field, src3d/pfarfield.c:36-62; bins zeroed, point lists and
  * istart uploaded once */
 static int
@@ -1103,7 +1237,7 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
             CK(hipMemsetAsync(g.ade_p, 0, n * 4, E->st));
         }
     }
-    if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E)
+    if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E) || setup_ltsf(E)
         || setup_flux(E) || setup_sums(E))
         return 1;
     for (int q = 0; q < 4; q++)
@@ -1120,7 +1254,7 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
         for (size_t q = 0; q < E->src_vals_host.size(); q++)
             if (q % 6 >= 3 && E->src_vals_host[q] != 0.f)
                 hsrc = 1;
-        E->fused_ok = !E->has_liao && !E->has_per && !hsrc && g.pc_nx == 0 && g.nx <= 1024
+        E->fused_ok = !E->has_liao && !E->has_per && !hsrc && !E->has_ltsf && g.pc_nx == 0 && g.nx <= 1024
                       && (unsigned long long)g.plane * (g.nz + 2) * 4ull < (1ull << 32)
                       && E->idbytes_e == E->idbytes_h;
         E->step_rows = 4;
@@ -1340,6 +1474,8 @@ sv_pool_gpu_apply_source_hstep(SvPool *mp, SvSet *set)
         CK(gsv_launch_tsf_line(&E->tsf, set->sc.step_act, E->st));
         CK(gsv_launch_tsf_faces(&E->g, &E->tsf, 1, E->st));
     }
+    if (E->has_ltsf)
+        CK(gsv_launch_ltsf_faces(&E->g, &E->ltsf, 1, set->sc.step_act, E->st));
     return 0;
 }
 
@@ -1394,6 +1530,8 @@ sv_pool_gpu_apply_source_estep(SvPool *mp, SvSet *set)
         CK(gsv_launch_point_src(&E->g, &E->src, set->sc.step_act, 0, E->st));
     if (E->has_tsf)
         CK(gsv_launch_tsf_faces(&E->g, &E->tsf, 0, E->st));
+    if (E->has_ltsf)
+        CK(gsv_launch_ltsf_faces(&E->g, &E->ltsf, 0, set->sc.step_act, E->st));
     return 0;
 }
 

@@ -71,6 +71,35 @@ typedef struct {
     double dx, dy, dz, dt, c;
 } GsvFarfieldTD;
 
+/* layered TSF, the split-field 1-D model along z (one workgroup, fields in
+ * LDS): res cells, nrows substeps of dt/3, every substep's ee eh ez he hh hz
+ * stored into hist [6][nrows][res] */
+typedef struct {
+    int res, nrows, ndata;
+    double multe, multh;      /* zdt / (eps0 zdx) / corr, zdt / (mu0 zdx) / corr */
+    double m0, m1;            /* Mur factors at the two ends */
+    double st;                /* sin(theta) */
+    double inj_e_s, inj_e_c;  /* zdt / zdx / (eps0 eps[10]) sin(psi), cos(psi) */
+    double inj_h_s, inj_h_c;  /* zdt / zdx / mu0 sin(psi), cos(psi) */
+    double cez, chz;          /* 2 sqrt(mu0 / eps0) sin(theta), 2 sqrt(eps0 / mu0) sin(theta) */
+    const double *eps, *mu;   /* relative material per cell */
+    const double *wave;       /* waveform, one sample per substep (shifted by 12) */
+    double *hist;
+} GsvLtsfLine;
+
+/* layered TSF face corrections of one axis (faces 0 / 1 = near / far) for E
+ * (hside 0) or H (hside 1): per face cell the lateral delay q (steps) and the
+ * coefficient (E: dt / dx x profile / eps of the cell, H: dt / dx / mu0 x profile) */
+typedef struct {
+    int i0, j0, k0, i1, j1, k1;
+    int skip[6];              /* i0 in j0 jn k0 kn */
+    int res, nrows;
+    double sph, cph;
+    const double *hist;
+    const double *q[2][3];    /* [hside][axis]: [face][a][b] */
+    const double *coef[2][3];
+} GsvLtsf;
+
 /* periodic far field: the z0 / zn planes of the box, (pimax - pimin) x
  * (pjmax - pjmin) images shifted by the box width; post != 0 bins only steps
  * >= ppstart with the postprocess mode's operation order */
@@ -164,6 +193,8 @@ hipError_t gsv_launch_liao_faces(const GsvGrid *g, const GsvLiao *L, hipStream_t
 hipError_t gsv_launch_zero_planes(const GsvGrid *g, int axis, hipStream_t st);
 hipError_t gsv_launch_periodic(const GsvGrid *g, const GsvPeriodic *p, int hside, hipStream_t st);
 hipError_t gsv_launch_farfield_td(const GsvGrid *g, const GsvFarfieldTD *f, int step, hipStream_t st);
+hipError_t gsv_launch_ltsf_line(const GsvLtsfLine *l, hipStream_t st);
+hipError_t gsv_launch_ltsf_faces(const GsvGrid *g, const GsvLtsf *l, int hside, int step, hipStream_t st);
 hipError_t gsv_launch_pfarfield(const GsvGrid *g, const GsvPFarfield *f, int step, hipStream_t st);
 hipError_t gsv_launch_dft_sample(const GsvGrid *g, const GsvDft *d, int slot, hipStream_t st);
 hipError_t gsv_launch_dft_flush(const GsvDft *d, int first_step, int nsamples, hipStream_t st);

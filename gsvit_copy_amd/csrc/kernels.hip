@@ -19,6 +19,8 @@
  *   k_periodic_h / _e     interior periodic plane copies, src3d/boundary.c:304-346, 889-932
  *   k_record_points       point traces, src3d/output.c:340-352
  *   k_farfield_td         time-domain Kirchhoff bins, src3d/farfield.c:174-407
+ *   k_ltsf_line           layered-TSF split-field 1-D model, src3d/zpool.c:172-338
+ *   k_ltsf_face           layered-TSF face corrections, src3d/source.c:921-1150, 1700-1900
  *   k_pfarfield           periodic Kirchhoff bins (z planes, images), src3d/pfarfield.c:120-401
  *   k_dft_sample / k_dft_flush   running DFT of tangential fields on the NFFF box
  *   k_box_sum             OUT_SUM box sums of E^2, src3d/output.c:259-306
@@ -1317,6 +1319,156 @@ k_farfield_td(GsvGrid g, GsvFarfieldTD f, int step)
     }
 }
 
+/* layered TSF 1-D model, src3d/zpool.c:172-338 in the order of
+ * src3d/source.c:1680-1689, one workgroup: every substep saves the end
+ * values, updates E (vacuum line e and the split fields ee, eh, ez), injects
+ * the line's h[9] into ee / eh at cell 10, applies the Mur ends to e, ee, eh
+ * and the hard source at e[0], updates H, injects e[10] into he / hh at cell
+ * 9 and stores the six split fields.  The line's reads beyond its ends (h[-1],
+ * e[res]) are zero guard cells.  Serial end work runs on thread 0 between
+ * barriers. */
+__global__ __launch_bounds__(1024) void
+k_ltsf_line(GsvLtsfLine L)
+{
+    extern __shared__ double lm[];
+    const int res = L.res, n2 = res + 2, tid = threadIdx.x, nt = blockDim.x;
+    double *e = lm + 1, *h = lm + n2 + 1;
+    double *ee = lm + 2 * n2, *eh = ee + n2, *ez = eh + n2, *he = ez + n2, *hh = he + n2, *hz = hh + n2;
+    for (int i = tid; i < 8 * n2; i += nt)
+        lm[i] = 0.0;
+    __syncthreads();
+    for (int s = 0; s < L.nrows; s++) {
+        double fe0 = 0, fe1 = 0, fen0 = 0, fen1 = 0, fee0 = 0, fee1 = 0, feen0 = 0, feen1 = 0;
+        double feh0 = 0, feh1 = 0, fehn0 = 0, fehn1 = 0;
+        if (tid == 0) {
+            fe0 = e[0]; fe1 = e[1]; fen0 = e[res - 1]; fen1 = e[res - 2];
+            fee0 = ee[0]; fee1 = ee[1]; feen0 = ee[res - 1]; feen1 = ee[res - 2];
+            feh0 = eh[0]; feh1 = eh[1]; fehn0 = eh[res - 1]; fehn1 = eh[res - 2];
+        }
+        __syncthreads();
+        for (int i = tid; i < res; i += nt) {
+            e[i] += 1.0 * L.multe * (h[i - 1] - h[i]);
+            if (i > 0) {
+                const double ep = L.eps[i];
+                ee[i] = ee[i] + 1.0 / ep * L.multe * (he[i - 1] - he[i]);
+                eh[i] = eh[i] + 1.0 / (ep - L.st * L.st) * L.multe * (hh[i - 1] - hh[i]);
+                ez[i] = -ez[i] - L.cez / ep * he[i];
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            ee[10] -= L.inj_e_s * h[9];
+            eh[10] -= L.inj_e_c * h[9];
+            e[0] = fe1 + L.m0 * (e[1] - fe0);
+            e[res - 1] = fen1 + L.m1 * (e[res - 2] - fen0);
+            ee[0] = fee1 + L.m0 * (ee[1] - fee0);
+            ee[res - 1] = feen1 + L.m1 * (ee[res - 2] - feen0);
+            eh[0] = feh1 + L.m0 * (eh[1] - feh0);
+            eh[res - 1] = fehn1 + L.m1 * (eh[res - 2] - fehn0);
+            const int ip = s - 12;
+            e[0] = (ip > 0 && ip < L.ndata - 1) ? L.wave[ip] : 0.0;
+        }
+        __syncthreads();
+        for (int i = tid; i < res; i += nt) {
+            h[i] += 1.0 * L.multh * (e[i] - e[i + 1]);
+            if (i < res - 1) {
+                const double ep = L.eps[i], m = L.mu[i];
+                he[i] = he[i] + 1.0 / m * ep / (ep - L.st * L.st) * L.multh * (ee[i] - ee[i + 1]);
+                hh[i] = hh[i] + 1.0 / m * L.multh * (eh[i] - eh[i + 1]);
+                hz[i] = -hz[i] + L.chz * eh[i];
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            he[9] -= L.inj_h_s * e[10];
+            hh[9] -= L.inj_h_c * e[10];
+        }
+        __syncthreads();
+        const double *F[6] = { ee, eh, ez, he, hh, hz };
+        for (int q = 0; q < 6; q++) {
+            double *dst = L.hist + ((long long)q * L.nrows + s) * res;
+            for (int i = tid; i < res; i += nt)
+                dst[i] = F[q][i];
+        }
+    }
+}
+
+/* the stored component q at z cell k for a face cell of delay qd (steps),
+ * src3d/source.c:286-322: timepos = 3 (step - qd) substeps, tp truncated
+ * toward zero then clamped at 0, linear between tp and tp + 1 */
+__device__ __forceinline__ double
+ltsf_at(const GsvLtsf &L, int q, int k, double tpos)
+{
+    int tp = (int)tpos;
+    const double atp = tpos - (double)tp;
+    if (tp < 0)
+        tp = 0;
+    const double *d = L.hist + (long long)q * L.nrows * L.res;
+    return (1.0 - atp) * d[(long long)tp * L.res + k] + atp * d[(long long)(tp + 1) * L.res + k];
+}
+
+/* layered TSF faces of one axis, src3d/source.c:921-1150 (E, hside 0) and
+ * 1700-1900 (H, hside 1): one thread per (face, a, b); E faces sample the
+ * model's H outside the near faces (i0 - 1, j0 - 1, k0 - 1), H faces sample
+ * its E at the face cells */
+__global__ __launch_bounds__(256) void
+k_ltsf_face(GsvGrid g, GsvLtsf L, int axis, int hside, int step)
+{
+    const int na = axis == 0 ? L.j1 - L.j0 + 1 : L.i1 - L.i0 + 1;
+    const int nb = axis == 2 ? L.j1 - L.j0 + 1 : L.k1 - L.k0 + 1;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 2 * na * nb)
+        return;
+    const int f = q / (na * nb), a = (q / nb) % na, b = q % nb;
+    if (L.skip[2 * axis + f])
+        return;
+    const double tpos = 3.0 * ((double)step - L.q[hside][axis][q]);
+    const double cf = L.coef[hside][axis][q];
+    const double sph = L.sph, cph = L.cph;
+    const long long px = g.px, pl = g.plane;
+    int i, j, k, ks;
+    if (axis == 0) { i = f ? L.i1 : L.i0; j = L.j0 + a; k = L.k0 + b; ks = k; }
+    else if (axis == 1) { i = L.i0 + a; j = f ? L.j1 : L.j0; k = L.k0 + b; ks = k; }
+    else { i = L.i0 + a; j = L.j0 + b; k = f ? L.k1 : L.k0; ks = (f || hside) ? k : k - 1; }
+    if (!hside) {
+        const double vhh = ltsf_at(L, 4, ks, tpos), vhe = ltsf_at(L, 3, ks, tpos);
+        const long long c = i + px * j + pl * k;
+        const double sg = f ? -1.0 : 1.0;
+        if (axis == 0) {
+            const double hz = ltsf_at(L, 5, ks, tpos), hy = -(vhh * sph - vhe * cph);
+            if (j < L.j1) g.ey[c] = (float)((double)g.ey[c] + sg * (cf * hz));
+            if (k < L.k1) g.ez[c] = (float)((double)g.ez[c] - sg * (cf * hy));
+        } else if (axis == 1) {
+            const double hz = ltsf_at(L, 5, ks, tpos), hx = -(vhh * cph + vhe * sph);
+            if (i < L.i1) g.ex[c] = (float)((double)g.ex[c] - sg * (cf * hz));
+            if (k < L.k1) g.ez[c] = (float)((double)g.ez[c] + sg * (cf * hx));
+        } else {
+            const double hx = -(vhh * cph + vhe * sph), hy = -(vhh * sph - vhe * cph);
+            if (i < L.i1) g.ex[c] = (float)((double)g.ex[c] + sg * (cf * hy));
+            if (j < L.j1) g.ey[c] = (float)((double)g.ey[c] - sg * (cf * hx));
+        }
+    } else {
+        const double veh = ltsf_at(L, 1, ks, tpos), vee = ltsf_at(L, 0, ks, tpos);
+        const double sg = f ? -1.0 : 1.0;
+        if (axis == 0) {
+            const long long c = (f ? i : i - 1) + px * j + pl * k;
+            const double ez = ltsf_at(L, 2, ks, tpos), ey = vee * sph + veh * cph;
+            if (k < L.k1) g.hy[c] = (float)((double)g.hy[c] - sg * (cf * ez));
+            if (j < L.j1) g.hz[c] = (float)((double)g.hz[c] + sg * (cf * ey));
+        } else if (axis == 1) {
+            const long long c = i + px * (f ? j : j - 1) + pl * k;
+            const double ez = ltsf_at(L, 2, ks, tpos), ex = vee * cph - veh * sph;
+            if (k < L.k1) g.hx[c] = (float)((double)g.hx[c] + sg * (cf * ez));
+            if (i < L.i1) g.hz[c] = (float)((double)g.hz[c] - sg * (cf * ex));
+        } else {
+            const long long c = i + px * j + pl * (f ? k : k - 1);
+            const double ex = vee * cph - veh * sph, ey = vee * sph + veh * cph;
+            if (j < L.j1) g.hx[c] = (float)((double)g.hx[c] - sg * (cf * ey));
+            if (i < L.i1) g.hy[c] = (float)((double)g.hy[c] + sg * (cf * ex));
+        }
+    }
+}
+
 /* periodic far field, src3d/pfarfield.c:120-401: one thread per (image,
  * plane, i, j) and point (blockIdx.y), i fastest so that a wave reads 64
  * consecutive cells.  The distance is formed in float with explicit
@@ -1658,6 +1810,39 @@ gsv_launch_periodic(const GsvGrid *g, const GsvPeriodic *p, int hside, hipStream
     else
         hipLaunchKernelGGL(k_periodic_e, grd, blk, 0, st, *g, *p);
     return hipGetLastError();
+}
+
+extern "C" hipError_t
+gsv_launch_ltsf_line(const GsvLtsfLine *l, hipStream_t st)
+{
+    const int nt = l->res >= 1024 ? 1024 : ((l->res + 63) / 64) * 64;
+    const size_t lds = 8 * (size_t)(l->res + 2) * sizeof(double);
+    if (lds > 160 * 1024)
+        return hipErrorInvalidValue;
+    if (lds > 64 * 1024) {
+        hipError_t err = hipFuncSetAttribute((const void *)k_ltsf_line, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (err != hipSuccess)
+            return err;
+    }
+    hipLaunchKernelGGL(k_ltsf_line, dim3(1), dim3(nt), lds, st, *l);
+    return hipGetLastError();
+}
+
+/* one launch per axis, x then y then z, as the reference's loops: faces of
+ * different axes correct the same components on the box edges */
+extern "C" hipError_t
+gsv_launch_ltsf_faces(const GsvGrid *g, const GsvLtsf *l, int hside, int step, hipStream_t st)
+{
+    for (int axis = 0; axis < 3; axis++) {
+        const int na = axis == 0 ? l->j1 - l->j0 + 1 : l->i1 - l->i0 + 1;
+        const int nb = axis == 2 ? l->j1 - l->j0 + 1 : l->k1 - l->k0 + 1;
+        const int n = 2 * na * nb;
+        hipLaunchKernelGGL(k_ltsf_face, dim3((n + 255) / 256), dim3(256), 0, st, *g, *l, axis, hside, step);
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess)
+            return err;
+    }
+    return hipSuccess;
 }
 
 extern "C" hipError_t

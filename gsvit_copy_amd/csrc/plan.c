@@ -496,6 +496,81 @@ load_sources(SvPool *mp, SvSet *set)
                                 "above 0.75 (DT_MULT <= %.4g keeps it stable)\n", s, set->sc.dtmult * 0.75 / s);
         }
     }
+    if (set->ss.ltsf.valid) {
+        /* src3d/plan.c:1600-1685 */
+        const SvSetSourceLTSF *t = &set->ss.ltsf;
+        SvSourceLTSF *ls = (SvSourceLTSF *)calloc(1, sizeof(SvSourceLTSF));
+        const int zres = set->sp.zres;
+        FILE *fr;
+        int n, q, k;
+        mp->src->ltsf = ls;
+        ls->box_i0 = t->box_i0; ls->box_j0 = t->box_j0; ls->box_k0 = t->box_k0;
+        ls->box_in = t->box_in; ls->box_jn = t->box_jn; ls->box_kn = t->box_kn;
+        ls->ia_theta = t->ia_theta; ls->ia_phi = t->ia_phi; ls->ia_psi = t->ia_psi;
+        if (ls->box_i0 < 1 || ls->box_j0 < 1 || ls->box_k0 < 1 || ls->box_in >= set->sp.xres || ls->box_jn >= set->sp.yres
+            || ls->box_kn >= zres || ls->box_in <= ls->box_i0 || ls->box_jn <= ls->box_j0 || ls->box_kn <= ls->box_k0) {
+            fprintf(stderr, "Error: LTSF box must lie inside the pool with one cell to spare\n");
+            return 1;
+        }
+        if (zres < 12) {
+            fprintf(stderr, "Error: LTSF needs at least 12 cells along z (its 1-D model injects at cell 10)\n");
+            return 1;
+        }
+        if (!(t->ia_theta >= 0 && t->ia_theta <= M_PI / 2)) {
+            fprintf(stderr, "Error: unsupported LTSF angle (theta must lie in 0 .. pi/2)\n");
+            return 1;
+        }
+        if (t->box_boundary_skipdepth >= 0 || set->ss.tsf.box_boundary_skipdepth >= 0) {
+            fprintf(stderr, "Error: LTSF with a material skip depth is not supported\n");
+            return 1;
+        }
+        fr = fopen(t->source_filename, "r");
+        if (!fr) {
+            fprintf(stderr, "Cannot load source file %s\n", t->source_filename);
+            return 1;
+        }
+        if (fscanf(fr, "%d", &n) != 1 || n <= 0) {
+            fprintf(stderr, "Error: cannot read number of values from file %s\n", t->source_filename);
+            fclose(fr);
+            return 1;
+        }
+        if (n < 3 * set->sc.nsteps)
+            fprintf(stderr, "Warning: less data (%d) in \"%s\" than scheduled computation steps (%d) in LTSF source "
+                            "(need to be three times more)\n", n, t->source_filename, set->sc.nsteps);
+        ls->ndata = n;
+        ls->e = (double *)calloc(n, sizeof(double));
+        for (q = 0; q < n; q++) {
+            int pos;
+            if (fscanf(fr, "%d %lf", &pos, &ls->e[q]) != 2) {
+                fprintf(stderr, "Error: cannot read LTSF sample %d from %s\n", q, t->source_filename);
+                fclose(fr);
+                return 1;
+            }
+        }
+        fclose(fr);
+        ls->corr = 1.0 / sqrt(pow(sin(ls->ia_theta), 4) * (pow(cos(ls->ia_phi), 4) + pow(sin(ls->ia_phi), 4))
+                              + pow(cos(ls->ia_theta), 4));
+        ls->zdx = set->sp.dx / ls->corr;
+        ls->zdt = set->plan.dt / 3;
+        /* a face cell's delay is 3 (step - r dx sin(theta) / (c dt)) substeps with
+         * r >= -(|cos phi| + |sin phi|) >= -sqrt(2) */
+        ls->nrows = 3 * set->sc.nsteps + (int)ceil(3 * 1.5 * set->sp.dx / (SV_LIGHT_SPEED * set->plan.dt)) + 3;
+        /* src3d/zpool.c:118-131: each layer from its z position to the end */
+        ls->zeps = (double *)malloc(zres * sizeof(double));
+        ls->zmu = (double *)malloc(zres * sizeof(double));
+        for (k = 0; k < zres; k++)
+            ls->zeps[k] = ls->zmu[k] = 1;
+        for (q = 0; q < t->layered_count; q++) {
+            if (t->layered_zpos[q] < 0) {
+                fprintf(stderr, "Error: LTSF layer %d starts below z = 0\n", q);
+                return 1;
+            }
+            for (k = t->layered_zpos[q]; k < zres; k++) {
+                ls->zeps[k] = t->layered_epsilon[q];
+                ls->zmu[k] = t->layered_mu[q];
+            }
+        }
+    }
     return 0;
 }
 
@@ -1742,6 +1817,12 @@ sv_pool_free(SvPool *mp, SvSet *set)
             free(mp->src->sp[n].ex);
         }
         free(mp->src->sp);
+        if (mp->src->ltsf) {
+            free(mp->src->ltsf->e);
+            free(mp->src->ltsf->zeps);
+            free(mp->src->ltsf->zmu);
+            free(mp->src->ltsf);
+        }
         if (mp->src->tsf) {
             free(mp->src->tsf->e);
             free(mp->src->tsf);

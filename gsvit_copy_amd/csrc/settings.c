@@ -65,6 +65,9 @@ clear_settings(SvSet *set, int remove)
     set->ss.tsf.source_wl = 0.633e-6;
     set->ss.tsf.source_amplitude = 1.0;
     set->ss.tsf.source_pulsewidth = 100;
+    set->ss.ltsf.box_i0 = set->ss.ltsf.box_j0 = set->ss.ltsf.box_k0 = 10;
+    set->ss.ltsf.box_in = set->ss.ltsf.box_jn = set->ss.ltsf.box_kn = 90;
+    set->ss.ltsf.box_boundary_skipdepth = -1;
     set->sb.depth_bx0 = set->sb.depth_bxn = set->sb.depth_by0 = 10;
     set->sb.depth_byn = set->sb.depth_bz0 = set->sb.depth_bzn = 10;
     set->smb.bxnpos = set->smb.bynpos = set->smb.bznpos = -1;
@@ -423,6 +426,81 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
                 if (get_int(fr, &t->box_boundary_skipdepth, key))
                     goto fail;
             }
+        } else if (!strcmp(key, "SOURCE_LTSF")) {
+            /* src3d/settings.c:4342-4433: box, theta phi psi, layer count, per layer
+             * z position and eps mu sigma sigma* (or 99 and a database material),
+             * source type and file */
+            SvSetSourceLTSF *t = &set->ss.ltsf;
+            int srctype, q;
+            if (get_int(fr, &t->box_i0, key) || get_int(fr, &t->box_j0, key) || get_int(fr, &t->box_k0, key)
+                || get_int(fr, &t->box_in, key) || get_int(fr, &t->box_jn, key) || get_int(fr, &t->box_kn, key)
+                || get_double(fr, &t->ia_theta, key) || get_double(fr, &t->ia_phi, key) || get_double(fr, &t->ia_psi, key)
+                || get_int(fr, &t->layered_count, key))
+                goto fail;
+            if (t->layered_count < 0 || t->layered_count > SV_LTSF_MAX_LAYERS) {
+                fprintf(stderr, "Error: SOURCE_LTSF layer count %d outside 0..%d\n", t->layered_count, SV_LTSF_MAX_LAYERS);
+                goto fail;
+            }
+            for (q = 0; q < t->layered_count; q++) {
+                char *end;
+                if (get_int(fr, &t->layered_zpos[q], key) || get_token(fr, value, sizeof(value)))
+                    goto fail;
+                if (!strcmp(value, "99")) {
+                    fprintf(stderr, "Error: SOURCE_LTSF database materials (99) are not supported\n");
+                    goto fail;
+                }
+                t->layered_epsilon[q] = strtod(value, &end);
+                if (*end != '\0') {
+                    fprintf(stderr, "Error parsing double value (key %s)\n", key);
+                    goto fail;
+                }
+                if (get_double(fr, &t->layered_mu[q], key) || get_double(fr, &t->layered_sigma[q], key)
+                    || get_double(fr, &t->layered_sigast[q], key))
+                    goto fail;
+            }
+            if (get_int(fr, &srctype, key))
+                goto fail;
+            t->source_mode = srctype;
+            if (srctype != 0) {
+                fprintf(stderr, "Error: SOURCE_LTSF needs a waveform file (type 0) sampled at dt/3\n");
+                goto fail;
+            }
+            if (get_rest_of_line(fr, t->source_filename, SV_NAME_LEN))
+                goto fail;
+            t->valid = 1;
+        } else if (!strcmp(key, "LTSF_SKIP")) {
+            SvSetSourceLTSF *t = &set->ss.ltsf;
+            if (get_token(fr, value, sizeof(value)))
+                goto fail;
+            if (!strcmp(value, "i0")) t->box_boundary_skipi0 = 1;
+            else if (!strcmp(value, "in")) t->box_boundary_skipin = 1;
+            else if (!strcmp(value, "j0")) t->box_boundary_skipj0 = 1;
+            else if (!strcmp(value, "jn")) t->box_boundary_skipjn = 1;
+            else if (!strcmp(value, "k0")) t->box_boundary_skipk0 = 1;
+            else if (!strcmp(value, "kn")) t->box_boundary_skipkn = 1;
+            else if (!strcmp(value, "depth")) {
+                if (get_int(fr, &t->box_boundary_skipdepth, key))
+                    goto fail;
+            }
+        } else if (!strcmp(key, "LTSF_GAUSSIAN")) {
+            SvSetSourceLTSF *t = &set->ss.ltsf;
+            t->gaussian = 1;
+            if (get_double(fr, &t->gaussian_fxpos, key) || get_double(fr, &t->gaussian_fypos, key)
+                || get_double(fr, &t->gaussian_rx, key) || get_double(fr, &t->gaussian_ry, key))
+                goto fail;
+        } else if (!strcmp(key, "LTSF_FIBER")) {
+            SvSetSourceLTSF *t = &set->ss.ltsf;
+            t->fiber = 1;
+            if (get_double(fr, &t->fiber_fxpos, key) || get_double(fr, &t->fiber_fypos, key)
+                || get_double(fr, &t->fiber_radius, key) || get_double(fr, &t->fiber_cutoff, key)
+                || get_double(fr, &t->fiber_epsilon_core, key) || get_double(fr, &t->fiber_epsilon_cladding, key))
+                goto fail;
+        } else if (!strcmp(key, "LTSF_RADIAL")) {
+            /* parsed by the reference but not used by its LTSF faces */
+            double v[4];
+            if (get_double(fr, v, key) || get_double(fr, v + 1, key) || get_double(fr, v + 2, key)
+                || get_double(fr, v + 3, key))
+                goto fail;
         } else if (!strcmp(key, "TSFSOURCE_MATERIAL")) {
             SvSetSourceTSF *t = &set->ss.tsf;
             if (get_double(fr, &t->layered_epsilon, key) || get_double(fr, &t->layered_mu, key)

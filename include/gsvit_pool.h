@@ -73,10 +73,28 @@ typedef struct {
     double jp_epsilon, jp_mu;   /* relative material of the 1-D line */
 } SvSourceTSF;
 
+/* layered TSF source, src3d/source.h (SvSourceLTSF) and src3d/plan.c:1600-1685:
+ * the waveform (one sample per dt/3), the angular correction and the 1-D
+ * split-field model along z (res = zres, spacing dx/corr, step dt/3) whose
+ * fields are kept for nrows substeps: 3 nsteps plus a margin for the face
+ * cells whose lateral delay is negative (they read up to a few substeps past
+ * 3 (nsteps - 1); the reference reads past its stored history there) */
+typedef struct {
+    int box_i0, box_j0, box_k0, box_in, box_jn, box_kn;
+    double ia_theta, ia_phi, ia_psi;
+    double corr;
+    int ndata;
+    double *e;
+    int nrows;
+    double zdx, zdt;
+    double *zeps, *zmu;         /* relative material of the 1-D model cells */
+} SvSourceLTSF;
+
 typedef struct {
     int npnts;
     SvSourcePoint *sp;
     SvSourceTSF *tsf;
+    SvSourceLTSF *ltsf;
 } SvSource;
 
 /* recorded traces, src3d/output.h (outpointdata[i][step*6 + comp]) */

@@ -171,10 +171,34 @@ typedef struct {
     int box_boundary_skipdepth;
 } SvSetSourceTSF;
 
+/* layered total/scattered-field source, src3d/settings.h (SvSetSourceLTSF)
+ * and src3d/settings.c:4342-4480: a plane wave precomputed by a split-field
+ * 1-D model along z with layers starting at layered_zpos (cells) */
+#define SV_LTSF_MAX_LAYERS 100
+typedef struct {
+    int valid;
+    int box_i0, box_j0, box_k0, box_in, box_jn, box_kn;
+    double ia_theta, ia_phi, ia_psi;
+    int layered_count;
+    int layered_zpos[SV_LTSF_MAX_LAYERS];
+    double layered_epsilon[SV_LTSF_MAX_LAYERS], layered_mu[SV_LTSF_MAX_LAYERS];
+    double layered_sigma[SV_LTSF_MAX_LAYERS], layered_sigast[SV_LTSF_MAX_LAYERS];
+    int source_mode;
+    char source_filename[SV_NAME_LEN];
+    int box_boundary_skipi0, box_boundary_skipj0, box_boundary_skipk0;
+    int box_boundary_skipin, box_boundary_skipjn, box_boundary_skipkn;
+    int box_boundary_skipdepth;
+    int gaussian;
+    double gaussian_fxpos, gaussian_fypos, gaussian_rx, gaussian_ry;
+    int fiber;
+    double fiber_fxpos, fiber_fypos, fiber_radius, fiber_cutoff, fiber_epsilon_core, fiber_epsilon_cladding;
+} SvSetSourceLTSF;
+
 typedef struct {
     int npnts;
     SvSetSourcePoint *pnts;
     SvSetSourceTSF tsf;
+    SvSetSourceLTSF ltsf;
     double lambda_min, lambda_center, lambda_max;
 } SvSetSource;
 

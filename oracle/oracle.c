@@ -25,6 +25,8 @@
  *   point sources ........... src3d/source.c:405-437, 1441-1462
  *   TSF faces ............... src3d/source.c:547-783 (E), 1464-1657 (H)
  *   1-D auxiliary line ...... src3d/1dpool.c:83-162
+ *   layered TSF ............. src3d/zpool.c:38-338, src3d/source.c:286-322,
+ *                             921-1150 (E faces), 1680-1900 (H faces)
  *   cubic line interpolation  src3d/dline.c:89-112
  *   time-domain far field ... src3d/farfield.c:174-407
  *   periodic far field ...... src3d/pfarfield.c:120-401
@@ -98,6 +100,7 @@ typedef struct {
     int no_average;                   /* test option: E coefficients without interface averaging */
     double *objacc;                   /* OBJECTIVE_DFT box: [cell][3][re, im] */
     double *pfstore;                  /* periodic far field, postprocess mode: [plane][6][ix][iy][nt] */
+    double *lz;                       /* layered TSF history: [6: ee eh ez he hh hz][nrows][nz] */
 } Oracle;
 
 #define IDX(o, i, j, k) (((size_t)(i) * (o)->ny + (j)) * (o)->nz + (k))
@@ -226,7 +229,7 @@ oracle_close(void *handle)
             free(o->px[q]); free(o->py[q]); free(o->pz[q]); free(o->pxp[q]); free(o->pyp[q]); free(o->pzp[q]);
         }
     }
-    free(o->je); free(o->jh); free(o->points); free(o->objacc); free(o->pfstore);
+    free(o->je); free(o->jh); free(o->points); free(o->objacc); free(o->pfstore); free(o->lz);
     sv_pool_free(o->mp, &o->set);
     clear_settings(&o->set, 1);
     free(o);
@@ -1222,6 +1225,291 @@ farfield(Oracle *o)
     }
 }
 
+/* --------------------------------------------------------------- */
+/* layered TSF (LTSF)                                                */
+/* --------------------------------------------------------------- */
+
+/* The split-field 1-D model along z, src3d/zpool.c:38-338, run for nrows
+ * substeps of dt/3 in the order of src3d/source.c:1680-1689 (boundary copy,
+ * E update, Mur ends on e / ee / eh, hard source at cell 0, H update, store).
+ * The vacuum line e, h drives the split fields through a TF/SF step at cell
+ * 10; its waveform sample for substep s is e[s - 12] (src3d/source.c:1410-1419,
+ * time shift 12).  The reference reads h[-1] and e[res] at the line's ends;
+ * the values they produce are overwritten before use, so they are taken as 0. */
+static void
+ltsf_precompute(Oracle *o)
+{
+    const SvSourceLTSF *ls = o->mp->src->ltsf;
+    const int res = o->nz, nr = ls->nrows;
+    const double st = sin(ls->ia_theta), sps = sin(ls->ia_psi), cps = cos(ls->ia_psi);
+    const double zdt = ls->zdt, zdx = ls->zdx, corr = ls->corr;
+    const double multe = zdt / (EPS0 * zdx) / corr, multh = zdt / (MU0 * zdx) / corr;
+    const double *eps = ls->zeps, *mu = ls->zmu;
+    const double ind0 = sqrt(eps[0]), ind1 = sqrt(eps[res - 1]);
+    const double m0 = (zdt * CL / ind0 - zdx) / (zdt * CL / ind0 + zdx), m1 = (zdt * CL / ind1 - zdx) / (zdt * CL / ind1 + zdx);
+    double *buf = (double *)zalloc(8 * (size_t)(res + 2), sizeof(double));
+    double *e = buf + 1, *h = buf + (res + 2) + 1;            /* h[-1] and e[res] read as 0 */
+    double *ee = buf + 2 * (res + 2), *eh = ee + (res + 2), *ez = eh + (res + 2);
+    double *he = ez + (res + 2), *hh = he + (res + 2), *hz = hh + (res + 2);
+    double *F[6] = { ee, eh, ez, he, hh, hz };
+    int s, i, q;
+    if (!o->lz)
+        o->lz = (double *)zalloc(6 * (size_t)nr * res, sizeof(double));
+    for (s = 0; s < nr; s++) {
+        const double fe0 = e[0], fe1 = e[1], fen0 = e[res - 1], fen1 = e[res - 2];
+        const double fee0 = ee[0], fee1 = ee[1], feen0 = ee[res - 1], feen1 = ee[res - 2];
+        const double feh0 = eh[0], feh1 = eh[1], fehn0 = eh[res - 1], fehn1 = eh[res - 2];
+        const int ipos = s - 12;
+        for (i = 0; i < res; i++)
+            e[i] += 1.0 * multe * (h[i - 1] - h[i]);
+        for (i = 1; i < res; i++) {
+            ee[i] = ee[i] + 1.0 / eps[i] * multe * (he[i - 1] - he[i]);
+            eh[i] = eh[i] + 1.0 / (eps[i] - st * st) * multe * (hh[i - 1] - hh[i]);
+            ez[i] = -ez[i] - 2 * sqrt(MU0 / EPS0) * st / eps[i] * he[i];
+        }
+        ee[10] -= zdt / zdx / (EPS0 * eps[10]) * h[9] * sps;
+        eh[10] -= zdt / zdx / (EPS0 * eps[10]) * h[9] * cps;
+        e[0] = fe1 + m0 * (e[1] - fe0);
+        e[res - 1] = fen1 + m1 * (e[res - 2] - fen0);
+        ee[0] = fee1 + m0 * (ee[1] - fee0);
+        ee[res - 1] = feen1 + m1 * (ee[res - 2] - feen0);
+        eh[0] = feh1 + m0 * (eh[1] - feh0);
+        eh[res - 1] = fehn1 + m1 * (eh[res - 2] - fehn0);
+        e[0] = (ipos > 0 && ipos < ls->ndata - 1) ? ls->e[ipos] : 0.0;
+        for (i = 0; i < res; i++)
+            h[i] += 1.0 * multh * (e[i] - e[i + 1]);
+        for (i = 0; i < res - 1; i++) {
+            he[i] = he[i] + 1.0 / mu[i] * eps[i] / (eps[i] - st * st) * multh * (ee[i] - ee[i + 1]);
+            hh[i] = hh[i] + 1.0 / mu[i] * multh * (eh[i] - eh[i + 1]);
+            hz[i] = -hz[i] + 2 * sqrt(EPS0 / MU0) * st * eh[i];
+        }
+        he[9] -= zdt / zdx / MU0 * e[10] * sps;
+        hh[9] -= zdt / zdx / MU0 * e[10] * cps;
+        for (q = 0; q < 6; q++)
+            memcpy(o->lz + ((size_t)q * nr + s) * res, F[q], res * sizeof(double));
+    }
+    free(buf);
+}
+
+/* lateral delay of a face sample, src3d/source.c:286-322 (getcoefs): the
+ * distance along the in-plane direction from the box corner the wave enters
+ * first, in substeps; tp truncated toward zero, then clamped at 0 */
+static void
+ltsf_coefs(const Oracle *o, int i, int j, double *timepos)
+{
+    const SvSourceLTSF *ls = o->mp->src->ltsf;
+    const double phi = ls->ia_phi;
+    int rx, ry;
+    double radius, alpha;
+    if (phi >= 0 && phi <= M_PI / 2.0) { rx = i - ls->box_i0; ry = j - ls->box_j0; }
+    else if (phi > M_PI / 2.0 && phi <= M_PI) { rx = i - ls->box_in; ry = j - ls->box_j0; }
+    else if (phi > M_PI && phi <= 3.0 * M_PI / 2.0) { rx = i - ls->box_in; ry = j - ls->box_jn; }
+    else { rx = i - ls->box_i0; ry = j - ls->box_jn; }
+    radius = sqrt(rx * rx + ry * ry);
+    alpha = atan2(ry, rx);
+    radius *= cos(alpha - phi);
+    *timepos = 3.0 * ((double)o->set.sc.step_act - radius * o->set.sp.dx * sin(ls->ia_theta) / (CL * o->set.plan.dt));
+}
+
+/* the stored component q (0..5 = ee eh ez he hh hz) at z cell k, linearly
+ * interpolated between substeps tp and tp + 1 */
+static double
+ltsf_val(const Oracle *o, int q, int i, int j, int k)
+{
+    const SvSourceLTSF *ls = o->mp->src->ltsf;
+    const size_t res = o->nz, nr = ls->nrows;
+    double timepos, atp;
+    int tp;
+    ltsf_coefs(o, i, j, &timepos);
+    tp = (int)timepos;
+    atp = timepos - (double)tp;
+    if (tp < 0)
+        tp = 0;
+    return (1.0 - atp) * o->lz[((size_t)q * nr + tp) * res + k] + atp * o->lz[((size_t)q * nr + tp + 1) * res + k];
+}
+
+/* beam profiles, src3d/source.c:231-251 */
+static double
+ltsf_gauss(const SvSetSourceLTSF *t, int x, int y)
+{
+    return exp(-((double)(x - t->gaussian_fxpos)) * ((double)(x - t->gaussian_fxpos)) / t->gaussian_rx / t->gaussian_rx / 2.0)
+           * exp(-((double)(y - t->gaussian_fypos)) * ((double)(y - t->gaussian_fypos)) / t->gaussian_ry / t->gaussian_ry / 2.0);
+}
+
+static double
+ltsf_fiber(const Oracle *o, int x, int y)
+{
+    const SvSetSourceLTSF *t = &o->set.ss.ltsf;
+    const double dx = o->set.sp.dx, lambda = o->set.ss.lambda_center;
+    double v = 2 * M_PI / lambda * t->fiber_radius * dx * (t->fiber_epsilon_core - t->fiber_epsilon_cladding);
+    double omegazero = (0.65 + 1.619 / v / sqrt(v) + 2.879 / pow(v, 6)) * t->fiber_radius * dx;
+    double radval = ((x - t->fiber_fxpos) * (x - t->fiber_fxpos) + (y - t->fiber_fypos) * (y - t->fiber_fypos));
+    if (radval < (t->fiber_cutoff * t->fiber_cutoff))
+        return exp(-radval * dx * dx / omegazero / omegazero);
+    return 0;
+}
+
+/* profile factor; the reference's H face at in applies the Gaussian only
+ * together with the fiber profile (its "if / if-else" there, source.c:1734-1741) */
+static double
+ltsf_mult(const Oracle *o, int x, int y, int h_in_face)
+{
+    const SvSetSourceLTSF *t = &o->set.ss.ltsf;
+    if (h_in_face) {
+        double m = 1.0;
+        if (t->gaussian)
+            m = ltsf_gauss(t, x, y);
+        if (t->fiber)
+            m = ltsf_fiber(o, x, y);
+        else
+            m = 1.0;
+        return m;
+    }
+    if (t->gaussian)
+        return ltsf_gauss(t, x, y);
+    if (t->fiber)
+        return ltsf_fiber(o, x, y);
+    return 1.0;
+}
+
+enum { LEE, LEH, LEZ, LHE, LHH, LHZ };
+
+/* E faces, src3d/source.c:921-1150: H of the 1-D model at the cell outside
+ * each near face (i0 - 1, j0 - 1, k0 - 1) and at the far face cells */
+static void
+ltsf_estep(Oracle *o)
+{
+    const SvSourceLTSF *ls = o->mp->src->ltsf;
+    const SvSetSourceLTSF *t = &o->set.ss.ltsf;
+    const double ecor = o->set.plan.dt / o->set.sp.dx;
+    int i, j, k, i0, j0, k0, i1, j1, k1;
+    double sph, cph;
+    if (!ls)
+        return;
+    sph = sin(ls->ia_phi); cph = cos(ls->ia_phi);
+    i0 = ls->box_i0; j0 = ls->box_j0; k0 = ls->box_k0; i1 = ls->box_in; j1 = ls->box_jn; k1 = ls->box_kn;
+    for (j = j0; j <= j1; j++)
+        for (k = k0; k <= k1; k++) {
+            if (!t->box_boundary_skipi0) {
+                double g = ecor * ltsf_mult(o, i0 - 1, j, 0);
+                double vhh = ltsf_val(o, LHH, i0 - 1, j, k), vhe = ltsf_val(o, LHE, i0 - 1, j, k), hz = ltsf_val(o, LHZ, i0 - 1, j, k);
+                double hy = -(vhh * sph - vhe * cph), eps = get_epsilon(o, i0, j, k);
+                if (j < j1) o->ey[IDX(o, i0, j, k)] += g / eps * hz;
+                if (k < k1) o->ez[IDX(o, i0, j, k)] -= g / eps * hy;
+            }
+            if (!t->box_boundary_skipin) {
+                double g = ecor * ltsf_mult(o, i1, j, 0);
+                double vhh = ltsf_val(o, LHH, i1, j, k), vhe = ltsf_val(o, LHE, i1, j, k), hz = ltsf_val(o, LHZ, i1, j, k);
+                double hy = -(vhh * sph - vhe * cph), eps = get_epsilon(o, i1, j, k);
+                if (j < j1) o->ey[IDX(o, i1, j, k)] -= g / eps * hz;
+                if (k < k1) o->ez[IDX(o, i1, j, k)] += g / eps * hy;
+            }
+        }
+    for (i = i0; i <= i1; i++)
+        for (k = k0; k <= k1; k++) {
+            if (!t->box_boundary_skipj0) {
+                double g = ecor * ltsf_mult(o, i, j0 - 1, 0);
+                double vhh = ltsf_val(o, LHH, i, j0 - 1, k), vhe = ltsf_val(o, LHE, i, j0 - 1, k), hz = ltsf_val(o, LHZ, i, j0 - 1, k);
+                double hx = -(vhh * cph + vhe * sph), eps = get_epsilon(o, i, j0, k);
+                if (i < i1) o->ex[IDX(o, i, j0, k)] -= g / eps * hz;
+                if (k < k1) o->ez[IDX(o, i, j0, k)] += g / eps * hx;
+            }
+            if (!t->box_boundary_skipjn) {
+                double g = ecor * ltsf_mult(o, i, j1, 0);
+                double vhh = ltsf_val(o, LHH, i, j1, k), vhe = ltsf_val(o, LHE, i, j1, k), hz = ltsf_val(o, LHZ, i, j1, k);
+                double hx = -(vhh * cph + vhe * sph), eps = get_epsilon(o, i, j1, k);
+                if (i < i1) o->ex[IDX(o, i, j1, k)] += g / eps * hz;
+                if (k < k1) o->ez[IDX(o, i, j1, k)] -= g / eps * hx;
+            }
+        }
+    for (i = i0; i <= i1; i++)
+        for (j = j0; j <= j1; j++) {
+            if (!t->box_boundary_skipk0) {
+                double g = ecor * ltsf_mult(o, i, j, 0);
+                double vhh = ltsf_val(o, LHH, i, j, k0 - 1), vhe = ltsf_val(o, LHE, i, j, k0 - 1);
+                double hx = -(vhh * cph + vhe * sph), hy = -(vhh * sph - vhe * cph), eps = get_epsilon(o, i, j, k0);
+                if (i < i1) o->ex[IDX(o, i, j, k0)] += g / eps * hy;
+                if (j < j1) o->ey[IDX(o, i, j, k0)] -= g / eps * hx;
+            }
+            if (!t->box_boundary_skipkn) {
+                double g = ecor * ltsf_mult(o, i, j, 0);
+                double vhh = ltsf_val(o, LHH, i, j, k1), vhe = ltsf_val(o, LHE, i, j, k1);
+                double hx = -(vhh * cph + vhe * sph), hy = -(vhh * sph - vhe * cph), eps = get_epsilon(o, i, j, k1);
+                if (i < i1) o->ex[IDX(o, i, j, k1)] -= g / eps * hy;
+                if (j < j1) o->ey[IDX(o, i, j, k1)] += g / eps * hx;
+            }
+        }
+}
+
+/* H faces, src3d/source.c:1680-1900: E of the 1-D model at the face cells;
+ * the model is precomputed before the first H update */
+static void
+ltsf_hstep(Oracle *o)
+{
+    const SvSourceLTSF *ls = o->mp->src->ltsf;
+    const SvSetSourceLTSF *t = &o->set.ss.ltsf;
+    const double hcor = o->set.plan.dt / o->set.sp.dx / MU0;
+    int i, j, k, i0, j0, k0, i1, j1, k1;
+    double sph, cph;
+    if (!ls)
+        return;
+    if (!o->lz)
+        ltsf_precompute(o);
+    sph = sin(ls->ia_phi); cph = cos(ls->ia_phi);
+    i0 = ls->box_i0; j0 = ls->box_j0; k0 = ls->box_k0; i1 = ls->box_in; j1 = ls->box_jn; k1 = ls->box_kn;
+    for (j = j0; j <= j1; j++)
+        for (k = k0; k <= k1; k++) {
+            if (!t->box_boundary_skipi0) {
+                double g = hcor * ltsf_mult(o, i0, j, 0);
+                double veh = ltsf_val(o, LEH, i0, j, k), vee = ltsf_val(o, LEE, i0, j, k), ez = ltsf_val(o, LEZ, i0, j, k);
+                double ey = vee * sph + veh * cph;
+                if (k < k1) o->hy[IDX(o, i0 - 1, j, k)] -= g * ez;
+                if (j < j1) o->hz[IDX(o, i0 - 1, j, k)] += g * ey;
+            }
+            if (!t->box_boundary_skipin) {
+                double g = hcor * ltsf_mult(o, i1, j, 1);
+                double veh = ltsf_val(o, LEH, i1, j, k), vee = ltsf_val(o, LEE, i1, j, k), ez = ltsf_val(o, LEZ, i1, j, k);
+                double ey = vee * sph + veh * cph;
+                if (k < k1) o->hy[IDX(o, i1, j, k)] += g * ez;
+                if (j < j1) o->hz[IDX(o, i1, j, k)] -= g * ey;
+            }
+        }
+    for (i = i0; i <= i1; i++)
+        for (k = k0; k <= k1; k++) {
+            if (!t->box_boundary_skipj0) {
+                double g = hcor * ltsf_mult(o, i, j0, 0);
+                double veh = ltsf_val(o, LEH, i, j0, k), vee = ltsf_val(o, LEE, i, j0, k), ez = ltsf_val(o, LEZ, i, j0, k);
+                double ex = vee * cph - veh * sph;
+                if (k < k1) o->hx[IDX(o, i, j0 - 1, k)] += g * ez;
+                if (i < i1) o->hz[IDX(o, i, j0 - 1, k)] -= g * ex;
+            }
+            if (!t->box_boundary_skipjn) {
+                double g = hcor * ltsf_mult(o, i, j1, 0);
+                double veh = ltsf_val(o, LEH, i, j1, k), vee = ltsf_val(o, LEE, i, j1, k), ez = ltsf_val(o, LEZ, i, j1, k);
+                double ex = vee * cph - veh * sph;
+                if (k < k1) o->hx[IDX(o, i, j1, k)] -= g * ez;
+                if (i < i1) o->hz[IDX(o, i, j1, k)] += g * ex;
+            }
+        }
+    for (i = i0; i <= i1; i++)
+        for (j = j0; j <= j1; j++) {
+            if (!t->box_boundary_skipk0) {
+                double g = hcor * ltsf_mult(o, i, j, 0);
+                double veh = ltsf_val(o, LEH, i, j, k0), vee = ltsf_val(o, LEE, i, j, k0);
+                double ex = vee * cph - veh * sph, ey = vee * sph + veh * cph;
+                if (j < j1) o->hx[IDX(o, i, j, k0 - 1)] -= g * ey;
+                if (i < i1) o->hy[IDX(o, i, j, k0 - 1)] += g * ex;
+            }
+            if (!t->box_boundary_skipkn) {
+                double g = hcor * ltsf_mult(o, i, j, 0);
+                double veh = ltsf_val(o, LEH, i, j, k1), vee = ltsf_val(o, LEE, i, j, k1);
+                double ex = vee * cph - veh * sph, ey = vee * sph + veh * cph;
+                if (j < j1) o->hx[IDX(o, i, j, k1)] += g * ey;
+                if (i < i1) o->hy[IDX(o, i, j, k1)] -= g * ex;
+            }
+        }
+}
+
 /* periodic far field, src3d/pfarfield.c:120-401: Kirchhoff integral of E
  * over the z0 / zn planes of the box and their periodic images, in the
  * reference's loop order (point, i, j, pi, pj, z0 then zn, step).  The
@@ -1530,10 +1818,12 @@ oracle_step(void *handle, int nsteps)
         boundary_copy(o);
         ystep_h(o);
         source_hstep(o);
+        ltsf_hstep(o);
         boundary_hstep(o);
         ystep_e(o);
         boundary_estep(o);
         source_estep(o);
+        ltsf_estep(o);
         getsum(o);
         farfield(o);
         pfarfield(o);
@@ -1590,6 +1880,24 @@ oracle_get_farfield(void *handle, double *out)
     int n, nd = o->mp->farfield->ndata;
     for (n = 0; n < o->set.sf.nrs; n++)
         memcpy(out + (size_t)n * 3 * nd, o->mp->farfield->rpoints[n].ex, 3 * (size_t)nd * sizeof(double));
+}
+
+/* layered TSF: dims[0..1] = stored substeps (0 before the first step or
+ * without LTSF), z cells; history as [6: ee eh ez he hh hz][nrows][nz] */
+void
+oracle_ltsf_dims(void *handle, int *dims)
+{
+    Oracle *o = (Oracle *)handle;
+    dims[0] = (o->mp->src->ltsf && o->lz) ? o->mp->src->ltsf->nrows : 0;
+    dims[1] = o->nz;
+}
+
+void
+oracle_get_ltsf(void *handle, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    if (o->mp->src->ltsf && o->lz)
+        memcpy(out, o->lz, 6 * (size_t)o->mp->src->ltsf->nrows * o->nz * sizeof(double));
 }
 
 /* periodic far field: dims[0..1] = nrs, ndata; bins as [nrs][3][ndata] */

@@ -515,6 +515,37 @@ def pfarfield_case(folder, blocks, n=40, nsteps=200, seed=3):
     return path
 
 
+def ltsf_case(folder, shape=(24, 24, 80), nsteps=400, theta=0.0, phi=0.0, psi=0.0, layers=(), box=None, skips=(),
+              halfspace=None, lam=400e-9, dx=10e-9, dtmult=0.5, points=(), extra=(), boundary="cpml 8 3 -1 0 1"):
+    """Layered TSF (SOURCE_LTSF) case: a Gaussian-modulated sine sampled at dt/3
+    (3 nsteps samples), layers given as (z position, eps_r), optionally a
+    dielectric half space (z from halfspace[0] to the end, eps_r halfspace[1])
+    in the 3-D pool itself; angles in degrees."""
+    os.makedirs(folder, exist_ok=True)
+    nx, ny, nz = shape
+    dt = _dt(dx, dx, dx, dtmult)
+    write_tsf_source(os.path.join(folder, "ltsf.txt"), gaussian_sine(3 * nsteps, dt / 3, lam, 1.2, delay_widths=1.6))
+    if box is None:
+        box = (8 + 3, 8 + 3, 8 + 3, nx - 12, ny - 12, nz - 12)
+    lay = " ".join("%d %.6g 1 0 0" % (z, e) for z, e in layers)
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (nx, ny, nz, dx, dx, dx), "COMP\n%d\n\n" % nsteps,
+             "DT_MULT\n%g\n\n" % dtmult, "BOUNDARY_ALL\n%s\n\n" % boundary,
+             "SOURCE_LTSF\n%d %d %d %d %d %d %.17g %.17g %.17g %d %s 0 ltsf.txt\n\n"
+             % (tuple(box) + (math.radians(theta), math.radians(phi), math.radians(psi), len(layers), lay))]
+    lines += ["LTSF_SKIP\n%s\n\n" % s for s in skips]
+    if halfspace is not None:
+        with open(os.path.join(folder, "mats.txt"), "w") as f:
+            f.write("8 -5 -5 %d %d %d %d 1 %.6g 1 0 0\n" % (halfspace[0], nx + 5, ny + 5, nz + 5, halfspace[1]))
+        lines.append("MEDIUM_VECTOR\nmats.txt\n\n")
+    for q, p in enumerate(points):
+        lines.append("OUT_POINT\nAll 1 %d %d %d pt%d\n\n" % (tuple(p) + (q,)))
+    lines += [b.rstrip("\n") + "\n\n" for b in extra]
+    path = os.path.join(folder, "ltsf.par")
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path
+
+
 def write_linear_medium(path, eps_r, sigma=None, mu_r=None, sigast=None):
     """MEDIUM_LINEAR file: 3 int32 dims, then float32 eps_r, sigma, mu_r, sigma* ([i][j][k], k fastest)."""
     eps_r = np.asarray(eps_r, dtype=np.float32)

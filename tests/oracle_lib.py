@@ -32,6 +32,8 @@ def lib(precision="f64"):
         L.oracle_get_points.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_farfield.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_pfarfield_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_ltsf_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_get_ltsf.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_pfarfield.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_maps.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.oracle_get_dft.restype = ctypes.c_long
@@ -112,6 +114,15 @@ class Oracle:
         if inf["nrs"]:
             self.L.oracle_get_farfield(self.h, _dptr(out))
         return out[: inf["nrs"]]
+
+    def ltsf_history(self):
+        """layered-TSF 1-D model history [6: ee eh ez he hh hz][substep][z] (after the first step)"""
+        d = (ctypes.c_int * 2)()
+        self.L.oracle_ltsf_dims(self.h, d)
+        out = np.zeros((6, max(d[0], 1), d[1]), dtype=np.float64)
+        if d[0]:
+            self.L.oracle_get_ltsf(self.h, _dptr(out))
+        return out[:, : d[0]]
 
     def pfarfield(self):
         """periodic far-field bins [nrs][3][ndata] (Ex, Ey, Ez, not negated)"""
