@@ -603,6 +603,10 @@ gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_int
         fprintf(stderr, "Error: the gradient needs DESIGN_BOX and OBJECTIVE_DFT in the parameter file\n");
         return 1;
     }
+    if (!E->forces.empty() || E->has_pff) {
+        fprintf(stderr, "Error: the gradient does not support force boxes or the periodic far field\n");
+        return 1;
+    }
     if (g.ade || E->has_tsf || E->has_ltsf || E->has_liao || E->has_per || g.perx || g.pery) {
         fprintf(stderr, "Error: the gradient supports non-dispersive media, CPML and point sources only\n");
         return 1;

@@ -127,6 +127,7 @@ batch_reject(const Engine *E)
     if (E->has_dft) return "running DFT";
     if (E->has_flux) return "flux spectra";
     if (!E->sums.empty()) return "box sums";
+    if (!E->forces.empty()) return "force boxes";
     if (E->npts) return "point outputs";
     if (E->adj) return "adjoint state";
     if (E->g.pc_nx > 0) return "a design region";

@@ -82,6 +82,7 @@ struct Engine {
     int flx_first, flx_pending;
     /* box sums */
     std::vector<GsvBoxSum> sums;
+    std::vector<GsvForce> forces;
     /* fused step (one pass over the fields, k_step): on when `fused`; the
      * buffers the next step writes (swapped with the grid's after each step),
      * the step's TSF H corrections, the launch shape */

@@ -911,6 +911,44 @@ setup_ltsf(Engine *E)
     return 0;
 }
 
+/* OUT_FORCE boxes: eps and mu of every face cell in the kernel's order,
+ * per-step results zeroed once (each step is added to once) */
+static int
+setup_forces(Engine *E)
+{
+    SvSet *set = E->set;
+    SvPool *mp = E->mp;
+    E->forces.clear();
+    for (int n = 0; n < set->so.nforces; n++) {
+        const SvOutputForce *o = set->so.forces + n;
+        GsvForce f;
+        memset(&f, 0, sizeof(f));
+        f.i0 = o->box_i0; f.j0 = o->box_j0; f.k0 = o->box_k0;
+        f.i1 = o->box_in; f.j1 = o->box_jn; f.k1 = o->box_kn;
+        f.ncells = 2 * ((long long)(f.j1 - f.j0) * (f.k1 - f.k0) + (long long)(f.i1 - f.i0) * (f.k1 - f.k0)
+                        + (long long)(f.i1 - f.i0) * (f.j1 - f.j0));
+        std::vector<double2> em(f.ncells);
+        for (long long q = 0; q < f.ncells; q++) {
+            int i, j, k, nx, ny, nz;
+            double eps, mu, s;
+            force_cell_of(f, q, i, j, k, nx, ny, nz);
+            sv_pool_epsilon_sigma(mp, i, j, k, &eps, &s);
+            sv_pool_mu_sigast(mp, i, j, k, &mu, &s);
+            em[q] = make_double2(eps, mu);
+        }
+        double2 *pe;
+        if (dupload(E, &pe, em.data(), em.size()))
+            return 1;
+        f.em = pe;
+        const size_t cnt = 3 * ((size_t)set->sc.nsteps + 1);
+        if (dmalloc(E, &f.out, cnt))
+            return 1;
+        CK(hipMemsetAsync(f.out, 0, cnt * 8, E->st));
+        E->forces.push_back(f);
+    }
+    return 0;
+}
+
 /* periodic far field, src3d/pfarfield.c:36-62; bins zeroed, point lists and
  * istart uploaded once */
 static int
@@ -1237,7 +1275,7 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
             CK(hipMemsetAsync(g.ade_p, 0, n * 4, E->st));
         }
     }
-    if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E) || setup_ltsf(E)
+    if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E) || setup_ltsf(E) || setup_forces(E)
         || setup_flux(E) || setup_sums(E))
         return 1;
     for (int q = 0; q < 4; q++)
@@ -1375,6 +1413,9 @@ fetch_aux(Engine *E, SvPool *mp, SvSet *set)
                               hipMemcpyDeviceToHost, E->st));
         CK(hipStreamSynchronize(E->st));
     }
+    for (size_t n = 0; n < E->forces.size(); n++)
+        CK(hipMemcpyAsync(mp->out->outforcedata[n], E->forces[n].out, 3 * ((size_t)set->sc.nsteps + 1) * 8,
+                          hipMemcpyDeviceToHost, E->st));
     for (size_t n = 0; n < E->sums.size(); n++) {
         CK(hipMemcpyAsync(mp->out->outsumdata[n], E->sums[n].out, ((size_t)set->sc.nsteps + 1) * 8,
                           hipMemcpyDeviceToHost, E->st));
@@ -1596,8 +1637,8 @@ extern "C" int
 sv_pool_gpu_getforce(SvPool *mp, SvSet *set)
 {
     ENGINE_OR_FAIL(mp);
-    (void)E;
-    (void)set;
+    for (size_t n = 0; n < E->forces.size(); n++)
+        CK(gsv_launch_force(&E->g, &E->forces[n], set->sc.step_act, E->st));
     return 0;
 }
 

@@ -136,6 +136,15 @@ typedef struct {
     float *ring;              /* [T][4][ncells] */
 } GsvDft;
 
+/* one OUT_FORCE box: per step the stress tensor summed over the six faces,
+ * src3d/output.c:355-512; per face cell its absolute eps and mu */
+typedef struct {
+    int i0, j0, k0, i1, j1, k1;
+    long long ncells;             /* 2 (ny nz + nx nz + nx ny) face cells */
+    const double2 *em;            /* eps, mu per face cell in the kernel's cell order */
+    double *out;                  /* [step][3] */
+} GsvForce;
+
 /* one OUT_SUM box: per step the sum (or maximum) of selected E components
  * squared over the box, src3d/output.c:259-306 */
 typedef struct {
@@ -195,6 +204,7 @@ hipError_t gsv_launch_periodic(const GsvGrid *g, const GsvPeriodic *p, int hside
 hipError_t gsv_launch_farfield_td(const GsvGrid *g, const GsvFarfieldTD *f, int step, hipStream_t st);
 hipError_t gsv_launch_ltsf_line(const GsvLtsfLine *l, hipStream_t st);
 hipError_t gsv_launch_ltsf_faces(const GsvGrid *g, const GsvLtsf *l, int hside, int step, hipStream_t st);
+hipError_t gsv_launch_force(const GsvGrid *g, const GsvForce *f, int step, hipStream_t st);
 hipError_t gsv_launch_pfarfield(const GsvGrid *g, const GsvPFarfield *f, int step, hipStream_t st);
 hipError_t gsv_launch_dft_sample(const GsvGrid *g, const GsvDft *d, int slot, hipStream_t st);
 hipError_t gsv_launch_dft_flush(const GsvDft *d, int first_step, int nsamples, hipStream_t st);
@@ -210,6 +220,36 @@ hipError_t gsv_launch_tsf_line_batch(const GsvTsf *tsfs, const int *runs, int nr
 hipError_t gsv_launch_tsf_faces_batch(const GsvGrid *grids, const GsvTsf *tsfs, const GsvFaceTile *tiles, int ntiles,
                                       int hside, hipStream_t st);
 #ifdef __cplusplus
+}
+#endif
+
+#ifdef __cplusplus
+/* face cell q of a force box: faces x0, xn (a = j, b = k), y0, yn (a = i,
+ * b = k), z0, zn (a = i, b = j); the normal points into the box */
+static inline __host__ __device__ void
+force_cell_of(const GsvForce &f, long long q, int &i, int &j, int &k, int &nx, int &ny, int &nz)
+{
+    const long long sx = (long long)(f.j1 - f.j0) * (f.k1 - f.k0), sy = (long long)(f.i1 - f.i0) * (f.k1 - f.k0);
+    const long long sz = (long long)(f.i1 - f.i0) * (f.j1 - f.j0);
+    nx = ny = nz = 0;
+    if (q < 2 * sx) {
+        const int far = q >= sx;
+        const long long r = q - far * sx;
+        i = far ? f.i1 : f.i0; j = f.j0 + (int)(r / (f.k1 - f.k0)); k = f.k0 + (int)(r % (f.k1 - f.k0));
+        nx = far ? -1 : 1;
+    } else if (q < 2 * sx + 2 * sy) {
+        const long long r0 = q - 2 * sx;
+        const int far = r0 >= sy;
+        const long long r = r0 - far * sy;
+        i = f.i0 + (int)(r / (f.k1 - f.k0)); j = far ? f.j1 : f.j0; k = f.k0 + (int)(r % (f.k1 - f.k0));
+        ny = far ? -1 : 1;
+    } else {
+        const long long r0 = q - 2 * sx - 2 * sy;
+        const int far = r0 >= sz;
+        const long long r = r0 - far * sz;
+        i = f.i0 + (int)(r / (f.j1 - f.j0)); j = f.j0 + (int)(r % (f.j1 - f.j0)); k = far ? f.k1 : f.k0;
+        nz = far ? -1 : 1;
+    }
 }
 #endif
 

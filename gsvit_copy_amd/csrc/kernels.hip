@@ -21,6 +21,7 @@
  *   k_farfield_td         time-domain Kirchhoff bins, src3d/farfield.c:174-407
  *   k_ltsf_line           layered-TSF split-field 1-D model, src3d/zpool.c:172-338
  *   k_ltsf_face           layered-TSF face corrections, src3d/source.c:921-1150, 1700-1900
+ *   k_force               stress tensor on force-box faces, src3d/output.c:355-512
  *   k_pfarfield           periodic Kirchhoff bins (z planes, images), src3d/pfarfield.c:120-401
  *   k_dft_sample / k_dft_flush   running DFT of tangential fields on the NFFF box
  *   k_box_sum             OUT_SUM box sums of E^2, src3d/output.c:259-306
@@ -1469,6 +1470,46 @@ k_ltsf_face(GsvGrid g, GsvLtsf L, int axis, int hside, int step)
     }
 }
 
+/* stress tensor times the inward normal, summed per block (wave64
+ * reductions, then LDS) and added to the step's three values */
+__global__ __launch_bounds__(256) void
+k_force(GsvGrid g, GsvForce f, int step)
+{
+    __shared__ double part[4][3];
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double a0 = 0, a1 = 0, a2 = 0;
+    if (q < f.ncells) {
+        int i, j, k, nx, ny, nz;
+        force_cell_of(f, q, i, j, k, nx, ny, nz);
+        const long long c = i + (long long)g.px * j + g.plane * k;
+        const double ex = g.ex[c], ey = g.ey[c], ez = g.ez[c], hx = g.hx[c], hy = g.hy[c], hz = g.hz[c];
+        const double eps = f.em[q].x, mu = f.em[q].y;
+        const double val = (eps * (ex * ex + ey * ey + ez * ez) + mu * (hx * hx + hy * hy + hz * hz)) / 2.0;
+        const double t00 = eps * ex * ex + mu * hx * hx - val, t11 = eps * ey * ey + mu * hy * hy - val;
+        const double t22 = eps * ez * ez + mu * hz * hz - val;
+        const double t01 = eps * ex * ey + mu * hx * hy, t02 = eps * ex * ez + mu * hx * hz, t12 = eps * ey * ez + mu * hy * hz;
+        a0 = t00 * nx + t01 * ny + t02 * nz;
+        a1 = t01 * nx + t11 * ny + t12 * nz;
+        a2 = t02 * nx + t12 * ny + t22 * nz;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a0 += __shfl_down(a0, o, 64);
+        a1 += __shfl_down(a1, o, 64);
+        a2 += __shfl_down(a2, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[w][0] = a0; part[w][1] = a1; part[w][2] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double s = 0;
+        for (int v = 0; v < (int)(blockDim.x >> 6); v++)
+            s += part[v][threadIdx.x];
+        atomicAdd(f.out + (long long)step * 3 + threadIdx.x, s);
+    }
+}
+
 /* periodic far field, src3d/pfarfield.c:120-401: one thread per (image,
  * plane, i, j) and point (blockIdx.y), i fastest so that a wave reads 64
  * consecutive cells.  The distance is formed in float with explicit
@@ -1843,6 +1884,13 @@ gsv_launch_ltsf_faces(const GsvGrid *g, const GsvLtsf *l, int hside, int step, h
             return err;
     }
     return hipSuccess;
+}
+
+extern "C" hipError_t
+gsv_launch_force(const GsvGrid *g, const GsvForce *f, int step, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_force, dim3((unsigned)((f->ncells + 255) / 256)), dim3(256), 0, st, *g, *f, step);
+    return hipGetLastError();
 }
 
 extern "C" hipError_t

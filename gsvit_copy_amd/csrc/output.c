@@ -29,6 +29,12 @@ is_output(SvPool *mp, SvSet *set)
         if (set->sc.step_act % set->so.sums[i].step == 0)
             return 1;
     }
+    for (i = 0; i < set->so.nforces; i++) {
+        if (set->so.forces[i].step == 0)
+            return 0;
+        if (set->sc.step_act % set->so.forces[i].step == 0)
+            return 1;
+    }
     for (i = 0; i < set->so.ncubs; i++) {
         const SvOutputRegion *o = set->so.cubs + i;
         if (o->step == 0)
@@ -342,6 +348,37 @@ sv_pool_output(SvPool *mp, SvSet *set)
         }
         for (j = 0; j < set->sc.nsteps; j++)
             fprintf(fh, "%d %g\n", j, mp->out->outsumdata[n][j] * (o->component == SV_SUM_MAX ? 1.0 : vol));
+        fclose(fh);
+    }
+
+    /* force logs, src3d/output.c:908-938: per step the force times dx dy and
+     * its mean over the preceding 5 lambda_center / (c dt) steps (the
+     * reference tests forces[i].step with a stale index there; the force's
+     * own step is used here) */
+    for (n = 0; n < set->so.nforces; n++) {
+        const SvOutputForce *o = set->so.forces + n;
+        const double *fd = mp->out->outforcedata[n], area = set->sp.dx * set->sp.dy;
+        if (o->step == 0)
+            continue;
+        if (set->sc.step_act % o->step != 0 && set->sc.step_act < set->sc.nsteps - 1)
+            continue;
+        fh = fopen(o->filename, "w");
+        if (!fh) {
+            fprintf(stderr, "Error: force: cannot open file %s for writing\n", o->filename);
+            continue;
+        }
+        for (i = 1; i < set->sc.nsteps; i++) {
+            double avx = 0, avy = 0, avz = 0, w = i - (5.0 * set->ss.lambda_center / set->plan.dt / SV_LIGHT_SPEED);
+            int j, nav = 0;
+            for (j = (int)(w > 0 ? w : 0); j < i; j++) {
+                avx += fd[3 * j];
+                avy += fd[3 * j + 1];
+                avz += fd[3 * j + 2];
+                nav++;
+            }
+            fprintf(fh, "%d %g %g %g %g %g %g\n", i, fd[3 * i] * area, fd[3 * i + 1] * area, fd[3 * i + 2] * area,
+                    avx / nav * area, avy / nav * area, avz / nav * area);
+        }
         fclose(fh);
     }
 

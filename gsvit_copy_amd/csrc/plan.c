@@ -1722,6 +1722,20 @@ init_and_make_plan(SvSet *set)
         }
     }
 
+    if (set->so.nforces) {
+        mp->out->outforcedata = (double **)calloc(set->so.nforces, sizeof(double *));
+        for (n = 0; n < set->so.nforces; n++) {
+            SvOutputForce *o = set->so.forces + n;
+            if (o->box_i0 < 0 || o->box_j0 < 0 || o->box_k0 < 0 || o->box_in >= xres || o->box_jn >= yres
+                || o->box_kn >= zres || o->box_in <= o->box_i0 || o->box_jn <= o->box_j0 || o->box_kn <= o->box_k0) {
+                fprintf(stderr, "Error: force box %d (%d %d %d - %d %d %d) must lie inside the pool\n", n, o->box_i0,
+                        o->box_j0, o->box_k0, o->box_in, o->box_jn, o->box_kn);
+                return NULL;
+            }
+            mp->out->outforcedata[n] = (double *)calloc(3 * (size_t)(set->sc.nsteps + 1), sizeof(double));
+        }
+    }
+
     if (set->sf.nrs || set->sf.ndft) {
         SvSetFarfield *f = &set->sf;
         if (f->box_i0 < 1 || f->box_j0 < 1 || f->box_k0 < 1 || f->box_in >= xres - 1 || f->box_jn >= yres - 1
@@ -1834,6 +1848,11 @@ sv_pool_free(SvPool *mp, SvSet *set)
             for (n = 0; n < mp->set->so.npnts; n++)
                 free(mp->out->outpointdata[n]);
             free(mp->out->outpointdata);
+        }
+        if (mp->out->outforcedata) {
+            for (n = 0; n < mp->set->so.nforces; n++)
+                free(mp->out->outforcedata[n]);
+            free(mp->out->outforcedata);
         }
         if (mp->out->outsumdata) {
             for (n = 0; n < mp->set->so.nsums; n++)

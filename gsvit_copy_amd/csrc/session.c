@@ -164,6 +164,19 @@ gsv_get_farfield(void *h, double *out)
     return 0;
 }
 
+int
+gsv_get_forces(void *h, double *out)
+{
+    GsvSession *s = (GsvSession *)h;
+    int n;
+    if (gsv_engine_fetch_aux(s->mp->gpu))
+        return 1;
+    for (n = 0; n < s->set.so.nforces; n++)
+        memcpy(out + (size_t)n * 3 * (s->set.sc.nsteps + 1), s->mp->out->outforcedata[n],
+               3 * (size_t)(s->set.sc.nsteps + 1) * sizeof(double));
+    return 0;
+}
+
 void
 gsv_pfarfield_dims(void *h, int *dims)
 {

@@ -4,7 +4,7 @@
  * Accepts the keyword format of src3d/settings.c:3822-5840 (whitespace separated
  * keywords, '#' comments, values read with the same integer/double rules as
  * get_int/get_double, src3d/settings.c:103-135).  Keywords outside the hot path
- * (images, volumes, sub-gridding, local/focused/layered sources, forces) are
+ * (sub-gridding, local and focused sources) are
  * rejected with an error instead of being silently ignored, so a parameter
  * file never runs with part of its physics dropped.  Keywords whose only
  * product is a Gwyddion container (OUT_IMAGE, OUT_FILE, the far-field
@@ -32,6 +32,7 @@ clear_settings(SvSet *set, int remove)
         free(set->ss.pnts);
         free(set->so.pnts);
         free(set->so.sums);
+        free(set->so.forces);
         free(set->so.plns);
         free(set->so.cubs);
         free(set->sf.ri);
@@ -521,6 +522,17 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             if (get_rest_of_line(fr, o->filebase, SV_NAME_LEN))
                 goto fail;
             set->so.npnts++;
+        } else if (!strcmp(key, "OUT_FORCE")) {
+            /* src3d/settings.c:5018-5040: step, box, file */
+            SvOutputForce *o;
+            set->so.forces = (SvOutputForce *)realloc(set->so.forces, (set->so.nforces + 1) * sizeof(SvOutputForce));
+            o = set->so.forces + set->so.nforces;
+            memset(o, 0, sizeof(*o));
+            if (get_int(fr, &o->step, key) || get_int(fr, &o->box_i0, key) || get_int(fr, &o->box_j0, key)
+                || get_int(fr, &o->box_k0, key) || get_int(fr, &o->box_in, key) || get_int(fr, &o->box_jn, key)
+                || get_int(fr, &o->box_kn, key) || get_rest_of_line(fr, o->filename, SV_NAME_LEN))
+                goto fail;
+            set->so.nforces++;
         } else if (!strcmp(key, "OUT_SUM")) {
             /* src3d/settings.c:4937-4981: component step i0 j0 k0 in jn kn eps mu sigma sigast file */
             SvOutputSum *o;

@@ -37,6 +37,8 @@ def library():
         L.gsv_get_points.argtypes = [vp, dp]
         L.gsv_get_farfield.restype = ctypes.c_int
         L.gsv_get_farfield.argtypes = [vp, dp]
+        L.gsv_get_forces.restype = ctypes.c_int
+        L.gsv_get_forces.argtypes = [vp, dp]
         L.gsv_pfarfield_dims.restype = None
         L.gsv_pfarfield_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.gsv_get_pfarfield.restype = ctypes.c_int
@@ -160,6 +162,14 @@ class Session:
         if inf["nrs"] and self.L.gsv_get_farfield(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
             raise RuntimeError("far-field download failed")
         return out[: inf["nrs"]]
+
+    def forces(self, nforces):
+        """OUT_FORCE values [nforces][nsteps + 1][3] (without the log's dx dy)"""
+        inf = self.info()
+        out = np.zeros((max(nforces, 1), inf["nsteps"] + 1, 3), dtype=np.float64)
+        if nforces and self.L.gsv_get_forces(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+            raise RuntimeError("force download failed")
+        return out[:nforces]
 
     def pfarfield(self):
         """periodic far-field bins [nrs][3][ndata] (Ex, Ey, Ez, not negated)"""

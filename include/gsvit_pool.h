@@ -101,6 +101,7 @@ typedef struct {
 typedef struct {
     double **outpointdata;
     double **outsumdata;
+    double **outforcedata;  /* OUT_FORCE: [force][step * 3 + component], src3d/output.c:121-126 */
     SvDCube **outabsdata;   /* OUT_VOLUME ABS / SUMALL / MAXALL accumulators, one per such output */
     int nabs;
 } SvOutput;

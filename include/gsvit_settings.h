@@ -260,11 +260,21 @@ typedef struct {
     char filebase[SV_NAME_LEN];
 } SvOutputRegion;
 
+/* OUT_FORCE, src3d/settings.c:5018-5040: Maxwell stress tensor integrated
+ * over the six faces of a box every step, log written every `step` steps */
+typedef struct {
+    int step;
+    int box_i0, box_j0, box_k0, box_in, box_jn, box_kn;
+    char filename[SV_NAME_LEN];
+} SvOutputForce;
+
 typedef struct {
     int npnts;
     SvOutputPar *pnts;
     int nsums;
     SvOutputSum *sums;
+    int nforces;
+    SvOutputForce *forces;
     int nplns;
     SvOutputRegion *plns;
     int ncubs;

@@ -34,6 +34,7 @@
  *                             DFT of recorded point traces in the tests)
  *   point traces ............ src3d/output.c:340-352
  *   box sums ................ src3d/output.c:259-306
+ *   forces (stress tensor) .. src3d/output.c:355-512
  *   flux spectra (FLUX_DFT, no reference counterpart): running DFT of Ex, Ey
  *                             and the mean of Hx, Hy over planes k-1 and k on
  *                             each flux plane (pinned by a numpy DFT of point
@@ -1650,6 +1651,71 @@ pfarfield(Oracle *o)
     }
 }
 
+/* Maxwell stress tensor on the faces of each OUT_FORCE box, src3d/output.c:
+ * 355-381 (T = eps E E + mu H H - (eps E^2 + mu H^2) / 2 with the local
+ * absolute eps and mu) and :384-512 (T n summed over the face cells with
+ * the normals +x at i0, -x at in, +y at j0, -y at jn, +z at k0, -z at kn) */
+static void
+force_cell(const Oracle *o, int i, int j, int k, int nx, int ny, int nz, double *acc)
+{
+    const size_t c = IDX(o, i, j, k);
+    const double ex = o->ex[c], ey = o->ey[c], ez = o->ez[c], hx = o->hx[c], hy = o->hy[c], hz = o->hz[c];
+    double eps, mu, s, t[3][3];
+    sv_pool_epsilon_sigma(o->mp, i, j, k, &eps, &s);
+    sv_pool_mu_sigast(o->mp, i, j, k, &mu, &s);
+    {
+        const double val = (eps * (ex * ex + ey * ey + ez * ez) + mu * (hx * hx + hy * hy + hz * hz)) / 2.0;
+        t[0][0] = eps * ex * ex + mu * hx * hx - val;
+        t[1][1] = eps * ey * ey + mu * hy * hy - val;
+        t[2][2] = eps * ez * ez + mu * hz * hz - val;
+        t[0][1] = t[1][0] = eps * ex * ey + mu * hx * hy;
+        t[0][2] = t[2][0] = eps * ex * ez + mu * hx * hz;
+        t[1][2] = t[2][1] = eps * ey * ez + mu * hy * hz;
+    }
+    acc[0] += t[0][0] * nx + t[0][1] * ny + t[0][2] * nz;
+    acc[1] += t[1][0] * nx + t[1][1] * ny + t[1][2] * nz;
+    acc[2] += t[2][0] * nx + t[2][1] * ny + t[2][2] * nz;
+}
+
+static void
+getforce(Oracle *o)
+{
+    const SvSet *set = &o->set;
+    int n, i, j, k;
+    for (n = 0; n < set->so.nforces; n++) {
+        const SvOutputForce *f = set->so.forces + n;
+        double acc[3] = { 0, 0, 0 };
+        for (j = f->box_j0; j < f->box_jn; j++)
+            for (k = f->box_k0; k < f->box_kn; k++) {
+                force_cell(o, f->box_i0, j, k, 1, 0, 0, acc);
+                force_cell(o, f->box_in, j, k, -1, 0, 0, acc);
+            }
+        for (i = f->box_i0; i < f->box_in; i++)
+            for (k = f->box_k0; k < f->box_kn; k++) {
+                force_cell(o, i, f->box_j0, k, 0, 1, 0, acc);
+                force_cell(o, i, f->box_jn, k, 0, -1, 0, acc);
+            }
+        for (i = f->box_i0; i < f->box_in; i++)
+            for (j = f->box_j0; j < f->box_jn; j++) {
+                force_cell(o, i, j, f->box_k0, 0, 0, 1, acc);
+                force_cell(o, i, j, f->box_kn, 0, 0, -1, acc);
+            }
+        for (i = 0; i < 3; i++)
+            o->mp->out->outforcedata[n][set->sc.step_act * 3 + i] = acc[i];
+    }
+}
+
+/* OUT_FORCE values as [nforces][nsteps + 1][3] (before the dx dy of the log) */
+void
+oracle_get_forces(void *handle, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    int n;
+    for (n = 0; n < o->set.so.nforces; n++)
+        memcpy(out + (size_t)n * 3 * (o->set.sc.nsteps + 1), o->mp->out->outforcedata[n],
+               3 * (size_t)(o->set.sc.nsteps + 1) * sizeof(double));
+}
+
 /* box sums of E^2, src3d/output.c:259-306: voxels whose permittivity
  * matches layered_epsilon within EPSILON_0/100 (or every voxel for -1) */
 static void
@@ -1831,6 +1897,7 @@ oracle_step(void *handle, int nsteps)
         flux_dft(o);
         objective_dft(o);
         getpoints(o);
+        getforce(o);
         o->set.sc.step_act++;
     }
     return o->set.sc.step_act;

@@ -32,6 +32,7 @@ def lib(precision="f64"):
         L.oracle_get_points.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_farfield.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_pfarfield_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_get_forces.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_ltsf_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.oracle_get_ltsf.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_get_pfarfield.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
@@ -114,6 +115,14 @@ class Oracle:
         if inf["nrs"]:
             self.L.oracle_get_farfield(self.h, _dptr(out))
         return out[: inf["nrs"]]
+
+    def forces(self, nforces):
+        """OUT_FORCE values [nforces][nsteps + 1][3] (without the log's dx dy)"""
+        inf = self.info()
+        out = np.zeros((max(nforces, 1), inf["nsteps"] + 1, 3), dtype=np.float64)
+        if nforces:
+            self.L.oracle_get_forces(self.h, _dptr(out))
+        return out[:nforces]
 
     def ltsf_history(self):
         """layered-TSF 1-D model history [6: ee eh ez he hh hz][substep][z] (after the first step)"""
