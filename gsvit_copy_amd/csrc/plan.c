@@ -1172,6 +1172,52 @@ bad:
     return 1;
 }
 
+/* MEDIUM_CROP, src3d/plan.c:5244-5285: voxels on or outside the crop box
+ * faces take the crop medium, as float per-voxel data without a material
+ * table, else as one tabulated material (the reference gives that one
+ * sigma = sigma* = 0 whatever the keyword says) */
+static void
+apply_crop(SvPool *mp, SvSet *set)
+{
+    SvYeeData *d = mp->d;
+    const int xres = set->sp.xres, yres = set->sp.yres, zres = set->sp.zres;
+    int i, j, k;
+    if (set->sm.crop) {
+        const SvSetMat *m = &set->sm;
+        SvMatProp cm;
+        int idx = -1;
+        if (mp->nmat > 0) {
+            memset(&cm, 0, sizeof(cm));
+            cm.type = SV_MAT_LINTAB;
+            cm.epsilon = m->crop_epsilon;
+            cm.mu = m->crop_mu;
+            idx = get_matindex(mp, &cm);
+            if (idx == -1) {
+                add_material(mp, &cm);
+                idx = cm.pos;
+            }
+        }
+        for (i = 0; i < xres; i++)
+            for (j = 0; j < yres; j++)
+                for (k = 0; k < zres; k++) {
+                    if (!(i <= m->crop_i0 || i >= m->crop_in || j <= m->crop_j0 || j >= m->crop_jn || k <= m->crop_k0
+                          || k >= m->crop_kn))
+                        continue;
+                    if (mp->nmat == 0) {
+                        if (set->plan.matmode == SV_MATMODE_FULL || set->plan.matmode == SV_MATMODE_ELECTRIC) {
+                            d->epsilon->data[i][j][k] = (float)(m->crop_epsilon * EPS0);
+                            d->sigma->data[i][j][k] = (float)m->crop_sigma;
+                        }
+                        if (set->plan.matmode == SV_MATMODE_FULL || set->plan.matmode == SV_MATMODE_MAGNETIC) {
+                            d->mu->data[i][j][k] = (float)(m->crop_mu * MU0);
+                            d->sigast->data[i][j][k] = (float)m->crop_sigast;
+                        }
+                    } else
+                        d->mat->data[i][j][k] = idx;
+                }
+    }
+}
+
 static int
 load_data(SvPool *mp, SvSet *set)
 {
@@ -1207,6 +1253,7 @@ load_data(SvPool *mp, SvSet *set)
             if (set->plan.matmode == SV_MATMODE_ELECTRIC) {
                 /* src3d/plan.c:4162-4165: returns before the vector file and the boundary map */
                 fclose(fr);
+                apply_crop(mp, set);
                 return 0;
             }
             for (i = 0; i < xres; i++)
@@ -1376,6 +1423,8 @@ vec_err:
         return 1;
     }
 after_vector:
+
+    apply_crop(mp, set);
 
     /* TSF skip map, src3d/plan.c:5297-5329 */
     if (mp->src->tsf && set->ss.tsf.box_boundary_skipdepth >= 0) {

@@ -351,6 +351,14 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
         } else if (!strcmp(key, "MATMODE_CHECK")) {
             if (get_int(fr, &set->sm.matmode_check, key))
                 goto fail;
+        } else if (!strcmp(key, "MEDIUM_CROP")) {
+            SvSetMat *m = &set->sm;
+            if (get_int(fr, &m->crop_i0, key) || get_int(fr, &m->crop_j0, key) || get_int(fr, &m->crop_k0, key)
+                || get_int(fr, &m->crop_in, key) || get_int(fr, &m->crop_jn, key) || get_int(fr, &m->crop_kn, key)
+                || get_double(fr, &m->crop_epsilon, key) || get_double(fr, &m->crop_mu, key)
+                || get_double(fr, &m->crop_sigma, key) || get_double(fr, &m->crop_sigast, key))
+                goto fail;
+            m->crop = 1;
         } else if (!strcmp(key, "MEDIUM_LINEAR")) {
             if (get_rest_of_line(fr, set->sm.in_voxel_filename, SV_NAME_LEN))
                 goto fail;

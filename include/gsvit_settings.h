@@ -307,6 +307,11 @@ typedef struct {
     char in_vector_filename[SV_NAME_LEN];
     char in_vtk_data_filename[SV_NAME_LEN];
     int matmode_check;
+    /* MEDIUM_CROP, src3d/settings.c:1898-1921: everything on or outside the
+     * box faces becomes the crop medium (relative eps, mu; sigma, sigma*) */
+    int crop;
+    int crop_i0, crop_j0, crop_k0, crop_in, crop_jn, crop_kn;
+    double crop_epsilon, crop_mu, crop_sigma, crop_sigast;
 } SvSetMat;
 
 typedef struct {

@@ -1,0 +1,100 @@
+"""MEDIUM_CROP (src3d/settings.c:1898-1921, src3d/plan.c:5244-5285): voxels
+on or outside the faces of the crop box take the crop medium.
+
+Pins: with per-voxel data (MEDIUM_LINEAR) a crop equals writing the crop
+medium into the voxel file on and outside the box faces, field for field
+after a run to 1e-8 of the field scale (the two round eps0 x eps_r to float
+differently; skipping the crop changes the fields by 1e-3); with tabulated materials the material map is unchanged
+strictly inside the box and one new tabulated material everywhere else, and
+that material's sigma is zero whatever the keyword gives (the reference adds
+it with sigma = sigma* = 0): runs with crop sigma 0 and 1e4 are identical.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from oracle_lib import Oracle
+
+CROP = (9, 8, 10, 38, 36, 39)
+
+
+def outside(shape, box=CROP):
+    i0, j0, k0, i1, j1, k1 = box
+    i, j, k = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    return (i <= i0) | (i >= i1) | (j <= j0) | (j >= j1) | (k <= k0) | (k >= k1)
+
+
+def run(par, steps=None):
+    o = Oracle(par)
+    o.step(steps or 10 ** 9)
+    f, m = o.fields(), o.maps()[0]
+    o.close()
+    return f, m
+
+
+def voxel_case(folder, crop, n=48, nsteps=120):
+    par = cases.small_case(folder, "vacuum_none", n=n, nsteps=nsteps)
+    rng = np.random.default_rng(5)
+    eps = rng.uniform(1.0, 3.0, (n, n, n)).astype(np.float32)
+    sig = rng.uniform(0.0, 1e3, (n, n, n)).astype(np.float32)
+    if crop == "explicit":
+        out = outside(eps.shape)
+        eps[out] = np.float32(2.25)
+        sig[out] = np.float32(300.0)
+    cases.write_linear_medium(os.path.join(folder, "lin.bin"), eps, sigma=sig)
+    with open(par, "a") as f:
+        f.write("MEDIUM_LINEAR\nlin.bin\n\n")
+        if crop == "keyword":
+            f.write("MEDIUM_CROP\n%d %d %d %d %d %d 2.25 1 300 0\n\n" % CROP)
+    return par
+
+
+def test_crop_equals_explicit_voxel_data(tmp_path):
+    a, _ = run(voxel_case(str(tmp_path / "kw"), "keyword"))
+    b, _ = run(voxel_case(str(tmp_path / "ex"), "explicit"))
+    c, _ = run(voxel_case(str(tmp_path / "no"), None))
+    # the voxel file's eps is scaled by eps0 in float, the crop's in double and
+    # then rounded (as the reference's (gfloat)(crop_epsilon * EPSILON_0)):
+    # one float ulp of material apart
+    scale = np.abs(b).max()
+    print("crop vs explicit", np.abs(a - b).max() / scale, "crop vs none", np.abs(a - c).max() / scale)
+    assert np.abs(a - b).max() <= 1e-8 * scale
+    assert np.abs(a - c).max() > 1e-4 * scale
+
+
+def tab_case(folder, sigma, crop=True):
+    par = cases.small_case(folder, "spheres_cpml", nsteps=120)
+    if crop:
+        with open(par, "a") as f:
+            f.write("MEDIUM_CROP\n%d %d %d %d %d %d 2.25 1 %g 0\n\n" % (CROP + (sigma,)))
+    return par
+
+
+def test_crop_of_tabulated_materials(tmp_path):
+    f0, m0 = run(tab_case(str(tmp_path / "a"), 0.0))
+    f1, m1 = run(tab_case(str(tmp_path / "b"), 1e4))
+    fn, mn = run(tab_case(str(tmp_path / "c"), 0.0, crop=False))
+    out = outside(m0.shape)
+    assert np.array_equal(m0[~out], mn[~out])
+    new = np.unique(m0[out])
+    assert len(new) == 1 and new[0] > mn.max()
+    assert np.array_equal(f0, f1)                    # the crop material's sigma is not used
+    assert not np.array_equal(f0, fn)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["voxel", "tabulated"])
+def test_gpu_crop_against_oracle(tmp_path, kind):
+    from gsvit_copy_amd import Session
+    from test_gpu_parity import field_errors
+    par = voxel_case(str(tmp_path / "v"), "keyword") if kind == "voxel" else tab_case(str(tmp_path / "t"), 50.0)
+    print("crop", kind, flush=True)
+    ref, _ = run(par)
+    s = Session(par)
+    s.run()
+    errs = field_errors(s.fields(), ref)
+    s.close()
+    print(kind, errs)
+    assert max(errs.values()) <= 1e-5
