@@ -1416,11 +1416,11 @@ fetch_aux(Engine *E, SvPool *mp, SvSet *set)
     for (size_t n = 0; n < E->forces.size(); n++)
         CK(hipMemcpyAsync(mp->out->outforcedata[n], E->forces[n].out, 3 * ((size_t)set->sc.nsteps + 1) * 8,
                           hipMemcpyDeviceToHost, E->st));
-    for (size_t n = 0; n < E->sums.size(); n++) {
+    for (size_t n = 0; n < E->sums.size(); n++)
         CK(hipMemcpyAsync(mp->out->outsumdata[n], E->sums[n].out, ((size_t)set->sc.nsteps + 1) * 8,
                           hipMemcpyDeviceToHost, E->st));
-        CK(hipStreamSynchronize(E->st));
-    }
+    /* the force and sum copies land in pageable host memory the caller reads next */
+    CK(hipStreamSynchronize(E->st));
     return 0;
 }
 

@@ -1175,7 +1175,10 @@ bad:
 /* MEDIUM_CROP, src3d/plan.c:5244-5285: voxels on or outside the crop box
  * faces take the crop medium, as float per-voxel data without a material
  * table, else as one tabulated material (the reference gives that one
- * sigma = sigma* = 0 whatever the keyword says) */
+ * sigma = sigma* = 0 whatever the keyword says).  The tabulated material is
+ * looked up with its relative constants, as the reference does, but stored
+ * with absolute ones: the table was converted to absolute units before the
+ * crop runs (sv_pool_add_material's multme, src3d/plan.c:2439-2442). */
 static void
 apply_crop(SvPool *mp, SvSet *set)
 {
@@ -1193,6 +1196,8 @@ apply_crop(SvPool *mp, SvSet *set)
             cm.mu = m->crop_mu;
             idx = get_matindex(mp, &cm);
             if (idx == -1) {
+                cm.epsilon *= EPS0;
+                cm.mu *= MU0;
                 add_material(mp, &cm);
                 idx = cm.pos;
             }
@@ -1251,9 +1256,9 @@ load_data(SvPool *mp, SvSet *set)
                         if (fread(d->sigma->data[i][j], sizeof(float), plane, fr) != plane) goto short_read;
             }
             if (set->plan.matmode == SV_MATMODE_ELECTRIC) {
-                /* src3d/plan.c:4162-4165: returns before the vector file and the boundary map */
+                /* src3d/plan.c:4162-4165: returns before the vector file, the crop
+                 * (src3d/plan.c:5244, same function) and the boundary map */
                 fclose(fr);
-                apply_crop(mp, set);
                 return 0;
             }
             for (i = 0; i < xres; i++)

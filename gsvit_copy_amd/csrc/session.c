@@ -164,17 +164,19 @@ gsv_get_farfield(void *h, double *out)
     return 0;
 }
 
-int
+long
 gsv_get_forces(void *h, double *out)
 {
     GsvSession *s = (GsvSession *)h;
+    const long per = 3L * (s->set.sc.nsteps + 1);
     int n;
+    if (!out)
+        return per * s->set.so.nforces;
     if (gsv_engine_fetch_aux(s->mp->gpu))
-        return 1;
+        return -1;
     for (n = 0; n < s->set.so.nforces; n++)
-        memcpy(out + (size_t)n * 3 * (s->set.sc.nsteps + 1), s->mp->out->outforcedata[n],
-               3 * (size_t)(s->set.sc.nsteps + 1) * sizeof(double));
-    return 0;
+        memcpy(out + (size_t)n * per, s->mp->out->outforcedata[n], (size_t)per * sizeof(double));
+    return per * s->set.so.nforces;
 }
 
 void

@@ -37,7 +37,7 @@ def library():
         L.gsv_get_points.argtypes = [vp, dp]
         L.gsv_get_farfield.restype = ctypes.c_int
         L.gsv_get_farfield.argtypes = [vp, dp]
-        L.gsv_get_forces.restype = ctypes.c_int
+        L.gsv_get_forces.restype = ctypes.c_long
         L.gsv_get_forces.argtypes = [vp, dp]
         L.gsv_pfarfield_dims.restype = None
         L.gsv_pfarfield_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
@@ -163,13 +163,15 @@ class Session:
             raise RuntimeError("far-field download failed")
         return out[: inf["nrs"]]
 
-    def forces(self, nforces):
-        """OUT_FORCE values [nforces][nsteps + 1][3] (without the log's dx dy)"""
+    def forces(self):
+        """OUT_FORCE values [nforces][nsteps + 1][3] (without the log's dx dy),
+        one row per force box of the parameter file"""
         inf = self.info()
-        out = np.zeros((max(nforces, 1), inf["nsteps"] + 1, 3), dtype=np.float64)
-        if nforces and self.L.gsv_get_forces(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+        n = self.L.gsv_get_forces(self.h, None)
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        if n > 0 and self.L.gsv_get_forces(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) < 0:
             raise RuntimeError("force download failed")
-        return out[:nforces]
+        return out[:n].reshape(-1, inf["nsteps"] + 1, 3)
 
     def pfarfield(self):
         """periodic far-field bins [nrs][3][ndata] (Ex, Ey, Ez, not negated)"""

@@ -35,8 +35,9 @@ int gsv_get_farfield(void *session, double *out);
 /* periodic far field (PERIODIC_NFFF, src3d/pfarfield.c): dims[0..1] = number of
  * points, bins per component; bins as [nrs][3][ndata] (Ex, Ey, Ez; the log file
  * holds them negated, src3d/output.c:1347-1352) */
-/* OUT_FORCE values as [nforces][nsteps+1][3] (before the log's dx dy factor) */
-int gsv_get_forces(void *session, double *out);
+/* OUT_FORCE values as [nforces][nsteps+1][3] (before the log's dx dy factor);
+ * returns the element count (out may be NULL, to size the buffer), -1 on error */
+long gsv_get_forces(void *session, double *out);
 void gsv_pfarfield_dims(void *session, int *dims);
 int gsv_get_pfarfield(void *session, double *out);
 /* running-DFT accumulators; returns the element count (out may be NULL) */

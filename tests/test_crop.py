@@ -8,6 +8,13 @@ differently; skipping the crop changes the fields by 1e-3); with tabulated mater
 strictly inside the box and one new tabulated material everywhere else, and
 that material's sigma is zero whatever the keyword gives (the reference adds
 it with sigma = sigma* = 0): runs with crop sigma 0 and 1e4 are identical.
+The tabulated crop material is stored in absolute units like every other
+table row (the reference's multme, src3d/plan.c:2439-2442): a vacuum crop
+(eps_r 1, mu_r 1) around a case is transparent, its fields equal the
+uncropped run's to float round-off at the crop faces, shell included.  With MATMODE_CHECK and a
+voxel file holding electric data only, the reference returns from its
+loader before the crop (src3d/plan.c:4162-4165), so the keyword changes
+nothing there.
 """
 import os
 
@@ -64,12 +71,55 @@ def test_crop_equals_explicit_voxel_data(tmp_path):
     assert np.abs(a - c).max() > 1e-4 * scale
 
 
-def tab_case(folder, sigma, crop=True):
+def tab_case(folder, sigma, crop=True, eps_r=2.25):
     par = cases.small_case(folder, "spheres_cpml", nsteps=120)
     if crop:
         with open(par, "a") as f:
-            f.write("MEDIUM_CROP\n%d %d %d %d %d %d 2.25 1 %g 0\n\n" % (CROP + (sigma,)))
+            f.write("MEDIUM_CROP\n%d %d %d %d %d %d %g 1 %g 0\n\n" % (CROP + (eps_r, sigma)))
     return par
+
+
+def shell_vacuum_case(folder, crop):
+    """a dielectric sphere strictly inside the crop box, vacuum in the shell, CPML"""
+    par = cases.small_case(folder, "vacuum_none", nsteps=120)
+    with open(os.path.join(folder, "sphere.txt"), "w") as f:
+        f.write("4 30 26 22 5 1 3 1 0 0\n")
+    with open(par, "a") as f:
+        f.write("BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\nMEDIUM_VECTOR\nsphere.txt\n\n")
+        if crop:
+            f.write("MEDIUM_CROP\n%d %d %d %d %d %d 1 1 0 0\n\n" % CROP)
+    return par
+
+
+def test_vacuum_crop_of_tabulated_materials_is_transparent(tmp_path):
+    fv, mv = run(shell_vacuum_case(str(tmp_path / "v"), True))
+    fn, mn = run(shell_vacuum_case(str(tmp_path / "n"), False))
+    out = outside(mv.shape)
+    assert len(np.unique(mv[out])) == 1 and mv[out][0] > mn.max()   # a new material, not vacuum's index
+    scale = np.abs(fn).max()
+    shell = np.abs(fv[:, out]).max() / np.abs(fn[:, out]).max()
+    print("vacuum crop vs none", np.abs(fv - fn).max() / scale, "shell amplitude ratio", shell)
+    # the crop faces become internal boundaries (a new material index), where
+    # the coefficients come from permittivities averaged in float: 1e-10 of the
+    # scale.  With relative constants in the table the shell froze (ratio ~0).
+    assert np.abs(fv - fn).max() <= 1e-8 * scale
+    assert 0.99 < shell < 1.01
+
+
+def test_crop_skipped_on_electric_voxel_file(tmp_path):
+    def case(folder, crop):
+        n = 40
+        par = cases.small_case(folder, "vacuum_none", n=n, nsteps=80)
+        eps = np.random.default_rng(3).uniform(1.0, 3.0, (n, n, n)).astype(np.float32)
+        cases.write_linear_medium(os.path.join(folder, "lin.bin"), eps)
+        with open(par, "a") as f:
+            f.write("MATMODE_CHECK\n1\n\nMEDIUM_LINEAR\nlin.bin\n\n")
+            if crop:
+                f.write("MEDIUM_CROP\n9 8 10 30 28 31 2.25 1 300 0\n\n")
+        return par
+    a, _ = run(case(str(tmp_path / "kw"), True))
+    b, _ = run(case(str(tmp_path / "no"), False))
+    assert np.array_equal(a, b)
 
 
 def test_crop_of_tabulated_materials(tmp_path):

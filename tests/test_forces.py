@@ -87,7 +87,8 @@ def test_gpu_forces_and_log_against_oracle(tmp_path):
     o.close()
     s = Session(par, write_outputs=True)
     s.run()
-    got = s.forces(2)
+    got = s.forces()
+    assert got.shape[0] == 2
     s.close()
     for b in range(2):
         for c in range(3):
