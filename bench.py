@@ -156,16 +156,9 @@ def batch_bench(args, work):
 
 
 def kernel_table(kt, h_bytes, e_bytes, traffic=None):
-    """Per-kernel times (HIP events) against algorithmic bytes.  The fused step
-    (one kernel per time step, its time reported as E) is charged with the
-    whole step's bytes; the two-pass step with the H and E passes' bytes."""
+    """Per-kernel times (HIP events) against the H and E passes' algorithmic bytes."""
     traffic = traffic or {}
     frac = lambda b, ms: round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None
-    if kt["h_n"] == 0 and kt["e_n"] > 0:
-        ms = kt["e_ms"] / kt["e_n"]
-        b = h_bytes + e_bytes
-        return {"k_step": {"ms": round(ms, 4), "algorithmic_bytes": b, "frac": frac(b, ms),
-                           "traffic": traffic.get("k_step")}}, "k_step", ms, b
     h_ms = kt["h_ms"] / max(kt["h_n"], 1)
     e_ms = kt["e_ms"] / max(kt["e_n"], 1)
     tab = {"k_hstep": {"ms": round(h_ms, 4), "algorithmic_bytes": h_bytes, "frac": frac(h_bytes, h_ms),
@@ -223,16 +216,11 @@ def film_bench(args, work):
     cpml_cells = 2.0 * depth * nx * ny
     h_bytes = 37.0 * cells + 16.0 * cpml_cells
     e_bytes = 37.0 * cells + 16.0 * cpml_cells + 216.0 * n_si + 168.0 * n_au
-    if args.two_pass:
-        s.set_step_mode(False)
     if args.film_tune:
         shape, used = s.autotune(2)
     else:
         shape, used = [int(v) for v in args.film_launch.split(",")], 0
         s.set_launch(shape)
-    mode = s.step_mode()
-    if mode["fused"]:
-        shape = [mode["rows"], mode["kchunk"]]
     s.run(20)
     s.sync()
     S = args.film_steps
@@ -253,14 +241,13 @@ def film_bench(args, work):
             traffic = {k: round(v["traffic_bytes"]) for k, v in tr["kernels"].items()}
     except (OSError, ValueError, KeyError):
         pass
-    log("bench: textured film %.3f ms/step, %.2f Gcell-updates/s (%s step); kernels %s"
-        % (1e3 * el / S, gc, "fused" if mode["fused"] else "two-pass",
-           {k: v["ms"] for k, v in kernel_table(kt, h_bytes, e_bytes)[0].items()}))
+    log("bench: textured film %.3f ms/step, %.2f Gcell-updates/s; kernels %s"
+        % (1e3 * el / S, gc, {k: v["ms"] for k, v in kernel_table(kt, h_bytes, e_bytes)[0].items()}))
     out = {"workload": "dispersive textured thin film 768x768x384, 4 nm, periodic x/y, CPML 16 z, SiO2 / 3-pole Si "
                        "with pyramids / Drude+2CP Au, TSF normal 400-1100 nm, flux DFT 3 planes x 32 wavelengths",
            "cells": int(cells), "si_ade_cells": n_si, "au_ade_cells": n_au, "ade_poles": info["ade_poles"],
            "timed_steps": S, "ms_per_step": round(1e3 * el / S, 4), "gcells": round(gc, 3),
-           "open_s": round(t_open, 2), "autotune_steps": used, "launch": shape, "fused_step": mode["fused"],
+           "open_s": round(t_open, 2), "autotune_steps": used, "launch": shape,
            "kernels": kernel_table(kt, h_bytes, e_bytes, traffic)[0],
            "traffic_source": "profiles/hbm_traffic_film768.md" if traffic else None,
            "metric": "Gcell-updates/s", "higher_is_better": True}
@@ -361,9 +348,6 @@ def main():
     ap.add_argument("--film-tune", action="store_true", help="autotune the textured film's launch shape instead")
     ap.add_argument("--film-full", action="store_true",
                     help="also run the textured thin film to its 6000 steps and report its spectra")
-    ap.add_argument("--two-pass", action="store_true", help="run the two-pass step (H kernel, then E kernel) "
-                    "instead of the fused step")
-    ap.add_argument("--step-shape", default=None, help="fused step shape rows,zchunk; implies no tuning")
     ap.add_argument("--launch", default=None,
                     help="fixed launch shape hx,hy,hz,ex,ey,ez (tile x width, rows, z-march depth of the H and E "
                          "kernels); implies no tuning")
@@ -396,19 +380,13 @@ def main():
     n_ade = int(dilate(mat == 1).sum())
     del mat, bnds
 
-    if args.two_pass:
-        s.set_step_mode(False)
-    elif args.step_shape:
-        r, kc = [int(v) for v in args.step_shape.split(",")]
-        s.set_step_mode(True, r, kc)
     shape = s.get_launch()
     if args.launch:
         shape = [int(v) for v in args.launch.split(",")]
         s.set_launch(shape)
-    elif args.tune_steps > 0 and not args.step_shape:
+    elif args.tune_steps > 0:
         shape, used = s.autotune(args.tune_steps)
-        log("bench: autotuned in %d steps: launch shape %s, step mode %s" % (used, shape, s.step_mode()))
-    mode = s.step_mode()
+        log("bench: autotuned in %d steps: launch shape %s" % (used, shape))
     log("bench: warmup %d steps" % args.warmup)
     s.run(args.warmup)
     s.sync()
@@ -432,7 +410,7 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     step_bytes = h_bytes + e_bytes
     log("bench: kernels %s" % ({k: (v["ms"], v["frac"]) for k, v in ktab.items()},))
-    kshape = [mode["rows"], mode["kchunk"]] if mode["fused"] else list(shape)
+    kshape = list(shape)
 
     # HBM traffic of the dominant kernel from the committed counter summary
     # (rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by the copy-kernel
@@ -502,7 +480,6 @@ def main():
                                "running-DFT NFFF at 3 wavelengths",
                    "grid": [n, n, n], "cell_nm": 5, "ade_cells": n_ade,
                    "e_classes": info["e_classes"], "h_classes": info["h_classes"],
-                   "step": "fused" if mode["fused"] else "two-pass",
                    "launch_shape": kshape},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

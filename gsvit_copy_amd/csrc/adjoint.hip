@@ -598,7 +598,6 @@ gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_int
         return 1;
     if (E->adj)
         return 0;
-    E->fused = 0;   /* the gradient runs the two-pass step (a design pool is never fused) */
     if (!set->sa.design || !set->sa.objective) {
         fprintf(stderr, "Error: the gradient needs DESIGN_BOX and OBJECTIVE_DFT in the parameter file\n");
         return 1;

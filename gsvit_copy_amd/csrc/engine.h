@@ -123,6 +123,9 @@ typedef struct {
     float *ade_p;           /* critical-point state blocks, GSV_ADE_P(ade_poles) states (NULL without poles) */
     long long ade_plane_d;  /* floats per z plane of ade_d = ny * pxa * GSV_ADE_D */
     long long ade_plane_p;  /* floats per z plane of ade_p */
+    int ade_cpml;           /* ADE_CPML_CONSISTENT: the ADE branch updates and applies the psi of its own
+                               components once per step (curl coefficient c0, all six faces) and the
+                               slab code leaves them alone; 0: the reference's double update */
 } GsvGrid;
 
 #endif

@@ -83,16 +83,7 @@ struct Engine {
     /* box sums */
     std::vector<GsvBoxSum> sums;
     std::vector<GsvForce> forces;
-    /* fused step (one pass over the fields, k_step): on when `fused`; the
-     * buffers the next step writes (swapped with the grid's after each step),
-     * the step's TSF H corrections, the launch shape */
-    int fused;
-    int fused_ok;             /* the pool has nothing the fused step leaves out */
-    int fused_alloc;          /* alt buffers allocated */
-    GsvStepOut alt;
-    GsvTsfDelta tsfd;
-    int step_rows, step_kchunk;
-    /* timing of the two volume kernels (the fused step counts as E) */
+    /* timing of the two volume kernels */
     int timing;
     hipEvent_t ev[4];
     double t_h_ms, t_e_ms;

@@ -392,6 +392,7 @@ sv_gpu_new(SvPool *mp, SvSet *set)
     E->g.plane = (long long)E->g.px * E->g.ny;
     E->g.perx = set->sb.bx0 == SV_BOUNDARY_PERIODIC;
     E->g.pery = set->sb.by0 == SV_BOUNDARY_PERIODIC;
+    E->g.ade_cpml = set->sb.ade_cpml_consistent != 0;
     E->g.dx = (float)set->sp.dx;
     E->g.dy = (float)set->sp.dy;
     E->g.dz = (float)set->sp.dz;
@@ -533,8 +534,6 @@ setup_cpml(Engine *E)
     return 0;
 }
 
-extern "C" int gsv_engine_set_step_mode(void *eng, int mode, int rows, int kchunk);
-static void swap_step_buffers(Engine *E);
 
 static int
 setup_tsf(Engine *E)
@@ -1284,23 +1283,6 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
     /* the host copy of the class ids is not needed any more */
     std::vector<uint8_t>().swap(E->ecls_host);
     std::vector<uint8_t>().swap(E->hcls_host);
-    /* the fused step covers pools without Liao faces, interior periodic planes,
-     * H point sources, a design region or the gradient state, rows of at most
-     * 1024 cells, 32-bit offsets and one class-id width */
-    {
-        int hsrc = 0;
-        for (size_t q = 0; q < E->src_vals_host.size(); q++)
-            if (q % 6 >= 3 && E->src_vals_host[q] != 0.f)
-                hsrc = 1;
-        E->fused_ok = !E->has_liao && !E->has_per && !hsrc && !E->has_ltsf && g.pc_nx == 0 && g.nx <= 1024
-                      && (unsigned long long)g.plane * (g.nz + 2) * 4ull < (1ull << 32)
-                      && E->idbytes_e == E->idbytes_h;
-        E->step_rows = 4;
-        E->step_kchunk = 16;
-        /* the two-pass step is the default: on MI355X the fused step moves
-         * fewer bytes but issues about twice the vector instructions per cell
-         * (per-cell addressing and boundary tests, halo rows) and runs slower */
-    }
     E->ready = 1;
     return 0;
 }
@@ -1468,26 +1450,6 @@ extern "C" int
 sv_pool_gpu_ystep_h(SvPool *mp, SvSet *set)
 {
     ENGINE_OR_FAIL(mp);
-    if (E->fused) {
-        /* the whole update in one pass: the auxiliary line first (its E feeds
-         * the H faces), then H, TSF H faces and E of every cell */
-        if (E->has_tsf)
-            CK(gsv_launch_tsf_line(&E->tsf, set->sc.step_act, E->st));
-        if (E->timing)
-            CK(hipEventRecord(E->ev[2], E->st));
-        CK(gsv_launch_step(&E->g, &E->alt, &E->tsf, &E->tsfd, E->has_tsf, E->idbytes_e, E->step_rows,
-                           E->step_kchunk, E->st));
-        if (E->timing) {
-            float ms;
-            CK(hipEventRecord(E->ev[3], E->st));
-            CK(hipEventSynchronize(E->ev[3]));
-            CK(hipEventElapsedTime(&ms, E->ev[2], E->ev[3]));
-            E->t_e_ms += ms;
-            E->n_e++;
-        }
-        swap_step_buffers(E);
-        return 0;
-    }
     (void)set;
     if (E->timing)
         CK(hipEventRecord(E->ev[0], E->st));
@@ -1507,8 +1469,6 @@ extern "C" int
 sv_pool_gpu_apply_source_hstep(SvPool *mp, SvSet *set)
 {
     ENGINE_OR_FAIL(mp);
-    if (E->fused)
-        return 0;   /* done inside the step (a fused pool has no H point sources) */
     if (E->has_src)
         CK(gsv_launch_point_src(&E->g, &E->src, set->sc.step_act, 1, E->st));
     if (E->has_tsf) {
@@ -1534,8 +1494,6 @@ extern "C" int
 sv_pool_gpu_ystep_e(SvPool *mp, SvSet *set)
 {
     ENGINE_OR_FAIL(mp);
-    if (E->fused)
-        return 0;   /* updated by the fused step in ystep_h */
     if (E->adj && gsv_adjoint_pre_e(E, set->sc.step_act))
         return 1;
     if (E->timing)
@@ -1640,97 +1598,6 @@ sv_pool_gpu_getforce(SvPool *mp, SvSet *set)
     for (size_t n = 0; n < E->forces.size(); n++)
         CK(gsv_launch_force(&E->g, &E->forces[n], set->sc.step_act, E->st));
     return 0;
-}
-
-/* ------------------------------------------------------------------ */
-/* fused step: buffers and mode                                         */
-/* ------------------------------------------------------------------ */
-
-/* the grid's field and H-psi pointers trade places with the ones the step
- * just wrote */
-static void
-swap_step_buffers(Engine *E)
-{
-    GsvGrid &g = E->g;
-    GsvStepOut &a = E->alt;
-    std::swap(g.exg, a.exg);
-    std::swap(g.eyg, a.eyg);
-    std::swap(g.ezg, a.ezg);
-    std::swap(g.hxg, a.hxg);
-    std::swap(g.hyg, a.hyg);
-    std::swap(g.hzg, a.hzg);
-    g.ex = g.exg + g.plane; g.ey = g.eyg + g.plane; g.ez = g.ezg + g.plane;
-    g.hx = g.hxg + g.plane; g.hy = g.hyg + g.plane; g.hz = g.hzg + g.plane;
-    E->fbase[0] = g.exg; E->fbase[1] = g.eyg; E->fbase[2] = g.ezg;
-    E->fbase[3] = g.hxg; E->fbase[4] = g.hyg; E->fbase[5] = g.hzg;
-    for (int q = 0; q < 6; q++)
-        if (g.f[q].on)
-            std::swap(g.f[q].ph, a.ph[q]);
-}
-
-/* mode 1: fused step with `rows` rows (1..16, LDS permitting) and `kchunk` planes per block;
- * mode 0: the two-pass step.  The second set of buffers is allocated (and
- * zeroed: guard planes and row padding are read, never written) on first use. */
-extern "C" int
-gsv_engine_set_step_mode(void *eng, int mode, int rows, int kchunk)
-{
-    Engine *E = (Engine *)eng;
-    GsvGrid &g = E->g;
-    if (mode == 0) {
-        E->fused = 0;
-        return 0;
-    }
-    if (!E->fused_ok || E->adj || rows < 1 || rows > 16 || kchunk < 1
-        || (size_t)((rows + 1) * 3 + rows * 2) * ((g.nx + 63) / 64 * 64) * 4 > 160 * 1024)
-        return 1;
-    if (!E->fused_alloc) {
-        float **fp[6] = { &E->alt.exg, &E->alt.eyg, &E->alt.ezg, &E->alt.hxg, &E->alt.hyg, &E->alt.hzg };
-        CK(hipSetDevice(E->dev));
-        for (int q = 0; q < 6; q++) {
-            if (dmalloc(E, fp[q], E->field_floats))
-                return 1;
-            CK(hipMemsetAsync(*fp[q], 0, E->field_floats * 4, E->st));
-        }
-        for (int q = 0; q < 6; q++) {
-            const GsvCpmlFace &F = g.f[q];
-            E->alt.ph[q] = NULL;
-            if (!F.on)
-                continue;
-            const int axis = q / 2;
-            size_t cnt;
-            if (axis == 0) cnt = (size_t)F.depth * g.ny * g.nz;
-            else if (axis == 1) cnt = (size_t)g.px * F.depth * g.nz;
-            else cnt = (size_t)g.px * g.ny * F.depth;
-            if (dmalloc(E, &E->alt.ph[q], cnt))
-                return 1;
-            CK(hipMemsetAsync(E->alt.ph[q], 0, cnt * 8, E->st));
-        }
-        for (int f = 0; f < 6; f++) {
-            E->tsfd.d[f] = NULL;
-            if (!E->has_tsf)
-                continue;
-            int na, nb;
-            tsf_face_extent(E->tsf, g.nx, g.ny, f / 2, &na, &nb);
-            if (dmalloc(E, &E->tsfd.d[f], (size_t)na * nb))
-                return 1;
-        }
-        E->fused_alloc = 1;
-    }
-    E->fused = 1;
-    E->step_rows = rows;
-    E->step_kchunk = kchunk;
-    return 0;
-}
-
-/* out[0..2] = mode (1 fused), rows, z chunk; out[3] = fused step available */
-extern "C" void
-gsv_engine_get_step_mode(void *eng, int *out)
-{
-    Engine *E = (Engine *)eng;
-    out[0] = E->fused;
-    out[1] = E->step_rows;
-    out[2] = E->step_kchunk;
-    out[3] = E->fused_ok && !E->adj;
 }
 
 /* ------------------------------------------------------------------ */

@@ -167,30 +167,9 @@ typedef struct {
     uint16_t run, face, ta, tb;
 } GsvFaceTile;
 
-/* the buffers a fused step writes (guard-plane bases, H-side psi pairs per
- * CPML face); the step reads the previous step's values from the grid's own
- * pointers, and the host swaps the two sets afterwards */
-typedef struct {
-    float *exg, *eyg, *ezg, *hxg, *hyg, *hzg;
-    float2 *ph[6];
-} GsvStepOut;
-
-/* a fused step's TSF H corrections: per face (x0, xn, y0, yn, z0, zn) and
- * face cell, the two deltas its face adds (x faces Hy, Hz; y faces Hx, Hz;
- * z faces Hx, Hy) */
-typedef struct {
-    float2 *d[6];
-} GsvTsfDelta;
-
 #ifdef __cplusplus
 extern "C" {
 #endif
-/* one fused time step (H update, TSF H faces when tsf != 0, E update), a block
- * per `rows` rows and `kchunk` planes; needs nx <= 1024, 32-bit offsets and
- * (5 rows + 3) x roundup(nx, 64) floats of LDS; with tsf it first fills dl
- * from the stepped auxiliary line */
-hipError_t gsv_launch_step(const GsvGrid *g, const GsvStepOut *w, const GsvTsf *t, const GsvTsfDelta *dl, int tsf,
-                           int idbytes, int rows, int kchunk, hipStream_t st);
 hipError_t gsv_launch_hstep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st);
 hipError_t gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st);
 hipError_t gsv_launch_tsf_line(const GsvTsf *t, int step, hipStream_t st);

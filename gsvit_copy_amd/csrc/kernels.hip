@@ -241,8 +241,89 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     if (xi && yi && !tz)
         ez = m.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz + cz + m.c2 * ezpv + m.c3 * ez;
 
-    /* the branch's own CPML update; its x0 correction multiplies by a zero cb */
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
+    if (g.ade_cpml) {
+        /* ADE_CPML_CONSISTENT: the psi of each dispersive component, once per
+         * step over the slab ranges of the ordinary update, applied with c0 */
+        const float c0 = m.c0;
+        if (F0.on && i >= 1 && i < F0.depth) {
+            const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
+            const float4 bc = ldg4(F0.ce + i);
+            const float2 ps = ld2(F0.pe, p);
+            const float a1 = ty ? ps.x : bc.x * ps.x + bc.y * (hz - hzi);
+            const float a2 = tz ? ps.y : bc.x * ps.y + bc.y * (hy - hyi);
+            st2(F0.pe, p, a1, a2);
+            if (yi && k != 0) {
+                if (!ty) ey -= c0 * a1;
+                if (!tz) ez += c0 * a2;
+            }
+        }
+        if (F1.on && i >= nx - F1.depth + 1) {
+            const int ir = i - (nx - F1.depth);
+            const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8;
+            const float4 bc = ldg4(F1.ce + ir);
+            const float2 ps = ld2(F1.pe, p);
+            const float a1 = ty ? ps.x : bc.x * ps.x + bc.y * (hz - hzi);
+            const float a2 = tz ? ps.y : bc.x * ps.y + bc.y * (hy - hyi);
+            st2(F1.pe, p, a1, a2);
+            if (yi && k != 0) {
+                if (!ty) ey -= c0 * a1;
+                if (!tz) ez += c0 * a2;
+            }
+        }
+        if (F2.on && j >= 1 && j < F2.depth) {
+            const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
+            const float4 bc = ldg4(F2.ce + j);
+            const float2 ps = ld2(F2.pe, p);
+            const float a1 = tx ? ps.x : bc.x * ps.x + bc.y * (hz - hzj);
+            const float a2 = tz ? ps.y : bc.x * ps.y + bc.y * (hx - hxj);
+            st2(F2.pe, p, a1, a2);
+            if (xi && k != 0) {
+                if (!tx) ex += c0 * a1;
+                if (!tz) ez -= c0 * a2;
+            }
+        }
+        if (F3.on && j >= ny - F3.depth + 1) {
+            const int jr = j - (ny - F3.depth);
+            const OFF p = (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8;
+            const float4 bc = ldg4(F3.ce + jr);
+            const float2 ps = ld2(F3.pe, p);
+            const float a1 = tx ? ps.x : bc.x * ps.x + bc.y * (hz - hzj);
+            const float a2 = tz ? ps.y : bc.x * ps.y + bc.y * (hx - hxj);
+            st2(F3.pe, p, a1, a2);
+            if (xi && k != 0) {
+                if (!tx) ex += c0 * a1;
+                if (!tz) ez -= c0 * a2;
+            }
+        }
+        if (F4.on && k >= 1 && k < F4.depth) {
+            const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
+            const float4 bc = ldg4(F4.ce + k);
+            const float2 ps = ld2(F4.pe, p);
+            const float a1 = ty ? ps.x : bc.x * ps.x + bc.y * (hx - hxkm1);
+            const float a2 = tx ? ps.y : bc.x * ps.y + bc.y * (hy - hykm1);
+            st2(F4.pe, p, a1, a2);
+            if (xi && yi) {
+                if (!ty) ey += c0 * a1;
+                if (!tx) ex -= c0 * a2;
+            }
+        }
+        if (F5.on && k >= nz - F5.depth + 1) {
+            const int kr = k - (nz - F5.depth);
+            const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
+            const float4 bc = ldg4(F5.ce + kr);
+            const float2 ps = ld2(F5.pe, p);
+            const float a1 = ty ? ps.x : bc.x * ps.x + bc.y * (hx - hxkm1);
+            const float a2 = tx ? ps.y : bc.x * ps.y + bc.y * (hy - hykm1);
+            st2(F5.pe, p, a1, a2);
+            if (xi && yi) {
+                if (!ty) ey += c0 * a1;
+                if (!tx) ex -= c0 * a2;
+            }
+        }
+    } else {
+    /* the branch's own CPML update (src3d/yee.c:950-1057); its x0 correction
+     * multiplies by a zero cb */
     if (F0.on && i > 1 && i < F0.depth) {
         const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
         const float4 bc = ldg4(F0.ce + i);
@@ -301,6 +382,7 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
         st2(F5.pe, p, a1, a2);
         if ((xi && yi) && !ty) ey += g.dy * m.cbm[1] * a1;
         if ((xi && yi) && !tx) ex -= g.dx * m.cbm[0] * a2;
+    }
     }
 
     /* polarisation state (all three components, with the values at this point) */
@@ -362,6 +444,7 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     const bool xi = i > 0 || perx, yi = j > 0 || pery;
     const OFF dIm = (perx && i == 0) ? (OFF)(nx - 1) * 4 : (OFF)0 - (OFF)4;
     const OFF dJm = (pery && j == 0) ? (OFF)(ny - 1) * dJ : (OFF)0 - dJ;
+    const bool cons = ADE && g.ade_cpml;
 
     float hxkm1 = ldf(g.hxg, o - dP), hykm1 = ldf(g.hyg, o - dP);
     for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
@@ -394,13 +477,15 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             if (fl & GSV_E_PECZ) ez = 0.f;
         }
 
-        /* CPML E psi and correction, src3d/boundary.c:691-887 */
+        /* CPML E psi and correction, src3d/boundary.c:691-887; under
+         * ADE_CPML_CONSISTENT the psi of dispersive components (flag clear)
+         * were advanced by the ADE branch */
         if (sx0) {
             const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
             const float4 bc = ldg4(F0.ce + i);
             const float2 ps = ld2(F0.pe, p);
-            const float a1 = bc.x * ps.x + bc.y * (hz - hzi);
-            const float a2 = bc.x * ps.y + bc.y * (hy - hyi);
+            const float a1 = (cons && !ty) ? ps.x : bc.x * ps.x + bc.y * (hz - hzi);
+            const float a2 = (cons && !tz) ? ps.y : bc.x * ps.y + bc.y * (hy - hyi);
             st2(F0.pe, p, a1, a2);
             if (yi && k != 0) {
                 if (ty) ey -= g.dx * ec.cb[1] * a1;
@@ -411,8 +496,8 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
             const float4 bc = ldg4(F2.ce + j);
             const float2 ps = ld2(F2.pe, p);
-            const float a1 = bc.x * ps.x + bc.y * (hz - hzj);
-            const float a2 = bc.x * ps.y + bc.y * (hx - hxj);
+            const float a1 = (cons && !tx) ? ps.x : bc.x * ps.x + bc.y * (hz - hzj);
+            const float a2 = (cons && !tz) ? ps.y : bc.x * ps.y + bc.y * (hx - hxj);
             st2(F2.pe, p, a1, a2);
             if (xi && k != 0) {
                 if (tx) ex += g.dy * ec.cb[0] * a1;
@@ -423,21 +508,22 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
             const float4 bc = ldg4(F4.ce + k);
             const float2 ps = ld2(F4.pe, p);
-            const float a1 = bc.x * ps.x + bc.y * (hx - hxkm1);
-            const float a2 = bc.x * ps.y + bc.y * (hy - hykm1);
+            const float a1 = (cons && !ty) ? ps.x : bc.x * ps.x + bc.y * (hx - hxkm1);
+            const float a2 = (cons && !tx) ? ps.y : bc.x * ps.y + bc.y * (hy - hykm1);
             st2(F4.pe, p, a1, a2);
             if (xi && yi) {
                 if (ty) ey += g.dx * ec.cb[1] * a1;
-                /* the reference gates this Ex correction on the z flag (src3d/boundary.c:800-801) */
-                if (tz) ex -= g.dx * ec.cb[0] * a2;
+                /* the reference gates this Ex correction on the z flag
+                 * (src3d/boundary.c:800-801); the x flag under ADE_CPML_CONSISTENT */
+                if (cons ? tx : tz) ex -= g.dx * ec.cb[0] * a2;
             }
         }
         if (sxn) {
             const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + irn) * 8;
             const float4 bc = ldg4(F1.ce + irn);
             const float2 ps = ld2(F1.pe, p);
-            const float a1 = bc.x * ps.x + bc.y * (hz - hzi);
-            const float a2 = bc.x * ps.y + bc.y * (hy - hyi);
+            const float a1 = (cons && !ty) ? ps.x : bc.x * ps.x + bc.y * (hz - hzi);
+            const float a2 = (cons && !tz) ? ps.y : bc.x * ps.y + bc.y * (hy - hyi);
             st2(F1.pe, p, a1, a2);
             if (yi && k != 0) {
                 if (ty) ey -= g.dx * ec.cb[1] * a1;
@@ -448,8 +534,8 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             const OFF p = (OFF)(((unsigned)k * F3.depth + jrn) * px + i) * 8;
             const float4 bc = ldg4(F3.ce + jrn);
             const float2 ps = ld2(F3.pe, p);
-            const float a1 = bc.x * ps.x + bc.y * (hz - hzj);
-            const float a2 = bc.x * ps.y + bc.y * (hx - hxj);
+            const float a1 = (cons && !tx) ? ps.x : bc.x * ps.x + bc.y * (hz - hzj);
+            const float a2 = (cons && !tz) ? ps.y : bc.x * ps.y + bc.y * (hx - hxj);
             st2(F3.pe, p, a1, a2);
             if (xi && k != 0) {
                 if (tx) ex += g.dy * ec.cb[0] * a1;
@@ -461,8 +547,8 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
             const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
             const float4 bc = ldg4(F5.ce + kr);
             const float2 ps = ld2(F5.pe, p);
-            const float a1 = bc.x * ps.x + bc.y * (hx - hxkm1);
-            const float a2 = bc.x * ps.y + bc.y * (hy - hykm1);
+            const float a1 = (cons && !ty) ? ps.x : bc.x * ps.x + bc.y * (hx - hxkm1);
+            const float a2 = (cons && !tx) ? ps.y : bc.x * ps.y + bc.y * (hy - hykm1);
             st2(F5.pe, p, a1, a2);
             if (xi && yi) {
                 if (ty) ey += g.dx * ec.cb[1] * a1;
@@ -643,439 +729,6 @@ __global__ __launch_bounds__(256) void
 k_tsf_e(GsvGrid g, GsvTsf t, int axis)
 {
     tsf_e_cell(g, t, axis, blockIdx.z, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y);
-}
-
-/* ------------------------------------------------------------------ */
-/* fused step: H then E in one pass over the fields                    */
-/* ------------------------------------------------------------------ */
-/*
- * A block covers whole x rows (thread i = cell i, up to 1024) of `BY` rows and
- * marches up a chunk of z planes; per plane it updates H of each row, trades
- * the row's new Hz, Hy through LDS for the E update's -x neighbour, and
- * updates E with H(j-1) and H(k-1) kept in registers.  The previous step's
- * E, H and H-side psi are read from the grid and the new ones written to
- * separate buffers (GsvStepOut), so the H of row j0-1 and of plane k0-1,
- * which neighbouring blocks own, is recomputed here from the old values
- * without being stored.  Each cell's E and H are read once and written once
- * per step (the two-pass step reads both in each pass).
- */
-
-/* The TSF H corrections of a step, per face cell, as the two float deltas
- * tsf_h_cell adds (its arithmetic, signs folded in; 0 where it adds nothing):
- * x faces (Hy, Hz), y faces (Hx, Hz), z faces (Hx, Hy). */
-__global__ __launch_bounds__(256) void
-k_tsf_hcorr(GsvGrid g, GsvTsf t, GsvTsfDelta dl, int axis)
-{
-    const int far = blockIdx.z, a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y * blockDim.y + threadIdx.y;
-    int na, nb;
-    tsf_face_extent(t, g.nx, g.ny, axis, &na, &nb);
-    if (a >= na || b >= nb)
-        return;
-    const float cf = gp(t.coefH[axis * 2 + far])[a + (long long)na * b];
-    float d0 = 0.f, d1 = 0.f;
-    if (cf != 0.f) {
-        const float s = far ? 1.f : -1.f;
-        if (axis == 0) {
-            const int j = t.j0 + a, k = t.k0 + b;
-            const double ve = line_val(t.je, t.res, tsf_d(t, far ? t.in : t.i0, j, k));
-            if (k < t.kn) d0 = s * (float)(cf * t.gez * ve);
-            if (j < t.jn) d1 = -(s * (float)(cf * t.gey * ve));
-        } else if (axis == 1) {
-            const int i = t.i0 + a, k = t.k0 + b;
-            const double ve = line_val(t.je, t.res, tsf_d(t, i, far ? t.jn : t.j0, k));
-            if (k < t.kn) d0 = -(s * (float)(cf * t.gez * ve));
-            if (i < t.in) d1 = s * (float)(cf * t.gex * ve);
-        } else {
-            const int i = t.i0 + a, j = t.j0 + b;
-            const double ve = line_val(t.je, t.res, tsf_d(t, i, j, far ? t.kn : t.k0));
-            if (j < t.jn) d0 = s * (float)(cf * t.gey * ve);
-            if (i < t.in) d1 = -(s * (float)(cf * t.gex * ve));
-        }
-    }
-    st2(dl.d[axis * 2 + far], (unsigned)(a + na * b) * 8u, d0, d1);
-}
-
-/* add the step's TSF H corrections of cell (i, j, k), faces in the order of
- * the face launches (x, then y, then z) */
-__device__ __forceinline__ void
-tsf_h_add(const GsvTsf &t, const GsvTsfDelta &dl, int nx, int ny, int i, int j, int k, float &hx, float &hy, float &hz)
-{
-    const int in = t.in < nx - 1 ? t.in : nx - 1, jn = t.jn < ny - 1 ? t.jn : ny - 1;
-    if (j >= t.j0 && j <= jn && k >= t.k0 && k <= t.kn && (i == t.i0 - 1 || i == t.in)) {
-        const float2 d = ld2(dl.d[i == t.in ? 1 : 0], (unsigned)((j - t.j0) + (jn - t.j0 + 1) * (k - t.k0)) * 8u);
-        hy += d.x;
-        hz += d.y;
-    }
-    if (i >= t.i0 && i <= in && k >= t.k0 && k <= t.kn && (j == t.j0 - 1 || j == t.jn)) {
-        const float2 d = ld2(dl.d[j == t.jn ? 3 : 2], (unsigned)((i - t.i0) + (in - t.i0 + 1) * (k - t.k0)) * 8u);
-        hx += d.x;
-        hz += d.y;
-    }
-    if (i >= t.i0 && i <= in && j >= t.j0 && j <= jn && (k == t.k0 - 1 || k == t.kn)) {
-        const float2 d = ld2(dl.d[k == t.kn ? 5 : 4], (unsigned)((i - t.i0) + (in - t.i0 + 1) * (j - t.j0)) * 8u);
-        hx += d.x;
-        hy += d.y;
-    }
-}
-
-/* this step's H of cell (i, j, k): the Yee update, the CPML psi update and
- * correction and far-plane zeroing of hstep_col, then the TSF H faces; the new
- * H and psi are stored only when `own` */
-template <typename ID, bool PER>
-__device__ __forceinline__ void
-fused_h(const GsvGrid &g, const GsvStepOut &w, const GsvTsf &t, const GsvTsfDelta &dl, bool tsf, bool own, int i,
-        int j, int k, float &hx, float &hy, float &hz)
-{
-    const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
-    const unsigned dP = (unsigned)g.plane * 4u, dJ = (unsigned)px * 4u;
-    const unsigned o = ((unsigned)i + (unsigned)px * (unsigned)j + (unsigned)g.plane * (unsigned)(k + 1)) * 4u;
-    const unsigned oc = ((unsigned)i + (unsigned)px * (unsigned)j + (unsigned)g.plane * (unsigned)k) * sizeof(ID);
-    const bool perx = PER && g.perx, pery = PER && g.pery;
-    const bool iin = (i >= 1 && i < nx - 1) || perx, jin = (j >= 1 && j < ny - 1) || pery;
-    const bool xe = i == nx - 1 && !perx, ye = j == ny - 1 && !pery;
-    const unsigned dIp = (perx && i == nx - 1) ? 0u - (unsigned)(nx - 1) * 4u : 4u;
-    const unsigned dJp = (pery && j == ny - 1) ? 0u - (unsigned)(ny - 1) * dJ : dJ;
-    const float exk = ldf(g.exg, o), eyk = ldf(g.eyg, o), ezk = ldf(g.ezg, o);
-    const float exk1 = ldf(g.exg, o + dP), eyk1 = ldf(g.eyg, o + dP);
-    const float ezj1 = ldf(g.ezg, o + dJp), exj1 = ldf(g.exg, o + dJp);
-    const float ezi1 = ldf(g.ezg, o + dIp), eyi1 = ldf(g.eyg, o + dIp);
-    const GsvHCoef hc = ldgs(g.htab + ldid<ID>(g.hcls, oc));
-    const float rkx = gp(g.rkhx)[i], rky = gp(g.rkhy)[j], rkz = gp(g.rkhz)[k];
-    hx = ldf(g.hxg, o);
-    hy = ldf(g.hyg, o);
-    hz = ldf(g.hzg, o);
-    const bool kin = (k >= 1 && k < nz - 1);
-    if (!(ye || k == nz - 1))
-        hx = hc.da * hx + hc.db[0] * ((eyk1 - eyk) * rkz - (ezj1 - ezk) * rky);
-    if (!(xe || k == nz - 1))
-        hy = hc.da * hy + hc.db[1] * ((ezi1 - ezk) * rkx - (exk1 - exk) * rkz);
-    if (!(xe || ye))
-        hz = hc.da * hz + hc.db[2] * ((exj1 - exk) * rky - (eyi1 - eyk) * rkx);
-
-    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
-    if (kin) {
-        if (F0.on && i < F0.depth - 1 && jin) {
-            const unsigned p = (((unsigned)k * ny + j) * F0.depth + i) * 8u;
-            const float2 bc = ldg2(F0.ch + i);
-            const float2 ps = ld2(F0.ph, p);
-            const float a1 = bc.x * ps.x + bc.y * (eyi1 - eyk);
-            const float a2 = bc.x * ps.y + bc.y * (ezi1 - ezk);
-            if (own) st2(w.ph[0], p, a1, a2);
-            hy += g.dx * hc.dbc * a2;
-            hz -= g.dx * hc.dbc * a1;
-        }
-        if (F1.on && i >= nx - F1.depth + 1 && i < nx - 1 && jin) {
-            const int irn = i - (nx - F1.depth);
-            const unsigned p = (((unsigned)k * ny + j) * F1.depth + irn) * 8u;
-            const float2 bc = ldg2(F1.ch + irn);
-            const float2 ps = ld2(F1.ph, p);
-            const float a1 = bc.x * ps.x + bc.y * (eyi1 - eyk);
-            const float a2 = bc.x * ps.y + bc.y * (ezi1 - ezk);
-            if (own) st2(w.ph[1], p, a1, a2);
-            hy += g.dx * hc.dbc * a2;
-            hz -= g.dx * hc.dbc * a1;
-        }
-        if (F2.on && j < F2.depth - 1 && iin) {
-            const unsigned p = (((unsigned)k * F2.depth + j) * px + i) * 8u;
-            const float2 bc = ldg2(F2.ch + j);
-            const float2 ps = ld2(F2.ph, p);
-            const float a1 = bc.x * ps.x + bc.y * (ezj1 - ezk);
-            const float a2 = bc.x * ps.y + bc.y * (exj1 - exk);
-            if (own) st2(w.ph[2], p, a1, a2);
-            hx -= g.dy * hc.dbc * a1;
-            hz += g.dy * hc.dbc * a2;
-        }
-        if (F3.on && j >= ny - F3.depth + 1 && j < ny - 1 && iin) {
-            const int jrn = j - (ny - F3.depth);
-            const unsigned p = (((unsigned)k * F3.depth + jrn) * px + i) * 8u;
-            const float2 bc = ldg2(F3.ch + jrn);
-            const float2 ps = ld2(F3.ph, p);
-            const float a1 = bc.x * ps.x + bc.y * (ezj1 - ezk);
-            const float a2 = bc.x * ps.y + bc.y * (exj1 - exk);
-            if (own) st2(w.ph[3], p, a1, a2);
-            hx -= g.dy * hc.dbc * a1;
-            hz += g.dy * hc.dbc * a2;
-        }
-    }
-    if (iin && jin) {
-        if (F4.on && k < F4.depth - 1) {
-            const unsigned p = (((unsigned)k * ny + j) * px + i) * 8u;
-            const float2 bc = ldg2(F4.ch + k);
-            const float2 ps = ld2(F4.ph, p);
-            const float a1 = bc.x * ps.x + bc.y * (exk1 - exk);
-            const float a2 = bc.x * ps.y + bc.y * (eyk1 - eyk);
-            if (own) st2(w.ph[4], p, a1, a2);
-            hy -= g.dz * hc.dbc * a1;
-            hx += g.dz * hc.dbc * a2;
-        }
-        if (F5.on && k >= nz - F5.depth + 1 && k < nz - 1) {
-            const int kr = k - (nz - F5.depth);
-            const unsigned p = (((unsigned)kr * ny + j) * px + i) * 8u;
-            const float2 bc = ldg2(F5.ch + kr);
-            const float2 ps = ld2(F5.ph, p);
-            const float a1 = bc.x * ps.x + bc.y * (exk1 - exk);
-            const float a2 = bc.x * ps.y + bc.y * (eyk1 - eyk);
-            if (own) st2(w.ph[5], p, a1, a2);
-            hy -= g.dz * hc.dbc * a1;
-            hx += g.dz * hc.dbc * a2;
-        }
-    }
-    const bool zx = F1.on && i == nx - 1 && jin, zy = F3.on && j == ny - 1 && iin;
-    if ((kin && (zx || zy)) || (F5.on && k == nz - 1 && iin && jin)) {
-        hx = 0.f;
-        hy = 0.f;
-        hz = 0.f;
-    }
-    if (tsf)
-        tsf_h_add(t, dl, nx, ny, i, j, k, hx, hy, hz);
-    if (own) {
-        stf(w.hxg, o, hx);
-        stf(w.hyg, o, hy);
-        stf(w.hzg, o, hz);
-    }
-}
-
-/* this step's E of cell (i, j, k) from this step's H around it: the update of
- * estep_col (classes, ADE branch, PEC, CPML psi and correction, near-plane
- * zeroing); psi_E and the ADE state are updated in place (only this cell
- * reads them), the new E goes to w */
-template <typename ID, int ADE, bool PER>
-__device__ __forceinline__ void
-fused_e(const GsvGrid &g, const GsvStepOut &w, int i, int j, int k, float hx, float hy, float hz, float hzj,
-        float hxj, float hzi, float hyi, float hxkm1, float hykm1)
-{
-    const int nx = g.nx, ny = g.ny, px = g.px, nz = g.nz;
-    const unsigned o = ((unsigned)i + (unsigned)px * (unsigned)j + (unsigned)g.plane * (unsigned)(k + 1)) * 4u;
-    const unsigned oc = ((unsigned)i + (unsigned)px * (unsigned)j + (unsigned)g.plane * (unsigned)k) * sizeof(ID);
-    const unsigned la = ADE ? ((unsigned)j * g.pxa * GSV_ADE_D + (unsigned)(i >> 6) * 64 * GSV_ADE_D + (i & 63)) * 4u : 0u;
-    constexpr int NP = ADE >= 2 ? ADE : 0;
-    const unsigned lp = NP ? ((unsigned)j * g.pxa * GSV_ADE_P(NP) + (unsigned)(i >> 6) * 64 * GSV_ADE_P(NP) + (i & 63)) * 4u : 0u;
-    const float rkx = gp(g.rkex)[i], rky = gp(g.rkey)[j], rkz = gp(g.rkez)[k];
-    const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
-    const bool perx = PER && g.perx, pery = PER && g.pery;
-    const bool xi = i > 0 || perx, yi = j > 0 || pery;
-    const GsvECoef ec = ldgs(g.etab + ldid<ID>(g.ecls, oc));
-    float ex = ldf(g.exg, o), ey = ldf(g.eyg, o), ez = ldf(g.ezg, o);
-    const unsigned fl = ec.flags;
-    const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
-
-    if (ADE && (fl & GSV_E_ADE))
-        ade_cell<NP, PER>(g, g.ade_d + (size_t)k * g.ade_plane_d, la, NP ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr,
-                          lp, i, j, k, fl, ex, ey, ez, hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
-
-    if (tx && yi && k > 0)
-        ex = ec.ca[0] * ex + ec.cb[0] * ((hz - hzj) * rky - (hy - hykm1) * rkz);
-    if (ty && xi && k > 0)
-        ey = ec.ca[1] * ey + ec.cb[1] * ((hx - hxkm1) * rkz - (hz - hzi) * rkx);
-    if (tz && xi && yi)
-        ez = ec.ca[2] * ez + ec.cb[2] * ((hy - hyi) * rkx - (hx - hxj) * rky);
-
-    if (fl & (GSV_E_PECX | GSV_E_PECY | GSV_E_PECZ)) {
-        if (fl & GSV_E_PECX) ex = 0.f;
-        if (fl & GSV_E_PECY) ey = 0.f;
-        if (fl & GSV_E_PECZ) ez = 0.f;
-    }
-
-    if (F0.on && i >= 1 && i < F0.depth) {
-        const unsigned p = (((unsigned)k * ny + j) * F0.depth + i) * 8u;
-        const float4 bc = ldg4(F0.ce + i);
-        const float2 ps = ld2(F0.pe, p);
-        const float a1 = bc.x * ps.x + bc.y * (hz - hzi);
-        const float a2 = bc.x * ps.y + bc.y * (hy - hyi);
-        st2(F0.pe, p, a1, a2);
-        if (yi && k != 0) {
-            if (ty) ey -= g.dx * ec.cb[1] * a1;
-            if (tz) ez += g.dx * ec.cb[2] * a2;
-        }
-    }
-    if (F2.on && j >= 1 && j < F2.depth) {
-        const unsigned p = (((unsigned)k * F2.depth + j) * px + i) * 8u;
-        const float4 bc = ldg4(F2.ce + j);
-        const float2 ps = ld2(F2.pe, p);
-        const float a1 = bc.x * ps.x + bc.y * (hz - hzj);
-        const float a2 = bc.x * ps.y + bc.y * (hx - hxj);
-        st2(F2.pe, p, a1, a2);
-        if (xi && k != 0) {
-            if (tx) ex += g.dy * ec.cb[0] * a1;
-            if (tz) ez -= g.dy * ec.cb[2] * a2;
-        }
-    }
-    if (F4.on && k >= 1 && k < F4.depth) {
-        const unsigned p = (((unsigned)k * ny + j) * px + i) * 8u;
-        const float4 bc = ldg4(F4.ce + k);
-        const float2 ps = ld2(F4.pe, p);
-        const float a1 = bc.x * ps.x + bc.y * (hx - hxkm1);
-        const float a2 = bc.x * ps.y + bc.y * (hy - hykm1);
-        st2(F4.pe, p, a1, a2);
-        if (xi && yi) {
-            if (ty) ey += g.dx * ec.cb[1] * a1;
-            if (tz) ex -= g.dx * ec.cb[0] * a2;   /* gated on the z flag as in src3d/boundary.c:800-801 */
-        }
-    }
-    if (F1.on && i >= nx - F1.depth + 1) {
-        const int irn = i - (nx - F1.depth);
-        const unsigned p = (((unsigned)k * ny + j) * F1.depth + irn) * 8u;
-        const float4 bc = ldg4(F1.ce + irn);
-        const float2 ps = ld2(F1.pe, p);
-        const float a1 = bc.x * ps.x + bc.y * (hz - hzi);
-        const float a2 = bc.x * ps.y + bc.y * (hy - hyi);
-        st2(F1.pe, p, a1, a2);
-        if (yi && k != 0) {
-            if (ty) ey -= g.dx * ec.cb[1] * a1;
-            if (tz) ez += g.dx * ec.cb[2] * a2;
-        }
-    }
-    if (F3.on && j >= ny - F3.depth + 1) {
-        const int jrn = j - (ny - F3.depth);
-        const unsigned p = (((unsigned)k * F3.depth + jrn) * px + i) * 8u;
-        const float4 bc = ldg4(F3.ce + jrn);
-        const float2 ps = ld2(F3.pe, p);
-        const float a1 = bc.x * ps.x + bc.y * (hz - hzj);
-        const float a2 = bc.x * ps.y + bc.y * (hx - hxj);
-        st2(F3.pe, p, a1, a2);
-        if (xi && k != 0) {
-            if (tx) ex += g.dy * ec.cb[0] * a1;
-            if (tz) ez -= g.dy * ec.cb[2] * a2;
-        }
-    }
-    if (F5.on && k >= nz - F5.depth + 1) {
-        const int kr = k - (nz - F5.depth);
-        const unsigned p = (((unsigned)kr * ny + j) * px + i) * 8u;
-        const float4 bc = ldg4(F5.ce + kr);
-        const float2 ps = ld2(F5.pe, p);
-        const float a1 = bc.x * ps.x + bc.y * (hx - hxkm1);
-        const float a2 = bc.x * ps.y + bc.y * (hy - hykm1);
-        st2(F5.pe, p, a1, a2);
-        if (xi && yi) {
-            if (ty) ey += g.dx * ec.cb[1] * a1;
-            if (tx) ex -= g.dx * ec.cb[0] * a2;
-        }
-    }
-    if ((F0.on && i == 0) || (F2.on && j == 0) || (F4.on && k == 0)) {
-        ex = 0.f;
-        ey = 0.f;
-        ez = 0.f;
-    }
-    stf(w.exg, o, ex);
-    stf(w.eyg, o, ey);
-    stf(w.ezg, o, ez);
-}
-
-/* LDS of a fused-step block (floats, X = blockDim.x): the plane's H of the
- * halo row and the block's rows, [rows + 1][3][X] (Hx, Hy, Hz), then this
- * step's Hx, Hy of the previous plane per row, [rows][2][X] */
-static size_t
-step_lds_floats(int rows, int X)
-{
-    return (size_t)(rows + 1) * 3 * X + (size_t)rows * 2 * X;
-}
-
-template <typename ID, int ADE, bool PER>
-__global__ __launch_bounds__(1024) void
-k_step(GsvGrid g, GsvStepOut w, GsvTsf t, GsvTsfDelta dl, int tsf, int rows, int kchunk)
-{
-    /* per plane: H of the halo row and of the block's rows into LDS, a
-     * barrier, E of the rows from LDS (own H, the -x neighbour's, the row
-     * below's, the plane below's), a barrier before the next plane; the row
-     * loops are not unrolled, so the loop body holds one H and one E update */
-    extern __shared__ float step_lds[];
-    const int X = blockDim.x;
-    float *hrow = step_lds;                              /* [rows + 1][3][X] */
-    float *hk = step_lds + (size_t)(rows + 1) * 3 * X;   /* [rows][2][X] */
-    const int i = threadIdx.x;
-    const bool act = i < g.nx;
-    const int j0 = blockIdx.x * rows;
-    const int nrow = g.ny - j0 < rows ? g.ny - j0 : rows;
-    const int k0 = blockIdx.y * kchunk;
-    const int k1 = k0 + kchunk < g.nz ? k0 + kchunk : g.nz;
-    const bool pery = PER && g.pery;
-    const bool halo = j0 > 0 || pery;
-    const int il = i > 0 ? i - 1 : g.nx - 1;
-    /* r = -1 .. nrow - 1: the halo row (j0 - 1, or ny - 1 across a periodic y
-     * axis) and the block's rows; the H of plane k0 - 1 fills hk */
-    for (int r = 0; r < nrow; r++) {
-        float a = 0.f, b = 0.f, c = 0.f;
-        if (act && k0 > 0)
-            fused_h<ID, PER>(g, w, t, dl, tsf, false, i, j0 + r, k0 - 1, a, b, c);
-        hk[(2 * r) * X + i] = a;
-        hk[(2 * r + 1) * X + i] = b;
-    }
-    for (int k = k0; k < k1; k++) {
-        for (int r = -1; r < nrow; r++) {
-            float a = 0.f, b = 0.f, c = 0.f;
-            if (act && (r >= 0 || halo)) {
-                const int j = r >= 0 ? j0 + r : (j0 > 0 ? j0 - 1 : g.ny - 1);
-                fused_h<ID, PER>(g, w, t, dl, tsf, r >= 0, i, j, k, a, b, c);
-            }
-            float *hr = hrow + (size_t)(3 * (r + 1)) * X;
-            hr[i] = a;
-            hr[X + i] = b;
-            hr[2 * X + i] = c;
-        }
-        __syncthreads();
-        if (act) {
-            for (int r = 0; r < nrow; r++) {
-                const float *hr = hrow + (size_t)(3 * (r + 1)) * X, *hb = hr - 3 * X;
-                const float hx = hr[i], hy = hr[X + i], hz = hr[2 * X + i];
-                float *hkr = hk + (size_t)(2 * r) * X;
-                fused_e<ID, ADE, PER>(g, w, i, j0 + r, k, hx, hy, hz, hb[2 * X + i], hb[i], hr[2 * X + il], hr[X + il],
-                                      hkr[i], hkr[X + i]);
-                hkr[i] = hx;
-                hkr[X + i] = hy;
-            }
-        }
-        __syncthreads();
-    }
-}
-
-template <typename ID, int ADE>
-static void
-launch_step_per(const GsvGrid *g, const GsvStepOut *w, const GsvTsf *t, const GsvTsfDelta *dl, int tsf, int rows,
-                int kchunk, dim3 blk, hipStream_t st)
-{
-    const dim3 grd((g->ny + rows - 1) / rows, (g->nz + kchunk - 1) / kchunk);
-    const size_t lds = step_lds_floats(rows, blk.x) * 4;
-    if (g->perx || g->pery)
-        hipLaunchKernelGGL((k_step<ID, ADE, true>), grd, blk, lds, st, *g, *w, *t, *dl, tsf, rows, kchunk);
-    else
-        hipLaunchKernelGGL((k_step<ID, ADE, false>), grd, blk, lds, st, *g, *w, *t, *dl, tsf, rows, kchunk);
-}
-
-template <typename ID>
-static void
-launch_step_ade(const GsvGrid *g, const GsvStepOut *w, const GsvTsf *t, const GsvTsfDelta *dl, int tsf, int ade,
-                int rows, int kchunk, dim3 blk, hipStream_t st)
-{
-    if (ade == 0) launch_step_per<ID, 0>(g, w, t, dl, tsf, rows, kchunk, blk, st);
-    else if (ade == 1) launch_step_per<ID, 1>(g, w, t, dl, tsf, rows, kchunk, blk, st);
-    else if (ade == 2) launch_step_per<ID, 2>(g, w, t, dl, tsf, rows, kchunk, blk, st);
-    else launch_step_per<ID, 3>(g, w, t, dl, tsf, rows, kchunk, blk, st);
-}
-
-static bool offsets32(const GsvGrid *g);
-
-extern "C" hipError_t
-gsv_launch_step(const GsvGrid *g, const GsvStepOut *w, const GsvTsf *t, const GsvTsfDelta *dl, int tsf, int idbytes,
-                int rows, int kchunk, hipStream_t st)
-{
-    const dim3 blk((g->nx + 63) / 64 * 64);
-    if (g->nx > 1024 || !offsets32(g) || kchunk < 1 || rows < 1 || step_lds_floats(rows, blk.x) * 4 > 160 * 1024)
-        return hipErrorInvalidValue;
-    const int ade = g->ade ? (g->ade_poles ? g->ade_poles : 1) : 0;
-    if (tsf) {
-        /* this step's TSF H corrections per face cell, before the step reads them */
-        for (int axis = 0; axis < 3; axis++) {
-            int na, nb;
-            tsf_face_extent(*t, g->nx, g->ny, axis, &na, &nb);
-            const dim3 fb(64, 4, 1), fg((na + 63) / 64, (nb + 3) / 4, 2);
-            hipLaunchKernelGGL(k_tsf_hcorr, fg, fb, 0, st, *g, *t, *dl, axis);
-        }
-    }
-    if (idbytes == 1)
-        launch_step_ade<uint8_t>(g, w, t, dl, tsf, ade, rows, kchunk, blk, st);
-    else
-        launch_step_ade<uint16_t>(g, w, t, dl, tsf, ade, rows, kchunk, blk, st);
-    return hipGetLastError();
 }
 
 /* ------------------------------------------------------------------ */

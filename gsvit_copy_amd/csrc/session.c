@@ -20,8 +20,6 @@ int gsv_engine_get_field(void *eng, int comp, float *out);
 int gsv_engine_fetch_aux(void *eng);
 void gsv_engine_free(void *eng);
 int gsv_engine_set_launch(void *eng, const int *shape);
-int gsv_engine_set_step_mode(void *eng, int mode, int rows, int kchunk);
-void gsv_engine_get_step_mode(void *eng, int *out);
 void gsv_engine_get_launch(void *eng, int *shape);
 int gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_interval);
 int gsv_engine_adjoint_forward(void *eng, int (*step_fn)(void *), void *ctx, double *J);
@@ -290,52 +288,6 @@ gsv_get_launch(void *h, int *shape)
     gsv_engine_get_launch(((GsvSession *)h)->mp->gpu, shape);
 }
 
-int
-gsv_set_step_mode(void *h, int mode, int rows, int kchunk)
-{
-    return gsv_engine_set_step_mode(((GsvSession *)h)->mp->gpu, mode, rows, kchunk);
-}
-
-void
-gsv_get_step_mode(void *h, int *out)
-{
-    gsv_engine_get_step_mode(((GsvSession *)h)->mp->gpu, out);
-}
-
-/* fused-step shapes (rows per block, z chunk) timed on real steps */
-static int
-autotune_fused(void *h, int steps_per_candidate)
-{
-    static const int rows[] = { 2, 4, 8 };
-    static const int kcs[] = { 8, 16, 32, 64 };
-    GsvSession *s = (GsvSession *)h;
-    void *eng = s->mp->gpu;
-    double best = 1e30, t[4];
-    int a, c, br = 4, bk = 16, start = s->set.sc.step_act;
-    for (a = 0; a < 3; a++)
-        for (c = 0; c < 4; c++) {
-            if (s->set.sc.step_act + steps_per_candidate > s->set.sc.nsteps)
-                goto done;
-            if (gsv_engine_set_step_mode(eng, 1, rows[a], kcs[c]))
-                continue;
-            gsv_engine_set_timing(eng, 1);
-            if (gsv_run(h, steps_per_candidate) < 0)
-                return -1;
-            gsv_engine_kernel_times(eng, t);
-            gsv_engine_set_timing(eng, 0);
-            if (t[3] > 0 && t[2] / t[3] < best) {
-                best = t[2] / t[3];
-                br = rows[a];
-                bk = kcs[c];
-            }
-        }
-done:
-    gsv_engine_set_step_mode(eng, 1, br, bk);
-    if (s->set.sc.verbose > 0)
-        printf("autotune: fused step %d rows z%d (%.3f ms)\n", br, bk, best);
-    return s->set.sc.step_act - start;
-}
-
 /* Launch-shape autotuner: each candidate tile (x width, rows, z-march depth)
  * runs `steps_per_candidate` real steps of the simulation with per-launch
  * event timing; the fastest shape is kept for the H and for the E kernel
@@ -349,11 +301,8 @@ gsv_autotune(void *h, int steps_per_candidate, int *best)
     GsvSession *s = (GsvSession *)h;
     void *eng = s->mp->gpu;
     double bh = 1e30, be = 1e30, t[4];
-    int a, b, c, used = 0, shape[6], start = s->set.sc.step_act, mode[4];
+    int a, b, c, used = 0, shape[6], start = s->set.sc.step_act;
     gsv_engine_get_launch(eng, best);
-    gsv_engine_get_step_mode(eng, mode);
-    if (mode[0])
-        return autotune_fused(h, steps_per_candidate);
     for (a = 0; a < 3; a++)
         for (b = 0; b < 4; b++)
             for (c = 0; c < 5; c++) {

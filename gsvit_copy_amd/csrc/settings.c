@@ -618,6 +618,10 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             b->sigma_bxn = b->sigma_by0 = b->sigma_byn = b->sigma_bz0 = b->sigma_bzn = b->sigma_bx0;
             b->a_bxn = b->a_by0 = b->a_byn = b->a_bz0 = b->a_bzn = b->a_bx0;
             b->kappa_bxn = b->kappa_by0 = b->kappa_byn = b->kappa_bz0 = b->kappa_bzn = b->kappa_bx0;
+        } else if (!strcmp(key, "ADE_CPML_CONSISTENT")) {
+            /* no reference counterpart: consistent CPML of dispersive cells */
+            if (get_int(fr, &set->sb.ade_cpml_consistent, key))
+                goto fail;
         } else if (!strcmp(key, "BOUNDARY_X0")) {
             SvSetBoundary *b = &set->sb;
             if (parse_cpml_face(fr, key, set, &b->bx0, &b->depth_bx0, &b->m_bx0, &b->sigma_bx0, &b->a_bx0, &b->kappa_bx0))

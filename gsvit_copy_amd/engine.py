@@ -56,10 +56,6 @@ def library():
         L.gsv_close.argtypes = [vp]
         L.gsv_set_launch.restype = ctypes.c_int
         L.gsv_set_launch.argtypes = [vp, ip]
-        L.gsv_set_step_mode.restype = ctypes.c_int
-        L.gsv_set_step_mode.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-        L.gsv_get_step_mode.restype = None
-        L.gsv_get_step_mode.argtypes = [vp, ip]
         L.gsv_get_launch.argtypes = [vp, ip]
         L.gsv_autotune.restype = ctypes.c_int
         L.gsv_autotune.argtypes = [vp, ctypes.c_int, ip]
@@ -238,17 +234,6 @@ class Session:
         v = (ctypes.c_int * 6)()
         self.L.gsv_get_launch(self.h, v)
         return list(v)
-
-    def set_step_mode(self, fused, rows=4, kchunk=16):
-        """fused=True: one pass over the fields per step (where the pool allows it);
-        False: the two-pass step"""
-        if self.L.gsv_set_step_mode(self.h, 1 if fused else 0, int(rows), int(kchunk)):
-            raise ValueError("the fused step is not available for this pool (or rows %r / chunk %r)" % (rows, kchunk))
-
-    def step_mode(self):
-        v = (ctypes.c_int * 4)()
-        self.L.gsv_get_step_mode(self.h, v)
-        return {"fused": bool(v[0]), "rows": v[1], "kchunk": v[2], "fused_available": bool(v[3])}
 
     def autotune(self, steps_per_candidate=2):
         """Times candidate launch shapes on real steps; returns (shape, steps consumed)."""

@@ -64,20 +64,8 @@ int gsv_set_launch(void *session, const int *shape);
 void gsv_get_launch(void *session, int *shape);
 /* pick the fastest launch shape for each volume kernel by timing candidate
  * shapes on real steps (steps_per_candidate each); writes the chosen shape to
- * best[0..5] and returns the number of steps consumed, -1 on error.  In the
- * fused step mode it tunes the fused step's rows and z chunk instead (best is
- * then the unchanged two-pass shape). */
+ * best[0..5] and returns the number of steps consumed, -1 on error. */
 int gsv_autotune(void *session, int steps_per_candidate, int *best);
-/* Step mode (no reference counterpart): 1 = fused step, one pass over the
- * fields per time step (H, TSF H faces and E in one kernel; E, H and H-side
- * psi double-buffered), with `rows` (1..16) rows and `kchunk` z planes per
- * block; 0 = the two-pass step (H kernel, then E kernel), the default.  The
- * fused step needs a pool without Liao faces, interior periodic planes, H
- * point sources or design region, at most 1024 cells along x, field arrays
- * below 4 GiB and one class-id width.  Returns 0, or 1 when the mode is not
- * available.  out[0..3] = mode, rows, z chunk, fused step available. */
-int gsv_set_step_mode(void *session, int mode, int rows, int kchunk);
-void gsv_get_step_mode(void *session, int *out);
 /* Gradient extension (no reference counterpart).  The parameter file
  * declares DESIGN_BOX i0 j0 k0 in jn kn (voxels with per-voxel permittivity
  * from MEDIUM_LINEAR) and OBJECTIVE_DFT i0 j0 k0 in jn kn lambda, the

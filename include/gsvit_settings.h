@@ -136,6 +136,15 @@ typedef struct {
     double sigma_bx0, sigma_bxn, sigma_by0, sigma_byn, sigma_bz0, sigma_bzn;
     double a_bx0, a_bxn, a_by0, a_byn, a_bz0, a_bzn;
     double kappa_bx0, kappa_bxn, kappa_by0, kappa_byn, kappa_bz0, kappa_bzn;
+    /* ADE_CPML_CONSISTENT 1 (no reference counterpart): inside CPML slabs the
+     * psi of a component that takes the dispersive (ADE) update is updated
+     * once per step and applied with that update's own curl coefficient, on
+     * all six faces.  0 (default): the reference's scheme, whose psi of such
+     * components is updated twice per step and applied with the coefficient
+     * of the material's eps/sigma, skipping x0 (src3d/yee.c:950-1057,
+     * src3d/boundary.c:691-887) -- a dispersive metal in the slabs then grows
+     * without bound after some thousand steps. */
+    int ade_cpml_consistent;
 } SvSetBoundary;
 
 typedef struct {

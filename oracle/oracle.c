@@ -99,6 +99,10 @@ typedef struct {
     real jb[8];
     double *points;                   /* [npts][nsteps+1][6] */
     int no_average;                   /* test option: E coefficients without interface averaging */
+    int mb_fault;                     /* test option: a planted mistake in the interior periodic copies
+                                         (1: H from plane bxnpos, 2: E copied the wrong way along x) */
+    int ade_cpml;                     /* ADE_CPML_CONSISTENT (SvSetBoundary): psi of dispersive components
+                                         updated once, applied with the dispersive update's curl coefficient */
     double *objacc;                   /* OBJECTIVE_DFT box: [cell][3][re, im] */
     double *pfstore;                  /* periodic far field, postprocess mode: [plane][6][ix][iy][nt] */
     double *lz;                       /* layered TSF history: [6: ee eh ez he hh hz][nrows][nz] */
@@ -176,6 +180,7 @@ oracle_open(const char *parfile)
 
     o->npoles = 2;
     o->perx = set->sb.bx0 == SV_BOUNDARY_PERIODIC;
+    o->ade_cpml = set->sb.ade_cpml_consistent;
     o->pery = set->sb.by0 == SV_BOUNDARY_PERIODIC;
     for (q = 0; q < o->mp->nmat; q++)
         if (o->mp->mats[q].type == SV_MAT_ADE) {
@@ -398,6 +403,69 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
                    + m->ade_c2 * o->ezp[c] + m->ade_c3 * o->ez[c]
                    + (o->npoles == 3 ? m->ade_c1 * (-m->ade_bp0[2] * o->pzp[2][c] + (1 - m->ade_bp1[2]) * o->pz[2][c]) : 0);
 
+    if (o->ade_cpml) {
+        /* ADE_CPML_CONSISTENT (no reference counterpart): every psi of a
+         * dispersive component is updated here once per step over the slab
+         * ranges of boundary_estep, and the correction carries the ADE curl
+         * coefficient c0 (the ordinary update's dx cb); boundary_estep then
+         * leaves these psi alone.  x0 gets its correction too. */
+        const double c0 = m->ade_c0;
+        int f;
+        for (f = 0; f < 6; f++) {
+            int axis = f / 2, far = f % 2, r, depth, in;
+            double be, ce, d1, d2;
+            size_t p, nb;
+            if (axis == 0) {
+                depth = far ? sb->depth_bxn : sb->depth_bx0;
+                if ((far ? sb->bxn : sb->bx0) != SV_BOUNDARY_CPML) continue;
+                r = far ? i - (nx - depth) : i;
+                in = far ? (i >= nx - depth + 1 && i < nx) : (i >= 1 && i < depth);
+                if (!in) continue;
+                p = pidx_x(o, r, j, k);
+                nb = c - sx;
+                be = far ? bnd->xn.be[r] : bnd->x0.be[r];
+                ce = far ? bnd->xn.ce[r] : bnd->x0.ce[r];
+                d1 = (o->hz[c] - o->hz[nb]) / dx;       /* Ey */
+                d2 = (o->hy[c] - o->hy[nb]) / dx;       /* Ez */
+                if (ytbc == 0) o->f[f].pe1[p] = be * o->f[f].pe1[p] + ce * d1;
+                if (ztbc == 0) o->f[f].pe2[p] = be * o->f[f].pe2[p] + ce * d2;
+                if (yi && k != 0 && ytbc == 0) o->ey[c] -= c0 * o->f[f].pe1[p];
+                if (yi && k != 0 && ztbc == 0) o->ez[c] += c0 * o->f[f].pe2[p];
+            } else if (axis == 1) {
+                depth = far ? sb->depth_byn : sb->depth_by0;
+                if ((far ? sb->byn : sb->by0) != SV_BOUNDARY_CPML) continue;
+                r = far ? j - (ny - depth) : j;
+                in = far ? (j >= ny - depth + 1 && j < ny) : (j >= 1 && j < depth);
+                if (!in) continue;
+                p = pidx_y(o, depth, i, r, k);
+                nb = c - sy;
+                be = far ? bnd->yn.be[r] : bnd->y0.be[r];
+                ce = far ? bnd->yn.ce[r] : bnd->y0.ce[r];
+                d1 = (o->hz[c] - o->hz[nb]) / dy;       /* Ex */
+                d2 = (o->hx[c] - o->hx[nb]) / dy;       /* Ez */
+                if (xtbc == 0) o->f[f].pe1[p] = be * o->f[f].pe1[p] + ce * d1;
+                if (ztbc == 0) o->f[f].pe2[p] = be * o->f[f].pe2[p] + ce * d2;
+                if (xi && k != 0 && xtbc == 0) o->ex[c] += c0 * o->f[f].pe1[p];
+                if (xi && k != 0 && ztbc == 0) o->ez[c] -= c0 * o->f[f].pe2[p];
+            } else {
+                depth = far ? sb->depth_bzn : sb->depth_bz0;
+                if ((far ? sb->bzn : sb->bz0) != SV_BOUNDARY_CPML) continue;
+                r = far ? k - (nz - depth) : k;
+                in = far ? (k >= nz - depth + 1 && k < nz) : (k >= 1 && k < depth);
+                if (!in) continue;
+                p = pidx_z(o, depth, i, j, r);
+                nb = c - 1;
+                be = far ? bnd->zn.be[r] : bnd->z0.be[r];
+                ce = far ? bnd->zn.ce[r] : bnd->z0.ce[r];
+                d1 = (o->hx[c] - o->hx[nb]) / dz;       /* Ey */
+                d2 = (o->hy[c] - o->hy[nb]) / dz;       /* Ex */
+                if (ytbc == 0) o->f[f].pe1[p] = be * o->f[f].pe1[p] + ce * d1;
+                if (xtbc == 0) o->f[f].pe2[p] = be * o->f[f].pe2[p] + ce * d2;
+                if (xi && yi && ytbc == 0) o->ey[c] += c0 * o->f[f].pe1[p];
+                if (xi && yi && xtbc == 0) o->ex[c] -= c0 * o->f[f].pe2[p];
+            }
+        }
+    } else
     /* inline CPML of the ADE branch, src3d/yee.c:952-1057 (x0 uses cb, which is 0 here) */
     {
         double cbm_x = (dt / m->epsilon / dx) / (1 + m->sigma * dt / 2 / m->epsilon);
@@ -628,8 +696,9 @@ boundary_hstep(Oracle *o)
     if (smb->bx0 == SV_BOUNDARY_PERIODIC || smb->bxn == SV_BOUNDARY_PERIODIC)
         for (j = smb->by0pos; j < smb->bynpos; j++)
             for (k = smb->bz0pos; k < smb->bznpos; k++) {
-                o->hy[IDX(o, smb->bx0pos - 1, j, k)] = o->hy[IDX(o, smb->bxnpos - 1, j, k)];
-                o->hz[IDX(o, smb->bx0pos - 1, j, k)] = o->hz[IDX(o, smb->bxnpos - 1, j, k)];
+                const int src = smb->bxnpos - 1 + (o->mb_fault == 1);
+                o->hy[IDX(o, smb->bx0pos - 1, j, k)] = o->hy[IDX(o, src, j, k)];
+                o->hz[IDX(o, smb->bx0pos - 1, j, k)] = o->hz[IDX(o, src, j, k)];
             }
     if (smb->by0 == SV_BOUNDARY_PERIODIC || smb->byn == SV_BOUNDARY_PERIODIC)
         for (i = smb->bx0pos; i < smb->bxnpos; i++)
@@ -848,8 +917,8 @@ boundary_estep(Oracle *o)
                     CPML_E_PRE(0)
                     size_t p = pidx_x(o, i, j, k);
                     double be = bnd->x0.be[i], ce = bnd->x0.ce[i];
-                    o->f[0].pe1[p] = be * o->f[0].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dx;
-                    o->f[0].pe2[p] = be * o->f[0].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dx;
+                    if (!o->ade_cpml || ytbc) o->f[0].pe1[p] = be * o->f[0].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dx;
+                    if (!o->ade_cpml || ztbc) o->f[0].pe2[p] = be * o->f[0].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dx;
                     if (((j != 0 || o->pery) && k != 0) && ytbc == 1) o->ey[c] -= dx * cby * o->f[0].pe1[p];
                     if (((j != 0 || o->pery) && k != 0) && ztbc == 1) o->ez[c] += dx * cbz * o->f[0].pe2[p];
                 }
@@ -868,8 +937,8 @@ boundary_estep(Oracle *o)
                     CPML_E_PRE(2)
                     size_t p = pidx_y(o, sb->depth_by0, i, j, k);
                     double be = bnd->y0.be[j], ce = bnd->y0.ce[j];
-                    o->f[2].pe1[p] = be * o->f[2].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dy;
-                    o->f[2].pe2[p] = be * o->f[2].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dy;
+                    if (!o->ade_cpml || xtbc) o->f[2].pe1[p] = be * o->f[2].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dy;
+                    if (!o->ade_cpml || ztbc) o->f[2].pe2[p] = be * o->f[2].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dy;
                     if (((i != 0 || o->perx) && k != 0) && xtbc == 1) o->ex[c] += dy * cbx * o->f[2].pe1[p];
                     if (((i != 0 || o->perx) && k != 0) && ztbc == 1) o->ez[c] -= dy * cbz * o->f[2].pe2[p];
                 }
@@ -888,11 +957,13 @@ boundary_estep(Oracle *o)
                     CPML_E_PRE(4)
                     size_t p = pidx_z(o, sb->depth_bz0, i, j, k);
                     double be = bnd->z0.be[k], ce = bnd->z0.ce[k];
-                    o->f[4].pe1[p] = be * o->f[4].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dz;
-                    o->f[4].pe2[p] = be * o->f[4].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dz;
+                    if (!o->ade_cpml || ytbc) o->f[4].pe1[p] = be * o->f[4].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dz;
+                    if (!o->ade_cpml || xtbc) o->f[4].pe2[p] = be * o->f[4].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dz;
                     if (((i != 0 || o->perx) && (j != 0 || o->pery)) && ytbc == 1) o->ey[c] += dx * cby * o->f[4].pe1[p];
-                    /* src3d/boundary.c:800-801 tests ztbc for the Ex correction */
-                    if (((i != 0 || o->perx) && (j != 0 || o->pery)) && ztbc == 1) o->ex[c] -= dx * cbx * o->f[4].pe2[p];
+                    /* src3d/boundary.c:800-801 tests ztbc for the Ex correction (xtbc under
+                     * ADE_CPML_CONSISTENT, where the dispersive Ex has its own correction) */
+                    if (((i != 0 || o->perx) && (j != 0 || o->pery)) && (o->ade_cpml ? xtbc : ztbc) == 1)
+                        o->ex[c] -= dx * cbx * o->f[4].pe2[p];
                 }
         }
     }
@@ -905,8 +976,8 @@ boundary_estep(Oracle *o)
                     CPML_E_PRE(1)
                     size_t p = pidx_x(o, ir, j, k);
                     double be = bnd->xn.be[ir], ce = bnd->xn.ce[ir];
-                    o->f[1].pe1[p] = be * o->f[1].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dx;
-                    o->f[1].pe2[p] = be * o->f[1].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dx;
+                    if (!o->ade_cpml || ytbc) o->f[1].pe1[p] = be * o->f[1].pe1[p] + ce * ((o->hz[c] - o->hz[c - sx])) / dx;
+                    if (!o->ade_cpml || ztbc) o->f[1].pe2[p] = be * o->f[1].pe2[p] + ce * ((o->hy[c] - o->hy[c - sx])) / dx;
                     if (((j != 0 || o->pery) && k != 0) && ytbc == 1) o->ey[c] -= dx * cby * o->f[1].pe1[p];
                     if (((j != 0 || o->pery) && k != 0) && ztbc == 1) o->ez[c] += dx * cbz * o->f[1].pe2[p];
                 }
@@ -921,8 +992,8 @@ boundary_estep(Oracle *o)
                     CPML_E_PRE(3)
                     size_t p = pidx_y(o, sb->depth_byn, i, jr, k);
                     double be = bnd->yn.be[jr], ce = bnd->yn.ce[jr];
-                    o->f[3].pe1[p] = be * o->f[3].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dy;
-                    o->f[3].pe2[p] = be * o->f[3].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dy;
+                    if (!o->ade_cpml || xtbc) o->f[3].pe1[p] = be * o->f[3].pe1[p] + ce * ((o->hz[c] - o->hz[c - sy])) / dy;
+                    if (!o->ade_cpml || ztbc) o->f[3].pe2[p] = be * o->f[3].pe2[p] + ce * ((o->hx[c] - o->hx[c - sy])) / dy;
                     if (((i != 0 || o->perx) && k != 0) && xtbc == 1) o->ex[c] += dy * cbx * o->f[3].pe1[p];
                     if (((i != 0 || o->perx) && k != 0) && ztbc == 1) o->ez[c] -= dy * cbz * o->f[3].pe2[p];
                 }
@@ -937,8 +1008,8 @@ boundary_estep(Oracle *o)
                     CPML_E_PRE(5)
                     size_t p = pidx_z(o, sb->depth_bzn, i, j, kr);
                     double be = bnd->zn.be[kr], ce = bnd->zn.ce[kr];
-                    o->f[5].pe1[p] = be * o->f[5].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dz;
-                    o->f[5].pe2[p] = be * o->f[5].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dz;
+                    if (!o->ade_cpml || ytbc) o->f[5].pe1[p] = be * o->f[5].pe1[p] + ce * ((o->hx[c] - o->hx[c - 1])) / dz;
+                    if (!o->ade_cpml || xtbc) o->f[5].pe2[p] = be * o->f[5].pe2[p] + ce * ((o->hy[c] - o->hy[c - 1])) / dz;
                     if (((i != 0 || o->perx) && (j != 0 || o->pery)) && ytbc == 1) o->ey[c] += dx * cby * o->f[5].pe1[p];
                     if (((i != 0 || o->perx) && (j != 0 || o->pery)) && xtbc == 1) o->ex[c] -= dx * cbx * o->f[5].pe2[p];
                 }
@@ -949,6 +1020,11 @@ boundary_estep(Oracle *o)
     if (smb->bx0 == SV_BOUNDARY_PERIODIC || smb->bxn == SV_BOUNDARY_PERIODIC)
         for (j = smb->by0pos; j < smb->bynpos; j++)
             for (k = smb->bz0pos; k < smb->bznpos; k++) {
+                if (o->mb_fault == 2) {
+                    o->ey[IDX(o, smb->bx0pos, j, k)] = o->ey[IDX(o, smb->bxnpos, j, k)];
+                    o->ez[IDX(o, smb->bx0pos, j, k)] = o->ez[IDX(o, smb->bxnpos, j, k)];
+                    continue;
+                }
                 o->ey[IDX(o, smb->bxnpos, j, k)] = o->ey[IDX(o, smb->bx0pos, j, k)];
                 o->ez[IDX(o, smb->bxnpos, j, k)] = o->ez[IDX(o, smb->bx0pos, j, k)];
             }
@@ -2047,7 +2123,7 @@ oracle_get_maps(void *handle, int *mat, int *bnds)
     }
 }
 
-/* test options: "no_average" (0/1) */
+/* test options: "no_average" (0/1), "mb_fault" (0, 1, 2) */
 int
 oracle_set_option(void *handle, const char *name, int value)
 {
@@ -2056,6 +2132,11 @@ oracle_set_option(void *handle, const char *name, int value)
         o->no_average = value;
         return 0;
     }
+    if (!strcmp(name, "mb_fault")) {
+        o->mb_fault = value;
+        return 0;
+    }
+
     return 1;
 }
 

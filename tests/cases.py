@@ -140,7 +140,7 @@ OBLIQUE_DT_MULT = 0.9
 
 def rough_surface(folder, n=512, nsteps=4000, dx=5e-9, depth=16, seed=2, rms_nm=20.0, corr_nm=100.0,
                   theta_deg=45.0, lam_center=600e-9, dft_lambdas=(500e-9, 600e-9, 700e-9), tsf_margin=4,
-                  nfff=True):
+                  nfff=True, ade_cpml_consistent=False):
     """Rough-surface scattering: lower half a Drude metal
     (eps_inf 1, omega_p 1.37e16 rad/s, gamma 2.7e13 1/s, ADE) below a
     Gaussian-correlated interface (rms 20 nm, correlation 100 nm); CPML 16;
@@ -166,6 +166,8 @@ def rough_surface(folder, n=512, nsteps=4000, dx=5e-9, depth=16, seed=2, rms_nm=
              "MEDIUM_VECTOR\nmetal.txt\n\n", "MEDIUM_VTK_DATA\nsurface.vtk\n\n",
              "SOURCE_TSF\n%d %d %d %d %d %d %.17g 0 %.17g 0 pulse.txt\n\n" % (b0, b0, b0, b1, b1, b1, theta, math.pi / 2),
              "TSF_SKIP\nk0\n\n"]
+    if ade_cpml_consistent:
+        lines.append("ADE_CPML_CONSISTENT\n1\n\n")
     if nfff:
         f0 = b0 + 6
         f1 = b1 - 6
@@ -482,6 +484,34 @@ def small_case(folder, kind, n=48, nsteps=160, seed=7):
             f.write("\n".join(mats) + "\n")
         lines.append("MEDIUM_VECTOR\nmats.txt\n\n")
     path = os.path.join(folder, kind + ".par")
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path
+
+
+def periodic_window(folder, n=40, nsteps=200, window=(8, 32), blocks=(), tsf=True):
+    """Interior periodic planes (MBOUNDARY_X0/XN/Y0/YN periodic at window[0],
+    window[1]) in a pool of n^3 10 nm cells with Liao faces; a normally
+    incident (+z) plane-wave pulse from a TSF box that spans the pool in x and y
+    and cells 8..n-9 in z.  `blocks`: (i0, j0, k0, i1, j1, k1, eps_r) boxes of
+    linear dielectric."""
+    os.makedirs(folder, exist_ok=True)
+    dx = 10e-9
+    dt = _dt(dx, dx, dx, 0.5)
+    write_tsf_source(os.path.join(folder, "tsf.txt"), gaussian_sine(nsteps, dt, 150e-9, 1.5, delay_widths=1.2))
+    w0, w1 = window
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (n, n, n, dx, dx, dx), "COMP\n%d\n\n" % nsteps, "DT_MULT\n0.5\n\n",
+             "BOUNDARY_ALL\nliao\n\n",
+             "MBOUNDARY_X0\nperiodic %d\n\nMBOUNDARY_XN\nperiodic %d\n\n" % (w0, w1),
+             "MBOUNDARY_Y0\nperiodic %d\n\nMBOUNDARY_YN\nperiodic %d\n\n" % (w0, w1)]
+    if tsf:
+        lines.append("SOURCE_TSF\n2 2 8 %d %d %d 0 0 0 0 tsf.txt\n\n" % (n - 3, n - 3, n - 9))
+    if blocks:
+        with open(os.path.join(folder, "blocks.txt"), "w") as f:
+            for b in blocks:
+                f.write("8 %d %d %d %d %d %d 0 %g 1 0 0\n" % b)
+        lines.append("MEDIUM_VECTOR\nblocks.txt\n\n")
+    path = os.path.join(folder, "window.par")
     with open(path, "w") as f:
         f.write("".join(lines))
     return path

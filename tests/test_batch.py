@@ -53,13 +53,12 @@ def test_sweep_run_writes_uneven_grid(tmp_path):
 
 
 def _single(paths):
-    """each run alone through the two-pass step (the default), which runs the
-    same per-cell arithmetic as the batched kernels"""
+    """each run alone, through the same per-cell column functions as the
+    batched kernels"""
     from gsvit_copy_amd import Session
     out = []
     for p in paths:
         with Session(p) as s:
-            s.set_step_mode(False)
             s.run()
             out.append(s.fields())
     return out
@@ -104,7 +103,6 @@ def test_batch_continues_single_steps_and_autotunes(tmp_path):
     sessions = [Session(p) for p in paths]
     start = [0, 17, 3, 0]                     # runs may start from different steps
     for s, n in zip(sessions, start):
-        s.set_step_mode(False)
         s.run(n)
     with Batch(sessions) as b:
         used, best = b.autotune(1)
