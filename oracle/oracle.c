@@ -52,9 +52,16 @@
  *
  * Pinning: the reference's self-test goldens and example logs pin the Yee,
  * averaging, ADE, CPML, TSF, point-source and time-domain far-field parts
- * (tests/test_oracle_goldens.py).  Parity unpinned: the interior periodic
- * copies (the only reference log with them is from an older version and
- * drifts); they follow src3d/boundary.c line by line in meaning.
+ * (tests/test_oracle_goldens.py).  The interior periodic copies (whose only
+ * reference log is from an older version and drifts) are pinned by the
+ * invariants they fix: a uniform window under normal incidence and exact
+ * translation of the window's fields with its scatterer, both of which a
+ * planted wrong copy index or direction breaks (tests/test_periodic_planes.py).
+ *
+ * ADE_CPML_CONSISTENT (no reference counterpart, off by default) replaces
+ * the reference's double psi update of dispersive components in CPML slabs
+ * by a single one applied with the ADE curl coefficient
+ * (tests/test_ade_cpml.py).
  */
 #include <stdio.h>
 #include <stdlib.h>
