@@ -30,6 +30,8 @@ void gsv_adjoint_free(Engine *E);
 int gsv_adjoint_pre_e(Engine *E, int step);
 int gsv_adjoint_post_e(Engine *E, int step);
 int gsv_adjoint_after_step(Engine *E, int step);
+int gsv_engine_dft_flush(Engine *E);   /* gpu_engine.hip: fold pending running-DFT samples */
+extern "C" int gsv_engine_farfield_dft(void *eng, int ndirs, const double *dirs, double *out);   /* nfff.hip */
 
 struct Engine {
     SvPool *mp;

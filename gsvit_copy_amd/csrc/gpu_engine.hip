@@ -1090,6 +1090,9 @@ setup_farfield(Engine *E)
             return 1;
         E->dft_first = 0;
         E->dft_pending = 0;
+        /* the far-field transform of these accumulators, for NFFF_DFT_PATTERN files */
+        mp->farfield->dft_transform = gsv_engine_farfield_dft;
+        mp->farfield->dft_ctx = E;
     }
     return 0;
 }
@@ -1167,6 +1170,12 @@ dft_flush(Engine *E)
 {
     return ring_flush(E, E->has_dft, E->dft, E->dft_first, E->dft_pending)
            || ring_flush(E, E->has_flux, E->flx, E->flx_first, E->flx_pending);
+}
+
+int
+gsv_engine_dft_flush(Engine *E)
+{
+    return dft_flush(E);
 }
 
 /* OUT_SUM boxes: the voxel selection (permittivity within EPSILON_0/100 of

@@ -1227,7 +1227,10 @@ k_pfarfield(GsvGrid g, GsvPFarfield f, int step)
 }
 
 /* running DFT, sampling half: the four tangential components of every NFFF
- * box face cell into ring slot `slot` */
+ * box face cell into ring slot `slot`.  E is taken on the face plane; H is the
+ * mean of the two planes either side of it (the cell and its neighbour along
+ * the face normal), which puts H on the face plane as well (face codes 0-5:
+ * x, y, z faces; 6: FLUX_DFT z planes) */
 __global__ __launch_bounds__(256) void
 k_dft_sample(GsvGrid g, GsvDft d, int slot)
 {
@@ -1237,10 +1240,16 @@ k_dft_sample(GsvGrid g, GsvDft d, int slot)
     const long long c = d.cell[q];
     const int face = d.face[q];
     float v0, v1, v2, v3;
-    if (face < 2) { v0 = g.ey[c]; v1 = g.ez[c]; v2 = g.hy[c]; v3 = g.hz[c]; }
-    else if (face < 4) { v0 = g.ex[c]; v1 = g.ez[c]; v2 = g.hx[c]; v3 = g.hz[c]; }
-    else if (face < 6) { v0 = g.ex[c]; v1 = g.ey[c]; v2 = g.hx[c]; v3 = g.hy[c]; }
-    else {
+    if (face < 2) {
+        v0 = g.ey[c]; v1 = g.ez[c];
+        v2 = 0.5f * (g.hy[c] + g.hy[c - 1]); v3 = 0.5f * (g.hz[c] + g.hz[c - 1]);
+    } else if (face < 4) {
+        v0 = g.ex[c]; v1 = g.ez[c];
+        v2 = 0.5f * (g.hx[c] + g.hx[c - g.px]); v3 = 0.5f * (g.hz[c] + g.hz[c - g.px]);
+    } else if (face < 6) {
+        v0 = g.ex[c]; v1 = g.ey[c];
+        v2 = 0.5f * (g.hx[c] + g.hx[c - g.plane]); v3 = 0.5f * (g.hy[c] + g.hy[c - g.plane]);
+    } else {
         v0 = g.ex[c];
         v1 = g.ey[c];
         v2 = 0.5f * (g.hx[c] + g.hx[c - g.plane]);

@@ -5,7 +5,8 @@
  * log "%d %g %g %g" of the negated bins), plane and volume outputs
  * (:712-905 volumes: ASCII "%g", VTK STRUCTURED_POINTS, raw doubles;
  * :1100-1263 planes: ASCII "%10.10g", raw doubles).  The running-DFT face data,
- * which has no reference counterpart, is written as one line per face cell.
+ * which has no reference counterpart, is written as one line per face cell,
+ * and its far-field pattern as one line per wavelength and direction.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -304,6 +305,55 @@ sv_flux_spectra(const SvPool *mp, const SvSet *set, double *out)
         }
 }
 
+/* NFFF_DFT_PATTERN: the far-field pattern F (E = F exp(-i k r) / r) of the
+ * running DFT on the requested angle grid, through the transform attached to
+ * the far-field data by the engine that holds the accumulators.  One line per
+ * wavelength and direction: lambda, theta, phi (degrees), re / im F_theta,
+ * re / im F_phi. */
+static void
+write_dft_pattern(SvPool *mp, SvSet *set)
+{
+    SvFarfield *ff = mp->farfield;
+    const int nt = set->sf.dft_pattern_ntheta, np = set->sf.dft_pattern_nphi, nd = nt * np;
+    double *dirs, *out;
+    int a, b, w;
+    FILE *fh;
+    if (!ff->dft_transform) {
+        fprintf(stderr, "Error: farfield DFT pattern: no transform available\n");
+        return;
+    }
+    dirs = (double *)malloc((size_t)nd * 2 * sizeof(double));
+    out = (double *)malloc((size_t)nd * ff->ndft * 4 * sizeof(double));
+    if (!dirs || !out) {
+        free(dirs);
+        free(out);
+        fprintf(stderr, "Error: farfield DFT pattern: out of memory\n");
+        return;
+    }
+    for (a = 0; a < nt; a++)
+        for (b = 0; b < np; b++) {
+            dirs[2 * (a * np + b)] = nt > 1 ? M_PI * a / (nt - 1) : 0.5 * M_PI;
+            dirs[2 * (a * np + b) + 1] = 2 * M_PI * b / np;
+        }
+    if (ff->dft_transform(ff->dft_ctx, nd, dirs, out)) {
+        fprintf(stderr, "Error: farfield DFT pattern: transform failed\n");
+    } else if (!(fh = fopen(set->sf.dft_pattern_filename, "w"))) {
+        fprintf(stderr, "Error: farfield DFT pattern: cannot open file %s for writing\n", set->sf.dft_pattern_filename);
+    } else {
+        fprintf(fh, "# far-field pattern of the running DFT, E = F exp(-i k r) / r: lambda theta phi (deg) "
+                    "re(F_theta) im(F_theta) re(F_phi) im(F_phi)\n");
+        for (w = 0; w < ff->ndft; w++)
+            for (a = 0; a < nd; a++) {
+                const double *o = out + ((size_t)a * ff->ndft + w) * 4;
+                fprintf(fh, "%g %g %g %g %g %g %g\n", set->sf.dft_lambda[w], dirs[2 * a] * 180 / M_PI,
+                        dirs[2 * a + 1] * 180 / M_PI, o[0], o[1], o[2], o[3]);
+            }
+        fclose(fh);
+    }
+    free(dirs);
+    free(out);
+}
+
 void
 sv_pool_output(SvPool *mp, SvSet *set)
 {
@@ -433,6 +483,8 @@ sv_pool_output(SvPool *mp, SvSet *set)
         }
         fclose(fh);
     }
+    if (set->sf.ndft && set->sf.dft_pattern_filename[0])
+        write_dft_pattern(mp, set);
     if (set->sx.nplanes && set->sx.filename[0]) {
         SvFarfield *ff = mp->farfield;
         double *fl = (double *)malloc((size_t)set->sx.nplanes * ff->nflux * sizeof(double));

@@ -35,6 +35,7 @@ int gsv_engine_batch_set_graph(void *b, int steps_per_graph);
 void gsv_engine_batch_set_timing(void *b, int on);
 void gsv_engine_batch_info(void *b, double *out);
 void gsv_engine_batch_free(void *b);
+int gsv_engine_farfield_dft(void *eng, int ndirs, const double *dirs, double *out);
 
 typedef struct {
     SvSet set;
@@ -129,6 +130,20 @@ gsv_info(void *h, int *info)
     info[7] = s->mp->nmat;
     info[8] = s->set.sc.step_act;
     gsv_engine_info(s->mp->gpu, info + 9);
+}
+
+void
+gsv_dft_dims(void *h, int *dims)
+{
+    GsvSession *s = (GsvSession *)h;
+    dims[0] = (int)s->mp->farfield->dft_ncells;
+    dims[1] = s->mp->farfield->ndft;
+}
+
+int
+gsv_farfield_dft(void *h, int ndirs, const double *dirs, double *out)
+{
+    return gsv_engine_farfield_dft(((GsvSession *)h)->mp->gpu, ndirs, dirs, out);
 }
 
 int

@@ -747,6 +747,16 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             f->ndft = n;
             if (get_rest_of_line(fr, f->dft_filename, SV_NAME_LEN))
                 goto fail;
+        } else if (!strcmp(key, "NFFF_DFT_PATTERN")) {
+            /* far-field pattern of the running DFT (no reference counterpart):
+             * NFFF_DFT_PATTERN ntheta nphi file */
+            SvSetFarfield *f = &set->sf;
+            if (get_int(fr, &f->dft_pattern_ntheta, key) || get_int(fr, &f->dft_pattern_nphi, key)
+                || f->dft_pattern_ntheta < 1 || f->dft_pattern_nphi < 1
+                || (long)f->dft_pattern_ntheta * f->dft_pattern_nphi > (1L << 22))
+                goto fail;
+            if (get_rest_of_line(fr, f->dft_pattern_filename, SV_NAME_LEN))
+                goto fail;
         } else if (!strcmp(key, "FLUX_DFT")) {
             /* flux spectra through planes (no reference counterpart):
              * FLUX_DFT i0 j0 i1 j1 n k_1 .. k_n m lambda_1 .. lambda_m file */

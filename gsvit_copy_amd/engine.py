@@ -46,6 +46,10 @@ def library():
         L.gsv_get_dft.restype = ctypes.c_long
         L.gsv_get_dft.argtypes = [vp, dp]
         L.gsv_get_maps.argtypes = [vp, ip, ip]
+        L.gsv_dft_dims.restype = None
+        L.gsv_dft_dims.argtypes = [vp, ip]
+        L.gsv_farfield_dft.restype = ctypes.c_int
+        L.gsv_farfield_dft.argtypes = [vp, ctypes.c_int, dp, dp]
         L.gsv_get_flux.restype = ctypes.c_long
         L.gsv_get_flux.argtypes = [vp, dp, dp]
         L.gsv_flux_dims.argtypes = [vp, ip]
@@ -184,6 +188,25 @@ class Session:
         if n > 0 and self.L.gsv_get_dft(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) < 0:
             raise RuntimeError("DFT download failed")
         return out[:n]
+
+    def dft_dims(self):
+        """(face cells, wavelengths) of the running DFT"""
+        d = (ctypes.c_int * 2)()
+        self.L.gsv_dft_dims(self.h, d)
+        return d[0], d[1]
+
+    def farfield_dft(self, theta, phi):
+        """far-field pattern of the running DFT at directions (theta, phi) in
+        radians: complex [ndirs, wavelengths, 2] = F_theta, F_phi with
+        E = F exp(-i k r) / r"""
+        th = np.asarray(theta, dtype=np.float64).ravel()
+        dirs = np.ascontiguousarray(np.stack([th, np.broadcast_to(np.asarray(phi, dtype=np.float64).ravel(), th.shape)], 1))
+        nw = self.dft_dims()[1]
+        out = np.zeros((th.size, max(nw, 1), 4), dtype=np.float64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        if self.L.gsv_farfield_dft(self.h, th.size, dirs.ctypes.data_as(dp), out.ctypes.data_as(dp)):
+            raise RuntimeError("far-field transform failed (see stderr)")
+        return out[..., 0::2] + 1j * out[..., 1::2]
 
     def flux(self, with_acc=False):
         """FLUX_DFT spectra [nplanes, nwavelengths] (with with_acc also the raw

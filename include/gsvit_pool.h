@@ -132,6 +132,11 @@ typedef struct {
     double *dft_omega;
     double *dft_data;
     long dft_ncells;
+    /* far-field transform of the running DFT (NFFF_DFT_PATTERN), attached by
+     * whatever holds the accumulators: dirs [ndirs][2] = theta, phi; out
+     * [ndirs][ndft][4] = re, im of F_theta and F_phi; returns 0 on success */
+    int (*dft_transform)(void *ctx, int ndirs, const double *dirs, double *out);
+    void *dft_ctx;
     /* FLUX_DFT planes (no reference counterpart): per plane cell (plane, j, i
      * order), Ex, Ey, mean Hx, mean Hy x nflux wavelengths x (re, im) */
     int nflux;

@@ -42,6 +42,17 @@ void gsv_pfarfield_dims(void *session, int *dims);
 int gsv_get_pfarfield(void *session, double *out);
 /* running-DFT accumulators; returns the element count (out may be NULL) */
 long gsv_get_dft(void *session, double *out);
+/* dims[0..1]: running-DFT face cells, wavelengths (NFFF_DFT) */
+void gsv_dft_dims(void *session, int *dims);
+/* far-field pattern of the running DFT (no reference counterpart; the
+ * frequency-domain counterpart of the time-domain far field of
+ * src3d/farfield.c:174-407): dirs [ndirs][2] = theta, phi (radians, host
+ * memory), out [ndirs][wavelengths][4] = re, im of F_theta and F_phi (host
+ * memory), E = F exp(-i k r) / r with r from the grid origin.  Surface
+ * equivalence on the NFFF box faces (J = n x H, M = E x n), NFFF_SKIP
+ * rectangles left out.  Device buffers are allocated for the call.  Returns 0
+ * on success. */
+int gsv_farfield_dft(void *session, int ndirs, const double *dirs, double *out);
 /* OUT_SUM values as [nsums][nsteps] (before the cell-volume scaling of the
  * log file); returns the element count (out may be NULL), -1 on error */
 long gsv_get_sums(void *session, double *out);

@@ -309,6 +309,11 @@ typedef struct {
     int ndft;
     double *dft_lambda;
     char dft_filename[SV_NAME_LEN];
+    /* NFFF_DFT_PATTERN ntheta nphi file (no reference counterpart): far-field
+     * pattern of the running DFT on ntheta polar angles 0..pi (inclusive) x
+     * nphi azimuths 0..2 pi (exclusive), written at the last step */
+    int dft_pattern_ntheta, dft_pattern_nphi;
+    char dft_pattern_filename[SV_NAME_LEN];
 } SvSetFarfield;
 
 typedef struct {

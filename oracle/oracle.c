@@ -1837,8 +1837,10 @@ getsum(Oracle *o)
 /* running DFT of the tangential fields on the NFFF box faces (no reference
  * counterpart): per face cell and frequency w, acc += F(t) exp(-i w t) for
  * the two tangential E components at t = (step+1) dt and the two tangential
- * H components at t = (step+1/2) dt.  Cell order: x faces (j, k loops; i0 then
- * in), y faces (i, k), z faces (i, j); 8 doubles per cell and frequency. */
+ * H components at t = (step+1/2) dt, H as the mean of the cell and its
+ * neighbour below along the face normal (so that it lies on the face plane
+ * with E).  Cell order: x faces (j, k loops; i0 then in), y faces (i, k),
+ * z faces (i, j); 8 doubles per cell and frequency. */
 static void
 running_dft(Oracle *o)
 {
@@ -1847,6 +1849,7 @@ running_dft(Oracle *o)
     const int i0 = sf->box_i0, i1 = sf->box_in, j0 = sf->box_j0, j1 = sf->box_jn, k0 = sf->box_k0, k1 = sf->box_kn;
     const int ix = i1 - i0, iy = j1 - j0, iz = k1 - k0;
     const double te = (o->set.sc.step_act + 1) * o->set.plan.dt, th = (o->set.sc.step_act + 0.5) * o->set.plan.dt;
+    const size_t sxs = (size_t)o->ny * o->nz, sys = o->nz;
     long q;
     if (!ff->ndft)
         return;
@@ -1859,17 +1862,20 @@ running_dft(Oracle *o)
         if (r < (long)iy * iz) {
             j = j0 + (int)(r / iz); k = k0 + (int)(r % iz); i = far ? i1 : i0;
             c = IDX(o, i, j, k);
-            v[0] = o->ey[c]; v[1] = o->ez[c]; v[2] = o->hy[c]; v[3] = o->hz[c];
+            v[0] = o->ey[c]; v[1] = o->ez[c];
+            v[2] = 0.5 * ((double)o->hy[c] + o->hy[c - sxs]); v[3] = 0.5 * ((double)o->hz[c] + o->hz[c - sxs]);
         } else if (r < (long)iy * iz + (long)ix * iz) {
             r -= (long)iy * iz;
             i = i0 + (int)(r / iz); k = k0 + (int)(r % iz); j = far ? j1 : j0;
             c = IDX(o, i, j, k);
-            v[0] = o->ex[c]; v[1] = o->ez[c]; v[2] = o->hx[c]; v[3] = o->hz[c];
+            v[0] = o->ex[c]; v[1] = o->ez[c];
+            v[2] = 0.5 * ((double)o->hx[c] + o->hx[c - sys]); v[3] = 0.5 * ((double)o->hz[c] + o->hz[c - sys]);
         } else {
             r -= (long)iy * iz + (long)ix * iz;
             i = i0 + (int)(r / iy); j = j0 + (int)(r % iy); k = far ? k1 : k0;
             c = IDX(o, i, j, k);
-            v[0] = o->ex[c]; v[1] = o->ey[c]; v[2] = o->hx[c]; v[3] = o->hy[c];
+            v[0] = o->ex[c]; v[1] = o->ey[c];
+            v[2] = 0.5 * ((double)o->hx[c] + o->hx[c - 1]); v[3] = 0.5 * ((double)o->hy[c] + o->hy[c - 1]);
         }
         for (w = 0; w < ff->ndft; w++) {
             double *a = ff->dft_data + ((size_t)q * ff->ndft + w) * 8;
@@ -2095,6 +2101,15 @@ oracle_get_sums(void *handle, double *out)
         memcpy(out + (size_t)n * o->set.sc.nsteps, o->mp->out->outsumdata[n], (size_t)o->set.sc.nsteps * sizeof(double));
 }
 
+/* dims[0..1]: running-DFT face cells, wavelengths */
+void
+oracle_dft_dims(void *handle, int *dims)
+{
+    Oracle *o = (Oracle *)handle;
+    dims[0] = (int)o->mp->farfield->dft_ncells;
+    dims[1] = o->mp->farfield->ndft;
+}
+
 /* running-DFT accumulators [ncells][ndft][8]; returns the element count (out may be NULL) */
 long
 oracle_get_dft(void *handle, double *out)
@@ -2105,6 +2120,127 @@ oracle_get_dft(void *handle, double *out)
     if (out && cnt)
         memcpy(out, ff->dft_data, (size_t)cnt * sizeof(double));
     return cnt;
+}
+
+/* Far field of the running DFT (NFFF_DFT; no reference counterpart, the
+ * frequency-domain counterpart of src3d/farfield.c:174-407).  Surface
+ * equivalence on the NFFF box: J = n x H and M = E x n (n the outward normal)
+ * from the accumulated tangential E and face-plane H, each component at its
+ * own Yee position on the face (x faces: Ey and Hz at (p, j+1/2, k), Ez and Hy
+ * at (p, j, k+1/2); y faces: Ex, Hz at (i+1/2, p, k), Ez, Hx at (i, p, k+1/2);
+ * z faces: Ex, Hy at (i+1/2, j, p), Ey, Hx at (i, j+1/2, p)), times dt (the
+ * DFT sums approximate Fourier integrals) and the cell area; radiation
+ * vectors N = sum J exp(+i k r.r') dA, L = sum M exp(+i k r.r') dA with r'
+ * from the grid origin; far-field pattern (E = F exp(-i k r) / r, fields
+ * ~ exp(+i w t)):
+ *   F_theta = -i k / (4 pi) (L_phi + eta0 N_theta),
+ *   F_phi   = +i k / (4 pi) (L_theta - eta0 N_phi).
+ * Face cells inside an NFFF_SKIP rectangle are left out, as in the time-domain
+ * transform.  dirs[d][2] = theta, phi; out[d][w][4] = Re F_theta, Im F_theta,
+ * Re F_phi, Im F_phi. */
+int
+oracle_farfield_dft(void *handle, int ndirs, const double *dirs, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    const SvSetFarfield *sf = &o->set.sf;
+    SvFarfield *ff = o->mp->farfield;
+    const int i0 = sf->box_i0, i1 = sf->box_in, j0 = sf->box_j0, j1 = sf->box_jn, k0 = sf->box_k0, k1 = sf->box_kn;
+    const int ix = i1 - i0, iy = j1 - j0, iz = k1 - k0;
+    const double dx = o->set.sp.dx, dy = o->set.sp.dy, dz = o->set.sp.dz, dt = o->set.plan.dt;
+    const double eta0 = SV_MU_0 * SV_LIGHT_SPEED;
+    int d;
+    if (!ff->ndft)
+        return 1;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (d = 0; d < ndirs; d++) {
+        const double st = sin(dirs[2 * d]), ct = cos(dirs[2 * d]), sp = sin(dirs[2 * d + 1]), cp = cos(dirs[2 * d + 1]);
+        const double rh[3] = { st * cp, st * sp, ct };
+        const double th[3] = { ct * cp, ct * sp, -st }, ph[3] = { -sp, cp, 0.0 };
+        int w;
+        for (w = 0; w < ff->ndft; w++) {
+            const double kk = ff->dft_omega[w] / SV_LIGHT_SPEED;
+            double N[3][2] = { { 0 } }, L[3][2] = { { 0 } };
+            long q;
+            for (q = 0; q < ff->dft_ncells; q++) {
+                long r = q / 2;
+                const int far = (int)(q % 2);
+                const double s = far ? 1.0 : -1.0;
+                const double *a = ff->dft_data + ((size_t)q * ff->ndft + w) * 8;
+                double pa[3], pb[3], area;
+                int i, j, k, skip, axis, ca, cb;  /* A pairs (J_ca from v3, M from v0), B pairs (J_cb from v2, M from v1) */
+                double sJa, sMa, sJb, sMb;
+                int cMa, cMb;
+                if (r < (long)iy * iz) {
+                    j = j0 + (int)(r / iz); k = k0 + (int)(r % iz); i = far ? i1 : i0;
+                    skip = far ? (sf->box_boundary_skipin && j >= sf->skipin_jmin && k >= sf->skipin_kmin && j <= sf->skipin_jmax && k <= sf->skipin_kmax)
+                               : (sf->box_boundary_skipi0 && j >= sf->skipi0_jmin && k >= sf->skipi0_kmin && j <= sf->skipi0_jmax && k <= sf->skipi0_kmax);
+                    axis = 0;
+                    pa[0] = i; pa[1] = j + 0.5; pa[2] = k;      /* Ey, Hz */
+                    pb[0] = i; pb[1] = j; pb[2] = k + 0.5;      /* Ez, Hy */
+                    area = dy * dz;
+                    ca = 1; sJa = -s; cMa = 2; sMa = -s;         /* J_y = -s Hz, M_z = -s Ey */
+                    cb = 2; sJb = s; cMb = 1; sMb = s;           /* J_z = s Hy,  M_y = s Ez */
+                } else if (r < (long)iy * iz + (long)ix * iz) {
+                    r -= (long)iy * iz;
+                    i = i0 + (int)(r / iz); k = k0 + (int)(r % iz); j = far ? j1 : j0;
+                    skip = far ? (sf->box_boundary_skipjn && i >= sf->skipjn_imin && k >= sf->skipjn_kmin && i <= sf->skipjn_imax && k <= sf->skipjn_kmax)
+                               : (sf->box_boundary_skipj0 && i >= sf->skipj0_imin && k >= sf->skipj0_kmin && i <= sf->skipj0_imax && k <= sf->skipj0_kmax);
+                    axis = 1;
+                    pa[0] = i + 0.5; pa[1] = j; pa[2] = k;      /* Ex, Hz */
+                    pb[0] = i; pb[1] = j; pb[2] = k + 0.5;      /* Ez, Hx */
+                    area = dx * dz;
+                    ca = 0; sJa = s; cMa = 2; sMa = s;           /* J_x = s Hz,  M_z = s Ex */
+                    cb = 2; sJb = -s; cMb = 0; sMb = -s;         /* J_z = -s Hx, M_x = -s Ez */
+                } else {
+                    r -= (long)iy * iz + (long)ix * iz;
+                    i = i0 + (int)(r / iy); j = j0 + (int)(r % iy); k = far ? k1 : k0;
+                    skip = far ? (sf->box_boundary_skipkn && i >= sf->skipkn_imin && j >= sf->skipkn_jmin && i <= sf->skipkn_imax && j <= sf->skipkn_jmax)
+                               : (sf->box_boundary_skipk0 && i >= sf->skipk0_imin && j >= sf->skipk0_jmin && i <= sf->skipk0_imax && j <= sf->skipk0_jmax);
+                    axis = 2;
+                    pa[0] = i + 0.5; pa[1] = j; pa[2] = k;      /* Ex, Hy */
+                    pb[0] = i; pb[1] = j + 0.5; pb[2] = k;      /* Ey, Hx */
+                    area = dx * dy;
+                    ca = 0; sJa = -s; cMa = 1; sMa = -s;         /* J_x = -s Hy, M_y = -s Ex */
+                    cb = 1; sJb = s; cMb = 0; sMb = s;           /* J_y = s Hx,  M_x = s Ey */
+                }
+                (void)axis;
+                if (skip)
+                    continue;
+                {
+                    const double fa = kk * (rh[0] * pa[0] * dx + rh[1] * pa[1] * dy + rh[2] * pa[2] * dz);
+                    const double fb = kk * (rh[0] * pb[0] * dx + rh[1] * pb[1] * dy + rh[2] * pb[2] * dz);
+                    const double ea_r = cos(fa) * area * dt, ea_i = sin(fa) * area * dt;
+                    const double eb_r = cos(fb) * area * dt, eb_i = sin(fb) * area * dt;
+                    /* value (re, im) x exp(i f): v3 (H) and v0 (E) at A, v2 (H) and v1 (E) at B */
+                    N[ca][0] += sJa * (a[6] * ea_r - a[7] * ea_i);
+                    N[ca][1] += sJa * (a[6] * ea_i + a[7] * ea_r);
+                    L[cMa][0] += sMa * (a[0] * ea_r - a[1] * ea_i);
+                    L[cMa][1] += sMa * (a[0] * ea_i + a[1] * ea_r);
+                    N[cb][0] += sJb * (a[4] * eb_r - a[5] * eb_i);
+                    N[cb][1] += sJb * (a[4] * eb_i + a[5] * eb_r);
+                    L[cMb][0] += sMb * (a[2] * eb_r - a[3] * eb_i);
+                    L[cMb][1] += sMb * (a[2] * eb_i + a[3] * eb_r);
+                }
+            }
+            {
+                double Nt[2], Np[2], Lt[2], Lp[2], At[2], Ap[2], f = kk / (4 * M_PI);
+                int c;
+                for (c = 0; c < 2; c++) {
+                    Nt[c] = N[0][c] * th[0] + N[1][c] * th[1] + N[2][c] * th[2];
+                    Np[c] = N[0][c] * ph[0] + N[1][c] * ph[1] + N[2][c] * ph[2];
+                    Lt[c] = L[0][c] * th[0] + L[1][c] * th[1] + L[2][c] * th[2];
+                    Lp[c] = L[0][c] * ph[0] + L[1][c] * ph[1] + L[2][c] * ph[2];
+                    At[c] = Lp[c] + eta0 * Nt[c];     /* F_theta = -i f At */
+                    Ap[c] = Lt[c] - eta0 * Np[c];     /* F_phi   = +i f Ap */
+                }
+                out[((size_t)d * ff->ndft + w) * 4 + 0] = f * At[1];
+                out[((size_t)d * ff->ndft + w) * 4 + 1] = -f * At[0];
+                out[((size_t)d * ff->ndft + w) * 4 + 2] = -f * Ap[1];
+                out[((size_t)d * ff->ndft + w) * 4 + 3] = f * Ap[0];
+            }
+        }
+    }
+    return 0;
 }
 
 /* FLUX_DFT accumulators [cell][nflux][8] (out may be NULL); returns the count */

@@ -44,6 +44,10 @@ def lib(precision="f64"):
         L.oracle_get_flux.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oracle_objective.restype = ctypes.c_double
         L.oracle_objective.argtypes = [ctypes.c_void_p]
+        L.oracle_farfield_dft.restype = ctypes.c_int
+        L.oracle_farfield_dft.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]
+        L.oracle_dft_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.oracle_real_size.restype = ctypes.c_int
         L.oracle_set_option.restype = ctypes.c_int
         L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
@@ -158,6 +162,22 @@ class Oracle:
         if n:
             self.L.oracle_get_dft(self.h, _dptr(out))
         return out[:n]
+
+    def dft_dims(self):
+        """(face cells, wavelengths) of the running DFT"""
+        d = (ctypes.c_int * 2)()
+        self.L.oracle_dft_dims(self.h, d)
+        return d[0], d[1]
+
+    def farfield_dft(self, theta, phi):
+        """far-field pattern of the running DFT at directions (theta, phi):
+        complex [ndirs, nwavelengths, 2] (F_theta, F_phi)"""
+        th = np.asarray(theta, dtype=np.float64).ravel()
+        dirs = np.ascontiguousarray(np.stack([th, np.broadcast_to(np.asarray(phi, dtype=np.float64).ravel(), th.shape)], 1))
+        out = np.zeros((th.size, max(self.dft_dims()[1], 1), 4), dtype=np.float64)
+        if self.L.oracle_farfield_dft(self.h, th.size, _dptr(dirs), _dptr(out)):
+            raise RuntimeError("no NFFF_DFT accumulators")
+        return out[..., 0::2] + 1j * out[..., 1::2]
 
     def flux_acc(self):
         """FLUX_DFT accumulators, flat [cell][wavelength][8]."""

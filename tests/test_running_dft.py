@@ -2,9 +2,11 @@
 
 The oracle's accumulators are pinned against an independent computation: a
 numpy DFT of the point traces that the oracle records at the same face cells
-(E sampled at (s+1) dt, H at (s+1/2) dt after step s).  A wrong time offset,
-exponent sign, component choice or cell ordering fails this check.  The GPU
-test then compares the engine's accumulators with the oracle's.
+and at their neighbours below along the face normal (E sampled at (s+1) dt on
+the face cell, H at (s+1/2) dt as the mean of the two cells, which puts it on
+the face plane).  A wrong time offset, exponent sign, component choice, cell
+ordering or neighbour fails this check.  The GPU test then compares the
+engine's accumulators with the oracle's.
 """
 import os
 
@@ -21,11 +23,18 @@ FACE_POINTS = [(12, 20, 30, 0), (36, 25, 17, 1), (20, 12, 30, 2), (30, 36, 22, 3
 TANGENTIAL = {0: (1, 2, 4, 5), 1: (1, 2, 4, 5), 2: (0, 2, 3, 5), 3: (0, 2, 3, 5), 4: (0, 1, 3, 4), 5: (0, 1, 3, 4)}
 
 
+def below(i, j, k, face):
+    """the neighbour along the face normal whose H is averaged with the cell's"""
+    return (i - 1, j, k) if face < 2 else (i, j - 1, k) if face < 4 else (i, j, k - 1)
+
+
 def dft_case(folder):
     par = cases.small_case(folder, "farfield_cpml", n=N)
     with open(par, "a") as f:
         for q, (i, j, k, _) in enumerate(FACE_POINTS):
             f.write("OUT_POINT\nAll 1 %d %d %d face%d\n\n" % (i, j, k, q))
+        for q, (i, j, k, face) in enumerate(FACE_POINTS):
+            f.write("OUT_POINT\nAll 1 %d %d %d below%d\n\n" % (below(i, j, k, face) + (q,)))
     return par
 
 
@@ -51,7 +60,9 @@ def test_oracle_dft_equals_numpy_dft_of_traces(tmp_path):
     o.step()
     inf = o.info()
     nsteps = inf["nsteps"]
-    tr = o.points()[-len(FACE_POINTS):]
+    npt = len(FACE_POINTS)
+    allp = o.points()
+    tr, trb = allp[-2 * npt:-npt], allp[-npt:]
     acc = o.dft().reshape(-1, 2, 8)
     o.close()
     dt = dt_of(par)
@@ -64,7 +75,8 @@ def test_oracle_dft_equals_numpy_dft_of_traces(tmp_path):
             want = []
             for slot, comp in enumerate(TANGENTIAL[face]):
                 t = (s + 1.0) * dt if comp < 3 else (s + 0.5) * dt
-                z = np.sum(tr[q, :nsteps, comp] * np.exp(-1j * om * t))
+                v = tr[q, :nsteps, comp] if comp < 3 else 0.5 * (tr[q, :nsteps, comp] + trb[q, :nsteps, comp])
+                z = np.sum(v * np.exp(-1j * om * t))
                 want += [z.real, z.imag]
             want = np.array(want)
             assert np.linalg.norm(want) > 0
