@@ -32,6 +32,7 @@ int gsv_adjoint_post_e(Engine *E, int step);
 int gsv_adjoint_after_step(Engine *E, int step);
 int gsv_engine_dft_flush(Engine *E);   /* gpu_engine.hip: fold pending running-DFT samples */
 extern "C" int gsv_engine_farfield_dft(void *eng, int ndirs, const double *dirs, double *out);   /* nfff.hip */
+extern "C" void *gsv_session_engine(void *session);   /* session.c: the engine of a session */
 
 struct Engine {
     SvPool *mp;

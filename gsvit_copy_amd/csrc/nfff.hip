@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
+#include <mutex>
 #include "engine_host.h"
 
 #define NFFF_THREADS 256   /* directions per block */
@@ -293,4 +294,70 @@ gsv_engine_farfield_dft(void *eng, int ndirs, const double *dirs, double *out)
     hipFree(d_part);
     hipFree(d_out);
     return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* the same transform without allocation                              */
+/* ------------------------------------------------------------------ */
+
+/* Partial sums of gsv_farfield_dft_noalloc: a fixed 16 MiB per GPU, held
+ * until the process ends.  Its host-side name is not a device address; the
+ * address comes from hipGetSymbolAddress once per device. */
+#define NFFF_SCRATCH_DOUBLES (2u << 20)
+static __device__ __align__(256) double scratch[NFFF_SCRATCH_DOUBLES];
+
+#define NFFF_MAX_DEVICES 64
+static double *scratch_ptr[NFFF_MAX_DEVICES];
+static std::mutex scratch_mutex;
+
+/* the device address of this device's scratch array (the device current) */
+static double *
+scratch_on(int dev)
+{
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (dev < 0 || dev >= NFFF_MAX_DEVICES)
+        return NULL;
+    if (!scratch_ptr[dev]) {
+        void *p = NULL;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            fprintf(stderr, "Error: far-field DFT: no scratch address on device %d (%s)\n", dev, hipGetErrorString(e));
+            return NULL;
+        }
+        scratch_ptr[dev] = (double *)p;
+    }
+    return scratch_ptr[dev];
+}
+
+/* As gsv_farfield_dft for callers that cannot allocate: dirs [ndirs][2] and
+ * out [ndirs][nw][4] are device memory; the partial sums go to the fixed
+ * per-GPU scratch array, so ndirs x wavelengths may not exceed
+ * NFFF_SCRATCH_DOUBLES / 12.  Calls on one GPU share that array: calls that
+ * overlap in time on one GPU race.  Runs on the session's stream and
+ * synchronises it before returning. */
+extern "C" int
+gsv_farfield_dft_noalloc(void *session, int ndirs, const double *dirs, double *out)
+{
+    Engine *E = (Engine *)gsv_session_engine(session);
+    if (!E || !E->ready || !E->has_dft) {
+        fprintf(stderr, "Error: far-field DFT: the pool has no NFFF_DFT accumulators\n");
+        return 1;
+    }
+    if (ndirs <= 0)
+        return ndirs < 0;
+    const int nw = E->dft.nw;
+    const long long nchunks = nfff_chunks(E->dft.ncells, ndirs, nw, NFFF_SCRATCH_DOUBLES);
+    if (nchunks < 1) {
+        fprintf(stderr, "Error: far-field DFT without allocation: %d directions x %d wavelengths exceed the "
+                        "scratch limit of %u (directions x wavelengths)\n", ndirs, nw, NFFF_SCRATCH_DOUBLES / 12);
+        return 1;
+    }
+    CK(hipSetDevice(E->dev));
+    double *part = scratch_on(E->dev);
+    if (!part)
+        return 1;
+    if (gsv_engine_dft_flush(E) || nfff_launch(E, dirs, ndirs, part, nchunks, out))
+        return 1;
+    CK(hipStreamSynchronize(E->st));
+    return 0;
 }

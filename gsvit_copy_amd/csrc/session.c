@@ -140,6 +140,12 @@ gsv_dft_dims(void *h, int *dims)
     dims[1] = s->mp->farfield->ndft;
 }
 
+void *
+gsv_session_engine(void *h)
+{
+    return h ? ((GsvSession *)h)->mp->gpu : NULL;
+}
+
 int
 gsv_farfield_dft(void *h, int ndirs, const double *dirs, double *out)
 {

@@ -53,6 +53,15 @@ void gsv_dft_dims(void *session, int *dims);
  * rectangles left out.  Device buffers are allocated for the call.  Returns 0
  * on success. */
 int gsv_farfield_dft(void *session, int ndirs, const double *dirs, double *out);
+/* gsv_farfield_dft without any allocation -- no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  dirs
+ * and out are DEVICE memory of the session's GPU, same layouts.  The partial
+ * sums use a fixed per-GPU scratch array of 16 MiB, so ndirs x wavelengths
+ * must not exceed 174762 (larger calls return 1 with a message).  Calls on
+ * one GPU share that array: calls that overlap in time on one GPU race.
+ * Synchronises the session's stream before returning; 0 on success. */
+int gsv_farfield_dft_noalloc(void *session, int ndirs, const double *dirs, double *out);
 /* OUT_SUM values as [nsums][nsteps] (before the cell-volume scaling of the
  * log file); returns the element count (out may be NULL), -1 on error */
 long gsv_get_sums(void *session, double *out);
