@@ -3,6 +3,7 @@
 Produces, in-tree:
   gsvit_copy_amd/lib/libgsvit_amd.so   C-ABI library (host C + HIP kernels for gfx950)
   gsvit_copy_amd/bin/gsvit_amd         command-line driver (parameter file in, logs out)
+  gsvit_copy_amd/lib/libgsv_probe.so   bench.py's streaming-copy probe (tooling, not the engine)
 and, for the test suite, the CPU oracle under oracle/_build/ (make).
 All code uses the ROCm LLVM OpenMP runtime so that one process can hold the
 library and the oracle together.
@@ -64,6 +65,17 @@ def build_library(verbose=False):
     return lib
 
 
+def build_probe(verbose=False):
+    """bench.py's measurement tooling (not part of the engine): the streaming
+    copy probe of bench_probe.hip as gsvit_copy_amd/lib/libgsv_probe.so"""
+    lib = os.path.join(PKG, "lib", "libgsv_probe.so")
+    os.makedirs(os.path.dirname(lib), exist_ok=True)
+    _run([HIPCC] + HIP_FLAGS + ["-shared", os.path.join(REPO, "bench_probe.hip"), "-o", lib])
+    if verbose:
+        print("built", lib)
+    return lib
+
+
 def build_oracle(verbose=False):
     _run(["make", "-s", "-B", "-C", os.path.join(REPO, "oracle"), "CC=" + CLANG,
           "LDFLAGS_OMP=-L%s -Wl,-rpath,%s" % (OMPLIB, OMPLIB), "all"])
@@ -73,6 +85,7 @@ def build_oracle(verbose=False):
 
 def build(verbose=False):
     build_library(verbose)
+    build_probe(verbose)
     build_oracle(verbose)
 
 

@@ -1730,52 +1730,6 @@ gsv_launch_tsf_faces_batch(const GsvGrid *grids, const GsvTsf *tsfs, const GsvFa
  * FETCH_SIZE / WRITE_SIZE counters and to measure attainable bandwidth    */
 /* ------------------------------------------------------------------ */
 
-__global__ __launch_bounds__(256) void
-k_copy_probe(const float4 *__restrict__ a, float4 *__restrict__ b, long long n)
-{
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (; q < n; q += stride)
-        b[q] = a[q];
-}
-
-extern "C" int
-gsv_copy_probe(long long nbytes, int reps, double *ms_per_copy)
-{
-    float4 *a = NULL, *b = NULL;
-    hipEvent_t e0, e1;
-    hipStream_t st;
-    float ms = 0;
-    const long long n = nbytes / 16;
-    if (n <= 0 || reps <= 0)
-        return 1;
-    if (hipMalloc(&a, n * 16) != hipSuccess)
-        return 1;
-    if (hipMalloc(&b, n * 16) != hipSuccess) {
-        hipFree(a);
-        return 1;
-    }
-    hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-    hipMemsetAsync(a, 0, n * 16, st);
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    const unsigned blocks = 256 * 32;
-    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, st, a, b, n);
-    hipEventRecord(e0, st);
-    for (int r = 0; r < reps; r++)
-        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, st, a, b, n);
-    hipEventRecord(e1, st);
-    hipEventSynchronize(e1);
-    hipEventElapsedTime(&ms, e0, e1);
-    *ms_per_copy = ms / reps;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipStreamDestroy(st);
-    hipFree(a);
-    hipFree(b);
-    return hipGetLastError() != hipSuccess;
-}
-
 extern "C" hipError_t
 gsv_launch_box_sum(const GsvGrid *g, const GsvBoxSum *b, int step, hipStream_t st)
 {

@@ -247,8 +247,11 @@ plan_dt(const SvSet *set)
                                        + 1.0 / set->sp.dz / set->sp.dz) * set->sc.dtmult;
 }
 
+/* one face's boundary keyword; `spacing` is the cell size the default CPML
+ * sigma uses: dx for BOUNDARY_ALL and the x faces, dy for the y faces, dz for
+ * the z faces (src3d/settings.c:5151, 5186, 5216, 5246, 5277, 5307, 5337) */
 static int
-parse_cpml_face(FILE *fr, const char *key, const SvSet *set, int *type, int *depth, int *m,
+parse_cpml_face(FILE *fr, const char *key, double spacing, int *type, int *depth, int *m,
                 double *sigma, double *a, double *kappa)
 {
     char value[64];
@@ -268,9 +271,9 @@ parse_cpml_face(FILE *fr, const char *key, const SvSet *set, int *type, int *dep
         *type = SV_BOUNDARY_CPML;
         if (get_int(fr, depth, key) || get_int(fr, m, key) || get_double(fr, sigma, key))
             return 1;
-        /* src3d/settings.c:5150-5154: sigma == -1 selects 0.8 m / (377 dx) */
+        /* sigma == -1 selects 0.8 m / (377 spacing) */
         if (*sigma == -1)
-            *sigma = 0.8 * (double)(*m) / (377 * set->sp.dx);
+            *sigma = 0.8 * (double)(*m) / (377 * spacing);
         if (get_double(fr, a, key) || get_double(fr, kappa, key))
             return 1;
     } else {
@@ -610,7 +613,7 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             skip_output_keyword(fr, key);
         } else if (!strcmp(key, "BOUNDARY_ALL")) {
             SvSetBoundary *b = &set->sb;
-            if (parse_cpml_face(fr, key, set, &b->bx0, &b->depth_bx0, &b->m_bx0, &b->sigma_bx0, &b->a_bx0, &b->kappa_bx0))
+            if (parse_cpml_face(fr, key, set->sp.dx, &b->bx0, &b->depth_bx0, &b->m_bx0, &b->sigma_bx0, &b->a_bx0, &b->kappa_bx0))
                 goto fail;
             b->bxn = b->by0 = b->byn = b->bz0 = b->bzn = b->bx0;
             b->depth_bxn = b->depth_by0 = b->depth_byn = b->depth_bz0 = b->depth_bzn = b->depth_bx0;
@@ -624,27 +627,27 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
                 goto fail;
         } else if (!strcmp(key, "BOUNDARY_X0")) {
             SvSetBoundary *b = &set->sb;
-            if (parse_cpml_face(fr, key, set, &b->bx0, &b->depth_bx0, &b->m_bx0, &b->sigma_bx0, &b->a_bx0, &b->kappa_bx0))
+            if (parse_cpml_face(fr, key, set->sp.dx, &b->bx0, &b->depth_bx0, &b->m_bx0, &b->sigma_bx0, &b->a_bx0, &b->kappa_bx0))
                 goto fail;
         } else if (!strcmp(key, "BOUNDARY_XN")) {
             SvSetBoundary *b = &set->sb;
-            if (parse_cpml_face(fr, key, set, &b->bxn, &b->depth_bxn, &b->m_bxn, &b->sigma_bxn, &b->a_bxn, &b->kappa_bxn))
+            if (parse_cpml_face(fr, key, set->sp.dx, &b->bxn, &b->depth_bxn, &b->m_bxn, &b->sigma_bxn, &b->a_bxn, &b->kappa_bxn))
                 goto fail;
         } else if (!strcmp(key, "BOUNDARY_Y0")) {
             SvSetBoundary *b = &set->sb;
-            if (parse_cpml_face(fr, key, set, &b->by0, &b->depth_by0, &b->m_by0, &b->sigma_by0, &b->a_by0, &b->kappa_by0))
+            if (parse_cpml_face(fr, key, set->sp.dy, &b->by0, &b->depth_by0, &b->m_by0, &b->sigma_by0, &b->a_by0, &b->kappa_by0))
                 goto fail;
         } else if (!strcmp(key, "BOUNDARY_YN")) {
             SvSetBoundary *b = &set->sb;
-            if (parse_cpml_face(fr, key, set, &b->byn, &b->depth_byn, &b->m_byn, &b->sigma_byn, &b->a_byn, &b->kappa_byn))
+            if (parse_cpml_face(fr, key, set->sp.dy, &b->byn, &b->depth_byn, &b->m_byn, &b->sigma_byn, &b->a_byn, &b->kappa_byn))
                 goto fail;
         } else if (!strcmp(key, "BOUNDARY_Z0")) {
             SvSetBoundary *b = &set->sb;
-            if (parse_cpml_face(fr, key, set, &b->bz0, &b->depth_bz0, &b->m_bz0, &b->sigma_bz0, &b->a_bz0, &b->kappa_bz0))
+            if (parse_cpml_face(fr, key, set->sp.dz, &b->bz0, &b->depth_bz0, &b->m_bz0, &b->sigma_bz0, &b->a_bz0, &b->kappa_bz0))
                 goto fail;
         } else if (!strcmp(key, "BOUNDARY_ZN")) {
             SvSetBoundary *b = &set->sb;
-            if (parse_cpml_face(fr, key, set, &b->bzn, &b->depth_bzn, &b->m_bzn, &b->sigma_bzn, &b->a_bzn, &b->kappa_bzn))
+            if (parse_cpml_face(fr, key, set->sp.dz, &b->bzn, &b->depth_bzn, &b->m_bzn, &b->sigma_bzn, &b->a_bzn, &b->kappa_bzn))
                 goto fail;
         } else if (!strncmp(key, "MBOUNDARY_", 10)) {
             /* src3d/settings.c:5047-5128 */

@@ -204,11 +204,13 @@ int sv_pool_gpu_apply_source_hstep(SvPool *mp, SvSet *set);
 int sv_pool_gpu_apply_source_estep(SvPool *mp, SvSet *set);
 /* src3d/pool.h:113 -- far-field accumulation (time-domain bins and running DFT) */
 int sv_pool_gpu_apply_farfield(SvPool *mp, SvSet *set);
-/* src3d/pool.h:119 -- periodic far field: not supported, returns 0 when unused */
+/* src3d/pool.h:119 -- periodic far field (PERIODIC_NFFF, src3d/pfarfield.c): Kirchhoff bins over the
+ * z planes and their periodic images, both modes; returns 0 when unused */
 int sv_pool_gpu_apply_pfarfield(SvPool *mp, SvSet *set);
 /* src3d/pool.h:114 -- record point traces */
 int sv_pool_gpu_getpoints(SvPool *mp, SvSet *set);
-/* src3d/pool.h:115 -- Maxwell-stress force: not supported, returns 0 when unused */
+/* src3d/pool.h:115 -- Maxwell-stress force boxes (OUT_FORCE, src3d/output.c:355-512): face sums of the
+ * stress tensor per step; returns 0 when unused */
 int sv_pool_gpu_getforce(SvPool *mp, SvSet *set);
 
 #ifdef __cplusplus

@@ -133,10 +133,6 @@ int gsv_batch_autotune(void *batch, int steps_per_candidate, int *best);
 void gsv_batch_set_timing(void *batch, int on);
 void gsv_batch_info(void *batch, double *out);
 void gsv_batch_close(void *batch);
-/* streaming copy of nbytes (read nbytes + write nbytes) repeated reps times on
- * the current device; *ms_per_copy = mean time of one copy.  Calibrates HBM
- * traffic counters and attainable bandwidth.  Returns 0 on success. */
-int gsv_copy_probe(long long nbytes, int reps, double *ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -2101,6 +2101,37 @@ oracle_get_sums(void *handle, double *out)
         memcpy(out + (size_t)n * o->set.sc.nsteps, o->mp->out->outsumdata[n], (size_t)o->set.sc.nsteps * sizeof(double));
 }
 
+/* the planner's CPML vectors of face f (0..5 = x0, xn, y0, yn, z0, zn) as
+ * out[6][depth] = be, ce, bh, ch, kappae, kappah; returns the depth (0 when the
+ * face has no CPML; out may be NULL) */
+int
+oracle_cpml(void *handle, int f, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    const SvBoundary *b = o->mp->bnd;
+    const SvCpmlFace *F[6] = { &b->x0, &b->xn, &b->y0, &b->yn, &b->z0, &b->zn };
+    int i;
+    if (f < 0 || f > 5 || !F[f]->be)
+        return 0;
+    if (out)
+        for (i = 0; i < F[f]->depth; i++) {
+            out[i] = F[f]->be[i];
+            out[F[f]->depth + i] = F[f]->ce[i];
+            out[2 * F[f]->depth + i] = F[f]->bh[i];
+            out[3 * F[f]->depth + i] = F[f]->ch[i];
+            out[4 * F[f]->depth + i] = F[f]->kappae[i];
+            out[5 * F[f]->depth + i] = F[f]->kappah[i];
+        }
+    return F[f]->depth;
+}
+
+/* the planner's time step */
+double
+oracle_dt(void *handle)
+{
+    return ((Oracle *)handle)->set.plan.dt;
+}
+
 /* dims[0..1]: running-DFT face cells, wavelengths */
 void
 oracle_dft_dims(void *handle, int *dims)

@@ -48,6 +48,10 @@ def lib(precision="f64"):
         L.oracle_farfield_dft.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double)]
         L.oracle_dft_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_cpml.restype = ctypes.c_int
+        L.oracle_cpml.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_dt.restype = ctypes.c_double
+        L.oracle_dt.argtypes = [ctypes.c_void_p]
         L.oracle_real_size.restype = ctypes.c_int
         L.oracle_set_option.restype = ctypes.c_int
         L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
@@ -162,6 +166,19 @@ class Oracle:
         if n:
             self.L.oracle_get_dft(self.h, _dptr(out))
         return out[:n]
+
+    def cpml(self, face):
+        """the planner's CPML vectors of face 0..5 (x0, xn, y0, yn, z0, zn):
+        dict of be, ce, bh, ch, kappae, kappah (None without CPML)"""
+        d = self.L.oracle_cpml(self.h, face, None)
+        if d == 0:
+            return None
+        out = np.zeros((6, d))
+        self.L.oracle_cpml(self.h, face, _dptr(out))
+        return dict(zip(("be", "ce", "bh", "ch", "kappae", "kappah"), out))
+
+    def dt(self):
+        return self.L.oracle_dt(self.h)
 
     def dft_dims(self):
         """(face cells, wavelengths) of the running DFT"""

@@ -4,9 +4,10 @@ and its adjoint gradient with respect to the design voxels' eps_r.
 The oracle's objective is pinned against a numpy DFT of point traces that the
 oracle records on every monitor cell.  The GPU objective is compared with the
 oracle's; the GPU adjoint gradient with central finite differences of the
-FP64 oracle (h = 1e-3 on eps_r), with and without the interface averaging of
-the E coefficients; the checkpointed backward pass with the one that keeps the
-whole history.
+FP64 oracle on 16 seeded voxels (h = 1e-3 on eps_r), with and without the
+interface averaging of the E coefficients, and with torch autograd through a
+pure-torch FDTD of the same scheme at 32^3; the checkpointed backward pass
+with the one that keeps the whole history.
 """
 import os
 
@@ -99,7 +100,7 @@ def test_adjoint_gradient_against_central_differences(tmp_path, magnetic):
     s.close()
     print("J gpu", J, "oracle", J_oracle)
     assert abs(J - J_oracle) / J_oracle < 1e-5
-    vox = seeded_voxels(12)
+    vox = seeded_voxels(16)
     fd = fd_gradient(folder, par, box, eps, vox, magnetic)
     ad = np.array([grad[v] for v in vox])
     err = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
@@ -152,8 +153,8 @@ def test_set_design_eps_equals_file(tmp_path):
     assert J_api == J_file
 
 
-def torch_problem(folder, n=24, nsteps=200, design=(6, 6, 4), cpml=5, gap=3):
-    """The inverse-design problem at a size the pure-torch FDTD differentiates quickly."""
+def torch_problem(folder, n=32, nsteps=240, design=(8, 8, 4), cpml=6, gap=3):
+    """The inverse-design problem at 32^3, a size the pure-torch FDTD differentiates quickly."""
     import torch_fdtd
     par, box, eps = cases.inverse_design(folder, n=n, nsteps=nsteps, design=design, cpml=cpml, gap=gap)
     c = n // 2

@@ -96,31 +96,88 @@ def test_unsupported_keyword_is_an_error(tmp_path):
         Oracle(par)
 
 
-def test_cpml_grading_closed_form():
-    """becebhch0 / becebhch (src3d/boundary.c:33-99) restated independently here:
-    the E-side kappa, b and c of a 10-cell layer at the near and far faces."""
-    from ctypes import c_double
-    depth, m, kappa, a, sigma = 10, 3, 4.0, 0.03, 1.0e5
-    dt = 1e-17
+CPML_PAR = """POOL
+30 32 34 1e-8 2e-8 3e-8
+
+COMP
+1
+
+DT_MULT
+0.7
+
+BOUNDARY_ALL
+cpml 10 3 -1 0.03 4
+
+BOUNDARY_YN
+cpml 7 2 -1 0.01 2.5
+
+BOUNDARY_Z0
+cpml 6 4 2.5e5 0 1
+
+BOUNDARY_ZN
+cpml 8 3 -1 0.02 1.5
+"""
+
+
+def grading(near, depth, m, sigma, alpha, kappa, dt):
+    """The CPML grading of src3d/boundary.c:33-99 written out independently,
+    as fractions of the layer: sigma and kappa grow as f^m and alpha as g,
+    with, at vector index i,
+      near faces (becebhch0(i)):      E  f = (d - i) / d,      g = i / d
+                                      H  f = (d - i - 1/2) / d, g = (i - 1/2) / d
+      far faces (becebhch(d - i)):    E  f = i / d,            g = (d - i) / d
+                                      H  f = (i + 1/2) / d,     g = (d - i + 1/2) / d
+    (the H samples' alpha takes the opposite half-cell offset from their
+    sigma and kappa, as the reference has it); b = exp(-(sigma / kappa + alpha)
+    dt / eps0), c = sigma / (sigma + kappa alpha) / kappa (b - 1), 0 where
+    sigma = 0."""
     eps0 = 8.854187817620389850e-12
-    for near in (True, False):
-        for i in range(depth):
-            arg = i if near else depth - i
-            pe = ((depth - arg) / depth) ** m
-            sve, kve, ave = sigma * pe, 1 + (kappa - 1) * pe, a * arg / depth
-            be = math.exp(-(sve / kve + ave) * dt / eps0)
-            ce = 0 if sve == 0 else sve / (sve + kve * ave) / kve * (be - 1)
-            assert 0 < be <= 1
-            assert ce <= 0
-            if near and i == 0:
-                assert kve == kappa
-            if not near and i == 0:
-                assert sve == 0 and ce == 0
+    i = np.arange(depth, dtype=float)
+    d = float(depth)
+    if near:
+        frac = {"e": ((d - i) / d, i / d), "h": ((d - i - 0.5) / d, (i - 0.5) / d)}
+    else:
+        frac = {"e": (i / d, (d - i) / d), "h": ((i + 0.5) / d, (d - i + 0.5) / d)}
+    out = {}
+    for side, (f, g) in frac.items():
+        p = f ** m
+        sg, kp, al = sigma * p, 1 + (kappa - 1) * p, alpha * g
+        b = np.exp(-(sg / kp + al) * dt / eps0)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            c = np.where(sg == 0, 0.0, sg / (sg + kp * al) / kp * (b - 1))
+        out["b" + side], out["c" + side], out["kappa" + side] = b, c, kp
+    return out
+
+
+def test_cpml_grading_closed_form(tmp_path):
+    """The planner's CPML vectors on all six faces against the closed form,
+    for a non-cubic cell where BOUNDARY_ALL and the per-face keywords take
+    their default sigma = 0.8 m / (377 d) with d = dx for BOUNDARY_ALL and the
+    face's own spacing for BOUNDARY_YN / ZN (src3d/settings.c:5151, 5277,
+    5337); dt from the same closed form as test_time_step_formula."""
+    par = write(str(tmp_path), "cpml.par", CPML_PAR)
+    o = Oracle(par)
+    dx, dy, dz = 1e-8, 2e-8, 3e-8
+    dt = 0.7 / (299792458.0 * math.sqrt(1 / dx ** 2 + 1 / dy ** 2 + 1 / dz ** 2))
+    faces = {0: (True, 10, 3, 0.8 * 3 / (377 * dx), 0.03, 4.0), 1: (False, 10, 3, 0.8 * 3 / (377 * dx), 0.03, 4.0),
+             2: (True, 10, 3, 0.8 * 3 / (377 * dx), 0.03, 4.0), 3: (False, 7, 2, 0.8 * 2 / (377 * dy), 0.01, 2.5),
+             4: (True, 6, 4, 2.5e5, 0.0, 1.0), 5: (False, 8, 3, 0.8 * 3 / (377 * dz), 0.02, 1.5)}
+    for f, (near, depth, m, sigma, alpha, kappa) in faces.items():
+        got = o.cpml(f)
+        want = grading(near, depth, m, sigma, alpha, kappa, dt)
+        for key in ("be", "ce", "bh", "ch", "kappae", "kappah"):
+            w = want[key]
+            assert got[key].shape == w.shape, (f, key)
+            assert np.allclose(got[key], w, rtol=1e-12, atol=1e-300), (f, key, got[key], w)
+        # the grading reaches sigma_max at the outer edge and 0 at the inner one
+        assert (got["ce"][0] < 0) if near else (got["ce"][0] == 0)
+    o.close()
 
 
 def test_time_step_formula(tmp_path):
-    par = write(str(tmp_path), "dt.par", "POOL\n12 14 16 1e-8 2e-8 3e-8\n\nCOMP\n1\n\nDT_MULT\n0.7\n\nOUT_POINT\nEz 1 5 5 5 p\n\nSOURCE_POINT\n6 6 6 1 5e-7 1 0 0\n")
+    """dt = DT_MULT / (c sqrt(1/dx^2 + 1/dy^2 + 1/dz^2)) (src3d/plan.c:5452)"""
+    par = write(str(tmp_path), "dt.par", "POOL\n12 14 16 1e-8 2e-8 3e-8\n\nCOMP\n1\n\nDT_MULT\n0.7\n\n")
     o = Oracle(par)
-    o.step(1)
-    # a generated sine source (src3d/settings.c:480-496): value at step 0 is 0, so Ez stays 0
-    assert np.all(o.points()[0][0] == 0)
+    want = 0.7 / (299792458.0 * math.sqrt(1e16 + 1 / 4e-16 + 1 / 9e-16))
+    assert abs(o.dt() - want) <= 1e-15 * want
+    o.close()
