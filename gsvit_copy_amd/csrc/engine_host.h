@@ -60,6 +60,12 @@ struct Engine {
     int tsf_face_on[6];       /* face has at least one nonzero correction coefficient */
     int has_src;
     GsvPointSrc src;
+    /* scattered-field source: its 1-D line and the PEC-rule cells it drives
+     * (i, j, k, component mask 1 Ex / 2 Ey / 4 Ez) */
+    int has_sf;
+    GsvTsf sfl;
+    int4 *sf_cells;
+    int n_sf;
     /* traces */
     int npts;
     long long *d_ppos;

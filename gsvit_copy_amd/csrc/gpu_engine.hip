@@ -535,19 +535,13 @@ setup_cpml(Engine *E)
 }
 
 
+/* the 1-D line of a plane-wave source (TSF or SF) on the device: where a cell
+ * reads the line (dcomp over the whole pool), the polarisation factors, the
+ * line fields and the waveform */
 static int
-setup_tsf(Engine *E)
+setup_plane_line(Engine *E, const SvSourceTSF *t, GsvTsf &T)
 {
-    SvPool *mp = E->mp;
     SvSet *set = E->set;
-    SvSourceTSF *t = mp->src->tsf;
-    GsvTsf &T = E->tsf;
-    memset(&T, 0, sizeof(T));
-    if (!t)
-        return 0;
-    E->has_tsf = 1;
-    T.i0 = t->box_i0; T.j0 = t->box_j0; T.k0 = t->box_k0;
-    T.in = t->box_in; T.jn = t->box_jn; T.kn = t->box_kn;
     double th = t->ia_theta, ph = t->ia_phi, ps = t->ia_psi;
     T.corr = t->corr;
     T.ax = sin(th) * cos(ph);
@@ -588,6 +582,73 @@ setup_tsf(Engine *E)
             return 1;
         T.wave = w;
     }
+    return 0;
+}
+
+/* scattered-field source (src3d/plan.c:1985-2040, src3d/yee.c:1139-1237):
+ * its line, and the cells whose components the PEC rule sets, from the class
+ * ids.  Their values are written after the E kernel, which also adds the
+ * CPML corrections; the reference sets them before those corrections, so
+ * PEC-rule cells inside CPML slabs are refused. */
+static int
+setup_sf(Engine *E)
+{
+    SvPool *mp = E->mp;
+    SvSourceTSF *t = mp->src->sf;
+    GsvGrid &g = E->g;
+    memset(&E->sfl, 0, sizeof(E->sfl));
+    E->has_sf = 0;
+    E->n_sf = 0;
+    if (!t)
+        return 0;
+    if (setup_plane_line(E, t, E->sfl))
+        return 1;
+    std::vector<int4> cells;
+    const GsvCpmlFace *F = g.f;
+    int slab = 0;
+    for (int k = 0; k < g.nz; k++)
+        for (int j = 0; j < g.ny; j++)
+            for (int i = 0; i < g.nx; i++) {
+                const size_t c = ((size_t)k * g.ny + j) * g.px + i;
+                const unsigned id = E->idbytes_e == 1 ? E->ecls_host[c]
+                                                      : ((const uint16_t *)E->ecls_host.data())[c];
+                const uint32_t fl = E->etab[id].flags;
+                const int mask = ((fl & GSV_E_PECX) ? 1 : 0) | ((fl & GSV_E_PECY) ? 2 : 0) | ((fl & GSV_E_PECZ) ? 4 : 0);
+                if (!mask)
+                    continue;
+                if ((F[0].on && i >= 1 && i < F[0].depth) || (F[1].on && i >= g.nx - F[1].depth + 1)
+                    || (F[2].on && j >= 1 && j < F[2].depth) || (F[3].on && j >= g.ny - F[3].depth + 1)
+                    || (F[4].on && k >= 1 && k < F[4].depth) || (F[5].on && k >= g.nz - F[5].depth + 1))
+                    slab = 1;
+                cells.push_back(make_int4(i, j, k, mask));
+            }
+    if (slab) {
+        fprintf(stderr, "Error: a scattered-field source with PEC cells inside CPML slabs is not supported\n");
+        return 1;
+    }
+    E->has_sf = 1;
+    E->n_sf = (int)cells.size();
+    if (E->n_sf && dupload(E, &E->sf_cells, cells.data(), cells.size()))
+        return 1;
+    return 0;
+}
+
+static int
+setup_tsf(Engine *E)
+{
+    SvPool *mp = E->mp;
+    SvSet *set = E->set;
+    SvSourceTSF *t = mp->src->tsf;
+    GsvTsf &T = E->tsf;
+    memset(&T, 0, sizeof(T));
+    if (!t)
+        return 0;
+    E->has_tsf = 1;
+    T.i0 = t->box_i0; T.j0 = t->box_j0; T.k0 = t->box_k0;
+    T.in = t->box_in; T.jn = t->box_jn; T.kn = t->box_kn;
+    if (setup_plane_line(E, t, T))
+        return 1;
+    double dt = set->plan.dt;
     /* per-face coefficients: activity (skip flags, skip-near-material map), PEC at z0, local epsilon */
     double ecor = dt / set->sp.dx, hcor = dt / set->sp.dx / MU0;
     int skips[6] = { t->skip_i0, t->skip_in, t->skip_j0, t->skip_jn, t->skip_k0, t->skip_kn };
@@ -1283,7 +1344,7 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
             CK(hipMemsetAsync(g.ade_p, 0, n * 4, E->st));
         }
     }
-    if (setup_cpml(E) || setup_tsf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E) || setup_ltsf(E) || setup_forces(E)
+    if (setup_cpml(E) || setup_tsf(E) || setup_sf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E) || setup_ltsf(E) || setup_forces(E)
         || setup_flux(E) || setup_sums(E))
         return 1;
     for (int q = 0; q < 4; q++)
@@ -1516,6 +1577,10 @@ sv_pool_gpu_ystep_e(SvPool *mp, SvSet *set)
         E->t_e_ms += ms;
         E->n_e++;
     }
+    /* the scattered-field source's values on the PEC rule's components (inside
+     * ystep_e in the reference, before its boundary E step) */
+    if (E->has_sf)
+        CK(gsv_launch_sf_pec(&E->g, &E->sfl, E->sf_cells, E->n_sf, E->st));
     /* the rest of the CPU boundary E step, in its order (src3d/boundary.c:521-936) */
     if (E->has_liao) {
         CK(gsv_launch_liao_faces(&E->g, &E->liao, E->st));
@@ -1540,6 +1605,13 @@ sv_pool_gpu_apply_source_estep(SvPool *mp, SvSet *set)
         CK(gsv_launch_tsf_faces(&E->g, &E->tsf, 0, E->st));
     if (E->has_ltsf)
         CK(gsv_launch_ltsf_faces(&E->g, &E->ltsf, 0, set->sc.step_act, E->st));
+    /* the scattered-field source's line, one step (two at step 0),
+     * src3d/source.c:1319-1338 */
+    if (E->has_sf) {
+        CK(gsv_launch_tsf_line(&E->sfl, set->sc.step_act, E->st));
+        if (set->sc.step_act == 0)
+            CK(gsv_launch_tsf_line(&E->sfl, set->sc.step_act, E->st));
+    }
     return 0;
 }
 

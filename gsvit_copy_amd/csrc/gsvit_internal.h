@@ -49,6 +49,8 @@ double sv_ff_dist(int x, int y, int z, int i, int j, int k);
 double sv_ff_angle(int i, int j, int k, int x, int y, int z, int face);
 /* FLUX_DFT spectra [plane][wavelength] from the accumulators (output.c) */
 void sv_flux_spectra(const SvPool *mp, const SvSet *set, double *out);
+/* AllFFT point spectrum: out[k] = |DFT of the detrended trace|^2 / n, k < n/2 */
+void sv_point_spectrum(const double *x, int n, double *out);
 
 #ifdef __cplusplus
 }

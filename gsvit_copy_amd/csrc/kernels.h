@@ -173,6 +173,9 @@ extern "C" {
 hipError_t gsv_launch_hstep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st);
 hipError_t gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st);
 hipError_t gsv_launch_tsf_line(const GsvTsf *t, int step, hipStream_t st);
+/* scattered-field source, src3d/yee.c:1139-1237: the listed PEC-rule cells
+ * (i, j, k, mask 1 Ex / 2 Ey / 4 Ez) take minus the incident field of line t */
+hipError_t gsv_launch_sf_pec(const GsvGrid *g, const GsvTsf *t, const int4 *cells, int n, hipStream_t st);
 hipError_t gsv_launch_tsf_faces(const GsvGrid *g, const GsvTsf *t, int hside, hipStream_t st);
 hipError_t gsv_launch_point_src(const GsvGrid *g, const GsvPointSrc *s, int step, int hside, hipStream_t st);
 hipError_t gsv_launch_record(const GsvGrid *g, const long long *pos, int n, float *rec, hipStream_t st);

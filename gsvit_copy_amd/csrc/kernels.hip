@@ -636,6 +636,23 @@ k_tsf_line(GsvTsf t, int step)
     tsf_line_step(t, step);
 }
 
+/* scattered-field source on the PEC rule's components: minus the incident
+ * field read from the SF line at the cell's distance along the wave
+ * (src3d/yee.c:1146-1230; every component of a cell reads the same sample) */
+__global__ __launch_bounds__(256) void
+k_sf_pec(GsvGrid g, GsvTsf t, const int4 *cells, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n)
+        return;
+    const int4 c = cells[q];
+    const double ve = line_val(t.je, t.res, tsf_d(t, c.x, c.y, c.z));
+    const long long o = c.x + (long long)g.px * c.y + g.plane * c.z;
+    if (c.w & 1) gp(g.ex)[o] = (float)(-t.gex * ve);
+    if (c.w & 2) gp(g.ey)[o] = (float)(-t.gey * ve);
+    if (c.w & 4) gp(g.ez)[o] = (float)(-t.gez * ve);
+}
+
 /* H correction of face cell (a, b) of the face normal to `axis` (far: 0 near, 1 far) */
 __device__ __forceinline__ void
 tsf_h_cell(const GsvGrid &g, const GsvTsf &t, int axis, int far, int a, int b)
@@ -1729,6 +1746,15 @@ gsv_launch_tsf_faces_batch(const GsvGrid *grids, const GsvTsf *tsfs, const GsvFa
 /* HBM probe: a plain streaming copy of known size, used to calibrate the
  * FETCH_SIZE / WRITE_SIZE counters and to measure attainable bandwidth    */
 /* ------------------------------------------------------------------ */
+
+extern "C" hipError_t
+gsv_launch_sf_pec(const GsvGrid *g, const GsvTsf *t, const int4 *cells, int n, hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_sf_pec, dim3((n + 255) / 256), dim3(256), 0, st, *g, *t, cells, n);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t
 gsv_launch_box_sum(const GsvGrid *g, const GsvBoxSum *b, int step, hipStream_t st)

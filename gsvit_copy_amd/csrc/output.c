@@ -305,6 +305,48 @@ sv_flux_spectra(const SvPool *mp, const SvSet *set, double *out)
         }
 }
 
+/* Power spectrum of a point trace for the AllFFT log (src3d/output.c:517-553,
+ * 607-640): the trace of n samples less its least-squares straight line,
+ * its discrete Fourier transform X_k, and out[k] = |X_k|^2 / n for
+ * k = 0 .. n/2 - 1.  The reference takes the transform from Gwyddion (which
+ * may resample the trace to a size its FFT prefers); this is a direct
+ * transform of the n samples, summed in double with the phases rotated by
+ * exact angle steps. */
+void
+sv_point_spectrum(const double *x, int n, double *out)
+{
+    double sx = 0, sxx = 0, sy = 0, sxy = 0, a = 0, b = 0;
+    double *y;
+    int i, k;
+    if (n < 2)
+        return;
+    y = (double *)malloc((size_t)n * sizeof(double));
+    if (!y)
+        return;
+    for (i = 0; i < n; i++) {
+        sx += i;
+        sxx += (double)i * i;
+        sy += x[i];
+        sxy += i * x[i];
+    }
+    b = (n * sxy - sx * sy) / (n * sxx - sx * sx);
+    a = (sy - b * sx) / n;
+    for (i = 0; i < n; i++)
+        y[i] = x[i] - a - b * i;
+    #pragma omp parallel for private(i) schedule(static)
+    for (k = 0; k < n / 2; k++) {
+        double re = 0, im = 0;
+        for (i = 0; i < n; i++) {
+            /* the angle 2 pi k i / n reduced exactly in integers */
+            const double ph = 2 * M_PI * (double)(((long long)k * i) % n) / n;
+            re += y[i] * cos(ph);
+            im -= y[i] * sin(ph);
+        }
+        out[k] = (re * re + im * im) / n;
+    }
+    free(y);
+}
+
 /* NFFF_DFT_PATTERN: the far-field pattern F (E = F exp(-i k r) / r) of the
  * running DFT on the requested angle grid, through the transform attached to
  * the far-field data by the engine that holds the accumulators.  One line per
@@ -369,6 +411,32 @@ sv_pool_output(SvPool *mp, SvSet *set)
         fh = fopen(o->filebase, "w");
         if (!fh) {
             fprintf(stderr, "Error: point output: cannot open file %s for writing\n", o->filebase);
+            continue;
+        }
+        if (o->component == SV_COMP_ALLFFT) {
+            /* src3d/output.c:607-640: written at the last step only, one line per
+             * frequency bin k = 1 .. n/2 - 1: wavelength c n dt / k, the summed
+             * power of Ex, Ey, Ez, then each component's amplitude */
+            const int n = o->stop - o->start;
+            if (set->sc.step_act == set->sc.nsteps - 1 && n >= 4) {
+                double *tr = (double *)malloc((size_t)n * sizeof(double));
+                double *sp = (double *)calloc((size_t)3 * (n / 2), sizeof(double));
+                int c;
+                if (tr && sp) {
+                    for (c = 0; c < 3; c++) {
+                        for (j = 0; j < n; j++)
+                            tr[j] = mp->out->outpointdata[i][(size_t)(j + o->start) * 6 + c];
+                        sv_point_spectrum(tr, n, sp + (size_t)c * (n / 2));
+                    }
+                    for (j = 1; j < n / 2; j++)
+                        fprintf(fh, "%g %g %g %g %g\n", SV_LIGHT_SPEED / ((double)j / ((double)n / 2.0 * set->plan.dt * 2.0)),
+                                sp[j] + sp[n / 2 + j] + sp[2 * (n / 2) + j], sqrt(sp[j]), sqrt(sp[n / 2 + j]),
+                                sqrt(sp[2 * (n / 2) + j]));
+                }
+                free(tr);
+                free(sp);
+            }
+            fclose(fh);
             continue;
         }
         for (j = 0; j < set->sc.step_act; j++) {

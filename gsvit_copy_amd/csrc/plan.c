@@ -354,6 +354,52 @@ read_point_source_file(const char *fname, SvSourcePoint *sp, int nsteps)
     return 0;
 }
 
+/* the waveform of a plane-wave source (TSF, SF): a file of "count" then
+ * "index value" lines, or a generated waveform rounded through "%g" as the
+ * reference writes and re-reads it (src3d/settings.c:444-548, src3d/plan.c:1430-1520) */
+static int
+load_plane_waveform(const SvSet *set, const SvSetSourceTSF *t, const char *what, SvSourceTSF *ts)
+{
+    int q;
+    if (t->source_mode == 0) {
+        FILE *fr = fopen(t->source_filename, "r");
+        int n;
+        if (!fr) {
+            fprintf(stderr, "Cannot load source file %s\n", t->source_filename);
+            return 1;
+        }
+        if (fscanf(fr, "%d", &n) != 1 || n <= 0) {
+            fprintf(stderr, "Error: cannot read number of values from file %s\n", t->source_filename);
+            fclose(fr);
+            return 1;
+        }
+        ts->ndata = n;
+        ts->e = (double *)calloc(n > set->sc.nsteps ? n : set->sc.nsteps, sizeof(double));
+        for (q = 0; q < n; q++) {
+            int pos;
+            if (fscanf(fr, "%d %lf", &pos, &ts->e[q]) != 2) {
+                fprintf(stderr, "Error: cannot read %s sample %d from %s\n", what, q, t->source_filename);
+                fclose(fr);
+                return 1;
+            }
+        }
+        fclose(fr);
+        if (n < set->sc.nsteps)
+            fprintf(stderr, "Warning: less data (%d) in \"%s\" than scheduled computation steps (%d) in %s source; "
+                    "missing samples are zero\n", n, t->source_filename, set->sc.nsteps, what);
+    } else {
+        int n = set->sc.nsteps;
+        ts->ndata = n;
+        ts->e = (double *)calloc(n, sizeof(double));
+        if (sv_generate_waveform(t->source_mode, n, t->source_wl, t->source_pulsewidth, set->plan.dt,
+                                 t->source_amplitude, ts->e))
+            return 1;
+        for (q = 0; q < n; q++)
+            ts->e[q] = sv_round_g6(ts->e[q]);
+    }
+    return 0;
+}
+
 static int
 load_sources(SvPool *mp, SvSet *set)
 {
@@ -440,42 +486,8 @@ load_sources(SvPool *mp, SvSet *set)
             fprintf(stderr, "Error: TSF box must lie inside the pool with one cell to spare\n");
             return 1;
         }
-        if (t->source_mode == 0) {
-            FILE *fr = fopen(t->source_filename, "r");
-            int n;
-            if (!fr) {
-                fprintf(stderr, "Cannot load source file %s\n", t->source_filename);
-                return 1;
-            }
-            if (fscanf(fr, "%d", &n) != 1 || n <= 0) {
-                fprintf(stderr, "Error: cannot read number of values from file %s\n", t->source_filename);
-                fclose(fr);
-                return 1;
-            }
-            ts->ndata = n;
-            ts->e = (double *)calloc(n > set->sc.nsteps ? n : set->sc.nsteps, sizeof(double));
-            for (q = 0; q < n; q++) {
-                int pos;
-                if (fscanf(fr, "%d %lf", &pos, &ts->e[q]) != 2) {
-                    fprintf(stderr, "Error: cannot read TSF sample %d from %s\n", q, t->source_filename);
-                    fclose(fr);
-                    return 1;
-                }
-            }
-            fclose(fr);
-            if (n < set->sc.nsteps)
-                fprintf(stderr, "Warning: less data (%d) in \"%s\" than scheduled computation steps (%d) in TSF source; "
-                        "missing samples are zero\n", n, t->source_filename, set->sc.nsteps);
-        } else {
-            int n = set->sc.nsteps;
-            ts->ndata = n;
-            ts->e = (double *)calloc(n, sizeof(double));
-            if (sv_generate_waveform(t->source_mode, n, t->source_wl, t->source_pulsewidth, set->plan.dt,
-                                     t->source_amplitude, ts->e))
-                return 1;
-            for (q = 0; q < n; q++)
-                ts->e[q] = sv_round_g6(ts->e[q]);
-        }
+        if (load_plane_waveform(set, t, "TSF", ts))
+            return 1;
         /* src3d/plan.c:1584-1595 */
         ts->corr = 1.0 / sqrt(pow(sin(ts->ia_theta), 4) * (pow(cos(ts->ia_phi), 4) + pow(sin(ts->ia_phi), 4))
                               + pow(cos(ts->ia_theta), 4));
@@ -495,6 +507,25 @@ load_sources(SvPool *mp, SvSet *set)
                 fprintf(stderr, "Warning: the TSF auxiliary line runs at Courant number %.4f; its update diverges "
                                 "above 0.75 (DT_MULT <= %.4g keeps it stable)\n", s, set->sc.dtmult * 0.75 / s);
         }
+    }
+    if (set->ss.sf.valid) {
+        /* src3d/plan.c:1985-2040: the waveform and a 1-D line of
+         * (int) sqrt(xres^2 + yres^2 + zres^2 + 10) cells of dx, hard source at
+         * cell 0, material of the layered keywords; the line is read without
+         * the TSF's angular correction */
+        SvSourceTSF *sf = (SvSourceTSF *)calloc(1, sizeof(SvSourceTSF));
+        src->sf = sf;
+        sf->ia_theta = set->ss.sf.ia_theta;
+        sf->ia_phi = set->ss.sf.ia_phi;
+        sf->ia_psi = set->ss.sf.ia_psi;
+        if (load_plane_waveform(set, &set->ss.sf, "SF", sf))
+            return 1;
+        sf->corr = 1.0;
+        sf->jp_res = (int)sqrt((double)(set->sp.xres * set->sp.xres + set->sp.yres * set->sp.yres
+                                        + set->sp.zres * set->sp.zres) + 10);
+        sf->jp_dx = set->sp.dx;
+        sf->jp_epsilon = set->ss.sf.layered_epsilon;
+        sf->jp_mu = set->ss.sf.layered_mu;
     }
     if (set->ss.ltsf.valid) {
         /* src3d/plan.c:1600-1685 */
@@ -1894,6 +1925,10 @@ sv_pool_free(SvPool *mp, SvSet *set)
         if (mp->src->tsf) {
             free(mp->src->tsf->e);
             free(mp->src->tsf);
+        }
+        if (mp->src->sf) {
+            free(mp->src->sf->e);
+            free(mp->src->sf);
         }
         free(mp->src);
     }

@@ -69,6 +69,8 @@ clear_settings(SvSet *set, int remove)
     set->ss.ltsf.box_i0 = set->ss.ltsf.box_j0 = set->ss.ltsf.box_k0 = 10;
     set->ss.ltsf.box_in = set->ss.ltsf.box_jn = set->ss.ltsf.box_kn = 90;
     set->ss.ltsf.box_boundary_skipdepth = -1;
+    set->ss.sf.layered_epsilon = 1;
+    set->ss.sf.layered_mu = 1;
     set->sb.depth_bx0 = set->sb.depth_bxn = set->sb.depth_by0 = 10;
     set->sb.depth_byn = set->sb.depth_bz0 = set->sb.depth_bzn = 10;
     set->smb.bxnpos = set->smb.bynpos = set->smb.bznpos = -1;
@@ -172,6 +174,7 @@ get_component(FILE *fr, int *val, const char *key)
     else if (!strcmp(c, "Hy") || !strcmp(c, "hy")) *val = SV_COMP_HY;
     else if (!strcmp(c, "Hz") || !strcmp(c, "hz")) *val = SV_COMP_HZ;
     else if (!strcmp(c, "All") || !strcmp(c, "all")) *val = SV_COMP_ALL;
+    else if (!strcmp(c, "AllFFT") || !strcmp(c, "allfft")) *val = SV_COMP_ALLFFT;
     else {
         fprintf(stderr, "Error: field component %s is not supported here (key %s)\n", c, key);
         return 1;
@@ -424,6 +427,31 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
                 goto fail;
             }
             t->valid = 1;
+        } else if (!strcmp(key, "SOURCE_SF")) {
+            /* src3d/settings.c:4698-4760: theta phi psi, then a waveform file (0)
+             * or generated-waveform parameters (1: wavelength amplitude, 2:
+             * wavelength pulse width amplitude) */
+            SvSetSourceTSF *t = &set->ss.sf;
+            int srctype;
+            if (get_double(fr, &t->ia_theta, key) || get_double(fr, &t->ia_phi, key) || get_double(fr, &t->ia_psi, key)
+                || get_int(fr, &srctype, key))
+                goto fail;
+            t->source_mode = srctype;
+            if (srctype == 0) {
+                if (get_rest_of_line(fr, t->source_filename, SV_NAME_LEN))
+                    goto fail;
+            } else if (srctype == 1 || srctype == 2) {
+                if (get_double(fr, &t->source_wl, key))
+                    goto fail;
+                if (srctype == 2 && get_int(fr, &t->source_pulsewidth, key))
+                    goto fail;
+                if (get_double(fr, &t->source_amplitude, key))
+                    goto fail;
+            } else {
+                fprintf(stderr, "Error: SF source type %d is not supported\n", srctype);
+                goto fail;
+            }
+            t->valid = 1;
         } else if (!strcmp(key, "TSF_SKIP")) {
             SvSetSourceTSF *t = &set->ss.tsf;
             if (get_token(fr, value, sizeof(value)))
@@ -530,6 +558,24 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             if (get_component(fr, &o->component, key) || get_int(fr, &o->step, key) || get_int(fr, &o->i, key)
                 || get_int(fr, &o->j, key) || get_int(fr, &o->k, key))
                 goto fail;
+            if (o->component == SV_COMP_ALLFFT) {
+                /* src3d/settings.c:4803-4823: step window, -1 for the whole run,
+                 * cropped to the step count known at this point of the file */
+                if (get_int(fr, &o->start, key) || get_int(fr, &o->stop, key))
+                    goto fail;
+                if (o->start == -1)
+                    o->start = 0;
+                if (o->stop == -1)
+                    o->stop = set->sc.nsteps;
+                if (o->start > set->sc.nsteps - 4) {
+                    o->start = set->sc.nsteps - 4;
+                    fprintf(stderr, "Warning: requesting FFT output outside of the data range, cropped to %d\n", o->start);
+                }
+                if (o->stop > set->sc.nsteps) {
+                    o->stop = set->sc.nsteps;
+                    fprintf(stderr, "Warning: requesting FFT output outside of the data range, cropped to %d\n", o->stop);
+                }
+            }
             if (get_rest_of_line(fr, o->filebase, SV_NAME_LEN))
                 goto fail;
             set->so.npnts++;

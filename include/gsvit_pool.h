@@ -95,6 +95,10 @@ typedef struct {
     SvSourcePoint *sp;
     SvSourceTSF *tsf;
     SvSourceLTSF *ltsf;
+    /* scattered-field source (src3d/plan.c:1985-2040): waveform and 1-D line
+     * of (int) sqrt(xres^2 + yres^2 + zres^2 + 10) cells of dx; corr = 1 and
+     * the box fields are unused */
+    SvSourceTSF *sf;
 } SvSource;
 
 /* recorded traces, src3d/output.h (outpointdata[i][step*6 + comp]) */

@@ -40,7 +40,8 @@ typedef enum {
     SV_COMP_HX = 3,
     SV_COMP_HY = 4,
     SV_COMP_HZ = 5,
-    SV_COMP_ALL = 6
+    SV_COMP_ALL = 6,
+    SV_COMP_ALLFFT = 21   /* point spectra of Ex, Ey, Ez at the last step (src3d/settings.h:69) */
 } SvComponent;
 
 typedef enum {
@@ -207,6 +208,9 @@ typedef struct {
     int npnts;
     SvSetSourcePoint *pnts;
     SvSetSourceTSF tsf;
+    /* scattered-field source, src3d/settings.c:4698-4760 (SOURCE_SF theta phi
+     * psi mode file-or-parameters); the box fields are unused */
+    SvSetSourceTSF sf;
     SvSetSourceLTSF ltsf;
     double lambda_min, lambda_center, lambda_max;
 } SvSetSource;
@@ -215,6 +219,7 @@ typedef struct {
     int component;
     int step;
     int i, j, k;
+    int start, stop;        /* AllFFT: the steps the spectrum covers */
     char filebase[SV_NAME_LEN];
 } SvOutputPar;
 

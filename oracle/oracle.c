@@ -100,6 +100,10 @@ typedef struct {
     real *px[3], *py[3], *pz[3], *pxp[3], *pyp[3], *pzp[3];
     int npoles;                       /* critical-point poles carried: 2, or 3 with a type-7 material */
     int perx, pery;                   /* domain-periodic x / y (no reference counterpart) */
+    /* scattered-field source line (SOURCE_SF) */
+    int sres;
+    real *se, *sh;
+    real sb[8];
     /* 1-D line */
     int jres;
     real *je, *jh;
@@ -131,6 +135,7 @@ zalloc(size_t n, size_t s)
 }
 
 void oracle_close(void *handle);
+static double dline_val(const real *data, int res, double x);
 
 void *
 oracle_open(const char *parfile)
@@ -212,6 +217,11 @@ oracle_open(const char *parfile)
         o->jres = o->mp->src->tsf->jp_res;
         o->je = (real *)zalloc(o->jres, sizeof(real));
         o->jh = (real *)zalloc(o->jres, sizeof(real));
+    }
+    if (o->mp->src->sf) {
+        o->sres = o->mp->src->sf->jp_res;
+        o->se = (real *)zalloc(o->sres, sizeof(real));
+        o->sh = (real *)zalloc(o->sres, sizeof(real));
     }
     if (set->so.npnts)
         o->points = (double *)zalloc((size_t)set->so.npnts * (set->sc.nsteps + 1) * 6, sizeof(double));
@@ -604,7 +614,10 @@ ystep_e(Oracle *o)
                     if (xi && yi && ztbc)
                         o->ez[c] = caz * o->ez[c] + cbz * ((hy - o->hy[cxm]) / kx - (hx - o->hx[cym]) / ky);
                 }
-                /* PEC, src3d/yee.c:1139-1237 (no scattered-field source on this path) */
+                /* PEC, src3d/yee.c:1139-1237: the components it names are zeroed,
+                 * or with a scattered-field source set to minus the incident
+                 * field, read from that source's line at the cell's distance
+                 * along the wave (dcomp over the whole pool, no correction) */
                 if (matindex >= 0 && i > 0 && j > 0 && k > 0) {
                     int p0 = is_pec_t(o, i, j, k);
                     int pxm = is_pec_t(o, i - 1, j, k), pym = is_pec_t(o, i, j - 1, k), pzm = is_pec_t(o, i, j, k - 1);
@@ -613,12 +626,21 @@ ystep_e(Oracle *o)
                     /* the entry test uses the material chosen by the dispersive flags (src3d/yee.c:1140) */
                     int pm = mp->mats[matindex].type == SV_MAT_PEC;
                     if (pm || pxm || pym || pzm || pxy || pyz || pxz || pxyz) {
-                        if (pxm != p0) { o->ey[c] = 0; o->ez[c] = 0; }
-                        if (pym != p0) { o->ex[c] = 0; o->ez[c] = 0; }
-                        if (pzm != p0) { o->ex[c] = 0; o->ey[c] = 0; }
-                        if (pxy && !p0) o->ez[c] = 0;
-                        if (pxz && !p0) o->ey[c] = 0;
-                        if (pyz && !p0) o->ex[c] = 0;
+                        double vx = 0, vy = 0, vz = 0;
+                        const SvSourceTSF *sf = mp->src->sf;
+                        if (sf) {
+                            const double ve = dline_val(o->se, o->sres, sv_dcomp(i, j, k, sf->ia_theta, sf->ia_phi,
+                                                                                 0, nx, 0, ny, 0, nz));
+                            vx = -sv_gex(ve, sf->ia_theta, sf->ia_phi, sf->ia_psi);
+                            vy = -sv_gey(ve, sf->ia_theta, sf->ia_phi, sf->ia_psi);
+                            vz = -sv_gez(ve, sf->ia_theta, sf->ia_phi, sf->ia_psi);
+                        }
+                        if (pxm != p0) { o->ey[c] = vy; o->ez[c] = vz; }
+                        if (pym != p0) { o->ex[c] = vx; o->ez[c] = vz; }
+                        if (pzm != p0) { o->ex[c] = vx; o->ey[c] = vy; }
+                        if (pxy && !p0) o->ez[c] = vz;
+                        if (pxz && !p0) o->ey[c] = vy;
+                        if (pyz && !p0) o->ex[c] = vx;
                     }
                 }
             }
@@ -1085,28 +1107,47 @@ dline_val(const real *data, int res, double x)
     return 0;
 }
 
+/* one step of a plane-wave source's 1-D line (TSF or SF): src3d/1dpool.c:83-162
+ * in the order of src3d/source.c:1469-1473 (and :1322-1326 for SF) */
 static void
-jp_step(Oracle *o)
+line_step(Oracle *o, const SvSourceTSF *t, real *je, real *jh, real *jb, int res)
 {
-    /* src3d/source.c:1469-1473 with src3d/1dpool.c:83-162 */
-    const SvSourceTSF *t = o->mp->src->tsf;
     double dt = o->set.plan.dt, dx = t->jp_dx;
     double multe = dt / (EPS0 * dx), multh = dt / (MU0 * dx);
     double coef = (dt * CL - dx) / (dt * CL + dx);
-    int res = o->jres, i;
+    int i;
     double ev = (o->set.sc.step_act < t->ndata) ? t->e[o->set.sc.step_act] : 0.0;
 
     for (i = 1; i < res; i++)
-        o->je[i] += 1.0 / t->jp_epsilon * multe * (o->jh[i - 1] - o->jh[i]);
-    o->je[0] = o->jb[1] + coef * (o->je[1] - o->jb[0]);
-    o->jh[0] = o->jb[3] + coef * (o->jh[1] - o->jb[2]);
-    o->je[res - 1] = o->jb[5] + coef * (o->je[res - 2] - o->jb[4]);
-    o->jh[res - 1] = o->jb[7] + coef * (o->jh[res - 2] - o->jb[6]);
-    o->jb[0] = o->je[0]; o->jb[1] = o->je[1]; o->jb[2] = o->jh[0]; o->jb[3] = o->jh[1];
-    o->jb[4] = o->je[res - 1]; o->jb[5] = o->je[res - 2]; o->jb[6] = o->jh[res - 1]; o->jb[7] = o->jh[res - 2];
-    o->je[0] = ev;
+        je[i] += 1.0 / t->jp_epsilon * multe * (jh[i - 1] - jh[i]);
+    je[0] = jb[1] + coef * (je[1] - jb[0]);
+    jh[0] = jb[3] + coef * (jh[1] - jb[2]);
+    je[res - 1] = jb[5] + coef * (je[res - 2] - jb[4]);
+    jh[res - 1] = jb[7] + coef * (jh[res - 2] - jb[6]);
+    jb[0] = je[0]; jb[1] = je[1]; jb[2] = jh[0]; jb[3] = jh[1];
+    jb[4] = je[res - 1]; jb[5] = je[res - 2]; jb[6] = jh[res - 1]; jb[7] = jh[res - 2];
+    je[0] = ev;
     for (i = 0; i < res - 1; i++)
-        o->jh[i] += 1.0 / t->jp_mu * multh * (o->je[i] - o->je[i + 1]);
+        jh[i] += 1.0 / t->jp_mu * multh * (je[i] - je[i + 1]);
+}
+
+static void
+jp_step(Oracle *o)
+{
+    line_step(o, o->mp->src->tsf, o->je, o->jh, o->jb, o->jres);
+}
+
+/* the scattered-field source's line, after the E-side sources
+ * (src3d/source.c:1319-1338): one step, two at step 0 */
+static void
+sf_estep(Oracle *o)
+{
+    const SvSourceTSF *t = o->mp->src->sf;
+    if (!t)
+        return;
+    line_step(o, t, o->se, o->sh, o->sb, o->sres);
+    if (o->set.sc.step_act == 0)
+        line_step(o, t, o->se, o->sh, o->sb, o->sres);
 }
 
 static void
@@ -1979,6 +2020,7 @@ oracle_step(void *handle, int nsteps)
         boundary_estep(o);
         source_estep(o);
         ltsf_estep(o);
+        sf_estep(o);
         getsum(o);
         farfield(o);
         pfarfield(o);
@@ -2123,6 +2165,13 @@ oracle_cpml(void *handle, int f, double *out)
             out[5 * F[f]->depth + i] = F[f]->kappah[i];
         }
     return F[f]->depth;
+}
+
+/* the product's AllFFT spectrum routine (output.c), for its CPU test */
+void
+oracle_point_spectrum(const double *x, int n, double *out)
+{
+    sv_point_spectrum(x, n, out);
 }
 
 /* the planner's time step */

@@ -269,6 +269,35 @@ def plane_wave_flux(folder, n=16, nz=120, nsteps=2200, dx=5e-9, depth=10, lam=50
     return path
 
 
+def pec_plate_wave(folder, source, n=12, nz=140, nsteps=900, dx=5e-9, depth=10, lam=500e-9, plate=(80, 84),
+                   theta=0.0, points=()):
+    """A plane-wave pulse (theta from +z, phi = psi = 0) in an n x n cell
+    periodic in x and y, CPML in z, onto a PEC plate filling the cell at
+    plate[0] <= k < plate[1].  source "tsf": a TSF box from k = 20 to nz - 20
+    injecting through its bottom face (below it the scattered field only);
+    source "sf": a scattered-field source (SOURCE_SF), the whole pool holding
+    the scattered field."""
+    os.makedirs(folder, exist_ok=True)
+    dt = _dt(dx, dx, dx, 1.0)
+    write_tsf_source(os.path.join(folder, "pulse.txt"), gaussian_sine(nsteps, dt, lam, 2, delay_widths=2.5))
+    with open(os.path.join(folder, "plate.txt"), "w") as f:
+        f.write("8 0 0 %d %d %d %d 10\n" % (plate[0], n, n, plate[1]))
+    lines = ["POOL\n%d %d %d %g %g %g\n\n" % (n, n, nz, dx, dx, dx), "COMP\n%d\n\n" % nsteps,
+             "BOUNDARY_X0\nperiodic\n\nBOUNDARY_XN\nperiodic\n\nBOUNDARY_Y0\nperiodic\n\nBOUNDARY_YN\nperiodic\n\n",
+             "BOUNDARY_Z0\ncpml %d 3 -1 0 1\n\nBOUNDARY_ZN\ncpml %d 3 -1 0 1\n\n" % (depth, depth),
+             "MEDIUM_VECTOR\nplate.txt\n\n"]
+    if source == "tsf":
+        lines += ["SOURCE_TSF\n0 0 %d %d %d %d %.17g 0 0 0 pulse.txt\n\n" % (20, n, n, nz - 20, theta), "TSF_SKIP\nkn\n\n"]
+    else:
+        lines.append("SOURCE_SF\n%.17g 0 0 0 pulse.txt\n\n" % theta)
+    for q, (i, j, k) in enumerate(points):
+        lines.append("OUT_POINT\nAll 1 %d %d %d pt%d\n\n" % (i, j, k, q))
+    path = os.path.join(folder, "plate_%s.par" % source)
+    with open(path, "w") as f:
+        f.write("".join(lines))
+    return path
+
+
 # gold-like Drude + 2 critical points (eps_inf, sigma, omega_D, gamma_D, then
 # A, phi, Omega, Gamma per pole; rad/s), in the range of published Drude-CP
 # fits of gold; passive (Im eps > 0 at every frequency)
