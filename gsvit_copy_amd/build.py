@@ -21,7 +21,7 @@ CLANG = os.path.join(ROCM, "llvm", "bin", "clang")
 OMPLIB = os.path.join(ROCM, "llvm", "lib")
 ARCH = os.environ.get("GSV_OFFLOAD_ARCH", "gfx950")
 
-C_SOURCES = ["settings.c", "plan.c", "containers.c", "output.c", "session.c"]
+C_SOURCES = ["settings.c", "plan.c", "database.c", "containers.c", "output.c", "session.c"]
 HIP_SOURCES = ["kernels.hip", "gpu_engine.hip", "adjoint.hip", "batch.hip", "nfff.hip"]
 INC = ["-I" + os.path.join(REPO, "include"), "-I" + SRC]
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-fopenmp",

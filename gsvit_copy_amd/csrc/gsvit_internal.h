@@ -6,6 +6,7 @@
 #ifndef GSVIT_INTERNAL_H
 #define GSVIT_INTERNAL_H
 
+#include <stdio.h>
 #include "gsvit_settings.h"
 #include "gsvit_pool.h"
 
@@ -51,6 +52,10 @@ double sv_ff_angle(int i, int j, int k, int x, int y, int z, int face);
 void sv_flux_spectra(const SvPool *mp, const SvSet *set, double *out);
 /* AllFFT point spectrum: out[k] = |DFT of the detrended trace|^2 / n, k < n/2 */
 void sv_point_spectrum(const double *x, int n, double *out);
+/* the MEDIUM_VECTOR file to read: the file itself, or an anonymous temporary
+ * file holding its rewrite when it names optical-database materials (type
+ * 99, database.c); NULL if it cannot be opened or resolved */
+FILE *sv_open_vector_file(const SvSet *set, const char *fname);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,9 @@
  *   - matmode selection including the electric+magnetic -> NONE case of
  *     test_script_file (src3d/plan.c:573-711, 5489-5518);
  *   - internal-boundary map on faces and 26-neighbour differences
- *     (src3d/plan.c:5381-5417).
+ *     (src3d/plan.c:5381-5417);
+ *   - a vector file naming optical-database materials is read through its
+ *     rewrite (every number through float and "%g", database.c).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -812,9 +814,9 @@ entity_nfloats(int type)
 }
 
 static int
-test_script_file(const char *fname)
+test_script_file(const SvSet *set, const char *fname)
 {
-    FILE *fr = fopen(fname, "r");
+    FILE *fr = sv_open_vector_file(set, fname);
     int type, mtype, is_e = 0, is_m = 0;
     float p[16];
     if (!fr) {
@@ -1308,7 +1310,7 @@ load_data(SvPool *mp, SvSet *set)
     }
 
     if (set->sm.in_vector_filename[0]) {
-        FILE *fr = fopen(set->sm.in_vector_filename, "r");
+        FILE *fr = sv_open_vector_file(set, set->sm.in_vector_filename);
         SvEntity *ents = NULL;
         int nent = 0, type;
         if (!fr) {
@@ -1708,7 +1710,7 @@ init_and_make_plan(SvSet *set)
         if (set->sm.in_voxel_filename[0])
             matmode_mat = test_material_file(set->sm.in_voxel_filename);
         if (set->sm.in_vector_filename[0])
-            matmode_script = test_script_file(set->sm.in_vector_filename);
+            matmode_script = test_script_file(set, set->sm.in_vector_filename);
         if (matmode_script == SV_MATMODE_ERROR)
             return NULL;
         if (matmode_mat == SV_MATMODE_NONE && matmode_script == SV_MATMODE_NONE)
@@ -1724,7 +1726,7 @@ init_and_make_plan(SvSet *set)
         else
             set->plan.matmode = SV_MATMODE_FULL;
     } else {
-        if (set->sm.in_vector_filename[0] && test_script_file(set->sm.in_vector_filename) == SV_MATMODE_ERROR)
+        if (set->sm.in_vector_filename[0] && test_script_file(set, set->sm.in_vector_filename) == SV_MATMODE_ERROR)
             return NULL;
         set->plan.matmode = SV_MATMODE_FULL;
     }

@@ -2346,6 +2346,30 @@ oracle_get_maps(void *handle, int *mat, int *bnds)
     }
 }
 
+/* material m of the planner's table: out[0..9] = type, epsilon, mu, sigma,
+ * sigast, drude omega_p, drude nu, ADE pole count, cp3_a[0], cp3_omega[0];
+ * returns the number of materials (m outside it: out untouched) */
+int
+oracle_get_material(void *handle, int m, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    const SvPool *mp = o->mp;
+    if (m >= 0 && m < mp->nmat && out) {
+        const SvMatProp *t = mp->mats + m;
+        out[0] = t->type;
+        out[1] = t->epsilon;
+        out[2] = t->mu;
+        out[3] = t->sigma;
+        out[4] = t->sigast;
+        out[5] = t->drude_omega_p;
+        out[6] = t->drude_nu;
+        out[7] = t->ade_npoles;
+        out[8] = t->cp3_a[0];
+        out[9] = t->cp3_omega[0];
+    }
+    return mp->nmat;
+}
+
 /* test options: "no_average" (0/1), "mb_fault" (0, 1, 2) */
 int
 oracle_set_option(void *handle, const char *name, int value)

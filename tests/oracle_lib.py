@@ -50,6 +50,8 @@ def lib(precision="f64"):
         L.oracle_dft_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.oracle_cpml.restype = ctypes.c_int
         L.oracle_cpml.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_get_material.restype = ctypes.c_int
+        L.oracle_get_material.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.oracle_dt.restype = ctypes.c_double
         L.oracle_dt.argtypes = [ctypes.c_void_p]
         L.oracle_real_size.restype = ctypes.c_int
@@ -203,6 +205,21 @@ class Oracle:
         if n:
             self.L.oracle_get_flux(self.h, _dptr(out))
         return out[:n]
+
+    def materials(self):
+        """the planner's material table: dicts of type, epsilon, mu, sigma,
+        sigast, omega_p, nu, poles, cp_a0, cp_omega0"""
+        keys = ("type", "epsilon", "mu", "sigma", "sigast", "omega_p", "nu", "poles", "cp_a0", "cp_omega0")
+        n = self.L.oracle_get_material(self.h, -1, None)
+        out = []
+        for m in range(n):
+            v = np.zeros(10)
+            self.L.oracle_get_material(self.h, m, _dptr(v))
+            d = dict(zip(keys, v.tolist()))
+            d["type"] = int(d["type"])
+            d["poles"] = int(d["poles"])
+            out.append(d)
+        return out
 
     def maps(self):
         inf = self.info()
