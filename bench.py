@@ -32,7 +32,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(info, n, depth, n_ade_cells):
+def algorithmic_bytes(info, shape, depth, n_ade_cells):
     """Algorithmic HBM bytes of one step, per kernel, by the reference
     formulation's per-cell figures.
 
@@ -46,8 +46,9 @@ def algorithmic_bytes(info, n, depth, n_ade_cells):
     move fewer bytes than this count; the measured traffic is reported beside
     it.
     """
-    cells = float(n) ** 3
-    slab_dir_cells = 6.0 * depth * n * n   # cell-directions inside the six slabs
+    nx, ny, nz = (shape, shape, shape) if np.isscalar(shape) else shape
+    cells = float(nx) * ny * nz
+    slab_dir_cells = 2.0 * depth * (float(ny) * nz + float(nx) * nz + float(nx) * ny)   # cell-directions in the six slabs
     h_bytes = 37.0 * cells + 16.0 * slab_dir_cells
     e_bytes = 37.0 * cells + 16.0 * slab_dir_cells + 72.0 * n_ade_cells
     return h_bytes, e_bytes
@@ -150,22 +151,44 @@ def batch_bench(args, work):
     paths, params = cases.sweep(os.path.join(work, "sweep"), nruns=64, seed=11, nsteps=2000)
     cells = sum(p["shape"][0] * p["shape"][1] * p["shape"][2] for p in params)
     log("bench: 64-run sweep (%.1f Mcells) written in %.1f s" % (cells / 1e6, time.time() - t0))
-    t_seq = 0.0
-    for p in paths:
-        with Session(p) as s:
-            s.run(5)
-            s.sync()
-            t0 = time.perf_counter()
-            s.run(S)
-            s.sync()
-            t_seq += time.perf_counter() - t0
-    seq = cells * S / t_seq / 1e9
-    log("bench: sequential plain runs %.2f Gcell-updates/s" % seq)
+    t_seq, seq = None, None
+    single_h = single_e = 0.0
+    if not args.batch_only:
+        t_seq = 0.0
+        for p in paths:
+            with Session(p) as s:
+                s.run(5)
+                s.sync()
+                t0 = time.perf_counter()
+                s.run(S)
+                s.sync()
+                t_seq += time.perf_counter() - t0
+                # the same volume kernels per run, timed with events (after the timed steps)
+                s.set_timing(True)
+                s.run(5)
+                kt = s.kernel_times()
+                single_h += kt["h_ms"] / max(kt["h_n"], 1)
+                single_e += kt["e_ms"] / max(kt["e_n"], 1)
+        seq = cells * S / t_seq / 1e9
+        log("bench: sequential plain runs %.2f Gcell-updates/s; their volume kernels H %.3f ms E %.3f ms per sweep step"
+            % (seq, single_h, single_e))
     t0 = time.time()
     sessions = [Session(p) for p in paths]
     t_open = time.time() - t0
+    # algorithmic bytes of one batched step (the single-run per-cell figures, summed over the runs)
+    h_bytes = e_bytes = 0.0
+    for s, p in zip(sessions, params):
+        mat, _ = s.maps()
+        hb, eb = algorithmic_bytes(None, p["shape"], 10, int(dilate(mat == 1).sum()))
+        h_bytes += hb
+        e_bytes += eb
+        del mat
     b = Batch(sessions)
-    used, best = b.autotune(2)
+    if args.batch_launch:
+        best, used = [int(v) for v in args.batch_launch.split(",")], 0
+        b.set_launch(best)
+    else:
+        used, best = b.autotune(2)
     b.set_graph(16)
     b.run(16)
     t0 = time.perf_counter()
@@ -179,15 +202,23 @@ def batch_bench(args, work):
     for s in sessions:
         s.close()
     bat = cells * S / t_b / 1e9
-    log("bench: batched %.2f Gcell-updates/s (x%.2f), launch %s, H %.3f ms E %.3f ms per step"
-        % (bat, bat / seq, best, inf["h_ms"], inf["e_ms"]))
+    h_gbs = h_bytes / (inf["h_ms"] * 1e-3) / 1e9
+    e_gbs = e_bytes / (inf["e_ms"] * 1e-3) / 1e9
+    log("bench: batched %.2f Gcell-updates/s (x%s), launch %s, H %.3f ms (%.0f GB/s) E %.3f ms (%.0f GB/s) per step"
+        % (bat, "%.2f" % (bat / seq) if seq else "-", best, inf["h_ms"], h_gbs, inf["e_ms"], e_gbs))
     return {"workload": "64-run rough-surface sweep, grid edges from {64,96,128,192,256} per axis, Drude ADE, "
                         "CPML 10, TSF p-pol U[0,70] deg, lambda U[450,750] nm",
             "runs": 64, "cells": cells, "timed_steps": S,
-            "sequential_gcells": round(seq, 3), "batched_gcells": round(bat, 3), "ratio": round(bat / seq, 3),
-            "sequential_ms_per_step": round(1e3 * t_seq / S, 3), "batched_ms_per_step": round(1e3 * t_b / S, 3),
+            "sequential_gcells": round(seq, 3) if seq else None, "batched_gcells": round(bat, 3),
+            "ratio": round(bat / seq, 3) if seq else None,
+            "sequential_ms_per_step": round(1e3 * t_seq / S, 3) if seq else None,
+            "batched_ms_per_step": round(1e3 * t_b / S, 3),
             "open_64_sessions_s": round(t_open, 2), "autotune_steps": used, "launch": best, "graph_steps": 16,
             "h_ms": round(inf["h_ms"], 4), "e_ms": round(inf["e_ms"], 4),
+            "h_algorithmic_bytes": h_bytes, "e_algorithmic_bytes": e_bytes,
+            "single_run_kernels_h_ms": round(single_h, 4) if seq else None,
+            "single_run_kernels_e_ms": round(single_e, 4) if seq else None,
+            "h_frac": round(h_gbs / HBM_PEAK_GBS, 4), "e_frac": round(e_gbs / HBM_PEAK_GBS, 4),
             "metric": "batched / sequential throughput", "higher_is_better": True}
 
 
@@ -377,6 +408,10 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=3000)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-sweep measurement")
     ap.add_argument("--batch-steps", type=int, default=400, help="timed steps of the batched-sweep comparison")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched leg, without the sequential runs (profiling); prints its JSON object")
+    ap.add_argument("--batch-launch", default=None,
+                    help="fixed batch launch shape (8 numbers as gsv_batch_set_launch; implies no batch tuning)")
     ap.add_argument("--batch-e2e", action="store_true",
                     help="also time the whole 64-run sweep end to end (one process per run vs one batch)")
     ap.add_argument("--no-film", action="store_true", help="skip the textured thin-film measurement")
@@ -402,6 +437,11 @@ def main():
         print("bench.py measures one GPU only: the engine has no multi-GPU path (a z-slab decomposition "
               "with halo exchange is the planned sharding); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.batch_only:
+        work = args.workdir or tempfile.mkdtemp(prefix="gsv_bench_")
+        print(json.dumps(batch_bench(args, work)), flush=True)
+        return
 
     import cases
     from gsvit_copy_amd import Session

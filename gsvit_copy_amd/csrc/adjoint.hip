@@ -9,8 +9,8 @@
  *   F_c = sum_{n=0}^{N-1} E_c^{n+1} exp(-i omega t_n),  t_n = (n+1) dt,
  * with E^{n+1} the field after step n (after its point sources).
  *
- * Scope: non-dispersive media, CPML or plain boundaries, hard E point
- * sources.  One forward step is x^{n+1} = S_n M_E M_H x^n + s_n for the state
+ * Scope: CPML or plain boundaries, hard E point sources, and dispersive (ADE)
+ * media outside the CPML slabs and away from the design cells.  One forward step is x^{n+1} = S_n M_E M_H x^n + s_n for the state
  * x = (E, H, psi_E, psi_H); the backward pass applies the transposed steps in
  * reverse order (M_H^T M_E^T S_n^T), in two bulk kernels per step (with the
  * CPML psi terms in four slab kernels around them, see k_slab_*):
@@ -24,6 +24,20 @@
  *            H = da h).
  * The adjoint of psi is stored without its decay factor b, which the next
  * (earlier) step applies when it reads it.
+ *
+ * Dispersive cells (kernels.hip, ade_cell, increment form): per component c
+ * the step computes A = the ADE update (c0 curl / d + cj J + sum_q (pcj Jq +
+ * pcp Pq) + c2 Ep + c3 E) for the components it owns, A = E for the others,
+ * advances J' = dj J + da (Ep + 2 E + A), Jq' = pj Jq + ps Pq + b2 Ep + b3 E +
+ * b4 A, Pq' = Pq + Jq', Ep' = E, and then gives the other components the
+ * ordinary update E' = ca A + cb curl.  The transpose keeps an adjoint of
+ * each state (Ep, J, Jq, Pq) in the forward state's layout: with u_q the
+ * adjoint of Jq' (its own plus that of Pq') and a_A = (ca or 1) E'bar + da Jbar'
+ * + sum b4 u_q, it feeds c0 / d a_A (or cb E'bar) to the curl transpose and
+ * returns Ebar = (c3 a_A or a_A) + 2 da Jbar' + sum b3 u_q + Epbar',
+ * Epbar = c2 a_A + da Jbar' + sum b2 u_q, Jbar = cj a_A + dj Jbar',
+ * Jqbar = pcj a_A + pj u_q, Pqbar = pcp a_A + ps u_q + Pqbar' (the a_A terms
+ * only for components the ADE update owns).
  *
  * The gradient needs, per design cell and step, E^n and the coefficient-free
  * increment D^n = (E^{n+1} - ca E^n) / cb of the E update.  They are recorded
@@ -54,6 +68,7 @@ typedef struct {
     float *ebx, *eby, *ebz, *hbx, *hby, *hbz;
     float *qx, *qy, *qz, *rx, *ry, *rz;
     float2 *te[6], *th[6];    /* psi adjoints per face (E side, H side) */
+    float *xad, *xap;         /* adjoints of the ADE state, the layout of GsvGrid::ade_d / ade_p */
     /* design coefficient box: history and gradient accumulators */
     long long ncoef;
     float *hist;              /* [slot][6][ncoef]: E^n x y z, D^n x y z */
@@ -145,10 +160,69 @@ e_coef(const GsvGrid &g, OFF oc, int i, int j, int k)
     return ec;
 }
 
+/* transpose of ade_cell's step for one dispersive cell (see the file head):
+ * e[c] holds the adjoint of E' on entry and the adjoint of E on return, q[c]
+ * the curl coefficient's adjoint; xd / xp the cell's adjoint state */
+template <int NP, typename OFF>
+__device__ __forceinline__ void
+adj_ade_cell(const GsvGrid &g, const GsvECoef &ec, float *xd, OFF la, float *xp, OFF lp, int i, int j, int k,
+             unsigned fl, float e[3], float q[3])
+{
+    const GsvAde &m = g.adem[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
+    const bool t[3] = { (fl & GSV_E_TBCX) != 0, (fl & GSV_E_TBCY) != 0, (fl & GSV_E_TBCZ) != 0 };
+    /* the update applies (either kind) where the forward step's bounds allow it */
+    const bool upd[3] = { j > 0 && k > 0, i > 0 && k > 0, i > 0 && j > 0 };
+    const float dinv[3] = { 1.f / g.dx, 1.f / g.dy, 1.f / g.dz };
+    const bool dr = NP == 0 || m.drude;
+    const int npl = NP ? m.poles : 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float aep = ldsn(xd, la, 256 * c);
+        const float aj = dr ? ldsn(xd, la, 768 + 256 * c) : 0.f;
+        float u[3] = {}, apq[3] = {};
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+            if (p < npl) {
+                apq[p] = ldsn(xp, lp, (3 * p + c) * 256);
+                u[p] = ldsn(xp, lp, (3 * NP + 3 * p + c) * 256) + apq[p];
+            }
+        /* the ordinary update after the ADE branch */
+        const bool ord = t[c] && upd[c], own = !t[c] && upd[c];
+        float abar = ord ? ec.ca[c] * e[c] : e[c];
+        q[c] = ord ? ec.cb[c] * e[c] : 0.f;
+        /* A feeds the state updates of this step */
+        abar += m.da * aj;
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+            if (p < npl)
+                abar += m.b4[p] * u[p];
+        float ebar = own ? m.c3 * abar : abar;
+        float epbar = own ? m.c2 * abar : 0.f;
+        float jbar = own ? m.cj * abar : 0.f;
+        if (own)
+            q[c] = m.c0 * dinv[c] * abar;
+        ebar += 2.f * m.da * aj + aep;
+        epbar += m.da * aj;
+        jbar += m.dj * aj;
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+            if (p < npl) {
+                ebar += m.b3[p] * u[p];
+                epbar += m.b2[p] * u[p];
+                stsn(xp, lp, (3 * NP + 3 * p + c) * 256, (own ? m.pcj[p] * abar : 0.f) + m.pj[p] * u[p]);
+                stsn(xp, lp, (3 * p + c) * 256, (own ? m.pcp[p] * abar : 0.f) + m.ps[p] * u[p] + apq[p]);
+            }
+        stsn(xd, la, 256 * c, epbar);
+        if (dr)
+            stsn(xd, la, 768 + 256 * c, jbar);
+        e[c] = ebar;
+    }
+}
+
 /* E cells: H-step transpose of step n+1 (gathers of r and the psi_H
  * adjoints), objective source and point-source projection of step n,
  * coefficient gradient, local part of the E-step transpose of step n */
-template <typename ID, typename OFF>
+template <typename ID, typename OFF, int ADE>
 __global__ __launch_bounds__(512) void
 k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int kchunk)
 {
@@ -163,6 +237,10 @@ k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int kchunk)
     const OFF dP = (OFF)g.plane * 4, dJ = (OFF)px * 4, dPc = (OFF)g.plane * sizeof(ID);
     OFF o = ((OFF)i + (OFF)px * j + (OFF)g.plane * (k0 + 1)) * 4;
     OFF oc = ((OFF)i + (OFF)px * j + (OFF)g.plane * k0) * sizeof(ID);
+    /* lane offsets in the blocked ADE state planes (as estep_col) */
+    constexpr int NP = ADE >= 2 ? ADE : 0;
+    const OFF la = ADE ? ((OFF)j * g.pxa * GSV_ADE_D + (OFF)(i >> 6) * 64 * GSV_ADE_D + (i & 63)) * 4 : 0;
+    const OFF lp = NP ? ((OFF)j * g.pxa * GSV_ADE_P(NP) + (OFF)(i >> 6) * 64 * GSV_ADE_P(NP) + (i & 63)) * 4 : 0;
 
     const float rhx = g.rkhx[i], rhy = g.rkhy[j];
     const float rhx1 = i > 0 ? g.rkhx[i - 1] : 0.f, rhy1 = j > 0 ? g.rkhy[j - 1] : 0.f;
@@ -230,6 +308,19 @@ k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int kchunk)
             a.gcb[q] += (double)ex * h[3 * a.ncoef];
             a.gcb[a.ncoef + q] += (double)ey * h[4 * a.ncoef];
             a.gcb[2 * a.ncoef + q] += (double)ez * h[5 * a.ncoef];
+        }
+        if (ADE && (fl & GSV_E_ADE)) {
+            /* dispersive cell (no PEC, no CPML slab, no design cell here) */
+            float e[3] = { ex, ey, ez }, q[3];
+            adj_ade_cell<NP>(g, ec, a.xad + (size_t)k * g.ade_plane_d, la, NP ? a.xap + (size_t)k * g.ade_plane_p : nullptr,
+                             lp, i, j, k, fl, e, q);
+            stf(a.qx, o, q[0]);
+            stf(a.qy, o, q[1]);
+            stf(a.qz, o, q[2]);
+            stn(a.ebx, o, e[0]);
+            stn(a.eby, o, e[1]);
+            stn(a.ebz, o, e[2]);
+            continue;
         }
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
         /* PEC zeroing acts on the curl update only (the corrections come after it) */
@@ -578,6 +669,15 @@ psi_pairs(const GsvCpmlFace &F, int axis, const GsvGrid &g)
     return (size_t)g.px * g.ny * F.depth;
 }
 
+/* floats of the forward ADE state (Drude blocks, then the pole blocks) */
+static size_t
+ade_floats(const GsvGrid &g)
+{
+    if (!g.ade)
+        return 0;
+    return (size_t)g.ade_plane_d * g.nz + (g.ade_p ? (size_t)g.ade_plane_p * g.nz : 0);
+}
+
 void
 gsv_adjoint_free(Engine *E)
 {
@@ -606,9 +706,39 @@ gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_int
         fprintf(stderr, "Error: the gradient does not support force boxes or the periodic far field\n");
         return 1;
     }
-    if (g.ade || E->has_tsf || E->has_ltsf || E->has_sf || E->has_liao || E->has_per || g.perx || g.pery) {
-        fprintf(stderr, "Error: the gradient supports non-dispersive media, CPML and point sources only\n");
+    if (E->has_tsf || E->has_ltsf || E->has_sf || E->has_liao || E->has_per || g.perx || g.pery) {
+        fprintf(stderr, "Error: the gradient supports CPML and point sources only\n");
         return 1;
+    }
+    if (g.ade && g.ade_cpml) {
+        fprintf(stderr, "Error: the gradient does not support ADE_CPML_CONSISTENT\n");
+        return 1;
+    }
+    if (g.ade) {
+        /* dispersive cells stay out of the CPML slabs (E side) and off the design cells;
+         * the class ids come back from the device (the host copy is freed after set-up) */
+        const size_t ib = E->idbytes_e;
+        std::vector<uint8_t> ids((size_t)g.plane * g.nz * ib);
+        CK(hipSetDevice(E->dev));
+        CK(hipMemcpy(ids.data(), g.ecls, ids.size(), hipMemcpyDeviceToHost));
+        for (int k = 0; k < g.nz; k++)
+            for (int j = 0; j < g.ny; j++)
+                for (int i = 0; i < g.nx; i++) {
+                    const size_t c = (size_t)i + (size_t)g.px * j + (size_t)g.plane * k;
+                    const unsigned cls = ib == 1 ? ids[c] : ((const uint16_t *)ids.data())[c];
+                    if (!(E->etab[cls].flags & GSV_E_ADE))
+                        continue;
+                    const bool slab = (g.f[0].on && i < g.f[0].depth) || (g.f[1].on && i >= g.nx - g.f[1].depth)
+                                      || (g.f[2].on && j < g.f[2].depth) || (g.f[3].on && j >= g.ny - g.f[3].depth)
+                                      || (g.f[4].on && k < g.f[4].depth) || (g.f[5].on && k >= g.nz - g.f[5].depth);
+                    const bool pc = i >= g.pc_i0 && i < g.pc_i0 + g.pc_nx && j >= g.pc_j0 && j < g.pc_j0 + g.pc_ny
+                                    && k >= g.pc_k0 && k < g.pc_k0 + g.pc_nz;
+                    if (slab || pc) {
+                        fprintf(stderr, "Error: the gradient needs dispersive cells outside the CPML slabs and the design "
+                                        "region (cell %d %d %d)\n", i, j, k);
+                        return 1;
+                    }
+                }
     }
     if (E->has_src)
         for (size_t q = 0; q < (size_t)E->src.nsteps * E->src.n; q++)
@@ -620,7 +750,8 @@ gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_int
     /* the slab kernels recover E'' = (adjoint E) / ca and H'' = (adjoint H) / da */
     for (const GsvECoef &c : E->etab)
         if (!(c.flags & GSV_E_PERCELL)
-            && ((c.flags & (GSV_E_PECX | GSV_E_PECY | GSV_E_PECZ)) || c.ca[0] == 0.f || c.ca[1] == 0.f || c.ca[2] == 0.f)) {
+            && ((c.flags & (GSV_E_PECX | GSV_E_PECY | GSV_E_PECZ))
+                || (!(c.flags & GSV_E_ADE) && (c.ca[0] == 0.f || c.ca[1] == 0.f || c.ca[2] == 0.f)))) {
             fprintf(stderr, "Error: the gradient supports no PEC cells and needs nonzero E update factors\n");
             return 1;
         }
@@ -663,6 +794,10 @@ gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_int
         }
     }
     if (dmalloc(E, &a.gca, 3 * (size_t)a.ncoef) || dmalloc(E, &a.gcb, 3 * (size_t)a.ncoef)) { delete A; return 1; }
+    if (g.ade) {
+        if (dmalloc(E, &a.xad, (size_t)g.ade_plane_d * g.nz)) { delete A; return 1; }
+        if (g.ade_p && dmalloc(E, &a.xap, (size_t)g.ade_plane_p * g.nz)) { delete A; return 1; }
+    }
     /* history: all steps if they fit the budget, else one segment of K steps */
     size_t per_step = 6 * (size_t)a.ncoef * 4;
     A->K = ckpt_interval > 0 ? ckpt_interval : 100;
@@ -670,7 +805,7 @@ gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_int
     A->hist_slots = A->history_all ? A->nsteps : (A->K < A->nsteps ? A->K : A->nsteps);
     if (dmalloc(E, &a.hist, per_step / 4 * A->hist_slots)) { delete A; return 1; }
     if (!A->history_all) {
-        size_t nf = 6 * E->field_floats;
+        size_t nf = 6 * E->field_floats + ade_floats(g);
         for (int f = 0; f < 6; f++)
             nf += 2 * 2 * psi_pairs(g.f[f], f / 2, g);
         A->ckpt_floats = nf;
@@ -715,6 +850,22 @@ ckpt_copy(Engine *E, float *buf, int save)
             else
                 CK(hipMemcpyAsync(ps[s], buf + off, n * 8, hipMemcpyDeviceToDevice, E->st));
             off += 2 * n;
+        }
+    }
+    if (g.ade) {
+        /* the dispersive state travels with the fields */
+        const size_t nd = (size_t)g.ade_plane_d * g.nz, np = g.ade_p ? (size_t)g.ade_plane_p * g.nz : 0;
+        if (save)
+            CK(hipMemcpyAsync(buf + off, g.ade_d, nd * 4, hipMemcpyDeviceToDevice, E->st));
+        else
+            CK(hipMemcpyAsync(g.ade_d, buf + off, nd * 4, hipMemcpyDeviceToDevice, E->st));
+        off += nd;
+        if (np) {
+            if (save)
+                CK(hipMemcpyAsync(buf + off, g.ade_p, np * 4, hipMemcpyDeviceToDevice, E->st));
+            else
+                CK(hipMemcpyAsync(g.ade_p, buf + off, np * 4, hipMemcpyDeviceToDevice, E->st));
+            off += np;
         }
     }
     return 0;
@@ -772,6 +923,11 @@ gsv_engine_reset_state(Engine *E)
             CK(hipMemsetAsync(g.f[f].pe, 0, n * 8, E->st));
             CK(hipMemsetAsync(g.f[f].ph, 0, n * 8, E->st));
         }
+    }
+    if (g.ade) {
+        CK(hipMemsetAsync(g.ade_d, 0, (size_t)g.ade_plane_d * g.nz * 4, E->st));
+        if (g.ade_p)
+            CK(hipMemsetAsync(g.ade_p, 0, (size_t)g.ade_plane_p * g.nz * 4, E->st));
     }
     CK(hipStreamSynchronize(E->st));
     return 0;
@@ -852,7 +1008,16 @@ launch_adj(Engine *E, int step, int slot)
     if (win[0]) hipLaunchKernelGGL(k_slab_e_pre<0>, GSV_WIN(win[0]), g, a);
     if (win[1]) hipLaunchKernelGGL(k_slab_e_pre<1>, GSV_WIN(win[1]), g, a);
     if (win[2]) hipLaunchKernelGGL(k_slab_e_pre<2>, GSV_WIN(win[2]), g, a);
-    hipLaunchKernelGGL((k_adj_e<ID, uint32_t>), grd, blk, 0, E->st, g, a, step, slot, kc);
+    /* the dispersive branch as in the forward E kernel: none, Drude only, 2 or 3 poles */
+    const int ade = g.ade ? (g.ade_poles ? g.ade_poles : 1) : 0;
+    if (ade == 0)
+        hipLaunchKernelGGL((k_adj_e<ID, uint32_t, 0>), grd, blk, 0, E->st, g, a, step, slot, kc);
+    else if (ade == 1)
+        hipLaunchKernelGGL((k_adj_e<ID, uint32_t, 1>), grd, blk, 0, E->st, g, a, step, slot, kc);
+    else if (ade == 2)
+        hipLaunchKernelGGL((k_adj_e<ID, uint32_t, 2>), grd, blk, 0, E->st, g, a, step, slot, kc);
+    else
+        hipLaunchKernelGGL((k_adj_e<ID, uint32_t, 3>), grd, blk, 0, E->st, g, a, step, slot, kc);
     if (fmax) hipLaunchKernelGGL(k_slab_e_post<ID>, dim3((unsigned)((fmax + 255) / 256), 6), dim3(256), 0, E->st, g, a);
     if (win[0]) hipLaunchKernelGGL(k_slab_h_pre<0>, GSV_WIN(win[0]), g, a);
     if (win[1]) hipLaunchKernelGGL(k_slab_h_pre<1>, GSV_WIN(win[1]), g, a);
@@ -895,6 +1060,10 @@ gsv_engine_adjoint_backward(void *eng, int (*step_fn)(void *), void *ctx, float 
     }
     CK(hipMemsetAsync(a.gca, 0, 3 * a.ncoef * 8, E->st));
     CK(hipMemsetAsync(a.gcb, 0, 3 * a.ncoef * 8, E->st));
+    if (a.xad)
+        CK(hipMemsetAsync(a.xad, 0, (size_t)g.ade_plane_d * g.nz * 4, E->st));
+    if (a.xap)
+        CK(hipMemsetAsync(a.xap, 0, (size_t)g.ade_plane_p * g.nz * 4, E->st));
     CK(hipEventRecord(e0, E->st));
     const int nseg = A->history_all ? 1 : (A->nsteps + A->K - 1) / A->K;
     for (int s = nseg - 1; s >= 0; s--) {

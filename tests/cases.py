@@ -629,13 +629,15 @@ def cw_ramp(nsteps, dt, lam, ramp_periods=3.0, amplitude=1.0):
 
 
 def inverse_design(folder, n=256, nsteps=3000, design=(64, 64, 32), dx=10e-9, cpml=12, seed=5, lam=1.0e-6,
-                   eps_range=(1.0, 12.0), gap=8, magnetic_voxel=False, dtmult=1.0):
+                   eps_range=(1.0, 12.0), gap=8, magnetic_voxel=False, dtmult=1.0, dispersive=None):
     """Differentiable design problem: a design box of per-voxel eps_r ~ U[eps_range]
     (MEDIUM_LINEAR) centred in x, y; an Ez dipole `gap` cells below it driven by
     a CW ramp at lam; the objective is the DFT |E|^2 at lam summed over a plane
     of the design's x-y extent `gap` cells above it.  CPML on all faces.
     magnetic_voxel puts mu_r = 2 in one voxel far from the design, which makes
-    the planner average the E coefficients at material interfaces."""
+    the planner average the E coefficients at material interfaces.
+    dispersive = (voxel box i0 j0 k0 i1 j1 k1, material text) adds a MEDIUM_VECTOR
+    voxel entity of that material (e.g. an ADE metal block beside the design)."""
     os.makedirs(folder, exist_ok=True)
     rng = np.random.default_rng(seed)
     dnx, dny, dnz = design
@@ -663,6 +665,11 @@ def inverse_design(folder, n=256, nsteps=3000, design=(64, 64, 32), dx=10e-9, cp
              "SOURCE_POINT\n%d %d %d 0 dipole.txt\n\n" % (c, c, sk),
              "DESIGN_BOX\n%d %d %d %d %d %d\n\n" % box,
              "OBJECTIVE_DFT\n%d %d %d %d %d %d %.9g\n\n" % (box[0], box[1], mk, box[3], box[4], mk + 1, lam)]
+    if dispersive:
+        vb, mtext = dispersive
+        with open(os.path.join(folder, "dispersive.txt"), "w") as f:
+            f.write("8 %d %d %d %d %d %d %s\n" % (tuple(vb) + (mtext,)))
+        lines.insert(4, "MEDIUM_VECTOR\ndispersive.txt\n\n")
     path = os.path.join(folder, "inverse_design.par")
     with open(path, "w") as f:
         f.write("".join(lines))
