@@ -576,6 +576,20 @@ k_estep(GsvGrid g, int kchunk)
                                 blockIdx.z, kchunk);
 }
 
+/* The same column held to seven waves per SIMD (at most 72 VGPRs), used for
+ * the builds that fit that bound without spilling: no poles, 32-bit offsets,
+ * no design region, no periodic axis (the Drude build needs 71 VGPRs, against
+ * 74 and six waves unconstrained).  More waves hide more of the z-march's
+ * load latency: 512^3 Drude pool, 256 x 1 x 8 tiles, 1.918 -> 1.805 ms with
+ * bit-identical fields. */
+template <typename ID, int ADE>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8))) void
+k_estep_w7(GsvGrid g, int kchunk)
+{
+    estep_col<ID, ADE, uint32_t, false, false>(g, blockIdx.x * blockDim.x + threadIdx.x,
+                                               blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z, kchunk);
+}
+
 /* ------------------------------------------------------------------ */
 /* TSF plane wave                                                       */
 /* ------------------------------------------------------------------ */
@@ -1396,7 +1410,13 @@ template <typename ID, typename OFF, bool PC, bool PER>
 static void
 launch_estep(const GsvGrid *g, int ade, dim3 grd, dim3 blk, int kchunk, hipStream_t st)
 {
-    if (ade == 0)
+    constexpr bool w7 = sizeof(OFF) == 4 && !PC && !PER;
+    if (w7 && ade <= 1) {
+        if (ade == 0)
+            hipLaunchKernelGGL((k_estep_w7<ID, 0>), grd, blk, 0, st, *g, kchunk);
+        else
+            hipLaunchKernelGGL((k_estep_w7<ID, 1>), grd, blk, 0, st, *g, kchunk);
+    } else if (ade == 0)
         hipLaunchKernelGGL((k_estep<ID, 0, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
     else if (ade == 1)
         hipLaunchKernelGGL((k_estep<ID, 1, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
@@ -1635,6 +1655,18 @@ k_estep_batch(const GsvGrid *__restrict__ grids, const GsvTile *__restrict__ til
     }
 }
 
+/* one workgroup per tile: without the resident loop the E column needs fewer
+ * registers (80 against 83 VGPRs for the Drude build, six waves per SIMD
+ * instead of five; a seven-wave bound would spill) */
+template <typename ID, int ADE, typename OFF, bool PER>
+__global__ __launch_bounds__(512) void
+k_estep_tile(const GsvGrid *__restrict__ grids, const GsvTile *__restrict__ tiles, int kchunk)
+{
+    const GsvTile e = tiles[blockIdx.x];
+    estep_col<ID, ADE, OFF, false, PER>(grids[e.run], e.tx * blockDim.x + threadIdx.x, e.ty * blockDim.y + threadIdx.y,
+                                        e.kz, kchunk);
+}
+
 /* one block per grid with a TSF line; the grid's step counter lives in device
  * memory so that a captured step sequence replays with advancing steps */
 __global__ __launch_bounds__(1024) void
@@ -1693,6 +1725,17 @@ static void
 launch_estep_batch(const GsvGrid *grids, const GsvTile *tiles, int ntiles, int ade, dim3 grd, dim3 blk, int kchunk,
                    hipStream_t st)
 {
+    if ((int)grd.x == ntiles) {
+        if (ade == 0)
+            hipLaunchKernelGGL((k_estep_tile<ID, 0, uint32_t, PER>), grd, blk, 0, st, grids, tiles, kchunk);
+        else if (ade == 1)
+            hipLaunchKernelGGL((k_estep_tile<ID, 1, uint32_t, PER>), grd, blk, 0, st, grids, tiles, kchunk);
+        else if (ade == 2)
+            hipLaunchKernelGGL((k_estep_tile<ID, 2, uint32_t, PER>), grd, blk, 0, st, grids, tiles, kchunk);
+        else
+            hipLaunchKernelGGL((k_estep_tile<ID, 3, uint32_t, PER>), grd, blk, 0, st, grids, tiles, kchunk);
+        return;
+    }
     if (ade == 0)
         hipLaunchKernelGGL((k_estep_batch<ID, 0, uint32_t, PER>), grd, blk, 0, st, grids, tiles, ntiles, kchunk);
     else if (ade == 1)
