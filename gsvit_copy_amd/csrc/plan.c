@@ -853,7 +853,8 @@ test_script_file(const SvSet *set, const char *fname)
             is_e = 1;
         } else if (mtype == SV_MAT_PEC) {
         } else {
-            fprintf(stderr, "Error: material type %d is not supported by the MI355X engine\n", mtype);
+            fprintf(stderr, "Error: material type %d is not supported by the MI355X engine (supported: 0, 1, 5 and 7 "
+                            "ADE Drude + critical points, 10 PEC)\n", mtype);
             goto err;
         }
     }
