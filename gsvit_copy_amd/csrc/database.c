@@ -357,3 +357,23 @@ sv_open_vector_file(const SvSet *set, const char *fname)
     rewind(fw);
     return fw;
 }
+
+int
+sv_database_linear(const SvSet *set, const char *name, double *eps, double *mu, double *sigma, double *sigast)
+{
+    FILE *fw = tmpfile();
+    int type = -1, rc = 1;
+    if (!fw) {
+        fprintf(stderr, "Error: cannot create a temporary material file\n");
+        return 1;
+    }
+    if (!write_database_material(set, name, fw)) {
+        rewind(fw);
+        if (fscanf(fw, "%d", &type) == 1 && type == SV_MAT_LINTAB && fscanf(fw, "%lf %lf %lf %lf", eps, mu, sigma, sigast) == 4)
+            rc = 0;
+        else
+            fprintf(stderr, "Error: database entry %s is not a linear (non-dispersive) material\n", name);
+    }
+    fclose(fw);
+    return rc;
+}

@@ -18,6 +18,8 @@ extern "C" {
 int sv_generate_waveform(int type, int nsteps, double wavelength, int pulse_width, double dt,
                          double amplitude, double *vals);
 double sv_round_g6(double v);
+int sv_generate_waveform_ltsf(int type, int nsteps, double wavelength, int pulse_width, double dt,
+                              double amplitude, double *vals);
 
 /* plan.c */
 int sv_material_is_tabulated(int type);   /* LINTAB, DRUDE, PLRC, ADE, CP, CP3 */
@@ -56,6 +58,9 @@ void sv_point_spectrum(const double *x, int n, double *out);
  * file holding its rewrite when it names optical-database materials (type
  * 99, database.c); NULL if it cannot be opened or resolved */
 FILE *sv_open_vector_file(const SvSet *set, const char *fname);
+/* a non-dispersive optical-database entry as relative eps, mu and sigma,
+ * sigma* (as a vector-file entry resolves it); 1 on error */
+int sv_database_linear(const SvSet *set, const char *name, double *eps, double *mu, double *sigma, double *sigast);
 
 #ifdef __cplusplus
 }

@@ -557,30 +557,43 @@ load_sources(SvPool *mp, SvSet *set)
             fprintf(stderr, "Error: LTSF with a material skip depth is not supported\n");
             return 1;
         }
-        fr = fopen(t->source_filename, "r");
-        if (!fr) {
-            fprintf(stderr, "Cannot load source file %s\n", t->source_filename);
-            return 1;
-        }
-        if (fscanf(fr, "%d", &n) != 1 || n <= 0) {
-            fprintf(stderr, "Error: cannot read number of values from file %s\n", t->source_filename);
-            fclose(fr);
-            return 1;
-        }
-        if (n < 3 * set->sc.nsteps)
-            fprintf(stderr, "Warning: less data (%d) in \"%s\" than scheduled computation steps (%d) in LTSF source "
-                            "(need to be three times more)\n", n, t->source_filename, set->sc.nsteps);
-        ls->ndata = n;
-        ls->e = (double *)calloc(n, sizeof(double));
-        for (q = 0; q < n; q++) {
-            int pos;
-            if (fscanf(fr, "%d %lf", &pos, &ls->e[q]) != 2) {
-                fprintf(stderr, "Error: cannot read LTSF sample %d from %s\n", q, t->source_filename);
+        if (t->source_mode == 1 || t->source_mode == 2) {
+            /* generated at dt / 3 and read back through "%g" as the reference's
+             * tmp_source_ltsf file (src3d/settings.c:4420-4428, 445-512) */
+            n = 3 * set->sc.nsteps;
+            ls->ndata = n;
+            ls->e = (double *)calloc(n, sizeof(double));
+            if (sv_generate_waveform_ltsf(t->source_mode, set->sc.nsteps, t->source_wl, t->source_pulsewidth,
+                                          set->plan.dt, t->source_amplitude, ls->e))
+                return 1;
+            for (q = 0; q < n; q++)
+                ls->e[q] = sv_round_g6(ls->e[q]);
+        } else {
+            fr = fopen(t->source_filename, "r");
+            if (!fr) {
+                fprintf(stderr, "Cannot load source file %s\n", t->source_filename);
+                return 1;
+            }
+            if (fscanf(fr, "%d", &n) != 1 || n <= 0) {
+                fprintf(stderr, "Error: cannot read number of values from file %s\n", t->source_filename);
                 fclose(fr);
                 return 1;
             }
+            if (n < 3 * set->sc.nsteps)
+                fprintf(stderr, "Warning: less data (%d) in \"%s\" than scheduled computation steps (%d) in LTSF source "
+                                "(need to be three times more)\n", n, t->source_filename, set->sc.nsteps);
+            ls->ndata = n;
+            ls->e = (double *)calloc(n, sizeof(double));
+            for (q = 0; q < n; q++) {
+                int pos;
+                if (fscanf(fr, "%d %lf", &pos, &ls->e[q]) != 2) {
+                    fprintf(stderr, "Error: cannot read LTSF sample %d from %s\n", q, t->source_filename);
+                    fclose(fr);
+                    return 1;
+                }
+            }
+            fclose(fr);
         }
-        fclose(fr);
         ls->corr = 1.0 / sqrt(pow(sin(ls->ia_theta), 4) * (pow(cos(ls->ia_phi), 4) + pow(sin(ls->ia_phi), 4))
                               + pow(cos(ls->ia_theta), 4));
         ls->zdx = set->sp.dx / ls->corr;
@@ -594,13 +607,22 @@ load_sources(SvPool *mp, SvSet *set)
         for (k = 0; k < zres; k++)
             ls->zeps[k] = ls->zmu[k] = 1;
         for (q = 0; q < t->layered_count; q++) {
+            double eps = t->layered_epsilon[q], mu = t->layered_mu[q];
             if (t->layered_zpos[q] < 0) {
                 fprintf(stderr, "Error: LTSF layer %d starts below z = 0\n", q);
                 return 1;
             }
+            if (t->layered_material[q][0]) {
+                /* a database layer: the reference reads its name and leaves the
+                 * layer's eps at 0 (src3d/settings.c:4372-4376); here it is
+                 * resolved at the centre wavelength like a vector-file entry */
+                double sg, sa;
+                if (sv_database_linear(set, t->layered_material[q], &eps, &mu, &sg, &sa))
+                    return 1;
+            }
             for (k = t->layered_zpos[q]; k < zres; k++) {
-                ls->zeps[k] = t->layered_epsilon[q];
-                ls->zmu[k] = t->layered_mu[q];
+                ls->zeps[k] = eps;
+                ls->zmu[k] = mu;
             }
         }
     }

@@ -217,11 +217,11 @@ round_g6(double v)
 
 /* Waveform of a generated source, src3d/settings.c:444-512 (SRC_SINE = 1,
  * SRC_PULSE = 2); values rounded as the reference's "%g" round trip. */
-int
-sv_generate_waveform(int type, int nsteps, double wavelength, int pulse_width, double dt,
-                     double amplitude, double *vals)
+static int
+generate_waveform(int type, int nsteps, double wavelength, int pulse_width, double dt, double amplitude, int dampval,
+                  double *vals)
 {
-    int i, dampval = 100;
+    int i;
     double wavefreq = SV_LIGHT_SPEED * 2 * M_PI / wavelength;
     double dpulse_width = pulse_width * dt;
 
@@ -240,6 +240,22 @@ sv_generate_waveform(int type, int nsteps, double wavelength, int pulse_width, d
         vals[i] = v;
     }
     return 0;
+}
+
+int
+sv_generate_waveform(int type, int nsteps, double wavelength, int pulse_width, double dt,
+                     double amplitude, double *vals)
+{
+    return generate_waveform(type, nsteps, wavelength, pulse_width, dt, amplitude, 100, vals);
+}
+
+/* the layered-TSF form (src3d/settings.c:460-465): 3 nsteps samples at dt / 3,
+ * pulse width and ramp in substeps (tripled) */
+int
+sv_generate_waveform_ltsf(int type, int nsteps, double wavelength, int pulse_width, double dt,
+                          double amplitude, double *vals)
+{
+    return generate_waveform(type, 3 * nsteps, wavelength, 3 * pulse_width, dt / 3, amplitude, 300, vals);
 }
 
 static double
@@ -485,9 +501,14 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
                 char *end;
                 if (get_int(fr, &t->layered_zpos[q], key) || get_token(fr, value, sizeof(value)))
                     goto fail;
+                t->layered_material[q][0] = 0;
                 if (!strcmp(value, "99")) {
-                    fprintf(stderr, "Error: SOURCE_LTSF database materials (99) are not supported\n");
-                    goto fail;
+                    /* a database entry, resolved at plan time */
+                    if (get_token(fr, t->layered_material[q], SV_NAME_LEN))
+                        goto fail;
+                    t->layered_epsilon[q] = t->layered_mu[q] = 1;
+                    t->layered_sigma[q] = t->layered_sigast[q] = 0;
+                    continue;
                 }
                 t->layered_epsilon[q] = strtod(value, &end);
                 if (*end != '\0') {
@@ -501,11 +522,20 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             if (get_int(fr, &srctype, key))
                 goto fail;
             t->source_mode = srctype;
-            if (srctype != 0) {
-                fprintf(stderr, "Error: SOURCE_LTSF needs a waveform file (type 0) sampled at dt/3\n");
+            if (srctype == 1 || srctype == 2) {
+                /* generated at plan time, src3d/settings.c:4400-4428 */
+                if (set->sc.nsteps == 0) {
+                    fprintf(stderr, "Error: COMP parameter (number of steps) should precede generated source command.\n");
+                    goto fail;
+                }
+                if (get_double(fr, &t->source_wl, key) || (srctype == 2 && get_int(fr, &t->source_pulsewidth, key))
+                    || get_double(fr, &t->source_amplitude, key))
+                    goto fail;
+            } else if (srctype != 0) {
+                fprintf(stderr, "Error: SOURCE_LTSF source type %d is not supported (0 file, 1 sine, 2 pulse; the "
+                                "broadband type 3 needs Gwyddion's FFT)\n", srctype);
                 goto fail;
-            }
-            if (get_rest_of_line(fr, t->source_filename, SV_NAME_LEN))
+            } else if (get_rest_of_line(fr, t->source_filename, SV_NAME_LEN))
                 goto fail;
             t->valid = 1;
         } else if (!strcmp(key, "LTSF_SKIP")) {

@@ -193,8 +193,14 @@ typedef struct {
     int layered_zpos[SV_LTSF_MAX_LAYERS];
     double layered_epsilon[SV_LTSF_MAX_LAYERS], layered_mu[SV_LTSF_MAX_LAYERS];
     double layered_sigma[SV_LTSF_MAX_LAYERS], layered_sigast[SV_LTSF_MAX_LAYERS];
+    /* a layer given as 99 and an optical-database entry: its name (empty otherwise) */
+    char layered_material[SV_LTSF_MAX_LAYERS][SV_NAME_LEN];
     int source_mode;
     char source_filename[SV_NAME_LEN];
+    /* generated waveform (source_mode 1 sine, 2 pulse), sampled at dt/3 */
+    double source_wl;
+    double source_amplitude;
+    int source_pulsewidth;
     int box_boundary_skipi0, box_boundary_skipj0, box_boundary_skipk0;
     int box_boundary_skipin, box_boundary_skipjn, box_boundary_skipkn;
     int box_boundary_skipdepth;

@@ -18,12 +18,15 @@ what the physics fixes:
 The HIP path is checked against the oracle (GPU tests below).
 """
 import math
+import os
 
 import numpy as np
 import pytest
 
 import cases
 from oracle_lib import Oracle
+
+C0 = 299792458.0
 
 
 def history(folder, theta, psi, layers):
@@ -92,11 +95,92 @@ def test_planner_rejects_what_is_not_supported(tmp_path):
     par = cases.ltsf_case(str(tmp_path / "d"), **base)
     with open(par) as f:
         txt = f.read()
-    with open(par, "w") as f:                  # a database material for the layer
+    with open(par, "w") as f:                  # a database layer whose entry does not exist
         f.write(txt.replace(" 0  0 ltsf.txt", " 1 5 99 Au 0 ltsf.txt"))
     assert "99 Au" in open(par).read()
     with pytest.raises(RuntimeError):
         Oracle(par)
+    with open(par, "w") as f:                  # the broadband generator (Gwyddion FFT) is not built
+        f.write(txt.replace(" 0 ltsf.txt", " 3 400e-9 100e-9 1"))
+    with pytest.raises(RuntimeError):
+        Oracle(par)
+
+
+def reference_ltsf_waveform(kind, nsteps, lam, pulse_width, dt, amplitude):
+    """the samples src3d/settings.c:445-512 writes for a generated LTSF source
+    (3 nsteps samples at dt / 3, pulse width and ramp tripled), as "%g" text"""
+    n, h, pw, ramp = 3 * nsteps, dt / 3, 3 * pulse_width, 300
+    w = C0 * 2 * math.pi / lam
+    out = []
+    for i in range(n):
+        v = amplitude * math.sin(w * h * i)
+        if i < ramp:
+            v *= (1.0 / ramp) * i
+        if kind == 2:
+            d = (i * h - 4.0 * pw * h) / math.sqrt(2.0) / (pw * h)
+            v *= math.exp(-d * d)
+        out.append(float("%g" % v))
+    return out
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_generated_waveform_equals_the_file_the_reference_writes(tmp_path, kind):
+    """SOURCE_LTSF with a generated sine (1) or pulse (2): the run equals the
+    same run reading the waveform file that the reference writes for it; its
+    every third sample is the TSF generator's sample at the same time (the
+    LTSF form only refines the time grid)"""
+    from cases import _dt
+    nsteps, lam, pw, amp = 160, 400e-9, 12, 1.3
+    base = dict(shape=(20, 20, 48), nsteps=nsteps, box=(5, 5, 5, 15, 15, 38), layers=[(30, 2.25)])
+    dt = _dt(10e-9, 10e-9, 10e-9, 0.5)
+    wave = reference_ltsf_waveform(kind, nsteps, lam, pw, dt, amp)
+    gen = cases.ltsf_case(str(tmp_path / "gen"), **base)
+    spec = ("1 %g %g" % (lam, amp)) if kind == 1 else ("2 %g %d %g" % (lam, pw, amp))
+    txt = open(gen).read()
+    with open(gen, "w") as f:
+        f.write(txt.replace(" 0 ltsf.txt", " " + spec))
+    filed = cases.ltsf_case(str(tmp_path / "file"), **base)
+    with open(os.path.join(str(tmp_path / "file"), "ltsf.txt"), "w") as f:
+        f.write("%d\n" % len(wave) + "".join("%d %.17g\n" % (i, v) for i, v in enumerate(wave)))
+    out = []
+    for par in (gen, filed):
+        o = Oracle(par)
+        o.step()
+        out.append(o.fields())
+        o.close()
+    assert np.array_equal(out[0], out[1]) and np.abs(out[0]).max() > 1e-3
+    # the TSF generator's samples (one per dt) are every third LTSF sample
+    w = C0 * 2 * math.pi / lam
+    for i in range(0, nsteps, 7):
+        v = amp * math.sin(w * dt * i) * (i / 100.0 if i < 100 else 1.0)
+        if kind == 2:
+            d = (i * dt - 4.0 * pw * dt) / math.sqrt(2.0) / (pw * dt)
+            v *= math.exp(-d * d)
+        assert wave[3 * i] == pytest.approx(float("%g" % v), rel=2e-5, abs=1e-12)
+
+
+def test_database_layer_equals_its_resolved_permittivity(tmp_path):
+    """a layer given as 99 and a database entry runs like the layer written with
+    the entry's permittivity at the centre wavelength (the reference reads the
+    name and leaves the layer's eps at 0)"""
+    base = dict(shape=(20, 20, 48), nsteps=160, box=(5, 5, 5, 15, 15, 38), layers=[(30, 2.0)],
+                extra=["SOURCE_WAVELENGTH\n300e-9 400e-9 500e-9"])
+    db = cases.ltsf_case(str(tmp_path / "db"), **base)
+    with open(os.path.join(str(tmp_path / "db"), "glass.txt"), "w") as f:
+        f.write("cauchy 0.2 1.0\nsome glass\n1.5 0.004 0\n")
+    txt = open(db).read()
+    assert " 30 2 1 0 0 " in txt
+    with open(db, "w") as f:
+        f.write(txt.replace(" 30 2 1 0 0 ", " 30 99 glass.txt "))
+    n = float(np.float32(1.5)) + float(np.float32(0.004)) / 0.4 ** 2
+    direct = cases.ltsf_case(str(tmp_path / "direct"), **dict(base, layers=[(30, float("%g" % (n * n)))]))
+    out = []
+    for par in (db, direct):
+        o = Oracle(par)
+        o.step()
+        out.append(o.fields())
+        o.close()
+    assert np.array_equal(out[0], out[1]) and np.abs(out[0]).max() > 1e-3
 
 
 GPU_CASES = {
