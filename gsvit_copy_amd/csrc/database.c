@@ -31,8 +31,10 @@
  *                the header range); n and k interpolated linearly at the centre
  *                wavelength give eps = n^2 - k^2, sigma = 4 pi n k c eps0 / lambda
  *                (src3d/plan.c:1028-1125).
- * Material types 2, 3, 4 and 6 are rewritten as the reference does and then
- * refused by the planner like the same types written directly.
+ * Material types 2, 3, 4 and 6 are rewritten as the reference does (a PLRC
+ * vector-file line comes back as type 5) and then planned like the same
+ * types written directly: types 2 and 4 as recursive convolution, type 3
+ * refused.
  */
 #include <stdio.h>
 #include <stdlib.h>

@@ -274,6 +274,10 @@ build_tables(Engine *E)
     E->g.ade_poles = 0;
     for (n = 0; n < mp->nmat; n++) {
         const SvMatProp *m = mp->mats + n;
+        if (m->type == SV_MAT_DRUDE || m->type == SV_MAT_CP || m->type == SV_MAT_PLRC) {
+            fprintf(stderr, "Error: recursive-convolution materials (types 2, 4, 6) are not yet supported on the GPU\n");
+            return 1;
+        }
         if (m->type != SV_MAT_ADE)
             continue;
         if (nslot >= GSV_MAX_ADE) {

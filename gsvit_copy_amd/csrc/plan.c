@@ -21,6 +21,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+#include <complex.h>
 #include "gsvit_internal.h"
 
 #define EPS0 SV_EPSILON_0
@@ -662,6 +663,44 @@ calculate_ade(SvMatProp *mat, double epsilon, double sigma, double omegap, doubl
     }
 }
 
+/* Susceptibility integrals of the piecewise-linear recursive convolution
+ * (type 6), src3d/plan.c:2765-2824: with chi(t) the material's
+ * susceptibility, chi0 = int_0^dt chi, xi0 = (1/dt) int_0^dt t chi, and the
+ * increments dchi = chi0 - chi1, dxi = xi0 - xi1 of the next interval; the
+ * Drude term chi(t) = (wp^2/nu)(1 - e^(-nu t)) is real, a critical point
+ * chi(t) = 2 A W e^(-G t) sin(W t - phi) is carried as a complex exponential
+ * (conjugate convention e^((-G - iW) t)) whose real part is the physical one. */
+static void
+calculate_plrc(SvMatProp *mat, double dt)
+{
+    const double w2 = mat->drude_omega_p * mat->drude_omega_p / (mat->drude_nu * mat->drude_nu);
+    const double nd = mat->drude_nu * dt;
+    const double lin = 1 / (1 - exp(nd)) + 1 / nd;
+    int q;
+    mat->plrc_d_chi = w2 * (exp(-nd) + nd - 1);
+    mat->plrc_d_xi = mat->plrc_d_chi * lin - w2 * (1 - nd / 2 * (1 / tanh(nd / 2)));
+    mat->plrc_d_dchi = -w2 * (1 - exp(-nd)) * (1 - exp(-nd));
+    mat->plrc_d_dxi = mat->plrc_d_dchi * lin;
+    for (q = 0; q < 2; q++) {
+        const double a = mat->cp3_a[q], phi = mat->cp3_phi[q], om = mat->cp3_omega[q], ga = mat->cp3_gamma[q];
+        /* amplitude -2 A W (sin phi - i cos phi), one-interval factor 1 - e^((-G - iW) dt) */
+        const double complex amp = -2 * a * om * sin(phi) + I * (2 * a * om * cos(phi));
+        const double complex f1 = (1 - exp(-dt * ga) * cos(dt * om)) + I * (exp(-dt * ga) * sin(dt * om));
+        /* the linear-interpolation terms divide by 1 - e^((G + iW) dt) and by (G + iW) dt */
+        const double complex fe = (1 - exp(dt * ga) * cos(dt * om)) - I * (exp(dt * ga) * sin(dt * om));
+        const double complex fl = dt * ga + I * (dt * om);
+        const double complex chi0 = amp * f1 / (ga + I * om);
+        const double complex dchi = chi0 * f1;
+        const double complex dxi = dchi / fe + dchi / fl;
+        mat->plrc_p_chi[q] = creal(chi0);
+        mat->plrc_p_xi[q] = creal(chi0 / fe + chi0 / fl);
+        mat->plrc_p_dchir[q] = creal(dchi);
+        mat->plrc_p_dchii[q] = cimag(dchi);
+        mat->plrc_p_dxir[q] = creal(dxi);
+        mat->plrc_p_dxii[q] = cimag(dxi);
+    }
+}
+
 static int
 scan_floats(FILE *fr, float *v, int n)
 {
@@ -693,6 +732,32 @@ scan_material(FILE *fr, SvMatProp *mat, double dt)
         mat->sigma = v[2];
         mat->sigast = v[3];
     } else if (type == SV_MAT_PEC) {
+    } else if (type == SV_MAT_DRUDE) {
+        /* epsilon, omega_p, nu (omega_p and nu in the units the reference
+         * scales by EV_TO_J after loading, src3d/plan.c:4673-4677) */
+        if (scan_floats(fr, v, 3))
+            return 1;
+        mat->epsilon = v[0];
+        mat->drude_omega_p = v[1];
+        mat->drude_nu = v[2];
+    } else if (type == SV_MAT_CP || type == SV_MAT_PLRC) {
+        /* CP: epsilon, omega_p, nu, 2 x (A, phi, Omega, Gamma); PLRC: epsilon,
+         * sigma, omega_p, nu, 2 x (A, phi, Omega, Gamma) */
+        const int off = type == SV_MAT_PLRC;
+        if (scan_floats(fr, v, 11 + off))
+            return 1;
+        mat->epsilon = v[0];
+        mat->sigma = off ? v[1] : 0;
+        mat->drude_omega_p = v[1 + off];
+        mat->drude_nu = v[2 + off];
+        for (q = 0; q < 2; q++) {
+            mat->cp3_a[q] = v[3 + off + 4 * q];
+            mat->cp3_phi[q] = v[4 + off + 4 * q];
+            mat->cp3_omega[q] = v[5 + off + 4 * q];
+            mat->cp3_gamma[q] = v[6 + off + 4 * q];
+        }
+        if (type == SV_MAT_PLRC)
+            calculate_plrc(mat, dt);
     } else if (type == SV_MAT_ADE || type == SV_MAT_ADE3) {
         np = type == SV_MAT_ADE3 ? 3 : 2;
         if (scan_floats(fr, v, 4 + 4 * np))
@@ -712,8 +777,9 @@ scan_material(FILE *fr, SvMatProp *mat, double dt)
         calculate_ade(mat, v[0], v[1], v[2], v[3], dt);
     } else {
         fprintf(stderr, "Error: material type %d is not supported by the MI355X engine "
-                "(supported: 0 linear, 1 tabulated linear, 5 ADE Drude + 2 critical points, 7 ADE Drude + 3 "
-                "critical points, 10 PEC)\n", type);
+                "(supported: 0 linear, 1 tabulated linear, 2 recursive-convolution Drude, 4 recursive-convolution "
+                "Drude + 2 critical points, 5 ADE Drude + 2 critical points, 6 piecewise-linear recursive "
+                "convolution, 7 ADE Drude + 3 critical points, 10 PEC)\n", type);
         return 1;
     }
     return 0;
@@ -869,14 +935,16 @@ test_script_file(const SvSet *set, const char *fname)
                 if (p[0] != 1.0 || p[2] > 0) is_e = 1;
                 if (p[1] != 1.0 || p[3] > 0) is_m = 1;
             }
-        } else if (mtype == SV_MAT_ADE || mtype == SV_MAT_ADE3) {
-            if (scan_floats(fr, p, mtype == SV_MAT_ADE3 ? 16 : 12))
+        } else if (mtype == SV_MAT_ADE || mtype == SV_MAT_ADE3 || mtype == SV_MAT_DRUDE || mtype == SV_MAT_CP
+                   || mtype == SV_MAT_PLRC) {
+            const int nv = mtype == SV_MAT_ADE3 ? 16 : mtype == SV_MAT_DRUDE ? 3 : mtype == SV_MAT_CP ? 11 : 12;
+            if (scan_floats(fr, p, nv))
                 goto err;
             is_e = 1;
         } else if (mtype == SV_MAT_PEC) {
         } else {
-            fprintf(stderr, "Error: material type %d is not supported by the MI355X engine (supported: 0, 1, 5 and 7 "
-                            "ADE Drude + critical points, 10 PEC)\n", mtype);
+            fprintf(stderr, "Error: material type %d is not supported by the MI355X engine (supported: 0, 1, 2, 4 "
+                            "and 6 recursive convolution, 5 and 7 ADE Drude + critical points, 10 PEC)\n", mtype);
             goto err;
         }
     }
@@ -1469,11 +1537,16 @@ load_data(SvPool *mp, SvSet *set)
         }
         free(ents);
 
-        /* src3d/plan.c:4640-4672: relative -> absolute material constants */
+        /* src3d/plan.c:4640-4699: relative -> absolute material constants;
+         * the Drude type's omega_p and nu are multiplied by EV_TO_J there */
         for (n = 0; n < mp->nmat; n++) {
             if (sv_material_is_tabulated(mp->mats[n].type)) {
                 mp->mats[n].epsilon *= EPS0;
                 mp->mats[n].mu *= MU0;
+            }
+            if (mp->mats[n].type == SV_MAT_DRUDE) {
+                mp->mats[n].drude_omega_p *= SV_EV_TO_J;
+                mp->mats[n].drude_nu *= SV_EV_TO_J;
             }
         }
         goto after_vector;

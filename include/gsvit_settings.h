@@ -110,6 +110,12 @@ typedef struct {
     double ade_c1;
     double ade_c2;
     double ade_c3;
+    /* piecewise-linear recursive convolution (type 6) susceptibility
+     * integrals: Drude term and two critical-point poles (real and imaginary
+     * parts of the increments), src3d/plan.c:2765-2824 */
+    double plrc_d_chi, plrc_d_xi, plrc_d_dchi, plrc_d_dxi;
+    double plrc_p_chi[2], plrc_p_xi[2];
+    double plrc_p_dchir[2], plrc_p_dxir[2], plrc_p_dchii[2], plrc_p_dxii[2];
     int pos;
 } SvMatProp;
 

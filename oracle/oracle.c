@@ -12,6 +12,16 @@
  *   Yee E update ............ src3d/yee.c:528-1247 (linear/tabulated with
  *                             component averaging :1396-1492, ADE branch
  *                             :914-1108 incl. its inline CPML, PEC :1139-1237)
+ *   recursive convolution ... src3d/yee.c:266-455, 620-905: Drude (type 2,
+ *                             Fu et al., omega_p and nu scaled by EV_TO_J at
+ *                             plan time), Drude + 2 critical points (type 4,
+ *                             Vial), piecewise-linear (type 6, Chun et al.,
+ *                             with its own inline CPML, src3d/yee.c:724-856);
+ *                             pinned by the reference's self-test goldens
+ *                             10026000 / 10027000 / 10029000 and, through
+ *                             oracle_rc_scalar, by the discrete constitutive
+ *                             relation with susceptibility integrals taken by
+ *                             quadrature of chi(t) (tests/test_rc_materials.py)
  *   third ADE pole .......... (material type 7, no reference counterpart) the
  *                             pole formulas of src3d/plan.c:2736-2762 and the
  *                             ADE branch applied to a third pole; pinned by the
@@ -99,6 +109,9 @@ typedef struct {
     real *exp_, *eyp, *ezp, *dpx, *dpy, *dpz, *dpxp, *dpyp, *dpzp;
     real *px[3], *py[3], *pz[3], *pxp[3], *pyp[3], *pzp[3];
     int npoles;                       /* critical-point poles carried: 2, or 3 with a type-7 material */
+    int rc;                           /* recursive-convolution materials (types 2, 4, 6) present */
+    real *rcd[3];                     /* their Drude accumulator per component */
+    real *rcr[2][3], *rci[2][3];      /* their pole accumulators (real, imaginary) per pole and component */
     int perx, pery;                   /* domain-periodic x / y (no reference counterpart) */
     /* scattered-field source line (SOURCE_SF) */
     int sres;
@@ -213,6 +226,22 @@ oracle_open(const char *parfile)
         }
     }
 
+    for (q = 0; q < o->mp->nmat; q++) {
+        const int t = o->mp->mats[q].type;
+        if (t == SV_MAT_DRUDE || t == SV_MAT_CP || t == SV_MAT_PLRC)
+            o->rc = 1;
+    }
+    if (o->rc) {
+        int c;
+        for (c = 0; c < 3; c++) {
+            o->rcd[c] = (real *)zalloc(o->n, sizeof(real));
+            for (q = 0; q < 2; q++) {
+                o->rcr[q][c] = (real *)zalloc(o->n, sizeof(real));
+                o->rci[q][c] = (real *)zalloc(o->n, sizeof(real));
+            }
+        }
+    }
+
     if (o->mp->src->tsf) {
         o->jres = o->mp->src->tsf->jp_res;
         o->je = (real *)zalloc(o->jres, sizeof(real));
@@ -250,6 +279,14 @@ oracle_close(void *handle)
         free(o->dpxp); free(o->dpyp); free(o->dpzp);
         for (q = 0; q < o->npoles; q++) {
             free(o->px[q]); free(o->py[q]); free(o->pz[q]); free(o->pxp[q]); free(o->pyp[q]); free(o->pzp[q]);
+        }
+    }
+    for (q = 0; q < 3; q++) {
+        int n;
+        free(o->rcd[q]);
+        for (n = 0; n < 2; n++) {
+            free(o->rcr[n][q]);
+            free(o->rci[n][q]);
         }
     }
     free(o->je); free(o->jh); free(o->points); free(o->objacc); free(o->pfstore); free(o->lz);
@@ -573,6 +610,265 @@ ade_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int z
     }
 }
 
+/* recursive-convolution materials, src3d/yee.c:266-455, 620-905.  One
+ * component's accumulators: the Drude term d and per critical point the
+ * complex accumulator (pr + i pi). */
+typedef struct {
+    double d, pr[2], pi[2];
+} RcAcc;
+
+/* CP (type 4) critical point q, src3d/yee.c:307-455: the first-interval
+ * susceptibility chi0 and its increment dchi of the complex kernel
+ * -2i A W e^(-i phi) e^((-G + iW) t) (the reference scales numerator and
+ * denominator of the prefactor by 1e-16, which cancels) */
+static void
+rc_cp_pole(const SvMatProp *m, int q, double dt, double *c0r, double *dcr, double *dci)
+{
+    const double a = m->cp3_a[q], ph = m->cp3_phi[q], om = m->cp3_omega[q], ga = m->cp3_gamma[q];
+    const double nr = -2 * a * om * sin(ph), ni = -2 * a * om * cos(ph);       /* numerator */
+    const double den = ga * ga + om * om;                                      /* |G - iW|^2 */
+    const double fr = (nr * ga - ni * om) / den, fi = (ni * ga + nr * om) / den; /* / (G - iW) */
+    const double er = 1 - exp(-ga * dt) * cos(om * dt), ei = -exp(-ga * dt) * sin(om * dt);
+    const double cr = fr * er - fi * ei, ci = fr * ei + fi * er;                /* chi0 */
+    *c0r = cr;
+    *dcr = cr * er - ci * ei;                                                  /* chi0 (1 - e^(beta dt)) */
+    *dci = cr * ei + ci * er;
+}
+
+/* CP: accumulators advance with E^n before the E update */
+static void
+rc_pre(const SvMatProp *m, double dt, double e, RcAcc *a)
+{
+    int q;
+    if (m->type != SV_MAT_CP)
+        return;
+    {
+        const double g = m->drude_nu, w = m->drude_omega_p, eg = exp(-g * dt);
+        a->d = eg * a->d - (w * w / g / g) * (1 - eg) * (1 - eg) * e;
+    }
+    for (q = 0; q < 2; q++) {
+        double c0, dr, di;
+        const double rr = exp(-m->cp3_gamma[q] * dt) * cos(m->cp3_omega[q] * dt);
+        const double ri = exp(-m->cp3_gamma[q] * dt) * sin(m->cp3_omega[q] * dt);
+        const double pr = a->pr[q], pi = a->pi[q];
+        rc_cp_pole(m, q, dt, &c0, &dr, &di);
+        a->pr[q] = rr * pr - ri * pi + dr * e;
+        a->pi[q] = rr * pi + ri * pr + di * e;
+    }
+}
+
+/* E^(n+1) of one component from E^n, the accumulators and the curl
+ * bracket (the reference divides by dx for every component of types 2 and
+ * 4, by the component's own spacing for type 6, whose bracket carries the
+ * 1 / kappa factors) */
+static double
+rc_e(const SvMatProp *m, double dt, double e, const RcAcc *a, double curl, double d)
+{
+    const double eps = m->epsilon / EPS0;
+    if (m->type == SV_MAT_DRUDE) {
+        /* src3d/yee.c:266-285, 620-650: the coefficients take omega_p / c and
+         * c dt where the Drude recursion has omega_p / nu and nu dt */
+        const double w = m->drude_omega_p / CL, e1 = exp(-CL * dt);
+        const double ch0 = w * w * CL * dt - w * w * (1 - e1), dch0 = -w * w * (1 - e1) * (1 - e1);
+        return (eps + dch0) / (eps + ch0) * e + 1.0 / (eps + ch0) * a->d + dt / (eps + ch0) / EPS0 / d * curl;
+    } else if (m->type == SV_MAT_CP) {
+        const double g = m->drude_nu, w = m->drude_omega_p;
+        double x0 = -(w * w / g / g) * (1 - exp(-g * dt)) + dt * w * w / g;
+        int q;
+        for (q = 0; q < 2; q++) {
+            double c0, dr, di;
+            rc_cp_pole(m, q, dt, &c0, &dr, &di);
+            x0 += c0;
+        }
+        return eps / (eps + x0) * e + 1.0 / (eps + x0) * (a->d + a->pr[0] + a->pr[1])
+               + dt / (d * EPS0 * (eps + x0)) * curl;
+    } else {
+        const double xi0 = m->plrc_d_xi + m->plrc_p_xi[0] + m->plrc_p_xi[1];
+        const double chi0 = m->plrc_d_chi + m->plrc_p_chi[0] + m->plrc_p_chi[1];
+        const double den = 2 * EPS0 * (eps - xi0 + chi0) + m->sigma * dt;
+        return (2 * EPS0 * (eps - xi0) - m->sigma * dt) / den * e + 2 * EPS0 / den * (a->d + a->pr[0] + a->pr[1])
+               + 2 * dt / den / d * curl;
+    }
+}
+
+/* Drude (type 2) and PLRC (type 6): accumulators advance after the E update
+ * (PLRC with E^(n+1) after its inline CPML correction) */
+static void
+rc_post(const SvMatProp *m, double dt, double e0, double e1, RcAcc *a)
+{
+    int q;
+    if (m->type == SV_MAT_DRUDE) {
+        const double w = m->drude_omega_p / CL, x = exp(-CL * dt);
+        a->d = exp(-m->drude_nu * dt) * a->d - w * w * x * (1 - x) * (1 - x) * e0;
+    } else if (m->type == SV_MAT_PLRC) {
+        a->d = (m->plrc_d_dchi - m->plrc_d_dxi) * e1 + m->plrc_d_dxi * e0 + exp(-m->drude_nu * dt) * a->d;
+        for (q = 0; q < 2; q++) {
+            /* rotation e^((-G - iW) dt) */
+            const double rr = exp(-dt * m->cp3_gamma[q]) * cos(dt * m->cp3_omega[q]);
+            const double ri = -exp(-dt * m->cp3_gamma[q]) * sin(dt * m->cp3_omega[q]);
+            const double pr = a->pr[q], pi = a->pi[q];
+            a->pr[q] = (m->plrc_p_dchir[q] - m->plrc_p_dxir[q]) * e1 + m->plrc_p_dxir[q] * e0 + (rr * pr - ri * pi);
+            a->pi[q] = (m->plrc_p_dchii[q] - m->plrc_p_dxii[q]) * e1 + m->plrc_p_dxii[q] * e0 + (rr * pi + ri * pr);
+        }
+    }
+}
+
+static void
+rc_load(const Oracle *o, size_t c, int comp, RcAcc *a)
+{
+    int q;
+    a->d = o->rcd[comp][c];
+    for (q = 0; q < 2; q++) {
+        a->pr[q] = o->rcr[q][comp][c];
+        a->pi[q] = o->rci[q][comp][c];
+    }
+}
+
+static void
+rc_store(Oracle *o, size_t c, int comp, const RcAcc *a)
+{
+    int q;
+    o->rcd[comp][c] = (real)a->d;
+    for (q = 0; q < 2; q++) {
+        o->rcr[q][comp][c] = (real)a->pr[q];
+        o->rci[q][comp][c] = (real)a->pi[q];
+    }
+}
+
+static void
+rc_cell(Oracle *o, int i, int j, int k, int matindex, int xtbc, int ytbc, int ztbc, double kx, double ky, double kz)
+{
+    const SvMatProp *m = o->mp->mats + matindex;
+    const SvSetBoundary *sb = &o->set.sb;
+    const SvBoundary *bnd = o->mp->bnd;
+    const int nx = o->nx, ny = o->ny, nz = o->nz;
+    const size_t sx = (size_t)ny * nz, sy = nz;
+    const size_t c = IDX(o, i, j, k);
+    const double dt = o->mp->d->dt, dx = o->mp->d->dx, dy = o->mp->d->dy, dz = o->mp->d->dz;
+    const size_t cxm = (i == 0 && o->perx) ? c + (size_t)(nx - 1) * sx : c - sx;
+    const size_t cym = (j == 0 && o->pery) ? c + (size_t)(ny - 1) * sy : c - sy;
+    const int xi = i > 0 || o->perx, yi = j > 0 || o->pery;
+    const int plrc = m->type == SV_MAT_PLRC;
+    const double e0[3] = { o->ex[c], o->ey[c], o->ez[c] };
+    double cx, cy, cz;
+    RcAcc a[3];
+    int q;
+
+    for (q = 0; q < 3; q++) {
+        rc_load(o, c, q, a + q);
+        rc_pre(m, dt, e0[q], a + q);
+    }
+    if (plrc) {
+        cx = (o->hz[c] - o->hz[cym]) / ky - (o->hy[c] - o->hy[c - 1]) / kz;
+        cy = (o->hx[c] - o->hx[c - 1]) / kz - (o->hz[c] - o->hz[cxm]) / kx;
+        cz = (o->hy[c] - o->hy[cxm]) / kx - (o->hx[c] - o->hx[cym]) / ky;
+    } else {
+        cx = (o->hz[c] - o->hz[cym]) - (o->hy[c] - o->hy[c - 1]);
+        cy = (o->hx[c] - o->hx[c - 1]) - (o->hz[c] - o->hz[cxm]);
+        cz = (o->hy[c] - o->hy[cxm]) - (o->hx[c] - o->hx[cym]);
+    }
+    if (yi && k > 0 && xtbc == 0) o->ex[c] = rc_e(m, dt, e0[0], a + 0, cx, plrc ? dx : dx);
+    if (xi && k > 0 && ytbc == 0) o->ey[c] = rc_e(m, dt, e0[1], a + 1, cy, plrc ? dy : dx);
+    if (xi && yi && ztbc == 0) o->ez[c] = rc_e(m, dt, e0[2], a + 2, cz, plrc ? dz : dx);
+
+    /* the PLRC branch's own CPML update, src3d/yee.c:752-856: the psi of the
+     * slab cell advances here and again in the boundary step; corrections
+     * with the material's cb on all six faces */
+    if (plrc) {
+        const double cbx = (dt / m->epsilon / dx) / (1 + m->sigma * dt / 2 / m->epsilon);
+        const double cby = (dt / m->epsilon / dy) / (1 + m->sigma * dt / 2 / m->epsilon);
+        const double cbz = (dt / m->epsilon / dz) / (1 + m->sigma * dt / 2 / m->epsilon);
+        double be, ce;
+        if (sb->bx0 == SV_BOUNDARY_CPML && i > 1 && i < sb->depth_bx0) {
+            size_t p = pidx_x(o, i, j, k);
+            be = bnd->x0.be[i]; ce = bnd->x0.ce[i];
+            o->f[0].pe1[p] = be * o->f[0].pe1[p] + ce * (o->hz[c] - o->hz[c - sx]) / dy;
+            o->f[0].pe2[p] = be * o->f[0].pe2[p] + ce * (o->hy[c] - o->hy[c - sx]) / dz;
+            if (yi && k != 0 && ytbc == 0) o->ey[c] -= dy * cby * o->f[0].pe1[p];
+            if (yi && k != 0 && ztbc == 0) o->ez[c] += dz * cbz * o->f[0].pe2[p];
+        }
+        if (sb->by0 == SV_BOUNDARY_CPML && j > 1 && j < sb->depth_by0) {
+            size_t p = pidx_y(o, sb->depth_by0, i, j, k);
+            be = bnd->y0.be[j]; ce = bnd->y0.ce[j];
+            o->f[2].pe1[p] = be * o->f[2].pe1[p] + ce * (o->hz[c] - o->hz[c - sy]) / dx;
+            o->f[2].pe2[p] = be * o->f[2].pe2[p] + ce * (o->hx[c] - o->hx[c - sy]) / dz;
+            if (xi && k != 0 && xtbc == 0) o->ex[c] += dx * cbx * o->f[2].pe1[p];
+            if (xi && k != 0 && ztbc == 0) o->ez[c] -= dz * cbz * o->f[2].pe2[p];
+        }
+        if (sb->bz0 == SV_BOUNDARY_CPML && k > 1 && k < sb->depth_bz0) {
+            size_t p = pidx_z(o, sb->depth_bz0, i, j, k);
+            be = bnd->z0.be[k]; ce = bnd->z0.ce[k];
+            o->f[4].pe1[p] = be * o->f[4].pe1[p] + ce * (o->hx[c] - o->hx[c - 1]) / dy;
+            o->f[4].pe2[p] = be * o->f[4].pe2[p] + ce * (o->hy[c] - o->hy[c - 1]) / dx;
+            if (xi && yi && ytbc == 0) o->ey[c] += dy * cby * o->f[4].pe1[p];
+            if (xi && yi && xtbc == 0) o->ex[c] -= dx * cbx * o->f[4].pe2[p];
+        }
+        if (sb->bxn == SV_BOUNDARY_CPML && i >= nx - sb->depth_bxn + 1 && i < nx) {
+            int ir = i - (nx - sb->depth_bxn);
+            size_t p = pidx_x(o, ir, j, k);
+            be = bnd->xn.be[ir]; ce = bnd->xn.ce[ir];
+            o->f[1].pe1[p] = be * o->f[1].pe1[p] + ce * (o->hz[c] - o->hz[c - sx]) / dy;
+            o->f[1].pe2[p] = be * o->f[1].pe2[p] + ce * (o->hy[c] - o->hy[c - sx]) / dz;
+            if (yi && k != 0 && ytbc == 0) o->ey[c] -= dy * cby * o->f[1].pe1[p];
+            if (yi && k != 0 && ztbc == 0) o->ez[c] += dz * cbz * o->f[1].pe2[p];
+        }
+        if (sb->byn == SV_BOUNDARY_CPML && j >= ny - sb->depth_byn + 1 && j < ny) {
+            int jr = j - (ny - sb->depth_byn);
+            size_t p = pidx_y(o, sb->depth_byn, i, jr, k);
+            be = bnd->yn.be[jr]; ce = bnd->yn.ce[jr];
+            o->f[3].pe1[p] = be * o->f[3].pe1[p] + ce * (o->hz[c] - o->hz[c - sy]) / dx;
+            o->f[3].pe2[p] = be * o->f[3].pe2[p] + ce * (o->hx[c] - o->hx[c - sy]) / dz;
+            if (xi && k != 0 && xtbc == 0) o->ex[c] += dx * cbx * o->f[3].pe1[p];
+            if (xi && k != 0 && ztbc == 0) o->ez[c] -= dz * cbz * o->f[3].pe2[p];
+        }
+        if (sb->bzn == SV_BOUNDARY_CPML && k >= nz - sb->depth_bzn + 1 && k < nz) {
+            int kr = k - (nz - sb->depth_bzn);
+            size_t p = pidx_z(o, sb->depth_bzn, i, j, kr);
+            be = bnd->zn.be[kr]; ce = bnd->zn.ce[kr];
+            o->f[5].pe1[p] = be * o->f[5].pe1[p] + ce * (o->hx[c] - o->hx[c - 1]) / dy;
+            o->f[5].pe2[p] = be * o->f[5].pe2[p] + ce * (o->hy[c] - o->hy[c - 1]) / dx;
+            if (xi && yi && ytbc == 0) o->ey[c] += dy * cby * o->f[5].pe1[p];
+            if (xi && yi && xtbc == 0) o->ex[c] -= dx * cbx * o->f[5].pe2[p];
+        }
+    }
+
+    {
+        const double e1[3] = { o->ex[c], o->ey[c], o->ez[c] };
+        for (q = 0; q < 3; q++) {
+            rc_post(m, dt, e0[q], e1[q], a + q);
+            rc_store(o, c, q, a + q);
+        }
+    }
+}
+
+/* Test hook: the scalar recursion of one component of material `matindex`
+ * from zero state, driven by curl[t] (the bracket of rc_e) over spacing d:
+ * out[t] = E^(t+1).  The same functions as the 3-D update. */
+int
+oracle_rc_scalar(void *handle, int matindex, int n, const double *curl, double d, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    const SvMatProp *m;
+    const double dt = o->mp->d->dt;
+    RcAcc a;
+    double e = 0;
+    int t;
+    if (matindex < 0 || matindex >= o->mp->nmat)
+        return 1;
+    m = o->mp->mats + matindex;
+    if (m->type != SV_MAT_DRUDE && m->type != SV_MAT_CP && m->type != SV_MAT_PLRC)
+        return 1;
+    memset(&a, 0, sizeof(a));
+    for (t = 0; t < n; t++) {
+        const double e0 = e;
+        rc_pre(m, dt, e0, &a);
+        e = rc_e(m, dt, e0, &a, curl[t], d);
+        rc_post(m, dt, e0, e, &a);
+        out[t] = e;
+    }
+    return 0;
+}
+
 static int
 is_pec_t(const Oracle *o, int i, int j, int k)
 {
@@ -601,6 +897,10 @@ ystep_e(Oracle *o)
                 matindex = sv_pool_get_tbc(mp, i, j, k, &xtbc, &ytbc, &ztbc);
                 if (mp->nmat > 0 && o->ade && xtbc * ytbc * ztbc == 0 && mp->mats[matindex].type == SV_MAT_ADE)
                     ade_cell(o, i, j, k, matindex, xtbc, ytbc, ztbc, kx, ky, kz);
+                else if (mp->nmat > 0 && o->rc && xtbc * ytbc * ztbc == 0
+                         && (mp->mats[matindex].type == SV_MAT_DRUDE || mp->mats[matindex].type == SV_MAT_CP
+                             || mp->mats[matindex].type == SV_MAT_PLRC))
+                    rc_cell(o, i, j, k, matindex, xtbc, ytbc, ztbc, kx, ky, kz);
                 if ((xtbc + ytbc + ztbc) != 0) {
                     /* -x / -y neighbours; on a periodic axis the near plane's is the far plane */
                     const size_t cxm = (i == 0 && o->perx) ? c + (size_t)(nx - 1) * sx : c - sx;

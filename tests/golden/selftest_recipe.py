@@ -18,7 +18,10 @@ MATERIAL_LINES = {
     3: "8 %d %d %d %d %d %d 1 2 1 0 0\n",
     4: "8 %d %d %d %d %d %d 1 1 2 0 0\n",
     5: "8 %d %d %d %d %d %d 10\n",
+    6: "8 %d %d %d %d %d %d 2 5.0 9.5 0.0987\n",
+    7: "8 %d %d %d %d %d %d 4  1.1431 1.3202e16 1.0805e14  0.26698 -1.2371 3.8711e15 4.4642e14  3.0834 -1.0968 4.1684e15 2.3555e15\n",
     8: "8 %d %d %d %d %d %d 5  1.1431 0 1.3202e16 1.0805e14  0.26698 -1.2371 3.8711e15 4.4642e14  3.0834 -1.0968 4.1684e15 2.3555e15\n",
+    9: "8 %d %d %d %d %d %d 6  1.1431 0 1.3202e16 1.0805e14  0.26698 -1.2371 3.8711e15 4.4642e14  3.0834 -1.0968 4.1684e15 2.3555e15\n",
 }
 
 

@@ -55,6 +55,9 @@ def lib(precision="f64"):
         L.oracle_dt.restype = ctypes.c_double
         L.oracle_dt.argtypes = [ctypes.c_void_p]
         L.oracle_real_size.restype = ctypes.c_int
+        L.oracle_rc_scalar.restype = ctypes.c_int
+        L.oracle_rc_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.oracle_set_option.restype = ctypes.c_int
         L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
         _libs[precision] = L
@@ -219,6 +222,16 @@ class Oracle:
             d["type"] = int(d["type"])
             d["poles"] = int(d["poles"])
             out.append(d)
+        return out
+
+    def rc_scalar(self, matindex, curl, d):
+        """one component of recursive-convolution material `matindex` stepped
+        from zero state by the oracle's own update, driven by the curl
+        bracket curl[t] over spacing d; returns E^1 .. E^n"""
+        c = np.ascontiguousarray(curl, dtype=np.float64)
+        out = np.zeros(c.size)
+        if self.L.oracle_rc_scalar(self.h, matindex, c.size, _dptr(c), d, _dptr(out)):
+            raise ValueError("material %d is not a recursive-convolution material" % matindex)
         return out
 
     def maps(self):

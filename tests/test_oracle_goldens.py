@@ -48,10 +48,14 @@ def run_selftest(tmp_path, test_id, no_average=False, precision="f64"):
     return rel_l2(got, gold)
 
 
-@pytest.mark.parametrize("test_id", ["10000000", "10021100", "10022100", "10024100", "10028000", "10035011"])
+@pytest.mark.parametrize("test_id", ["10000000", "10021100", "10022100", "10024100", "10026000", "10027000",
+                                     "10028000", "10029000", "10035011"])
 def test_selftest_matches_current_source(tmp_path, test_id):
     """no boundary, Liao + linear electric/magnetic, Liao + tabulated magnetic,
-    Liao + ADE (Drude + 2 critical points), CPML + PEC + TSF + far field"""
+    Liao + recursive-convolution Drude (type 2), Liao + recursive-convolution
+    Drude + 2 critical points (type 4), Liao + ADE (Drude + 2 critical points),
+    Liao + piecewise-linear recursive convolution (type 6), CPML + PEC + TSF +
+    far field"""
     e = run_selftest(tmp_path, test_id)
     print(test_id, "oracle vs golden rel L2", e)
     assert e <= TOL
