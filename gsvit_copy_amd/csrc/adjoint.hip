@@ -714,6 +714,11 @@ gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_int
         fprintf(stderr, "Error: the gradient does not support ADE_CPML_CONSISTENT\n");
         return 1;
     }
+    if (g.rc) {
+        fprintf(stderr, "Error: the gradient supports ADE dispersive materials (types 5, 7), not the recursive-convolution "
+                        "types 2, 4, 6\n");
+        return 1;
+    }
     if (g.ade) {
         /* dispersive cells stay out of the CPML slabs (E side) and off the design cells;
          * the class ids come back from the device (the host copy is freed after set-up) */

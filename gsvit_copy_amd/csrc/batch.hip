@@ -132,6 +132,7 @@ batch_reject(const Engine *E)
     if (E->npts) return "point outputs";
     if (E->adj) return "adjoint state";
     if (E->g.pc_nx > 0) return "a design region";
+    if (E->g.rc) return "recursive-convolution materials";
     if ((unsigned long long)E->g.plane * (E->g.nz + 2) * 4ull >= (1ull << 32)) return "field arrays of 4 GiB or more";
     return NULL;
 }

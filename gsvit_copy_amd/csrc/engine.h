@@ -77,6 +77,26 @@ typedef struct {
     int drude;       /* the Drude current is driven (da != 0); without it J stays 0 and is neither read nor written */
 } GsvAde;
 
+/* recursive-convolution material (types 2 Drude, 4 Drude + 2 critical points,
+ * 6 piecewise-linear), src3d/yee.c:266-455, 620-905, with the coefficients the
+ * reference forms per cell and step combined once on the host in double.
+ * Per component the state is a Drude accumulator d and, per critical point,
+ * a complex accumulator p; the E update is
+ *     E' = ca E + cs (d + Re p_0 + Re p_1) + cb[c] curl
+ * with curl the bracket of H differences (over kappa for type 6 only).  Type 4
+ * advances d and p with E^n before the E update, types 2 and 6 after it:
+ *     d' = dd d + d0 E^n + d1 E',   p' = rot p + q0 E^n + q1 E'  (complex). */
+typedef struct {
+    int kind;            /* SV_MAT_DRUDE, SV_MAT_CP or SV_MAT_PLRC */
+    int poles;           /* 2 for types 4 and 6 (critical-point accumulators carried), 0 for type 2 */
+    float ca, cs;
+    float cb[3];         /* per component, the spacing's division included */
+    float dd, d0, d1;
+    float rr[2], ri[2];  /* per-step rotation of each pole's accumulator */
+    float q0r[2], q0i[2], q1r[2], q1i[2];
+    float cbm[3];        /* type 6: the material cb of its inline CPML correction, src3d/yee.c:752-856 */
+} GsvRc;
+
 /* one CPML face on the device: grading coefficients per index along the
  * stretched axis, packed so that one load brings all of them, and the two
  * psi values of a slab cell stored as a pair */
@@ -123,6 +143,9 @@ typedef struct {
     float *ade_p;           /* critical-point state blocks, GSV_ADE_P(ade_poles) states (NULL without poles) */
     long long ade_plane_d;  /* floats per z plane of ade_d = ny * pxa * GSV_ADE_D */
     long long ade_plane_p;  /* floats per z plane of ade_p */
+    int rc;                 /* the dispersive cells are recursive-convolution materials (table rcm, state in
+                               the ADE blocks: d in Drude states 0-2, pole q's p as P / J slots 3q + c) */
+    const GsvRc *rcm;
     int ade_cpml;           /* ADE_CPML_CONSISTENT: the ADE branch updates and applies the psi of its own
                                components once per step (curl coefficient c0, all six faces) and the
                                slab code leaves them alone; 0: the reference's double update */

@@ -51,6 +51,7 @@ struct Engine {
     std::vector<float> src_vals_host;            /* point-source table [nsteps][n][6] */
     int ade_slot_of_mat[256];
     GsvAde adem_host[GSV_MAX_ADE];
+    GsvRc rcm_host[GSV_MAX_ADE];
     std::vector<void *> allocs;
     float *fbase[6];
     size_t field_floats;

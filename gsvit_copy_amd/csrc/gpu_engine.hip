@@ -129,7 +129,7 @@ e_key_raw(const Engine *E, int i, int j, int k, EKey *key)
     if (xt) fl |= GSV_E_TBCX;
     if (yt) fl |= GSV_E_TBCY;
     if (zt) fl |= GSV_E_TBCZ;
-    if (mp->nmat > 0 && xt * yt * zt == 0 && mp->mats[mi].type == SV_MAT_ADE)
+    if (mp->nmat > 0 && xt * yt * zt == 0 && sv_material_is_dispersive(mp->mats[mi].type))
         fl |= GSV_E_ADE | ((uint32_t)E->ade_slot_of_mat[mi] << GSV_E_SLOT_SHIFT);
     if (mi >= 0 && i > 0 && j > 0 && k > 0) {
         /* src3d/yee.c:1139-1237 */
@@ -259,6 +259,9 @@ build_classes(Engine *E, KEYFN keyfn, std::vector<KEY> &table, std::vector<uint8
     return 0;
 }
 
+static int build_rc_tables(Engine *E);
+static int build_class_tables(Engine *E, int nslot);
+
 static int
 build_tables(Engine *E)
 {
@@ -272,12 +275,28 @@ build_tables(Engine *E)
     }
     E->g.ade = 0;
     E->g.ade_poles = 0;
-    for (n = 0; n < mp->nmat; n++) {
-        const SvMatProp *m = mp->mats + n;
-        if (m->type == SV_MAT_DRUDE || m->type == SV_MAT_CP || m->type == SV_MAT_PLRC) {
-            fprintf(stderr, "Error: recursive-convolution materials (types 2, 4, 6) are not yet supported on the GPU\n");
+    E->g.rc = 0;
+    {
+        int nade = 0, nrc = 0;
+        for (n = 0; n < mp->nmat; n++) {
+            const int t = mp->mats[n].type;
+            nade += t == SV_MAT_ADE;
+            nrc += t == SV_MAT_DRUDE || t == SV_MAT_CP || t == SV_MAT_PLRC;
+            if (t == SV_MAT_CP3) {
+                fprintf(stderr, "Error: material type 3 (CP3) is not supported by the MI355X engine\n");
+                return 1;
+            }
+        }
+        if (nade && nrc) {
+            fprintf(stderr, "Error: ADE materials (types 5, 7) and recursive-convolution materials (types 2, 4, 6) "
+                            "cannot share one pool on the MI355X engine\n");
             return 1;
         }
+        if (nrc)
+            return build_rc_tables(E) || build_class_tables(E, nrc);
+    }
+    for (n = 0; n < mp->nmat; n++) {
+        const SvMatProp *m = mp->mats + n;
         if (m->type != SV_MAT_ADE)
             continue;
         if (nslot >= GSV_MAX_ADE) {
@@ -317,7 +336,107 @@ build_tables(Engine *E)
         E->ade_slot_of_mat[n] = nslot++;
         E->g.ade = 1;
     }
+    return build_class_tables(E, nslot);
+}
 
+/* the recursive-convolution material table: the coefficients that the
+ * reference's branches of src3d/yee.c:266-455, 620-905 form per cell and step,
+ * combined in double (type 6 from the planner's susceptibility integrals,
+ * src3d/plan.c:2765-2824) */
+static int
+build_rc_tables(Engine *E)
+{
+    SvPool *mp = E->mp;
+    const double dt = mp->d->dt, sp[3] = { mp->d->dx, mp->d->dy, mp->d->dz };
+    int n, nslot = 0;
+    if (mp->set->sb.ade_cpml_consistent) {
+        fprintf(stderr, "Error: ADE_CPML_CONSISTENT applies to ADE materials only\n");
+        return 1;
+    }
+    for (n = 0; n < mp->nmat; n++) {
+        const SvMatProp *m = mp->mats + n;
+        if (m->type != SV_MAT_DRUDE && m->type != SV_MAT_CP && m->type != SV_MAT_PLRC)
+            continue;
+        if (nslot >= GSV_MAX_ADE) {
+            fprintf(stderr, "Error: at most %d dispersive materials are supported\n", GSV_MAX_ADE);
+            return 1;
+        }
+        GsvRc &r = E->rcm_host[nslot];
+        memset(&r, 0, sizeof(r));
+        const double eps = m->epsilon / SV_EPSILON_0;
+        r.kind = m->type;
+        if (m->type == SV_MAT_DRUDE) {
+            /* omega_p / c and c dt in the chi terms, the material's nu in the decay */
+            const double w = m->drude_omega_p / SV_LIGHT_SPEED, x = exp(-SV_LIGHT_SPEED * dt);
+            const double ch0 = w * w * SV_LIGHT_SPEED * dt - w * w * (1 - x), dch0 = -w * w * (1 - x) * (1 - x);
+            r.ca = (float)((eps + dch0) / (eps + ch0));
+            r.cs = (float)(1.0 / (eps + ch0));
+            for (int c = 0; c < 3; c++)
+                r.cb[c] = (float)(dt / (eps + ch0) / SV_EPSILON_0 / sp[0]);
+            r.dd = (float)exp(-m->drude_nu * dt);
+            r.d0 = (float)(-w * w * x * (1 - x) * (1 - x));
+        } else if (m->type == SV_MAT_CP) {
+            const double g = m->drude_nu, w = m->drude_omega_p, eg = exp(-g * dt);
+            double x0 = -(w * w / g / g) * (1 - eg) + dt * w * w / g;
+            r.poles = 2;
+            for (int q = 0; q < 2; q++) {
+                /* kernel -2i A W e^(-i phi) e^((-G + iW) t): chi0 = f (1 - e^(beta dt)), f its
+                 * prefactor over (G - iW); the accumulator's drive is chi0 (1 - e^(beta dt)) */
+                const double a = m->cp3_a[q], ph = m->cp3_phi[q], om = m->cp3_omega[q], ga = m->cp3_gamma[q];
+                const double nr = -2 * a * om * sin(ph), ni = -2 * a * om * cos(ph), den = ga * ga + om * om;
+                const double fr = (nr * ga - ni * om) / den, fi = (ni * ga + nr * om) / den;
+                const double er = 1 - exp(-ga * dt) * cos(om * dt), ei = -exp(-ga * dt) * sin(om * dt);
+                const double cr = fr * er - fi * ei, ci = fr * ei + fi * er;
+                x0 += cr;
+                r.q0r[q] = (float)(cr * er - ci * ei);
+                r.q0i[q] = (float)(cr * ei + ci * er);
+                r.rr[q] = (float)(exp(-ga * dt) * cos(om * dt));
+                r.ri[q] = (float)(exp(-ga * dt) * sin(om * dt));
+            }
+            r.ca = (float)(eps / (eps + x0));
+            r.cs = (float)(1.0 / (eps + x0));
+            for (int c = 0; c < 3; c++)
+                r.cb[c] = (float)(dt / (sp[0] * SV_EPSILON_0 * (eps + x0)));
+            r.dd = (float)eg;
+            r.d0 = (float)(-(w * w / g / g) * (1 - eg) * (1 - eg));
+        } else {
+            const double xi0 = m->plrc_d_xi + m->plrc_p_xi[0] + m->plrc_p_xi[1];
+            const double chi0 = m->plrc_d_chi + m->plrc_p_chi[0] + m->plrc_p_chi[1];
+            const double den = 2 * SV_EPSILON_0 * (eps - xi0 + chi0) + m->sigma * dt;
+            r.poles = 2;
+            r.ca = (float)((2 * SV_EPSILON_0 * (eps - xi0) - m->sigma * dt) / den);
+            r.cs = (float)(2 * SV_EPSILON_0 / den);
+            for (int c = 0; c < 3; c++) {
+                r.cb[c] = (float)(2 * dt / den / sp[c]);
+                r.cbm[c] = (float)((dt / m->epsilon / sp[c]) / (1 + m->sigma * dt / 2 / m->epsilon));
+            }
+            r.dd = (float)exp(-m->drude_nu * dt);
+            r.d0 = (float)m->plrc_d_dxi;
+            r.d1 = (float)(m->plrc_d_dchi - m->plrc_d_dxi);
+            for (int q = 0; q < 2; q++) {
+                /* rotation e^((-G - iW) dt) */
+                r.rr[q] = (float)(exp(-dt * m->cp3_gamma[q]) * cos(dt * m->cp3_omega[q]));
+                r.ri[q] = (float)(-exp(-dt * m->cp3_gamma[q]) * sin(dt * m->cp3_omega[q]));
+                r.q0r[q] = (float)m->plrc_p_dxir[q];
+                r.q0i[q] = (float)m->plrc_p_dxii[q];
+                r.q1r[q] = (float)(m->plrc_p_dchir[q] - m->plrc_p_dxir[q]);
+                r.q1i[q] = (float)(m->plrc_p_dchii[q] - m->plrc_p_dxii[q]);
+            }
+        }
+        if (r.poles > E->g.ade_poles)
+            E->g.ade_poles = r.poles;
+        E->ade_slot_of_mat[n] = nslot++;
+        E->g.ade = 1;
+        E->g.rc = 1;
+    }
+    return 0;
+}
+
+/* E and H class tables (after the dispersive material slots are known) */
+static int
+build_class_tables(Engine *E, int nslot)
+{
+    SvPool *mp = E->mp;
     std::vector<EKey> ekeys;
     std::vector<HKey> hkeys;
     auto ek = [E](int i, int j, int k, EKey *key) { e_key(E, i, j, k, key); };
@@ -348,7 +467,7 @@ build_tables(Engine *E)
         c.pad[0] = c.pad[1] = c.pad[2] = 0;
     }
     if (mp->set->sc.verbose > 0)
-        printf("GPU plan: %zu E classes (%d-byte ids), %zu H classes (%d-byte ids), %d ADE materials\n",
+        printf("GPU plan: %zu E classes (%d-byte ids), %zu H classes (%d-byte ids), %d dispersive materials\n",
                E->etab.size(), E->idbytes_e, E->htab.size(), E->idbytes_h, nslot);
     return 0;
 }
@@ -1326,6 +1445,12 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
             if (dupload(E, &pt, E->pctab_host.data(), E->pctab_host.size())) return 1;
             g.pctab = pt;
         }
+    }
+    if (g.rc) {
+        GsvRc *rm;
+        if (dupload(E, &rm, E->rcm_host, GSV_MAX_ADE))
+            return 1;
+        g.rcm = rm;
     }
     if (g.ade) {
         GsvAde *am;

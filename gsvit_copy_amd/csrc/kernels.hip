@@ -410,6 +410,152 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     stsn(ad, la, 512, vezp);
 }
 
+/* Recursive-convolution branch of one cell (src3d/yee.c:620-905): types 2
+ * (Drude), 4 (Drude + 2 critical points) and 6 (piecewise-linear, with its
+ * own inline CPML); coefficient forms in GsvRc.  Like the ADE branch it is
+ * taken in dispersive cells and their +1 neighbours, updates the components
+ * whose flag is clear and advances the accumulators of all three components.
+ * State: the Drude accumulator of component c at slot c of the Drude block,
+ * pole q's complex accumulator at slots 3q + c (real) and 6 + 3q + c
+ * (imaginary) of the pole block. */
+template <bool PER, typename OFF>
+__device__ __forceinline__ void
+rc_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, int k, unsigned fl, float &ex,
+        float &ey, float &ez, float hx, float hy, float hz, float hxkm1, float hykm1, float hzi, float hyi, float hzj,
+        float hxj, float rkx, float rky, float rkz)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
+    const GSV_GLOBAL GsvRc &m = gp(g.rcm)[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
+    const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
+    const bool xi = i > 0 || (PER && g.perx), yi = j > 0 || (PER && g.pery);
+    const bool plrc = m.kind == SV_MAT_PLRC, pre = m.kind == SV_MAT_CP, pl = m.poles != 0;
+    const float e0[3] = { ex, ey, ez };
+    float d[3], pr[2][3] = {}, pi[2][3] = {};
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        d[c] = ldsn(ad, la, c * 256);
+    if (pl) {
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                pr[q][c] = ldsn(ap, lp, (3 * q + c) * 256);
+                pi[q][c] = ldsn(ap, lp, (6 + 3 * q + c) * 256);
+            }
+    }
+    /* d' = dd d + d0 a + d1 b, p' = rot p + q0 a + q1 b */
+    auto advance = [&](const float *a, const float *b) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            d[c] = m.dd * d[c] + m.d0 * a[c] + m.d1 * b[c];
+            if (pl) {
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const float r = pr[q][c], im = pi[q][c];
+                    pr[q][c] = m.rr[q] * r - m.ri[q] * im + m.q0r[q] * a[c] + m.q1r[q] * b[c];
+                    pi[q][c] = m.rr[q] * im + m.ri[q] * r + m.q0i[q] * a[c] + m.q1i[q] * b[c];
+                }
+            }
+        }
+    };
+    if (pre)
+        advance(e0, e0);   /* q1 = d1 = 0 for this type */
+    /* only type 6 divides the H differences by kappa */
+    const float kx = plrc ? rkx : 1.f, ky = plrc ? rky : 1.f, kz = plrc ? rkz : 1.f;
+    if (yi && k > 0 && !tx)
+        ex = m.ca * ex + m.cs * (d[0] + pr[0][0] + pr[1][0]) + m.cb[0] * ((hz - hzj) * ky - (hy - hykm1) * kz);
+    if (xi && k > 0 && !ty)
+        ey = m.ca * ey + m.cs * (d[1] + pr[0][1] + pr[1][1]) + m.cb[1] * ((hx - hxkm1) * kz - (hz - hzi) * kx);
+    if (xi && yi && !tz)
+        ez = m.ca * ez + m.cs * (d[2] + pr[0][2] + pr[1][2]) + m.cb[2] * ((hy - hyi) * kx - (hx - hxj) * ky);
+
+    if (plrc) {
+        /* the type-6 branch's own CPML update (src3d/yee.c:752-856): psi of the
+         * slab cell advanced here and again by the slab code, the material's cb
+         * on all six faces (x0 included) */
+        const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
+        if (F0.on && i > 1 && i < F0.depth) {
+            const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
+            const float4 bc = ldg4(F0.ce + i);
+            const float2 ps = ld2(F0.pe, p);
+            const float a1 = bc.x * ps.x + bc.z * (hz - hzi) / g.dy;
+            const float a2 = bc.x * ps.y + bc.z * (hy - hyi) / g.dz;
+            st2(F0.pe, p, a1, a2);
+            if ((yi && k != 0) && !ty) ey -= g.dy * m.cbm[1] * a1;
+            if ((yi && k != 0) && !tz) ez += g.dz * m.cbm[2] * a2;
+        }
+        if (F2.on && j > 1 && j < F2.depth) {
+            const OFF p = (OFF)(((unsigned)k * F2.depth + j) * px + i) * 8;
+            const float4 bc = ldg4(F2.ce + j);
+            const float2 ps = ld2(F2.pe, p);
+            const float a1 = bc.x * ps.x + bc.z * (hz - hzj) / g.dx;
+            const float a2 = bc.x * ps.y + bc.z * (hx - hxj) / g.dz;
+            st2(F2.pe, p, a1, a2);
+            if ((xi && k != 0) && !tx) ex += g.dx * m.cbm[0] * a1;
+            if ((xi && k != 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
+        }
+        if (F4.on && k > 1 && k < F4.depth) {
+            const OFF p = (OFF)(((unsigned)k * ny + j) * px + i) * 8;
+            const float4 bc = ldg4(F4.ce + k);
+            const float2 ps = ld2(F4.pe, p);
+            const float a1 = bc.x * ps.x + bc.z * (hx - hxkm1) / g.dy;
+            const float a2 = bc.x * ps.y + bc.z * (hy - hykm1) / g.dx;
+            st2(F4.pe, p, a1, a2);
+            if ((xi && yi) && !ty) ey += g.dy * m.cbm[1] * a1;
+            if ((xi && yi) && !tx) ex -= g.dx * m.cbm[0] * a2;
+        }
+        if (F1.on && i >= nx - F1.depth + 1) {
+            const int ir = i - (nx - F1.depth);
+            const OFF p = (OFF)(((unsigned)k * ny + j) * F1.depth + ir) * 8;
+            const float4 bc = ldg4(F1.ce + ir);
+            const float2 ps = ld2(F1.pe, p);
+            const float a1 = bc.x * ps.x + bc.z * (hz - hzi) / g.dy;
+            const float a2 = bc.x * ps.y + bc.z * (hy - hyi) / g.dz;
+            st2(F1.pe, p, a1, a2);
+            if ((yi && k != 0) && !ty) ey -= g.dy * m.cbm[1] * a1;
+            if ((yi && k != 0) && !tz) ez += g.dz * m.cbm[2] * a2;
+        }
+        if (F3.on && j >= ny - F3.depth + 1) {
+            const int jr = j - (ny - F3.depth);
+            const OFF p = (OFF)(((unsigned)k * F3.depth + jr) * px + i) * 8;
+            const float4 bc = ldg4(F3.ce + jr);
+            const float2 ps = ld2(F3.pe, p);
+            const float a1 = bc.x * ps.x + bc.z * (hz - hzj) / g.dx;
+            const float a2 = bc.x * ps.y + bc.z * (hx - hxj) / g.dz;
+            st2(F3.pe, p, a1, a2);
+            if ((xi && k != 0) && !tx) ex += g.dx * m.cbm[0] * a1;
+            if ((xi && k != 0) && !tz) ez -= g.dz * m.cbm[2] * a2;
+        }
+        if (F5.on && k >= nz - F5.depth + 1) {
+            const int kr = k - (nz - F5.depth);
+            const OFF p = (OFF)(((unsigned)kr * ny + j) * px + i) * 8;
+            const float4 bc = ldg4(F5.ce + kr);
+            const float2 ps = ld2(F5.pe, p);
+            const float a1 = bc.x * ps.x + bc.z * (hx - hxkm1) / g.dy;
+            const float a2 = bc.x * ps.y + bc.z * (hy - hykm1) / g.dx;
+            st2(F5.pe, p, a1, a2);
+            if ((xi && yi) && !ty) ey += g.dy * m.cbm[1] * a1;
+            if ((xi && yi) && !tx) ex -= g.dx * m.cbm[0] * a2;
+        }
+    }
+    if (!pre) {
+        const float e1[3] = { ex, ey, ez };
+        advance(e0, e1);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        stsn(ad, la, c * 256, d[c]);
+    if (pl) {
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                stsn(ap, lp, (3 * q + c) * 256, pr[q][c]);
+                stsn(ap, lp, (6 + 3 * q + c) * 256, pi[q][c]);
+            }
+    }
+}
+
 template <typename ID, int ADE, typename OFF, bool PC, bool PER>
 __device__ __forceinline__ void
 estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
@@ -427,7 +573,8 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
 
     /* lane offsets in the blocked ADE state planes, [j][i/64][state][64] */
     const OFF la = ADE ? ((OFF)j * g.pxa * GSV_ADE_D + (OFF)(i >> 6) * 64 * GSV_ADE_D + (i & 63)) * 4 : 0;
-    constexpr int NP = ADE >= 2 ? ADE : 0;   /* critical-point poles: ADE 2 -> 2, ADE 3 -> 3 */
+    /* critical-point poles: ADE 2 -> 2, ADE 3 -> 3; ADE 4 (recursive convolution) -> 2 */
+    constexpr int NP = ADE == 4 ? 2 : ADE >= 2 ? ADE : 0;
     const OFF lp = NP ? ((OFF)j * g.pxa * GSV_ADE_P(NP) + (OFF)(i >> 6) * 64 * GSV_ADE_P(NP) + (i & 63)) * 4 : 0;
 
     const float rkx = gp(g.rkex)[i], rky = gp(g.rkey)[j];
@@ -459,7 +606,11 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
         const unsigned fl = ec.flags;
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
 
-        if (ADE && (fl & GSV_E_ADE))
+        if (ADE == 4 && (fl & GSV_E_ADE))
+            rc_cell<PER>(g, g.ade_d + (size_t)k * g.ade_plane_d, la,
+                         g.ade_p ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr, lp, i, j, k, fl, ex, ey, ez,
+                         hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
+        else if (ADE && (fl & GSV_E_ADE))
             ade_cell<NP, PER>(g, g.ade_d + (size_t)k * g.ade_plane_d, la,
                          NP ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr, lp, i, j, k, fl, ex, ey, ez,
                          hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
@@ -1422,14 +1573,19 @@ launch_estep(const GsvGrid *g, int ade, dim3 grd, dim3 blk, int kchunk, hipStrea
         hipLaunchKernelGGL((k_estep<ID, 1, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
     else if (ade == 2)
         hipLaunchKernelGGL((k_estep<ID, 2, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
-    else
+    else if (ade == 3)
         hipLaunchKernelGGL((k_estep<ID, 3, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
+    else
+        hipLaunchKernelGGL((k_estep<ID, 4, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
 }
 
-/* dispersive build of a grid: 0 none, 1 Drude only, 2 / 3 Drude + 2 / 3 critical-point poles */
+/* dispersive build of a grid: 0 none, 1 Drude only, 2 / 3 Drude + 2 / 3
+ * critical-point poles, 4 recursive-convolution materials */
 static int
 ade_mode(const GsvGrid *g)
 {
+    if (g->rc)
+        return 4;
     return g->ade ? (g->ade_poles ? g->ade_poles : 1) : 0;
 }
 

@@ -484,6 +484,20 @@ def small_case(folder, kind, n=48, nsteps=160, seed=7):
         mats.append("4 %d %d %d 5 %s" % (c + 8, c + 6, c + 2, med))
         lines.append("BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\n")
         lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c + 6))
+    elif kind in ("rc_cpml", "rc_cpml_aniso"):
+        # the three recursive-convolution types together: a PLRC (type 6, with
+        # conductivity) half space through every side slab, a Drude + 2 CP (type
+        # 4) sphere and an eV-scaled Drude (type 2) box through the x0 slab;
+        # "rc_cpml_aniso" has 10 x 8 x 6 nm cells (the types take their
+        # spacings differently)
+        poles = "0.26698 -1.2371 3.8711e15 4.4642e14 3.0834 -1.0968 4.1684e15 2.3555e15"
+        mats.append("8 %d %d %d %d %d %d 6 1.1431 3e5 1.3202e16 1.0805e14 %s" % (0, 0, 0, n, n, c - 4, poles))
+        mats.append("4 %d %d %d 5 4 1.1431 1.3202e16 1.0805e14 %s" % (c + 8, c + 6, c + 2, poles))
+        mats.append("8 %d %d %d %d %d %d 2 3.0 9.5 0.0987" % (2, c + 4, c + 6, 14, n - 4, c + 12))
+        lines.append("BOUNDARY_ALL\ncpml 8 3 -1 0.03 4\n\n")
+        lines.append("SOURCE_POINT\n%d %d %d 0 src.txt\n\n" % (c, c, c + 6))
+        if kind == "rc_cpml_aniso":
+            lines[0] = "POOL\n%d %d %d %g %g %g\n\n" % (n, n, n, 10e-9, 8e-9, 6e-9)
     elif kind == "tsf_pec_liao":
         mats.append("4 %d %d %d 7 10" % (c, c, c))
         lines.append("BOUNDARY_ALL\nliao\n\n")
@@ -547,7 +561,7 @@ def periodic_window(folder, n=40, nsteps=200, window=(8, 32), blocks=(), tsf=Tru
 
 
 SMALL_KINDS = ["vacuum_none", "spheres_cpml", "magnetic_liao", "ade_cpml", "tsf_pec_liao", "tsf_ade_cpml",
-               "periodic_liao", "farfield_cpml", "solids_cpml"]
+               "periodic_liao", "farfield_cpml", "solids_cpml", "rc_cpml", "rc_cpml_aniso"]
 
 
 def pfarfield_case(folder, blocks, n=40, nsteps=200, seed=3):

@@ -163,9 +163,10 @@ def test_ade_entry_equals_direct_and_plrc_rewritten(tmp_path):
     assert mats[1]["nu"] == float(F32(g6(float(F32(2.7e13)))))
     mat, _ = o.maps()
     assert (mat == 1).sum() == 8 ** 3 + 8 ** 3 + 6 * 6 * 6
-    # without a database entry the PLRC line is refused, as type 6 is not built
-    with pytest.raises(RuntimeError):
-        Oracle(par_file(str(tmp_path / "p"), "8 3 14 3 9 20 9 6 %s\n" % ade))
+    # without a database entry the file is not rewritten and the line stays a
+    # piecewise-linear recursive-convolution material (type 6)
+    p = Oracle(par_file(str(tmp_path / "p"), "8 3 14 3 9 20 9 6 %s\n" % ade))
+    assert [m["type"] for m in p.materials()[1:]] == [6]
 
 
 def test_lookup_and_errors(tmp_path, monkeypatch):

@@ -83,7 +83,8 @@ def test_small_case_fields_and_traces(kind, tmp_path):
     s.close()
 
 
-@pytest.mark.parametrize("test_id", ["10000000", "10021100", "10022100", "10024100", "10028000", "10035011"])
+@pytest.mark.parametrize("test_id", ["10000000", "10021100", "10022100", "10024100", "10026000", "10027000",
+                                     "10028000", "10029000", "10035011"])
 def test_gpu_reproduces_reference_selftest(test_id, tmp_path):
     """The reference's own golden traces (printed with %g), recipe src3d/tests.c:92-257."""
     par, nsteps = write_selftest(str(tmp_path / test_id), test_id)
