@@ -64,6 +64,39 @@ ldgs(const T *p)
     return out;
 }
 
+/* an E class entry without its padding word: 16 + 12 bytes, so that no
+ * register receives a value nobody reads (the compiler would otherwise wait
+ * for that load before reusing the register) */
+template <typename T>
+__device__ __forceinline__ T
+ldec(const T *p)
+{
+    static_assert(sizeof(T) == 32, "an E class entry is 32 bytes");
+    typedef float v3f __attribute__((ext_vector_type(3)));
+    T out;
+    const gsv_v4f a = *(const GSV_GLOBAL gsv_v4f *)(const void *)p;
+    const GSV_GLOBAL float *q = (const GSV_GLOBAL float *)(const void *)p;
+    float *o = (float *)(void *)&out;
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = q[4]; o[5] = q[5]; o[6] = q[6]; o[7] = 0.f;
+    return out;
+}
+
+/* an H class entry without its padding: 16 + 4 bytes */
+template <typename T>
+__device__ __forceinline__ T
+ldhc(const T *p)
+{
+    static_assert(sizeof(T) == 32, "an H class entry is 32 bytes");
+    T out;
+    const gsv_v4f a = *(const GSV_GLOBAL gsv_v4f *)(const void *)p;
+    float *o = (float *)(void *)&out;
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = ((const GSV_GLOBAL float *)(const void *)p)[4];
+    o[5] = o[6] = o[7] = 0.f;
+    return out;
+}
+
 /* Every per-cell array is addressed by a byte offset from its allocation
  * start (the guard plane), shared by all arrays: with a 32-bit offset the
  * loads take the uniform-base + 32-bit-offset form, so the z-march carries

@@ -63,7 +63,16 @@ typedef struct {
  * equal P values with separately rounded coefficients (whose rounding, with
  * a0 + a1 - 1 no longer zero, also acts as a spurious restoring term).  The
  * Drude term has s = 0 identically (a0 = (nu dt - 2)/(nu dt + 2), a1 = 4/(nu
- * dt + 2)), so its state is J alone. */
+ * dt + 2)), so its state is J alone.
+ *
+ * Drude-only form.  Without critical points c2 = -a2 / den and cj = a0 / den,
+ * so cj da = dj c2 exactly, and Q^n = cj J^n + c2 E^(n-1) obeys
+ *     E^(n+1) = c0 curl / d + Q^n + c3 E^n,
+ *     Q^(n+1) = dj Q^n + qa E^n + qb E^(n+1),   qa = 2 cj da + c2, qb = cj da:
+ * one state per component instead of two (E^(n-1) and J).  The E kernels of
+ * Drude-only pools without a design region carry Q in states 0-2; the
+ * design-region builds (whose state the adjoint transposes and checkpoints)
+ * keep E^(n-1) and J. */
 typedef struct {
     float c0, c2, c3;  /* E update: c0 curl / d + ... + c2 E^(n-1) + c3 E^n */
     float cj;          /* E-update coefficient of the Drude J: c1 a0 */
@@ -75,6 +84,7 @@ typedef struct {
     float cbm[3];    /* material cb used by the ADE branch's inline CPML, src3d/yee.c:952-1057 */
     int poles;       /* critical-point poles with nonzero terms: 0, 2 or 3 (the third has no reference counterpart) */
     int drude;       /* the Drude current is driven (da != 0); without it J stays 0 and is neither read nor written */
+    float qa, qb;    /* Drude-only form: Q' = dj Q + qa E^n + qb E^(n+1) */
 } GsvAde;
 
 /* recursive-convolution material (types 2 Drude, 4 Drude + 2 critical points,
@@ -126,6 +136,7 @@ typedef struct {
     const GsvECoef *etab;
     const GsvHCoef *htab;
     int netab, nhtab;
+    int ade_id0;            /* E classes with the dispersive flag are numbered from ade_id0 up */
     const float *rkex, *rkey, *rkez;      /* 1/kappa on the E grid, per axis */
     const float *rkhx, *rkhy, *rkhz;      /* 1/kappa on the H grid, per axis */
     GsvCpmlFace f[6];                     /* x0, xn, y0, yn, z0, zn */

@@ -183,10 +183,15 @@ gsv_build_pctab(Engine *E)
     return 0;
 }
 
-/* classes for every cell; ids in device layout (x fastest, row pitch px) */
-template <typename KEY, typename HASH, typename KEYFN>
+/* classes for every cell; ids in device layout (x fastest, row pitch px).
+ * Classes are numbered in two groups, those with late(key) false first: the
+ * E kernel tells a dispersive class from its id alone (id >= the group
+ * boundary, returned in *nearly) and starts the class's state loads with the
+ * plane's field loads instead of after the table lookup. */
+template <typename KEY, typename HASH, typename KEYFN, typename LATEFN>
 static int
-build_classes(Engine *E, KEYFN keyfn, std::vector<KEY> &table, std::vector<uint8_t> &ids_out, int *idbytes)
+build_classes(Engine *E, KEYFN keyfn, LATEFN late, std::vector<KEY> &table, std::vector<uint8_t> &ids_out,
+              int *idbytes, int *nearly)
 {
     const int nx = E->g.nx, ny = E->g.ny, nz = E->g.nz, px = E->g.px;
     const size_t ncell = (size_t)px * ny * nz;
@@ -244,6 +249,26 @@ build_classes(Engine *E, KEYFN keyfn, std::vector<KEY> &table, std::vector<uint8
     if (table.size() > 65536) {
         fprintf(stderr, "Error: %zu distinct coefficient classes exceed the 65536 the engine supports\n", table.size());
         return 1;
+    }
+    {
+        /* stable renumbering: the early group, then the late one */
+        std::vector<uint32_t> newid(table.size());
+        uint32_t nn = 0;
+        for (int pass = 0; pass < 2; pass++)
+            for (size_t q = 0; q < table.size(); q++)
+                if ((int)late(table[q]) == pass)
+                    newid[q] = nn++;
+        std::vector<KEY> sorted(table.size());
+        int ne = 0;
+        for (size_t q = 0; q < table.size(); q++) {
+            sorted[newid[q]] = table[q];
+            ne += !late(table[q]);
+        }
+        table.swap(sorted);
+        for (auto &r : remap)
+            for (uint32_t &v : r)
+                v = newid[v];
+        *nearly = ne;
     }
     *idbytes = table.size() <= 256 ? 1 : 2;
     ids_out.assign(ncell * (*idbytes), 0);
@@ -331,6 +356,16 @@ build_tables(Engine *E)
             if (m->cp3_a[q] != 0 || m->ade_bp0[q] != -1 || m->ade_bp1[q] != 2)
                 a.poles = q < 2 ? 2 : 3;
         a.drude = a.da != 0.f;
+        /* the Drude-only form (engine.h) needs cj da = dj c2, true without poles */
+        {
+            const double cj = m->ade_c1 * m->ade_a0, da = m->ade_a2, dj = -m->ade_a0, c2 = m->ade_c2;
+            a.qa = (float)(2 * cj * da + c2);
+            a.qb = (float)(cj * da);
+            if (a.poles == 0 && fabs(cj * da - dj * c2) > 1e-12 * fabs(cj * da)) {
+                fprintf(stderr, "Error: Drude coefficients of material %d do not satisfy cj da = dj c2\n", n);
+                return 1;
+            }
+        }
         if (a.poles > E->g.ade_poles)
             E->g.ade_poles = a.poles;
         E->ade_slot_of_mat[n] = nslot++;
@@ -446,10 +481,14 @@ build_class_tables(Engine *E, int nslot)
         for (int q = 0; q < 5; q++)
             key->w[q] = fbits(c[q]);
     };
-    if (build_classes<EKey, EKeyHash>(E, ek, ekeys, E->ecls_host, &E->idbytes_e))
+    auto e_late = [](const EKey &key) { return (key.w[6] & GSV_E_ADE) != 0; };
+    auto h_late = [](const HKey &) { return false; };
+    int ne = 0, nh = 0;
+    if (build_classes<EKey, EKeyHash>(E, ek, e_late, ekeys, E->ecls_host, &E->idbytes_e, &ne))
         return 1;
-    if (build_classes<HKey, HKeyHash>(E, hk, hkeys, E->hcls_host, &E->idbytes_h))
+    if (build_classes<HKey, HKeyHash>(E, hk, h_late, hkeys, E->hcls_host, &E->idbytes_h, &nh))
         return 1;
+    E->g.ade_id0 = ne;
     E->etab.resize(ekeys.size());
     for (size_t q = 0; q < ekeys.size(); q++) {
         GsvECoef &c = E->etab[q];

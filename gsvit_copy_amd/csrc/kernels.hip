@@ -79,14 +79,18 @@ hstep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     const bool ijin = iin && jin;
 
     float exk = ldf(g.exg, o), eyk = ldf(g.eyg, o);
+    /* class id one plane ahead: the coefficient gather starts with the plane's loads */
+    unsigned idn = ldid<ID>(g.hcls, oc);
     for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
         const float ezk = ldf(g.ezg, o);
         const float exk1 = ldf(g.exg, o + dP), eyk1 = ldf(g.eyg, o + dP);
         const float ezj1 = ldf(g.ezg, o + dJp), exj1 = ldf(g.exg, o + dJp);
         const float ezi1 = ldf(g.ezg, o + dIp), eyi1 = ldf(g.eyg, o + dIp);
-        const GsvHCoef hc = ldgs(g.htab + ldid<ID>(g.hcls, oc));
-        const float rkz = gp(g.rkhz)[k];
         float hx = ldn(g.hxg, o), hy = ldn(g.hyg, o), hz = ldn(g.hzg, o);
+        const GsvHCoef hc = ldhc(g.htab + idn);
+        if (k + 1 < k1)
+            idn = ldid<ID>(g.hcls, oc + dPc);
+        const float rkz = gp(g.rkhz)[k];
         const bool kin = (k >= 1 && k < nz - 1);
 
         if (!(ye || k == nz - 1))
@@ -193,24 +197,50 @@ k_hstep(GsvGrid g, int kchunk)
  * and their +1 neighbours.  ad/ap: this plane's ADE state blocks, la/lp: the
  * lane's offset in them; state q of a cell sits q * 256 bytes after its first
  * state. */
-template <int NP, bool PER, typename OFF>
+/* the Drude states a dispersive cell reads whatever its material: states 0-2
+ * (E^(n-1), or Q in the Drude-only form) and, in Drude-only builds that keep
+ * J, states 3-5; loaded by the caller with the plane's field loads */
+struct AdePre {
+    float s[6];
+};
+
+template <int NP, bool QF, typename OFF>
+__device__ __forceinline__ AdePre
+ade_preload(const float *ad, OFF la)
+{
+    AdePre p;
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+        p.s[q] = ldsn(ad, la, q * 256);
+#pragma unroll
+    for (int q = 3; q < 6; q++)
+        p.s[q] = (NP == 0 && !QF) ? ldsn(ad, la, q * 256) : 0.f;
+    return p;
+}
+
+template <int NP, bool PER, bool QF, typename OFF>
 __device__ __forceinline__ void
-ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, int k, unsigned fl, float &ex,
-         float &ey, float &ez, float hx, float hy, float hz, float hxkm1, float hykm1, float hzi, float hyi, float hzj, float hxj,
-         float rkx, float rky, float rkz)
+ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, const AdePre &pre, int i, int j, int k, unsigned fl,
+         float &ex, float &ey, float &ez, float hx, float hy, float hz, float hxkm1, float hykm1, float hzi, float hyi,
+         float hzj, float hxj, float rkx, float rky, float rkz)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
     const GSV_GLOBAL GsvAde &m = gp(g.adem)[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
     const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
     const bool xi = i > 0 || (PER && g.perx), yi = j > 0 || (PER && g.pery);   /* not on a (non-periodic) near plane */
     const float vexp = ex, veyp = ey, vezp = ez;
-    const float expv = ldsn(ad, la, 0), eypv = ldsn(ad, la, 256), ezpv = ldsn(ad, la, 512);
+    /* states 0-2: E^(n-1), or Q^n in the Drude-only form (QF, engine.h) */
+    const float expv = pre.s[0], eypv = pre.s[1], ezpv = pre.s[2];
     /* state that stays zero (an undriven Drude current, the poles beyond the
      * material's own in a pool of wider materials) is neither read nor written */
-    const bool dr = NP == 0 || m.drude;   /* Drude-only builds always carry J */
+    const bool dr = !QF && (NP == 0 || m.drude);   /* Drude-only builds always carry J (or Q) */
     const int npl = NP ? m.poles : 0;
     float jx = 0.f, jy = 0.f, jz = 0.f;
-    if (dr) {
+    if (NP == 0) {
+        jx = pre.s[3];
+        jy = pre.s[4];
+        jz = pre.s[5];
+    } else if (dr) {
         jx = ldsn(ad, la, 768);
         jy = ldsn(ad, la, 1024);
         jz = ldsn(ad, la, 1280);
@@ -234,12 +264,21 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
         cz += m.pcj[q] * jj[q][2] + m.pcp[q] * pp[q][2];
     }
 
-    if (yi && k > 0 && !tx)
-        ex = m.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx + cx + m.c2 * expv + m.c3 * ex;
-    if (xi && k > 0 && !ty)
-        ey = m.c0 * ((hx - hxkm1) * rkz - (hz - hzi) * rkx) / g.dy + cy + m.c2 * eypv + m.c3 * ey;
-    if (xi && yi && !tz)
-        ez = m.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz + cz + m.c2 * ezpv + m.c3 * ez;
+    if (QF) {
+        if (yi && k > 0 && !tx)
+            ex = m.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx + expv + m.c3 * ex;
+        if (xi && k > 0 && !ty)
+            ey = m.c0 * ((hx - hxkm1) * rkz - (hz - hzi) * rkx) / g.dy + eypv + m.c3 * ey;
+        if (xi && yi && !tz)
+            ez = m.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz + ezpv + m.c3 * ez;
+    } else {
+        if (yi && k > 0 && !tx)
+            ex = m.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx + cx + m.c2 * expv + m.c3 * ex;
+        if (xi && k > 0 && !ty)
+            ey = m.c0 * ((hx - hxkm1) * rkz - (hz - hzi) * rkx) / g.dy + cy + m.c2 * eypv + m.c3 * ey;
+        if (xi && yi && !tz)
+            ez = m.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz + cz + m.c2 * ezpv + m.c3 * ez;
+    }
 
     const GsvCpmlFace &F0 = g.f[0], &F1 = g.f[1], &F2 = g.f[2], &F3 = g.f[3], &F4 = g.f[4], &F5 = g.f[5];
     if (g.ade_cpml) {
@@ -386,6 +425,12 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, int i, int j, i
     }
 
     /* polarisation state (all three components, with the values at this point) */
+    if (QF) {
+        stsn(ad, la, 0, m.dj * expv + m.qa * vexp + m.qb * ex);
+        stsn(ad, la, 256, m.dj * eypv + m.qa * veyp + m.qb * ey);
+        stsn(ad, la, 512, m.dj * ezpv + m.qa * vezp + m.qb * ez);
+        return;
+    }
     if (dr) {
         stsn(ad, la, 768, m.dj * jx + m.da * (expv + 2.f * vexp + ex));
         stsn(ad, la, 1024, m.dj * jy + m.da * (eypv + 2.f * veyp + ey));
@@ -593,27 +638,38 @@ estep_col(const GsvGrid &g, int i, int j, int kz, int kchunk)
     const OFF dJm = (pery && j == 0) ? (OFF)(ny - 1) * dJ : (OFF)0 - dJ;
     const bool cons = ADE && g.ade_cpml;
 
+    constexpr bool QF = ADE == 1 && !PC;   /* Drude-only state form (engine.h) */
     float hxkm1 = ldf(g.hxg, o - dP), hykm1 = ldf(g.hyg, o - dP);
+    /* the class id is loaded one plane ahead, so that the coefficient gather
+     * and (in a dispersive class: ids from ade_id0 up) the state loads start
+     * with the plane's field loads instead of after them */
+    unsigned idn = ldid<ID>(g.ecls, oc);
     for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
+        const unsigned id = idn;
         const float hx = ldn(g.hxg, o), hy = ldn(g.hyg, o), hz = ldn(g.hzg, o);
         const float hzj = ldf(g.hzg, o + dJm), hxj = ldf(g.hxg, o + dJm);
         const float hzi = ldf(g.hzg, o + dIm), hyi = ldf(g.hyg, o + dIm);
-        const float rkz = gp(g.rkez)[k];
-        GsvECoef ec = ldgs(g.etab + ldid<ID>(g.ecls, oc));
-        if (PC && (ec.flags & GSV_E_PERCELL))
-            ec = ldgs(g.pctab + ((i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))));
         float ex = ldn(g.exg, o), ey = ldn(g.eyg, o), ez = ldn(g.ezg, o);
+        GsvECoef ec = ldec(g.etab + id);
+        const bool dis = ADE != 0 && ADE != 4 && id >= (unsigned)g.ade_id0;
+        float *adk = ADE ? g.ade_d + (size_t)k * g.ade_plane_d : nullptr;
+        AdePre pre = {};
+        if (dis)
+            pre = ade_preload<NP, QF>(adk, la);
+        if (k + 1 < k1)
+            idn = ldid<ID>(g.ecls, oc + dPc);
+        const float rkz = gp(g.rkez)[k];
+        if (PC && (ec.flags & GSV_E_PERCELL))
+            ec = ldec(g.pctab + ((i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))));
         const unsigned fl = ec.flags;
         const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
 
         if (ADE == 4 && (fl & GSV_E_ADE))
-            rc_cell<PER>(g, g.ade_d + (size_t)k * g.ade_plane_d, la,
-                         g.ade_p ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr, lp, i, j, k, fl, ex, ey, ez,
-                         hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
-        else if (ADE && (fl & GSV_E_ADE))
-            ade_cell<NP, PER>(g, g.ade_d + (size_t)k * g.ade_plane_d, la,
-                         NP ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr, lp, i, j, k, fl, ex, ey, ez,
-                         hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
+            rc_cell<PER>(g, adk, la, g.ade_p ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr, lp, i, j, k, fl, ex, ey,
+                         ez, hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
+        else if (dis)
+            ade_cell<NP, PER, QF>(g, adk, la, NP ? g.ade_p + (size_t)k * g.ade_plane_p : nullptr, lp, pre, i, j, k, fl,
+                                  ex, ey, ez, hx, hy, hz, hxkm1, hykm1, hzi, hyi, hzj, hxj, rkx, rky, rkz);
 
         if (tx && yi && k > 0)
             ex = ec.ca[0] * ex + ec.cb[0] * ((hz - hzj) * rky - (hy - hykm1) * rkz);
