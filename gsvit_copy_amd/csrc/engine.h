@@ -74,18 +74,26 @@ typedef struct {
  * design-region builds (whose state the adjoint transposes and checkpoints)
  * keep E^(n-1) and J. */
 typedef struct {
-    float c0, c2, c3;  /* E update: c0 curl / d + ... + c2 E^(n-1) + c3 E^n */
+    float c0, c3, dj;  /* E update: c0 curl / d + ... + c2 E^(n-1) + c3 E^n; Drude: J' = dj J + ..., dj = -a0 */
+    float qa, qb;      /* Drude-only form: Q' = dj Q + qa E^n + qb E^(n+1) */
+    float cbm[3];    /* material cb used by the ADE branch's inline CPML, src3d/yee.c:952-1057 */
+    /* (the 32 bytes above are all the Drude-only form reads: GsvAdeQ) */
+    float c2;
     float cj;          /* E-update coefficient of the Drude J: c1 a0 */
-    float dj, da;      /* Drude: J' = dj J + da (E^(n-1) + 2 E^n + E^(n+1)); dj = -a0, da = a2 */
+    float da;          /* Drude: J' = dj J + da (E^(n-1) + 2 E^n + E^(n+1)); da = a2 */
     float pcj[3];      /* E-update coefficients of a pole's J and P: c1 bp0, -c1 s */
     float pcp[3];
     float pj[3], ps[3];           /* pole J recursion coefficients: -bp0, s */
     float b2[3], b3[3], b4[3];    /* pole E coefficients: bp2, bp3, bp4 */
-    float cbm[3];    /* material cb used by the ADE branch's inline CPML, src3d/yee.c:952-1057 */
     int poles;       /* critical-point poles with nonzero terms: 0, 2 or 3 (the third has no reference counterpart) */
     int drude;       /* the Drude current is driven (da != 0); without it J stays 0 and is neither read nor written */
-    float qa, qb;    /* Drude-only form: Q' = dj Q + qa E^n + qb E^(n+1) */
 } GsvAde;
+
+/* the leading 32 bytes of a GsvAde: the coefficients of the Drude-only form */
+typedef struct {
+    float c0, c3, dj, qa, qb;
+    float cbm[3];
+} GsvAdeQ;
 
 /* recursive-convolution material (types 2 Drude, 4 Drude + 2 critical points,
  * 6 piecewise-linear), src3d/yee.c:266-455, 620-905, with the coefficients the

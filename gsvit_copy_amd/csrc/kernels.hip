@@ -226,6 +226,20 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, const AdePre &p
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz, px = g.px;
     const GSV_GLOBAL GsvAde &m = gp(g.adem)[(fl >> GSV_E_SLOT_SHIFT) & 0xF];
+    /* the Drude-only form's coefficients in one 16-byte gather and one load,
+     * instead of one dependent load per coefficient and component (the inline
+     * CPML's cb, read in slab cells only, stays where it is used) */
+    struct {
+        float c0, c3, dj, qa, qb;
+    } mq = {};
+    if (QF) {
+        const gsv_v4f a = *(const GSV_GLOBAL gsv_v4f *)(const void *)&m;
+        mq.c0 = a.x;
+        mq.c3 = a.y;
+        mq.dj = a.z;
+        mq.qa = a.w;
+        mq.qb = m.qb;
+    }
     const bool tx = fl & GSV_E_TBCX, ty = fl & GSV_E_TBCY, tz = fl & GSV_E_TBCZ;
     const bool xi = i > 0 || (PER && g.perx), yi = j > 0 || (PER && g.pery);   /* not on a (non-periodic) near plane */
     const float vexp = ex, veyp = ey, vezp = ez;
@@ -266,11 +280,11 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, const AdePre &p
 
     if (QF) {
         if (yi && k > 0 && !tx)
-            ex = m.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx + expv + m.c3 * ex;
+            ex = mq.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx + expv + mq.c3 * ex;
         if (xi && k > 0 && !ty)
-            ey = m.c0 * ((hx - hxkm1) * rkz - (hz - hzi) * rkx) / g.dy + eypv + m.c3 * ey;
+            ey = mq.c0 * ((hx - hxkm1) * rkz - (hz - hzi) * rkx) / g.dy + eypv + mq.c3 * ey;
         if (xi && yi && !tz)
-            ez = m.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz + ezpv + m.c3 * ez;
+            ez = mq.c0 * ((hy - hyi) * rkx - (hx - hxj) * rky) / g.dz + ezpv + mq.c3 * ez;
     } else {
         if (yi && k > 0 && !tx)
             ex = m.c0 * ((hz - hzj) * rky - (hy - hykm1) * rkz) / g.dx + cx + m.c2 * expv + m.c3 * ex;
@@ -284,7 +298,7 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, const AdePre &p
     if (g.ade_cpml) {
         /* ADE_CPML_CONSISTENT: the psi of each dispersive component, once per
          * step over the slab ranges of the ordinary update, applied with c0 */
-        const float c0 = m.c0;
+        const float c0 = QF ? mq.c0 : m.c0;
         if (F0.on && i >= 1 && i < F0.depth) {
             const OFF p = (OFF)(((unsigned)k * ny + j) * F0.depth + i) * 8;
             const float4 bc = ldg4(F0.ce + i);
@@ -426,9 +440,9 @@ ade_cell(const GsvGrid &g, float *ad, OFF la, float *ap, OFF lp, const AdePre &p
 
     /* polarisation state (all three components, with the values at this point) */
     if (QF) {
-        stsn(ad, la, 0, m.dj * expv + m.qa * vexp + m.qb * ex);
-        stsn(ad, la, 256, m.dj * eypv + m.qa * veyp + m.qb * ey);
-        stsn(ad, la, 512, m.dj * ezpv + m.qa * vezp + m.qb * ez);
+        stsn(ad, la, 0, mq.dj * expv + mq.qa * vexp + mq.qb * ex);
+        stsn(ad, la, 256, mq.dj * eypv + mq.qa * veyp + mq.qb * ey);
+        stsn(ad, la, 512, mq.dj * ezpv + mq.qa * vezp + mq.qb * ez);
         return;
     }
     if (dr) {
@@ -781,20 +795,6 @@ k_estep(GsvGrid g, int kchunk)
 {
     estep_col<ID, ADE, OFF, PC, PER>(g, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
                                 blockIdx.z, kchunk);
-}
-
-/* The same column held to seven waves per SIMD (at most 72 VGPRs), used for
- * the builds that fit that bound without spilling: no poles, 32-bit offsets,
- * no design region, no periodic axis (the Drude build needs 71 VGPRs, against
- * 74 and six waves unconstrained).  More waves hide more of the z-march's
- * load latency: 512^3 Drude pool, 256 x 1 x 8 tiles, 1.918 -> 1.805 ms with
- * bit-identical fields. */
-template <typename ID, int ADE>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7, 8))) void
-k_estep_w7(GsvGrid g, int kchunk)
-{
-    estep_col<ID, ADE, uint32_t, false, false>(g, blockIdx.x * blockDim.x + threadIdx.x,
-                                               blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z, kchunk);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1617,13 +1617,7 @@ template <typename ID, typename OFF, bool PC, bool PER>
 static void
 launch_estep(const GsvGrid *g, int ade, dim3 grd, dim3 blk, int kchunk, hipStream_t st)
 {
-    constexpr bool w7 = sizeof(OFF) == 4 && !PC && !PER;
-    if (w7 && ade <= 1) {
-        if (ade == 0)
-            hipLaunchKernelGGL((k_estep_w7<ID, 0>), grd, blk, 0, st, *g, kchunk);
-        else
-            hipLaunchKernelGGL((k_estep_w7<ID, 1>), grd, blk, 0, st, *g, kchunk);
-    } else if (ade == 0)
+    if (ade == 0)
         hipLaunchKernelGGL((k_estep<ID, 0, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
     else if (ade == 1)
         hipLaunchKernelGGL((k_estep<ID, 1, OFF, PC, PER>), grd, blk, 0, st, *g, kchunk);
