@@ -150,16 +150,6 @@ k_rec_post(GsvGrid g, GsvAdj a, int slot)
  * volume kernels, address every per-cell array by one 32-bit byte offset from
  * its guard plane and carry the neighbour plane they share in registers. */
 
-template <typename ID, typename OFF>
-__device__ __forceinline__ GsvECoef
-e_coef(const GsvGrid &g, OFF oc, int i, int j, int k)
-{
-    GsvECoef ec = g.etab[ldid<ID>(g.ecls, oc)];
-    if (ec.flags & GSV_E_PERCELL)
-        ec = g.pctab[(i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))];
-    return ec;
-}
-
 /* transpose of ade_cell's step for one dispersive cell (see the file head):
  * e[c] holds the adjoint of E' on entry and the adjoint of E on return, q[c]
  * the curl coefficient's adjoint; xd / xp the cell's adjoint state */
@@ -250,9 +240,14 @@ k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int kchunk)
 
     /* r of plane k-1 (the guard plane below k = 0 is zero) */
     float rxm = ldf(a.rx, o - dP), rym = ldf(a.ry, o - dP);
+    /* class id one plane ahead: the coefficient gather starts with the plane's loads */
+    unsigned idn = ldid<ID>(g.ecls, oc);
     for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
         float ex = ldn(a.ebx, o), ey = ldn(a.eby, o), ez = ldn(a.ebz, o);
         const float rxc = ldf(a.rx, o), ryc = ldf(a.ry, o), rzc = ldf(a.rz, o);
+        GsvECoef ec = ldec(g.etab + idn);
+        if (k + 1 < k1)
+            idn = ldid<ID>(g.ecls, oc + dPc);
         const float rhz = g.rkhz[k], rhz1 = k > 0 ? g.rkhz[k - 1] : 0.f;
         /* transpose of the H curl of step n+1 (H-grid 1/kappa) */
         ex += rhz * ryc - rhy * rzc - rhz1 * rym;
@@ -296,7 +291,8 @@ k_adj_e(GsvGrid g, GsvAdj a, int step, int slot, int kchunk)
             ey = 0.f;
             ez = 0.f;
         }
-        const GsvECoef ec = e_coef<ID>(g, oc, i, j, k);
+        if (ec.flags & GSV_E_PERCELL)
+            ec = g.pctab[(i - g.pc_i0) + g.pc_nx * ((j - g.pc_j0) + g.pc_ny * (k - g.pc_k0))];
         const unsigned fl = ec.flags;
         if (in_pc_ij && k >= g.pc_k0 && k < g.pc_k0 + g.pc_nz) {
             /* design coefficient cell: dJ/dca += adj(E^{n+1}) E^n, dJ/dcb += adj(E^{n+1}) D^n */
@@ -362,8 +358,13 @@ k_adj_h(GsvGrid g, GsvAdj a, int kchunk)
 
     /* q of plane k (carried), the plane above is loaded each step (guard plane above nz-1 is zero) */
     float qxk = ldf(a.qx, o), qyk = ldf(a.qy, o);
+    /* class id one plane ahead: the coefficient gather starts with the plane's loads */
+    unsigned idn = ldid<ID>(g.hcls, oc);
     for (int k = k0; k < k1; ++k, o += dP, oc += dPc) {
         float hx = ldn(a.hbx, o), hy = ldn(a.hby, o), hz = ldn(a.hbz, o);
+        const GsvHCoef hc = ldhc(g.htab + idn);
+        if (k + 1 < k1)
+            idn = ldid<ID>(g.hcls, oc + dPc);
         const float qzc = ldf(a.qz, o);
         const float qxk1 = ldf(a.qx, o + dP), qyk1 = ldf(a.qy, o + dP);
         const float rez = g.rkez[k], rez1 = k + 1 < nz ? g.rkez[k + 1] : 0.f;
@@ -388,7 +389,6 @@ k_adj_h(GsvGrid g, GsvAdj a, int kchunk)
             hy = 0.f;
             hz = 0.f;
         }
-        const GsvHCoef hc = g.htab[ldid<ID>(g.hcls, oc)];
         const bool ux = !(j == ny - 1 || k == nz - 1), uy = !(i == nx - 1 || k == nz - 1);
         const bool uz = !(i == nx - 1 || j == ny - 1);
         stf(a.rx, o, ux ? hc.db[0] * hx : 0.f);
