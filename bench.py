@@ -32,7 +32,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(info, shape, depth, n_ade_cells):
+def algorithmic_bytes(info, shape, depth, n_ade_cells, engine=False):
     """Algorithmic HBM bytes of one step, per kernel, by the reference
     formulation's per-cell figures.
 
@@ -41,16 +41,17 @@ def algorithmic_bytes(info, shape, depth, n_ade_cells):
     CPML: per stretched direction inside a slab, 2 psi read + 2 written per
     half-step = 16 B/cell/half-step.  Drude ADE: the reference's state E(n-1),
     dP, dP(n-1) per component read and written in the E half-step = 72 B per
-    dispersive cell.  The engine keeps the Drude state in increment form
-    (E(n-1) and J = P^n - P^(n-1), 48 B per dispersive cell), so its kernels
-    move fewer bytes than this count; the measured traffic is reported beside
-    it.
+    dispersive cell.  With engine=True the E count is the engine's own: a
+    Drude-only pool keeps one state per component (Q = cj J + c2 E(n-1), 24 B
+    per dispersive cell read and written), so its kernels move fewer bytes
+    than the reference-formulation count; the measured traffic is reported
+    beside both.
     """
     nx, ny, nz = (shape, shape, shape) if np.isscalar(shape) else shape
     cells = float(nx) * ny * nz
     slab_dir_cells = 2.0 * depth * (float(ny) * nz + float(nx) * nz + float(nx) * ny)   # cell-directions in the six slabs
     h_bytes = 37.0 * cells + 16.0 * slab_dir_cells
-    e_bytes = 37.0 * cells + 16.0 * slab_dir_cells + 72.0 * n_ade_cells
+    e_bytes = 37.0 * cells + 16.0 * slab_dir_cells + (24.0 if engine else 72.0) * n_ade_cells
     return h_bytes, e_bytes
 
 
@@ -500,15 +501,17 @@ def main():
     # HBM traffic of the dominant kernel from the committed counter summary
     # (rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by the copy-kernel
     # calibration), used only when it describes this workload and launch shape
-    traffic, traffic_src = None, None
+    traffic, traffic_src, traffic_shape = None, None, None
     try:
         with open(os.path.join(HERE, "profiles", "hbm_traffic_rough512.json")) as f:
             tr = json.load(f)
-        if n == 512 and tr["launch_shape"] == kshape and dom in tr["kernels"]:
+        if n == 512 and dom in tr["kernels"]:
             traffic = round(tr["kernels"][dom]["traffic_bytes"])
             traffic_src = "profiles/hbm_traffic_rough512.md"
+            traffic_shape = tr["launch_shape"]   # the counters' launch shape (the halo re-reads depend on it)
     except (OSError, ValueError, KeyError):
         pass
+    engine_bytes = algorithmic_bytes(info, n, depth, n_ade, engine=True)[1 if dom == "k_estep" else 0]
 
     attainable = None
     try:
@@ -580,7 +583,12 @@ def main():
                      "kernels": ktab,
                      "step_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "attainable": attainable,
-                     "frac_of_attainable": round(achieved / attainable["GB_s"], 4) if attainable else None},
+                     "traffic_launch_shape": traffic_shape,
+                     "traffic_GB_s": round(traffic / (dom_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_frac_of_attainable": (round(traffic / (dom_ms * 1e-3) / 1e9 / attainable["GB_s"], 4)
+                                                    if traffic and attainable else None),
+                     "engine_bytes": engine_bytes,
+                     "engine_frac": round(engine_bytes / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "batch": batch,

@@ -795,6 +795,41 @@ setup_sf(Engine *E)
     return 0;
 }
 
+/* Beam profiles at lateral cell position (x, y), src3d/source.c:231-251:
+ * the Gaussian exp(-(x-xc)^2/2rx^2) exp(-(y-yc)^2/2ry^2) and the fiber mode
+ * exp(-r^2 dx^2/w0^2) inside the cutoff radius, with the mode width w0 from
+ * the V number at the centre wavelength (V taken with the permittivity
+ * difference, as the reference does) */
+static double
+beam_gauss(int x, int y, double xc, double yc, double rx, double ry)
+{
+    return exp(-((double)(x - xc)) * ((double)(x - xc)) / rx / rx / 2.0) * exp(-((double)(y - yc)) * ((double)(y - yc)) / ry / ry / 2.0);
+}
+
+static double
+beam_fiber(int x, int y, double xc, double yc, double radius, double cutoff, double eps_core, double eps_clad,
+           double dx, double lambda)
+{
+    const double v = 2 * M_PI / lambda * radius * dx * (eps_core - eps_clad);
+    const double w0 = (0.65 + 1.619 / v / sqrt(v) + 2.879 / pow(v, 6)) * radius * dx;
+    const double rv = (x - xc) * (x - xc) + (y - yc) * (y - yc);
+    return rv < cutoff * cutoff ? exp(-rv * dx * dx / w0 / w0) : 0.0;
+}
+
+/* TSF face multiplier: the faces' "if Gaussian, else if fiber" at
+ * src3d/source.c:558-742, 1482-1649 */
+static double
+tsf_profile(const SvSet *set, int x, int y)
+{
+    const SvSetSourceTSF *t = &set->ss.tsf;
+    if (t->gaussian)
+        return beam_gauss(x, y, t->gaussian_fxpos, t->gaussian_fypos, t->gaussian_rx, t->gaussian_ry);
+    if (t->fiber)
+        return beam_fiber(x, y, t->fiber_fxpos, t->fiber_fypos, t->fiber_radius, t->fiber_cutoff, t->fiber_epsilon_core,
+                          t->fiber_epsilon_cladding, set->sp.dx, set->ss.lambda_center);
+    return 1.0;
+}
+
 static int
 setup_tsf(Engine *E)
 {
@@ -830,8 +865,12 @@ setup_tsf(Engine *E)
                     continue;
                 double eps, sig;
                 sv_pool_epsilon_sigma(mp, i, j, k, &eps, &sig);
-                ce[(size_t)a + (size_t)na * b] = (float)(ecor / eps);
-                chh[(size_t)a + (size_t)na * b] = (float)hcor;
+                /* the profile is taken at the cell whose line value the face uses:
+                 * E faces at the near side one cell outside the box (i0 - 1, j0 - 1),
+                 * H faces on the face plane; z faces at (i, j) */
+                const int xe = axis == 0 && !far ? i - 1 : i, ye = axis == 1 && !far ? j - 1 : j;
+                ce[(size_t)a + (size_t)na * b] = (float)(ecor * tsf_profile(set, xe, ye) / eps);
+                chh[(size_t)a + (size_t)na * b] = (float)(hcor * tsf_profile(set, i, j));
             }
         E->tsf_face_on[f] = 0;
         long active = 0;
@@ -1028,15 +1067,10 @@ ltsf_profile(const SvSet *set, int x, int y, int h_far_x)
     const SvSetSourceLTSF *t = &set->ss.ltsf;
     double gm = 1.0, fm = 1.0;
     if (t->gaussian)
-        gm = exp(-((double)(x - t->gaussian_fxpos)) * ((double)(x - t->gaussian_fxpos)) / t->gaussian_rx / t->gaussian_rx / 2.0)
-             * exp(-((double)(y - t->gaussian_fypos)) * ((double)(y - t->gaussian_fypos)) / t->gaussian_ry / t->gaussian_ry / 2.0);
-    if (t->fiber) {
-        const double dx = set->sp.dx, lambda = set->ss.lambda_center;
-        const double v = 2 * M_PI / lambda * t->fiber_radius * dx * (t->fiber_epsilon_core - t->fiber_epsilon_cladding);
-        const double w0 = (0.65 + 1.619 / v / sqrt(v) + 2.879 / pow(v, 6)) * t->fiber_radius * dx;
-        const double rv = (x - t->fiber_fxpos) * (x - t->fiber_fxpos) + (y - t->fiber_fypos) * (y - t->fiber_fypos);
-        fm = rv < t->fiber_cutoff * t->fiber_cutoff ? exp(-rv * dx * dx / w0 / w0) : 0.0;
-    }
+        gm = beam_gauss(x, y, t->gaussian_fxpos, t->gaussian_fypos, t->gaussian_rx, t->gaussian_ry);
+    if (t->fiber)
+        fm = beam_fiber(x, y, t->fiber_fxpos, t->fiber_fypos, t->fiber_radius, t->fiber_cutoff, t->fiber_epsilon_core,
+                        t->fiber_epsilon_cladding, set->sp.dx, set->ss.lambda_center);
     if (h_far_x)
         return t->fiber ? fm : 1.0;
     return t->gaussian ? gm : (t->fiber ? fm : 1.0);

@@ -552,6 +552,25 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
                 if (get_int(fr, &t->box_boundary_skipdepth, key))
                     goto fail;
             }
+        } else if (!strcmp(key, "TSF_GAUSSIAN_Z")) {
+            SvSetSourceTSF *t = &set->ss.tsf;
+            t->gaussian = 1;
+            if (get_double(fr, &t->gaussian_fxpos, key) || get_double(fr, &t->gaussian_fypos, key)
+                || get_double(fr, &t->gaussian_rx, key) || get_double(fr, &t->gaussian_ry, key))
+                goto fail;
+        } else if (!strcmp(key, "TSF_FIBER_Z")) {
+            SvSetSourceTSF *t = &set->ss.tsf;
+            t->fiber = 1;
+            if (get_double(fr, &t->fiber_fxpos, key) || get_double(fr, &t->fiber_fypos, key)
+                || get_double(fr, &t->fiber_radius, key) || get_double(fr, &t->fiber_cutoff, key)
+                || get_double(fr, &t->fiber_epsilon_core, key) || get_double(fr, &t->fiber_epsilon_cladding, key))
+                goto fail;
+        } else if (!strcmp(key, "TSF_RADIAL_Z")) {
+            /* the reference's CPU faces evaluate this profile at a loop index
+             * that is never set on the i faces (src3d/source.c:574-580) and its
+             * GPU path leaves it out (src3d/gpu.c:2405) */
+            fprintf(stderr, "Error: TSF_RADIAL_Z is not supported by the MI355X engine\n");
+            goto fail;
         } else if (!strcmp(key, "LTSF_GAUSSIAN")) {
             SvSetSourceLTSF *t = &set->ss.ltsf;
             t->gaussian = 1;
@@ -725,6 +744,18 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             SvSetBoundary *b = &set->sb;
             if (parse_cpml_face(fr, key, set->sp.dz, &b->bzn, &b->depth_bzn, &b->m_bzn, &b->sigma_bzn, &b->a_bzn, &b->kappa_bzn))
                 goto fail;
+        } else if (!strcmp(key, "MBOUNDARY_ALL")) {
+            /* src3d/settings.c:5109-5128: one condition, then the x0 xn y0 yn z0 zn plane positions */
+            SvSetMediumBoundary *m = &set->smb;
+            if (get_token(fr, value, sizeof(value)) || get_int(fr, &m->bx0pos, key) || get_int(fr, &m->bxnpos, key)
+                || get_int(fr, &m->by0pos, key) || get_int(fr, &m->bynpos, key) || get_int(fr, &m->bz0pos, key)
+                || get_int(fr, &m->bznpos, key))
+                goto fail;
+            if (strcmp(value, "periodic")) {
+                fprintf(stderr, "Error: unsupported medium boundary condition (%s)\n", value);
+                goto fail;
+            }
+            m->bx0 = m->bxn = m->by0 = m->byn = m->bz0 = m->bzn = SV_BOUNDARY_PERIODIC;
         } else if (!strncmp(key, "MBOUNDARY_", 10)) {
             /* src3d/settings.c:5047-5128 */
             int pos, *type = NULL, *ppos = NULL;

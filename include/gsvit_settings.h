@@ -185,6 +185,12 @@ typedef struct {
     int box_boundary_skipi0, box_boundary_skipj0, box_boundary_skipk0;
     int box_boundary_skipin, box_boundary_skipjn, box_boundary_skipkn;
     int box_boundary_skipdepth;
+    /* beam profiles of the face corrections, TSF_GAUSSIAN_Z and TSF_FIBER_Z
+     * (src3d/settings.c:4308-4339, src3d/source.c:231-251) */
+    int gaussian;
+    double gaussian_fxpos, gaussian_fypos, gaussian_rx, gaussian_ry;
+    int fiber;
+    double fiber_fxpos, fiber_fypos, fiber_radius, fiber_cutoff, fiber_epsilon_core, fiber_epsilon_cladding;
 } SvSetSourceTSF;
 
 /* layered total/scattered-field source, src3d/settings.h (SvSetSourceLTSF)

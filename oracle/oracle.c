@@ -1450,6 +1450,27 @@ sf_estep(Oracle *o)
         line_step(o, t, o->se, o->sh, o->sb, o->sres);
 }
 
+/* TSF beam profile at lateral position (x, y), src3d/source.c:231-251: the
+ * faces take the Gaussian if it is set, else the fiber mode (:558-742, 1482-1649) */
+static double
+tsf_profile(const Oracle *o, int x, int y)
+{
+    const SvSetSourceTSF *t = &o->set.ss.tsf;
+    if (t->gaussian)
+        return exp(-((double)(x - t->gaussian_fxpos)) * ((double)(x - t->gaussian_fxpos)) / t->gaussian_rx / t->gaussian_rx / 2.0)
+               * exp(-((double)(y - t->gaussian_fypos)) * ((double)(y - t->gaussian_fypos)) / t->gaussian_ry / t->gaussian_ry / 2.0);
+    if (t->fiber) {
+        const double dx = o->set.sp.dx, lambda = o->set.ss.lambda_center;
+        double v = 2 * M_PI / lambda * t->fiber_radius * dx * (t->fiber_epsilon_core - t->fiber_epsilon_cladding);
+        double omegazero = (0.65 + 1.619 / v / sqrt(v) + 2.879 / pow(v, 6)) * t->fiber_radius * dx;
+        double radval = ((x - t->fiber_fxpos) * (x - t->fiber_fxpos) + (y - t->fiber_fypos) * (y - t->fiber_fypos));
+        if (radval < (t->fiber_cutoff * t->fiber_cutoff))
+            return exp(-radval * dx * dx / omegazero / omegazero);
+        return 0;
+    }
+    return 1.0;
+}
+
 static void
 source_hstep(Oracle *o)
 {
@@ -1476,40 +1497,46 @@ source_hstep(Oracle *o)
             for (k = t->box_k0; k <= t->box_kn; k++) {
                 if (!t->skip_i0 && sv_tsf_face_active(o->mp, set, t->box_i0, j, k)) {
                     double ve = VE(t->box_i0, j, k);
-                    if (k < t->box_kn) o->hy[IDX(o, t->box_i0 - 1, j, k)] -= hcor * sv_gez(ve, th, ph, ps);
-                    if (j < t->box_jn) o->hz[IDX(o, t->box_i0 - 1, j, k)] += hcor * sv_gey(ve, th, ph, ps);
+                    const double gh = hcor * tsf_profile(o, t->box_i0, j);
+                    if (k < t->box_kn) o->hy[IDX(o, t->box_i0 - 1, j, k)] -= gh * sv_gez(ve, th, ph, ps);
+                    if (j < t->box_jn) o->hz[IDX(o, t->box_i0 - 1, j, k)] += gh * sv_gey(ve, th, ph, ps);
                 }
                 if (!t->skip_in && sv_tsf_face_active(o->mp, set, t->box_in, j, k)) {
                     double ve = VE(t->box_in, j, k);
-                    if (k < t->box_kn) o->hy[IDX(o, t->box_in, j, k)] += hcor * sv_gez(ve, th, ph, ps);
-                    if (j < t->box_jn) o->hz[IDX(o, t->box_in, j, k)] -= hcor * sv_gey(ve, th, ph, ps);
+                    const double gh = hcor * tsf_profile(o, t->box_in, j);
+                    if (k < t->box_kn) o->hy[IDX(o, t->box_in, j, k)] += gh * sv_gez(ve, th, ph, ps);
+                    if (j < t->box_jn) o->hz[IDX(o, t->box_in, j, k)] -= gh * sv_gey(ve, th, ph, ps);
                 }
             }
         for (i = t->box_i0; i <= t->box_in && i < X; i++)
             for (k = t->box_k0; k <= t->box_kn; k++) {
                 if (!t->skip_j0 && sv_tsf_face_active(o->mp, set, i, t->box_j0, k)) {
                     double ve = VE(i, t->box_j0, k);
-                    if (k < t->box_kn) o->hx[IDX(o, i, t->box_j0 - 1, k)] += hcor * sv_gez(ve, th, ph, ps);
-                    if (i < t->box_in) o->hz[IDX(o, i, t->box_j0 - 1, k)] -= hcor * sv_gex(ve, th, ph, ps);
+                    const double gh = hcor * tsf_profile(o, i, t->box_j0);
+                    if (k < t->box_kn) o->hx[IDX(o, i, t->box_j0 - 1, k)] += gh * sv_gez(ve, th, ph, ps);
+                    if (i < t->box_in) o->hz[IDX(o, i, t->box_j0 - 1, k)] -= gh * sv_gex(ve, th, ph, ps);
                 }
                 if (!t->skip_jn && sv_tsf_face_active(o->mp, set, i, t->box_jn, k)) {
                     double ve = VE(i, t->box_jn, k);
-                    if (k < t->box_kn) o->hx[IDX(o, i, t->box_jn, k)] -= hcor * sv_gez(ve, th, ph, ps);
-                    if (i < t->box_in) o->hz[IDX(o, i, t->box_jn, k)] += hcor * sv_gex(ve, th, ph, ps);
+                    const double gh = hcor * tsf_profile(o, i, t->box_jn);
+                    if (k < t->box_kn) o->hx[IDX(o, i, t->box_jn, k)] -= gh * sv_gez(ve, th, ph, ps);
+                    if (i < t->box_in) o->hz[IDX(o, i, t->box_jn, k)] += gh * sv_gex(ve, th, ph, ps);
                 }
             }
         for (i = t->box_i0; i <= t->box_in && i < X; i++)
             for (j = t->box_j0; j <= t->box_jn && j < Y; j++) {
                 if (!t->skip_k0 && sv_tsf_face_active(o->mp, set, i, j, t->box_k0)) {
                     double ve = VE(i, j, t->box_k0);
+                    const double gh = hcor * tsf_profile(o, i, j);
                     int pec = sv_pool_is_pec(o->mp, i, j, t->box_k0);
-                    if (j < t->box_jn && !pec) o->hx[IDX(o, i, j, t->box_k0 - 1)] -= hcor * sv_gey(ve, th, ph, ps);
-                    if (i < t->box_in && !pec) o->hy[IDX(o, i, j, t->box_k0 - 1)] += hcor * sv_gex(ve, th, ph, ps);
+                    if (j < t->box_jn && !pec) o->hx[IDX(o, i, j, t->box_k0 - 1)] -= gh * sv_gey(ve, th, ph, ps);
+                    if (i < t->box_in && !pec) o->hy[IDX(o, i, j, t->box_k0 - 1)] += gh * sv_gex(ve, th, ph, ps);
                 }
                 if (!t->skip_kn && sv_tsf_face_active(o->mp, set, i, j, t->box_kn)) {
                     double ve = VE(i, j, t->box_kn);
-                    if (j < t->box_jn) o->hx[IDX(o, i, j, t->box_kn)] += hcor * sv_gey(ve, th, ph, ps);
-                    if (i < t->box_in) o->hy[IDX(o, i, j, t->box_kn)] -= hcor * sv_gex(ve, th, ph, ps);
+                    const double gh = hcor * tsf_profile(o, i, j);
+                    if (j < t->box_jn) o->hx[IDX(o, i, j, t->box_kn)] += gh * sv_gey(ve, th, ph, ps);
+                    if (i < t->box_in) o->hy[IDX(o, i, j, t->box_kn)] -= gh * sv_gex(ve, th, ph, ps);
                 }
             }
 #undef VE
@@ -1541,40 +1568,46 @@ source_estep(Oracle *o)
             for (k = t->box_k0; k <= t->box_kn; k++) {
                 if (!t->skip_i0 && sv_tsf_face_active(o->mp, set, t->box_i0, j, k)) {
                     double vh = VH(t->box_i0 - 1, j, k), eps = get_epsilon(o, t->box_i0, j, k);
-                    if (j < t->box_jn) o->ey[IDX(o, t->box_i0, j, k)] += ecor / eps * sv_ghz(vh, th, ph, ps);
-                    if (k < t->box_kn) o->ez[IDX(o, t->box_i0, j, k)] -= ecor / eps * sv_ghy(vh, th, ph, ps);
+                    const double ge = ecor * tsf_profile(o, t->box_i0 - 1, j);
+                    if (j < t->box_jn) o->ey[IDX(o, t->box_i0, j, k)] += ge / eps * sv_ghz(vh, th, ph, ps);
+                    if (k < t->box_kn) o->ez[IDX(o, t->box_i0, j, k)] -= ge / eps * sv_ghy(vh, th, ph, ps);
                 }
                 if (!t->skip_in && sv_tsf_face_active(o->mp, set, t->box_in, j, k)) {
                     double vh = VH(t->box_in, j, k), eps = get_epsilon(o, t->box_in, j, k);
-                    if (j < t->box_jn) o->ey[IDX(o, t->box_in, j, k)] -= ecor / eps * sv_ghz(vh, th, ph, ps);
-                    if (k < t->box_kn) o->ez[IDX(o, t->box_in, j, k)] += ecor / eps * sv_ghy(vh, th, ph, ps);
+                    const double ge = ecor * tsf_profile(o, t->box_in, j);
+                    if (j < t->box_jn) o->ey[IDX(o, t->box_in, j, k)] -= ge / eps * sv_ghz(vh, th, ph, ps);
+                    if (k < t->box_kn) o->ez[IDX(o, t->box_in, j, k)] += ge / eps * sv_ghy(vh, th, ph, ps);
                 }
             }
         for (i = t->box_i0; i <= t->box_in && i < X; i++)
             for (k = t->box_k0; k <= t->box_kn; k++) {
                 if (!t->skip_j0 && sv_tsf_face_active(o->mp, set, i, t->box_j0, k)) {
                     double vh = VH(i, t->box_j0 - 1, k), eps = get_epsilon(o, i, t->box_j0, k);
-                    if (i < t->box_in) o->ex[IDX(o, i, t->box_j0, k)] -= ecor / eps * sv_ghz(vh, th, ph, ps);
-                    if (k < t->box_kn) o->ez[IDX(o, i, t->box_j0, k)] += ecor / eps * sv_ghx(vh, th, ph, ps);
+                    const double ge = ecor * tsf_profile(o, i, t->box_j0 - 1);
+                    if (i < t->box_in) o->ex[IDX(o, i, t->box_j0, k)] -= ge / eps * sv_ghz(vh, th, ph, ps);
+                    if (k < t->box_kn) o->ez[IDX(o, i, t->box_j0, k)] += ge / eps * sv_ghx(vh, th, ph, ps);
                 }
                 if (!t->skip_jn && sv_tsf_face_active(o->mp, set, i, t->box_jn, k)) {
                     double vh = VH(i, t->box_jn, k), eps = get_epsilon(o, i, t->box_jn, k);
-                    if (i < t->box_in) o->ex[IDX(o, i, t->box_jn, k)] += ecor / eps * sv_ghz(vh, th, ph, ps);
-                    if (k < t->box_kn) o->ez[IDX(o, i, t->box_jn, k)] -= ecor / eps * sv_ghx(vh, th, ph, ps);
+                    const double ge = ecor * tsf_profile(o, i, t->box_jn);
+                    if (i < t->box_in) o->ex[IDX(o, i, t->box_jn, k)] += ge / eps * sv_ghz(vh, th, ph, ps);
+                    if (k < t->box_kn) o->ez[IDX(o, i, t->box_jn, k)] -= ge / eps * sv_ghx(vh, th, ph, ps);
                 }
             }
         for (i = t->box_i0; i <= t->box_in && i < X; i++)
             for (j = t->box_j0; j <= t->box_jn && j < Y; j++) {
                 if (!t->skip_k0 && sv_tsf_face_active(o->mp, set, i, j, t->box_k0)) {
                     double vh = VH(i, j, t->box_k0 - 1), eps = get_epsilon(o, i, j, t->box_k0);
+                    const double ge = ecor * tsf_profile(o, i, j);
                     int pec = sv_pool_is_pec(o->mp, i, j, t->box_k0);
-                    if (i < t->box_in && !pec) o->ex[IDX(o, i, j, t->box_k0)] += ecor / eps * sv_ghy(vh, th, ph, ps);
-                    if (j < t->box_jn && !pec) o->ey[IDX(o, i, j, t->box_k0)] -= ecor / eps * sv_ghx(vh, th, ph, ps);
+                    if (i < t->box_in && !pec) o->ex[IDX(o, i, j, t->box_k0)] += ge / eps * sv_ghy(vh, th, ph, ps);
+                    if (j < t->box_jn && !pec) o->ey[IDX(o, i, j, t->box_k0)] -= ge / eps * sv_ghx(vh, th, ph, ps);
                 }
                 if (!t->skip_kn && sv_tsf_face_active(o->mp, set, i, j, t->box_kn)) {
                     double vh = VH(i, j, t->box_kn), eps = get_epsilon(o, i, j, t->box_kn);
-                    if (i < t->box_in) o->ex[IDX(o, i, j, t->box_kn)] -= ecor / eps * sv_ghy(vh, th, ph, ps);
-                    if (j < t->box_jn) o->ey[IDX(o, i, j, t->box_kn)] += ecor / eps * sv_ghx(vh, th, ph, ps);
+                    const double ge = ecor * tsf_profile(o, i, j);
+                    if (i < t->box_in) o->ex[IDX(o, i, j, t->box_kn)] -= ge / eps * sv_ghy(vh, th, ph, ps);
+                    if (j < t->box_jn) o->ey[IDX(o, i, j, t->box_kn)] += ge / eps * sv_ghx(vh, th, ph, ps);
                 }
             }
 #undef VH

@@ -101,3 +101,24 @@ def test_gpu_periodic_planes_against_oracle(tmp_path, kind):
         spread = np.abs(w - w[:, :1, :1, :]).max() / np.abs(w).max()
         print("GPU window spread", spread)
         assert spread <= 1e-6
+
+
+def test_mboundary_all_equals_the_six_keywords(tmp_path):
+    """MBOUNDARY_ALL periodic x0 xn y0 yn z0 zn (src3d/settings.c:5109-5128)
+    plans the same six interior periodic planes as the six MBOUNDARY_* keys"""
+    import os
+    par = cases.periodic_window(str(tmp_path / "a"), n=40, nsteps=60, blocks=(INSIDE,))
+    text = open(par).read()
+    six = "".join("MBOUNDARY_%s\nperiodic %d\n\n" % (f, p) for f, p in (("Z0", 6), ("ZN", 34)))
+    one = text
+    for f in ("X0", "XN", "Y0", "YN"):
+        one = one.replace("MBOUNDARY_%s\nperiodic %d\n\n" % (f, W0 if f.endswith("0") else W1), "")
+    one += "MBOUNDARY_ALL\nperiodic %d %d %d %d 6 34\n\n" % (W0, W1, W0, W1)
+    pa = os.path.join(os.path.dirname(par), "six.par")
+    pb = os.path.join(os.path.dirname(par), "all.par")
+    open(pa, "w").write(text + six)
+    open(pb, "w").write(one)
+    assert "MBOUNDARY_X0" not in one
+    fa, fb = run_oracle(pa), run_oracle(pb)
+    assert all(np.array_equal(x, y) for x, y in zip(fa, fb))
+    assert max(float(np.abs(x).max()) for x in fa) > 0
