@@ -441,7 +441,10 @@ def main():
 
     if args.batch_only:
         work = args.workdir or tempfile.mkdtemp(prefix="gsv_bench_")
-        print(json.dumps(batch_bench(args, work)), flush=True)
+        batch = batch_bench(args, work)
+        if args.batch_e2e:
+            batch["end_to_end"] = batch_e2e(work)
+        print(json.dumps(batch), flush=True)
         return
 
     import cases

@@ -508,6 +508,8 @@ sv_pool_output(SvPool *mp, SvSet *set)
         return;
     for (n = 0; n < set->sf.nrs; n++) {
         SvRPoint *r = mp->farfield->rpoints + n;
+        if (!set->sf.source_filename[n][0])
+            continue;
         fh = fopen(set->sf.source_filename[n], "w");
         if (!fh) {
             fprintf(stderr, "Error: farfield: cannot open file %s for writing\n", set->sf.source_filename[n]);

@@ -1695,7 +1695,7 @@ farfield_new(SvSet *set)
 {
     SvFarfield *ff = (SvFarfield *)calloc(1, sizeof(SvFarfield));
     double div = SV_LIGHT_SPEED * set->plan.dt / set->sp.dx;
-    int n;
+    int n, set_now = 0, set_istart = 0;
 
     ff->ndata = (int)(2000 + 2 * set->sc.nsteps
                       + sqrt((double)(set->sp.xres * set->sp.xres + set->sp.yres * set->sp.yres + set->sp.zres * set->sp.zres)) / div);
@@ -1705,7 +1705,17 @@ farfield_new(SvSet *set)
     for (n = 0; n < ff->nrs; n++) {
         double ri = set->sf.ri[n] / div, rj = set->sf.rj[n] / div, rk = set->sf.rk[n] / div;
         double v = sqrt(ri * ri + rj * rj + rk * rk) - 500;
+        const int ind = set->sf.individual ? set->sf.individual[n] : 0;
         ff->rpoints[n].istart = (int)(v > 0 ? v : 0);
+        /* the points of an area set share the first point's istart
+         * (src3d/farfield.c:81-91) */
+        if (ind > 0) {
+            if (ind != set_now) {
+                set_now = ind;
+                set_istart = ff->rpoints[n].istart;
+            }
+            ff->rpoints[n].istart = set_istart;
+        }
         ff->rpoints[n].ex = (double *)calloc(3 * (size_t)ff->ndata, sizeof(double));
         ff->rpoints[n].ey = ff->rpoints[n].ex + ff->ndata;
         ff->rpoints[n].ez = ff->rpoints[n].ey + ff->ndata;
@@ -1798,6 +1808,19 @@ init_and_make_plan(SvSet *set)
     }
     if (load_sources(mp, set))
         return NULL;
+
+    /* OUT_SUMTAB filters: the database material at the centre wavelength, as
+     * absolute permittivity and permeability (src3d/plan.c:5475-5486) */
+    for (n = 0; n < set->so.nsums; n++) {
+        SvOutputSum *o = set->so.sums + n;
+        if (!o->stringbased)
+            continue;
+        if (sv_database_linear(set, o->string, &o->layered_epsilon, &o->layered_mu, &o->layered_sigma, &o->layered_sigast))
+            return NULL;
+        o->layered_epsilon *= EPS0;
+        o->layered_mu *= MU0;
+        o->stringbased = 0;
+    }
 
     /* matmode, src3d/plan.c:5489-5518 */
     if (set->sm.matmode_check) {

@@ -7,9 +7,10 @@
  * (sub-gridding, local and focused sources) are
  * rejected with an error instead of being silently ignored, so a parameter
  * file never runs with part of its physics dropped.  Keywords whose only
- * product is a Gwyddion container (OUT_IMAGE, OUT_FILE, the far-field
- * spherical areas NFFF_SPHERICAL_AREA and PERIODIC_NFFF_SPHERICAL_AREA) are
- * consumed and ignored with a warning.
+ * product is a Gwyddion container (OUT_IMAGE, OUT_FILE) are consumed and
+ * ignored with a warning; the far-field areas (NFFF_SPHERICAL_AREA,
+ * NFFF_PLANAR_AREA, PERIODIC_NFFF_SPHERICAL_AREA) add their points and
+ * per-point logs, only the Gwyddion image of each set is not written.
  *
  * Generated sources (types 1 and 2, src3d/settings.c:444-548) are produced here
  * with the reference's waveform formulas and then rounded to 6 significant
@@ -312,6 +313,130 @@ skip_output_keyword(FILE *fr, const char *key)
     char buf[SV_NAME_LEN];
     fprintf(stderr, "Warning: %s output is not produced by this engine, ignored\n", key);
     return get_rest_of_line(fr, buf, sizeof(buf)) && 0;
+}
+
+/* room for `add` more far-field points; individual may be NULL (periodic set) */
+static int
+far_points_grow(int **ri, int **rj, int **rk, int **individual, char (**fn)[SV_NAME_LEN], int nrs, int add)
+{
+    int *a = (int *)realloc(*ri, (size_t)(nrs + add) * sizeof(int));
+    if (a) *ri = a;
+    int *b = (int *)realloc(*rj, (size_t)(nrs + add) * sizeof(int));
+    if (b) *rj = b;
+    int *c = (int *)realloc(*rk, (size_t)(nrs + add) * sizeof(int));
+    if (c) *rk = c;
+    char (*f)[SV_NAME_LEN] = (char (*)[SV_NAME_LEN])realloc(*fn, (size_t)(nrs + add) * SV_NAME_LEN);
+    if (f) *fn = f;
+    int ok = a && b && c && f;
+    if (individual) {
+        int *d = (int *)realloc(*individual, (size_t)(nrs + add) * sizeof(int));
+        if (d) *individual = d;
+        ok = ok && d;
+    }
+    if (!ok)
+        fprintf(stderr, "Error: cannot allocate %d far-field points\n", nrs + add);
+    return !ok;
+}
+
+/* NFFF_SPHERICAL_AREA / PERIODIC_NFFF_SPHERICAL_AREA thetares phires radius
+ * thetafrom phifrom thetato phito savefile (src3d/settings.c:5532-5620,
+ * 5621-5704): thetares x phires points at `radius` cells from the pool centre,
+ * theta and phi (radians) stepped evenly from `from` to `to`, cell indices
+ * truncated toward zero; logs xnff_set<set>_<i>_<j> when savefile != 0 */
+static int
+parse_spherical_area(FILE *fr, const char *key, const SvSet *set, int *nrs, int *nsets, int **ri, int **rj, int **rk,
+                     int **individual, char (**fn)[SV_NAME_LEN])
+{
+    int tres, pres, radius, savefile, i, j, n;
+    double tf, pf, tt, pt;
+    if (get_int(fr, &tres, key) || get_int(fr, &pres, key) || get_int(fr, &radius, key) || get_double(fr, &tf, key)
+        || get_double(fr, &pf, key) || get_double(fr, &tt, key) || get_double(fr, &pt, key) || get_int(fr, &savefile, key))
+        return 1;
+    if (tres < 1 || pres < 1) {
+        fprintf(stderr, "Error: %s needs at least one point per direction\n", key);
+        return 1;
+    }
+    if (far_points_grow(ri, rj, rk, individual, fn, *nrs, tres * pres))
+        return 1;
+    n = *nrs;
+    (*nsets)++;
+    fprintf(stderr, "Warning: %s set %d: its total-power image (a Gwyddion container) is not written, its points are\n",
+            key, *nsets);
+    for (i = 0; i < tres; i++)
+        for (j = 0; j < pres; j++) {
+            double theta = tres > 1 ? tf + (double)i * (tt - tf) / (tres - 1) : tf;
+            double phi = pres > 1 ? pf + (double)j * (pt - pf) / (pres - 1) : pf;
+            (*ri)[n] = (int)(radius * sin(theta) * cos(phi)) + set->sp.xres / 2;
+            (*rj)[n] = (int)(radius * sin(theta) * sin(phi)) + set->sp.yres / 2;
+            (*rk)[n] = (int)(radius * cos(theta)) + set->sp.zres / 2;
+            if (individual)
+                (*individual)[n] = *nsets;
+            if (savefile)
+                snprintf((*fn)[n], SV_NAME_LEN, "xnff_set%03d_%03d_%03d", *nsets, i, j);
+            else
+                (*fn)[n][0] = 0;
+            n++;
+        }
+    *nrs = n;
+    return 0;
+}
+
+/* NFFF_PLANAR_AREA ijres jkres ijfrom jkfrom ijto jkto orientation distance
+ * savefile (src3d/settings.c:5705-5816): an ijres x jkres grid of points on
+ * the plane normal to x (0), y (1) or z (2) at `distance` (m) from the pool
+ * centre, in-plane coordinates (m) stepped evenly from `from` to `to`, each
+ * divided by the cell size and truncated toward zero.  The reference divides
+ * by ijres - 1 and jkres - 1 unguarded; one-point rows are refused here. */
+static int
+parse_planar_area(FILE *fr, const char *key, SvSet *set)
+{
+    SvSetFarfield *f = &set->sf;
+    int ijres, jkres, orientation, savefile, i, j, n;
+    double ijfrom, jkfrom, ijto, jkto, distance;
+    if (get_int(fr, &ijres, key) || get_int(fr, &jkres, key) || get_double(fr, &ijfrom, key) || get_double(fr, &jkfrom, key)
+        || get_double(fr, &ijto, key) || get_double(fr, &jkto, key) || get_int(fr, &orientation, key)
+        || get_double(fr, &distance, key) || get_int(fr, &savefile, key))
+        return 1;
+    if (ijres < 2 || jkres < 2) {
+        fprintf(stderr, "Error: %s needs at least two points per direction\n", key);
+        return 1;
+    }
+    if (orientation < 0 || orientation > 2) {
+        fprintf(stderr, "Error: unsupported NFFF area orientation (%d)\n", orientation);
+        return 1;
+    }
+    if (far_points_grow(&f->ri, &f->rj, &f->rk, &f->individual, &f->source_filename, f->nrs, ijres * jkres))
+        return 1;
+    n = f->nrs;
+    f->nsets++;
+    fprintf(stderr, "Warning: %s set %d: its total-power image (a Gwyddion container) is not written, its points are\n",
+            key, f->nsets);
+    for (i = 0; i < ijres; i++)
+        for (j = 0; j < jkres; j++) {
+            const double u = ijfrom + (ijto - ijfrom) * ((double)i) / ((double)ijres - 1);
+            const double v = jkfrom + (jkto - jkfrom) * ((double)j) / ((double)jkres - 1);
+            if (orientation == 0) {
+                f->ri[n] = (int)(distance / set->sp.dx) + set->sp.xres / 2;
+                f->rj[n] = (int)(u / set->sp.dy) + set->sp.yres / 2;
+                f->rk[n] = (int)(v / set->sp.dz) + set->sp.zres / 2;
+            } else if (orientation == 1) {
+                f->ri[n] = (int)(u / set->sp.dx) + set->sp.xres / 2;
+                f->rj[n] = (int)(distance / set->sp.dy) + set->sp.yres / 2;
+                f->rk[n] = (int)(v / set->sp.dz) + set->sp.zres / 2;
+            } else {
+                f->ri[n] = (int)(u / set->sp.dx) + set->sp.xres / 2;
+                f->rj[n] = (int)(v / set->sp.dy) + set->sp.yres / 2;
+                f->rk[n] = (int)(distance / set->sp.dz) + set->sp.zres / 2;
+            }
+            f->individual[n] = f->nsets;
+            if (savefile)
+                snprintf(f->source_filename[n], SV_NAME_LEN, "xnff_set%03d_%03d_%03d", f->nsets, i, j);
+            else
+                f->source_filename[n][0] = 0;
+            n++;
+        }
+    f->nrs = n;
+    return 0;
 }
 
 int
@@ -639,8 +764,9 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
                 || get_int(fr, &o->box_kn, key) || get_rest_of_line(fr, o->filename, SV_NAME_LEN))
                 goto fail;
             set->so.nforces++;
-        } else if (!strcmp(key, "OUT_SUM")) {
-            /* src3d/settings.c:4937-4981: component step i0 j0 k0 in jn kn eps mu sigma sigast file */
+        } else if (!strcmp(key, "OUT_SUM") || !strcmp(key, "OUT_SUMTAB")) {
+            /* src3d/settings.c:4937-4981: component step i0 j0 k0 in jn kn eps mu sigma sigast file;
+             * OUT_SUMTAB (:4982-5017) names a database material instead of eps mu sigma sigast */
             SvOutputSum *o;
             char comp[16];
             set->so.sums = (SvOutputSum *)realloc(set->so.sums, (set->so.nsums + 1) * sizeof(SvOutputSum));
@@ -660,9 +786,14 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             }
             if (get_int(fr, &o->step, key) || get_int(fr, &o->box_i0, key) || get_int(fr, &o->box_j0, key)
                 || get_int(fr, &o->box_k0, key) || get_int(fr, &o->box_in, key) || get_int(fr, &o->box_jn, key)
-                || get_int(fr, &o->box_kn, key) || get_double(fr, &o->layered_epsilon, key)
-                || get_double(fr, &o->layered_mu, key) || get_double(fr, &o->layered_sigma, key)
-                || get_double(fr, &o->layered_sigast, key))
+                || get_int(fr, &o->box_kn, key))
+                goto fail;
+            if (!strcmp(key, "OUT_SUMTAB")) {
+                o->stringbased = 1;
+                if (get_token(fr, o->string, sizeof(o->string)))
+                    goto fail;
+            } else if (get_double(fr, &o->layered_epsilon, key) || get_double(fr, &o->layered_mu, key)
+                       || get_double(fr, &o->layered_sigma, key) || get_double(fr, &o->layered_sigast, key))
                 goto fail;
             if (get_rest_of_line(fr, o->filename, SV_NAME_LEN))
                 goto fail;
@@ -703,9 +834,20 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
             if (o->component >= SV_OVOLUME_ABS)
                 o->k = nabs;
             set->so.ncubs++;
-        } else if (!strcmp(key, "OUT_IMAGE") || !strcmp(key, "OUT_FILE") || !strcmp(key, "NFFF_SPHERICAL_AREA")
-                   || !strcmp(key, "PERIODIC_NFFF_SPHERICAL_AREA")) {
+        } else if (!strcmp(key, "OUT_IMAGE") || !strcmp(key, "OUT_FILE")) {
             skip_output_keyword(fr, key);
+        } else if (!strcmp(key, "NFFF_SPHERICAL_AREA")) {
+            SvSetFarfield *f = &set->sf;
+            if (parse_spherical_area(fr, key, set, &f->nrs, &f->nsets, &f->ri, &f->rj, &f->rk, &f->individual,
+                                     &f->source_filename))
+                goto fail;
+        } else if (!strcmp(key, "PERIODIC_NFFF_SPHERICAL_AREA")) {
+            SvSetPFarfield *f = &set->spf;
+            if (parse_spherical_area(fr, key, set, &f->nrs, &f->nsets, &f->ri, &f->rj, &f->rk, NULL, &f->source_filename))
+                goto fail;
+        } else if (!strcmp(key, "NFFF_PLANAR_AREA")) {
+            if (parse_planar_area(fr, key, set))
+                goto fail;
         } else if (!strcmp(key, "BOUNDARY_ALL")) {
             SvSetBoundary *b = &set->sb;
             if (parse_cpml_face(fr, key, set->sp.dx, &b->bx0, &b->depth_bx0, &b->m_bx0, &b->sigma_bx0, &b->a_bx0, &b->kappa_bx0))

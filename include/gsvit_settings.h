@@ -259,6 +259,11 @@ typedef struct {
     double layered_epsilon; /* absolute permittivity a voxel must match within EPSILON_0/100; -1 = every voxel */
     double layered_mu, layered_sigma, layered_sigast;
     char filename[SV_NAME_LEN];
+    /* OUT_SUMTAB (src3d/settings.c:4982-5017): the voxel filter is the
+     * optical-database material `string`, resolved at the centre wavelength
+     * when the pool is planned (src3d/plan.c:5475-5486) */
+    int stringbased;
+    char string[SV_NAME_LEN];
 } SvOutputSum;
 
 /* OUT_VOLUME components, src3d/settings.h:73-90 */
@@ -326,8 +331,9 @@ typedef struct {
     int skipkn_imin, skipkn_jmin, skipkn_imax, skipkn_jmax;
     int nrs;
     int *ri, *rj, *rk;
-    int *individual;
-    char (*source_filename)[SV_NAME_LEN];
+    int *individual;        /* 0: a single point; s > 0: point of area set s (its bins share one istart) */
+    char (*source_filename)[SV_NAME_LEN];  /* "" = no log file */
+    int nsets;              /* NFFF_SPHERICAL_AREA / NFFF_PLANAR_AREA sets, counted from 1 */
     /* running-DFT accumulation on the same box (no reference counterpart) */
     int ndft;
     double *dft_lambda;
@@ -400,7 +406,8 @@ typedef struct {
     int skipkn_imin, skipkn_jmin, skipkn_imax, skipkn_jmax;
     int nrs;
     int *ri, *rj, *rk;
-    char (*source_filename)[SV_NAME_LEN];
+    char (*source_filename)[SV_NAME_LEN];  /* "" = no log file */
+    int nsets;              /* PERIODIC_NFFF_SPHERICAL_AREA sets, counted from 1 */
 } SvSetPFarfield;
 
 typedef struct {

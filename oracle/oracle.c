@@ -2413,6 +2413,25 @@ oracle_get_farfield(void *handle, double *out)
         memcpy(out + (size_t)n * 3 * nd, o->mp->farfield->rpoints[n].ex, 3 * (size_t)nd * sizeof(double));
 }
 
+/* far-field points as planned: [nrs][4] = ri, rj, rk, istart (periodic != 0:
+ * the periodic far field's points); returns nrs */
+int
+oracle_farfield_points(void *handle, int periodic, int *out)
+{
+    Oracle *o = (Oracle *)handle;
+    const int nrs = periodic ? o->set.spf.nrs : o->set.sf.nrs;
+    int n;
+    if (!out)
+        return nrs;
+    for (n = 0; n < nrs; n++) {
+        out[4 * n + 0] = periodic ? o->set.spf.ri[n] : o->set.sf.ri[n];
+        out[4 * n + 1] = periodic ? o->set.spf.rj[n] : o->set.sf.rj[n];
+        out[4 * n + 2] = periodic ? o->set.spf.rk[n] : o->set.sf.rk[n];
+        out[4 * n + 3] = periodic ? o->mp->pfarfield->prpoints[n].istart : o->mp->farfield->rpoints[n].istart;
+    }
+    return nrs;
+}
+
 /* layered TSF: dims[0..1] = stored substeps (0 before the first step or
  * without LTSF), z cells; history as [6: ee eh ez he hh hz][nrows][nz] */
 void

@@ -59,6 +59,8 @@ def lib(precision="f64"):
         L.oracle_rc_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                        ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.oracle_set_option.restype = ctypes.c_int
+        L.oracle_farfield_points.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_farfield_points.restype = ctypes.c_int
         L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
         _libs[precision] = L
     return _libs[precision]
@@ -128,6 +130,13 @@ class Oracle:
         if inf["nrs"]:
             self.L.oracle_get_farfield(self.h, _dptr(out))
         return out[: inf["nrs"]]
+
+    def farfield_points(self, periodic=False):
+        """planned far-field points [nrs][4] = ri, rj, rk, istart"""
+        n = self.L.oracle_farfield_points(self.h, int(periodic), None)
+        out = np.zeros((max(n, 1), 4), dtype=np.int32)
+        self.L.oracle_farfield_points(self.h, int(periodic), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return out[:n]
 
     def forces(self, nforces):
         """OUT_FORCE values [nforces][nsteps + 1][3] (without the log's dx dy)"""
