@@ -80,6 +80,8 @@ struct Engine {
     /* far field */
     int has_ff;
     GsvFarfieldTD ff;
+    int has_tsff;                 /* focused TSF: per-wave lines, wave table, E face tables */
+    GsvTsff tsff;
     int has_ltsf;                 /* layered TSF: 1-D model history and face tables */
     GsvLtsf ltsf;
     int has_pff;                  /* periodic far field */

@@ -830,6 +830,98 @@ tsf_profile(const SvSet *set, int x, int y)
     return 1.0;
 }
 
+/* focused TSF (src3d/plan.c:1855-1979, src3d/source.c:773-930, 1892-2060):
+ * the waves' vacuum lines in one [nw][res] array, the per-wave distance and
+ * polarisation table, and dt/dx/eps per E face cell */
+static int
+setup_tsff(Engine *E)
+{
+    SvPool *mp = E->mp;
+    SvSet *set = E->set;
+    const SvSourceTSFF *f = mp->src->tsff;
+    GsvTsff &T = E->tsff;
+    memset(&T, 0, sizeof(T));
+    if (!f)
+        return 0;
+    E->has_tsff = 1;
+    const SvSourceTSF *w0 = f->w;
+    T.i0 = w0->box_i0; T.j0 = w0->box_j0; T.k0 = w0->box_k0;
+    T.in = w0->box_in; T.jn = w0->box_jn; T.kn = w0->box_kn;
+    T.skip[0] = w0->skip_i0; T.skip[1] = w0->skip_in; T.skip[2] = w0->skip_j0;
+    T.skip[3] = w0->skip_jn; T.skip[4] = w0->skip_k0; T.skip[5] = w0->skip_kn;
+    T.nw = f->nw;
+    T.res = w0->jp_res;
+    const double dt = set->plan.dt;
+    T.hcor = (float)(dt / set->sp.dx / MU0);
+    double *je, *jh, *jb, *wave;
+    if (dmalloc(E, &je, (size_t)T.nw * T.res) || dmalloc(E, &jh, (size_t)T.nw * T.res) || dmalloc(E, &jb, (size_t)T.nw * 8)
+        || dupload(E, &wave, w0->e, (size_t)w0->ndata))
+        return 1;
+    CK(hipMemsetAsync(je, 0, (size_t)T.nw * T.res * 8, E->st));
+    CK(hipMemsetAsync(jh, 0, (size_t)T.nw * T.res * 8, E->st));
+    CK(hipMemsetAsync(jb, 0, (size_t)T.nw * 64, E->st));
+    T.je = je;
+    T.jh = jh;
+    std::vector<GsvTsf> lines(T.nw);
+    std::vector<GsvTsffWave> waves(T.nw);
+    const double cx = set->sp.xres / 2 + f->xshift, cy = set->sp.yres / 2 + f->yshift, cz = set->sp.zres / 2 + f->zshift;
+    for (int w = 0; w < T.nw; w++) {
+        const SvSourceTSF *t = f->w + w;
+        GsvTsf &L = lines[w];
+        memset(&L, 0, sizeof(L));
+        L.res = T.res;
+        L.je = je + (size_t)w * T.res;
+        L.jh = jh + (size_t)w * T.res;
+        L.jb = jb + (size_t)w * 8;
+        L.wave = wave;
+        L.ndata = t->ndata;
+        L.multe = 1.0 / t->jp_epsilon * dt / (EPS0 * t->jp_dx);
+        L.multh = 1.0 / t->jp_mu * dt / (MU0 * t->jp_dx);
+        L.coef = (dt * SV_LIGHT_SPEED - t->jp_dx) / (dt * SV_LIGHT_SPEED + t->jp_dx);
+        const double th = t->ia_theta, ph = t->ia_phi, ps = t->ia_psi, a = f->acorr[w];
+        GsvTsffWave &W = waves[w];
+        W.corr = t->corr;
+        W.ax = sin(th) * cos(ph);
+        W.ay = sin(th) * sin(ph);
+        W.az = cos(th);
+        W.cx = cx; W.cy = cy; W.cz = cz;
+        W.pe[0] = a * (sin(ps) * sin(ph) + cos(ps) * cos(th) * cos(ph));
+        W.pe[1] = a * (-sin(ps) * cos(ph) + cos(ps) * cos(th) * sin(ph));
+        W.pe[2] = a * (-cos(ps) * sin(th));
+        W.ph[0] = a * (cos(ps) * sin(ph) - sin(ps) * cos(th) * cos(ph));
+        W.ph[1] = a * (-cos(ps) * cos(ph) - sin(ps) * cos(th) * sin(ph));
+        W.ph[2] = a * (sin(ps) * sin(th));
+    }
+    GsvTsf *dl;
+    GsvTsffWave *dw;
+    if (dupload(E, &dl, lines.data(), lines.size()) || dupload(E, &dw, waves.data(), waves.size()))
+        return 1;
+    T.lines = dl;
+    T.waves = dw;
+    const double ecor = dt / set->sp.dx;
+    const int lo[3] = { T.i0, T.j0, T.k0 }, hi[3] = { T.in, T.jn, T.kn };
+    for (int face = 0; face < 6; face++) {
+        const int ax = face / 2, A = (ax + 1) % 3, B = (ax + 2) % 3;
+        const int na = hi[A] - lo[A] + 1, nb = hi[B] - lo[B] + 1;
+        std::vector<float> ce((size_t)na * nb);
+        for (int b = 0; b < nb; b++)
+            for (int a = 0; a < na; a++) {
+                int c[3];
+                c[A] = lo[A] + a;
+                c[B] = lo[B] + b;
+                c[ax] = (face % 2) ? hi[ax] : lo[ax];
+                double eps, sig;
+                sv_pool_epsilon_sigma(mp, c[0], c[1], c[2], &eps, &sig);
+                ce[(size_t)a + (size_t)na * b] = (float)(ecor / eps);
+            }
+        float *d;
+        if (dupload(E, &d, ce.data(), ce.size()))
+            return 1;
+        T.coefE[face] = d;
+    }
+    return 0;
+}
+
 static int
 setup_tsf(Engine *E)
 {
@@ -1546,7 +1638,7 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
             CK(hipMemsetAsync(g.ade_p, 0, n * 4, E->st));
         }
     }
-    if (setup_cpml(E) || setup_tsf(E) || setup_sf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E) || setup_ltsf(E) || setup_forces(E)
+    if (setup_cpml(E) || setup_tsf(E) || setup_tsff(E) || setup_sf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E) || setup_ltsf(E) || setup_forces(E)
         || setup_flux(E) || setup_sums(E))
         return 1;
     for (int q = 0; q < 4; q++)
@@ -1749,6 +1841,10 @@ sv_pool_gpu_apply_source_hstep(SvPool *mp, SvSet *set)
     }
     if (E->has_ltsf)
         CK(gsv_launch_ltsf_faces(&E->g, &E->ltsf, 1, set->sc.step_act, E->st));
+    if (E->has_tsff) {
+        CK(gsv_launch_tsff_line(&E->tsff, set->sc.step_act, E->st));
+        CK(gsv_launch_tsff_faces(&E->g, &E->tsff, 1, E->st));
+    }
     return 0;
 }
 
@@ -1807,6 +1903,8 @@ sv_pool_gpu_apply_source_estep(SvPool *mp, SvSet *set)
         CK(gsv_launch_tsf_faces(&E->g, &E->tsf, 0, E->st));
     if (E->has_ltsf)
         CK(gsv_launch_ltsf_faces(&E->g, &E->ltsf, 0, set->sc.step_act, E->st));
+    if (E->has_tsff)
+        CK(gsv_launch_tsff_faces(&E->g, &E->tsff, 0, E->st));
     /* the scattered-field source's line, one step (two at step 0),
      * src3d/source.c:1319-1338 */
     if (E->has_sf) {

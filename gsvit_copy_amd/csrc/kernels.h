@@ -27,6 +27,27 @@ typedef struct {
     const float *coefH[6];
 } GsvTsf;
 
+/* focused TSF plane wave: line distance d(i,j,k) = corr (250 - (ax (cx - i) +
+ * ay (cy - j) + az (cz - k))) with c = pool centre + shift (srdcomp,
+ * src3d/source.c:173-194); polarisation factors times the wave's amplitude */
+typedef struct {
+    double corr, ax, ay, az, cx, cy, cz;
+    double pe[3];             /* acorr (ghx, ghy, ghz): E faces, from the H line */
+    double ph[3];             /* acorr (gex, gey, gez): H faces, from the E line */
+} GsvTsffWave;
+
+/* focused TSF: nw plane waves sharing one box; lines as [nw][res] */
+typedef struct {
+    int i0, j0, k0, in, jn, kn;
+    int skip[6];              /* x0, xn, y0, yn, z0, zn */
+    int nw, res;
+    const GsvTsffWave *waves;
+    const GsvTsf *lines;      /* per-wave line descriptors for the line step */
+    const double *je, *jh;
+    float hcor;               /* dt / dx / mu0 */
+    const float *coefE[6];    /* dt / dx / eps of the face cell */
+} GsvTsff;
+
 /* cells of a TSF face along its two in-plane axes; a box that spans a
  * periodic axis ends at the pool's last plane */
 static __host__ __device__ __forceinline__ void
@@ -173,6 +194,8 @@ extern "C" {
 hipError_t gsv_launch_hstep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st);
 hipError_t gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st);
 hipError_t gsv_launch_tsf_line(const GsvTsf *t, int step, hipStream_t st);
+hipError_t gsv_launch_tsff_line(const GsvTsff *t, int step, hipStream_t st);
+hipError_t gsv_launch_tsff_faces(const GsvGrid *g, const GsvTsff *t, int hside, hipStream_t st);
 /* scattered-field source, src3d/yee.c:1139-1237: the listed PEC-rule cells
  * (i, j, k, mask 1 Ex / 2 Ey / 4 Ez) take minus the incident field of line t */
 hipError_t gsv_launch_sf_pec(const GsvGrid *g, const GsvTsf *t, const int4 *cells, int n, hipStream_t st);

@@ -13,6 +13,8 @@
  * Surface kernels:
  *   k_tsf_line            1-D auxiliary line step, src3d/1dpool.c:83-162
  *   k_tsf_h / k_tsf_e     TSF face corrections, src3d/source.c:547-783, 1464-1657
+ *   k_tsff_line / k_tsff_face   focused TSF: one line per plane wave, face
+ *                         corrections summed over the waves, src3d/source.c:773-930, 1892-2060
  *   k_point_src           hard point sources, src3d/source.c:405-437, 1441-1462
  *   k_liao_store / k_liao_face   Liao ABC, src3d/boundary.c:549-689, 938-1066
  *   k_zero_planes         CPML near-plane zeroing after Liao on mixed boundaries
@@ -857,6 +859,68 @@ k_tsf_line(GsvTsf t, int step)
     tsf_line_step(t, step);
 }
 
+/* focused TSF: every plane wave's line in one launch, one workgroup each */
+__global__ __launch_bounds__(256) void
+k_tsff_line(const GsvTsf *__restrict__ lines, int step)
+{
+    const GsvTsf t = lines[blockIdx.x];
+    tsf_line_step(t, step);
+}
+
+/* focused TSF faces normal to AXIS (both sides): per face cell the plain TSF
+ * correction of every wave, summed in double and added once.  With A, B the
+ * in-plane axes in cyclic order after AXIS and p the wave's polarisation
+ * factors, F_A += s p_B v and F_B -= s p_A v, s = +1 on the near E face and
+ * the far H face, -1 on the others (src3d/source.c:800-915, 1921-2037); E
+ * faces read the H line one cell outside the near face, H faces the E line
+ * on the face plane, and write the H cells one outside the near face. */
+template <int AXIS>
+__global__ __launch_bounds__(256) void
+k_tsff_face(GsvGrid g, GsvTsff f, int hside)
+{
+    constexpr int A = (AXIS + 1) % 3, B = (AXIS + 2) % 3;
+    const int lo[3] = { f.i0, f.j0, f.k0 }, hi[3] = { f.in, f.jn, f.kn };
+    const int na = hi[A] - lo[A] + 1, nb = hi[B] - lo[B] + 1;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 2 * na * nb)
+        return;
+    const int side = q / (na * nb), r = q - side * na * nb, a = r % na, b = r / na;
+    const int face = 2 * AXIS + side;
+    if (f.skip[face])
+        return;
+    int c[3], e[3];
+    c[A] = e[A] = lo[A] + a;
+    c[B] = e[B] = lo[B] + b;
+    const int plane = side ? hi[AXIS] : lo[AXIS];
+    e[AXIS] = (!side && !hside) ? plane - 1 : plane;     /* where the line is read */
+    c[AXIS] = (!side && hside) ? plane - 1 : plane;      /* the cell corrected */
+    const double *line = hside ? f.je : f.jh;
+    double sa = 0.0, sb = 0.0;
+    for (int w = 0; w < f.nw; w++) {
+        const GsvTsffWave &W = f.waves[w];
+        const double d = W.corr * (250.0 - (W.ax * (W.cx - e[0]) + W.ay * (W.cy - e[1]) + W.az * (W.cz - e[2])));
+        const double v = line_val(line + (size_t)w * f.res, f.res, d);
+        const double *p = hside ? W.ph : W.pe;
+        sa += p[B] * v;
+        sb += p[A] * v;
+    }
+    const double s = ((side != 0) != (hside != 0)) ? -1.0 : 1.0;
+    const double coef = hside ? (double)f.hcor : (double)gp(f.coefE[face])[a + na * b];
+    const long long o = c[0] + (long long)g.px * c[1] + g.plane * c[2];
+    float *fa, *fb;
+    if (hside) {
+        fa = A == 0 ? g.hx : A == 1 ? g.hy : g.hz;
+        fb = B == 0 ? g.hx : B == 1 ? g.hy : g.hz;
+    } else {
+        fa = A == 0 ? g.ex : A == 1 ? g.ey : g.ez;
+        fb = B == 0 ? g.ex : B == 1 ? g.ey : g.ez;
+    }
+    if (c[A] < hi[A])
+        gp(fa)[o] += (float)(s * coef * sa);
+    if (c[B] < hi[B])
+        gp(fb)[o] -= (float)(s * coef * sb);
+}
+
 /* scattered-field source on the PEC rule's components: minus the incident
  * field read from the SF line at the cell's distance along the wave
  * (src3d/yee.c:1146-1230; every component of a cell reads the same sample) */
@@ -1673,6 +1737,23 @@ gsv_launch_tsf_line(const GsvTsf *t, int step, hipStream_t st)
 }
 
 extern "C" hipError_t
+gsv_launch_tsff_line(const GsvTsff *t, int step, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tsff_line, dim3(t->nw), dim3(256), 0, st, t->lines, step);
+    return hipGetLastError();
+}
+
+hipError_t
+gsv_launch_tsff_faces(const GsvGrid *g, const GsvTsff *t, int hside, hipStream_t st)
+{
+    const int nx = t->in - t->i0 + 1, ny = t->jn - t->j0 + 1, nz = t->kn - t->k0 + 1;
+    hipLaunchKernelGGL(k_tsff_face<0>, dim3((2 * ny * nz + 255) / 256), dim3(256), 0, st, *g, *t, hside);
+    hipLaunchKernelGGL(k_tsff_face<1>, dim3((2 * nx * nz + 255) / 256), dim3(256), 0, st, *g, *t, hside);
+    hipLaunchKernelGGL(k_tsff_face<2>, dim3((2 * nx * ny + 255) / 256), dim3(256), 0, st, *g, *t, hside);
+    return hipGetLastError();
+}
+
+hipError_t
 gsv_launch_tsf_faces(const GsvGrid *g, const GsvTsf *t, int hside, hipStream_t st)
 {
     for (int axis = 0; axis < 3; axis++) {

@@ -403,6 +403,106 @@ load_plane_waveform(const SvSet *set, const SvSetSourceTSF *t, const char *what,
     return 0;
 }
 
+/* Gauss-Legendre nodes (ascending) and weights on [-1, 1]: Newton's method on
+ * P_n from the three-term recurrence, started at cos(pi (i + 3/4) / (n + 1/2))
+ * (Abramowitz & Stegun 25.4.29; the reference's get_gl_coeffs,
+ * src3d/plan.c:1402-1427, is the same textbook method) */
+static void
+gauss_legendre(int n, double *x, double *w)
+{
+    int i, j, it;
+    for (i = 0; i < (n + 1) / 2; i++) {
+        double z = cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+        for (it = 0; it < 100; it++) {
+            double p0 = 1.0, p1 = 0.0, zn;
+            for (j = 1; j <= n; j++) {          /* p0 = P_j(z), p1 = P_{j-1}(z) */
+                const double pm = p1;
+                p1 = p0;
+                p0 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * pm) / j;
+            }
+            dp = n * (z * p0 - p1) / (z * z - 1.0);
+            zn = z - p0 / dp;
+            if (fabs(zn - z) <= 1e-15) {
+                z = zn;
+                break;
+            }
+            z = zn;
+        }
+        x[i] = -z;
+        x[n - 1 - i] = z;
+        w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+/* focused TSF, src3d/plan.c:1855-1979: Gauss-Legendre nodes in theta over
+ * [0, thetamax], midpoints in phi; psi = phi + pol */
+static int
+load_tsff(SvPool *mp, SvSet *set)
+{
+    const SvSetSourceTSFF *f = &set->ss.tsff;
+    const SvSetSourceTSF *t = &f->w;
+    const int nip = f->focused_nip, mip = f->focused_mip;
+    SvSourceTSFF *ff;
+    SvSourceTSF base;
+    double *xs, *ws;
+    int a, b;
+    if (nip < 1 || mip < 1 || nip * mip > 4096) {
+        fprintf(stderr, "Error: TSFF needs 1 .. 4096 plane waves (nip x mip = %d x %d)\n", nip, mip);
+        return 1;
+    }
+    if (t->box_i0 < 1 || t->box_j0 < 1 || t->box_k0 < 1 || t->box_in >= set->sp.xres || t->box_jn >= set->sp.yres
+        || t->box_kn >= set->sp.zres || t->box_in <= t->box_i0 || t->box_jn <= t->box_j0 || t->box_kn <= t->box_k0) {
+        fprintf(stderr, "Error: TSFF box must lie inside the pool with one cell to spare\n");
+        return 1;
+    }
+    if (!(f->focused_thetamax > 0 && f->focused_thetamax <= M_PI / 2)) {
+        /* the amplitude takes sqrt(cos theta) */
+        fprintf(stderr, "Error: TSFF aperture angle must lie in (0, pi/2]\n");
+        return 1;
+    }
+    if (set->sb.bx0 == SV_BOUNDARY_PERIODIC || set->sb.by0 == SV_BOUNDARY_PERIODIC) {
+        fprintf(stderr, "Error: TSFF in a periodic pool is not supported\n");
+        return 1;
+    }
+    memset(&base, 0, sizeof(base));
+    base.box_i0 = t->box_i0; base.box_j0 = t->box_j0; base.box_k0 = t->box_k0;
+    base.box_in = t->box_in; base.box_jn = t->box_jn; base.box_kn = t->box_kn;
+    base.skip_i0 = t->box_boundary_skipi0; base.skip_j0 = t->box_boundary_skipj0; base.skip_k0 = t->box_boundary_skipk0;
+    base.skip_in = t->box_boundary_skipin; base.skip_jn = t->box_boundary_skipjn; base.skip_kn = t->box_boundary_skipkn;
+    if (load_plane_waveform(set, t, "TSFF", &base))
+        return 1;
+    base.jp_res = (int)(sqrt((double)(set->sp.xres * set->sp.xres + set->sp.yres * set->sp.yres
+                                      + set->sp.zres * set->sp.zres)) + 250);
+    base.jp_epsilon = base.jp_mu = 1.0;
+    ff = (SvSourceTSFF *)calloc(1, sizeof(SvSourceTSFF));
+    mp->src->tsff = ff;
+    ff->nw = nip * mip;
+    ff->w = (SvSourceTSF *)calloc(ff->nw, sizeof(SvSourceTSF));
+    ff->acorr = (double *)calloc(ff->nw, sizeof(double));
+    ff->xshift = f->xshift; ff->yshift = f->yshift; ff->zshift = f->zshift;
+    xs = (double *)malloc(nip * sizeof(double));
+    ws = (double *)malloc(nip * sizeof(double));
+    gauss_legendre(nip, xs, ws);
+    for (a = 0; a < nip; a++)
+        for (b = 0; b < mip; b++) {
+            SvSourceTSF *w = ff->w + a * mip + b;
+            const double an = ws[a] * f->focused_thetamax / 2, bm = 2.0 * M_PI / mip;
+            *w = base;
+            w->ia_theta = (xs[a] + 1) * f->focused_thetamax / 2.0;
+            w->ia_phi = (b + 0.5) * 2 * M_PI / mip;
+            w->ia_psi = w->ia_phi + f->focused_pol;
+            w->corr = 1.0 / sqrt(pow(sin(w->ia_theta), 4) * (pow(cos(w->ia_phi), 4) + pow(sin(w->ia_phi), 4))
+                                 + pow(cos(w->ia_theta), 4));
+            w->jp_dx = set->sp.dx / w->corr;
+            /* fs_amplitude, src3d/source.c:167-171, with freal = fdist dx */
+            ff->acorr[a * mip + b] = 1e15 * an * bm * (f->focused_fdist * set->sp.dx / (2 * M_PI * SV_LIGHT_SPEED))
+                                     * sqrt(cos(w->ia_theta)) * sin(w->ia_theta);
+        }
+    free(xs);
+    free(ws);
+    return 0;
+}
+
 static int
 load_sources(SvPool *mp, SvSet *set)
 {
@@ -511,6 +611,8 @@ load_sources(SvPool *mp, SvSet *set)
                                 "above 0.75 (DT_MULT <= %.4g keeps it stable)\n", s, set->sc.dtmult * 0.75 / s);
         }
     }
+    if (set->ss.tsff.w.valid && load_tsff(mp, set))
+        return 1;
     if (set->ss.sf.valid) {
         /* src3d/plan.c:1985-2040: the waveform and a 1-D line of
          * (int) sqrt(xres^2 + yres^2 + zres^2 + 10) cells of dx, hard source at
@@ -2046,6 +2148,13 @@ sv_pool_free(SvPool *mp, SvSet *set)
         if (mp->src->tsf) {
             free(mp->src->tsf->e);
             free(mp->src->tsf);
+        }
+        if (mp->src->tsff) {
+            if (mp->src->tsff->nw)
+                free(mp->src->tsff->w[0].e);
+            free(mp->src->tsff->w);
+            free(mp->src->tsff->acorr);
+            free(mp->src->tsff);
         }
         if (mp->src->sf) {
             free(mp->src->sf->e);

@@ -568,6 +568,52 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
                 goto fail;
             }
             t->valid = 1;
+        } else if (!strcmp(key, "SOURCE_TSFF")) {
+            SvSetSourceTSFF *f = &set->ss.tsff;
+            SvSetSourceTSF *t = &f->w;
+            int srctype;
+            if (get_int(fr, &t->box_i0, key) || get_int(fr, &t->box_j0, key) || get_int(fr, &t->box_k0, key)
+                || get_int(fr, &t->box_in, key) || get_int(fr, &t->box_jn, key) || get_int(fr, &t->box_kn, key)
+                || get_double(fr, &f->focused_thetamax, key) || get_double(fr, &f->focused_fdist, key)
+                || get_double(fr, &f->focused_pol, key) || get_int(fr, &f->focused_nip, key)
+                || get_int(fr, &f->focused_mip, key) || get_int(fr, &srctype, key))
+                goto fail;
+            t->source_mode = srctype;
+            if (srctype == 0) {
+                if (get_rest_of_line(fr, t->source_filename, SV_NAME_LEN))
+                    goto fail;
+            } else if (srctype == 1 || srctype == 2) {
+                if (get_double(fr, &t->source_wl, key))
+                    goto fail;
+                if (srctype == 2 && get_int(fr, &t->source_pulsewidth, key))
+                    goto fail;
+                if (get_double(fr, &t->source_amplitude, key))
+                    goto fail;
+            } else {
+                fprintf(stderr, "Error: TSFF source type %d is not supported\n", srctype);
+                goto fail;
+            }
+            t->valid = 1;
+        } else if (!strcmp(key, "TSFF_SHIFT")) {
+            SvSetSourceTSFF *f = &set->ss.tsff;
+            if (get_double(fr, &f->xshift, key) || get_double(fr, &f->yshift, key) || get_double(fr, &f->zshift, key))
+                goto fail;
+        } else if (!strcmp(key, "TSFF_SKIP")) {
+            SvSetSourceTSF *t = &set->ss.tsff.w;
+            if (get_token(fr, value, sizeof(value)))
+                goto fail;
+            if (!strcmp(value, "i0")) t->box_boundary_skipi0 = 1;
+            else if (!strcmp(value, "in")) t->box_boundary_skipin = 1;
+            else if (!strcmp(value, "j0")) t->box_boundary_skipj0 = 1;
+            else if (!strcmp(value, "jn")) t->box_boundary_skipjn = 1;
+            else if (!strcmp(value, "k0")) t->box_boundary_skipk0 = 1;
+            else if (!strcmp(value, "kn")) t->box_boundary_skipkn = 1;
+            else if (!strcmp(value, "depth")) {
+                /* the focused faces never read the skip depth (src3d/source.c:773-930, 1892-2060) */
+                int depth;
+                if (get_int(fr, &depth, key))
+                    goto fail;
+            }
         } else if (!strcmp(key, "SOURCE_SF")) {
             /* src3d/settings.c:4698-4760: theta phi psi, then a waveform file (0)
              * or generated-waveform parameters (1: wavelength amplitude, 2:

@@ -90,10 +90,23 @@ typedef struct {
     double *zeps, *zmu;         /* relative material of the 1-D model cells */
 } SvSourceLTSF;
 
+/* focused TSF (src3d/source.h SvSourceTSFF, src3d/plan.c:1855-1979): nw plane
+ * waves, each an SvSourceTSF with its own angles, corr and vacuum 1-D line of
+ * (int) sqrt(xres^2 + yres^2 + zres^2) + 250 cells (sharing one waveform),
+ * amplitude acorr (fs_amplitude, src3d/source.c:167-171) and distance
+ * corr (250 - a . (centre + shift - r)) (srdcomp, src3d/source.c:173-194) */
+typedef struct {
+    int nw;
+    SvSourceTSF *w;
+    double *acorr;
+    double xshift, yshift, zshift;
+} SvSourceTSFF;
+
 typedef struct {
     int npnts;
     SvSourcePoint *sp;
     SvSourceTSF *tsf;
+    SvSourceTSFF *tsff;
     SvSourceLTSF *ltsf;
     /* scattered-field source (src3d/plan.c:1985-2040): waveform and 1-D line
      * of (int) sqrt(xres^2 + yres^2 + zres^2 + 10) cells of dx; corr = 1 and

@@ -222,10 +222,22 @@ typedef struct {
     double fiber_fxpos, fiber_fypos, fiber_radius, fiber_cutoff, fiber_epsilon_core, fiber_epsilon_cladding;
 } SvSetSourceLTSF;
 
+/* focused TSF, src3d/settings.c:4605-4697 (SOURCE_TSFF i0 j0 k0 in jn kn
+ * thetamax fdist pol nip mip mode file-or-parameters, TSFF_SHIFT, TSFF_SKIP):
+ * nip x mip plane waves over the aperture cone theta <= thetamax, their phase
+ * origin at the pool centre plus the shift (cells) */
+typedef struct {
+    SvSetSourceTSF w;         /* box, skipped faces and waveform; the angles are unused */
+    double focused_thetamax, focused_fdist, focused_pol;
+    int focused_nip, focused_mip;
+    double xshift, yshift, zshift;
+} SvSetSourceTSFF;
+
 typedef struct {
     int npnts;
     SvSetSourcePoint *pnts;
     SvSetSourceTSF tsf;
+    SvSetSourceTSFF tsff;
     /* scattered-field source, src3d/settings.c:4698-4760 (SOURCE_SF theta phi
      * psi mode file-or-parameters); the box fields are unused */
     SvSetSourceTSF sf;

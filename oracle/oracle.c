@@ -121,6 +121,8 @@ typedef struct {
     int jres;
     real *je, *jh;
     real jb[8];
+    /* focused TSF: one line per plane wave, [nw][res] fields and [nw][8] boundary scalars */
+    real *fe, *fh, *fb;
     double *points;                   /* [npts][nsteps+1][6] */
     int no_average;                   /* test option: E coefficients without interface averaging */
     int mb_fault;                     /* test option: a planted mistake in the interior periodic copies
@@ -247,6 +249,12 @@ oracle_open(const char *parfile)
         o->je = (real *)zalloc(o->jres, sizeof(real));
         o->jh = (real *)zalloc(o->jres, sizeof(real));
     }
+    if (o->mp->src->tsff) {
+        const SvSourceTSFF *f = o->mp->src->tsff;
+        o->fe = (real *)zalloc((size_t)f->nw * f->w[0].jp_res, sizeof(real));
+        o->fh = (real *)zalloc((size_t)f->nw * f->w[0].jp_res, sizeof(real));
+        o->fb = (real *)zalloc((size_t)f->nw * 8, sizeof(real));
+    }
     if (o->mp->src->sf) {
         o->sres = o->mp->src->sf->jp_res;
         o->se = (real *)zalloc(o->sres, sizeof(real));
@@ -289,7 +297,7 @@ oracle_close(void *handle)
             free(o->rci[n][q]);
         }
     }
-    free(o->je); free(o->jh); free(o->points); free(o->objacc); free(o->pfstore); free(o->lz);
+    free(o->je); free(o->jh); free(o->fe); free(o->fh); free(o->fb); free(o->points); free(o->objacc); free(o->pfstore); free(o->lz);
     sv_pool_free(o->mp, &o->set);
     clear_settings(&o->set, 1);
     free(o);
@@ -1450,6 +1458,142 @@ sf_estep(Oracle *o)
         line_step(o, t, o->se, o->sh, o->sb, o->sres);
 }
 
+/* focused TSF distance along wave w, srdcomp (src3d/source.c:173-194):
+ * 250 - a . (pool centre + shift - r), in cells of the line */
+static double
+tsff_d(const Oracle *o, const SvSourceTSF *w, int i, int j, int k)
+{
+    const SvSourceTSFF *f = o->mp->src->tsff;
+    const double st = sin(w->ia_theta), ct = cos(w->ia_theta), sp = sin(w->ia_phi), cp = cos(w->ia_phi);
+    const double rx = -i + o->set.sp.xres / 2 + f->xshift, ry = -j + o->set.sp.yres / 2 + f->yshift,
+                 rz = -k + o->set.sp.zres / 2 + f->zshift;
+    return w->corr * (250 - (st * cp * rx + st * sp * ry + ct * rz));
+}
+
+/* focused TSF, H side (src3d/source.c:1892-2060): per plane wave, its line
+ * step, then the H face corrections of a plain TSF wave scaled by acorr, read
+ * at the srdcomp distance, with no beam profile, skip depth or PEC rule */
+static void
+tsff_hstep(Oracle *o)
+{
+    const SvSourceTSFF *f = o->mp->src->tsff;
+    const double hcor = o->set.plan.dt / o->set.sp.dx / MU0;
+    int w, i, j, k;
+    if (!f)
+        return;
+    for (w = 0; w < f->nw; w++) {
+        const SvSourceTSF *t = f->w + w;
+        const int res = t->jp_res;
+        real *je = o->fe + (size_t)w * res, *jh = o->fh + (size_t)w * res;
+        const double th = t->ia_theta, ph = t->ia_phi, ps = t->ia_psi, a = f->acorr[w];
+        const double cez = sin(ps) * sin(th), cey = -cos(ps) * cos(ph) - sin(ps) * cos(th) * sin(ph),
+                     cex = cos(ps) * sin(ph) - sin(ps) * cos(th) * cos(ph);
+        line_step(o, t, je, jh, o->fb + 8 * w, res);
+#define VE(ii, jj, kk) (a * dline_val(je, res, tsff_d(o, t, ii, jj, kk)))
+        for (j = t->box_j0; j <= t->box_jn; j++)
+            for (k = t->box_k0; k <= t->box_kn; k++) {
+                if (!t->skip_i0) {
+                    const double ve = VE(t->box_i0, j, k);
+                    if (k < t->box_kn) o->hy[IDX(o, t->box_i0 - 1, j, k)] -= hcor * ve * cez;
+                    if (j < t->box_jn) o->hz[IDX(o, t->box_i0 - 1, j, k)] += hcor * ve * cey;
+                }
+                if (!t->skip_in) {
+                    const double ve = VE(t->box_in, j, k);
+                    if (k < t->box_kn) o->hy[IDX(o, t->box_in, j, k)] += hcor * ve * cez;
+                    if (j < t->box_jn) o->hz[IDX(o, t->box_in, j, k)] -= hcor * ve * cey;
+                }
+            }
+        for (i = t->box_i0; i <= t->box_in; i++)
+            for (k = t->box_k0; k <= t->box_kn; k++) {
+                if (!t->skip_j0) {
+                    const double ve = VE(i, t->box_j0, k);
+                    if (k < t->box_kn) o->hx[IDX(o, i, t->box_j0 - 1, k)] += hcor * ve * cez;
+                    if (i < t->box_in) o->hz[IDX(o, i, t->box_j0 - 1, k)] -= hcor * ve * cex;
+                }
+                if (!t->skip_jn) {
+                    const double ve = VE(i, t->box_jn, k);
+                    if (k < t->box_kn) o->hx[IDX(o, i, t->box_jn, k)] -= hcor * ve * cez;
+                    if (i < t->box_in) o->hz[IDX(o, i, t->box_jn, k)] += hcor * ve * cex;
+                }
+            }
+        for (i = t->box_i0; i <= t->box_in; i++)
+            for (j = t->box_j0; j <= t->box_jn; j++) {
+                if (!t->skip_k0) {
+                    const double ve = VE(i, j, t->box_k0);
+                    if (j < t->box_jn) o->hx[IDX(o, i, j, t->box_k0 - 1)] -= hcor * ve * cey;
+                    if (i < t->box_in) o->hy[IDX(o, i, j, t->box_k0 - 1)] += hcor * ve * cex;
+                }
+                if (!t->skip_kn) {
+                    const double ve = VE(i, j, t->box_kn);
+                    if (j < t->box_jn) o->hx[IDX(o, i, j, t->box_kn)] += hcor * ve * cey;
+                    if (i < t->box_in) o->hy[IDX(o, i, j, t->box_kn)] -= hcor * ve * cex;
+                }
+            }
+#undef VE
+    }
+}
+
+/* focused TSF, E side (src3d/source.c:773-930): the E face corrections of each
+ * plane wave from its line's H, over the local permittivity of the face cell */
+static void
+tsff_estep(Oracle *o)
+{
+    const SvSourceTSFF *f = o->mp->src->tsff;
+    const double ecor = o->set.plan.dt / o->set.sp.dx;
+    int w, i, j, k;
+    if (!f)
+        return;
+    for (w = 0; w < f->nw; w++) {
+        const SvSourceTSF *t = f->w + w;
+        const int res = t->jp_res;
+        const real *jh = o->fh + (size_t)w * res;
+        const double th = t->ia_theta, ph = t->ia_phi, ps = t->ia_psi, a = f->acorr[w];
+        const double chz = -cos(ps) * sin(th), chy = -sin(ps) * cos(ph) + cos(ps) * cos(th) * sin(ph),
+                     chx = sin(ps) * sin(ph) + cos(ps) * cos(th) * cos(ph);
+#define VH(ii, jj, kk) (a * dline_val(jh, res, tsff_d(o, t, ii, jj, kk)))
+        for (j = t->box_j0; j <= t->box_jn; j++)
+            for (k = t->box_k0; k <= t->box_kn; k++) {
+                if (!t->skip_i0) {
+                    const double vh = VH(t->box_i0 - 1, j, k), c = ecor / get_epsilon(o, t->box_i0, j, k);
+                    if (j < t->box_jn) o->ey[IDX(o, t->box_i0, j, k)] += c * vh * chz;
+                    if (k < t->box_kn) o->ez[IDX(o, t->box_i0, j, k)] -= c * vh * chy;
+                }
+                if (!t->skip_in) {
+                    const double vh = VH(t->box_in, j, k), c = ecor / get_epsilon(o, t->box_in, j, k);
+                    if (j < t->box_jn) o->ey[IDX(o, t->box_in, j, k)] -= c * vh * chz;
+                    if (k < t->box_kn) o->ez[IDX(o, t->box_in, j, k)] += c * vh * chy;
+                }
+            }
+        for (i = t->box_i0; i <= t->box_in; i++)
+            for (k = t->box_k0; k <= t->box_kn; k++) {
+                if (!t->skip_j0) {
+                    const double vh = VH(i, t->box_j0 - 1, k), c = ecor / get_epsilon(o, i, t->box_j0, k);
+                    if (i < t->box_in) o->ex[IDX(o, i, t->box_j0, k)] -= c * vh * chz;
+                    if (k < t->box_kn) o->ez[IDX(o, i, t->box_j0, k)] += c * vh * chx;
+                }
+                if (!t->skip_jn) {
+                    const double vh = VH(i, t->box_jn, k), c = ecor / get_epsilon(o, i, t->box_jn, k);
+                    if (i < t->box_in) o->ex[IDX(o, i, t->box_jn, k)] += c * vh * chz;
+                    if (k < t->box_kn) o->ez[IDX(o, i, t->box_jn, k)] -= c * vh * chx;
+                }
+            }
+        for (i = t->box_i0; i <= t->box_in; i++)
+            for (j = t->box_j0; j <= t->box_jn; j++) {
+                if (!t->skip_k0) {
+                    const double vh = VH(i, j, t->box_k0 - 1), c = ecor / get_epsilon(o, i, j, t->box_k0);
+                    if (i < t->box_in) o->ex[IDX(o, i, j, t->box_k0)] += c * vh * chy;
+                    if (j < t->box_jn) o->ey[IDX(o, i, j, t->box_k0)] -= c * vh * chx;
+                }
+                if (!t->skip_kn) {
+                    const double vh = VH(i, j, t->box_kn), c = ecor / get_epsilon(o, i, j, t->box_kn);
+                    if (i < t->box_in) o->ex[IDX(o, i, j, t->box_kn)] -= c * vh * chy;
+                    if (j < t->box_jn) o->ey[IDX(o, i, j, t->box_kn)] += c * vh * chx;
+                }
+            }
+#undef VH
+    }
+}
+
 /* TSF beam profile at lateral position (x, y), src3d/source.c:231-251: the
  * faces take the Gaussian if it is set, else the fiber mode (:558-742, 1482-1649) */
 static double
@@ -2348,11 +2492,13 @@ oracle_step(void *handle, int nsteps)
         ystep_h(o);
         source_hstep(o);
         ltsf_hstep(o);
+        tsff_hstep(o);
         boundary_hstep(o);
         ystep_e(o);
         boundary_estep(o);
         source_estep(o);
         ltsf_estep(o);
+        tsff_estep(o);
         sf_estep(o);
         getsum(o);
         farfield(o);
@@ -2411,6 +2557,26 @@ oracle_get_farfield(void *handle, double *out)
     int n, nd = o->mp->farfield->ndata;
     for (n = 0; n < o->set.sf.nrs; n++)
         memcpy(out + (size_t)n * 3 * nd, o->mp->farfield->rpoints[n].ex, 3 * (size_t)nd * sizeof(double));
+}
+
+/* focused TSF plane waves as planned: [nw][5] = theta, phi, psi, corr, acorr;
+ * returns nw (0 without TSFF) */
+int
+oracle_tsff_waves(void *handle, double *out)
+{
+    Oracle *o = (Oracle *)handle;
+    const SvSourceTSFF *f = o->mp->src->tsff;
+    int w;
+    if (!f)
+        return 0;
+    for (w = 0; out && w < f->nw; w++) {
+        out[5 * w + 0] = f->w[w].ia_theta;
+        out[5 * w + 1] = f->w[w].ia_phi;
+        out[5 * w + 2] = f->w[w].ia_psi;
+        out[5 * w + 3] = f->w[w].corr;
+        out[5 * w + 4] = f->acorr[w];
+    }
+    return f->nw;
 }
 
 /* far-field points as planned: [nrs][4] = ri, rj, rk, istart (periodic != 0:

@@ -59,6 +59,8 @@ def lib(precision="f64"):
         L.oracle_rc_scalar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                        ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.oracle_set_option.restype = ctypes.c_int
+        L.oracle_tsff_waves.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_tsff_waves.restype = ctypes.c_int
         L.oracle_farfield_points.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.oracle_farfield_points.restype = ctypes.c_int
         L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
@@ -130,6 +132,13 @@ class Oracle:
         if inf["nrs"]:
             self.L.oracle_get_farfield(self.h, _dptr(out))
         return out[: inf["nrs"]]
+
+    def tsff_waves(self):
+        """focused-TSF plane waves [nw][5] = theta, phi, psi, corr, acorr"""
+        n = self.L.oracle_tsff_waves(self.h, None)
+        out = np.zeros((max(n, 1), 5), dtype=np.float64)
+        self.L.oracle_tsff_waves(self.h, _dptr(out))
+        return out[:n]
 
     def farfield_points(self, periodic=False):
         """planned far-field points [nrs][4] = ri, rj, rk, istart"""
