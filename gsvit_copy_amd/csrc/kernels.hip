@@ -915,9 +915,13 @@ k_tsff_face(GsvGrid g, GsvTsff f, int hside)
         fa = A == 0 ? g.ex : A == 1 ? g.ey : g.ez;
         fb = B == 0 ? g.ex : B == 1 ? g.ey : g.ez;
     }
-    if (c[A] < hi[A])
+    /* an E component exists below the box end along its own axis, an H
+     * component below the box end along the other in-plane axis */
+    const bool wa = hside ? c[B] < hi[B] : c[A] < hi[A];
+    const bool wb = hside ? c[A] < hi[A] : c[B] < hi[B];
+    if (wa)
         gp(fa)[o] += (float)(s * coef * sa);
-    if (c[B] < hi[B])
+    if (wb)
         gp(fb)[o] -= (float)(s * coef * sb);
 }
 
