@@ -215,7 +215,7 @@ int sv_pool_gpu_boundary_copy(SvPool *mp, SvSet *set);
 int sv_pool_gpu_boundary_hstep(SvPool *mp, SvSet *set);
 /* src3d/pool.h:109 -- no-op: the E boundary work runs inside ystep_e */
 int sv_pool_gpu_boundary_estep(SvPool *mp, SvSet *set);
-/* src3d/pool.h:99 -- point H sources, 1-D line step, TSF H faces */
+/* src3d/pool.h:99 -- point H sources, 1-D line steps, TSF, layered and focused TSF H faces */
 int sv_pool_gpu_apply_source_hstep(SvPool *mp, SvSet *set);
 /* src3d/pool.h:100 -- point E sources, TSF E faces */
 int sv_pool_gpu_apply_source_estep(SvPool *mp, SvSet *set);

@@ -34,6 +34,12 @@
  *   interior periodic copies  src3d/boundary.c:304-346, 889-932
  *   point sources ........... src3d/source.c:405-437, 1441-1462
  *   TSF faces ............... src3d/source.c:547-783 (E), 1464-1657 (H)
+ *   TSF beam profiles ....... src3d/source.c:231-251 (pinned face by face against
+ *                             the closed forms, tests/test_tsf_profile.py)
+ *   focused TSF ............. src3d/source.c:167-194, 773-930 (E), 1892-2060 (H),
+ *                             src3d/plan.c:1855-1979 (pinned against numpy's
+ *                             Gauss-Legendre rule, the plain TSF for one wave and
+ *                             the focus position, tests/test_tsff.py)
  *   1-D auxiliary line ...... src3d/1dpool.c:83-162
  *   layered TSF ............. src3d/zpool.c:38-338, src3d/source.c:286-322,
  *                             921-1150 (E faces), 1680-1900 (H faces)
