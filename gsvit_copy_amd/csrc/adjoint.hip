@@ -706,7 +706,7 @@ gsv_engine_adjoint_setup(void *eng, long long history_budget_bytes, int ckpt_int
         fprintf(stderr, "Error: the gradient does not support force boxes or the periodic far field\n");
         return 1;
     }
-    if (E->has_tsf || E->has_ltsf || E->has_tsff || E->has_sf || E->has_liao || E->has_per || g.perx || g.pery) {
+    if (E->has_tsf || E->has_ltsf || E->has_tsff || E->has_ext || E->has_sf || E->has_liao || E->has_per || g.perx || g.pery) {
         fprintf(stderr, "Error: the gradient supports CPML and point sources only\n");
         return 1;
     }

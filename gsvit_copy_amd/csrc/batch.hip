@@ -125,6 +125,7 @@ batch_reject(const Engine *E)
     if (E->has_pff) return "periodic far field";
     if (E->has_ltsf) return "layered TSF";
     if (E->has_tsff) return "focused TSF";
+    if (E->has_ext) return "external plane source";
     if (E->has_sf) return "scattered-field source";
     if (E->has_dft) return "running DFT";
     if (E->has_flux) return "flux spectra";

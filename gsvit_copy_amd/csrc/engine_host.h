@@ -80,6 +80,10 @@ struct Engine {
     /* far field */
     int has_ff;
     GsvFarfieldTD ff;
+    int has_ext;                  /* external plane source: window buffers, host staging */
+    GsvExt ext;
+    float *ext_px, *ext_py;
+    std::vector<float> ext_planes, ext_win;
     int has_tsff;                 /* focused TSF: per-wave lines, wave table, E face tables */
     GsvTsff tsff;
     int has_ltsf;                 /* layered TSF: 1-D model history and face tables */

@@ -922,6 +922,66 @@ setup_tsff(Engine *E)
     return 0;
 }
 
+/* external plane source: window geometry, dt/dx/eps per window cell */
+static int
+setup_ext(Engine *E)
+{
+    SvSet *set = E->set;
+    GsvExt &X = E->ext;
+    memset(&X, 0, sizeof(X));
+    if (!set->ss.ext.valid)
+        return 0;
+    E->has_ext = 1;
+    X.k = set->ss.ext.k;
+    X.ni = set->ss.ext.iextto - set->ss.ext.iextfrom;
+    X.nj = set->ss.ext.jextto - set->ss.ext.jextfrom;
+    X.ijstart = set->ss.ext.ijstart;
+    X.jkstart = set->ss.ext.jkstart;
+    X.hcor = (float)(set->plan.dt / set->sp.dx / MU0);
+    std::vector<float> ce((size_t)X.ni * X.nj);
+    for (int b = 0; b < X.nj; b++)
+        for (int a = 0; a < X.ni; a++) {
+            double eps, sig;
+            sv_pool_epsilon_sigma(E->mp, X.ijstart + a, X.jkstart + b, X.k, &eps, &sig);
+            ce[(size_t)a + (size_t)X.ni * b] = (float)(set->plan.dt / set->sp.dx / eps);
+        }
+    float *d;
+    if (dupload(E, &d, ce.data(), ce.size()) || dmalloc(E, &E->ext_px, ce.size()) || dmalloc(E, &E->ext_py, ce.size()))
+        return 1;
+    X.ce = d;
+    X.px = E->ext_px;
+    X.py = E->ext_py;
+    E->ext_planes.resize(3 * (size_t)set->ss.ext.extxres * set->ss.ext.extyres);
+    E->ext_win.resize(2 * ce.size());
+    return 0;
+}
+
+/* one step's external planes of a family (0: E planes for the H step, 1: H
+ * planes for the E step): read on the host, the window uploaded, one launch;
+ * a missing file skips the step as the reference does */
+static int
+run_ext(Engine *E, int family, int hside)
+{
+    SvSet *set = E->set;
+    const int nx = set->ss.ext.extxres, n = nx * set->ss.ext.extyres;
+    GsvExt &X = E->ext;
+    if (sv_ext_read_planes(set, family, set->sc.step_act, E->ext_planes.data()))
+        return 0;
+    const size_t w = (size_t)X.ni * X.nj;
+    for (int b = 0; b < X.nj; b++)
+        for (int a = 0; a < X.ni; a++) {
+            const size_t q = (size_t)(set->ss.ext.jextfrom + b) * nx + set->ss.ext.iextfrom + a;
+            E->ext_win[(size_t)a + (size_t)X.ni * b] = E->ext_planes[q];
+            E->ext_win[w + (size_t)a + (size_t)X.ni * b] = E->ext_planes[(size_t)n + q];
+        }
+    /* the staging vector is reused next call: copy synchronously */
+    CK(hipMemcpyAsync(E->ext_px, E->ext_win.data(), w * sizeof(float), hipMemcpyHostToDevice, E->st));
+    CK(hipMemcpyAsync(E->ext_py, E->ext_win.data() + w, w * sizeof(float), hipMemcpyHostToDevice, E->st));
+    CK(gsv_launch_ext(&E->g, &X, hside, E->st));
+    CK(hipStreamSynchronize(E->st));
+    return 0;
+}
+
 static int
 setup_tsf(Engine *E)
 {
@@ -1638,7 +1698,7 @@ sv_pool_gpu_init(SvPool *mp, SvSet *set)
             CK(hipMemsetAsync(g.ade_p, 0, n * 4, E->st));
         }
     }
-    if (setup_cpml(E) || setup_tsf(E) || setup_tsff(E) || setup_sf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E) || setup_ltsf(E) || setup_forces(E)
+    if (setup_cpml(E) || setup_tsf(E) || setup_tsff(E) || setup_ext(E) || setup_sf(E) || setup_sources_and_outputs(E) || setup_liao_periodic(E) || setup_farfield(E) || setup_pfarfield(E) || setup_ltsf(E) || setup_forces(E)
         || setup_flux(E) || setup_sums(E))
         return 1;
     for (int q = 0; q < 4; q++)
@@ -1845,6 +1905,8 @@ sv_pool_gpu_apply_source_hstep(SvPool *mp, SvSet *set)
         CK(gsv_launch_tsff_line(&E->tsff, set->sc.step_act, E->st));
         CK(gsv_launch_tsff_faces(&E->g, &E->tsff, 1, E->st));
     }
+    if (E->has_ext && run_ext(E, 0, 1))
+        return 1;
     return 0;
 }
 
@@ -1912,6 +1974,8 @@ sv_pool_gpu_apply_source_estep(SvPool *mp, SvSet *set)
         if (set->sc.step_act == 0)
             CK(gsv_launch_tsf_line(&E->sfl, set->sc.step_act, E->st));
     }
+    if (E->has_ext && run_ext(E, 1, 0))
+        return 1;
     return 0;
 }
 

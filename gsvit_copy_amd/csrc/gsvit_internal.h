@@ -18,6 +18,10 @@ extern "C" {
 int sv_generate_waveform(int type, int nsteps, double wavelength, int pulse_width, double dt,
                          double amplitude, double *vals);
 double sv_round_g6(double v);
+/* external plane source: the three planes of one family (0: ex ey ez, 1: hx hy
+ * hz) for `step`, read from <base>_%.4d of step - shift into out[3][extyres][extxres];
+ * 1 (with a message) when a file is missing or short, as the reference skips that step */
+int sv_ext_read_planes(const SvSet *set, int family, int step, float *out);
 int sv_generate_waveform_ltsf(int type, int nsteps, double wavelength, int pulse_width, double dt,
                               double amplitude, double *vals);
 

@@ -48,6 +48,14 @@ typedef struct {
     const float *coefE[6];    /* dt / dx / eps of the face cell */
 } GsvTsff;
 
+/* external plane source window: ni x nj cells from (ijstart, jkstart) on plane k */
+typedef struct {
+    int k, ni, nj, ijstart, jkstart;
+    float hcor;               /* dt / dx / mu0 */
+    const float *ce;          /* dt / dx / eps per window cell on plane k */
+    const float *px, *py;     /* this step's x and y plane values over the window */
+} GsvExt;
+
 /* cells of a TSF face along its two in-plane axes; a box that spans a
  * periodic axis ends at the pool's last plane */
 static __host__ __device__ __forceinline__ void
@@ -195,6 +203,7 @@ hipError_t gsv_launch_hstep(const GsvGrid *g, int idbytes, int bx, int by, int k
 hipError_t gsv_launch_estep(const GsvGrid *g, int idbytes, int bx, int by, int kchunk, hipStream_t st);
 hipError_t gsv_launch_tsf_line(const GsvTsf *t, int step, hipStream_t st);
 hipError_t gsv_launch_tsff_line(const GsvTsff *t, int step, hipStream_t st);
+hipError_t gsv_launch_ext(const GsvGrid *g, const GsvExt *x, int hside, hipStream_t st);
 hipError_t gsv_launch_tsff_faces(const GsvGrid *g, const GsvTsff *t, int hside, hipStream_t st);
 /* scattered-field source, src3d/yee.c:1139-1237: the listed PEC-rule cells
  * (i, j, k, mask 1 Ex / 2 Ey / 4 Ez) take minus the incident field of line t */

@@ -925,6 +925,28 @@ k_tsff_face(GsvGrid g, GsvTsff f, int hside)
         gp(fb)[o] -= (float)(s * coef * sb);
 }
 
+/* external plane source (src3d/source.c:1377-1394, 2247-2259): E on plane k
+ * from the stored H plane values, H on plane k - 1 from the stored E values */
+__global__ __launch_bounds__(256) void
+k_ext_plane(GsvGrid g, GsvExt x, int hside)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= x.ni * x.nj)
+        return;
+    const int i = x.ijstart + q % x.ni, j = x.jkstart + q / x.ni;
+    const float vx = gp(x.px)[q], vy = gp(x.py)[q];
+    if (hside) {
+        const long long o = i + (long long)g.px * j + g.plane * (x.k - 1);
+        gp(g.hx)[o] -= x.hcor * vy;
+        gp(g.hy)[o] += x.hcor * vx;
+    } else {
+        const long long o = i + (long long)g.px * j + g.plane * x.k;
+        const float c = gp(x.ce)[q];
+        gp(g.ex)[o] += c * vy;
+        gp(g.ey)[o] -= c * vx;
+    }
+}
+
 /* scattered-field source on the PEC rule's components: minus the incident
  * field read from the SF line at the cell's distance along the wave
  * (src3d/yee.c:1146-1230; every component of a cell reads the same sample) */
@@ -1741,6 +1763,13 @@ gsv_launch_tsf_line(const GsvTsf *t, int step, hipStream_t st)
 }
 
 extern "C" hipError_t
+gsv_launch_ext(const GsvGrid *g, const GsvExt *x, int hside, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ext_plane, dim3((x->ni * x->nj + 255) / 256), dim3(256), 0, st, *g, *x, hside);
+    return hipGetLastError();
+}
+
+hipError_t
 gsv_launch_tsff_line(const GsvTsff *t, int step, hipStream_t st)
 {
     hipLaunchKernelGGL(k_tsff_line, dim3(t->nw), dim3(256), 0, st, t->lines, step);

@@ -503,6 +503,54 @@ load_tsff(SvPool *mp, SvSet *set)
     return 0;
 }
 
+int
+sv_ext_read_planes(const SvSet *set, int family, int step, float *out)
+{
+    /* src3d/source.c:1340-1399 (H planes for the E step), 2206-2266 (E planes) */
+    const int n = set->ss.ext.extxres * set->ss.ext.extyres;
+    char name[SV_NAME_LEN + 16];
+    int c, q;
+    for (c = 0; c < 3; c++) {
+        FILE *f;
+        snprintf(name, sizeof(name), "%s_%.4d", set->ss.ext.filebase[3 * family + c], step - set->ss.ext.shift);
+        f = fopen(name, "r");
+        if (!f) {
+            fprintf(stderr, "Error: cannot open external source files (e.g. %s, shifted step %d), skipping.\n", name,
+                    step - set->ss.ext.shift);
+            return 1;
+        }
+        for (q = 0; q < n; q++)
+            if (fscanf(f, "%f", out + (size_t)c * n + q) != 1) {
+                fprintf(stderr, "Error: external source file %s holds fewer than %d values, skipping.\n", name, n);
+                fclose(f);
+                return 1;
+            }
+        fclose(f);
+    }
+    return 0;
+}
+
+/* SOURCE_EXT geometry: the reference's bounds tests on the target cells can
+ * never fail (src3d/source.c:1382-1385), so a region outside the pool is
+ * refused here */
+static int
+check_ext(const SvSet *set)
+{
+    const int ni = set->ss.ext.iextto - set->ss.ext.iextfrom, nj = set->ss.ext.jextto - set->ss.ext.jextfrom;
+    if (set->ss.ext.i != -1 || set->ss.ext.j != -1) {
+        fprintf(stderr, "Error: SOURCE_EXT supports the z-plane mode only (i = j = -1); the reference applies no other\n");
+        return 1;
+    }
+    if (set->ss.ext.extxres < 1 || set->ss.ext.extyres < 1 || ni < 1 || nj < 1 || set->ss.ext.iextfrom < 0
+        || set->ss.ext.jextfrom < 0 || set->ss.ext.iextto > set->ss.ext.extxres || set->ss.ext.jextto > set->ss.ext.extyres
+        || set->ss.ext.ijstart < 0 || set->ss.ext.jkstart < 0 || set->ss.ext.ijstart + ni > set->sp.xres
+        || set->ss.ext.jkstart + nj > set->sp.yres || set->ss.ext.k < 1 || set->ss.ext.k >= set->sp.zres) {
+        fprintf(stderr, "Error: SOURCE_EXT region must lie inside its planes and inside the pool (1 <= k < zres)\n");
+        return 1;
+    }
+    return 0;
+}
+
 static int
 load_sources(SvPool *mp, SvSet *set)
 {
@@ -612,6 +660,8 @@ load_sources(SvPool *mp, SvSet *set)
         }
     }
     if (set->ss.tsff.w.valid && load_tsff(mp, set))
+        return 1;
+    if (set->ss.ext.valid && check_ext(set))
         return 1;
     if (set->ss.sf.valid) {
         /* src3d/plan.c:1985-2040: the waveform and a 1-D line of

@@ -595,6 +595,18 @@ parse_settings(const char *filename, SvSet *set, int called_from_gsvit)
                 goto fail;
             }
             t->valid = 1;
+        } else if (!strcmp(key, "SOURCE_EXT")) {
+            int *v[12] = { &set->ss.ext.i, &set->ss.ext.j, &set->ss.ext.k, &set->ss.ext.ijstart, &set->ss.ext.jkstart,
+                           &set->ss.ext.extxres, &set->ss.ext.extyres, &set->ss.ext.iextfrom, &set->ss.ext.jextfrom,
+                           &set->ss.ext.iextto, &set->ss.ext.jextto, &set->ss.ext.shift };
+            int q;
+            for (q = 0; q < 12; q++)
+                if (get_int(fr, v[q], key))
+                    goto fail;
+            for (q = 0; q < 6; q++)
+                if (get_token(fr, set->ss.ext.filebase[q], SV_NAME_LEN))
+                    goto fail;
+            set->ss.ext.valid = 1;
         } else if (!strcmp(key, "TSFF_SHIFT")) {
             SvSetSourceTSFF *f = &set->ss.tsff;
             if (get_double(fr, &f->xshift, key) || get_double(fr, &f->yshift, key) || get_double(fr, &f->zshift, key))

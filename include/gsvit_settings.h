@@ -238,6 +238,16 @@ typedef struct {
     SvSetSourcePoint *pnts;
     SvSetSourceTSF tsf;
     SvSetSourceTSFF tsff;
+    /* external plane source, src3d/settings.c:4147-4186 (SOURCE_EXT i j k
+     * ijstart jkstart extxres extyres iextfrom jextfrom iextto jextto shift,
+     * then the six file bases): per step the text planes <base>_%.4d of step
+     * - shift (extyres rows of extxres values) drive the z plane k; only
+     * i = j = -1 is a mode the reference applies (src3d/source.c:1377, 2251) */
+    struct {
+        int valid;
+        int i, j, k, ijstart, jkstart, extxres, extyres, iextfrom, jextfrom, iextto, jextto, shift;
+        char filebase[6][SV_NAME_LEN];   /* ex ey ez hx hy hz */
+    } ext;
     /* scattered-field source, src3d/settings.c:4698-4760 (SOURCE_SF theta phi
      * psi mode file-or-parameters); the box fields are unused */
     SvSetSourceTSF sf;

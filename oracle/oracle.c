@@ -1600,6 +1600,40 @@ tsff_estep(Oracle *o)
     }
 }
 
+/* external plane source (src3d/source.c:1340-1399, 2206-2266): the stored
+ * planes of step - shift drive the z plane k, E from the H planes
+ * (ex += dt/dx/eps hy, ey -= dt/dx/eps hx) and H at k - 1 from the E planes
+ * (hx -= dt/dx/mu0 ey, hy += dt/dx/mu0 ex) over the requested window */
+static void
+ext_step(Oracle *o, int hside)
+{
+    const SvSet *set = &o->set;
+    const int nx = set->ss.ext.extxres, n = nx * set->ss.ext.extyres;
+    float *pl;
+    int i, j;
+    if (!set->ss.ext.valid)
+        return;
+    pl = (float *)malloc(3 * (size_t)n * sizeof(float));
+    if (!sv_ext_read_planes(set, hside ? 0 : 1, set->sc.step_act, pl)) {
+        const float *px = pl, *py = pl + n;
+        for (i = set->ss.ext.iextfrom; i < set->ss.ext.iextto; i++)
+            for (j = set->ss.ext.jextfrom; j < set->ss.ext.jextto; j++) {
+                const int ip = i - set->ss.ext.iextfrom + set->ss.ext.ijstart, jp = j - set->ss.ext.jextfrom + set->ss.ext.jkstart;
+                const size_t q = (size_t)j * nx + i;
+                if (hside) {
+                    const double hcor = set->plan.dt / set->sp.dx / MU0;
+                    o->hx[IDX(o, ip, jp, set->ss.ext.k - 1)] -= hcor * py[q];
+                    o->hy[IDX(o, ip, jp, set->ss.ext.k - 1)] += hcor * px[q];
+                } else {
+                    const double ecor = set->plan.dt / set->sp.dx / get_epsilon(o, ip, jp, set->ss.ext.k);
+                    o->ex[IDX(o, ip, jp, set->ss.ext.k)] += ecor * py[q];
+                    o->ey[IDX(o, ip, jp, set->ss.ext.k)] -= ecor * px[q];
+                }
+            }
+    }
+    free(pl);
+}
+
 /* TSF beam profile at lateral position (x, y), src3d/source.c:231-251: the
  * faces take the Gaussian if it is set, else the fiber mode (:558-742, 1482-1649) */
 static double
@@ -2499,6 +2533,7 @@ oracle_step(void *handle, int nsteps)
         source_hstep(o);
         ltsf_hstep(o);
         tsff_hstep(o);
+        ext_step(o, 1);
         boundary_hstep(o);
         ystep_e(o);
         boundary_estep(o);
@@ -2506,6 +2541,7 @@ oracle_step(void *handle, int nsteps)
         ltsf_estep(o);
         tsff_estep(o);
         sf_estep(o);
+        ext_step(o, 0);
         getsum(o);
         farfield(o);
         pfarfield(o);
