@@ -4,8 +4,7 @@
  * Accepts the keyword format of src3d/settings.c:3822-5840 (whitespace separated
  * keywords, '#' comments, values read with the same integer/double rules as
  * get_int/get_double, src3d/settings.c:103-135).  Keywords outside the hot path
- * (sub-gridding, local, external and focused layered sources, geometry
- * modifiers) are
+ * (sub-gridding, local and focused layered sources, geometry modifiers) are
  * rejected with an error instead of being silently ignored, so a parameter
  * file never runs with part of its physics dropped.  Keywords whose only
  * product is a Gwyddion container (OUT_IMAGE, OUT_FILE) are consumed and

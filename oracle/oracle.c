@@ -40,6 +40,8 @@
  *                             src3d/plan.c:1855-1979 (pinned against numpy's
  *                             Gauss-Legendre rule, the plain TSF for one wave and
  *                             the focus position, tests/test_tsff.py)
+ *   external plane source ... src3d/source.c:1340-1399 (E), 2206-2266 (H) (pinned by
+ *                             the first injection's closed form, tests/test_ext_source.py)
  *   1-D auxiliary line ...... src3d/1dpool.c:83-162
  *   layered TSF ............. src3d/zpool.c:38-338, src3d/source.c:286-322,
  *                             921-1150 (E faces), 1680-1900 (H faces)
