@@ -1271,6 +1271,11 @@ setup_ltsf(Engine *E)
                         const int si = (!hs && axis == 0 && !f) ? i - 1 : i, sj = (!hs && axis == 1 && !f) ? j - 1 : j;
                         const size_t at = ((size_t)f * na + a) * nb + b;
                         q[at] = ltsf_delay(ls, set, si, sj);
+                        /* only the k0 face consults the skip map (src3d/source.c:1073, 1833) */
+                        if (axis == 2 && !f && !sv_tsf_face_active(mp, set, i, j, k)) {
+                            cf[at] = 0;
+                            continue;
+                        }
                         if (hs) {
                             cf[at] = hcor * ltsf_profile(set, si, sj, axis == 0 && f);
                         } else {

@@ -279,8 +279,8 @@ double sv_ghz(double f, double t, double p, double s) { return f * (-cos(s) * si
 int
 sv_tsf_face_active(const SvPool *mp, const SvSet *set, int i, int j, int k)
 {
-    /* src3d/source.c:253-264 */
-    if (set->ss.tsf.box_boundary_skipdepth == -1)
+    /* src3d/source.c:253-264: one map for the TSF and the layered TSF */
+    if (set->ss.tsf.box_boundary_skipdepth == -1 && set->ss.ltsf.box_boundary_skipdepth == -1)
         return 1;
     if (mp->d->sourceskip && mp->d->sourceskip->data[i][j][k])
         return 0;
@@ -704,10 +704,6 @@ load_sources(SvPool *mp, SvSet *set)
         }
         if (!(t->ia_theta >= 0 && t->ia_theta <= M_PI / 2)) {
             fprintf(stderr, "Error: unsupported LTSF angle (theta must lie in 0 .. pi/2)\n");
-            return 1;
-        }
-        if (t->box_boundary_skipdepth >= 0 || set->ss.tsf.box_boundary_skipdepth >= 0) {
-            fprintf(stderr, "Error: LTSF with a material skip depth is not supported\n");
             return 1;
         }
         if (t->source_mode == 1 || t->source_mode == 2) {
@@ -1712,18 +1708,30 @@ after_vector:
 
     apply_crop(mp, set);
 
-    /* TSF skip map, src3d/plan.c:5297-5329 */
-    if (mp->src->tsf && set->ss.tsf.box_boundary_skipdepth >= 0) {
-        SvSourceTSF *t = mp->src->tsf;
-        int sd = set->ss.tsf.box_boundary_skipdepth;
+    /* TSF and layered-TSF skip maps, src3d/plan.c:5297-5357: cells near the
+     * box faces with non-vacuum material within the skip depth; the layered
+     * TSF's map replaces the TSF's when both ask for one, as there */
+    for (n = 0; n < 2; n++) {
+        const int sd = n ? set->ss.ltsf.box_boundary_skipdepth : set->ss.tsf.box_boundary_skipdepth;
+        int b[6];
+        if (sd < 0 || (n ? !mp->src->ltsf : !mp->src->tsf))
+            continue;
+        if (n) {
+            const SvSourceLTSF *t = mp->src->ltsf;
+            b[0] = t->box_i0; b[1] = t->box_j0; b[2] = t->box_k0; b[3] = t->box_in; b[4] = t->box_jn; b[5] = t->box_kn;
+        } else {
+            const SvSourceTSF *t = mp->src->tsf;
+            b[0] = t->box_i0; b[1] = t->box_j0; b[2] = t->box_k0; b[3] = t->box_in; b[4] = t->box_jn; b[5] = t->box_kn;
+        }
+        sv_icube_free(d->sourceskip);
         d->sourceskip = sv_icube_new(xres, yres, zres);
         #pragma omp parallel for private(j, k) schedule(dynamic, 1)
         for (i = 0; i < xres; i++)
             for (j = 0; j < yres; j++)
                 for (k = 0; k < zres; k++) {
                     int ii, jj, kk, hit = 0;
-                    if (i > (t->box_i0 + sd) && j > (t->box_j0 + sd) && k > (t->box_k0 + sd)
-                        && i < (t->box_in - sd) && j < (t->box_jn - sd) && k < (t->box_kn - sd))
+                    if (i > (b[0] + sd) && j > (b[1] + sd) && k > (b[2] + sd)
+                        && i < (b[3] - sd) && j < (b[4] - sd) && k < (b[5] - sd))
                         continue;
                     for (ii = (i - sd > 0 ? i - sd : 0); ii < (i + sd < xres ? i + sd : xres) && !hit; ii++)
                         for (jj = (j - sd > 0 ? j - sd : 0); jj < (j + sd < yres ? j + sd : yres) && !hit; jj++)

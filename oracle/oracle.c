@@ -2068,7 +2068,7 @@ ltsf_estep(Oracle *o)
         }
     for (i = i0; i <= i1; i++)
         for (j = j0; j <= j1; j++) {
-            if (!t->box_boundary_skipk0) {
+            if (!t->box_boundary_skipk0 && sv_tsf_face_active(o->mp, &o->set, i, j, k0)) {
                 double g = ecor * ltsf_mult(o, i, j, 0);
                 double vhh = ltsf_val(o, LHH, i, j, k0 - 1), vhe = ltsf_val(o, LHE, i, j, k0 - 1);
                 double hx = -(vhh * cph + vhe * sph), hy = -(vhh * sph - vhe * cph), eps = get_epsilon(o, i, j, k0);
@@ -2137,7 +2137,7 @@ ltsf_hstep(Oracle *o)
         }
     for (i = i0; i <= i1; i++)
         for (j = j0; j <= j1; j++) {
-            if (!t->box_boundary_skipk0) {
+            if (!t->box_boundary_skipk0 && sv_tsf_face_active(o->mp, &o->set, i, j, k0)) {
                 double g = hcor * ltsf_mult(o, i, j, 0);
                 double veh = ltsf_val(o, LEH, i, j, k0), vee = ltsf_val(o, LEE, i, j, k0);
                 double ex = vee * cph - veh * sph, ey = vee * sph + veh * cph;

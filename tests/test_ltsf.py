@@ -88,8 +88,6 @@ def test_planner_rejects_what_is_not_supported(tmp_path):
     base = dict(shape=(16, 16, 40), nsteps=10, box=(4, 4, 4, 12, 12, 30))
     with pytest.raises(RuntimeError):          # theta beyond 90 degrees (the reference prints an error)
         Oracle(cases.ltsf_case(str(tmp_path / "a"), theta=120, **base))
-    with pytest.raises(RuntimeError):          # material skip depth
-        Oracle(cases.ltsf_case(str(tmp_path / "b"), extra=["LTSF_SKIP\ndepth 2"], **base))
     with pytest.raises(RuntimeError):          # box touching the pool edge
         Oracle(cases.ltsf_case(str(tmp_path / "c"), shape=(16, 16, 40), nsteps=10, box=(0, 4, 4, 12, 12, 30)))
     par = cases.ltsf_case(str(tmp_path / "d"), **base)
@@ -183,11 +181,55 @@ def test_database_layer_equals_its_resolved_permittivity(tmp_path):
     assert np.array_equal(out[0], out[1]) and np.abs(out[0]).max() > 1e-3
 
 
+def test_skip_depth_switches_off_the_k0_face_near_material(tmp_path):
+    """LTSF_SKIP depth d (src3d/plan.c:5330-5357, src3d/source.c:253-264,
+    1073, 1833): on the k0 face only, a cell with non-vacuum material in the
+    window [i-d, i+d) x [j-d, j+d) x [k0-d, k0+d) injects nothing; everything
+    else is the run without the skip depth, bit for bit"""
+    base = dict(shape=(24, 24, 60), nsteps=60, box=(4, 4, 10, 20, 20, 50), boundary="cpml 3 3 -1 0 1")
+    fold = [str(tmp_path / "a"), str(tmp_path / "b")]
+    pars = [cases.ltsf_case(fold[0], extra=["LTSF_SKIP\ndepth 2", "MEDIUM_VECTOR\nmats.txt"], **base),
+            cases.ltsf_case(fold[1], extra=["MEDIUM_VECTOR\nmats.txt"], **base)]
+    for f in fold:
+        with open(os.path.join(f, "mats.txt"), "w") as fh:
+            fh.write("8 6 7 7 12 11 9 1 2.5 1 0 0\n")
+    oa, ob = Oracle(pars[0]), Oracle(pars[1])
+    mat, _ = ob.maps()
+    for step in range(base["nsteps"]):
+        oa.step(1); ob.step(1)
+        fa, fb = oa.fields(), ob.fields()
+        if any(np.abs(x).max() > 0 for x in fb):
+            break
+    oa.close(); ob.close()
+    i0, j0, k0, i1, j1 = 4, 4, 10, 20, 20
+    skip = np.zeros((24, 24), dtype=bool)
+    for i in range(i0, i1 + 1):
+        for j in range(j0, j1 + 1):
+            w = mat[max(i - 2, 0):i + 2, max(j - 2, 0):j + 2, k0 - 2:k0 + 2]
+            skip[i, j] = bool((w != 0).any())
+    assert 0 < skip.sum() < (i1 - i0 + 1) * (j1 - j0 + 1)
+    # first injecting step: H is its k0-face injection alone (plane k0 - 1);
+    # E also holds the curl of that H, so it may differ on planes k0 - 1, k0 only
+    injected = False
+    others = [k for k in range(60) if k != k0 - 1]
+    for c in range(3, 6):
+        d = fa[c] != fb[c]
+        assert not d[:, :, others].any()
+        assert not (d[:, :, k0 - 1] & ~skip).any()
+        assert not fa[c][:, :, k0 - 1][skip].any()
+        injected |= bool(np.abs(fb[c][:, :, k0 - 1][skip]).max() > 0)
+    assert injected
+    for c in range(3):
+        assert not (fa[c] != fb[c])[:, :, [k for k in range(60) if k not in (k0 - 1, k0)]].any()
+
+
 GPU_CASES = {
     "vacuum": dict(theta=0, phi=30, psi=60, layers=[], halfspace=None),
     "half space": dict(theta=0, phi=30, psi=60, layers=[(36, 4.0)], halfspace=(36, 4.0)),
     "oblique, Gaussian, two faces off": dict(theta=30, phi=110, psi=40, layers=[(20, 2.25), (40, 4.0)], halfspace=(40, 4.0),
                                              extra=["LTSF_GAUSSIAN\n16 16 5 7", "LTSF_SKIP\nj0", "LTSF_SKIP\nkn"]),
+    "skip depth near a block under the k0 face": dict(theta=0, phi=30, psi=60, layers=[], halfspace=(30, 2.25),
+                                                      extra=["LTSF_SKIP\ndepth 2"]),
     "fiber profile": dict(theta=20, phi=250, psi=0, layers=[], halfspace=None,
                           extra=["SOURCE_WAVELENGTH\n150e-9 200e-9 250e-9", "LTSF_FIBER\n16 15 3 6 2.1 2.0"]),
 }
